@@ -1,0 +1,277 @@
+// fgpu_filter.hip — WHERE predicate scans, stream compaction and upload-time
+// column statistics for gfx950.
+//
+// Row layout per wavefront: lane l of a wave handles row 64*w + l of mask word
+// w, so a 64-bit ballot of the predicate *is* the reference's LSB-first mask
+// word (src/storage/column.rs:132-133; src/query/expr.rs:1076-1175).  Each
+// wave walks FILTER_UNROLL words per step so several independent loads are in
+// flight per lane.  COUNT(*) is a per-lane count reduced once per workgroup
+// (db.rs:1439-1476 restated as one fused pass); compaction is an exclusive scan
+// of per-word popcounts followed by an mbcnt-style in-word rank, which keeps
+// the reference's row order (gather_by_bitmask, src/storage/column.rs:1077-1123).
+#include "fgpu_launch.h"
+
+namespace fgpu {
+
+constexpr int FILTER_UNROLL = 4;
+
+__global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *out) {
+    __shared__ unsigned long long part[BLOCK / 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    const int64_t n_words = (n + 63) >> 6;
+    unsigned int cnt = 0;
+    for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        bool pass[FILTER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) {
+            const int64_t r = ((w0 + u) << 6) + lane;
+            pass[u] = r < n && eval_pred(p, c, r);
+        }
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) cnt += pass[u] ? 1u : 0u;
+    }
+    // wave reduction
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane == 0) part[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[i];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+__global__ void k_filter_mask(DCols c, DPred p, int64_t n, uint64_t *words) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t n_words = (n + 63) >> 6;
+    for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        bool pass[FILTER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) {
+            const int64_t r = ((w0 + u) << 6) + lane;
+            pass[u] = r < n && eval_pred(p, c, r);
+        }
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) {
+            const uint64_t b = __ballot(pass[u]);
+            if (lane == u && w0 + u < n_words) words[w0 + u] = b;
+        }
+    }
+}
+
+// ─── exclusive scans (block of 256 threads x 16 elements = 4096 per block) ──
+constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_TILE = BLOCK * SCAN_PER_THREAD;
+
+struct PopSrc {
+    const uint64_t *w;
+    FGPU_DEV int64_t operator()(int64_t i) const { return __popcll(w[i]); }
+};
+struct U32Src {
+    const uint32_t *c;
+    FGPU_DEV int64_t operator()(int64_t i) const { return c[i]; }
+};
+
+// Block-wide exclusive scan of one int64 per thread; returns the block total.
+FGPU_DEV int64_t block_excl_scan(int64_t v, int64_t &excl) {
+    __shared__ int64_t wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int64_t before = 0, total = 0;
+    for (int i = 0; i < BLOCK / 64; ++i) {
+        if (i < wave) before += wsum[i];
+        total += wsum[i];
+    }
+    __syncthreads();
+    excl = before + incl - v;
+    return total;
+}
+
+template <class Src>
+__global__ void k_scan_tiles(Src src, int64_t n, int64_t *tile_sums) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER_THREAD;
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k)
+        if (base + k < n) s += src(base + k);
+    int64_t excl;
+    const int64_t total = block_excl_scan(s, excl);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Single workgroup: exclusive scan of the tile sums in place; t[n_tiles] = total.
+__global__ void k_scan_tile_sums(int64_t *t, int64_t n_tiles) {
+    int64_t carry = 0;
+    for (int64_t b = 0; b < n_tiles; b += BLOCK) {
+        const int64_t i = b + threadIdx.x;
+        const int64_t v = i < n_tiles ? t[i] : 0;
+        int64_t excl;
+        const int64_t total = block_excl_scan(v, excl);
+        if (i < n_tiles) t[i] = carry + excl;
+        carry += total;
+    }
+    if (threadIdx.x == 0) t[n_tiles] = carry;
+}
+
+template <class Src>
+__global__ void k_scan_write(Src src, int64_t n, const int64_t *tile_offs, int64_t *offs) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER_THREAD;
+    int64_t v[SCAN_PER_THREAD];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        v[k] = base + k < n ? src(base + k) : 0;
+        s += v[k];
+    }
+    int64_t excl;
+    block_excl_scan(s, excl);
+    int64_t run = tile_offs[blockIdx.x] + excl;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        if (base + k < n) offs[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const int64_t nt = gridDim.x;
+        offs[n] = tile_offs[nt];
+    }
+}
+
+template <class Src>
+static hipError_t scan_impl(hipStream_t s, Src src, int64_t n, int64_t *offs, int64_t *tmp) {
+    const int64_t tiles = std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_tiles<Src>, dim3((unsigned)tiles), dim3(BLOCK), 0, s, src, n, tmp);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(BLOCK), 0, s, tmp, tiles);
+    hipLaunchKernelGGL(k_scan_write<Src>, dim3((unsigned)tiles), dim3(BLOCK), 0, s, src, n, tmp, offs);
+    return hipGetLastError();
+}
+
+hipError_t launch_word_offsets(hipStream_t s, const uint64_t *words, int64_t n_words,
+                               int64_t *offs, int64_t *block_tmp) {
+    return scan_impl(s, PopSrc{words}, n_words, offs, block_tmp);
+}
+hipError_t launch_scan_u32(hipStream_t s, const uint32_t *cnt, int64_t n, int64_t *offs,
+                           int64_t *block_tmp) {
+    return scan_impl(s, U32Src{cnt}, n, offs, block_tmp);
+}
+
+// One wave per mask word: the lanes whose bit is set write their row at
+// offs[word] + (number of set bits below the lane).
+__global__ void k_gather(const uint64_t *words, const int64_t *offs, int64_t n_words,
+                         const void *src, int elem_bytes, const uint64_t *src_valid, void *dst,
+                         uint8_t *dst_valid) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n_words;
+         w += waves) {
+        const uint64_t bits = words[w];
+        if (!((bits >> lane) & 1)) continue;
+        const int64_t r = (w << 6) + lane;
+        const int64_t o = offs[w] + __popcll(bits & ((1ull << lane) - 1));
+        if (elem_bytes == 8) ((int64_t *)dst)[o] = ((const int64_t *)src)[r];
+        else ((int32_t *)dst)[o] = ((const int32_t *)src)[r];
+        if (dst_valid) dst_valid[o] = src_valid ? (uint8_t)((src_valid[r >> 6] >> (r & 63)) & 1) : 1;
+    }
+}
+
+hipError_t launch_filter_count(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
+                               unsigned long long *d_count, int grid) {
+    hipLaunchKernelGGL(k_filter_count, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_count);
+    return hipGetLastError();
+}
+hipError_t launch_filter_mask(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
+                              uint64_t *d_words, int grid) {
+    hipLaunchKernelGGL(k_filter_mask, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_words);
+    return hipGetLastError();
+}
+hipError_t launch_gather(hipStream_t s, const uint64_t *words, const int64_t *offs,
+                         int64_t n_words, const void *src, int elem_bytes,
+                         const uint64_t *src_valid, void *dst, uint8_t *dst_valid, int grid) {
+    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(BLOCK), 0, s, words, offs, n_words, src,
+                       elem_bytes, src_valid, dst, dst_valid);
+    return hipGetLastError();
+}
+
+// ─── column statistics ─────────────────────────────────────────────────────
+__global__ void k_col_stats(const void *values, int dtype, const uint64_t *valid, int64_t n,
+                            ColStatsDev *st) {
+    __shared__ unsigned int hll[4096];
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) hll[i] = 0;
+    __syncthreads();
+    long long imin = INT64_MAX, imax = INT64_MIN;
+    unsigned long long fmin = ~0ull, fmax = 0ull, nonfinite = 0, nulls = 0;
+    int emin = 1 << 20, emax = -(1 << 20);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        bool ok;
+        int64_t v;
+        if (dtype == FGPU_UTF8) {
+            const uint32_t code = ((const uint32_t *)values)[r];
+            ok = code != NULL_CODE;
+            v = code;
+        } else {
+            ok = valid ? ((valid[r >> 6] >> (r & 63)) & 1) : true;
+            v = dtype == FGPU_INT32 ? (int64_t)((const int32_t *)values)[r]
+                                    : ((const int64_t *)values)[r];
+        }
+        if (!ok) {
+            ++nulls;
+            continue;
+        }
+        if (dtype == FGPU_FLOAT64) {
+            const uint64_t bits = (uint64_t)v;
+            const int be = (int)((bits >> 52) & 0x7ff);
+            if (be == 0x7ff) {
+                nonfinite |= (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
+                if (bits & 0x000fffffffffffffull) continue;  // NaN: no min/max
+            } else if ((bits & 0x7fffffffffffffffull) != 0) {
+                const int e = be == 0 ? -1074 : be - 1075;
+                emin = e < emin ? e : emin;
+                emax = e > emax ? e : emax;
+            }
+            const uint64_t enc = f64_order(bits);
+            fmin = enc < fmin ? enc : fmin;
+            fmax = enc > fmax ? enc : fmax;
+            v = (int64_t)bits;
+        } else {
+            imin = v < imin ? v : imin;
+            imax = v > imax ? v : imax;
+        }
+        // HyperLogLog, p = 12
+        const uint64_t h = hash64((uint64_t)v);
+        const unsigned idx = (unsigned)(h >> 52);
+        const uint64_t rest = (h << 12) | (1ull << 11);
+        const unsigned rank = (unsigned)__clzll((long long)rest) + 1;
+        atomicMax(&hll[idx], rank);
+    }
+    atomicMin(&st->imin, imin);
+    atomicMax(&st->imax, imax);
+    atomicMin(&st->fmin_enc, fmin);
+    atomicMax(&st->fmax_enc, fmax);
+    atomicMin(&st->emin, emin);
+    atomicMax(&st->emax, emax);
+    if (nonfinite) atomicOr(&st->nonfinite, nonfinite);
+    if (nulls) atomicAdd(&st->nulls, nulls);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x)
+        if (hll[i]) atomicMax(&st->hll[i], hll[i]);
+}
+
+hipError_t launch_col_stats(hipStream_t s, const void *values, int dtype, const uint64_t *valid,
+                            int64_t n, ColStatsDev *d_stats, int grid) {
+    hipLaunchKernelGGL(k_col_stats, dim3(grid), dim3(BLOCK), 0, s, values, dtype, valid, n, d_stats);
+    return hipGetLastError();
+}
+
+}  // namespace fgpu

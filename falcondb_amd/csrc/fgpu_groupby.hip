@@ -1,0 +1,379 @@
+// fgpu_groupby.hip — fused WHERE + GROUP BY aggregation for gfx950.
+//
+// Replaces the reference's execute_select -> GroupByExec::execute pipeline
+// (src/query/select.rs:105-160; src/query/aggregate.rs:535-1420): instead of
+// materialising the filtered columns and building per-batch hash maps that a
+// single thread merges, one kernel streams every row once, evaluates the WHERE
+// predicate in registers, and folds the surviving rows into group states.
+//
+// Strategies (chosen on the host from upload-time column statistics):
+//   GB_DENSE_LDS    key range (max-min+1, or dictionary size) small enough that
+//                   every group's state fits in LDS: each workgroup aggregates
+//                   privately in LDS and merges once into HBM.
+//   GB_DENSE_GLOBAL key range moderate: state indexed by key - min in HBM,
+//                   updated with device-scope atomics (no hashing, no CAS).
+//   GB_HASH         open-addressing table in HBM (64-bit CAS on the key slot,
+//                   linear probing) behind a per-workgroup LDS cache that absorbs
+//                   the frequent keys (Zipf heavy hitters) before they reach HBM.
+//   GB_SCALAR       no GROUP BY: per-thread register accumulators (exact integer,
+//                   double-double float) reduced once per workgroup.
+//
+// Group state = S 64-bit words; word 0 counts rows; the DOp list says how each
+// aggregate input updates its words.  Float SUM is exact (fixed-point words),
+// so results do not depend on the order in which rows arrive.
+#include "fgpu_state.h"
+
+namespace fgpu {
+
+// Loads the aggregate inputs of row r.
+FGPU_DEV Inputs load_inputs(const GbArgs &a, int64_t r) {
+    Inputs in{0, 0, 0, 0, 0, 0, 0};
+    auto one = [&](int j, int64_t &dst) {
+        if (j < a.n_in) {
+            const int k = a.in_col[j];
+            dst = load_raw(a.cols, k, r);
+            if (load_valid(a.cols, k, r)) in.valid |= 1u << j;
+        }
+    };
+    one(0, in.v0);
+    one(1, in.v1);
+    one(2, in.v2);
+    one(3, in.v3);
+    one(4, in.v4);
+    one(5, in.v5);
+    return in;
+}
+
+// Key of row r; returns false for the NULL group (int key NULL or INT64_MIN,
+// which the reference folds together, src/query/aggregate.rs:867, :1207).
+FGPU_DEV bool load_key(const GbArgs &a, int64_t r, int64_t &key) {
+    if (a.key_is_utf8) {
+        const uint32_t code = ((const uint32_t *)a.cols.v[a.key_col])[r];
+        key = code == NULL_CODE ? (int64_t)a.utf8_null_code : (int64_t)code;
+        return true;
+    }
+    if (!load_valid(a.cols, a.key_col, r)) return false;
+    key = load_raw(a.cols, a.key_col, r);
+    return key != EMPTY_KEY;
+}
+
+// Find-or-insert in the HBM table; -1 if the table is full.
+FGPU_DEV int64_t hbm_slot(const GbArgs &a, int64_t key, uint64_t h) {
+    uint64_t i = h & a.cap_mask;
+    for (uint64_t p = 0; p <= a.cap_mask; ++p) {
+        int64_t k = __hip_atomic_load(a.keys + i, __ATOMIC_RELAXED, DEV_SCOPE);
+        if (k == key) return (int64_t)i;
+        if (k == EMPTY_KEY) {
+            int64_t expect = EMPTY_KEY;
+            if (__hip_atomic_compare_exchange_strong(a.keys + i, &expect, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, DEV_SCOPE)) {
+                atomicAdd(a.counters, 1ull);
+                return (int64_t)i;
+            }
+            if (expect == key) return (int64_t)i;
+        }
+        i = (i + 1) & a.cap_mask;
+    }
+    return -1;
+}
+
+// Find-or-insert in the LDS cache with a bounded number of probes; -1 = miss.
+FGPU_DEV int lds_slot(int64_t *lkeys, int lmask, int limit, int64_t key, uint64_t h) {
+    int i = (int)(h >> 40) & lmask;  // different hash bits than the HBM table
+    for (int p = 0; p < limit; ++p) {
+        int64_t k = __hip_atomic_load(lkeys + i, __ATOMIC_RELAXED, LDS_SCOPE);
+        if (k == key) return i;
+        if (k == EMPTY_KEY) {
+            int64_t expect = EMPTY_KEY;
+            if (__hip_atomic_compare_exchange_strong(lkeys + i, &expect, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, LDS_SCOPE))
+                return i;
+            if (expect == key) return i;
+        }
+        i = (i + 1) & lmask;
+    }
+    return -1;
+}
+
+constexpr int GB_UNROLL = 4;  // rows per thread per tile (independent loads in flight)
+
+// ─── dense keys, LDS-private partial states ─────────────────────────────────
+__global__ void k_gb_dense_lds(GbArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int S = a.ops.S;
+    const int64_t groups = a.range + 1;  // + NULL group
+    for (int64_t i = threadIdx.x; i < groups * S; i += blockDim.x) lds[i] = a.ops.ident[i % S];
+    __syncthreads();
+    const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
+    for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        bool pass[GB_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+        }
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            if (!pass[u]) continue;
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            int64_t key;
+            const int64_t g = load_key(a, r, key) ? (key - a.key_min) : a.range;
+            update_row<LDS_SCOPE>(lds + g * S, a.ops, load_inputs(a, r));
+        }
+    }
+    __syncthreads();
+    for (int64_t g = threadIdx.x; g < groups; g += blockDim.x)
+        if (lds[g * S]) merge_state(a.state + g * S, lds + g * S, a.ops);
+}
+
+// ─── dense keys, HBM states ─────────────────────────────────────────────────
+__global__ void k_gb_dense_global(GbArgs a) {
+    const int S = a.ops.S;
+    const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
+    for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        bool pass[GB_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+        }
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            if (!pass[u]) continue;
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            int64_t key;
+            const int64_t g = load_key(a, r, key) ? (key - a.key_min) : a.range;
+            update_row<DEV_SCOPE>(a.state + g * S, a.ops, load_inputs(a, r));
+        }
+    }
+}
+
+// ─── hash table in HBM behind an LDS cache ──────────────────────────────────
+__global__ void k_gb_hash(GbArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int S = a.ops.S;
+    const int ls = a.lds_slots;
+    int64_t *lkeys = (int64_t *)lds;
+    uint64_t *lstate = lds + ls;
+    for (int i = threadIdx.x; i < ls; i += blockDim.x) lkeys[i] = EMPTY_KEY;
+    for (int i = threadIdx.x; i < ls * S; i += blockDim.x) lstate[i] = a.ops.ident[i % S];
+    __syncthreads();
+    const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
+    for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        bool pass[GB_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+        }
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            if (!pass[u]) continue;
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            int64_t key;
+            const bool has_key = load_key(a, r, key);
+            const Inputs in = load_inputs(a, r);
+            if (!has_key) {  // NULL group: state row cap
+                update_row<DEV_SCOPE>(a.state + (int64_t)(a.cap_mask + 1) * S, a.ops, in);
+                continue;
+            }
+            const uint64_t h = hash64((uint64_t)key);
+            const int li = ls ? lds_slot(lkeys, ls - 1, a.probe_limit, key, h) : -1;
+            if (li >= 0) {
+                update_row<LDS_SCOPE>(lstate + (int64_t)li * S, a.ops, in);
+            } else {
+                const int64_t gi = hbm_slot(a, key, h);
+                if (gi < 0) {  // table full: the host retries with a larger one
+                    atomicExch(a.counters + 1, 1ull);
+                    continue;
+                }
+                update_row<DEV_SCOPE>(a.state + gi * S, a.ops, in);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ls; i += blockDim.x) {
+        const int64_t key = lkeys[i];
+        if (key == EMPTY_KEY || lstate[(int64_t)i * S] == 0) continue;
+        const int64_t gi = hbm_slot(a, key, hash64((uint64_t)key));
+        if (gi < 0) {
+            atomicExch(a.counters + 1, 1ull);
+            continue;
+        }
+        merge_state(a.state + gi * S, lstate + (int64_t)i * S, a.ops);
+    }
+}
+
+// ─── no GROUP BY: register accumulators ─────────────────────────────────────
+// Each thread folds its rows into registers (exact int64; double-double for
+// float sums), then the workgroup merges them into an LDS state row, which one
+// pass of atomics adds to the HBM state row.
+__global__ void k_gb_scalar(GbArgs a) {
+    __shared__ uint64_t lrow[64];
+    const int S = a.ops.S;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) lrow[i] = a.ops.ident[i];
+    __syncthreads();
+    int64_t rows = 0;
+    int64_t acc[MAX_OPS];
+    double hi[MAX_OPS], lo[MAX_OPS];
+#pragma unroll
+    for (int i = 0; i < MAX_OPS; ++i) {
+        acc[i] = i < a.ops.n ? reg_ident(a.ops.op[i].kind) : 0;
+        hi[i] = 0.0;
+        lo[i] = 0.0;
+    }
+    const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
+    for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        bool pass[GB_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+        }
+#pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) {
+            if (!pass[u]) continue;
+            const int64_t r = base + u * blockDim.x + threadIdx.x;
+            const Inputs in = load_inputs(a, r);
+            ++rows;
+#pragma unroll
+            for (int i = 0; i < MAX_OPS; ++i) {
+                if (i < a.ops.n) {
+                    const DOp o = a.ops.op[i];
+                    if ((in.valid >> o.col) & 1) reg_add(o.kind, pick(in, o.col), acc[i], hi[i], lo[i]);
+                }
+            }
+        }
+    }
+    // fold registers into the LDS row
+    if (rows) a_add<LDS_SCOPE>(lrow, (uint64_t)rows);
+#pragma unroll
+    for (int i = 0; i < MAX_OPS; ++i)
+        if (i < a.ops.n) fold_reg(lrow, a.ops.op[i], acc[i], hi[i], lo[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) merge_state(a.state, lrow, a.ops);
+}
+
+// ─── emit: occupied groups -> output columns ────────────────────────────────
+__global__ void k_gb_emit(EmitArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool have = false;
+    const uint64_t *st = nullptr;
+    if (i < a.n_slots) {
+        st = a.state + i * a.S;
+        if (a.mode == GB_SCALAR) have = true;
+        else if (a.mode == GB_HASH) have = (i == a.n_slots - 1 || a.keys[i] != EMPTY_KEY) && st[0] > 0;
+        else have = st[0] > 0;
+    }
+    // wave-aggregated output slot
+    const uint64_t ballot = __ballot(have);
+    if (ballot == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)ballot) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.out_count, (unsigned long long)__popcll(ballot));
+    base = __shfl(base, leader);
+    if (!have) return;
+    const int64_t o = (int64_t)base + __popcll(ballot & ((1ull << lane) - 1));
+    if (a.out_key) {
+        if (a.mode == GB_HASH) {
+            const bool null_group = i == a.n_slots - 1;
+            a.out_key[o] = null_group ? 0 : a.keys[i];
+            a.out_key_valid[o] = null_group ? 0 : 1;
+        } else {
+            const bool null_group = i == a.n_slots - 1 && !a.key_is_utf8;
+            a.out_key[o] = null_group ? 0 : i + a.key_min;
+            a.out_key_valid[o] = null_group ? 0 : 1;
+        }
+    }
+    for (int k = 0; k < a.outs.n; ++k) {
+        const DOut d = a.outs.o[k];
+        const int64_t cnt = (int64_t)st[d.cnt_word];
+        const uint64_t w = st[d.word];
+        int64_t out = 0;
+        uint8_t valid = cnt > 0;
+        switch (d.kind) {
+            case OUT_COUNT_ROWS: out = (int64_t)st[0]; valid = 1; break;
+            case OUT_COUNT: out = cnt; valid = 1; break;
+            case OUT_SUM_I:
+            case OUT_MIN_I:
+            case OUT_MAX_I: out = (int64_t)w; break;
+            case OUT_AVG_I: out = __double_as_longlong(cnt ? (double)(int64_t)w / (double)cnt : 0.0); break;
+            case OUT_SUM_F:
+            case OUT_AVG_F: {
+                double s = fx_finish((const int64_t *)(st + d.word), d.W, d.L,
+                                     d.has_flags ? st[d.word + d.W] : 0ull);
+                if (d.kind == OUT_AVG_F) s = cnt ? s / (double)cnt : 0.0;
+                out = __double_as_longlong(s);
+                break;
+            }
+            case OUT_MIN_F:
+                out = (int64_t)f64_unorder(w);
+                valid = valid && w != ~0ull;
+                break;
+            case OUT_MAX_F:
+                out = (int64_t)f64_unorder(w);
+                valid = valid && w != 0ull;
+                break;
+        }
+        a.out_val[k][o] = out;
+        a.out_valid[k][o] = valid;
+    }
+}
+
+__global__ void k_state_init(int64_t *keys, int64_t n_keys, uint64_t *state, int64_t n_words,
+                             DOps ops) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_keys; i += stride)
+        keys[i] = EMPTY_KEY;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += stride)
+        state[i] = ops.ident[i % ops.S];
+}
+
+__global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v) {
+    if (n <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint64_t *state,
+                             int64_t n_groups, const DOps &ops) {
+    const int64_t n = std::max<int64_t>(n_keys, n_groups * ops.S);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_state_init, dim3(grid), dim3(256), 0, s, keys, n_keys, state,
+                       n_groups * ops.S, ops);
+    return hipGetLastError();
+}
+
+hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,
+                          size_t lds_bytes) {
+    switch (mode) {
+        case GB_DENSE_LDS:
+            hipLaunchKernelGGL(k_gb_dense_lds, dim3(grid), dim3(block), lds_bytes, s, a);
+            break;
+        case GB_DENSE_GLOBAL:
+            hipLaunchKernelGGL(k_gb_dense_global, dim3(grid), dim3(block), 0, s, a);
+            break;
+        case GB_HASH:
+            hipLaunchKernelGGL(k_gb_hash, dim3(grid), dim3(block), lds_bytes, s, a);
+            break;
+        default:
+            hipLaunchKernelGGL(k_gb_scalar, dim3(grid), dim3(block), 0, s, a);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(hipStream_t s, const EmitArgs &a) {
+    const int grid = (int)std::max<int64_t>(1, (a.n_slots + 255) / 256);
+    hipLaunchKernelGGL(k_gb_emit, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace fgpu

@@ -1,0 +1,316 @@
+// fgpu_internal.h — definitions shared by the host runtime and the gfx950 kernels.
+//
+// Data layout in HBM (one device, one table):
+//   column      contiguous array of n_rows values (int32 / int64 / double / u32 code)
+//   validity    optional ceil(n/64) LSB-first u64 words (bit set = valid)
+//   group table keys[C] (int64, EMPTY = INT64_MIN) + state[(C+1) * S] u64 words,
+//               row C of state is the NULL-key group; S words per group
+//   join index  keys[Cb] (int64, EMPTY = INT64_MIN) + rows[Cb] (u32 build row)
+//
+// Exact float sums: a double x = m * 2^e (m a 53-bit integer) is added into a
+// fixed-point number with least significant bit 2^L, where L is the smallest
+// mantissa-LSB exponent of the column (computed at upload).  The fixed-point
+// number is kept in carry-save form: W signed 64-bit words, word k holding
+// digits of weight 2^(L + 32k); every addend contributes < 2^32 per word, so up
+// to 2^31 additions cannot overflow a word.  The final value is normalised and
+// rounded to nearest-even once, so the result is the exactly rounded sum,
+// independent of the order of the additions.
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/falcon_gpu.h"
+
+#define FGPU_DEV __device__ __forceinline__
+#define FGPU_HD __host__ __device__ __forceinline__
+
+namespace fgpu {
+
+constexpr int64_t EMPTY_KEY = INT64_MIN;     // empty hash slot / NULL int key sentinel
+constexpr uint32_t NULL_CODE = 0xFFFFFFFFu;  // Utf8 NULL code
+constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
+constexpr int MAX_COLS = 16;    // columns one kernel can read
+constexpr int MAX_OPS = 16;     // state operations per group
+constexpr int MAX_FX_WORDS = 8; // fixed-point words of one exact float sum
+constexpr int BLOCK = 256;      // threads per workgroup for the scan kernels
+
+// ─── hashing ────────────────────────────────────────────────────────────────
+// MurmurHash3 64-bit finaliser (a bijection on 64-bit values).
+FGPU_HD uint64_t hash64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+// ─── order-preserving encoding of doubles for atomic min / max ──────────────
+FGPU_HD uint64_t f64_order(uint64_t bits) {
+    return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+}
+FGPU_HD uint64_t f64_unorder(uint64_t enc) {
+    return (enc >> 63) ? (enc & 0x7fffffffffffffffull) : ~enc;
+}
+
+// ─── exact fixed-point digits of a double ───────────────────────────────────
+// Splits finite nonzero x into three signed digits (< 2^32 in magnitude except
+// the top one, < 2^22) added at words w0, w0+1, w0+2 of a fixed point number
+// with LSB 2^L.  Returns false for zero (nothing to add).  Requires x to be a
+// multiple of 2^L (true for every value whose mantissa-LSB exponent is >= L).
+FGPU_HD bool fx_digits(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1, int64_t &d2) {
+    uint64_t m = bits & 0x000fffffffffffffull;
+    const int be = (int)((bits >> 52) & 0x7ff);
+    int e;
+    if (be == 0) {
+        if (m == 0) return false;
+        e = -1074;
+    } else {
+        m |= 0x0010000000000000ull;
+        e = be - 1075;
+    }
+    if (e < L) {  // trailing zeros cover the difference (x is a multiple of 2^L)
+        const int s = L - e;
+        m = s >= 64 ? 0 : (m >> s);
+        e = L;
+    }
+    const int shift = e - L;
+    w0 = shift >> 5;
+    const int off = shift & 31;
+    const uint64_t lo = m << off;
+    const uint64_t hi = off ? (m >> (64 - off)) : 0;
+    d0 = (int64_t)(lo & 0xffffffffull);
+    d1 = (int64_t)(lo >> 32);
+    d2 = (int64_t)hi;
+    if (bits >> 63) {
+        d0 = -d0;
+        d1 = -d1;
+        d2 = -d2;
+    }
+    return true;
+}
+
+// Converts W carry-save words (LSB weight 2^L) to the nearest double
+// (round half to even).  `w` is read-only.
+FGPU_HD double fx_to_double(const int64_t *w, int W, int L) {
+    // Normalise into base-2^32 digits of a two's complement integer.
+    uint32_t dig[MAX_FX_WORDS + 3];
+    int64_t carry = 0;
+    const int ND = W + 3;
+    for (int k = 0; k < ND; ++k) {
+        // t = w[k] + carry may exceed int64 only if |w[k]| ~ 2^63; words stay
+        // below 2^63 by construction (see header comment).
+        const int64_t t = (k < W ? w[k] : 0) + carry;
+        dig[k] = (uint32_t)(t & 0xffffffffll);
+        carry = t >> 32;  // arithmetic shift
+    }
+    const bool neg = (dig[ND - 1] >> 31) & 1;
+    if (neg) {  // two's complement negate
+        uint64_t c = 1;
+        for (int k = 0; k < ND; ++k) {
+            const uint64_t v = (uint64_t)(uint32_t)~dig[k] + c;
+            dig[k] = (uint32_t)v;
+            c = v >> 32;
+        }
+    }
+    int top = ND - 1;
+    while (top >= 0 && dig[top] == 0) --top;
+    if (top < 0) return 0.0;
+    const int lz = __builtin_clz(dig[top]);
+    const int msb = top * 32 + (31 - lz);  // bit index of the leading one
+    // bits [lo, lo + 63] of the magnitude (bits below 0 read as 0)
+    auto digit = [&](int q) -> uint64_t { return (q >= 0 && q < ND) ? (uint64_t)dig[q] : 0ull; };
+    auto get64 = [&](int lo) -> uint64_t {
+        const int q = lo >= 0 ? lo / 32 : -((-lo + 31) / 32);
+        const int s = lo - q * 32;  // 0..31
+        const uint64_t a = digit(q) | (digit(q + 1) << 32);
+        const uint64_t b = digit(q + 2);
+        return s == 0 ? a : ((a >> s) | (b << (64 - s)));
+    };
+    const uint64_t win = get64(msb - 63);  // leading one at bit 63
+    uint64_t mant = win >> 11;             // 53 bits
+    const uint64_t rest = win & 0x7ffull;  // 11 bits below the mantissa
+    const uint64_t half = 0x400ull;
+    bool sticky = false;
+    const int lim = msb - 64;  // highest bit index below the window
+    for (int q = 0; q < ND && 32 * q <= lim; ++q) {
+        const int span = lim - 32 * q + 1;  // bits of this digit below the window
+        const uint32_t m = span >= 32 ? 0xffffffffu : ((1u << span) - 1u);
+        if (dig[q] & m) sticky = true;
+    }
+    int exp2 = msb - 52 + L;  // value = mant * 2^exp2
+    if (rest > half || (rest == half && (sticky || (mant & 1)))) {
+        ++mant;
+        if (mant == (1ull << 53)) {
+            mant >>= 1;
+            ++exp2;
+        }
+    }
+    // mant < 2^53 exactly representable; scaling by 2^exp2 is exact unless the
+    // result overflows (-> inf) or is subnormal (then mant had no bits to round).
+    double r = (double)mant;
+#ifdef __HIP_DEVICE_COMPILE__
+    r = ldexp(r, exp2);
+#else
+    r = __builtin_ldexp(r, exp2);
+#endif
+    return neg ? -r : r;
+}
+
+// Error-free addition (Knuth TwoSum): a + b = s + err exactly.
+FGPU_HD void two_sum(double a, double b, double &s, double &err) {
+    s = a + b;
+    const double bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+}
+
+// ─── kernel-side column / predicate / aggregate descriptors ─────────────────
+struct DCols {
+    const void *v[MAX_COLS];
+    const uint64_t *valid[MAX_COLS];
+    int8_t dtype[MAX_COLS];
+    int n;
+};
+
+// Loads value of column c at row r as a 64-bit pattern (int32 sign-extended,
+// Utf8 code zero-extended, double bits).
+FGPU_DEV int64_t load_raw(const DCols &c, int k, int64_t r) {
+    switch (c.dtype[k]) {
+        case FGPU_INT32: return (int64_t)((const int32_t *)c.v[k])[r];
+        case FGPU_UTF8: return (int64_t)((const uint32_t *)c.v[k])[r];
+        default: return ((const int64_t *)c.v[k])[r];
+    }
+}
+FGPU_DEV bool load_valid(const DCols &c, int k, int64_t r) {
+    if (c.dtype[k] == FGPU_UTF8) return ((const uint32_t *)c.v[k])[r] != NULL_CODE;
+    const uint64_t *vb = c.valid[k];
+    return vb ? ((vb[r >> 6] >> (r & 63)) & 1) : true;
+}
+
+// Comparison classes resolved on the host from (column type, literal type),
+// following fast_cmp_mask (src/query/expr.rs:1076-1175).
+enum : int8_t {
+    CMP_F64 = 0,     // double column vs double literal
+    CMP_I64 = 1,     // int64 column vs int64 literal
+    CMP_I64_AS_F64 = 2,  // int64 column widened to double
+    CMP_I32 = 3,     // int32 column vs literal narrowed with `as i32`
+    CMP_DICT = 4,    // Utf8 code -> per-dictionary pass table
+    CMP_FALSE = 5,   // unsupported pairing: no row passes
+    CMP_IS_NULL = 6,
+    CMP_IS_NOT_NULL = 7
+};
+
+struct DPredNode {
+    int8_t kind;   // FGPU_P_*
+    int8_t cls;    // CMP_*
+    int8_t op;     // FGPU_EQ..FGPU_GE
+    int8_t col;    // column slot in DCols
+    int64_t lit_i;
+    double lit_f;
+    const uint8_t *pass;  // CMP_DICT
+};
+struct DPred {
+    int n;  // 0 = no predicate (all rows pass)
+    DPredNode nodes[FGPU_MAX_PRED_NODES];
+};
+
+template <class T>
+FGPU_DEV bool cmp_op(T x, T v, int op) {
+    switch (op) {
+        case FGPU_EQ: return x == v;
+        case FGPU_NE: return x != v;
+        case FGPU_LT: return x < v;
+        case FGPU_LE: return x <= v;
+        case FGPU_GT: return x > v;
+        default: return x >= v;
+    }
+}
+
+FGPU_DEV bool eval_leaf(const DPredNode &nd, const DCols &c, int64_t r) {
+    const int k = nd.col;
+    if (nd.cls == CMP_IS_NULL) return !load_valid(c, k, r);
+    if (nd.cls == CMP_IS_NOT_NULL) return load_valid(c, k, r);
+    if (nd.cls == CMP_FALSE) return false;
+    bool h;
+    switch (nd.cls) {
+        case CMP_F64: {
+            const double x = ((const double *)c.v[k])[r];
+            h = cmp_op(x, nd.lit_f, nd.op);
+            break;
+        }
+        case CMP_I64: h = cmp_op(((const int64_t *)c.v[k])[r], nd.lit_i, nd.op); break;
+        case CMP_I64_AS_F64: h = cmp_op((double)((const int64_t *)c.v[k])[r], nd.lit_f, nd.op); break;
+        case CMP_I32: h = cmp_op(((const int32_t *)c.v[k])[r], (int32_t)nd.lit_i, nd.op); break;
+        default: {  // CMP_DICT: NULL code fails; the pass table has one byte per entry
+            const uint32_t code = ((const uint32_t *)c.v[k])[r];
+            return code != NULL_CODE && nd.pass[code];
+        }
+    }
+    // clear_null_bits: a comparison against NULL is never true
+    if (c.valid[k] && !((c.valid[k][r >> 6] >> (r & 63)) & 1)) h = false;
+    return h;
+}
+
+FGPU_DEV bool eval_pred(const DPred &p, const DCols &c, int64_t r) {
+    if (p.n == 0) return true;
+    if (p.n == 1) return eval_leaf(p.nodes[0], c, r);
+    uint32_t st = 0;  // bit stack
+    int sp = 0;
+    for (int i = 0; i < p.n; ++i) {
+        const DPredNode &nd = p.nodes[i];
+        if (nd.kind == FGPU_P_AND || nd.kind == FGPU_P_OR) {
+            const uint32_t b = (st >> (sp - 1)) & 1, a = (st >> (sp - 2)) & 1;
+            const uint32_t v = nd.kind == FGPU_P_AND ? (a & b) : (a | b);
+            sp -= 2;
+            st = (st & ~(3u << sp)) | (v << sp);
+            ++sp;
+        } else {
+            const uint32_t v = eval_leaf(nd, c, r) ? 1u : 0u;
+            st = (st & ~(1u << sp)) | (v << sp);
+            ++sp;
+        }
+    }
+    return (st >> (sp - 1)) & 1;
+}
+
+// ─── group state operations ─────────────────────────────────────────────────
+// Word 0 of every group state is its row count.
+enum : int8_t {
+    OP_CNT = 1,    // non-NULL count of a column
+    OP_ISUM = 2,   // exact int64 sum (wrapping)
+    OP_IMIN = 3,
+    OP_IMAX = 4,
+    OP_FSUM = 5,   // exact fixed-point sum: W words at `word`, then a flags word
+    OP_FMIN = 6,   // order-encoded double min (NaN skipped)
+    OP_FMAX = 7
+};
+
+struct DOp {
+    int8_t kind;
+    int8_t col;     // column slot
+    int8_t W;       // OP_FSUM words
+    int8_t has_flags;  // OP_FSUM: column holds NaN/inf, flags word follows the W words
+    int16_t word;   // first state word
+    int16_t L;      // OP_FSUM fixed-point LSB exponent
+};
+struct DOps {
+    int n;
+    int S;  // words per group state
+    DOp op[MAX_OPS];
+    uint64_t ident[64];  // identity value of each state word (S <= 64)
+};
+
+// Flags word bits for non-finite float inputs.
+constexpr uint64_t FL_NAN = 1, FL_PINF = 2, FL_NINF = 4;
+
+FGPU_HD double fx_finish(const int64_t *w, int W, int L, uint64_t flags) {
+    if (flags) {
+        if ((flags & FL_NAN) || ((flags & FL_PINF) && (flags & FL_NINF))) return __builtin_nan("");
+        return (flags & FL_PINF) ? __builtin_inf() : -__builtin_inf();
+    }
+    return fx_to_double(w, W, L);
+}
+
+}  // namespace fgpu

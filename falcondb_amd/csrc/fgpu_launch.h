@@ -1,0 +1,165 @@
+// fgpu_launch.h — kernel argument blocks and host-side launchers (one set per
+// kernel source file).  Everything here is internal to libfalcon_gpu.so.
+#pragma once
+
+#include "fgpu_internal.h"
+
+namespace fgpu {
+
+constexpr int MAX_IN = 6;  // aggregate input columns per query
+
+// Output descriptor of one aggregate, resolved from the state layout.
+enum : int8_t {
+    OUT_COUNT_ROWS = 0,  // COUNT(*): word 0
+    OUT_COUNT = 1,       // COUNT(col): a count word
+    OUT_SUM_I = 2,
+    OUT_AVG_I = 3,
+    OUT_SUM_F = 4,
+    OUT_AVG_F = 5,
+    OUT_MIN_I = 6,
+    OUT_MAX_I = 7,
+    OUT_MIN_F = 8,
+    OUT_MAX_F = 9
+};
+struct DOut {
+    int8_t kind;
+    int8_t W;
+    int8_t has_flags;
+    int8_t pad;
+    int16_t word;      // value word (first fixed-point word for float sums)
+    int16_t cnt_word;  // non-NULL count word (0 = row count)
+    int16_t L;
+};
+struct DOuts {
+    int n;
+    DOut o[FGPU_MAX_AGGS];
+};
+
+// ─── GROUP BY ───────────────────────────────────────────────────────────────
+enum : int { GB_DENSE_LDS = 0, GB_DENSE_GLOBAL = 1, GB_HASH = 2, GB_SCALAR = 3 };
+
+struct GbArgs {
+    DCols cols;
+    DPred pred;
+    DOps ops;
+    int n_in;
+    int8_t in_col[MAX_IN];  // column slot of each aggregate input
+    int key_col;            // column slot of the key (-1 = scalar)
+    int key_is_utf8;
+    uint32_t utf8_null_code;  // code NULL Utf8 keys are mapped to
+    int64_t key_min;          // dense modes: index = key - key_min
+    int64_t range;            // dense modes: key slots; slot `range` = NULL group
+    int64_t n_rows;
+    int64_t *keys;            // hash mode: cap keys
+    uint64_t *state;          // (cap + 1) * S words (dense: (range + 1) * S)
+    uint64_t cap_mask;        // hash mode: cap - 1
+    unsigned long long *counters;  // [0] groups inserted, [1] overflow flag
+    int lds_slots;            // hash mode: LDS cache slots (power of two, 0 = off)
+    int probe_limit;          // hash mode: LDS probes before falling back to HBM
+};
+
+struct EmitArgs {
+    DOuts outs;
+    const int64_t *keys;      // hash mode (else NULL)
+    const uint64_t *state;
+    int S;
+    int64_t n_slots;          // hash: cap + 1; dense: range + 1; scalar: 1
+    int mode;
+    int64_t key_min;
+    int key_is_utf8;
+    int64_t *out_key;
+    uint8_t *out_key_valid;
+    int64_t *out_val[FGPU_MAX_AGGS];
+    uint8_t *out_valid[FGPU_MAX_AGGS];
+    unsigned long long *out_count;
+};
+
+hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint64_t *state,
+                             int64_t n_groups, const DOps &ops);
+hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,
+                          size_t lds_bytes);
+hipError_t launch_emit(hipStream_t s, const EmitArgs &a);
+
+// ─── WHERE ──────────────────────────────────────────────────────────────────
+hipError_t launch_filter_count(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
+                               unsigned long long *d_count, int grid);
+hipError_t launch_filter_mask(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
+                              uint64_t *d_words, int grid);
+// Exclusive prefix sum of per-word popcounts: offs[i] = sum_{j<i} popcount(words[j]);
+// offs[n_words] = total.  `block_tmp` holds ceil(n_words / 4096) + 1 entries.
+hipError_t launch_word_offsets(hipStream_t s, const uint64_t *words, int64_t n_words,
+                               int64_t *offs, int64_t *block_tmp);
+hipError_t launch_gather(hipStream_t s, const uint64_t *words, const int64_t *offs,
+                         int64_t n_words, const void *src, int elem_bytes,
+                         const uint64_t *src_valid, void *dst, uint8_t *dst_valid, int grid);
+// Exclusive prefix sum of u32 counts into int64 offsets (offs[n] = total).
+hipError_t launch_scan_u32(hipStream_t s, const uint32_t *cnt, int64_t n, int64_t *offs,
+                           int64_t *block_tmp);
+
+// ─── column statistics (upload) ─────────────────────────────────────────────
+struct ColStatsDev {
+    long long imin, imax;               // integers / codes
+    unsigned long long fmin_enc, fmax_enc;  // doubles (order-encoded, NaN skipped)
+    int emin, emax;                     // mantissa-LSB exponents of finite nonzero doubles
+    unsigned long long nonfinite;       // flags FL_*
+    unsigned long long nulls;
+    unsigned int hll[4096];             // HyperLogLog registers (p = 12)
+};
+hipError_t launch_col_stats(hipStream_t s, const void *values, int dtype, const uint64_t *valid,
+                            int64_t n, ColStatsDev *d_stats, int grid);
+
+// ─── hash join ──────────────────────────────────────────────────────────────
+struct JoinBuildArgs {
+    const void *keys_col;   // build key column
+    int key_dtype;          // FGPU_INT32 / FGPU_INT64
+    const uint64_t *key_valid;
+    int64_t n;
+    int64_t *keys;          // cap slots, EMPTY = INT64_MIN
+    uint32_t *rows;
+    uint64_t cap_mask;
+    unsigned long long *counters;  // [0] inserted, [1] overflow, [2] rows with key INT64_MIN
+    uint32_t *min_key_rows;        // build rows whose key is INT64_MIN (list)
+    int64_t min_key_cap;
+};
+
+// Per-thread register aggregate ops of a join / scalar aggregate.
+struct JoinOp {
+    int8_t kind;     // OP_CNT .. OP_FMAX (OP_CNT with col < 0 = COUNT(*))
+    int8_t side;     // 0 probe, 1 build
+    int8_t col;      // column slot within that side's DCols
+    int8_t pad;
+    int16_t word;    // destination word in the global state row
+    int16_t L;
+    int8_t W;
+    int8_t has_flags;
+};
+struct JoinProbeArgs {
+    DCols pcols;      // probe columns
+    DCols bcols;      // build columns
+    DPred pred;       // probe-side WHERE
+    int pkey;         // probe key slot in pcols
+    int type;         // FGPU_INNER..FGPU_ANTI
+    int64_t n_probe;
+    const int64_t *keys;
+    const uint32_t *rows;
+    uint64_t cap_mask;
+    const uint32_t *min_key_rows;  // build rows with key INT64_MIN
+    int64_t n_min_key_rows;
+    int n_ops;
+    JoinOp ops[FGPU_MAX_AGGS];
+    int S;
+    uint64_t *state;  // one state row (S words), identity-initialised
+    // pair mode
+    uint32_t *counts;     // pass 1: output rows per probe row
+    const int64_t *offs;  // pass 2: exclusive offsets
+    uint32_t *out_l;
+    uint32_t *out_r;
+};
+hipError_t launch_join_build(hipStream_t s, const JoinBuildArgs &a, int grid);
+hipError_t launch_join_probe_agg(hipStream_t s, const JoinProbeArgs &a, int grid, int block);
+hipError_t launch_join_pairs(hipStream_t s, const JoinProbeArgs &a, int pass, int grid);
+
+// Fill helpers
+hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v);
+
+}  // namespace fgpu

@@ -1,0 +1,1380 @@
+// fgpu_runtime.hip — host side of libfalcon_gpu.so: contexts, device-resident
+// tables, plan compilation (predicates, aggregate state layouts, strategy
+// choice) and the C ABI declared in include/falcon_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fgpu_launch.h"
+
+using namespace fgpu;
+
+// ─── errors ─────────────────────────────────────────────────────────────────
+namespace {
+thread_local std::string g_err;
+
+int set_err(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            const int c_ = e_ == hipErrorOutOfMemory ? FGPU_E_OUT_OF_MEMORY : FGPU_E_DEVICE; \
+            return set_err(c_, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+        }                                                                              \
+    } while (0)
+
+int64_t next_pow2(int64_t x) {
+    int64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+int env_int(const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+}
+}  // namespace
+
+// ─── device column / table ──────────────────────────────────────────────────
+struct DevCol {
+    std::string name;
+    int dtype = 0;
+    void *d_values = nullptr;
+    uint64_t *d_valid = nullptr;
+    int64_t null_count = 0;
+    std::vector<std::string> dict;
+    std::vector<const char *> dictp;
+    int64_t empty_code = -1;  // dictionary index of "" (Utf8)
+    // statistics
+    int64_t imin = 0, imax = -1;
+    double fmin = 0, fmax = 0;
+    int emin = 0, emax = -1;  // float mantissa-LSB exponents (emin > emax: no finite nonzero)
+    uint64_t nonfinite = 0;
+    int64_t distinct = 0;
+    size_t elem_bytes() const { return dtype == FGPU_INT64 || dtype == FGPU_FLOAT64 ? 8 : 4; }
+};
+
+struct fgpu_table {
+    int device = 0;
+    int64_t n_rows = 0;
+    int32_t batch_rows = 65536;
+    std::vector<DevCol> cols;
+    int64_t bytes = 0;
+};
+
+struct fgpu_join_index {
+    int device = 0;
+    const fgpu_table *build = nullptr;
+    int key_col = -1;
+    int64_t cap = 0;
+    int64_t *keys = nullptr;
+    uint32_t *rows = nullptr;
+    uint32_t *min_key_rows = nullptr;
+    int64_t n_min_key_rows = 0;
+    int64_t bytes = 0;
+};
+
+// ─── results ────────────────────────────────────────────────────────────────
+struct RCol {
+    std::string name;
+    int dtype = FGPU_INT64;
+    std::vector<int64_t> i64;
+    std::vector<double> f64;
+    std::vector<uint32_t> u32;
+    std::vector<uint8_t> valid;
+    const DevCol *dict_src = nullptr;
+};
+struct fgpu_result {
+    int64_t rows = 0;
+    std::vector<RCol> cols;
+};
+
+// ─── context ────────────────────────────────────────────────────────────────
+enum BufSlot {
+    B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT, B_NSLOTS
+};
+
+struct Tuning {
+    int gb_block = 256;
+    int gb_waves_per_cu = 16;      // target resident waves per CU for streaming kernels
+    int hash_lds_slots = 1024;
+    int probe_limit = 8;
+    int dense_lds_bytes = 48 * 1024;
+    int join_block = 256;
+    int join_waves_per_cu = 16;
+    int filter_waves_per_cu = 32;
+    int keep_on_device = 0;  // 1: results stay in device scratch (only the row count is copied)
+};
+
+struct fgpu_ctx {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    struct Buf {
+        void *p = nullptr;
+        size_t n = 0;
+    } bufs[B_NSLOTS];
+    bool timing = false;
+    hipEvent_t ev[6] = {};
+    fgpu_timing last{};
+    Tuning tune;
+    std::mutex mu;
+};
+
+namespace {
+
+int buf(fgpu_ctx *c, BufSlot s, size_t bytes, void **out) {
+    auto &b = c->bufs[s];
+    if (b.n < bytes) {
+        if (b.p) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipFree(b.p));
+            b.p = nullptr;
+            b.n = 0;
+        }
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        HIPCHK(hipMalloc(&b.p, want));
+        b.n = want;
+    }
+    *out = b.p;
+    return FGPU_OK;
+}
+
+int set_device(int dev) {
+    HIPCHK(hipSetDevice(dev));
+    return FGPU_OK;
+}
+
+// Events: 0 start, 1 main begin, 2 main end, 3 end, 4 build begin, 5 build end.
+void mark(fgpu_ctx *c, int i) {
+    if (c->timing) (void)hipEventRecord(c->ev[i], c->stream);
+}
+void finish_timing(fgpu_ctx *c, int64_t bytes_main, bool has_build) {
+    if (!c->timing) return;
+    (void)hipEventSynchronize(c->ev[3]);
+    float t = 0;
+    c->last = fgpu_timing{};
+    if (hipEventElapsedTime(&t, c->ev[0], c->ev[3]) == hipSuccess) c->last.total_ms = t;
+    if (hipEventElapsedTime(&t, c->ev[1], c->ev[2]) == hipSuccess) c->last.main_ms = t;
+    if (has_build && hipEventElapsedTime(&t, c->ev[4], c->ev[5]) == hipSuccess) c->last.build_ms = t;
+    if (hipEventElapsedTime(&t, c->ev[2], c->ev[3]) == hipSuccess) c->last.finalize_ms = t;
+    c->last.bytes_main = bytes_main;
+}
+
+int grid_for(const fgpu_ctx *c, int64_t work_items, int block, int waves_per_cu) {
+    const int64_t need = std::max<int64_t>(1, (work_items + block - 1) / block);
+    const int64_t resident = (int64_t)c->num_cus * std::max(1, waves_per_cu * 64 / block);
+    return (int)std::max<int64_t>(1, std::min(need, resident));
+}
+
+// Maps table columns to kernel column slots.
+struct Binder {
+    const fgpu_table *t;
+    DCols dc{};
+    int slot_of[256];
+    explicit Binder(const fgpu_table *tt) : t(tt) {
+        dc.n = 0;
+        for (int &x : slot_of) x = -1;
+    }
+    int bind(int col) {
+        if (col < 0 || col >= (int)t->cols.size()) return -1;
+        if (col < 256 && slot_of[col] >= 0) return slot_of[col];
+        if (dc.n >= MAX_COLS) return -2;
+        const DevCol &c = t->cols[col];
+        const int s = dc.n++;
+        dc.v[s] = c.d_values;
+        dc.valid[s] = c.d_valid;
+        dc.dtype[s] = (int8_t)c.dtype;
+        if (col < 256) slot_of[col] = s;
+        return s;
+    }
+};
+
+// Compiles a predicate (postfix program) into device form.
+int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b, DPred &out) {
+    out = DPred{};
+    out.n = 0;
+    if (!p || p->n_nodes == 0) return FGPU_OK;
+    if (p->n_nodes > FGPU_MAX_PRED_NODES)
+        return set_err(FGPU_E_INVALID_QUERY, "predicate has %d nodes (max %d)", p->n_nodes, FGPU_MAX_PRED_NODES);
+    // dictionary pass tables live in one device buffer
+    size_t pass_bytes = 0;
+    for (int i = 0; i < p->n_nodes; ++i) {
+        const auto &nd = p->nodes[i];
+        if (nd.kind == FGPU_P_CMP && nd.col >= 0 && nd.col < (int)t->cols.size() &&
+            t->cols[nd.col].dtype == FGPU_UTF8 && nd.lit_type == FGPU_UTF8)
+            pass_bytes += (t->cols[nd.col].dict.size() + 255) & ~size_t(255);
+    }
+    uint8_t *d_pass = nullptr;
+    std::vector<uint8_t> h_pass(pass_bytes);
+    if (pass_bytes) {
+        int rc = buf(c, B_PASS, pass_bytes, (void **)&d_pass);
+        if (rc) return rc;
+    }
+    size_t pass_off = 0;
+    int depth = 0;
+    for (int i = 0; i < p->n_nodes; ++i) {
+        const fgpu_pred_node &nd = p->nodes[i];
+        DPredNode &o = out.nodes[i];
+        o = DPredNode{};
+        o.kind = (int8_t)nd.kind;
+        if (nd.kind == FGPU_P_AND || nd.kind == FGPU_P_OR) {
+            if (depth < 2) return set_err(FGPU_E_INVALID_QUERY, "AND/OR needs two operands");
+            --depth;
+            continue;
+        }
+        if (nd.kind != FGPU_P_CMP && nd.kind != FGPU_P_IS_NULL && nd.kind != FGPU_P_IS_NOT_NULL)
+            return set_err(FGPU_E_INVALID_QUERY, "unknown predicate node kind %d", nd.kind);
+        if (nd.col < 0 || nd.col >= (int)t->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "predicate column %d not in table", nd.col);
+        const int slot = b.bind(nd.col);
+        if (slot < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads more than %d columns", MAX_COLS);
+        o.col = (int8_t)slot;
+        ++depth;
+        if (nd.kind == FGPU_P_IS_NULL) { o.cls = CMP_IS_NULL; continue; }
+        if (nd.kind == FGPU_P_IS_NOT_NULL) { o.cls = CMP_IS_NOT_NULL; continue; }
+        if (nd.op < FGPU_EQ || nd.op > FGPU_GE) return set_err(FGPU_E_INVALID_QUERY, "bad comparison op %d", nd.op);
+        o.op = (int8_t)nd.op;
+        const DevCol &col = t->cols[nd.col];
+        const int lt = nd.lit_type;
+        const bool lit_int = lt == FGPU_INT64 || lt == FGPU_INT32;
+        // fast_cmp_mask pairings (src/query/expr.rs:1111-1171)
+        if (col.dtype == FGPU_FLOAT64 && (lt == FGPU_FLOAT64 || lit_int)) {
+            o.cls = CMP_F64;
+            o.lit_f = lt == FGPU_FLOAT64 ? nd.lit_f64 : (double)nd.lit_i64;
+        } else if (col.dtype == FGPU_INT64 && lit_int) {
+            o.cls = CMP_I64;
+            o.lit_i = nd.lit_i64;
+        } else if (col.dtype == FGPU_INT64 && lt == FGPU_FLOAT64) {
+            o.cls = CMP_I64_AS_F64;
+            o.lit_f = nd.lit_f64;
+        } else if (col.dtype == FGPU_INT32 && lit_int) {
+            o.cls = CMP_I32;
+            o.lit_i = (int64_t)(int32_t)nd.lit_i64;  // `as i32`
+        } else if (col.dtype == FGPU_UTF8 && lt == FGPU_UTF8) {
+            o.cls = CMP_DICT;
+            const char *lit = nd.lit_str ? nd.lit_str : "";
+            for (size_t k = 0; k < col.dict.size(); ++k) {
+                const int r = std::strcmp(col.dict[k].c_str(), lit);
+                bool h = false;
+                switch (nd.op) {
+                    case FGPU_EQ: h = r == 0; break;
+                    case FGPU_NE: h = r != 0; break;
+                    case FGPU_LT: h = r < 0; break;
+                    case FGPU_LE: h = r <= 0; break;
+                    case FGPU_GT: h = r > 0; break;
+                    default: h = r >= 0; break;
+                }
+                h_pass[pass_off + k] = h;
+            }
+            o.pass = d_pass + pass_off;
+            pass_off += (col.dict.size() + 255) & ~size_t(255);
+        } else {
+            o.cls = CMP_FALSE;  // unsupported pairing: no rows pass
+        }
+    }
+    if (depth != 1) return set_err(FGPU_E_INVALID_QUERY, "predicate program does not reduce to one value");
+    out.n = p->n_nodes;
+    if (pass_bytes) HIPCHK(hipMemcpyAsync(d_pass, h_pass.data(), pass_bytes, hipMemcpyHostToDevice, c->stream));
+    return FGPU_OK;
+}
+
+// ─── aggregate state layout ─────────────────────────────────────────────────
+struct OpKey {
+    int kind, side, col;
+};
+
+struct AggLayout {
+    DOps ops{};
+    std::vector<OpKey> keys;  // per op
+    DOuts outs{};
+    std::vector<std::string> names;
+};
+
+int fx_words(const DevCol &c, int &L, int &W) {
+    if (c.emin > c.emax) {  // no finite nonzero value
+        L = 0;
+        W = 1;
+        return FGPU_OK;
+    }
+    // digits of one value reach bit (emax - emin) + 52 + 31 (in-word offset);
+    // a per-thread partial sum (< 2^31 addends) may be 31 bits larger still.
+    L = c.emin;
+    W = ((c.emax - c.emin) + 53 + 31 + 31) / 32 + 1;
+    if (W > MAX_FX_WORDS)
+        return set_err(FGPU_E_UNSUPPORTED,
+                       "column '%s': float exponent range 2^%d..2^%d is too wide for the exact SUM "
+                       "accumulator (%d words > %d)", c.name.c_str(), c.emin, c.emax + 52, W, MAX_FX_WORDS);
+    return FGPU_OK;
+}
+
+// Adds (or finds) an op; returns its index.  `col` is a table column index.
+int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot) {
+    for (size_t i = 0; i < L.keys.size(); ++i)
+        if (L.keys[i].kind == kind && L.keys[i].side == side && L.keys[i].col == col) return (int)i;
+    if (L.ops.n >= MAX_OPS) return set_err(FGPU_E_UNSUPPORTED, "too many aggregate states (max %d)", MAX_OPS) * 0 - 1;
+    DOp o{};
+    o.kind = (int8_t)kind;
+    o.col = (int8_t)slot;
+    o.word = (int16_t)L.ops.S;
+    int nwords = 1;
+    if (kind == OP_FSUM) {
+        int l, w;
+        if (fx_words(dc, l, w)) return -1;
+        o.L = (int16_t)l;
+        o.W = (int8_t)w;
+        o.has_flags = dc.nonfinite ? 1 : 0;
+        nwords = w + o.has_flags;
+    }
+    if (L.ops.S + nwords > 64) {
+        set_err(FGPU_E_UNSUPPORTED, "aggregate state exceeds 64 words");
+        return -1;
+    }
+    for (int k = 0; k < nwords; ++k) {
+        uint64_t id = 0;
+        if (kind == OP_IMIN) id = (uint64_t)INT64_MAX;
+        else if (kind == OP_IMAX) id = (uint64_t)INT64_MIN;
+        else if (kind == OP_FMIN) id = ~0ull;
+        L.ops.ident[L.ops.S + k] = id;
+    }
+    L.ops.S += nwords;
+    L.ops.op[L.ops.n++] = o;
+    L.keys.push_back({kind, side, col});
+    return L.ops.n - 1;
+}
+
+std::string alias_of(const fgpu_agg &a, int i) {
+    if (a.alias && *a.alias) return a.alias;
+    return "col" + std::to_string(i + 1);
+}
+
+// Builds ops + outputs for aggregates over one or two tables.  `slot_of(side,
+// col)` binds a column and returns its slot (or <0).
+template <class SlotFn>
+int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const fgpu_table *t1,
+                 SlotFn slot_of, AggLayout &L) {
+    L = AggLayout{};
+    L.ops.n = 0;
+    L.ops.S = 1;  // word 0 = rows
+    L.ops.ident[0] = 0;
+    if (n_aggs > FGPU_MAX_AGGS) return set_err(FGPU_E_UNSUPPORTED, "at most %d aggregates", FGPU_MAX_AGGS);
+    L.outs.n = n_aggs;
+    for (int i = 0; i < n_aggs; ++i) {
+        const fgpu_agg &a = aggs[i];
+        L.names.push_back(alias_of(a, i));
+        DOut &o = L.outs.o[i];
+        o = DOut{};
+        if (a.func < FGPU_COUNT || a.func > FGPU_MAX) return set_err(FGPU_E_INVALID_QUERY, "unknown aggregate %d", a.func);
+        if (a.col < 0) {
+            if (a.func != FGPU_COUNT) return set_err(FGPU_E_INVALID_QUERY, "only COUNT may take *");
+            o.kind = OUT_COUNT_ROWS;
+            continue;
+        }
+        const int side = t1 ? a.side : 0;
+        const fgpu_table *t = side == 1 ? t1 : t0;
+        if (!t || a.col >= (int)t->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "aggregate column %d not found", a.col);
+        const DevCol &dc = t->cols[a.col];
+        const int slot = slot_of(side, a.col);
+        if (slot < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads too many columns");
+        const bool nullable = dc.null_count > 0;
+        const bool isf = dc.dtype == FGPU_FLOAT64;
+        if (dc.dtype == FGPU_UTF8 && a.func != FGPU_COUNT)
+            return set_err(FGPU_E_INVALID_QUERY, "Cannot aggregate '%s' as float", L.names.back().c_str());
+        int cnt_word = 0;
+        // A NULL build side (LEFT) makes every build column nullable.
+        const bool need_cnt = nullable || side == 1 || a.func == FGPU_AVG || a.func == FGPU_COUNT;
+        if (need_cnt && (nullable || side == 1)) {
+            const int ci = add_op(L, OP_CNT, side, a.col, dc, slot);
+            if (ci < 0) return FGPU_E_UNSUPPORTED;
+            cnt_word = L.ops.op[ci].word;
+        }
+        o.cnt_word = (int16_t)cnt_word;
+        int kind = 0;
+        switch (a.func) {
+            case FGPU_COUNT:
+                o.kind = cnt_word ? OUT_COUNT : OUT_COUNT_ROWS;
+                o.word = (int16_t)cnt_word;
+                continue;
+            case FGPU_SUM: kind = isf ? OP_FSUM : OP_ISUM; o.kind = isf ? OUT_SUM_F : OUT_SUM_I; break;
+            case FGPU_AVG: kind = isf ? OP_FSUM : OP_ISUM; o.kind = isf ? OUT_AVG_F : OUT_AVG_I; break;
+            case FGPU_MIN: kind = isf ? OP_FMIN : OP_IMIN; o.kind = isf ? OUT_MIN_F : OUT_MIN_I; break;
+            default: kind = isf ? OP_FMAX : OP_IMAX; o.kind = isf ? OUT_MAX_F : OUT_MAX_I; break;
+        }
+        const int oi = add_op(L, kind, side, a.col, dc, slot);
+        if (oi < 0) return g_err.empty() ? set_err(FGPU_E_UNSUPPORTED, "aggregate layout") : FGPU_E_UNSUPPORTED;
+        const DOp &op = L.ops.op[oi];
+        o.word = op.word;
+        o.W = op.W;
+        o.L = op.L;
+        o.has_flags = op.has_flags;
+    }
+    return FGPU_OK;
+}
+
+// Copies the emitted rows of the last GROUP BY / aggregate into a result.
+int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key, int key_dtype,
+            const std::string &key_name, int64_t *d_key, uint8_t *d_key_valid,
+            int64_t **d_val, uint8_t **d_valid, unsigned long long *d_count, fgpu_result **out) {
+    unsigned long long n = 0;
+    HIPCHK(hipMemcpyAsync(&n, d_count, sizeof n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((int64_t)n > n_slots) return set_err(FGPU_E_DEVICE, "emit counter %llu exceeds %lld slots", n, (long long)n_slots);
+    auto *r = new fgpu_result();
+    r->rows = (int64_t)n;
+    if (c->tune.keep_on_device) n = 0;  // columns stay in device scratch
+    if (key) {
+        RCol kc;
+        kc.name = key_name;
+        kc.dtype = key_dtype == FGPU_UTF8 ? FGPU_UTF8 : FGPU_INT64;
+        kc.i64.resize(n);
+        kc.valid.resize(n);
+        if (n) {
+            (void)hipMemcpyAsync(kc.i64.data(), d_key, n * 8, hipMemcpyDeviceToHost, c->stream);
+            (void)hipMemcpyAsync(kc.valid.data(), d_key_valid, n, hipMemcpyDeviceToHost, c->stream);
+        }
+        kc.dict_src = key_dtype == FGPU_UTF8 ? key : nullptr;
+        r->cols.push_back(std::move(kc));
+    }
+    for (int i = 0; i < L.outs.n; ++i) {
+        RCol rc;
+        rc.name = L.names[i];
+        const int k = L.outs.o[i].kind;
+        const bool is_f = k == OUT_AVG_I || k == OUT_SUM_F || k == OUT_AVG_F || k == OUT_MIN_F || k == OUT_MAX_F;
+        rc.dtype = is_f ? FGPU_FLOAT64 : FGPU_INT64;
+        rc.valid.resize(n);
+        if (is_f) rc.f64.resize(n);
+        else rc.i64.resize(n);
+        if (n) {
+            void *dst = is_f ? (void *)rc.f64.data() : (void *)rc.i64.data();
+            (void)hipMemcpyAsync(dst, d_val[i], n * 8, hipMemcpyDeviceToHost, c->stream);
+            (void)hipMemcpyAsync(rc.valid.data(), d_valid[i], n, hipMemcpyDeviceToHost, c->stream);
+        }
+        r->cols.push_back(std::move(rc));
+    }
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        delete r;
+        return set_err(FGPU_E_DEVICE, "result copy: %s", hipGetErrorString(e));
+    }
+    // Utf8 keys: codes as u32
+    if (key && key_dtype == FGPU_UTF8) {
+        RCol &kc = r->cols[0];
+        kc.u32.resize(n);
+        for (size_t i = 0; i < n; ++i) kc.u32[i] = (uint32_t)kc.i64[i];
+        kc.i64.clear();
+        kc.i64.shrink_to_fit();
+    }
+    *out = r;
+    return FGPU_OK;
+}
+
+// Emit-stage buffers carved from the B_OUT slot.
+struct OutBufs {
+    int64_t *key;
+    uint8_t *key_valid;
+    int64_t *val[FGPU_MAX_AGGS];
+    uint8_t *valid[FGPU_MAX_AGGS];
+    unsigned long long *count;
+};
+int out_bufs(fgpu_ctx *c, int64_t n_slots, int n_aggs, OutBufs &ob) {
+    const size_t per = (size_t)((n_slots * 9 + 255) & ~int64_t(255));
+    void *p;
+    int rc = buf(c, B_OUT, per * (size_t)(n_aggs + 1) + 256, &p);
+    if (rc) return rc;
+    char *b = (char *)p;
+    ob.count = (unsigned long long *)b;
+    b += 256;
+    ob.key = (int64_t *)b;
+    ob.key_valid = (uint8_t *)(b + n_slots * 8);
+    b += per;
+    for (int i = 0; i < n_aggs; ++i) {
+        ob.val[i] = (int64_t *)b;
+        ob.valid[i] = (uint8_t *)(b + n_slots * 8);
+        b += per;
+    }
+    for (int i = n_aggs; i < FGPU_MAX_AGGS; ++i) {
+        ob.val[i] = nullptr;
+        ob.valid[i] = nullptr;
+    }
+    return FGPU_OK;
+}
+
+double hll_estimate(const unsigned int *reg) {
+    const int m = 4096;
+    double sum = 0;
+    int zeros = 0;
+    for (int i = 0; i < m; ++i) {
+        sum += std::ldexp(1.0, -(int)reg[i]);
+        zeros += reg[i] == 0;
+    }
+    const double alpha = 0.7213 / (1.0 + 1.079 / m);
+    double e = alpha * m * (double)m / sum;
+    if (e <= 2.5 * m && zeros) e = m * std::log((double)m / zeros);  // linear counting
+    return e;
+}
+
+}  // namespace
+
+// ─── C ABI ──────────────────────────────────────────────────────────────────
+extern "C" {
+
+const char *fgpu_last_error(void) { return g_err.c_str(); }
+int fgpu_abi_version(void) { return FGPU_ABI_VERSION; }
+
+int fgpu_open(int device, fgpu_ctx **out) {
+    if (!out) return set_err(FGPU_E_INVALID_QUERY, "out is NULL");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(FGPU_E_DEVICE, "device %d not present (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    auto *c = new fgpu_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return set_err(FGPU_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+    Tuning &t = c->tune;
+    t.gb_block = env_int("FGPU_GB_BLOCK", t.gb_block);
+    t.gb_waves_per_cu = env_int("FGPU_GB_WAVES", t.gb_waves_per_cu);
+    t.hash_lds_slots = env_int("FGPU_HASH_LDS_SLOTS", t.hash_lds_slots);
+    t.probe_limit = env_int("FGPU_PROBE_LIMIT", t.probe_limit);
+    t.join_block = env_int("FGPU_JOIN_BLOCK", t.join_block);
+    t.join_waves_per_cu = env_int("FGPU_JOIN_WAVES", t.join_waves_per_cu);
+    t.filter_waves_per_cu = env_int("FGPU_FILTER_WAVES", t.filter_waves_per_cu);
+    *out = c;
+    return FGPU_OK;
+}
+
+void fgpu_close(fgpu_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->bufs)
+        if (b.p) (void)hipFree(b.p);
+    for (auto &ev : c->ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void *fgpu_stream(fgpu_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int fgpu_set_timing(fgpu_ctx *c, int on) {
+    if (!c) return set_err(FGPU_E_INVALID_QUERY, "ctx is NULL");
+    c->timing = on != 0;
+    return FGPU_OK;
+}
+int fgpu_last_timing(const fgpu_ctx *c, fgpu_timing *out) {
+    if (!c || !out) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    *out = c->last;
+    return FGPU_OK;
+}
+int64_t fgpu_scratch_bytes(const fgpu_ctx *c) {
+    int64_t s = 0;
+    if (c)
+        for (auto &b : c->bufs) s += (int64_t)b.n;
+    return s;
+}
+
+// Tuning knobs (the autotuner sets these): name = value; unknown name -> error.
+int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
+    if (!c || !name) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    Tuning &t = c->tune;
+    const std::string n = name;
+    if (n == "gb_block") t.gb_block = value;
+    else if (n == "gb_waves_per_cu") t.gb_waves_per_cu = value;
+    else if (n == "hash_lds_slots") t.hash_lds_slots = value;
+    else if (n == "probe_limit") t.probe_limit = value;
+    else if (n == "dense_lds_bytes") t.dense_lds_bytes = value;
+    else if (n == "join_block") t.join_block = value;
+    else if (n == "join_waves_per_cu") t.join_waves_per_cu = value;
+    else if (n == "filter_waves_per_cu") t.filter_waves_per_cu = value;
+    else if (n == "keep_on_device") t.keep_on_device = value;
+    else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
+    if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
+        t.join_block > 1024 || t.join_block % 64)
+        return set_err(FGPU_E_INVALID_QUERY, "block sizes must be multiples of 64 in [64, 1024]");
+    if (t.hash_lds_slots & (t.hash_lds_slots - 1))
+        return set_err(FGPU_E_INVALID_QUERY, "hash_lds_slots must be a power of two");
+    return FGPU_OK;
+}
+
+// ─── tables ─────────────────────────────────────────────────────────────────
+void fgpu_table_free(fgpu_table *t) {
+    if (!t) return;
+    (void)hipSetDevice(t->device);
+    for (auto &c : t->cols) {
+        if (c.d_values) (void)hipFree(c.d_values);
+        if (c.d_valid) (void)hipFree(c.d_valid);
+    }
+    delete t;
+}
+
+int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols, int64_t n_rows,
+                      int32_t batch_rows, fgpu_table **out) {
+    if (!ctx || !out || (n_cols > 0 && !cols)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    if (n_rows < 0 || n_cols < 0) return set_err(FGPU_E_INVALID_QUERY, "negative size");
+    if (n_rows >= (int64_t(1) << 31))
+        return set_err(FGPU_E_UNSUPPORTED, "tables are limited to 2^31-1 rows (u32 row ids)");
+    if (batch_rows <= 0 || batch_rows % 64) return set_err(FGPU_E_INVALID_QUERY, "batch_rows must be a positive multiple of 64");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int rc = set_device(ctx->device)) return rc;
+    auto *t = new fgpu_table();
+    t->device = ctx->device;
+    t->n_rows = n_rows;
+    t->batch_rows = batch_rows;
+    t->cols.resize(n_cols);
+    auto fail = [&](int rc) {
+        fgpu_table_free(t);
+        return rc;
+    };
+    ColStatsDev *d_st = nullptr;
+    if (hipMalloc(&d_st, sizeof(ColStatsDev)) != hipSuccess)
+        return fail(set_err(FGPU_E_OUT_OF_MEMORY, "stats buffer"));
+    int rc = FGPU_OK;
+    for (int i = 0; i < n_cols && rc == FGPU_OK; ++i) {
+        const fgpu_column_desc &d = cols[i];
+        DevCol &c = t->cols[i];
+        c.name = d.name ? d.name : "";
+        c.dtype = d.dtype;
+        if (d.dtype != FGPU_INT32 && d.dtype != FGPU_INT64 && d.dtype != FGPU_FLOAT64 && d.dtype != FGPU_UTF8) {
+            rc = set_err(FGPU_E_TYPE_MISMATCH, "column '%s': unsupported dtype %d", c.name.c_str(), d.dtype);
+            break;
+        }
+        if (n_rows > 0 && !d.values) {
+            rc = set_err(FGPU_E_INVALID_QUERY, "column '%s': values is NULL", c.name.c_str());
+            break;
+        }
+        if (d.dtype == FGPU_UTF8) {
+            if (d.dict_len < 0 || (d.dict_len > 0 && !d.dict)) {
+                rc = set_err(FGPU_E_INVALID_QUERY, "column '%s': bad dictionary", c.name.c_str());
+                break;
+            }
+            for (int k = 0; k < d.dict_len; ++k) {
+                c.dict.push_back(d.dict[k] ? d.dict[k] : "");
+                if (c.empty_code < 0 && c.dict.back().empty()) c.empty_code = k;
+            }
+            for (auto &s : c.dict) c.dictp.push_back(s.c_str());
+            // every non-NULL code must index the dictionary
+            const uint32_t *codes = (const uint32_t *)d.values;
+            for (int64_t r = 0; r < n_rows; ++r)
+                if (codes[r] != NULL_CODE && codes[r] >= (uint32_t)d.dict_len) {
+                    rc = set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "column '%s': code %u at row %lld >= dictionary size %d",
+                                 c.name.c_str(), codes[r], (long long)r, d.dict_len);
+                    break;
+                }
+            if (rc) break;
+        }
+        const size_t bytes = (size_t)std::max<int64_t>(n_rows, 1) * c.elem_bytes();
+        hipError_t e = hipMalloc(&c.d_values, bytes);
+        if (e != hipSuccess) { rc = set_err(FGPU_E_OUT_OF_MEMORY, "column '%s': %zu bytes", c.name.c_str(), bytes); break; }
+        t->bytes += (int64_t)bytes;
+        if (n_rows) e = hipMemcpy(c.d_values, d.values, (size_t)n_rows * c.elem_bytes(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "upload: %s", hipGetErrorString(e)); break; }
+        if (d.validity && d.dtype != FGPU_UTF8) {
+            const size_t vb = (size_t)((n_rows + 63) / 64) * 8;
+            e = hipMalloc(&c.d_valid, std::max<size_t>(vb, 8));
+            if (e == hipSuccess && vb) e = hipMemcpy(c.d_valid, d.validity, vb, hipMemcpyHostToDevice);
+            if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "validity upload: %s", hipGetErrorString(e)); break; }
+            t->bytes += (int64_t)vb;
+        }
+        // statistics
+        ColStatsDev h{};
+        h.imin = INT64_MAX;
+        h.imax = INT64_MIN;
+        h.fmin_enc = ~0ull;
+        h.fmax_enc = 0;
+        h.emin = 1 << 20;
+        h.emax = -(1 << 20);
+        e = hipMemcpy(d_st, &h, sizeof h, hipMemcpyHostToDevice);
+        if (e == hipSuccess && n_rows) {
+            const int grid = grid_for(ctx, n_rows, BLOCK, 8);
+            e = launch_col_stats(nullptr, c.d_values, c.dtype, c.d_valid, n_rows, d_st, grid);
+        }
+        if (e == hipSuccess) e = hipMemcpy(&h, d_st, sizeof h, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "column stats: %s", hipGetErrorString(e)); break; }
+        c.null_count = (int64_t)h.nulls;
+        c.imin = h.imin;
+        c.imax = h.imax;
+        c.fmin = __builtin_bit_cast(double, f64_unorder(h.fmin_enc));
+        c.fmax = __builtin_bit_cast(double, f64_unorder(h.fmax_enc));
+        c.emin = h.emin;
+        c.emax = h.emax;
+        c.nonfinite = h.nonfinite;
+        c.distinct = (int64_t)std::llround(hll_estimate(h.hll));
+    }
+    (void)hipFree(d_st);
+    if (rc) return fail(rc);
+    *out = t;
+    return FGPU_OK;
+}
+
+int64_t fgpu_table_rows(const fgpu_table *t) { return t ? t->n_rows : -1; }
+int64_t fgpu_table_device_bytes(const fgpu_table *t) { return t ? t->bytes : 0; }
+
+int fgpu_table_column_stats(const fgpu_table *t, int col, int64_t *min_i64, int64_t *max_i64,
+                            int64_t *null_count, int64_t *distinct_est) {
+    if (!t || col < 0 || col >= (int)t->cols.size()) return set_err(FGPU_E_COLUMN_NOT_FOUND, "column %d", col);
+    const DevCol &c = t->cols[col];
+    if (min_i64) *min_i64 = c.dtype == FGPU_FLOAT64 ? __builtin_bit_cast(int64_t, c.fmin) : c.imin;
+    if (max_i64) *max_i64 = c.dtype == FGPU_FLOAT64 ? __builtin_bit_cast(int64_t, c.fmax) : c.imax;
+    if (null_count) *null_count = c.null_count;
+    if (distinct_est) *distinct_est = c.distinct;
+    return FGPU_OK;
+}
+
+// Float exponent range recorded at upload (mantissa-LSB exponents).
+int fgpu_table_float_range(const fgpu_table *t, int col, int *emin, int *emax, uint64_t *nonfinite) {
+    if (!t || col < 0 || col >= (int)t->cols.size()) return set_err(FGPU_E_COLUMN_NOT_FOUND, "column %d", col);
+    const DevCol &c = t->cols[col];
+    if (emin) *emin = c.emin;
+    if (emax) *emax = c.emax;
+    if (nonfinite) *nonfinite = c.nonfinite;
+    return FGPU_OK;
+}
+
+// ─── WHERE ──────────────────────────────────────────────────────────────────
+int fgpu_filter_count(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int64_t *out_count) {
+    if (!c || !t || !out_count) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    Binder b(t);
+    DPred dp;
+    if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    unsigned long long *d_cnt;
+    if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return rc;
+    mark(c, 0);
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_cnt, 1, 0));
+    const int grid = grid_for(c, (t->n_rows + 63) / 64 * 64, BLOCK, c->tune.filter_waves_per_cu);
+    mark(c, 1);
+    if (t->n_rows) HIPCHK(launch_filter_count(c->stream, b.dc, dp, t->n_rows, d_cnt, grid));
+    mark(c, 2);
+    unsigned long long n = 0;
+    HIPCHK(hipMemcpyAsync(&n, d_cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    mark(c, 3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int64_t bytes = 0;
+    for (int s = 0; s < b.dc.n; ++s) bytes += t->n_rows * (b.dc.dtype[s] == FGPU_INT64 || b.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
+    finish_timing(c, bytes, false);
+    *out_count = (int64_t)n;
+    return FGPU_OK;
+}
+
+static int filter_mask_dev(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, uint64_t **d_words) {
+    Binder b(t);
+    DPred dp;
+    if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    const int64_t nw = (t->n_rows + 63) / 64;
+    if (int rc = buf(c, B_MASK, (size_t)std::max<int64_t>(nw, 1) * 8, (void **)d_words)) return rc;
+    const int grid = grid_for(c, nw * 64, BLOCK, c->tune.filter_waves_per_cu);
+    mark(c, 1);
+    if (nw) HIPCHK(launch_filter_mask(c->stream, b.dc, dp, t->n_rows, *d_words, grid));
+    mark(c, 2);
+    return FGPU_OK;
+}
+
+int fgpu_filter_mask(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, uint64_t *host_words) {
+    if (!c || !t || !host_words) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    uint64_t *d_words;
+    mark(c, 0);
+    if (int rc = filter_mask_dev(c, t, p, &d_words)) return rc;
+    const int64_t nw = (t->n_rows + 63) / 64;
+    if (nw) HIPCHK(hipMemcpyAsync(host_words, d_words, nw * 8, hipMemcpyDeviceToHost, c->stream));
+    mark(c, 3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    finish_timing(c, 0, false);
+    return FGPU_OK;
+}
+
+int fgpu_filter_select(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const int32_t *sel,
+                       int n_sel, fgpu_result **out) {
+    if (!c || !t || !out || (n_sel > 0 && !sel)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    for (int i = 0; i < n_sel; ++i)
+        if (sel[i] < 0 || sel[i] >= (int)t->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "column %d not in table", sel[i]);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    uint64_t *d_words;
+    mark(c, 0);
+    if (int rc = filter_mask_dev(c, t, p, &d_words)) return rc;
+    const int64_t nw = (t->n_rows + 63) / 64;
+    int64_t *d_offs, *d_tmp;
+    if (int rc = buf(c, B_OFFS, (size_t)(nw + 1) * 8, (void **)&d_offs)) return rc;
+    if (int rc = buf(c, B_TMP, (size_t)(nw / 4096 + 2) * 8, (void **)&d_tmp)) return rc;
+    HIPCHK(launch_word_offsets(c->stream, d_words, nw, d_offs, d_tmp));
+    int64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, d_offs + nw, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto *r = new fgpu_result();
+    r->rows = total;
+    void *d_out;
+    int rc = buf(c, B_OUT, (size_t)std::max<int64_t>(total, 1) * 9 + 64, &d_out);
+    if (rc) {
+        delete r;
+        return rc;
+    }
+    const int grid = grid_for(c, nw * 64, BLOCK, 16);
+    for (int i = 0; i < n_sel; ++i) {
+        const DevCol &dc = t->cols[sel[i]];
+        RCol rc2;
+        rc2.name = dc.name;
+        rc2.dtype = dc.dtype == FGPU_FLOAT64 ? FGPU_FLOAT64 : dc.dtype == FGPU_UTF8 ? FGPU_UTF8 : FGPU_INT64;
+        const int eb = (int)dc.elem_bytes();
+        uint8_t *d_valid = (uint8_t *)d_out + (size_t)std::max<int64_t>(total, 1) * 8;
+        hipError_t e = launch_gather(c->stream, d_words, d_offs, nw, dc.d_values, eb, dc.d_valid, d_out, d_valid, grid);
+        rc2.valid.resize(total);
+        std::vector<int32_t> tmp32;
+        if (e == hipSuccess && total) {
+            if (eb == 8) {
+                if (rc2.dtype == FGPU_FLOAT64) { rc2.f64.resize(total); e = hipMemcpyAsync(rc2.f64.data(), d_out, total * 8, hipMemcpyDeviceToHost, c->stream); }
+                else { rc2.i64.resize(total); e = hipMemcpyAsync(rc2.i64.data(), d_out, total * 8, hipMemcpyDeviceToHost, c->stream); }
+            } else {
+                tmp32.resize(total);
+                e = hipMemcpyAsync(tmp32.data(), d_out, total * 4, hipMemcpyDeviceToHost, c->stream);
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(rc2.valid.data(), d_valid, total, hipMemcpyDeviceToHost, c->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            delete r;
+            return set_err(FGPU_E_DEVICE, "filter_select: %s", hipGetErrorString(e));
+        }
+        if (eb == 4) {
+            if (dc.dtype == FGPU_UTF8) {
+                rc2.u32.resize(total);
+                for (int64_t k = 0; k < total; ++k) {
+                    rc2.u32[k] = (uint32_t)tmp32[k];
+                    rc2.valid[k] = rc2.u32[k] != NULL_CODE;
+                }
+                rc2.dict_src = &dc;
+            } else {
+                rc2.i64.resize(total);
+                for (int64_t k = 0; k < total; ++k) rc2.i64[k] = tmp32[k];
+            }
+        }
+        r->cols.push_back(std::move(rc2));
+    }
+    mark(c, 3);
+    finish_timing(c, 0, false);
+    *out = r;
+    return FGPU_OK;
+}
+
+// ─── GROUP BY ───────────────────────────────────────────────────────────────
+int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
+                 const fgpu_groupby_spec *spec, fgpu_result **out) {
+    if (!c || !t || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    const int kc = spec->key_col;
+    if (kc >= (int)t->cols.size()) return set_err(FGPU_E_COLUMN_NOT_FOUND, "group key column %d not in table", kc);
+    const DevCol *key = kc >= 0 ? &t->cols[kc] : nullptr;
+    if (key && key->dtype == FGPU_FLOAT64)
+        return set_err(FGPU_E_INVALID_QUERY, "GROUP BY on a Float64 column is not supported");
+    Binder b(t);
+    DPred dp;
+    if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    GbArgs a{};
+    a.pred = dp;
+    a.key_col = key ? b.bind(kc) : -1;
+    if (key && a.key_col < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads too many columns");
+    // aggregate inputs
+    int in_of_col[256];
+    for (int &x : in_of_col) x = -1;
+    a.n_in = 0;
+    bool too_many = false;
+    auto slot_of = [&](int /*side*/, int col) -> int {
+        if (col < 256 && in_of_col[col] >= 0) return in_of_col[col];
+        if (a.n_in >= MAX_IN) { too_many = true; return -1; }
+        const int s = b.bind(col);
+        if (s < 0) return -1;
+        a.in_col[a.n_in] = (int8_t)s;
+        if (col < 256) in_of_col[col] = a.n_in;
+        return a.n_in++;
+    };
+    AggLayout L;
+    if (int rc = build_layout(spec->aggs, spec->n_aggs, t, nullptr, slot_of, L))
+        return too_many ? set_err(FGPU_E_UNSUPPORTED, "at most %d distinct aggregate input columns", MAX_IN) : rc;
+    a.cols = b.dc;
+    a.ops = L.ops;
+    a.n_rows = t->n_rows;
+    const int S = L.ops.S;
+    // strategy
+    int mode;
+    int64_t n_state_groups;
+    if (!key) {
+        mode = GB_SCALAR;
+        a.range = 0;
+        n_state_groups = 1;
+    } else {
+        a.key_is_utf8 = key->dtype == FGPU_UTF8;
+        int64_t range;
+        if (a.key_is_utf8) {
+            a.key_min = 0;
+            range = (int64_t)key->dict.size() + 1;  // + synthetic "" group for NULL keys
+            a.utf8_null_code = key->empty_code >= 0 ? (uint32_t)key->empty_code : (uint32_t)key->dict.size();
+        } else if (key->imin > key->imax) {  // all NULL / empty
+            a.key_min = 0;
+            range = 1;
+        } else {
+            a.key_min = key->imin;
+            const unsigned __int128 span = (unsigned __int128)((__int128)key->imax - (__int128)key->imin) + 1;
+            range = span > (unsigned __int128)(int64_t(1) << 40) ? (int64_t(1) << 40) : (int64_t)span;
+        }
+        a.range = range;
+        const int64_t dense_bytes = (range + 1) * S * 8;
+        if (dense_bytes <= c->tune.dense_lds_bytes) {
+            mode = GB_DENSE_LDS;
+            n_state_groups = range + 1;
+        } else if (a.key_is_utf8 || (range <= (int64_t(1) << 24) && range <= std::max<int64_t>(4 * key->distinct, 65536))) {
+            mode = GB_DENSE_GLOBAL;
+            n_state_groups = range + 1;
+        } else {
+            mode = GB_HASH;
+            n_state_groups = 0;
+        }
+    }
+    int64_t cap = 0;
+    if (mode == GB_HASH)
+        cap = next_pow2(std::max<int64_t>(1024, (int64_t)(2.2 * (double)std::max<int64_t>(key->distinct, 1))));
+    unsigned long long *d_cnt;
+    if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return rc;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (mode == GB_HASH) n_state_groups = cap + 1;
+        int64_t *d_keys = nullptr;
+        uint64_t *d_state;
+        if (mode == GB_HASH)
+            if (int rc = buf(c, B_KEYS, (size_t)cap * 8, (void **)&d_keys)) return rc;
+        if (int rc = buf(c, B_STATE, (size_t)n_state_groups * S * 8, (void **)&d_state)) return rc;
+        a.keys = d_keys;
+        a.state = d_state;
+        a.cap_mask = cap ? (uint64_t)(cap - 1) : 0;
+        a.counters = d_cnt;
+        mark(c, 0);
+        HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_cnt, 8, 0));
+        mark(c, 4);
+        HIPCHK(launch_state_init(c->stream, d_keys, cap, d_state, n_state_groups, L.ops));
+        mark(c, 5);
+        // launch shape
+        int block = c->tune.gb_block;
+        size_t lds = 0;
+        if (mode == GB_DENSE_LDS) lds = (size_t)n_state_groups * S * 8;
+        if (mode == GB_HASH) {
+            int slots = c->tune.hash_lds_slots;
+            while (slots > 0 && (size_t)slots * (8 + (size_t)S * 8) > 64 * 1024) slots >>= 1;
+            a.lds_slots = slots;
+            a.probe_limit = std::max(1, c->tune.probe_limit);
+            lds = (size_t)slots * (8 + (size_t)S * 8);
+        }
+        if (mode == GB_SCALAR) block = 256;
+        const int grid = grid_for(c, (t->n_rows + 3) / 4, block, c->tune.gb_waves_per_cu);
+        mark(c, 1);
+        if (t->n_rows) HIPCHK(launch_groupby(c->stream, mode, a, grid, block, lds));
+        mark(c, 2);
+        unsigned long long h_cnt[2] = {0, 0};
+        if (mode == GB_HASH) {
+            HIPCHK(hipMemcpyAsync(h_cnt, d_cnt, 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (h_cnt[1]) {  // table overflowed: grow and rerun
+                cap *= 4;
+                continue;
+            }
+        }
+        // emit
+        OutBufs ob;
+        if (int rc = out_bufs(c, n_state_groups, L.outs.n, ob)) return rc;
+        EmitArgs e{};
+        e.outs = L.outs;
+        e.keys = d_keys;
+        e.state = d_state;
+        e.S = S;
+        e.n_slots = n_state_groups;
+        e.mode = mode;
+        e.key_min = a.key_min;
+        e.key_is_utf8 = a.key_is_utf8;
+        e.out_key = key ? ob.key : nullptr;
+        e.out_key_valid = ob.key_valid;
+        for (int i = 0; i < FGPU_MAX_AGGS; ++i) {
+            e.out_val[i] = ob.val[i];
+            e.out_valid[i] = ob.valid[i];
+        }
+        e.out_count = ob.count;
+        HIPCHK(launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0));
+        HIPCHK(launch_emit(c->stream, e));
+        mark(c, 3);
+        int64_t bytes = 0;
+        for (int s = 0; s < b.dc.n; ++s) bytes += t->n_rows * (b.dc.dtype[s] == FGPU_INT64 || b.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
+        int rc = collect(c, L, n_state_groups, key, key ? key->dtype : 0, key ? key->name : "", ob.key,
+                         ob.key_valid, ob.val, ob.valid, ob.count, out);
+        if (rc) return rc;
+        finish_timing(c, bytes, true);
+        return FGPU_OK;
+    }
+    return set_err(FGPU_E_OUT_OF_MEMORY, "group table kept overflowing (last capacity %lld)", (long long)cap);
+}
+
+// ─── hash join ──────────────────────────────────────────────────────────────
+void fgpu_join_index_free(fgpu_join_index *idx) {
+    if (!idx) return;
+    (void)hipSetDevice(idx->device);
+    if (idx->keys) (void)hipFree(idx->keys);
+    if (idx->rows) (void)hipFree(idx->rows);
+    if (idx->min_key_rows) (void)hipFree(idx->min_key_rows);
+    delete idx;
+}
+
+int64_t fgpu_join_index_device_bytes(const fgpu_join_index *idx) { return idx ? idx->bytes : 0; }
+
+static int join_build_impl(fgpu_ctx *c, const fgpu_table *build, int32_t key_col, fgpu_join_index **out) {
+    if (key_col < 0 || key_col >= (int)build->cols.size())
+        return set_err(FGPU_E_COLUMN_NOT_FOUND, "build key column %d not in table", key_col);
+    const DevCol &kc = build->cols[key_col];
+    if (kc.dtype != FGPU_INT64 && kc.dtype != FGPU_INT32)
+        return set_err(FGPU_E_TYPE_MISMATCH, "join key '%s' must be Int64 or Int32", kc.name.c_str());
+    auto *idx = new fgpu_join_index();
+    idx->device = c->device;
+    idx->build = build;
+    idx->key_col = key_col;
+    const int64_t n = build->n_rows;
+    idx->cap = next_pow2(std::max<int64_t>(1024, 2 * n));
+    auto fail = [&](int rc) {
+        fgpu_join_index_free(idx);
+        return rc;
+    };
+    if (hipMalloc(&idx->keys, idx->cap * 8) != hipSuccess || hipMalloc(&idx->rows, idx->cap * 4) != hipSuccess)
+        return fail(set_err(FGPU_E_OUT_OF_MEMORY, "join index of %lld slots", (long long)idx->cap));
+    idx->bytes = idx->cap * 12;
+    const bool has_min = kc.dtype == FGPU_INT64 && kc.imin == INT64_MIN;
+    const int64_t min_cap = has_min ? std::max<int64_t>(n, 1) : 0;
+    if (min_cap && hipMalloc(&idx->min_key_rows, min_cap * 4) != hipSuccess)
+        return fail(set_err(FGPU_E_OUT_OF_MEMORY, "join index side list"));
+    unsigned long long *d_cnt;
+    if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return fail(rc);
+    mark(c, 4);
+    hipError_t e = launch_fill_u64(c->stream, (uint64_t *)d_cnt, 8, 0);
+    if (e == hipSuccess) e = launch_fill_u64(c->stream, (uint64_t *)idx->keys, idx->cap, (uint64_t)EMPTY_KEY);
+    JoinBuildArgs ba{};
+    ba.keys_col = kc.d_values;
+    ba.key_dtype = kc.dtype;
+    ba.key_valid = kc.d_valid;
+    ba.n = n;
+    ba.keys = idx->keys;
+    ba.rows = idx->rows;
+    ba.cap_mask = (uint64_t)(idx->cap - 1);
+    ba.counters = d_cnt;
+    ba.min_key_rows = idx->min_key_rows;
+    ba.min_key_cap = min_cap;
+    if (e == hipSuccess && n) e = launch_join_build(c->stream, ba, grid_for(c, n, BLOCK, 16));
+    mark(c, 5);
+    unsigned long long h[3] = {0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_cnt, 24, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_err(FGPU_E_DEVICE, "join build: %s", hipGetErrorString(e)));
+    if (h[1]) return fail(set_err(FGPU_E_DEVICE, "join index overflow"));
+    idx->n_min_key_rows = (int64_t)h[2];
+    if (idx->min_key_rows) {
+        // slot order of the side list is arbitrary; sort it so probes see ascending rows
+        std::vector<uint32_t> v(h[2]);
+        if (h[2]) {
+            (void)hipMemcpy(v.data(), idx->min_key_rows, h[2] * 4, hipMemcpyDeviceToHost);
+            std::sort(v.begin(), v.end());
+            (void)hipMemcpy(idx->min_key_rows, v.data(), h[2] * 4, hipMemcpyHostToDevice);
+        }
+    }
+    *out = idx;
+    return FGPU_OK;
+}
+
+int fgpu_join_build(fgpu_ctx *c, const fgpu_table *build, int32_t key_col, fgpu_join_index **out) {
+    if (!c || !build || !out) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    return join_build_impl(c, build, key_col, out);
+}
+
+struct JoinPlan {
+    JoinProbeArgs a{};
+    AggLayout L;
+    Binder *pb = nullptr, *bb = nullptr;
+};
+
+static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
+                        const fgpu_table *build, const fgpu_join_index *idx,
+                        const fgpu_join_spec *spec, Binder &pb, Binder &bb, JoinProbeArgs &a,
+                        AggLayout &L) {
+    if (spec->type < FGPU_INNER || spec->type > FGPU_ANTI)
+        return set_err(FGPU_E_INVALID_QUERY, "unknown join type %d", spec->type);
+    if (spec->probe_key < 0 || spec->probe_key >= (int)probe->cols.size())
+        return set_err(FGPU_E_COLUMN_NOT_FOUND, "probe key column %d not in table", spec->probe_key);
+    const DevCol &pk = probe->cols[spec->probe_key];
+    if (pk.dtype != FGPU_INT64 && pk.dtype != FGPU_INT32)
+        return set_err(FGPU_E_TYPE_MISMATCH, "join key '%s' must be Int64 or Int32", pk.name.c_str());
+    if (idx && (idx->build != build || idx->key_col != spec->build_key))
+        return set_err(FGPU_E_INVALID_QUERY, "join index was built on another table or key");
+    for (int i = 0; i < spec->n_aggs; ++i)
+        if (spec->aggs[i].col >= 0 && spec->aggs[i].side == 1 &&
+            (spec->type == FGPU_SEMI || spec->type == FGPU_ANTI))
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "SEMI/ANTI output has only probe columns");
+    DPred dp;
+    if (int rc = compile_pred(c, probe, pred, pb, dp)) return rc;
+    a = JoinProbeArgs{};
+    a.pred = dp;
+    a.pkey = pb.bind(spec->probe_key);
+    a.type = spec->type;
+    a.n_probe = probe->n_rows;
+    auto slot_of = [&](int side, int col) -> int { return side == 1 ? bb.bind(col) : pb.bind(col); };
+    if (int rc = build_layout(spec->aggs, spec->n_aggs, probe, build, slot_of, L)) return rc;
+    if (L.ops.n > 4) return set_err(FGPU_E_UNSUPPORTED, "join aggregates need %d accumulators (max 4)", L.ops.n);
+    a.pcols = pb.dc;
+    a.bcols = bb.dc;
+    a.n_ops = L.ops.n;
+    for (int i = 0; i < L.ops.n; ++i) {
+        const DOp &o = L.ops.op[i];
+        JoinOp &j = a.ops[i];
+        j.kind = o.kind;
+        j.side = (int8_t)L.keys[i].side;
+        j.col = o.col;
+        j.word = o.word;
+        j.L = o.L;
+        j.W = o.W;
+        j.has_flags = o.has_flags;
+    }
+    a.S = L.ops.S;
+    return FGPU_OK;
+}
+
+int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                  const fgpu_join_index *idx, const fgpu_join_spec *spec, fgpu_result **out) {
+    if (!c || !probe || !build || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    Binder pb(probe), bb(build);
+    JoinProbeArgs a;
+    AggLayout L;
+    if (int rc = join_prepare(c, probe, pred, build, idx, spec, pb, bb, a, L)) return rc;
+    fgpu_join_index *own = nullptr;
+    mark(c, 0);
+    if (!idx) {
+        if (int rc = join_build_impl(c, build, spec->build_key, &own)) return rc;
+        idx = own;
+    }
+    a.keys = idx->keys;
+    a.rows = idx->rows;
+    a.cap_mask = (uint64_t)(idx->cap - 1);
+    a.min_key_rows = idx->min_key_rows;
+    a.n_min_key_rows = idx->n_min_key_rows;
+    uint64_t *d_state;
+    int rc = buf(c, B_STATE, (size_t)a.S * 8, (void **)&d_state);
+    if (rc) {
+        fgpu_join_index_free(own);
+        return rc;
+    }
+    a.state = d_state;
+    hipError_t e = launch_state_init(c->stream, nullptr, 0, d_state, 1, L.ops);
+    const int grid = grid_for(c, (probe->n_rows + 3) / 4, c->tune.join_block, c->tune.join_waves_per_cu);
+    mark(c, 1);
+    if (e == hipSuccess && probe->n_rows) e = launch_join_probe_agg(c->stream, a, grid, c->tune.join_block);
+    mark(c, 2);
+    if (e != hipSuccess) {
+        fgpu_join_index_free(own);
+        return set_err(FGPU_E_DEVICE, "join probe: %s", hipGetErrorString(e));
+    }
+    OutBufs ob;
+    rc = out_bufs(c, 1, L.outs.n, ob);
+    if (rc) {
+        fgpu_join_index_free(own);
+        return rc;
+    }
+    EmitArgs em{};
+    em.outs = L.outs;
+    em.state = d_state;
+    em.S = a.S;
+    em.n_slots = 1;
+    em.mode = GB_SCALAR;
+    em.out_key = nullptr;
+    for (int i = 0; i < FGPU_MAX_AGGS; ++i) {
+        em.out_val[i] = ob.val[i];
+        em.out_valid[i] = ob.valid[i];
+    }
+    em.out_count = ob.count;
+    e = launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0);
+    if (e == hipSuccess) e = launch_emit(c->stream, em);
+    mark(c, 3);
+    if (e != hipSuccess) {
+        fgpu_join_index_free(own);
+        return set_err(FGPU_E_DEVICE, "join emit: %s", hipGetErrorString(e));
+    }
+    rc = collect(c, L, 1, nullptr, 0, "", nullptr, nullptr, ob.val, ob.valid, ob.count, out);
+    int64_t bytes = 0;
+    for (int s = 0; s < pb.dc.n; ++s) bytes += probe->n_rows * (pb.dc.dtype[s] == FGPU_INT64 || pb.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
+    for (int s = 0; s < bb.dc.n; ++s) bytes += build->n_rows * (bb.dc.dtype[s] == FGPU_INT64 || bb.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
+    if (!rc) finish_timing(c, bytes, own != nullptr);
+    fgpu_join_index_free(own);
+    return rc;
+}
+
+int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                    const fgpu_join_index *idx, const fgpu_join_spec *spec, uint32_t *probe_rows,
+                    uint32_t *build_rows, int64_t capacity, int64_t *n_out) {
+    if (!c || !probe || !build || !spec || !n_out) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    Binder pb(probe), bb(build);
+    JoinProbeArgs a;
+    AggLayout L;
+    fgpu_join_spec s2 = *spec;
+    s2.n_aggs = 0;
+    if (int rc = join_prepare(c, probe, pred, build, idx, &s2, pb, bb, a, L)) return rc;
+    fgpu_join_index *own = nullptr;
+    if (!idx) {
+        if (int rc = join_build_impl(c, build, spec->build_key, &own)) return rc;
+        idx = own;
+    }
+    a.keys = idx->keys;
+    a.rows = idx->rows;
+    a.cap_mask = (uint64_t)(idx->cap - 1);
+    a.min_key_rows = idx->min_key_rows;
+    a.n_min_key_rows = idx->n_min_key_rows;
+    const int64_t n = probe->n_rows;
+    int rc = FGPU_OK;
+    uint32_t *d_cnt = nullptr;
+    int64_t *d_offs = nullptr, *d_tmp = nullptr;
+    if (!rc) rc = buf(c, B_CNT, (size_t)std::max<int64_t>(n, 1) * 4, (void **)&d_cnt);
+    if (!rc) rc = buf(c, B_OFFS, (size_t)(n + 1) * 8, (void **)&d_offs);
+    if (!rc) rc = buf(c, B_TMP, (size_t)(n / 4096 + 2) * 8, (void **)&d_tmp);
+    if (rc) {
+        fgpu_join_index_free(own);
+        return rc;
+    }
+    a.counts = d_cnt;
+    const int grid = grid_for(c, n, BLOCK, 16);
+    hipError_t e = n ? launch_join_pairs(c->stream, a, 0, grid) : hipSuccess;
+    if (e == hipSuccess) e = launch_scan_u32(c->stream, d_cnt, n, d_offs, d_tmp);
+    int64_t total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_offs + n, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        fgpu_join_index_free(own);
+        return set_err(FGPU_E_DEVICE, "join pairs: %s", hipGetErrorString(e));
+    }
+    *n_out = total;
+    if (total > capacity || (total > 0 && (!probe_rows || !build_rows))) {
+        fgpu_join_index_free(own);
+        return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "join produces %lld pairs, capacity %lld",
+                       (long long)total, (long long)capacity);
+    }
+    uint32_t *d_pairs;
+    rc = buf(c, B_PAIRS, (size_t)std::max<int64_t>(total, 1) * 8, (void **)&d_pairs);
+    if (rc) {
+        fgpu_join_index_free(own);
+        return rc;
+    }
+    a.offs = d_offs;
+    a.out_l = d_pairs;
+    a.out_r = d_pairs + std::max<int64_t>(total, 1);
+    e = (n && total) ? launch_join_pairs(c->stream, a, 1, grid) : hipSuccess;
+    std::vector<uint32_t> l(total), r(total);
+    if (e == hipSuccess && total) e = hipMemcpyAsync(l.data(), a.out_l, total * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && total) e = hipMemcpyAsync(r.data(), a.out_r, total * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    fgpu_join_index_free(own);
+    if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "join pairs: %s", hipGetErrorString(e));
+    // within one probe row, list build rows in ascending order
+    int64_t i = 0;
+    while (i < total) {
+        int64_t j = i + 1;
+        while (j < total && l[j] == l[i]) ++j;
+        if (j - i > 1) std::sort(r.begin() + i, r.begin() + j);
+        i = j;
+    }
+    if (total) {
+        std::memcpy(probe_rows, l.data(), total * 4);
+        std::memcpy(build_rows, r.data(), total * 4);
+    }
+    return FGPU_OK;
+}
+
+// ─── results ────────────────────────────────────────────────────────────────
+int64_t fgpu_result_rows(const fgpu_result *r) { return r ? r->rows : -1; }
+int fgpu_result_cols(const fgpu_result *r) { return r ? (int)r->cols.size() : -1; }
+int fgpu_result_column(const fgpu_result *r, int i, fgpu_column_view *v) {
+    if (!r || !v) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    if (i < 0 || i >= (int)r->cols.size()) return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "column %d of %zu", i, r->cols.size());
+    const RCol &c = r->cols[i];
+    v->name = c.name.c_str();
+    v->dtype = c.dtype;
+    v->values = c.dtype == FGPU_FLOAT64 ? (const void *)c.f64.data()
+              : c.dtype == FGPU_UTF8   ? (const void *)c.u32.data()
+                                       : (const void *)c.i64.data();
+    v->valid = c.valid.empty() ? nullptr : c.valid.data();
+    v->dict_len = c.dict_src ? (int32_t)c.dict_src->dictp.size() : 0;
+    v->dict = c.dict_src ? c.dict_src->dictp.data() : nullptr;
+    return FGPU_OK;
+}
+void fgpu_result_free(fgpu_result *r) { delete r; }
+
+// ─── host-side check of the exact float accumulator ─────────────────────────
+// Sums x[0..n) with the same digit decomposition and normalisation the kernels
+// use (LSB exponent L and word count W derived from the data like an upload
+// would), entirely on the host.  Returns the exactly rounded sum in *out.
+int fgpu_host_exact_sum(const double *x, int64_t n, double *out) {
+    if (!out || (n > 0 && !x)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    int emin = 1 << 20, emax = -(1 << 20);
+    uint64_t flags = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, x[i]);
+        const int be = (int)((b >> 52) & 0x7ff);
+        if (be == 0x7ff) {
+            flags |= (b & 0x000fffffffffffffull) ? FL_NAN : (b >> 63) ? FL_NINF : FL_PINF;
+            continue;
+        }
+        if ((b & 0x7fffffffffffffffull) == 0) continue;
+        const int e = be == 0 ? -1074 : be - 1075;
+        emin = std::min(emin, e);
+        emax = std::max(emax, e);
+    }
+    DevCol dc;
+    dc.name = "x";
+    dc.emin = emin;
+    dc.emax = emax;
+    int L, W;
+    if (int rc = fx_words(dc, L, W)) return rc;
+    int64_t w[MAX_FX_WORDS] = {0};
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, x[i]);
+        if (((b >> 52) & 0x7ff) == 0x7ff) continue;
+        int w0;
+        int64_t d0, d1, d2;
+        if (!fx_digits(b, L, w0, d0, d1, d2)) continue;
+        w[w0] += d0;
+        if (w0 + 1 < W) w[w0 + 1] += d1;
+        if (w0 + 2 < W) w[w0 + 2] += d2;
+    }
+    *out = fx_finish(w, W, L, flags);
+    return FGPU_OK;
+}
+
+}  // extern "C"

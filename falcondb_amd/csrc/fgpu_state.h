@@ -1,0 +1,193 @@
+// fgpu_state.h — device helpers that update and merge group states
+// (shared by the GROUP BY and join kernels).
+#pragma once
+
+#include "fgpu_launch.h"
+
+namespace fgpu {
+
+// ─── scoped atomics ────────────────────────────────────────────────────────
+template <int SCOPE>
+FGPU_DEV void a_add(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, SCOPE);
+}
+template <int SCOPE>
+FGPU_DEV void a_smin(uint64_t *p, int64_t v) {
+    __hip_atomic_fetch_min((int64_t *)p, v, __ATOMIC_RELAXED, SCOPE);
+}
+template <int SCOPE>
+FGPU_DEV void a_smax(uint64_t *p, int64_t v) {
+    __hip_atomic_fetch_max((int64_t *)p, v, __ATOMIC_RELAXED, SCOPE);
+}
+template <int SCOPE>
+FGPU_DEV void a_umin(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SCOPE);
+}
+template <int SCOPE>
+FGPU_DEV void a_umax(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, SCOPE);
+}
+template <int SCOPE>
+FGPU_DEV void a_or(uint64_t *p, uint64_t v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, SCOPE);
+}
+
+constexpr int LDS_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
+constexpr int DEV_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
+
+// The aggregate inputs of one row, as separate registers (an array indexed by a
+// runtime op field would be placed in scratch memory).
+struct Inputs {
+    int64_t v0, v1, v2, v3, v4, v5;
+    uint32_t valid;  // bit j set = input j is not NULL
+};
+static_assert(MAX_IN == 6, "Inputs holds MAX_IN registers");
+
+FGPU_DEV int64_t pick(const Inputs &in, int j) {
+    switch (j) {
+        case 0: return in.v0;
+        case 1: return in.v1;
+        case 2: return in.v2;
+        case 3: return in.v3;
+        case 4: return in.v4;
+        default: return in.v5;
+    }
+}
+
+// Adds one row (input values iv, validity bits ivalid) into a group state.
+template <int SCOPE>
+FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
+    a_add<SCOPE>(st, 1ull);
+    for (int i = 0; i < ops.n; ++i) {
+        const DOp o = ops.op[i];
+        if (!((in.valid >> o.col) & 1)) continue;
+        const int64_t v = pick(in, o.col);
+        uint64_t *w = st + o.word;
+        switch (o.kind) {
+            case OP_CNT: a_add<SCOPE>(w, 1ull); break;
+            case OP_ISUM: a_add<SCOPE>(w, (uint64_t)v); break;
+            case OP_IMIN: a_smin<SCOPE>(w, v); break;
+            case OP_IMAX: a_smax<SCOPE>(w, v); break;
+            case OP_FSUM: {
+                const uint64_t bits = (uint64_t)v;
+                if (((bits >> 52) & 0x7ff) == 0x7ff) {  // NaN / inf
+                    const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN
+                                        : (bits >> 63)                ? FL_NINF
+                                                                      : FL_PINF;
+                    a_or<SCOPE>(w + o.W, fl);
+                    break;
+                }
+                int w0;
+                int64_t d0, d1, d2;
+                if (!fx_digits(bits, o.L, w0, d0, d1, d2)) break;
+                if (d0) a_add<SCOPE>(w + w0, (uint64_t)d0);
+                if (d1) a_add<SCOPE>(w + w0 + 1, (uint64_t)d1);
+                if (d2) a_add<SCOPE>(w + w0 + 2, (uint64_t)d2);
+                break;
+            }
+            case OP_FMIN:
+            case OP_FMAX: {
+                const uint64_t bits = (uint64_t)v;
+                if (((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull)) break;  // NaN
+                const uint64_t e = f64_order(bits);
+                if (o.kind == OP_FMIN) a_umin<SCOPE>(w, e);
+                else a_umax<SCOPE>(w, e);
+                break;
+            }
+        }
+    }
+}
+
+// Merges a whole state (src, e.g. an LDS partial) into dst (HBM).
+FGPU_DEV void merge_state(uint64_t *dst, const uint64_t *src, const DOps &ops) {
+    a_add<DEV_SCOPE>(dst, src[0]);
+    for (int i = 0; i < ops.n; ++i) {
+        const DOp o = ops.op[i];
+        uint64_t *w = dst + o.word;
+        const uint64_t *s = src + o.word;
+        switch (o.kind) {
+            case OP_CNT:
+            case OP_ISUM:
+                if (s[0]) a_add<DEV_SCOPE>(w, s[0]);
+                break;
+            case OP_IMIN: a_smin<DEV_SCOPE>(w, (int64_t)s[0]); break;
+            case OP_IMAX: a_smax<DEV_SCOPE>(w, (int64_t)s[0]); break;
+            case OP_FSUM:
+                for (int k = 0; k < o.W; ++k)
+                    if (s[k]) a_add<DEV_SCOPE>(w + k, s[k]);
+                if (o.has_flags && s[o.W]) a_or<DEV_SCOPE>(w + o.W, s[o.W]);
+                break;
+            case OP_FMIN: a_umin<DEV_SCOPE>(w, s[0]); break;
+            case OP_FMAX: a_umax<DEV_SCOPE>(w, s[0]); break;
+        }
+    }
+}
+
+
+// Folds one thread's register accumulators (acc: integer / encoded word,
+// hi+lo: double-double float sum) for op `o` into an LDS state row.
+FGPU_DEV void fold_reg(uint64_t *row, const DOp &o, int64_t acc, double hi, double lo) {
+    uint64_t *w = row + o.word;
+    switch (o.kind) {
+        case OP_CNT:
+        case OP_ISUM:
+            if (acc) a_add<LDS_SCOPE>(w, (uint64_t)acc);
+            break;
+        case OP_IMIN: a_smin<LDS_SCOPE>(w, acc); break;
+        case OP_IMAX: a_smax<LDS_SCOPE>(w, acc); break;
+        case OP_FMIN: a_umin<LDS_SCOPE>(w, (uint64_t)acc); break;
+        case OP_FMAX: a_umax<LDS_SCOPE>(w, (uint64_t)acc); break;
+        case OP_FSUM: {
+            if (acc) a_or<LDS_SCOPE>(w + o.W, (uint64_t)acc);
+            const double parts[2] = {hi, lo};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                int w0;
+                int64_t d0, d1, d2;
+                if (!fx_digits((uint64_t)__double_as_longlong(parts[q]), o.L, w0, d0, d1, d2)) continue;
+                if (d0) a_add<LDS_SCOPE>(w + w0, (uint64_t)d0);
+                if (d1) a_add<LDS_SCOPE>(w + w0 + 1, (uint64_t)d1);
+                if (d2) a_add<LDS_SCOPE>(w + w0 + 2, (uint64_t)d2);
+            }
+            break;
+        }
+    }
+}
+
+// Identity of a register accumulator for op kind k.
+FGPU_DEV int64_t reg_ident(int k) {
+    return k == OP_IMIN ? INT64_MAX : k == OP_IMAX ? INT64_MIN : k == OP_FMIN ? (int64_t)~0ull : 0;
+}
+
+// Adds value v (raw 64-bit pattern) into a register accumulator of op kind k.
+FGPU_DEV void reg_add(int k, int64_t v, int64_t &acc, double &hi, double &lo) {
+    switch (k) {
+        case OP_CNT: acc += 1; break;
+        case OP_ISUM: acc = (int64_t)((uint64_t)acc + (uint64_t)v); break;
+        case OP_IMIN: acc = v < acc ? v : acc; break;
+        case OP_IMAX: acc = v > acc ? v : acc; break;
+        case OP_FSUM: {
+            const uint64_t bits = (uint64_t)v;
+            if (((bits >> 52) & 0x7ff) == 0x7ff) {
+                acc |= (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
+            } else {
+                double s, e;
+                two_sum(hi, __longlong_as_double(v), s, e);
+                hi = s;
+                lo += e;
+            }
+            break;
+        }
+        case OP_FMIN:
+        case OP_FMAX: {
+            const uint64_t bits = (uint64_t)v;
+            if (((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull)) break;
+            const uint64_t e = f64_order(bits);
+            if (k == OP_FMIN) acc = (int64_t)(e < (uint64_t)acc ? e : (uint64_t)acc);
+            else acc = (int64_t)(e > (uint64_t)acc ? e : (uint64_t)acc);
+            break;
+        }
+    }
+}
+
+}  // namespace fgpu

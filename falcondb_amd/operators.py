@@ -1,0 +1,584 @@
+"""Python mirror of FalconDB's physical-operator interface on the MI355X backend.
+
+The names follow the reference's query layer so code written against it reads
+the same here:
+
+  Expr.eval_mask_bits        src/query/expr.rs:635        packed LSB-first mask
+  execute_select             src/query/select.rs:105      WHERE + projection
+  count_star                 src/db.rs:1439-1476          COUNT(*) ... WHERE
+  GroupByExec.execute        src/query/aggregate.rs:535   GROUP BY (fused with WHERE)
+  execute_agg_only           src/query/aggregate.rs:1394  aggregates without GROUP BY
+  execute_hash_join          src/query/join.rs:184        INNER / LEFT join + aggregates
+  execute_hash_semi_anti_join src/query/join.rs:49        EXISTS / NOT EXISTS + aggregates
+
+Every call goes through libfalcon_gpu.so (include/falcon_gpu.h); errors come
+back as FalconError with the reference's variant name.  Results are
+RecordBatch objects of numpy columns; `to_reference_types()` applies the
+reference's output typing (integer SUM/MIN/MAX as Float64,
+src/query/aggregate.rs:3390-3420).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import (ANTI, AVG, COUNT, EQ, FLOAT64, GE, GT, INNER, INT32, INT64, LE, LEFT, LT,
+                   MAX, MIN, NE, NO_ROW, P_AND, P_CMP, P_IS_NOT_NULL, P_IS_NULL, P_OR, SEMI,
+                   SUM, UTF8)
+
+
+class FalconError(Exception):
+    """An error returned through the C ABI.  `kind` is the FalconError variant."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.kind = _abi.ERROR_NAMES.get(code, f"Error{code}")
+        super().__init__(f"{self.kind}: {message}")
+
+
+def _lib():
+    return _abi.load()
+
+
+def _check(rc: int) -> None:
+    if rc != _abi.OK:
+        raise FalconError(rc, _lib().fgpu_last_error().decode(errors="replace"))
+
+
+# ─── context ─────────────────────────────────────────────────────────────────
+class Context:
+    """A device, a HIP stream and a scratch arena (fgpu_open)."""
+
+    def __init__(self, device: int = 0, timing: bool = False):
+        h = C.c_void_p()
+        _check(_lib().fgpu_open(device, C.byref(h)))
+        self._h = h
+        self.device = device
+        if timing:
+            self.set_timing(True)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_timing(self, on: bool) -> None:
+        _check(_lib().fgpu_set_timing(self._h, 1 if on else 0))
+
+    def last_timing(self) -> dict:
+        t = _abi.Timing()
+        _check(_lib().fgpu_last_timing(self._h, C.byref(t)))
+        return {"total_ms": t.total_ms, "main_ms": t.main_ms, "build_ms": t.build_ms,
+                "finalize_ms": t.finalize_ms, "bytes_main": t.bytes_main}
+
+    def set_tuning(self, name: str, value: int) -> None:
+        _check(_lib().fgpu_set_tuning(self._h, name.encode(), int(value)))
+
+    def scratch_bytes(self) -> int:
+        return int(_lib().fgpu_scratch_bytes(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib().fgpu_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ─── columns and tables ──────────────────────────────────────────────────────
+@dataclass
+class DictColumn:
+    """Dictionary-encoded text column: u32 codes (0xFFFFFFFF = NULL) + strings
+    (Utf8Column{dict, codes}, src/storage/column.rs:1272-1277)."""
+    codes: np.ndarray
+    dictionary: Sequence[str]
+
+
+@dataclass
+class NullableColumn:
+    """A primitive column with a validity mask (True = valid)."""
+    values: np.ndarray
+    valid: np.ndarray
+
+
+def pack_validity(valid: np.ndarray) -> np.ndarray:
+    """bool per row -> LSB-first u64 words (src/storage/column.rs:132-133)."""
+    valid = np.asarray(valid, dtype=bool)
+    n = valid.size
+    pad = (-n) % 64
+    bits = np.packbits(np.concatenate([valid, np.zeros(pad, bool)]), bitorder="little")
+    return bits.view(np.uint64).copy() if n else np.zeros(0, np.uint64)
+
+
+def _np_dtype_code(a: np.ndarray) -> int:
+    if a.dtype == np.int32:
+        return INT32
+    if a.dtype == np.int64:
+        return INT64
+    if a.dtype == np.float64:
+        return FLOAT64
+    raise FalconError(2, f"unsupported column dtype {a.dtype}")
+
+
+class Table:
+    """A device-resident table (fgpu_table_upload).  Column names resolve
+    case-insensitively (Schema::column_index, src/storage/schema.rs:81-86)."""
+
+    def __init__(self, ctx: Context, columns: dict, batch_rows: int = 65536):
+        self.ctx = ctx
+        self.names = list(columns.keys())
+        self.dtypes: list[int] = []
+        self.dicts: list[list[str] | None] = []
+        keep = []  # host buffers alive for the duration of the upload
+        descs = (_abi.ColumnDesc * max(len(columns), 1))()
+        n_rows = None
+        for i, (name, col) in enumerate(columns.items()):
+            d = descs[i]
+            d.name = name.encode()
+            valid_words = None
+            if isinstance(col, DictColumn):
+                codes = np.ascontiguousarray(col.codes, dtype=np.uint32)
+                strs = [s.encode() for s in col.dictionary]
+                arr = (C.c_char_p * max(len(strs), 1))(*strs)
+                keep += [codes, strs, arr]
+                d.dtype, d.values = UTF8, codes.ctypes.data
+                d.dict_len, d.dict = len(strs), C.cast(arr, C.POINTER(C.c_char_p))
+                n = codes.size
+                self.dicts.append(list(col.dictionary))
+            else:
+                if isinstance(col, NullableColumn):
+                    vals = np.ascontiguousarray(col.values)
+                    valid_words = pack_validity(col.valid)
+                    if np.asarray(col.valid).size != vals.size:
+                        raise FalconError(6, f"column '{name}': validity length mismatch")
+                else:
+                    vals = np.ascontiguousarray(col)
+                d.dtype = _np_dtype_code(vals)
+                d.values = vals.ctypes.data
+                keep.append(vals)
+                n = vals.size
+                self.dicts.append(None)
+            if valid_words is not None:
+                keep.append(valid_words)
+                d.validity = valid_words.ctypes.data
+            self.dtypes.append(d.dtype)
+            if n_rows is None:
+                n_rows = n
+            elif n != n_rows:
+                raise FalconError(6, f"column '{name}' has {n} rows, expected {n_rows}")
+        h = C.c_void_p()
+        _check(_lib().fgpu_table_upload(ctx.handle, descs, len(columns), int(n_rows or 0),
+                                        batch_rows, C.byref(h)))
+        self._h = h
+        self.n_rows = int(n_rows or 0)
+        self.batch_rows = batch_rows
+
+    @property
+    def handle(self):
+        return self._h
+
+    def column_index(self, name: str) -> int:
+        low = name.lower()
+        if "." in low:  # qualified t.col: resolve by the bare name
+            low = low.split(".", 1)[1]
+        for i, n in enumerate(self.names):
+            if n.lower() == low:
+                return i
+        raise FalconError(1, f"column '{name}' not found")
+
+    def device_bytes(self) -> int:
+        return int(_lib().fgpu_table_device_bytes(self._h))
+
+    def column_stats(self, name: str) -> dict:
+        mn, mx, nulls, nd = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(_lib().fgpu_table_column_stats(self._h, self.column_index(name), C.byref(mn),
+                                              C.byref(mx), C.byref(nulls), C.byref(nd)))
+        return {"min": mn.value, "max": mx.value, "nulls": nulls.value, "distinct": nd.value}
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            _lib().fgpu_table_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ─── expressions ─────────────────────────────────────────────────────────────
+_OPS = {"=": EQ, "!=": NE, "<": LT, "<=": LE, ">": GT, ">=": GE}
+_SWAP = {EQ: EQ, NE: NE, LT: GT, LE: GE, GT: LT, GE: LE}
+
+
+@dataclass
+class _Node:
+    kind: int
+    col: str | None = None
+    op: int = 0
+    lit: object = None
+    lit_type: int = INT64
+
+
+class Expr:
+    """A WHERE predicate as a postfix program of comparisons and AND/OR
+    (the subset Expr::try_fast_mask evaluates, src/query/expr.rs:656-808)."""
+
+    def __init__(self, nodes: list[_Node]):
+        self.nodes = nodes
+
+    def __and__(self, other: "Expr") -> "Expr":
+        return Expr(self.nodes + other.nodes + [_Node(P_AND)])
+
+    def __or__(self, other: "Expr") -> "Expr":
+        return Expr(self.nodes + other.nodes + [_Node(P_OR)])
+
+    def _compile(self, table: Table):
+        nodes = (_abi.PredNode * len(self.nodes))()
+        keep = []
+        for i, n in enumerate(self.nodes):
+            d = nodes[i]
+            d.kind = n.kind
+            if n.kind in (P_AND, P_OR):
+                continue
+            d.col = table.column_index(n.col)
+            d.op = n.op
+            d.lit_type = n.lit_type
+            if n.lit_type == UTF8:
+                b = str(n.lit).encode()
+                keep.append(b)
+                d.lit_str = b
+            elif n.lit_type == FLOAT64:
+                d.lit_f64 = float(n.lit)
+            elif n.kind == P_CMP:
+                d.lit_i64 = int(n.lit)
+        p = _abi.Pred(len(self.nodes), C.cast(nodes, C.POINTER(_abi.PredNode)))
+        return p, (nodes, keep)
+
+    def eval_mask_bits(self, ctx: Context, table: Table) -> np.ndarray:
+        """Packed mask of the rows that pass: ceil(n/64) LSB-first u64 words."""
+        p, keep = self._compile(table)
+        out = np.zeros((table.n_rows + 63) // 64, dtype=np.uint64)
+        _check(_lib().fgpu_filter_mask(ctx.handle, table.handle, C.byref(p), out.ctypes.data))
+        return out
+
+
+def _lit_type(v) -> int:
+    if isinstance(v, str):
+        return UTF8
+    if isinstance(v, (float, np.floating)):
+        return FLOAT64
+    if isinstance(v, np.int32):
+        return INT32
+    return INT64
+
+
+class ColumnRef:
+    """`col("name")`; comparing it with a literal yields an Expr."""
+
+    def __init__(self, name: str):
+        self.name = name
+
+    def _cmp(self, op: int, v) -> Expr:
+        return Expr([_Node(P_CMP, self.name, op, v, _lit_type(v))])
+
+    def __gt__(self, v): return self._cmp(GT, v)
+    def __ge__(self, v): return self._cmp(GE, v)
+    def __lt__(self, v): return self._cmp(LT, v)
+    def __le__(self, v): return self._cmp(LE, v)
+    def __eq__(self, v): return self._cmp(EQ, v)  # type: ignore[override]
+    def __ne__(self, v): return self._cmp(NE, v)  # type: ignore[override]
+    __hash__ = object.__hash__
+
+    def is_null(self) -> Expr:
+        return Expr([_Node(P_IS_NULL, self.name)])
+
+    def is_not_null(self) -> Expr:
+        return Expr([_Node(P_IS_NOT_NULL, self.name)])
+
+
+def col(name: str) -> ColumnRef:
+    return ColumnRef(name)
+
+
+def cmp(column: str, op: str, literal) -> Expr:
+    """`cmp("score", ">", 50)` == `col("score") > 50`."""
+    return col(column)._cmp(_OPS[op], literal)
+
+
+def lit_cmp(literal, op: str, column: str) -> Expr:
+    """`literal op column`, swapped like the reference (src/query/expr.rs:679-684)."""
+    return col(column)._cmp(_SWAP[_OPS[op]], literal)
+
+
+# ─── aggregates ──────────────────────────────────────────────────────────────
+class AggFunc:
+    Count = COUNT
+    Sum = SUM
+    Avg = AVG
+    Min = MIN
+    Max = MAX
+
+
+@dataclass
+class AggExpr:
+    """One aggregate.  input=None is COUNT(*); `side` picks the join input
+    ("probe"/"left" or "build"/"right")."""
+    func: int
+    input: str | None = None
+    alias: str | None = None
+    side: str = "probe"
+
+
+def _aggs_struct(aggs: Sequence[AggExpr], resolve):
+    arr = (_abi.Agg * max(len(aggs), 1))()
+    keep = []
+    for i, a in enumerate(aggs):
+        arr[i].func = a.func
+        side = 1 if a.side in ("build", "right") else 0
+        arr[i].side = side
+        arr[i].col = -1 if a.input is None else resolve(side, a.input)
+        if a.alias:
+            b = a.alias.encode()
+            keep.append(b)
+            arr[i].alias = b
+    return arr, keep
+
+
+# ─── results ─────────────────────────────────────────────────────────────────
+_DT_NAME = {INT64: "Int64", FLOAT64: "Float64", UTF8: "Utf8", INT32: "Int32"}
+
+
+@dataclass
+class RecordBatch:
+    """Schema + columns (RecordBatch, src/storage/table.rs:45-49): `columns[i]`
+    is a numpy array; `validity[i]` a bool array or None; Utf8 columns hold
+    python strings (NULL = None)."""
+    names: list[str]
+    dtypes: list[str]
+    columns: list[np.ndarray]
+    validity: list[np.ndarray | None]
+    agg_funcs: list[int | None] = field(default_factory=list)
+
+    @property
+    def num_rows(self) -> int:
+        return int(self.columns[0].size) if self.columns else 0
+
+    def column(self, name: str) -> np.ndarray:
+        return self.columns[self.names.index(name)]
+
+    def to_reference_types(self) -> "RecordBatch":
+        """Integer SUM / MIN / MAX as Float64, like the reference's build_agg_col."""
+        cols, dts = [], []
+        for c, dt, f in zip(self.columns, self.dtypes, self.agg_funcs or [None] * len(self.columns)):
+            if f in (SUM, MIN, MAX) and dt == "Int64":
+                cols.append(c.astype(np.float64))
+                dts.append("Float64")
+            else:
+                cols.append(c)
+                dts.append(dt)
+        return RecordBatch(list(self.names), dts, cols, list(self.validity), list(self.agg_funcs))
+
+    def sort_by(self, name: str) -> "RecordBatch":
+        key = self.column(name)
+        order = np.argsort(np.array(key, dtype=object) if key.dtype == object else key, kind="stable")
+        return RecordBatch(list(self.names), list(self.dtypes), [c[order] for c in self.columns],
+                           [None if v is None else v[order] for v in self.validity],
+                           list(self.agg_funcs))
+
+
+def _read_result(h, agg_funcs: list[int | None]) -> RecordBatch:
+    lib = _lib()
+    try:
+        n = int(lib.fgpu_result_rows(h))
+        nc = int(lib.fgpu_result_cols(h))
+        names, dts, cols, valid = [], [], [], []
+        for i in range(nc):
+            v = _abi.ColumnView()
+            _check(lib.fgpu_result_column(h, i, C.byref(v)))
+            names.append(v.name.decode())
+            if v.dtype == UTF8:
+                codes = np.ctypeslib.as_array(C.cast(v.values, C.POINTER(C.c_uint32)), (n,)).copy() if n else np.zeros(0, np.uint32)
+                d = [v.dict[k].decode() for k in range(v.dict_len)]
+                strs = np.array([d[c] if c < len(d) else ("" if c == len(d) else None) for c in codes.tolist()], dtype=object)
+                cols.append(strs)
+                dts.append("Utf8")
+            else:
+                ct = C.c_double if v.dtype == FLOAT64 else C.c_int64
+                arr = np.ctypeslib.as_array(C.cast(v.values, C.POINTER(ct)), (n,)).copy() if n else np.zeros(0, np.float64 if v.dtype == FLOAT64 else np.int64)
+                cols.append(arr)
+                dts.append(_DT_NAME[v.dtype])
+            if v.valid and n:
+                valid.append(np.ctypeslib.as_array(v.valid, (n,)).astype(bool))
+            else:
+                valid.append(None)
+        return RecordBatch(names, dts, cols, valid, agg_funcs)
+    finally:
+        lib.fgpu_result_free(h)
+
+
+# ─── operators ───────────────────────────────────────────────────────────────
+def _pred(filter: Expr | None, table: Table):
+    if filter is None:
+        return None, None
+    p, keep = filter._compile(table)
+    return C.byref(p), (p, keep)
+
+
+def count_star(ctx: Context, table: Table, filter: Expr | None = None) -> int:
+    """SELECT COUNT(*) FROM t WHERE filter (src/db.rs:1439-1476)."""
+    p, keep = _pred(filter, table)
+    out = C.c_int64()
+    _check(_lib().fgpu_filter_count(ctx.handle, table.handle, p, C.byref(out)))
+    return out.value
+
+
+def execute_select(ctx: Context, table: Table, filter: Expr | None,
+                   columns: Iterable[str]) -> RecordBatch:
+    """Rows passing `filter`, columns `columns`, in row order
+    (execute_select + filter_by_bitmask_cols)."""
+    p, keep = _pred(filter, table)
+    idx = [table.column_index(c) for c in columns]
+    arr = (C.c_int32 * max(len(idx), 1))(*idx)
+    h = C.c_void_p()
+    _check(_lib().fgpu_filter_select(ctx.handle, table.handle, p, arr, len(idx), C.byref(h)))
+    return _read_result(h, [None] * len(idx))
+
+
+class GroupByExec:
+    """GROUP BY over one key column with COUNT/SUM/AVG/MIN/MAX aggregates
+    (GroupByExec, src/query/aggregate.rs:535).  The WHERE filter is fused into
+    the same device pass instead of materialising filtered batches."""
+
+    def __init__(self, group_by: Sequence[str | ColumnRef], aggs: Sequence[AggExpr]):
+        if len(group_by) != 1:
+            raise FalconError(6, "GROUP BY takes exactly one key column on this backend")
+        g = group_by[0]
+        self.key = g.name if isinstance(g, ColumnRef) else g
+        self.aggs = list(aggs)
+
+    def execute(self, ctx: Context, table: Table, filter: Expr | None = None) -> RecordBatch:
+        return _groupby(ctx, table, filter, table.column_index(self.key), self.aggs)
+
+
+def execute_agg_only(ctx: Context, table: Table, aggs: Sequence[AggExpr],
+                     filter: Expr | None = None) -> RecordBatch:
+    """Aggregates without GROUP BY: one row (src/query/aggregate.rs:1394)."""
+    return _groupby(ctx, table, filter, -1, list(aggs))
+
+
+def _groupby(ctx, table, filter, key_col, aggs) -> RecordBatch:
+    p, keep = _pred(filter, table)
+    arr, keep2 = _aggs_struct(aggs, lambda side, name: table.column_index(name))
+    spec = _abi.GroupBySpec(key_col, len(aggs), C.cast(arr, C.POINTER(_abi.Agg)))
+    h = C.c_void_p()
+    _check(_lib().fgpu_groupby(ctx.handle, table.handle, p, C.byref(spec), C.byref(h)))
+    funcs = ([None] if key_col >= 0 else []) + [a.func for a in aggs]
+    return _read_result(h, funcs)
+
+
+class JoinIndex:
+    """Device hash index over a build table's key (fgpu_join_build); reusable."""
+
+    def __init__(self, ctx: Context, build: Table, key: str):
+        self.build = build
+        self.key_col = build.column_index(key)
+        h = C.c_void_p()
+        _check(_lib().fgpu_join_build(ctx.handle, build.handle, self.key_col, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        return int(_lib().fgpu_join_index_device_bytes(self._h))
+
+    def free(self):
+        if getattr(self, "_h", None):
+            _lib().fgpu_join_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class JoinType:
+    Inner = INNER
+    Left = LEFT
+    Semi = SEMI
+    Anti = ANTI
+
+
+def _join_spec(probe, build, probe_key, build_key, join_type, aggs):
+    def resolve(side, name):
+        return (build if side == 1 else probe).column_index(name)
+    arr, keep = _aggs_struct(aggs, resolve)
+    spec = _abi.JoinSpec(join_type, probe.column_index(probe_key), build.column_index(build_key),
+                         len(aggs), C.cast(arr, C.POINTER(_abi.Agg)))
+    return spec, (arr, keep)
+
+
+def execute_hash_join(ctx: Context, probe: Table, build: Table, probe_key: str, build_key: str,
+                      join_type: int, aggs: Sequence[AggExpr], filter: Expr | None = None,
+                      index: JoinIndex | None = None) -> RecordBatch:
+    """SELECT aggs FROM probe [LEFT] JOIN build ON probe.k = build.k [WHERE filter on
+    probe] -- the join fused with the aggregate that consumes it
+    (execute_hash_join_projected + execute_agg_only).  SEMI/ANTI also accepted."""
+    spec, keep = _join_spec(probe, build, probe_key, build_key, join_type, aggs)
+    p, keep2 = _pred(filter, probe)
+    h = C.c_void_p()
+    _check(_lib().fgpu_join_agg(ctx.handle, probe.handle, p, build.handle,
+                                index.handle if index else None, C.byref(spec), C.byref(h)))
+    return _read_result(h, [a.func for a in aggs])
+
+
+def execute_hash_semi_anti_join(ctx: Context, probe: Table, build: Table, probe_key: str,
+                                build_key: str, anti: bool, aggs: Sequence[AggExpr],
+                                filter: Expr | None = None,
+                                index: JoinIndex | None = None) -> RecordBatch:
+    """WHERE [NOT] EXISTS (SELECT 1 FROM build WHERE build.k = probe.k), aggregated
+    (execute_hash_semi_anti_join, src/query/join.rs:49-132)."""
+    return execute_hash_join(ctx, probe, build, probe_key, build_key, ANTI if anti else SEMI,
+                             aggs, filter, index)
+
+
+def join_pairs(ctx: Context, probe: Table, build: Table, probe_key: str, build_key: str,
+               join_type: int, filter: Expr | None = None,
+               index: JoinIndex | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """(probe row, build row) pairs of the join, NO_ROW = 0xFFFFFFFF."""
+    spec, keep = _join_spec(probe, build, probe_key, build_key, join_type, [])
+    p, keep2 = _pred(filter, probe)
+    n = C.c_int64()
+    lib = _lib()
+    rc = lib.fgpu_join_pairs(ctx.handle, probe.handle, p, build.handle,
+                             index.handle if index else None, C.byref(spec), None, None, 0,
+                             C.byref(n))
+    if rc not in (_abi.OK, 3):
+        _check(rc)
+    cap = max(n.value, 1)
+    l = np.zeros(cap, np.uint32)
+    r = np.zeros(cap, np.uint32)
+    _check(lib.fgpu_join_pairs(ctx.handle, probe.handle, p, build.handle,
+                               index.handle if index else None, C.byref(spec), l.ctypes.data,
+                               r.ctypes.data, cap, C.byref(n)))
+    return l[:n.value], r[:n.value]
+
+
+def host_exact_sum(x: np.ndarray) -> float:
+    """The kernels' exact fixed-point float summation, run on the host."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = C.c_double()
+    _check(_lib().fgpu_host_exact_sum(x.ctypes.data, x.size, C.byref(out)))
+    return out.value

@@ -1,0 +1,248 @@
+/*
+ * falcon_gpu.h — C ABI of the MI355X (gfx950) execution backend for FalconDB's
+ * WHERE filter, hash GROUP BY aggregation and IntHashTable hash join.
+ *
+ * Every entry point takes plain pointers and sizes; no HIP or torch type crosses
+ * the boundary (a stream, where one is accepted, is passed as `void*`).  Each
+ * declaration names the reference routine it replaces (paths are relative to the
+ * FalconDB source tree).  A Rust `extern "C"` block declaring exactly these
+ * functions is the binding the reference's query layer would add.
+ *
+ * Conventions (mirroring src/api/c_ffi.rs:26-144 and src/storage/types.rs:303-333):
+ *   - Functions return an `int` status: FGPU_OK (0) or one of the FGPU_E_* codes.
+ *     Codes 1..9 are the FalconError variants in declaration order; 100+ are
+ *     device-side failures.  fgpu_last_error() returns the thread-local message
+ *     of the most recent failure on the calling thread.
+ *   - Handles are opaque; whoever receives one frees it with the matching *_free.
+ *   - Host input buffers are borrowed only for the duration of a call.  Device
+ *     copies (tables, join indexes) are owned by their handle.
+ *   - A table is immutable after upload and may be shared by every context and
+ *     thread on the same device (like the reference's Arc<Column>).  One context
+ *     serialises its own calls on its own stream; use one context per concurrent
+ *     caller.
+ *   - Result rows of GROUP BY come back in unspecified order (SQL gives none;
+ *     the reference's first-seen order is restated by the host layer on request).
+ */
+#ifndef FALCON_GPU_H
+#define FALCON_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGPU_ABI_VERSION 1
+
+/* ── status codes ─────────────────────────────────────────────────────────── */
+enum {
+    FGPU_OK = 0,
+    FGPU_E_COLUMN_NOT_FOUND = 1,     /* FalconError::ColumnNotFound */
+    FGPU_E_TYPE_MISMATCH = 2,        /* FalconError::TypeMismatch */
+    FGPU_E_INDEX_OUT_OF_BOUNDS = 3,  /* FalconError::IndexOutOfBounds */
+    FGPU_E_PARSE = 4,                /* FalconError::ParseError */
+    FGPU_E_IO = 5,                   /* FalconError::Io */
+    FGPU_E_INVALID_QUERY = 6,        /* FalconError::InvalidQuery */
+    FGPU_E_DIVISION_BY_ZERO = 7,     /* FalconError::DivisionByZero */
+    FGPU_E_OVERFLOW = 8,             /* FalconError::Overflow */
+    FGPU_E_CONSTRAINT = 9,           /* FalconError::ConstraintViolation */
+    FGPU_E_DEVICE = 100,             /* HIP runtime failure */
+    FGPU_E_OUT_OF_MEMORY = 101,      /* device allocation failed / HBM budget exhausted */
+    FGPU_E_UNSUPPORTED = 102         /* input shape the kernels do not accept (message says why) */
+};
+
+/* ── column types (src/storage/types.rs:4-19 subset on this path) ─────────── */
+enum {
+    FGPU_INT32 = 1,
+    FGPU_INT64 = 2,
+    FGPU_FLOAT64 = 3,
+    FGPU_UTF8 = 4   /* dictionary-encoded: u32 codes, code 0xFFFFFFFF = NULL
+                       (src/storage/column.rs:1272-1277) */
+};
+
+typedef struct fgpu_ctx fgpu_ctx;
+typedef struct fgpu_table fgpu_table;
+typedef struct fgpu_result fgpu_result;
+typedef struct fgpu_join_index fgpu_join_index;
+
+/* One column of a table to upload.  Mirrors PrimitiveColumn{values, validity}
+ * (src/storage/column.rs:842-845) and Utf8Column{dict, codes}
+ * (src/storage/column.rs:1272-1277). */
+typedef struct {
+    const char *name;
+    int32_t dtype;              /* FGPU_INT32 / FGPU_INT64 / FGPU_FLOAT64 / FGPU_UTF8 */
+    const void *values;         /* n_rows elements: int32 / int64 / double / uint32 codes */
+    const uint64_t *validity;   /* NULL = no NULLs; else ceil(n/64) words, LSB-first,
+                                   bit set = valid (src/storage/column.rs:132-133) */
+    int32_t dict_len;           /* FGPU_UTF8 only */
+    const char *const *dict;    /* FGPU_UTF8 only: dict_len NUL-terminated strings */
+} fgpu_column_desc;
+
+/* ── predicates (Expr::eval_mask_bits, src/query/expr.rs:635-808) ────────── */
+enum { FGPU_EQ = 0, FGPU_NE = 1, FGPU_LT = 2, FGPU_LE = 3, FGPU_GT = 4, FGPU_GE = 5 };
+enum {
+    FGPU_P_CMP = 0,         /* column op literal */
+    FGPU_P_AND = 1,         /* pops two, pushes a AND b */
+    FGPU_P_OR = 2,          /* pops two, pushes a OR b */
+    FGPU_P_IS_NULL = 3,
+    FGPU_P_IS_NOT_NULL = 4
+};
+/* A predicate is a postfix program of at most FGPU_MAX_PRED_NODES nodes. */
+#define FGPU_MAX_PRED_NODES 16
+typedef struct {
+    int32_t kind;        /* FGPU_P_* */
+    int32_t col;         /* CMP / IS_NULL / IS_NOT_NULL: column index in the table */
+    int32_t op;          /* CMP: FGPU_EQ..FGPU_GE */
+    int32_t lit_type;    /* CMP: FGPU_INT64, FGPU_INT32, FGPU_FLOAT64 or FGPU_UTF8 */
+    int64_t lit_i64;
+    double lit_f64;
+    const char *lit_str; /* FGPU_UTF8 literal */
+} fgpu_pred_node;
+typedef struct {
+    int32_t n_nodes;
+    const fgpu_pred_node *nodes;
+} fgpu_pred;
+
+/* ── aggregates (AggFunc, src/query/aggregate.rs:489-1420) ────────────────── */
+enum { FGPU_COUNT = 0, FGPU_SUM = 1, FGPU_AVG = 2, FGPU_MIN = 3, FGPU_MAX = 4 };
+#define FGPU_MAX_AGGS 8
+typedef struct {
+    int32_t func;        /* FGPU_COUNT..FGPU_MAX */
+    int32_t col;         /* column index; -1 = COUNT(*) */
+    int32_t side;        /* joins only: 0 = probe (FROM) table, 1 = build table */
+    const char *alias;   /* output name; NULL = "col<i>" like src/query/sql.rs:780-788 */
+} fgpu_agg;
+
+typedef struct {
+    int32_t key_col;     /* GROUP BY column; -1 = no GROUP BY (execute_agg_only) */
+    int32_t n_aggs;
+    const fgpu_agg *aggs;
+} fgpu_groupby_spec;
+
+enum { FGPU_INNER = 0, FGPU_LEFT = 1, FGPU_SEMI = 2, FGPU_ANTI = 3 };
+typedef struct {
+    int32_t type;        /* FGPU_INNER / FGPU_LEFT / FGPU_SEMI / FGPU_ANTI */
+    int32_t probe_key;   /* key column of the probe (left, FROM) table */
+    int32_t build_key;   /* key column of the build (right) table */
+    int32_t n_aggs;
+    const fgpu_agg *aggs;
+} fgpu_join_spec;
+
+/* A result column.  Host memory, owned by the fgpu_result. */
+typedef struct {
+    const char *name;
+    int32_t dtype;            /* FGPU_INT64 / FGPU_FLOAT64 / FGPU_UTF8 (codes) */
+    const void *values;       /* n_rows elements (int64 / double / uint32 codes) */
+    const uint8_t *valid;     /* n_rows bytes, 1 = valid; NULL = all valid */
+    int32_t dict_len;         /* FGPU_UTF8 key column: the table's dictionary */
+    const char *const *dict;
+} fgpu_column_view;
+
+/* Per-kernel timing of the last call on a context (hipEvents on its stream). */
+typedef struct {
+    float total_ms;           /* first kernel start .. last kernel end */
+    float main_ms;            /* the dominant (scan / probe) kernel */
+    float build_ms;           /* join build / table clear */
+    float finalize_ms;        /* compaction of the result */
+    int64_t bytes_main;       /* algorithmic bytes read+written by the dominant kernel */
+} fgpu_timing;
+
+/* ── context ─────────────────────────────────────────────────────────────── */
+/* Opens device `device` and creates a stream and a scratch arena for this
+ * context.  (Database::in_memory, src/db.rs:93; falcondb_open, src/api/c_ffi.rs:26) */
+int fgpu_open(int device, fgpu_ctx **out);
+void fgpu_close(fgpu_ctx *ctx);
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char *fgpu_last_error(void);
+int fgpu_abi_version(void);
+/* Returns the context's hipStream_t as void*. */
+void *fgpu_stream(fgpu_ctx *ctx);
+/* Turn per-kernel hipEvent timing on/off for this context (off by default). */
+int fgpu_set_timing(fgpu_ctx *ctx, int on);
+int fgpu_last_timing(const fgpu_ctx *ctx, fgpu_timing *out);
+/* Device bytes currently held by the context's scratch arena. */
+int64_t fgpu_scratch_bytes(const fgpu_ctx *ctx);
+
+/* ── tables (Database::create_table_from_batches, src/db.rs:180-204) ─────── */
+/* Copies `n_cols` host columns of `n_rows` rows to HBM.  `batch_rows` records the
+ * reference's batch size (65,536, src/storage/table.rs:339) for batch-relative
+ * semantics; the device layout is one contiguous array per column.  Upload also
+ * computes per-column statistics (min/max, NULL count, float exponent range,
+ * approximate distinct count) used to choose kernels. */
+int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols,
+                      int64_t n_rows, int32_t batch_rows, fgpu_table **out);
+void fgpu_table_free(fgpu_table *t);
+int64_t fgpu_table_rows(const fgpu_table *t);
+int64_t fgpu_table_device_bytes(const fgpu_table *t);
+/* Column statistics computed at upload: min/max as int64 (integers, codes) or
+ * double bits (floats); approximate distinct count. */
+int fgpu_table_column_stats(const fgpu_table *t, int col, int64_t *min_i64,
+                            int64_t *max_i64, int64_t *null_count, int64_t *distinct_est);
+
+/* ── WHERE ───────────────────────────────────────────────────────────────── */
+/* COUNT(*) ... WHERE p.  (db.rs:1439-1476 COUNT(*)-with-WHERE fast path:
+ * Σ popcount(eval_mask_bits)) */
+int fgpu_filter_count(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
+                      int64_t *out_count);
+/* The packed mask: ceil(n/64) LSB-first words into host memory.
+ * (Expr::eval_mask_bits, src/query/expr.rs:635; fast_cmp_mask :1076-1175;
+ *  clear_null_bits :961-973) */
+int fgpu_filter_mask(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
+                     uint64_t *host_words);
+/* Materialise the selected rows of `cols` in row order (execute_select +
+ * RecordBatch::filter_by_bitmask_cols, src/query/select.rs:105-160, :457-487;
+ * src/storage/table.rs:172-189; gather_by_bitmask, src/storage/column.rs:1077-1123). */
+int fgpu_filter_select(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
+                       const int32_t *cols, int n_cols, fgpu_result **out);
+
+/* ── GROUP BY / scalar aggregates ────────────────────────────────────────── */
+/* SELECT key, aggs... FROM t [WHERE p] GROUP BY key, fused in one pass.
+ * (GroupByExec::execute, src/query/aggregate.rs:535-1420, preceded by the
+ *  execute_select filter; key_col = -1 is execute_agg_only, :1394-1528).
+ * Integer SUM/MIN/MAX are returned exactly as INT64 (the reference returns
+ * Float64, Appendix semantics restated by the host layer); float SUM/AVG are
+ * exactly rounded sums.  p may be NULL. */
+int fgpu_groupby(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
+                 const fgpu_groupby_spec *spec, fgpu_result **out);
+
+/* ── hash join ───────────────────────────────────────────────────────────── */
+/* Builds a device hash index over `key_col` of the build table
+ * (build_hash_table / IntHashTable::insert, src/query/join.rs:607-645,
+ *  :1054-1107, :1318-1324).  Reusable across probes and contexts. */
+int fgpu_join_build(fgpu_ctx *ctx, const fgpu_table *build, int32_t key_col,
+                    fgpu_join_index **out);
+void fgpu_join_index_free(fgpu_join_index *idx);
+int64_t fgpu_join_index_device_bytes(const fgpu_join_index *idx);
+/* SELECT aggs FROM probe [LEFT] JOIN build ON probe.k = build.k [WHERE probe_pred]
+ * and the EXISTS / NOT EXISTS forms (SEMI / ANTI), fused with the aggregates.
+ * (execute_hash_join_projected, src/query/join.rs:184-282 + execute_agg_only,
+ *  src/query/aggregate.rs:1394; execute_hash_semi_anti_join, src/query/join.rs:49-132)
+ * `idx` may be NULL (the index is then built and dropped inside the call); it
+ * must have been built on spec->build_key of `build`. */
+int fgpu_join_agg(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *probe_pred,
+                  const fgpu_table *build, const fgpu_join_index *idx,
+                  const fgpu_join_spec *spec, fgpu_result **out);
+/* Pair mode: the (probe row, build row) pairs the join produces, NO_ROW =
+ * 0xFFFFFFFF for a LEFT row without a match; SEMI/ANTI give the kept probe rows
+ * with build row NO_ROW.  Pairs are ordered by probe row; equal probe rows list
+ * build rows in ascending order.  (probe_one / collect_matches,
+ * src/query/join.rs:657-723, :1328-1337; output_plan :828-899)  If `capacity` is
+ * too small, *n_out receives the required count and FGPU_E_INDEX_OUT_OF_BOUNDS is
+ * returned. */
+int fgpu_join_pairs(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *probe_pred,
+                    const fgpu_table *build, const fgpu_join_index *idx,
+                    const fgpu_join_spec *spec, uint32_t *probe_rows,
+                    uint32_t *build_rows, int64_t capacity, int64_t *n_out);
+
+/* ── results (QueryResult{batches}, src/db.rs:25-27) ─────────────────────── */
+int64_t fgpu_result_rows(const fgpu_result *r);
+int fgpu_result_cols(const fgpu_result *r);
+int fgpu_result_column(const fgpu_result *r, int i, fgpu_column_view *out);
+void fgpu_result_free(fgpu_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FALCON_GPU_H */

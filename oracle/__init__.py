@@ -1,0 +1,242 @@
+"""Python binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product package never
+imports it.
+
+Inputs use the same column dictionaries as falcondb_amd.Table (numpy arrays,
+DictColumn, NullableColumn) and the same Expr / AggExpr objects, so a test
+hands identical inputs to the GPU path and to this restatement.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from falcondb_amd import _abi
+from falcondb_amd.operators import (AggExpr, DictColumn, Expr, NullableColumn, RecordBatch,
+                                    _aggs_struct, _np_dtype_code, pack_validity)
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
+        L.ora_last_error.restype = C.c_char_p
+        L.ora_set_threads.argtypes = [C.c_int]
+        L.ora_threads.restype = C.c_int
+        L.ora_exact_sum.restype = C.c_double
+        L.ora_exact_sum.argtypes = [vp, i64]
+        L.ora_sum_lanes.restype = C.c_double
+        L.ora_sum_lanes.argtypes = [vp, i64]
+        L.ora_filter_mask.argtypes = [vp, C.c_int, i64, i32, vp, vp]
+        L.ora_filter_count.argtypes = [vp, C.c_int, i64, i32, vp, C.POINTER(i64)]
+        L.ora_filter_select.argtypes = [vp, C.c_int, i64, i32, vp, vp, C.c_int, C.POINTER(vp)]
+        L.ora_groupby.argtypes = [vp, C.c_int, i64, i32, vp, vp, C.c_int, C.POINTER(vp)]
+        L.ora_join.argtypes = [vp, C.c_int, i64, vp, C.c_int, i64, i32, vp, vp, C.c_int,
+                               C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(i64)]
+        L.ora_free.argtypes = [vp]
+        L.ora_result_rows.restype = i64
+        L.ora_result_rows.argtypes = [vp]
+        L.ora_result_cols.argtypes = [vp]
+        L.ora_result_column.argtypes = [vp, C.c_int, C.c_int, C.POINTER(_abi.ColumnView)]
+        L.ora_result_free.argtypes = [vp]
+        for f in ("ora_filter_mask", "ora_filter_count", "ora_filter_select", "ora_groupby",
+                  "ora_join", "ora_result_column", "ora_result_cols"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class OracleError(Exception):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(f"{_abi.ERROR_NAMES.get(rc, rc)}: {lib().ora_last_error().decode()}")
+
+
+def set_threads(t: int) -> None:
+    lib().ora_set_threads(int(t))
+
+
+def threads() -> int:
+    return int(lib().ora_threads())
+
+
+def exact_sum(x: np.ndarray) -> float:
+    x = np.ascontiguousarray(x, np.float64)
+    return float(lib().ora_exact_sum(x.ctypes.data, x.size))
+
+
+def sum_lanes(x: np.ndarray) -> float:
+    x = np.ascontiguousarray(x, np.float64)
+    return float(lib().ora_sum_lanes(x.ctypes.data, x.size))
+
+
+class _Cols:
+    """Column descriptors over host arrays (kept alive with the object)."""
+
+    def __init__(self, columns: dict):
+        self.names = list(columns.keys())
+        self.keep = []
+        self.descs = (_abi.ColumnDesc * max(len(columns), 1))()
+        self.n = 0
+        for i, (name, col) in enumerate(columns.items()):
+            d = self.descs[i]
+            d.name = name.encode()
+            self.keep.append(d.name)
+            if isinstance(col, DictColumn):
+                codes = np.ascontiguousarray(col.codes, np.uint32)
+                strs = [s.encode() for s in col.dictionary]
+                arr = (C.c_char_p * max(len(strs), 1))(*strs)
+                self.keep += [codes, strs, arr]
+                d.dtype, d.values = _abi.UTF8, codes.ctypes.data
+                d.dict_len, d.dict = len(strs), C.cast(arr, C.POINTER(C.c_char_p))
+                self.n = codes.size
+            else:
+                if isinstance(col, NullableColumn):
+                    vals = np.ascontiguousarray(col.values)
+                    vw = pack_validity(col.valid)
+                    self.keep.append(vw)
+                    d.validity = vw.ctypes.data
+                else:
+                    vals = np.ascontiguousarray(col)
+                d.dtype = _np_dtype_code(vals)
+                d.values = vals.ctypes.data
+                self.keep.append(vals)
+                self.n = vals.size
+
+    def column_index(self, name: str) -> int:
+        low = name.lower().split(".", 1)[-1]
+        for i, n in enumerate(self.names):
+            if n.lower() == low:
+                return i
+        raise OracleError(f"column '{name}' not found")
+
+    def pred(self, expr: Expr | None):
+        if expr is None:
+            return None, None
+        p, keep = expr._compile(self)
+        return C.byref(p), (p, keep)
+
+
+@dataclass
+class OracleResult:
+    exact: RecordBatch      # exact integers, exactly rounded float sums
+    reference: RecordBatch  # the reference's accumulation order and output types
+
+
+def _read(h, funcs) -> OracleResult:
+    L = lib()
+    try:
+        n = int(L.ora_result_rows(h))
+        nc = int(L.ora_result_cols(h))
+        out = []
+        for which in (0, 1):
+            names, dts, cols, valid = [], [], [], []
+            for i in range(nc):
+                v = _abi.ColumnView()
+                _check(L.ora_result_column(h, i, which, C.byref(v)))
+                names.append(v.name.decode())
+                if v.dtype == _abi.FLOAT64:
+                    arr = np.ctypeslib.as_array(C.cast(v.values, C.POINTER(C.c_double)), (n,)).copy() if n else np.zeros(0)
+                    dts.append("Float64")
+                else:
+                    arr = np.ctypeslib.as_array(C.cast(v.values, C.POINTER(C.c_int64)), (n,)).copy() if n else np.zeros(0, np.int64)
+                    dts.append("Utf8" if v.dtype == _abi.UTF8 else "Int64")
+                    if v.dtype == _abi.UTF8:
+                        d = [v.dict[k].decode() for k in range(v.dict_len)]
+                        arr = np.array([d[c] if c < len(d) else "" for c in arr.tolist()], dtype=object)
+                cols.append(arr)
+                valid.append(np.ctypeslib.as_array(v.valid, (n,)).astype(bool) if (v.valid and n) else None)
+            out.append(RecordBatch(names, dts, cols, valid, list(funcs)))
+        return OracleResult(out[0], out[1])
+    finally:
+        L.ora_result_free(h)
+
+
+def filter_mask(columns: dict, expr: Expr | None, batch_rows: int = 65536) -> np.ndarray:
+    c = _Cols(columns)
+    p, keep = c.pred(expr)
+    out = np.zeros((c.n + 63) // 64, np.uint64)
+    _check(lib().ora_filter_mask(c.descs, len(c.names), c.n, batch_rows, p, out.ctypes.data))
+    return out
+
+
+def filter_count(columns: dict, expr: Expr | None, batch_rows: int = 65536) -> int:
+    c = _Cols(columns)
+    p, keep = c.pred(expr)
+    out = C.c_int64()
+    _check(lib().ora_filter_count(c.descs, len(c.names), c.n, batch_rows, p, C.byref(out)))
+    return out.value
+
+
+def filter_select(columns: dict, expr: Expr | None, names: list[str],
+                  batch_rows: int = 65536) -> RecordBatch:
+    c = _Cols(columns)
+    p, keep = c.pred(expr)
+    idx = [c.column_index(n) for n in names]
+    arr = (C.c_int32 * max(len(idx), 1))(*idx)
+    h = C.c_void_p()
+    _check(lib().ora_filter_select(c.descs, len(c.names), c.n, batch_rows, p, arr, len(idx),
+                                   C.byref(h)))
+    return _read(h, [None] * len(idx)).exact
+
+
+def groupby(columns: dict, expr: Expr | None, key: str | None, aggs: list[AggExpr],
+            exact: bool = True, batch_rows: int = 65536) -> OracleResult:
+    c = _Cols(columns)
+    p, keep = c.pred(expr)
+    arr, keep2 = _aggs_struct(aggs, lambda side, name: c.column_index(name))
+    spec = _abi.GroupBySpec(c.column_index(key) if key else -1, len(aggs),
+                            C.cast(arr, C.POINTER(_abi.Agg)))
+    h = C.c_void_p()
+    _check(lib().ora_groupby(c.descs, len(c.names), c.n, batch_rows, p, C.byref(spec),
+                             1 if exact else 0, C.byref(h)))
+    return _read(h, ([None] if key else []) + [a.func for a in aggs])
+
+
+def join(probe: dict, build: dict, probe_key: str, build_key: str, join_type: int,
+         aggs: list[AggExpr], expr: Expr | None = None, exact: bool = True,
+         pairs: bool = False, batch_rows: int = 65536):
+    pc, bc = _Cols(probe), _Cols(build)
+    p, keep = pc.pred(expr)
+
+    def resolve(side, name):
+        return (bc if side == 1 else pc).column_index(name)
+    arr, keep2 = _aggs_struct(aggs, resolve)
+    spec = _abi.JoinSpec(join_type, pc.column_index(probe_key), bc.column_index(build_key),
+                         len(aggs), C.cast(arr, C.POINTER(_abi.Agg)))
+    h = C.c_void_p()
+    pl, pr, npairs = C.c_void_p(), C.c_void_p(), C.c_int64()
+    _check(lib().ora_join(pc.descs, len(pc.names), pc.n, bc.descs, len(bc.names), bc.n,
+                          batch_rows, p, C.byref(spec), 1 if exact else 0, C.byref(h),
+                          C.byref(pl) if pairs else None, C.byref(pr) if pairs else None,
+                          C.byref(npairs) if pairs else None))
+    res = _read(h, [a.func for a in aggs])
+    if not pairs:
+        return res
+    n = npairs.value
+    l = np.ctypeslib.as_array(C.cast(pl, C.POINTER(C.c_uint32)), (max(n, 1),))[:n].copy()
+    r = np.ctypeslib.as_array(C.cast(pr, C.POINTER(C.c_uint32)), (max(n, 1),))[:n].copy()
+    lib().ora_free(pl)
+    lib().ora_free(pr)
+    return res, l, r

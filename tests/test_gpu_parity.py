@@ -1,0 +1,321 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer results and group keys must match bit for bit;
+float SUM / AVG are exactly rounded on both sides, so they must match bit for
+bit too, and they must lie within 1e-12 relative of the reference-order
+float sum (the tolerance the reference itself uses,
+src/query/aggregate.rs:3662-3667)."""
+import math
+
+import numpy as np
+import pytest
+
+import falcondb_amd as fa
+import oracle as ora
+from falcondb_amd import datagen as dg
+from falcondb_amd._abi import ANTI, AVG, COUNT, INNER, LEFT, MAX, MIN, NO_ROW, SEMI, SUM
+from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
+
+pytestmark = pytest.mark.gpu
+
+BOUNDARY_ROWS = [0, 1, 63, 64, 65, 127, 128, 129, 1000, 4095, 10000, 65536 + 7]
+FLOAT_RTOL = 1e-12
+
+
+def _table(ctx, cols, batch_rows=65536):
+    return fa.Table(ctx, cols, batch_rows=batch_rows)
+
+
+def assert_batches_equal(gpu: fa.RecordBatch, exp: fa.RecordBatch, key: str | None,
+                         ref: fa.RecordBatch | None = None):
+    assert gpu.names == exp.names
+    assert gpu.num_rows == exp.num_rows, (gpu.num_rows, exp.num_rows)
+    if key is not None:
+        gpu = gpu.sort_by(key)
+        exp = exp.sort_by(key)
+        if ref is not None:
+            ref = ref.sort_by(key)
+    for i, name in enumerate(exp.names):
+        g, e = gpu.columns[i], exp.columns[i]
+        gv = gpu.validity[i] if gpu.validity[i] is not None else np.ones(g.size, bool)
+        ev = exp.validity[i] if exp.validity[i] is not None else np.ones(e.size, bool)
+        assert np.array_equal(gv, ev), f"validity of {name}"
+        if g.dtype == object:
+            assert list(g[gv]) == list(e[ev]), name
+            continue
+        if gpu.dtypes[i] == "Float64":
+            ge, ee = g[gv], e[ev].astype(np.float64)
+            assert np.array_equal(ge.view(np.int64), ee.view(np.int64)) or np.array_equal(ge, ee), \
+                f"{name}: float results differ: max |d| {np.max(np.abs(ge - ee)) if ge.size else 0}"
+            if ref is not None and ref.dtypes[i] == "Float64":
+                r = ref.columns[i][ev]
+                ok = np.isclose(ge, r, rtol=FLOAT_RTOL, atol=0) | (ge == r)
+                # the reference-order sum may itself be off for cancelling groups
+                frac = 1.0 - ok.mean() if ok.size else 0.0
+                assert frac < 1e-3, f"{name}: {frac:.2%} beyond 1e-12 of reference order"
+        else:
+            assert np.array_equal(g[gv], e[ev].astype(g.dtype)), name
+
+
+# ─── WHERE ───────────────────────────────────────────────────────────────────
+PREDS = [
+    lambda: col("score") > 50,
+    lambda: col("score") >= 50.5,
+    lambda: col("score") < 10,
+    lambda: col("score") <= 0.5,
+    lambda: col("score") == 50.5,
+    lambda: col("score") != 50.5,
+    lambda: col("id") > 500,
+    lambda: col("id") == 7,
+    lambda: col("k32") < 40,
+    lambda: col("k32") >= 4294967336,  # Int64 literal narrowed `as i32` -> 40
+    lambda: col("dept") == "dept03",
+    lambda: col("dept") > "dept05",
+    lambda: (col("score") > 20) & ((col("id") < 300) | (col("dept") != "dept01")),
+    lambda: col("id") > 12.5,  # Int64 column widened to f64
+    lambda: col("score") > "x",  # unsupported pairing -> no rows
+]
+
+
+def _filter_cols(n, seed=7):
+    i = np.arange(n, dtype=np.int64)
+    score = (i % 100) + 0.5
+    return {
+        "id": i,
+        "score": score,
+        "k32": (i % 100).astype(np.int32),
+        "dept": DictColumn((i % 12).astype(np.uint32), dg.DEPTS),
+    }
+
+
+@pytest.mark.parametrize("n", BOUNDARY_ROWS)
+def test_filter_count_and_mask_boundaries(ctx, n):
+    cols = _filter_cols(n)
+    t = _table(ctx, cols)
+    for mk in PREDS:
+        p = mk()
+        assert fa.count_star(ctx, t, p) == ora.filter_count(cols, p)
+        assert np.array_equal(p.eval_mask_bits(ctx, t), ora.filter_mask(cols, p))
+    # closed form: (i % 100) + 0.5 > 50 keeps exactly half of every 100 rows
+    if n % 100 == 0:
+        assert fa.count_star(ctx, t, col("score") > 50) == n // 2
+
+
+def test_filter_nulls_and_nan(ctx):
+    n = 5000
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=n)
+    v[::17] = np.nan
+    valid = rng.random(n) > 0.2
+    iv = rng.integers(-100, 100, n)
+    ivalid = rng.random(n) > 0.3
+    codes = rng.integers(0, 12, n).astype(np.uint32)
+    codes[::11] = 0xFFFFFFFF
+    cols = {"v": NullableColumn(v, valid), "i": NullableColumn(iv, ivalid),
+            "d": DictColumn(codes, dg.DEPTS)}
+    t = _table(ctx, cols)
+    for p in [col("v") > 0, col("v") != 0.25, col("i") <= 3, col("d") == "dept02",
+              col("d") != "dept02", col("v").is_null(), col("i").is_not_null(),
+              col("d").is_null(), (col("v") > -1) | col("i").is_null()]:
+        assert np.array_equal(p.eval_mask_bits(ctx, t), ora.filter_mask(cols, p))
+        assert fa.count_star(ctx, t, p) == ora.filter_count(cols, p)
+
+
+def test_filter_select_materialises_in_row_order(ctx):
+    n = 200_003
+    t1 = dg.c1_table(n)
+    cols = {"id": t1["id"], "dept": DictColumn(t1["dept"], dg.DEPTS), "score": t1["score"],
+            "salary": t1["salary"]}
+    t = _table(ctx, cols)
+    p = col("score") > 50
+    g = fa.execute_select(ctx, t, p, ["id", "score", "dept", "salary"])
+    e = ora.filter_select(cols, p, ["id", "score", "dept", "salary"])
+    assert_batches_equal(g, e, None)
+
+
+# ─── GROUP BY ────────────────────────────────────────────────────────────────
+C1_AGGS = [AggExpr(COUNT), AggExpr(SUM, "score"), AggExpr(AVG, "salary")]
+
+
+def test_groupby_c1_one_million_rows(ctx):
+    t1 = dg.c1_table(1_000_000)
+    cols = {"id": t1["id"], "dept": DictColumn(t1["dept"], dg.DEPTS), "score": t1["score"],
+            "salary": t1["salary"]}
+    t = _table(ctx, cols)
+    p = col("score") > 50
+    g = fa.GroupByExec(["dept"], C1_AGGS).execute(ctx, t, p)
+    o = ora.groupby(cols, p, "dept", C1_AGGS)
+    assert g.names == ["dept", "col1", "col2", "col3"]
+    assert g.dtypes == ["Utf8", "Int64", "Float64", "Float64"]
+    assert_batches_equal(g, o.exact, "dept", o.reference)
+
+
+C2_AGGS = [AggExpr(COUNT), AggExpr(SUM, "v1"), AggExpr(MIN, "v1"), AggExpr(MAX, "v1"),
+           AggExpr(SUM, "v2")]
+
+
+@pytest.mark.parametrize("n,nk", [(1000, 100), (300_000, 50_000), (2_000_000, 200_000)])
+def test_groupby_high_cardinality_zipf(ctx, n, nk):
+    K = dg.keys(nk, 40)
+    cols = dg.c2_table(n, K)
+    t = _table(ctx, cols)
+    p = col("f") < 40
+    g = fa.GroupByExec(["key"], C2_AGGS).execute(ctx, t, p)
+    o = ora.groupby(cols, p, "key", C2_AGGS)
+    assert_batches_equal(g, o.exact, "key", o.reference)
+    # reference output typing: SUM/MIN/MAX of Int64 come back as Float64
+    assert g.to_reference_types().dtypes == ["Int64", "Int64", "Float64", "Float64", "Float64", "Float64"]
+
+
+@pytest.mark.parametrize("card", [8, 1000, 70_000, 1_000_000])
+def test_groupby_int32_dense_keys(ctx, card):
+    cols = dg.c4_batch(400_000, card, 1234 + card)
+    t = _table(ctx, cols)
+    p = col("score") < 37.5
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "val"), AggExpr(MIN, "val"), AggExpr(MAX, "val"),
+            AggExpr(SUM, "score")]
+    g = fa.GroupByExec(["key"], aggs).execute(ctx, t, p)
+    o = ora.groupby(cols, p, "key", aggs)
+    assert_batches_equal(g, o.exact, "key", o.reference)
+
+
+def test_groupby_nulls_and_float_minmax(ctx):
+    n = 50_000
+    rng = np.random.default_rng(11)
+    key = rng.integers(-50, 50, n)
+    kvalid = rng.random(n) > 0.1
+    key[::97] = np.iinfo(np.int64).min  # folds into the NULL group like the reference
+    x = rng.normal(size=n) * 1e3
+    xvalid = rng.random(n) > 0.25
+    y = rng.integers(-10**12, 10**12, n)
+    cols = {"k": NullableColumn(key, kvalid), "x": NullableColumn(x, xvalid), "y": y}
+    t = _table(ctx, cols)
+    aggs = [AggExpr(COUNT), AggExpr(COUNT, "x"), AggExpr(SUM, "x"), AggExpr(AVG, "x"),
+            AggExpr(MIN, "x"), AggExpr(MAX, "x"), AggExpr(AVG, "y"), AggExpr(SUM, "y")]
+    g = fa.GroupByExec(["k"], aggs).execute(ctx, t, None)
+    o = ora.groupby(cols, None, "k", aggs)
+    # NULL group sorts by its placeholder key; compare keyed by (valid, key)
+    gk = np.where(g.validity[0], g.columns[0], np.iinfo(np.int64).min)
+    ok = np.where(o.exact.validity[0], o.exact.columns[0], np.iinfo(np.int64).min)
+    g.columns[0], o.exact.columns[0] = gk, ok
+    assert_batches_equal(g, o.exact, "k")
+
+
+def test_groupby_utf8_with_null_keys(ctx):
+    n = 30_000
+    rng = np.random.default_rng(5)
+    codes = rng.integers(0, 5, n).astype(np.uint32)
+    codes[::7] = 0xFFFFFFFF
+    d = ["a", "b", "", "c", "d"]  # NULL joins the "" group
+    cols = {"g": DictColumn(codes, d), "v": rng.integers(0, 100, n)}
+    t = _table(ctx, cols)
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "v")]
+    g = fa.GroupByExec(["g"], aggs).execute(ctx, t, col("v") > 10)
+    o = ora.groupby(cols, col("v") > 10, "g", aggs)
+    assert_batches_equal(g, o.exact, "g")
+
+
+def test_scalar_aggregates(ctx):
+    n = 1_234_567
+    rng = np.random.default_rng(8)
+    cols = {"a": rng.normal(size=n), "b": rng.integers(-10**6, 10**6, n),
+            "c": rng.integers(0, 100, n).astype(np.int32)}
+    t = _table(ctx, cols)
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "a"), AggExpr(MIN, "b"), AggExpr(MAX, "a"),
+            AggExpr(AVG, "b"), AggExpr(SUM, "c")]
+    for p in [None, col("c") < 30, col("c") > 1000]:
+        g = fa.execute_agg_only(ctx, t, aggs, p)
+        o = ora.groupby(cols, p, None, aggs)
+        assert_batches_equal(g, o.exact, None)
+
+
+def test_closed_form_sums(ctx):
+    # diag_agg: score = (i % 100) + 0.5 over 100k rows -> SUM 5,000,000, MIN 0.5,
+    # MAX 99.5, AVG 50, SUM(id) = n(n-1)/2 (tests/diag_agg.rs:51-111;
+    # benches/query_bench.rs:697)
+    n = 100_000
+    i = np.arange(n, dtype=np.int64)
+    t = _table(ctx, {"id": i, "score": (i % 100) + 0.5}, batch_rows=1024)
+    r = fa.execute_agg_only(ctx, t, [AggExpr(SUM, "score"), AggExpr(COUNT), AggExpr(MIN, "score"),
+                                     AggExpr(MAX, "score"), AggExpr(AVG, "score"),
+                                     AggExpr(SUM, "id")])
+    assert [c[0] for c in r.columns] == [5_000_000.0, 100_000, 0.5, 99.5, 50.0, n * (n - 1) // 2]
+
+
+# ─── JOIN ────────────────────────────────────────────────────────────────────
+def _c3(nb, np_):
+    K = dg.keys(nb, 40, seed_=99)
+    b = dg.c3_build(K)
+    p = dg.c3_probe(np_, K)
+    return p, b
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT, SEMI, ANTI])
+def test_join_aggregates_match_oracle(ctx, jt):
+    pcols, bcols = _c3(20_000, 400_000)
+    pt, bt = _table(ctx, pcols), _table(ctx, bcols)
+    if jt in (INNER, LEFT):
+        aggs = [AggExpr(SUM, "bpay", side="build"), AggExpr(COUNT)]
+    else:
+        aggs = [AggExpr(SUM, "ppay"), AggExpr(COUNT)]
+    g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs)
+    o = ora.join(pcols, bcols, "pk", "bk", jt, aggs)
+    assert_batches_equal(g, o.exact, None, o.reference)
+    idx = fa.JoinIndex(ctx, bt, "bk")
+    g2 = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs, index=idx)
+    assert_batches_equal(g2, o.exact, None)
+    # pairs as multisets
+    gl, gr = fa.join_pairs(ctx, pt, bt, "pk", "bk", jt)
+    _, ol, orr = ora.join(pcols, bcols, "pk", "bk", jt, [], pairs=True)
+    gp = np.sort(gl.astype(np.uint64) << 32 | gr)
+    op = np.sort(ol.astype(np.uint64) << 32 | orr)
+    assert np.array_equal(gp, op)
+
+
+def test_join_stride_keys_closed_form(ctx):
+    # src/query/join.rs:1747-1848: 70,000 left rows, right = every 5th id ->
+    # INNER 14,000, LEFT 70,000, SEMI 14,000, ANTI 56,000
+    left = {"id": np.arange(70_000, dtype=np.int64), "lv": np.arange(70_000, dtype=np.int64) * 2}
+    right = {"rid": np.arange(0, 70_000, 5, dtype=np.int64),
+             "rv": np.arange(0, 70_000, 5, dtype=np.float64)}
+    lt, rt = _table(ctx, left), _table(ctx, right)
+    expect = {INNER: 14_000, LEFT: 70_000, SEMI: 14_000, ANTI: 56_000}
+    for jt, n in expect.items():
+        r = fa.execute_hash_join(ctx, lt, rt, "id", "rid", jt, [AggExpr(COUNT)])
+        assert r.columns[0][0] == n
+
+
+def test_join_duplicates_nulls_and_sentinel_keys(ctx):
+    rng = np.random.default_rng(21)
+    nb, np_ = 3000, 20000
+    bk = rng.integers(0, 500, nb)
+    bk[::101] = np.iinfo(np.int64).min  # a real key equal to the empty-slot sentinel
+    bvalid = rng.random(nb) > 0.05
+    pk = rng.integers(0, 700, np_)
+    pk[::203] = np.iinfo(np.int64).min
+    pvalid = rng.random(np_) > 0.05
+    bcols = {"bk": NullableColumn(bk, bvalid), "bx": rng.normal(size=nb)}
+    pcols = {"pk": NullableColumn(pk, pvalid), "px": rng.integers(-50, 50, np_)}
+    pt, bt = _table(ctx, pcols), _table(ctx, bcols)
+    for jt in (INNER, LEFT, SEMI, ANTI):
+        gl, gr = fa.join_pairs(ctx, pt, bt, "pk", "bk", jt)
+        _, ol, orr = ora.join(pcols, bcols, "pk", "bk", jt, [], pairs=True)
+        assert np.array_equal(np.sort(gl.astype(np.uint64) << 32 | gr),
+                              np.sort(ol.astype(np.uint64) << 32 | orr)), jt
+        aggs = [AggExpr(COUNT), AggExpr(SUM, "px"), AggExpr(MAX, "px")]
+        if jt in (INNER, LEFT):
+            aggs += [AggExpr(SUM, "bx", side="build")]
+        g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs)
+        o = ora.join(pcols, bcols, "pk", "bk", jt, aggs)
+        assert_batches_equal(g, o.exact, None)
+
+
+def test_join_with_probe_predicate(ctx):
+    pcols, bcols = _c3(5000, 100_000)
+    pcols["f"] = (np.arange(100_000) % 100).astype(np.int32)
+    pt, bt = _table(ctx, pcols), _table(ctx, bcols)
+    p = col("f") < 30
+    for jt in (INNER, SEMI):
+        aggs = [AggExpr(COUNT), AggExpr(SUM, "ppay")]
+        g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs, filter=p)
+        o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
+        assert_batches_equal(g, o.exact, None)

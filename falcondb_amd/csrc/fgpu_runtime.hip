@@ -290,7 +290,10 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
     }
     if (depth != 1) return set_err(FGPU_E_INVALID_QUERY, "predicate program does not reduce to one value");
     out.n = p->n_nodes;
-    if (pass_bytes) HIPCHK(hipMemcpyAsync(d_pass, h_pass.data(), pass_bytes, hipMemcpyHostToDevice, c->stream));
+    if (pass_bytes) {  // h_pass is a local: wait for the copy before it goes away
+        HIPCHK(hipMemcpyAsync(d_pass, h_pass.data(), pass_bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     return FGPU_OK;
 }
 

@@ -13,8 +13,9 @@
 
 namespace fgpu {
 
-constexpr int FILTER_UNROLL = 4;
+constexpr int FILTER_UNROLL = 8;
 
+template <int NPC>
 __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *out) {
     __shared__ unsigned long long part[BLOCK / 64];
     const int lane = threadIdx.x & 63;
@@ -24,12 +25,11 @@ __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *
     const int64_t n_words = (n + 63) >> 6;
     unsigned int cnt = 0;
     for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        int64_t rows[FILTER_UNROLL];
         bool pass[FILTER_UNROLL];
 #pragma unroll
-        for (int u = 0; u < FILTER_UNROLL; ++u) {
-            const int64_t r = ((w0 + u) << 6) + lane;
-            pass[u] = r < n && eval_pred(p, c, r);
-        }
+        for (int u = 0; u < FILTER_UNROLL; ++u) rows[u] = ((w0 + u) << 6) + lane;
+        pred_tile<FILTER_UNROLL, NPC>(p, c, rows, n, pass);
 #pragma unroll
         for (int u = 0; u < FILTER_UNROLL; ++u) cnt += pass[u] ? 1u : 0u;
     }
@@ -44,18 +44,18 @@ __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *
     }
 }
 
+template <int NPC>
 __global__ void k_filter_mask(DCols c, DPred p, int64_t n, uint64_t *words) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t n_words = (n + 63) >> 6;
     for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        int64_t rows[FILTER_UNROLL];
         bool pass[FILTER_UNROLL];
 #pragma unroll
-        for (int u = 0; u < FILTER_UNROLL; ++u) {
-            const int64_t r = ((w0 + u) << 6) + lane;
-            pass[u] = r < n && eval_pred(p, c, r);
-        }
+        for (int u = 0; u < FILTER_UNROLL; ++u) rows[u] = ((w0 + u) << 6) + lane;
+        pred_tile<FILTER_UNROLL, NPC>(p, c, rows, n, pass);
 #pragma unroll
         for (int u = 0; u < FILTER_UNROLL; ++u) {
             const uint64_t b = __ballot(pass[u]);
@@ -187,12 +187,16 @@ __global__ void k_gather(const uint64_t *words, const int64_t *offs, int64_t n_w
 
 hipError_t launch_filter_count(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
                                unsigned long long *d_count, int grid) {
-    hipLaunchKernelGGL(k_filter_count, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_count);
+    if (p.n_pcols <= 1) hipLaunchKernelGGL(k_filter_count<1>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_count);
+    else if (p.n_pcols == 2) hipLaunchKernelGGL(k_filter_count<2>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_count);
+    else hipLaunchKernelGGL(k_filter_count<MAX_PCOLS>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_count);
     return hipGetLastError();
 }
 hipError_t launch_filter_mask(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
                               uint64_t *d_words, int grid) {
-    hipLaunchKernelGGL(k_filter_mask, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_words);
+    if (p.n_pcols <= 1) hipLaunchKernelGGL(k_filter_mask<1>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_words);
+    else if (p.n_pcols == 2) hipLaunchKernelGGL(k_filter_mask<2>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_words);
+    else hipLaunchKernelGGL(k_filter_mask<MAX_PCOLS>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_words);
     return hipGetLastError();
 }
 hipError_t launch_gather(hipStream_t s, const uint64_t *words, const int64_t *offs,

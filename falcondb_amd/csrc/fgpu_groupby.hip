@@ -66,10 +66,8 @@ FGPU_DEV int64_t hbm_slot(const GbArgs &a, int64_t key, uint64_t h) {
         if (k == EMPTY_KEY) {
             int64_t expect = EMPTY_KEY;
             if (__hip_atomic_compare_exchange_strong(a.keys + i, &expect, key, __ATOMIC_RELAXED,
-                                                     __ATOMIC_RELAXED, DEV_SCOPE)) {
-                atomicAdd(a.counters, 1ull);
+                                                     __ATOMIC_RELAXED, DEV_SCOPE))
                 return (int64_t)i;
-            }
             if (expect == key) return (int64_t)i;
         }
         i = (i + 1) & a.cap_mask;
@@ -98,6 +96,7 @@ FGPU_DEV int lds_slot(int64_t *lkeys, int lmask, int limit, int64_t key, uint64_
 constexpr int GB_UNROLL = 4;  // rows per thread per tile (independent loads in flight)
 
 // ─── dense keys, LDS-private partial states ─────────────────────────────────
+template <int NPC>
 __global__ void k_gb_dense_lds(GbArgs a) {
     extern __shared__ uint64_t lds[];
     const int S = a.ops.S;
@@ -106,19 +105,30 @@ __global__ void k_gb_dense_lds(GbArgs a) {
     __syncthreads();
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
+        pred_tile<GB_UNROLL, NPC>(a.pred, a.cols, rows, a.n_rows, pass);
+        // stage B: key and aggregate inputs of every passing row, loads first
+        int64_t key[GB_UNROLL];
+        bool has_key[GB_UNROLL];
+        Inputs in[GB_UNROLL];
+#pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+            key[u] = 0;
+            has_key[u] = false;
+            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+            if (pass[u]) {
+                if (a.key_col >= 0) has_key[u] = load_key(a, rows[u], key[u]);
+                in[u] = load_inputs(a, rows[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
             if (!pass[u]) continue;
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            int64_t key;
-            const int64_t g = load_key(a, r, key) ? (key - a.key_min) : a.range;
-            update_row<LDS_SCOPE>(lds + g * S, a.ops, load_inputs(a, r));
+            const int64_t g = has_key[u] ? (key[u] - a.key_min) : a.range;
+            update_row<LDS_SCOPE>(lds + g * S, a.ops, in[u]);
         }
     }
     __syncthreads();
@@ -127,28 +137,41 @@ __global__ void k_gb_dense_lds(GbArgs a) {
 }
 
 // ─── dense keys, HBM states ─────────────────────────────────────────────────
+template <int NPC>
 __global__ void k_gb_dense_global(GbArgs a) {
     const int S = a.ops.S;
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
+        pred_tile<GB_UNROLL, NPC>(a.pred, a.cols, rows, a.n_rows, pass);
+        // stage B: key and aggregate inputs of every passing row, loads first
+        int64_t key[GB_UNROLL];
+        bool has_key[GB_UNROLL];
+        Inputs in[GB_UNROLL];
+#pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+            key[u] = 0;
+            has_key[u] = false;
+            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+            if (pass[u]) {
+                if (a.key_col >= 0) has_key[u] = load_key(a, rows[u], key[u]);
+                in[u] = load_inputs(a, rows[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
             if (!pass[u]) continue;
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            int64_t key;
-            const int64_t g = load_key(a, r, key) ? (key - a.key_min) : a.range;
-            update_row<DEV_SCOPE>(a.state + g * S, a.ops, load_inputs(a, r));
+            const int64_t g = has_key[u] ? (key[u] - a.key_min) : a.range;
+            update_row<DEV_SCOPE>(a.state + g * S, a.ops, in[u]);
         }
     }
 }
 
 // ─── hash table in HBM behind an LDS cache ──────────────────────────────────
+template <int NPC>
 __global__ void k_gb_hash(GbArgs a) {
     extern __shared__ uint64_t lds[];
     const int S = a.ops.S;
@@ -160,34 +183,43 @@ __global__ void k_gb_hash(GbArgs a) {
     __syncthreads();
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
+        pred_tile<GB_UNROLL, NPC>(a.pred, a.cols, rows, a.n_rows, pass);
+        // stage B: key and aggregate inputs of every passing row, loads first
+        int64_t key[GB_UNROLL];
+        bool has_key[GB_UNROLL];
+        Inputs in[GB_UNROLL];
+#pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+            key[u] = 0;
+            has_key[u] = false;
+            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+            if (pass[u]) {
+                if (a.key_col >= 0) has_key[u] = load_key(a, rows[u], key[u]);
+                in[u] = load_inputs(a, rows[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
             if (!pass[u]) continue;
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            int64_t key;
-            const bool has_key = load_key(a, r, key);
-            const Inputs in = load_inputs(a, r);
-            if (!has_key) {  // NULL group: state row cap
-                update_row<DEV_SCOPE>(a.state + (int64_t)(a.cap_mask + 1) * S, a.ops, in);
+            if (!has_key[u]) {  // NULL group: state row cap
+                update_row<DEV_SCOPE>(a.state + (int64_t)(a.cap_mask + 1) * S, a.ops, in[u]);
                 continue;
             }
-            const uint64_t h = hash64((uint64_t)key);
-            const int li = ls ? lds_slot(lkeys, ls - 1, a.probe_limit, key, h) : -1;
+            const uint64_t h = hash64((uint64_t)key[u]);
+            const int li = ls ? lds_slot(lkeys, ls - 1, a.probe_limit, key[u], h) : -1;
             if (li >= 0) {
-                update_row<LDS_SCOPE>(lstate + (int64_t)li * S, a.ops, in);
+                update_row<LDS_SCOPE>(lstate + (int64_t)li * S, a.ops, in[u]);
             } else {
-                const int64_t gi = hbm_slot(a, key, h);
+                const int64_t gi = hbm_slot(a, key[u], h);
                 if (gi < 0) {  // table full: the host retries with a larger one
                     atomicExch(a.counters + 1, 1ull);
                     continue;
                 }
-                update_row<DEV_SCOPE>(a.state + gi * S, a.ops, in);
+                update_row<DEV_SCOPE>(a.state + gi * S, a.ops, in[u]);
             }
         }
     }
@@ -208,47 +240,61 @@ __global__ void k_gb_hash(GbArgs a) {
 // Each thread folds its rows into registers (exact int64; double-double for
 // float sums), then the workgroup merges them into an LDS state row, which one
 // pass of atomics adds to the HBM state row.
+constexpr int SCALAR_OPS = 8;  // register accumulators per thread (the host checks)
+
+template <int NPC>
 __global__ void k_gb_scalar(GbArgs a) {
     __shared__ uint64_t lrow[64];
     const int S = a.ops.S;
     for (int i = threadIdx.x; i < S; i += blockDim.x) lrow[i] = a.ops.ident[i];
     __syncthreads();
-    int64_t rows = 0;
-    int64_t acc[MAX_OPS];
-    double hi[MAX_OPS], lo[MAX_OPS];
+    int64_t n_out = 0;
+    int64_t acc[SCALAR_OPS];
+    double hi[SCALAR_OPS], lo[SCALAR_OPS];
 #pragma unroll
-    for (int i = 0; i < MAX_OPS; ++i) {
+    for (int i = 0; i < SCALAR_OPS; ++i) {
         acc[i] = i < a.ops.n ? reg_ident(a.ops.op[i].kind) : 0;
         hi[i] = 0.0;
         lo[i] = 0.0;
     }
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
+        for (int u = 0; u < GB_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
+        pred_tile<GB_UNROLL, NPC>(a.pred, a.cols, rows, a.n_rows, pass);
+        // stage B: key and aggregate inputs of every passing row, loads first
+        int64_t key[GB_UNROLL];
+        bool has_key[GB_UNROLL];
+        Inputs in[GB_UNROLL];
+#pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            pass[u] = r < a.n_rows && eval_pred(a.pred, a.cols, r);
+            key[u] = 0;
+            has_key[u] = false;
+            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+            if (pass[u]) {
+                if (a.key_col >= 0) has_key[u] = load_key(a, rows[u], key[u]);
+                in[u] = load_inputs(a, rows[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
             if (!pass[u]) continue;
-            const int64_t r = base + u * blockDim.x + threadIdx.x;
-            const Inputs in = load_inputs(a, r);
-            ++rows;
+            ++n_out;
 #pragma unroll
-            for (int i = 0; i < MAX_OPS; ++i) {
+            for (int i = 0; i < SCALAR_OPS; ++i) {
                 if (i < a.ops.n) {
                     const DOp o = a.ops.op[i];
-                    if ((in.valid >> o.col) & 1) reg_add(o.kind, pick(in, o.col), acc[i], hi[i], lo[i]);
+                    if ((in[u].valid >> o.col) & 1) reg_add(o.kind, pick(in[u], o.col), acc[i], hi[i], lo[i]);
                 }
             }
         }
     }
     // fold registers into the LDS row
-    if (rows) a_add<LDS_SCOPE>(lrow, (uint64_t)rows);
+    if (n_out) a_add<LDS_SCOPE>(lrow, (uint64_t)n_out);
 #pragma unroll
-    for (int i = 0; i < MAX_OPS; ++i)
+    for (int i = 0; i < SCALAR_OPS; ++i)
         if (i < a.ops.n) fold_reg(lrow, a.ops.op[i], acc[i], hi[i], lo[i]);
     __syncthreads();
     if (threadIdx.x == 0) merge_state(a.state, lrow, a.ops);
@@ -351,22 +397,20 @@ hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint6
     return hipGetLastError();
 }
 
+template <int NPC>
+static void launch_gb_npc(hipStream_t s, int mode, const GbArgs &a, int grid, int block, size_t lds) {
+    switch (mode) {
+        case GB_DENSE_LDS: hipLaunchKernelGGL(k_gb_dense_lds<NPC>, dim3(grid), dim3(block), lds, s, a); break;
+        case GB_DENSE_GLOBAL: hipLaunchKernelGGL(k_gb_dense_global<NPC>, dim3(grid), dim3(block), 0, s, a); break;
+        case GB_HASH: hipLaunchKernelGGL(k_gb_hash<NPC>, dim3(grid), dim3(block), lds, s, a); break;
+        default: hipLaunchKernelGGL(k_gb_scalar<NPC>, dim3(grid), dim3(block), 0, s, a); break;
+    }
+}
+
 hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,
                           size_t lds_bytes) {
-    switch (mode) {
-        case GB_DENSE_LDS:
-            hipLaunchKernelGGL(k_gb_dense_lds, dim3(grid), dim3(block), lds_bytes, s, a);
-            break;
-        case GB_DENSE_GLOBAL:
-            hipLaunchKernelGGL(k_gb_dense_global, dim3(grid), dim3(block), 0, s, a);
-            break;
-        case GB_HASH:
-            hipLaunchKernelGGL(k_gb_hash, dim3(grid), dim3(block), lds_bytes, s, a);
-            break;
-        default:
-            hipLaunchKernelGGL(k_gb_scalar, dim3(grid), dim3(block), 0, s, a);
-            break;
-    }
+    if (a.pred.n_pcols <= 2) launch_gb_npc<2>(s, mode, a, grid, block, lds_bytes);
+    else launch_gb_npc<MAX_PCOLS>(s, mode, a, grid, block, lds_bytes);
     return hipGetLastError();
 }
 

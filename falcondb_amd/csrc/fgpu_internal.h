@@ -202,17 +202,22 @@ enum : int8_t {
     CMP_IS_NOT_NULL = 7
 };
 
+constexpr int MAX_PCOLS = 4;  // distinct columns one predicate may read
+
 struct DPredNode {
     int8_t kind;   // FGPU_P_*
     int8_t cls;    // CMP_*
     int8_t op;     // FGPU_EQ..FGPU_GE
     int8_t col;    // column slot in DCols
+    int8_t pc;     // index of that column in DPred::pcol (tile evaluation)
     int64_t lit_i;
     double lit_f;
     const uint8_t *pass;  // CMP_DICT
 };
 struct DPred {
-    int n;  // 0 = no predicate (all rows pass)
+    int n;        // 0 = no predicate (all rows pass)
+    int n_pcols;  // distinct columns the predicate reads
+    int8_t pcol[MAX_PCOLS];  // their DCols slots
     DPredNode nodes[FGPU_MAX_PRED_NODES];
 };
 
@@ -273,6 +278,96 @@ FGPU_DEV bool eval_pred(const DPred &p, const DCols &c, int64_t r) {
         }
     }
     return (st >> (sp - 1)) & 1;
+}
+
+// ─── tile evaluation: all loads of a tile issued before any use ─────────────
+// Loads the value (64-bit pattern) and validity of column k at row r.
+FGPU_DEV void load_cell(const DCols &c, int k, int64_t r, int64_t &v, bool &ok) {
+    switch (c.dtype[k]) {
+        case FGPU_INT32: v = (int64_t)((const int32_t *)c.v[k])[r]; break;
+        case FGPU_UTF8: v = (int64_t)((const uint32_t *)c.v[k])[r]; break;
+        default: v = ((const int64_t *)c.v[k])[r]; break;
+    }
+    const uint64_t *vb = c.valid[k];
+    ok = vb ? ((vb[r >> 6] >> (r & 63)) & 1) : (c.dtype[k] != FGPU_UTF8 || (uint32_t)v != NULL_CODE);
+}
+
+template <int NPC>
+FGPU_DEV int64_t pickn(const int64_t (&v)[NPC], int j) {
+    int64_t r = v[0];
+#pragma unroll
+    for (int k = 1; k < NPC; ++k) r = j == k ? v[k] : r;
+    return r;
+}
+
+FGPU_DEV bool leaf_reg(const DPredNode &nd, int64_t v, bool ok) {
+    switch (nd.cls) {
+        case CMP_IS_NULL: return !ok;
+        case CMP_IS_NOT_NULL: return ok;
+        case CMP_FALSE: return false;
+        case CMP_F64: return ok && cmp_op(__builtin_bit_cast(double, v), nd.lit_f, nd.op);
+        case CMP_I64: return ok && cmp_op(v, nd.lit_i, nd.op);
+        case CMP_I64_AS_F64: return ok && cmp_op((double)v, nd.lit_f, nd.op);
+        case CMP_I32: return ok && cmp_op((int32_t)v, (int32_t)nd.lit_i, nd.op);
+        default: {
+            const uint32_t code = (uint32_t)v;
+            return code != NULL_CODE && nd.pass[code];
+        }
+    }
+}
+
+template <int NPC>
+FGPU_DEV bool eval_reg(const DPred &p, const int64_t (&v)[NPC], uint32_t ok) {
+    if (p.n == 1) return leaf_reg(p.nodes[0], pickn<NPC>(v, p.nodes[0].pc), (ok >> p.nodes[0].pc) & 1);
+    uint32_t st = 0;
+    int sp = 0;
+    for (int i = 0; i < p.n; ++i) {
+        const DPredNode &nd = p.nodes[i];
+        if (nd.kind == FGPU_P_AND || nd.kind == FGPU_P_OR) {
+            const uint32_t b = (st >> (sp - 1)) & 1, a = (st >> (sp - 2)) & 1;
+            const uint32_t r = nd.kind == FGPU_P_AND ? (a & b) : (a | b);
+            sp -= 2;
+            st = (st & ~(3u << sp)) | (r << sp);
+            ++sp;
+        } else {
+            const uint32_t r = leaf_reg(nd, pickn<NPC>(v, nd.pc), (ok >> nd.pc) & 1) ? 1u : 0u;
+            st = (st & ~(1u << sp)) | (r << sp);
+            ++sp;
+        }
+    }
+    return (st >> (sp - 1)) & 1;
+}
+
+// pass[u] = row rows[u] exists (< n) and satisfies p.  Every predicate column
+// of every row of the tile is loaded before the first comparison, so a wave
+// keeps U * n_pcols independent loads in flight.
+// NPC (1, 2 or 4) bounds the predicate columns the caller was compiled for;
+// the host launches an instantiation with NPC >= p.n_pcols.
+template <int U, int NPC = MAX_PCOLS>
+FGPU_DEV void pred_tile(const DPred &p, const DCols &c, const int64_t (&rows)[U], int64_t n,
+                        bool (&pass)[U]) {
+    if (p.n == 0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) pass[u] = rows[u] < n;
+        return;
+    }
+    int64_t v[U][NPC];
+    uint32_t ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        ok[u] = 0;
+#pragma unroll
+        for (int j = 0; j < NPC; ++j) {
+            v[u][j] = 0;
+            if (j < p.n_pcols && rows[u] < n) {
+                bool okj;
+                load_cell(c, p.pcol[j], rows[u], v[u][j], okj);
+                ok[u] |= (okj ? 1u : 0u) << j;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pass[u] = rows[u] < n && eval_reg<NPC>(p, v[u], ok[u]);
 }
 
 // ─── group state operations ─────────────────────────────────────────────────

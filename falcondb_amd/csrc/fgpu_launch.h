@@ -114,10 +114,9 @@ struct JoinBuildArgs {
     int key_dtype;          // FGPU_INT32 / FGPU_INT64
     const uint64_t *key_valid;
     int64_t n;
-    int64_t *keys;          // cap slots, EMPTY = INT64_MIN
-    uint32_t *rows;
+    int64_t *slots;         // 2 * cap words: slot i = {key (EMPTY = INT64_MIN), build row}
     uint64_t cap_mask;
-    unsigned long long *counters;  // [0] inserted, [1] overflow, [2] rows with key INT64_MIN
+    unsigned long long *counters;  // [1] overflow, [2] rows with key INT64_MIN, [3] duplicate keys
     uint32_t *min_key_rows;        // build rows whose key is INT64_MIN (list)
     int64_t min_key_cap;
 };
@@ -140,11 +139,11 @@ struct JoinProbeArgs {
     int pkey;         // probe key slot in pcols
     int type;         // FGPU_INNER..FGPU_ANTI
     int64_t n_probe;
-    const int64_t *keys;
-    const uint32_t *rows;
+    const int64_t *slots;  // 2 * cap words: {key, build row}
     uint64_t cap_mask;
     const uint32_t *min_key_rows;  // build rows with key INT64_MIN
     int64_t n_min_key_rows;
+    int unique_keys;  // build keys are distinct: a probe stops at its first match
     int n_ops;
     JoinOp ops[FGPU_MAX_AGGS];
     int S;
@@ -155,6 +154,39 @@ struct JoinProbeArgs {
     uint32_t *out_l;
     uint32_t *out_r;
 };
+// Radix-partitioned join.  The build side is split by the top `log_p` hash
+// bits into P sub-tables of T 16-byte slots {key, payload} (payload = the one
+// build column the aggregates read, as raw bits, or the build row); pass 1
+// streams the probe side once and appends records {key, probe payload...} to
+// per-(workgroup, partition) cells; pass 2 probes each partition's records
+// against its sub-table with every workgroup of one XCD on the same partition,
+// so the sub-table stays in that XCD's L2.
+struct JoinPartArgs {
+    JoinProbeArgs pa;         // columns, predicate, type, ops, state row, side list
+    int log_p;                // P = 1 << log_p
+    int T;                    // slots per sub-table (power of two)
+    int64_t *sub;             // P * T * 2 words
+    unsigned int *sub_cnt;    // keys per sub-table
+    int build_pay_col;        // bcols slot of the inlined build column (-1 = build row)
+    int G;                    // pass-1 workgroups
+    int64_t rows_per_wg;
+    int Cw;                   // records per cell
+    int RW;                   // words per record (key + probe payloads)
+    int n_pay;                // probe payload columns carried in records (<= 2)
+    int8_t pay_col[2];        // their pcols slots
+    int8_t field_of_op[FGPU_MAX_AGGS];  // probe-side op -> record field (1..n_pay)
+    int64_t *cells;           // G * P * Cw * RW words
+    unsigned int *cell_cnt;   // G * P
+    unsigned long long *flags;  // [0] cell overflow, [1] sub-table overflow, [2] side-list rows,
+                                // [3] duplicate build keys seen
+    uint32_t *min_key_rows;   // build rows with key INT64_MIN (pass-1 build output)
+    int64_t min_key_cap;
+    int split;                // pass-2 workgroups per XCD
+};
+hipError_t launch_join_part_count(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);
+hipError_t launch_join_part_build(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);
+hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int block);
+hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int block);
 hipError_t launch_join_build(hipStream_t s, const JoinBuildArgs &a, int grid);
 hipError_t launch_join_probe_agg(hipStream_t s, const JoinProbeArgs &a, int grid, int block);
 hipError_t launch_join_pairs(hipStream_t s, const JoinProbeArgs &a, int pass, int grid);

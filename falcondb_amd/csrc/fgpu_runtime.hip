@@ -82,10 +82,10 @@ struct fgpu_join_index {
     const fgpu_table *build = nullptr;
     int key_col = -1;
     int64_t cap = 0;
-    int64_t *keys = nullptr;
-    uint32_t *rows = nullptr;
+    int64_t *slots = nullptr;  // 2 * cap words: {key, build row}
     uint32_t *min_key_rows = nullptr;
     int64_t n_min_key_rows = 0;
+    int unique_keys = 0;
     int64_t bytes = 0;
 };
 
@@ -106,7 +106,8 @@ struct fgpu_result {
 
 // ─── context ────────────────────────────────────────────────────────────────
 enum BufSlot {
-    B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT, B_NSLOTS
+    B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT,
+    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NSLOTS
 };
 
 struct Tuning {
@@ -119,6 +120,11 @@ struct Tuning {
     int join_waves_per_cu = 16;
     int filter_waves_per_cu = 32;
     int keep_on_device = 0;  // 1: results stay in device scratch (only the row count is copied)
+    int join_partition = 1;        // radix-partitioned join for large probes
+    int join_part_min_rows = 1 << 21;
+    int join_partitions = 32;      // radix partitions of the build / probe sides
+    int join_part_wgs_per_cu = 4;  // pass-1 workgroups per CU
+    int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
 };
 
 struct fgpu_ctx {
@@ -209,6 +215,7 @@ struct Binder {
 int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b, DPred &out) {
     out = DPred{};
     out.n = 0;
+    out.n_pcols = 0;
     if (!p || p->n_nodes == 0) return FGPU_OK;
     if (p->n_nodes > FGPU_MAX_PRED_NODES)
         return set_err(FGPU_E_INVALID_QUERY, "predicate has %d nodes (max %d)", p->n_nodes, FGPU_MAX_PRED_NODES);
@@ -245,6 +252,14 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
         const int slot = b.bind(nd.col);
         if (slot < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads more than %d columns", MAX_COLS);
         o.col = (int8_t)slot;
+        int pc = 0;
+        while (pc < out.n_pcols && out.pcol[pc] != slot) ++pc;
+        if (pc == out.n_pcols) {
+            if (out.n_pcols == MAX_PCOLS)
+                return set_err(FGPU_E_UNSUPPORTED, "a predicate may read at most %d distinct columns", MAX_PCOLS);
+            out.pcol[out.n_pcols++] = (int8_t)slot;
+        }
+        o.pc = (int8_t)pc;
         ++depth;
         if (nd.kind == FGPU_P_IS_NULL) { o.cls = CMP_IS_NULL; continue; }
         if (nd.kind == FGPU_P_IS_NOT_NULL) { o.cls = CMP_IS_NOT_NULL; continue; }
@@ -564,6 +579,8 @@ int fgpu_open(int device, fgpu_ctx **out) {
     t.join_block = env_int("FGPU_JOIN_BLOCK", t.join_block);
     t.join_waves_per_cu = env_int("FGPU_JOIN_WAVES", t.join_waves_per_cu);
     t.filter_waves_per_cu = env_int("FGPU_FILTER_WAVES", t.filter_waves_per_cu);
+    t.join_partition = env_int("FGPU_JOIN_PARTITION", t.join_partition);
+    t.join_partitions = env_int("FGPU_JOIN_PARTITIONS", t.join_partitions);
     *out = c;
     return FGPU_OK;
 }
@@ -612,12 +629,21 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_waves_per_cu") t.join_waves_per_cu = value;
     else if (n == "filter_waves_per_cu") t.filter_waves_per_cu = value;
     else if (n == "keep_on_device") t.keep_on_device = value;
+    else if (n == "join_partition") t.join_partition = value;
+    else if (n == "join_part_min_rows") t.join_part_min_rows = value;
+    else if (n == "join_partitions") t.join_partitions = value;
+    else if (n == "join_part_wgs_per_cu") t.join_part_wgs_per_cu = value;
+    else if (n == "join_probe_wgs_per_cu") t.join_probe_wgs_per_cu = value;
     else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
     if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
         t.join_block > 1024 || t.join_block % 64)
         return set_err(FGPU_E_INVALID_QUERY, "block sizes must be multiples of 64 in [64, 1024]");
     if (t.hash_lds_slots & (t.hash_lds_slots - 1))
         return set_err(FGPU_E_INVALID_QUERY, "hash_lds_slots must be a power of two");
+    if (t.join_partitions < 8 || t.join_partitions > 1024 || (t.join_partitions & (t.join_partitions - 1)))
+        return set_err(FGPU_E_INVALID_QUERY, "join_partitions must be a power of two in [8, 1024]");
+    if (t.join_part_wgs_per_cu < 1 || t.join_probe_wgs_per_cu < 1)
+        return set_err(FGPU_E_INVALID_QUERY, "join workgroup knobs out of range");
     return FGPU_OK;
 }
 
@@ -927,9 +953,14 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
     // strategy
     int mode;
     int64_t n_state_groups;
-    if (!key) {
+    if (!key && L.ops.n <= 8) {  // register accumulators of the scalar kernel
         mode = GB_SCALAR;
         a.range = 0;
+        n_state_groups = 1;
+    } else if (!key) {  // many aggregates, no GROUP BY: one dense group
+        a.key_min = 0;
+        a.range = 0;
+        mode = GB_DENSE_LDS;
         n_state_groups = 1;
     } else {
         a.key_is_utf8 = key->dtype == FGPU_UTF8;
@@ -1014,7 +1045,7 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
         e.state = d_state;
         e.S = S;
         e.n_slots = n_state_groups;
-        e.mode = mode;
+        e.mode = key ? mode : GB_SCALAR;  // no GROUP BY: always one row
         e.key_min = a.key_min;
         e.key_is_utf8 = a.key_is_utf8;
         e.out_key = key ? ob.key : nullptr;
@@ -1042,8 +1073,7 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
 void fgpu_join_index_free(fgpu_join_index *idx) {
     if (!idx) return;
     (void)hipSetDevice(idx->device);
-    if (idx->keys) (void)hipFree(idx->keys);
-    if (idx->rows) (void)hipFree(idx->rows);
+    if (idx->slots) (void)hipFree(idx->slots);
     if (idx->min_key_rows) (void)hipFree(idx->min_key_rows);
     delete idx;
 }
@@ -1066,9 +1096,9 @@ static int join_build_impl(fgpu_ctx *c, const fgpu_table *build, int32_t key_col
         fgpu_join_index_free(idx);
         return rc;
     };
-    if (hipMalloc(&idx->keys, idx->cap * 8) != hipSuccess || hipMalloc(&idx->rows, idx->cap * 4) != hipSuccess)
+    if (hipMalloc(&idx->slots, idx->cap * 16) != hipSuccess)
         return fail(set_err(FGPU_E_OUT_OF_MEMORY, "join index of %lld slots", (long long)idx->cap));
-    idx->bytes = idx->cap * 12;
+    idx->bytes = idx->cap * 16;
     const bool has_min = kc.dtype == FGPU_INT64 && kc.imin == INT64_MIN;
     const int64_t min_cap = has_min ? std::max<int64_t>(n, 1) : 0;
     if (min_cap && hipMalloc(&idx->min_key_rows, min_cap * 4) != hipSuccess)
@@ -1077,26 +1107,26 @@ static int join_build_impl(fgpu_ctx *c, const fgpu_table *build, int32_t key_col
     if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return fail(rc);
     mark(c, 4);
     hipError_t e = launch_fill_u64(c->stream, (uint64_t *)d_cnt, 8, 0);
-    if (e == hipSuccess) e = launch_fill_u64(c->stream, (uint64_t *)idx->keys, idx->cap, (uint64_t)EMPTY_KEY);
+    if (e == hipSuccess) e = launch_fill_u64(c->stream, (uint64_t *)idx->slots, 2 * idx->cap, (uint64_t)EMPTY_KEY);
     JoinBuildArgs ba{};
     ba.keys_col = kc.d_values;
     ba.key_dtype = kc.dtype;
     ba.key_valid = kc.d_valid;
     ba.n = n;
-    ba.keys = idx->keys;
-    ba.rows = idx->rows;
+    ba.slots = idx->slots;
     ba.cap_mask = (uint64_t)(idx->cap - 1);
     ba.counters = d_cnt;
     ba.min_key_rows = idx->min_key_rows;
     ba.min_key_cap = min_cap;
     if (e == hipSuccess && n) e = launch_join_build(c->stream, ba, grid_for(c, n, BLOCK, 16));
     mark(c, 5);
-    unsigned long long h[3] = {0, 0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(h, d_cnt, 24, hipMemcpyDeviceToHost, c->stream);
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_cnt, 32, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(set_err(FGPU_E_DEVICE, "join build: %s", hipGetErrorString(e)));
     if (h[1]) return fail(set_err(FGPU_E_DEVICE, "join index overflow"));
     idx->n_min_key_rows = (int64_t)h[2];
+    idx->unique_keys = !h[3] && h[2] <= 1;
     if (idx->min_key_rows) {
         // slot order of the side list is arbitrary; sort it so probes see ascending rows
         std::vector<uint32_t> v(h[2]);
@@ -1168,6 +1198,123 @@ static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *p
     return FGPU_OK;
 }
 
+// Radix-partitioned probe (see JoinPartArgs).  Returns FGPU_OK with the state
+// row filled, a negative value when the inputs do not suit it (the caller then
+// runs the direct probe), or an error code.
+static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu_table *build,
+                                const fgpu_join_spec *spec, JoinProbeArgs &a, const AggLayout &L) {
+    const Tuning &tu = c->tune;
+    if (!tu.join_partition || probe->n_rows < tu.join_part_min_rows) return -1;
+    // probe payload columns (non-NULL, at most 2) and build columns the aggregates read
+    std::vector<int> pcols_used, bcols_used;
+    for (int i = 0; i < spec->n_aggs; ++i) {
+        const fgpu_agg &g = spec->aggs[i];
+        if (g.col < 0) continue;
+        auto &v = g.side == 1 ? bcols_used : pcols_used;
+        if (std::find(v.begin(), v.end(), g.col) == v.end()) v.push_back(g.col);
+    }
+    if (pcols_used.size() > 2) return -1;
+    for (int col : pcols_used)
+        if (probe->cols[col].null_count) return -1;
+    JoinPartArgs jp{};
+    jp.pa = a;
+    jp.n_pay = (int)pcols_used.size();
+    Binder pb(probe), bb(build);  // slots must match those in `a`: rebind identically
+    (void)pb;
+    (void)bb;
+    for (int k = 0; k < jp.n_pay; ++k) {
+        // find the slot `a` uses for this column
+        int slot = -1;
+        for (int s2 = 0; s2 < a.pcols.n; ++s2)
+            if (a.pcols.v[s2] == probe->cols[pcols_used[k]].d_values) slot = s2;
+        if (slot < 0) return -1;
+        jp.pay_col[k] = (int8_t)slot;
+    }
+    for (int i = 0; i < L.ops.n; ++i) {
+        jp.field_of_op[i] = 0;
+        if (L.keys[i].side == 0)
+            for (int k = 0; k < jp.n_pay; ++k)
+                if (L.keys[i].col == pcols_used[k]) jp.field_of_op[i] = (int8_t)(k + 1);
+    }
+    jp.build_pay_col = -1;
+    if (bcols_used.size() == 1 && !build->cols[bcols_used[0]].null_count) {
+        for (int s2 = 0; s2 < a.bcols.n; ++s2)
+            if (a.bcols.v[s2] == build->cols[bcols_used[0]].d_values) jp.build_pay_col = s2;
+    }
+    const DevCol &bkc = build->cols[spec->build_key];
+    const int64_t nb = build->n_rows;
+    // P partitions (>= 8: one or more per XCD), each with a table at load <= 0.5
+    const int64_t P = std::max<int64_t>(8, next_pow2(tu.join_partitions));
+    jp.T = (int)std::max<int64_t>(1024, next_pow2(4 * ((nb + P - 1) / P)));  // load <= 0.25
+    if (jp.T > (1 << 24)) return -1;
+    jp.log_p = 0;
+    while ((int64_t(1) << jp.log_p) < P) ++jp.log_p;
+    jp.G = c->num_cus * tu.join_part_wgs_per_cu;
+    jp.rows_per_wg = (probe->n_rows + jp.G - 1) / jp.G;
+    jp.Cw = (int)std::ceil((double)jp.rows_per_wg / (double)P * 1.25) + 32;
+    jp.RW = 1 + jp.n_pay;
+    jp.split = std::max(1, c->num_cus / 8 * tu.join_probe_wgs_per_cu);
+    const size_t cell_words = (size_t)jp.G * P * jp.Cw * jp.RW;
+    int rc = FGPU_OK;
+    void *p;
+    if ((rc = buf(c, B_CELLS, cell_words * 8, &p))) return rc;
+    jp.cells = (int64_t *)p;
+    if ((rc = buf(c, B_CELLCNT, (size_t)jp.G * P * 4, &p))) return rc;
+    jp.cell_cnt = (unsigned int *)p;
+    if ((rc = buf(c, B_SUB, (size_t)P * jp.T * 16, &p))) return rc;
+    jp.sub = (int64_t *)p;
+    if ((rc = buf(c, B_SUBCNT, (size_t)P * 4 + 64, &p))) return rc;
+    jp.sub_cnt = (unsigned int *)p;
+    unsigned long long *d_flags;
+    if ((rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_flags))) return rc;
+    jp.flags = d_flags;
+    const bool has_min = bkc.dtype == FGPU_INT64 && bkc.imin == INT64_MIN;
+    jp.min_key_cap = has_min ? std::max<int64_t>(nb, 1) : 0;
+    jp.min_key_rows = nullptr;
+    if (has_min) {
+        if ((rc = buf(c, B_MINKEY, (size_t)jp.min_key_cap * 4, &p))) return rc;
+        jp.min_key_rows = (uint32_t *)p;
+    }
+    JoinBuildArgs ba{};
+    ba.keys_col = bkc.d_values;
+    ba.key_dtype = bkc.dtype;
+    ba.key_valid = bkc.d_valid;
+    ba.n = nb;
+    mark(c, 4);
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_flags, 8, 0));
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.sub, P * jp.T * 2, (uint64_t)EMPTY_KEY));
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.sub_cnt, (P + 1) / 2, 0));
+    std::vector<unsigned int> counts((size_t)P);
+    if (nb) {
+        HIPCHK(launch_join_part_count(c->stream, jp, ba, grid_for(c, nb, BLOCK, 8)));
+        HIPCHK(hipMemcpyAsync(counts.data(), jp.sub_cnt, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const unsigned mx = *std::max_element(counts.begin(), counts.end());
+        if (mx > (unsigned)(jp.T - jp.T / 8)) return -1;  // skewed keys: direct probe
+        HIPCHK(launch_join_part_build(c->stream, jp, ba, grid_for(c, nb, BLOCK, 16)));
+    }
+    mark(c, 5);
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, d_flags, 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[1]) return -1;  // an insert found no slot: direct probe
+    jp.pa.min_key_rows = jp.min_key_rows;
+    jp.pa.n_min_key_rows = (int64_t)h[2];
+    jp.pa.unique_keys = !h[3] && h[2] <= 1;
+    HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
+    mark(c, 1);
+    HIPCHK(launch_join_part_scatter(c->stream, jp, c->tune.join_block));
+    HIPCHK(launch_join_part_probe(c->stream, jp, c->tune.join_block));
+    mark(c, 2);
+    HIPCHK(hipMemcpyAsync(h, d_flags, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[0]) {  // a cell overflowed (skewed keys): redo on the direct path
+        HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
+        return -1;
+    }
+    return FGPU_OK;
+}
+
 int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
                   const fgpu_join_index *idx, const fgpu_join_spec *spec, fgpu_result **out) {
     if (!c || !probe || !build || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
@@ -1180,27 +1327,29 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     if (int rc = join_prepare(c, probe, pred, build, idx, spec, pb, bb, a, L)) return rc;
     fgpu_join_index *own = nullptr;
     mark(c, 0);
-    if (!idx) {
-        if (int rc = join_build_impl(c, build, spec->build_key, &own)) return rc;
-        idx = own;
-    }
-    a.keys = idx->keys;
-    a.rows = idx->rows;
-    a.cap_mask = (uint64_t)(idx->cap - 1);
-    a.min_key_rows = idx->min_key_rows;
-    a.n_min_key_rows = idx->n_min_key_rows;
     uint64_t *d_state;
     int rc = buf(c, B_STATE, (size_t)a.S * 8, (void **)&d_state);
-    if (rc) {
-        fgpu_join_index_free(own);
-        return rc;
-    }
+    if (rc) return rc;
     a.state = d_state;
-    hipError_t e = launch_state_init(c->stream, nullptr, 0, d_state, 1, L.ops);
-    const int grid = grid_for(c, (probe->n_rows + 3) / 4, c->tune.join_block, c->tune.join_waves_per_cu);
-    mark(c, 1);
-    if (e == hipSuccess && probe->n_rows) e = launch_join_probe_agg(c->stream, a, grid, c->tune.join_block);
-    mark(c, 2);
+    hipError_t e = hipSuccess;
+    const int prc = join_agg_partitioned(c, probe, build, spec, a, L);
+    if (prc > 0) return prc;
+    if (prc < 0) {  // direct probe of the open-addressing index
+        if (!idx) {
+            if (int rc2 = join_build_impl(c, build, spec->build_key, &own)) return rc2;
+            idx = own;
+        }
+        a.slots = idx->slots;
+        a.cap_mask = (uint64_t)(idx->cap - 1);
+        a.min_key_rows = idx->min_key_rows;
+        a.n_min_key_rows = idx->n_min_key_rows;
+        a.unique_keys = idx->unique_keys;
+        e = launch_state_init(c->stream, nullptr, 0, d_state, 1, L.ops);
+        const int grid = grid_for(c, (probe->n_rows + 3) / 4, c->tune.join_block, c->tune.join_waves_per_cu);
+        mark(c, 1);
+        if (e == hipSuccess && probe->n_rows) e = launch_join_probe_agg(c->stream, a, grid, c->tune.join_block);
+        mark(c, 2);
+    }
     if (e != hipSuccess) {
         fgpu_join_index_free(own);
         return set_err(FGPU_E_DEVICE, "join probe: %s", hipGetErrorString(e));
@@ -1256,8 +1405,7 @@ int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
         if (int rc = join_build_impl(c, build, spec->build_key, &own)) return rc;
         idx = own;
     }
-    a.keys = idx->keys;
-    a.rows = idx->rows;
+    a.slots = idx->slots;
     a.cap_mask = (uint64_t)(idx->cap - 1);
     a.min_key_rows = idx->min_key_rows;
     a.n_min_key_rows = idx->n_min_key_rows;

@@ -19,3 +19,23 @@ def ctx():
     c = fa.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx_part():
+    """A context that takes the radix-partitioned join for every probe size."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("join_part_min_rows", 1)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx_direct():
+    """A context that always probes the open-addressing index directly."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("join_partition", 0)
+    yield c
+    c.close()

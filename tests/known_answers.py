@@ -1,0 +1,152 @@
+"""The reference's own known-answer tests (tests/golden/reference_known_answers.json),
+restated as inputs for any engine that exposes filter_count / groupby / join.
+Shared by the CPU oracle tests and the GPU tests."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+from falcondb_amd._abi import ANTI, AVG, COUNT, INNER, LEFT, MAX, MIN, SEMI, SUM
+from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_known_answers.json").read_text())
+
+# The count_star_filter predicates as Expr builders (IN / NOT IN spelled as
+# OR / AND of comparisons, the expansion the reference's IN mask computes).
+PRED = {
+    "n >= 0": lambda: col("n") >= 0,
+    "n < 0": lambda: col("n") < 0,
+    "n >= 0 AND n < 100000": lambda: (col("n") >= 0) & (col("n") < 100000),
+    "n > 10": lambda: col("n") > 10,
+    "maybe IS NULL": lambda: col("maybe").is_null(),
+    "maybe IS NOT NULL": lambda: col("maybe").is_not_null(),
+    "s = 'a'": lambda: col("s") == "a",
+    "s <> 'a'": lambda: col("s") != "a",
+    "n IN (1, 2, 3)": lambda: ((col("n") == 1) | (col("n") == 2)) | (col("n") == 3),
+    "n NOT IN (1, 2, 3)": lambda: ((col("n") != 1) & (col("n") != 2)) & (col("n") != 3),
+    "n > 5 AND n < 20": lambda: (col("n") > 5) & (col("n") < 20),
+    "n < 2 OR n > 9995": lambda: (col("n") < 2) | (col("n") > 9995),
+}
+
+
+def count_star_table(rows: int) -> dict:
+    i = np.arange(rows, dtype=np.int64)
+    return {
+        "n": i,
+        "maybe": NullableColumn(i.copy(), i % 3 != 0),
+        "s": DictColumn((i % 3).astype(np.uint32), ["a", "b", "c"]),
+    }
+
+
+def closed_form_count(pred: str, rows: int) -> int:
+    """The number of rows each predicate selects, from the table's closed form."""
+    n = np.arange(rows)
+    m = {
+        "n >= 0": n >= 0,
+        "n < 0": n < 0,
+        "n >= 0 AND n < 100000": (n >= 0) & (n < 100000),
+        "n > 10": n > 10,
+        "maybe IS NULL": n % 3 == 0,
+        "maybe IS NOT NULL": n % 3 != 0,
+        "s = 'a'": n % 3 == 0,
+        "s <> 'a'": n % 3 != 0,
+        "n IN (1, 2, 3)": (n >= 1) & (n <= 3),
+        "n NOT IN (1, 2, 3)": ~((n >= 1) & (n <= 3)),
+        "n > 5 AND n < 20": (n > 5) & (n < 20),
+        "n < 2 OR n > 9995": (n < 2) | (n > 9995),
+    }[pred]
+    return int(m.sum())
+
+
+def scalar(batch, i=0):
+    c = batch.columns[i]
+    v = batch.validity[i]
+    if v is not None and not v[0]:
+        return None
+    return c[0].item() if hasattr(c[0], "item") else c[0]
+
+
+def check_all(engine) -> int:
+    """Runs every known answer through `engine`; returns the number of checks."""
+    checks = 0
+    g = GOLDEN["count_star_filter"]
+    for rows in g["rows"]:
+        cols = count_star_table(rows)
+        for p in g["predicates"]:
+            got = engine.filter_count(cols, PRED[p]())
+            assert got == closed_form_count(p, rows), (rows, p, got)
+            checks += 1
+
+    g = GOLDEN["count_star_declines"]
+    cols = count_star_table(g["rows"])
+    r = engine.groupby(cols, col("n") >= 0, None, [AggExpr(COUNT, "maybe")])
+    assert scalar(r) == g["count_maybe_where_n_ge_0"]
+    r = engine.groupby(cols, col("n") < 10, None, [AggExpr(COUNT), AggExpr(SUM, "n")])
+    assert [scalar(r, 0), float(scalar(r, 1))] == g["count_and_sum_where_n_lt_10"]
+    r = engine.groupby(cols, col("n") >= 0, "s", [AggExpr(COUNT)])
+    assert r.num_rows == g["groups_by_s"]
+    checks += 3
+
+    g = GOLDEN["diag_agg"]
+    i = np.arange(g["rows"], dtype=np.int64)
+    cols = {"id": i, "score": (i % 100) + 0.5}
+    r = engine.groupby(cols, None, None, [AggExpr(SUM, "score"), AggExpr(COUNT), AggExpr(MIN, "score"),
+                                          AggExpr(MAX, "score"), AggExpr(AVG, "score")],
+                       batch_rows=g["batch_rows"])
+    assert [scalar(r, k) for k in range(5)] == [g["sum"], g["count"], g["min"], g["max"], g["avg"]]
+    vals = [x for b in g["nulls_batches"] for x in b]
+    v = np.array([0.0 if x is None else x for x in vals])
+    ok = np.array([x is not None for x in vals])
+    cols = {"v": NullableColumn(v, ok)}
+    r = engine.groupby(cols, None, None, [AggExpr(SUM, "v"), AggExpr(MIN, "v"), AggExpr(MAX, "v"),
+                                          AggExpr(AVG, "v"), AggExpr(COUNT, "v"), AggExpr(COUNT)])
+    assert [scalar(r, k) for k in range(6)] == [g["nulls_sum"], g["nulls_min"], g["nulls_max"],
+                                               g["nulls_avg"], g["nulls_count_v"], g["nulls_count_star"]]
+    cols = {"v": NullableColumn(np.zeros(2), np.zeros(2, bool))}
+    r = engine.groupby(cols, None, None, [AggExpr(SUM, "v"), AggExpr(MIN, "v")])
+    assert scalar(r, 0) is None and scalar(r, 1) is None
+    checks += 3
+
+    g = GOLDEN["sql_group_by"]
+
+    def grouped(d, vals, agg):
+        names = sorted(set(d))
+        cols = {"dept": DictColumn(np.array([names.index(x) for x in d], np.uint32), names)}
+        aggs = [AggExpr(COUNT)] if vals is None else [AggExpr(agg, "v")]
+        if vals is not None:
+            cols["v"] = np.array(vals, dtype=np.int64)
+        r = engine.groupby(cols, None, "dept", aggs)
+        return {k: (r.columns[1][j].item() if r.validity[1] is None or r.validity[1][j] else None)
+                for j, k in enumerate(r.columns[0])}
+    assert grouped(g["count"]["dept"], None, None) == g["count"]["expect"]
+    assert grouped(g["sum"]["dept"], g["sum"]["salary"], SUM) == g["sum"]["expect"]
+    assert grouped(g["avg"]["dept"], g["avg"]["score"], AVG) == g["avg"]["expect"]
+    assert grouped(g["min_max"]["dept"], g["min_max"]["score"], MIN) == {"eng": g["min_max"]["expect_min"]}
+    assert grouped(g["min_max"]["dept"], g["min_max"]["score"], MAX) == {"eng": g["min_max"]["expect_max"]}
+    assert grouped(g["single_group"]["dept"], g["single_group"]["n"], SUM) == {"eng": g["single_group"]["expect"]}
+    assert len(grouped(g["all_unique"]["dept"], g["all_unique"]["n"], SUM)) == g["all_unique"]["expect_groups"]
+    checks += 7
+
+    g = GOLDEN["join_stride"]
+    left = {"k": np.arange(g["left_rows"], dtype=np.int64)}
+    right = {"rk": np.arange(0, g["left_rows"], g["stride"], dtype=np.int64)}
+    for jt, name in [(INNER, "inner"), (LEFT, "left"), (SEMI, "semi"), (ANTI, "anti")]:
+        r = engine.join(left, right, "k", "rk", jt, [AggExpr(COUNT)])
+        assert scalar(r) == g[name], (name, scalar(r))
+        checks += 1
+
+    g = GOLDEN["join_null_keys"]
+
+    def side(name, vals):
+        return {name: NullableColumn(np.array([0 if x is None else x for x in vals], np.int64),
+                                     np.array([x is not None for x in vals]))}
+    left, right = side("k", g["left"]), side("rk", g["right"])
+    for jt, name in [(INNER, "inner"), (LEFT, "left_join"), (SEMI, "semi"), (ANTI, "anti")]:
+        r = engine.join(left, right, "k", "rk", jt, [AggExpr(COUNT)])
+        assert scalar(r) == g[name], (name, scalar(r))
+        checks += 1
+    sl, sr = side("k", g["self_join_keys"]), side("rk", g["self_join_keys"])
+    assert scalar(engine.join(sl, sr, "k", "rk", INNER, [AggExpr(COUNT)])) == g["self_inner"]
+    return checks + 1

@@ -1,0 +1,98 @@
+"""CPU-only checks of the native boundary: the C-ABI library loads and exports
+every symbol include/falcon_gpu.h declares, the host copy of the kernels'
+exact float accumulator agrees with math.fsum, the synthetic generators are
+deterministic, and the host-side helpers behave.  Nothing here touches a GPU."""
+import ctypes as C
+import math
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from falcondb_amd import _abi, datagen as dg
+from falcondb_amd.operators import host_exact_sum, pack_validity
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "falcon_gpu.h"
+
+
+def header_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fgpu_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(str(_abi.GPU_LIB_PATH))
+    syms = header_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the Python declarations cover the header exactly
+    assert sorted(_abi.DECLARED) == syms
+
+
+def test_abi_version_and_error_string_without_gpu():
+    lib = _abi.load()
+    assert lib.fgpu_abi_version() == 1
+    assert isinstance(lib.fgpu_last_error(), bytes)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_host_exact_sum_matches_fsum(seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(200):
+        n = int(rng.integers(0, 80))
+        scale = 10.0 ** rng.integers(-12, 12, size=n)
+        x = rng.normal(size=n) * scale
+        if seed % 2 and n:
+            x = np.concatenate([x, -x[: n // 2]])
+        assert host_exact_sum(x) == math.fsum(x)
+
+
+def test_host_exact_sum_special_values():
+    assert host_exact_sum(np.array([])) == 0.0
+    assert host_exact_sum(np.array([0.1] * 10)) == math.fsum([0.1] * 10)
+    assert math.isnan(host_exact_sum(np.array([1.0, np.nan])))
+    assert host_exact_sum(np.array([np.inf, 1.0])) == np.inf
+    assert math.isnan(host_exact_sum(np.array([np.inf, -np.inf])))
+    assert host_exact_sum(np.array([5e-324, 5e-324, 1e-320])) == math.fsum([5e-324, 5e-324, 1e-320])
+    # ties round to even
+    assert host_exact_sum(np.array([1.0, 2.0 ** -53])) == 1.0
+    assert host_exact_sum(np.array([1.0, 2.0 ** -53, 2.0 ** -105])) == 1.0 + 2.0 ** -52
+
+
+def test_generators_are_deterministic():
+    K = dg.keys(1000, 40, seed_=5)
+    assert len(set(K.tolist())) == 1000 and K.min() >= 0 and K.max() < 2 ** 40
+    a, b = dg.c2_table(50_000, K), dg.c2_table(50_000, K)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    assert set(np.unique(a["key"]).tolist()) <= set(K.tolist())
+    assert a["f"].min() >= 0 and a["f"].max() < 100
+    assert abs(a["v2"].mean()) < 0.05 and abs(a["v2"].std() - 1) < 0.05
+    # Zipf(1.1) over 1000 ranks: the top rank holds 1 / sum_k k^-1.1 of the rows
+    top = np.bincount(np.searchsorted(np.sort(K), a["key"])).max() / a["key"].size
+    expect = 1.0 / np.sum(np.arange(1, 1001, dtype=np.float64) ** -1.1)
+    assert abs(top - expect) < 0.01
+    p = dg.c3_probe(100_000, K)
+    assert 0.88 < np.isin(p["pk"], K).mean() < 0.92
+    c1 = dg.c1_table(10_000)
+    assert sorted(c1["id"].tolist()) == list(range(10_000))
+    assert c1["dept"].max() == 11 and 20000 <= c1["salary"].min() and c1["salary"].max() <= 200000
+
+
+def test_pack_validity_lsb_first():
+    v = np.zeros(130, bool)
+    v[[0, 63, 64, 129]] = True
+    w = pack_validity(v)
+    assert w.tolist() == [1 | (1 << 63), 1, 2]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr

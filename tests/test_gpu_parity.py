@@ -249,8 +249,14 @@ def _c3(nb, np_):
     return p, b
 
 
+@pytest.fixture(params=["partitioned", "direct"])
+def jctx(request, ctx_part, ctx_direct):
+    return ctx_part if request.param == "partitioned" else ctx_direct
+
+
 @pytest.mark.parametrize("jt", [INNER, LEFT, SEMI, ANTI])
-def test_join_aggregates_match_oracle(ctx, jt):
+def test_join_aggregates_match_oracle(jctx, jt):
+    ctx = jctx
     pcols, bcols = _c3(20_000, 400_000)
     pt, bt = _table(ctx, pcols), _table(ctx, bcols)
     if jt in (INNER, LEFT):
@@ -271,7 +277,8 @@ def test_join_aggregates_match_oracle(ctx, jt):
     assert np.array_equal(gp, op)
 
 
-def test_join_stride_keys_closed_form(ctx):
+def test_join_stride_keys_closed_form(jctx):
+    ctx = jctx
     # src/query/join.rs:1747-1848: 70,000 left rows, right = every 5th id ->
     # INNER 14,000, LEFT 70,000, SEMI 14,000, ANTI 56,000
     left = {"id": np.arange(70_000, dtype=np.int64), "lv": np.arange(70_000, dtype=np.int64) * 2}
@@ -284,7 +291,8 @@ def test_join_stride_keys_closed_form(ctx):
         assert r.columns[0][0] == n
 
 
-def test_join_duplicates_nulls_and_sentinel_keys(ctx):
+def test_join_duplicates_nulls_and_sentinel_keys(jctx):
+    ctx = jctx
     rng = np.random.default_rng(21)
     nb, np_ = 3000, 20000
     bk = rng.integers(0, 500, nb)
@@ -309,7 +317,8 @@ def test_join_duplicates_nulls_and_sentinel_keys(ctx):
         assert_batches_equal(g, o.exact, None)
 
 
-def test_join_with_probe_predicate(ctx):
+def test_join_with_probe_predicate(jctx):
+    ctx = jctx
     pcols, bcols = _c3(5000, 100_000)
     pcols["f"] = (np.arange(100_000) % 100).astype(np.int32)
     pt, bt = _table(ctx, pcols), _table(ctx, bcols)
@@ -319,3 +328,18 @@ def test_join_with_probe_predicate(ctx):
         g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs, filter=p)
         o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
         assert_batches_equal(g, o.exact, None)
+
+
+def test_join_partitioned_matches_direct_at_scale(ctx_part, ctx_direct):
+    # 4M probe rows over a 200k build: both paths, several aggregates per side
+    pcols, bcols = _c3(200_000, 4_000_000)
+    pcols["f"] = (np.arange(4_000_000) % 100).astype(np.int32)
+    for jt in (INNER, LEFT, SEMI, ANTI):
+        aggs = [AggExpr(COUNT), AggExpr(SUM, "ppay"), AggExpr(MIN, "ppay")]
+        if jt in (INNER, LEFT):
+            aggs += [AggExpr(SUM, "bpay", side="build")]
+        res = []
+        for c in (ctx_part, ctx_direct):
+            pt, bt = _table(c, pcols), _table(c, bcols)
+            res.append(fa.execute_hash_join(c, pt, bt, "pk", "bk", jt, aggs, filter=col("f") >= 25))
+        assert_batches_equal(res[0], res[1], None)

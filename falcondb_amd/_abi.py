@@ -112,6 +112,7 @@ EXTRA = {
     "fgpu_table_float_range": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "fgpu_host_exact_sum": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
+    "fgpu_last_plan": (C.c_char_p, [C.c_void_p]),
 }
 
 _lib = None

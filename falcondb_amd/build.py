@@ -28,8 +28,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-pass-failed", f"-I{ROOT / 'include'}",
 ]
-HIP_SOURCES = ["fgpu_runtime.hip", "fgpu_filter.hip", "fgpu_groupby.hip", "fgpu_join.hip",
-               "fgpu_batch.hip", "fgpu_server.hip"]
+HIP_SOURCES = ["fgpu_runtime.hip", "fgpu_filter.hip", "fgpu_groupby.hip", "fgpu_gbpart.hip",
+               "fgpu_join.hip", "fgpu_batch.hip", "fgpu_server.hip"]
 
 GPU_LIB = PKG / "libfalcon_gpu.so"
 GEN_LIB = PKG / "libfgpu_datagen.so"

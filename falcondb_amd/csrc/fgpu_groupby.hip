@@ -332,39 +332,7 @@ __global__ void k_gb_emit(EmitArgs a) {
             a.out_key_valid[o] = null_group ? 0 : 1;
         }
     }
-    for (int k = 0; k < a.outs.n; ++k) {
-        const DOut d = a.outs.o[k];
-        const int64_t cnt = (int64_t)st[d.cnt_word];
-        const uint64_t w = st[d.word];
-        int64_t out = 0;
-        uint8_t valid = cnt > 0;
-        switch (d.kind) {
-            case OUT_COUNT_ROWS: out = (int64_t)st[0]; valid = 1; break;
-            case OUT_COUNT: out = cnt; valid = 1; break;
-            case OUT_SUM_I:
-            case OUT_MIN_I:
-            case OUT_MAX_I: out = (int64_t)w; break;
-            case OUT_AVG_I: out = __double_as_longlong(cnt ? (double)(int64_t)w / (double)cnt : 0.0); break;
-            case OUT_SUM_F:
-            case OUT_AVG_F: {
-                double s = fx_finish((const int64_t *)(st + d.word), d.W, d.L,
-                                     d.has_flags ? st[d.word + d.W] : 0ull);
-                if (d.kind == OUT_AVG_F) s = cnt ? s / (double)cnt : 0.0;
-                out = __double_as_longlong(s);
-                break;
-            }
-            case OUT_MIN_F:
-                out = (int64_t)f64_unorder(w);
-                valid = valid && w != ~0ull;
-                break;
-            case OUT_MAX_F:
-                out = (int64_t)f64_unorder(w);
-                valid = valid && w != 0ull;
-                break;
-        }
-        a.out_val[k][o] = out;
-        a.out_valid[k][o] = valid;
-    }
+    emit_row(a.outs, st, o, a.out_val, a.out_valid);
 }
 
 __global__ void k_state_init(int64_t *keys, int64_t n_keys, uint64_t *state, int64_t n_words,

@@ -74,6 +74,40 @@ struct EmitArgs {
     unsigned long long *out_count;
 };
 
+// Radix-partitioned GROUP BY for high-cardinality keys.  Pass 1 streams the
+// rows once: keys found in (or admitted to) a per-workgroup LDS cache are folded
+// there; every other row becomes a record {key, inputs[, validity]} appended to
+// cell (workgroup, partition) from the front; at the end the cache is written
+// as state records {key, S words} into the back of the same cells.  Pass 2
+// aggregates one partition per workgroup entirely in an LDS table and emits its
+// groups; NULL-key partial states are merged by a last one-workgroup kernel.
+struct GbPartArgs {
+    GbArgs g;                 // columns, predicate, state layout, key
+    int log_p;                // P = 1 << log_p partitions
+    int H;                    // pass-1 LDS cache slots (power of two)
+    int probe_limit;
+    int G;                    // pass-1 workgroups
+    int64_t rows_per_wg;
+    int Cw;                   // record slots per cell
+    int RW;                   // words per record slot
+    int has_valid_word;       // records carry the inputs' validity bits
+    int SR;                   // record slots per state record
+    int64_t *cells;           // G * P * Cw * RW words
+    unsigned int *cell_cnt;   // G * P * 2: row records, state records
+    uint64_t *nullbuf;        // G * S
+    unsigned long long *flags;  // [0] cell full, [1] pass-2 table full, [2] output full
+    int T2;                   // pass-2 LDS table slots (power of two)
+    DOuts outs;
+    int64_t out_cap;
+    int64_t *out_key;
+    uint8_t *out_key_valid;
+    int64_t *out_val[FGPU_MAX_AGGS];
+    uint8_t *out_valid[FGPU_MAX_AGGS];
+    unsigned long long *out_count;
+};
+hipError_t launch_gbp_scan(hipStream_t s, const GbPartArgs &a, int block, size_t lds);
+hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t lds);
+
 hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint64_t *state,
                              int64_t n_groups, const DOps &ops);
 hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,

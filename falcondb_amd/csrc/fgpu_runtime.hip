@@ -107,7 +107,7 @@ struct fgpu_result {
 // ─── context ────────────────────────────────────────────────────────────────
 enum BufSlot {
     B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT,
-    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NSLOTS
+    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_NSLOTS
 };
 
 struct Tuning {
@@ -122,6 +122,15 @@ struct Tuning {
     int keep_on_device = 0;  // 1: results stay in device scratch (only the row count is copied)
     int join_partition = 1;        // radix-partitioned join for large probes
     int join_part_min_rows = 1 << 21;
+    int gb_partition = 1;          // radix-partitioned GROUP BY for high-cardinality keys
+    int gb_part_min_rows = 1 << 20;
+    int gbp_cache_slots = 1024;    // pass-1 LDS cache slots
+    int gbp_probe_limit = 8;
+    int gbp_table_slots = 2048;    // pass-2 LDS table slots
+    int gbp_scan_block = 1024;
+    int gbp_agg_block = 512;
+    int gbp_scan_wgs_per_cu = 1;
+    int lds_limit = 160 * 1024;    // LDS bytes one workgroup may use
     int join_partitions = 32;      // radix partitions of the build / probe sides
     int join_part_wgs_per_cu = 4;  // pass-1 workgroups per CU
     int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
@@ -139,6 +148,7 @@ struct fgpu_ctx {
     hipEvent_t ev[6] = {};
     fgpu_timing last{};
     Tuning tune;
+    std::string plan;  // strategy of the last call (diagnostics)
     std::mutex mu;
 };
 
@@ -581,6 +591,7 @@ int fgpu_open(int device, fgpu_ctx **out) {
     t.filter_waves_per_cu = env_int("FGPU_FILTER_WAVES", t.filter_waves_per_cu);
     t.join_partition = env_int("FGPU_JOIN_PARTITION", t.join_partition);
     t.join_partitions = env_int("FGPU_JOIN_PARTITIONS", t.join_partitions);
+    t.gb_partition = env_int("FGPU_GB_PARTITION", t.gb_partition);
     *out = c;
     return FGPU_OK;
 }
@@ -615,6 +626,9 @@ int64_t fgpu_scratch_bytes(const fgpu_ctx *c) {
     return s;
 }
 
+// The strategy the last call on this context used (diagnostics).
+const char *fgpu_last_plan(const fgpu_ctx *c) { return c ? c->plan.c_str() : ""; }
+
 // Tuning knobs (the autotuner sets these): name = value; unknown name -> error.
 int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     if (!c || !name) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
@@ -630,6 +644,14 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "filter_waves_per_cu") t.filter_waves_per_cu = value;
     else if (n == "keep_on_device") t.keep_on_device = value;
     else if (n == "join_partition") t.join_partition = value;
+    else if (n == "gb_partition") t.gb_partition = value;
+    else if (n == "gb_part_min_rows") t.gb_part_min_rows = value;
+    else if (n == "gbp_cache_slots") t.gbp_cache_slots = value;
+    else if (n == "gbp_probe_limit") t.gbp_probe_limit = value;
+    else if (n == "gbp_table_slots") t.gbp_table_slots = value;
+    else if (n == "gbp_scan_block") t.gbp_scan_block = value;
+    else if (n == "gbp_agg_block") t.gbp_agg_block = value;
+    else if (n == "gbp_scan_wgs_per_cu") t.gbp_scan_wgs_per_cu = value;
     else if (n == "join_part_min_rows") t.join_part_min_rows = value;
     else if (n == "join_partitions") t.join_partitions = value;
     else if (n == "join_part_wgs_per_cu") t.join_part_wgs_per_cu = value;
@@ -640,6 +662,11 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
         return set_err(FGPU_E_INVALID_QUERY, "block sizes must be multiples of 64 in [64, 1024]");
     if (t.hash_lds_slots & (t.hash_lds_slots - 1))
         return set_err(FGPU_E_INVALID_QUERY, "hash_lds_slots must be a power of two");
+    if ((t.gbp_cache_slots & (t.gbp_cache_slots - 1)) || t.gbp_table_slots < 64 ||
+        (t.gbp_table_slots & (t.gbp_table_slots - 1)) || t.gbp_scan_block % 64 || t.gbp_agg_block % 64 ||
+        t.gbp_scan_block < 64 || t.gbp_scan_block > 1024 || t.gbp_agg_block < 64 ||
+        t.gbp_agg_block > 1024 || t.gbp_scan_wgs_per_cu < 1 || t.gbp_probe_limit < 1)
+        return set_err(FGPU_E_INVALID_QUERY, "partitioned GROUP BY knobs out of range");
     if (t.join_partitions < 8 || t.join_partitions > 1024 || (t.join_partitions & (t.join_partitions - 1)))
         return set_err(FGPU_E_INVALID_QUERY, "join_partitions must be a power of two in [8, 1024]");
     if (t.join_part_wgs_per_cu < 1 || t.join_probe_wgs_per_cu < 1)
@@ -911,6 +938,96 @@ int fgpu_filter_select(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, con
 }
 
 // ─── GROUP BY ───────────────────────────────────────────────────────────────
+// Radix-partitioned GROUP BY (GbPartArgs).  Returns FGPU_OK with *out set, a
+// negative value when the query should take the single-table hash path
+// instead (inputs unsuitable, or a cell / table / output bound was exceeded),
+// or an error code.
+static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *key, const GbArgs &a,
+                               const AggLayout &L, const Binder &b, fgpu_result **out) {
+    const Tuning &tu = c->tune;
+    if (!tu.gb_partition || t->n_rows < tu.gb_part_min_rows) return -1;
+    const int S = L.ops.S;
+    GbPartArgs A{};
+    A.g = a;
+    A.outs = L.outs;
+    bool nullable_inputs = false;
+    for (int j = 0; j < a.n_in; ++j)
+        for (size_t col = 0; col < t->cols.size(); ++col)
+            if (t->cols[col].d_values == b.dc.v[a.in_col[j]] && t->cols[col].null_count) nullable_inputs = true;
+    A.has_valid_word = nullable_inputs ? 1 : 0;
+    A.RW = 1 + a.n_in + A.has_valid_word;
+    A.SR = (1 + S + A.RW - 1) / A.RW;
+    // pass-2 table: as many slots as the LDS allows, at most the tuned count
+    int T2 = tu.gbp_table_slots;
+    while (T2 > 64 && (size_t)T2 * (8 + 8 * (size_t)S) > (size_t)tu.lds_limit) T2 >>= 1;
+    A.T2 = T2;
+    const double per_part = std::max(16.0, T2 * 0.45);
+    int64_t P = next_pow2(std::max<int64_t>(8, (int64_t)std::ceil((double)std::max<int64_t>(key->distinct, 1) / per_part)));
+    if (P > 16384) return -1;
+    A.log_p = 0;
+    while ((int64_t(1) << A.log_p) < P) ++A.log_p;
+    int H = tu.gbp_cache_slots;
+    auto lds1 = [&](int h) { return (size_t)(h + (size_t)h * S + S) * 8 + (size_t)P * 8 + 2048 * 4; };
+    while (H > 0 && lds1(H) > (size_t)tu.lds_limit) H >>= 1;
+    if (lds1(H) > (size_t)tu.lds_limit) return -1;
+    A.H = H;
+    A.probe_limit = tu.gbp_probe_limit;
+    A.G = c->num_cus * tu.gbp_scan_wgs_per_cu;
+    A.rows_per_wg = (t->n_rows + A.G - 1) / A.G;
+    A.Cw = (int)std::ceil((double)A.rows_per_wg / (double)P * 1.25) +
+           A.SR * ((int)((4 * (int64_t)H + P - 1) / P) + 4) + 32;
+    const int64_t out_cap = std::min<int64_t>(t->n_rows + 1, std::max<int64_t>(65536, 2 * key->distinct + 1024));
+    A.out_cap = out_cap;
+    int rc;
+    void *p;
+    if ((rc = buf(c, B_CELLS, (size_t)A.G * P * A.Cw * A.RW * 8, &p))) return rc;
+    A.cells = (int64_t *)p;
+    if ((rc = buf(c, B_CELLCNT, (size_t)A.G * P * 8, &p))) return rc;
+    A.cell_cnt = (unsigned int *)p;
+    if ((rc = buf(c, B_NULLBUF, (size_t)A.G * S * 8, &p))) return rc;
+    A.nullbuf = (uint64_t *)p;
+    unsigned long long *d_flags;
+    if ((rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_flags))) return rc;
+    A.flags = d_flags;
+    OutBufs ob;
+    if ((rc = out_bufs(c, out_cap, L.outs.n, ob))) return rc;
+    A.out_key = ob.key;
+    A.out_key_valid = ob.key_valid;
+    for (int i = 0; i < FGPU_MAX_AGGS; ++i) {
+        A.out_val[i] = ob.val[i];
+        A.out_valid[i] = ob.valid[i];
+    }
+    A.out_count = ob.count;
+    mark(c, 0);
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_flags, 4, 0));
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0));
+    mark(c, 1);
+    HIPCHK(launch_gbp_scan(c->stream, A, tu.gbp_scan_block, lds1(H)));
+    mark(c, 2);
+    HIPCHK(launch_gbp_agg(c->stream, A, tu.gbp_agg_block, (size_t)T2 * (8 + 8 * (size_t)S)));
+    mark(c, 3);
+    unsigned long long h[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, d_flags, 24, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    char msg[256];
+    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d Cw=%d RW=%d", (long long)P, H, T2,
+             A.G, A.Cw, A.RW);
+    c->plan = msg;
+    if (h[0] || h[1] || h[2]) {  // a bound was exceeded: single-table hash path
+        c->plan += h[0] ? " -> fallback: cell full" : h[1] ? " -> fallback: pass-2 table full"
+                                                       : " -> fallback: output full";
+        return -1;
+    }
+    if ((rc = collect(c, L, out_cap, key, key->dtype, key->name, ob.key, ob.key_valid, ob.val, ob.valid,
+                      ob.count, out)))
+        return rc;
+    int64_t bytes = 0;
+    for (int s2 = 0; s2 < b.dc.n; ++s2)
+        bytes += t->n_rows * (b.dc.dtype[s2] == FGPU_INT64 || b.dc.dtype[s2] == FGPU_FLOAT64 ? 8 : 4);
+    finish_timing(c, bytes, false);
+    return FGPU_OK;
+}
+
 int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
                  const fgpu_groupby_spec *spec, fgpu_result **out) {
     if (!c || !t || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
@@ -989,6 +1106,16 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
             mode = GB_HASH;
             n_state_groups = 0;
         }
+    }
+    c->plan.clear();
+    if (mode == GB_HASH && !a.key_is_utf8) {
+        const int prc = groupby_partitioned(c, t, key, a, L, b, out);
+        if (prc >= 0) return prc;
+    }
+    {
+        static const char *names[] = {"dense_lds", "dense_global", "hash", "scalar"};
+        if (!c->plan.empty()) c->plan += "; ";
+        c->plan += std::string("groupby:") + names[mode];
     }
     int64_t cap = 0;
     if (mode == GB_HASH)
@@ -1332,8 +1459,10 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     if (rc) return rc;
     a.state = d_state;
     hipError_t e = hipSuccess;
+    c->plan.clear();
     const int prc = join_agg_partitioned(c, probe, build, spec, a, L);
     if (prc > 0) return prc;
+    c->plan += prc == 0 ? "join:partitioned" : "join:direct";
     if (prc < 0) {  // direct probe of the open-addressing index
         if (!idx) {
             if (int rc2 = join_build_impl(c, build, spec->build_key, &own)) return rc2;

@@ -98,9 +98,10 @@ FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
     }
 }
 
-// Merges a whole state (src, e.g. an LDS partial) into dst (HBM).
+// Merges a whole state (src, e.g. an LDS partial) into dst (HBM by default).
+template <int SCOPE = DEV_SCOPE>
 FGPU_DEV void merge_state(uint64_t *dst, const uint64_t *src, const DOps &ops) {
-    a_add<DEV_SCOPE>(dst, src[0]);
+    a_add<SCOPE>(dst, src[0]);
     for (int i = 0; i < ops.n; ++i) {
         const DOp o = ops.op[i];
         uint64_t *w = dst + o.word;
@@ -108,17 +109,17 @@ FGPU_DEV void merge_state(uint64_t *dst, const uint64_t *src, const DOps &ops) {
         switch (o.kind) {
             case OP_CNT:
             case OP_ISUM:
-                if (s[0]) a_add<DEV_SCOPE>(w, s[0]);
+                if (s[0]) a_add<SCOPE>(w, s[0]);
                 break;
-            case OP_IMIN: a_smin<DEV_SCOPE>(w, (int64_t)s[0]); break;
-            case OP_IMAX: a_smax<DEV_SCOPE>(w, (int64_t)s[0]); break;
+            case OP_IMIN: a_smin<SCOPE>(w, (int64_t)s[0]); break;
+            case OP_IMAX: a_smax<SCOPE>(w, (int64_t)s[0]); break;
             case OP_FSUM:
                 for (int k = 0; k < o.W; ++k)
-                    if (s[k]) a_add<DEV_SCOPE>(w + k, s[k]);
-                if (o.has_flags && s[o.W]) a_or<DEV_SCOPE>(w + o.W, s[o.W]);
+                    if (s[k]) a_add<SCOPE>(w + k, s[k]);
+                if (o.has_flags && s[o.W]) a_or<SCOPE>(w + o.W, s[o.W]);
                 break;
-            case OP_FMIN: a_umin<DEV_SCOPE>(w, s[0]); break;
-            case OP_FMAX: a_umax<DEV_SCOPE>(w, s[0]); break;
+            case OP_FMIN: a_umin<SCOPE>(w, s[0]); break;
+            case OP_FMAX: a_umax<SCOPE>(w, s[0]); break;
         }
     }
 }
@@ -187,6 +188,45 @@ FGPU_DEV void reg_add(int k, int64_t v, int64_t &acc, double &hi, double &lo) {
             else acc = (int64_t)(e > (uint64_t)acc ? e : (uint64_t)acc);
             break;
         }
+    }
+}
+
+// Converts one group state into its output row o (one value + validity byte
+// per aggregate).
+FGPU_DEV void emit_row(const DOuts &outs, const uint64_t *st, int64_t o, int64_t *const *out_val,
+                       uint8_t *const *out_valid) {
+    for (int k = 0; k < outs.n; ++k) {
+        const DOut d = outs.o[k];
+        const int64_t cnt = (int64_t)st[d.cnt_word];
+        const uint64_t w = st[d.word];
+        int64_t out = 0;
+        uint8_t valid = cnt > 0;
+        switch (d.kind) {
+            case OUT_COUNT_ROWS: out = (int64_t)st[0]; valid = 1; break;
+            case OUT_COUNT: out = cnt; valid = 1; break;
+            case OUT_SUM_I:
+            case OUT_MIN_I:
+            case OUT_MAX_I: out = (int64_t)w; break;
+            case OUT_AVG_I: out = __double_as_longlong(cnt ? (double)(int64_t)w / (double)cnt : 0.0); break;
+            case OUT_SUM_F:
+            case OUT_AVG_F: {
+                double s2 = fx_finish((const int64_t *)(st + d.word), d.W, d.L,
+                                      d.has_flags ? st[d.word + d.W] : 0ull);
+                if (d.kind == OUT_AVG_F) s2 = cnt ? s2 / (double)cnt : 0.0;
+                out = __double_as_longlong(s2);
+                break;
+            }
+            case OUT_MIN_F:
+                out = (int64_t)f64_unorder(w);
+                valid = valid && w != ~0ull;
+                break;
+            case OUT_MAX_F:
+                out = (int64_t)f64_unorder(w);
+                valid = valid && w != 0ull;
+                break;
+        }
+        out_val[k][o] = out;
+        out_valid[k][o] = valid;
     }
 }
 

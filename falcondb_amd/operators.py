@@ -77,6 +77,10 @@ class Context:
     def set_tuning(self, name: str, value: int) -> None:
         _check(_lib().fgpu_set_tuning(self._h, name.encode(), int(value)))
 
+    def last_plan(self) -> str:
+        """The strategy the last call used, e.g. "groupby:partitioned P=2048 ..."."""
+        return _lib().fgpu_last_plan(self._h).decode()
+
     def scratch_bytes(self) -> int:
         return int(_lib().fgpu_scratch_bytes(self._h))
 

@@ -39,3 +39,25 @@ def ctx_direct():
     c.set_tuning("join_partition", 0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx_gbp():
+    """A context that takes the radix-partitioned GROUP BY for every size."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("gb_part_min_rows", 1)
+    c.mode = "partitioned"
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx_gbh():
+    """A context that keeps high-cardinality GROUP BY on the single hash table."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("gb_partition", 0)
+    c.mode = "hash"
+    yield c
+    c.close()

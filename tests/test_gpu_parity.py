@@ -153,13 +153,24 @@ C2_AGGS = [AggExpr(COUNT), AggExpr(SUM, "v1"), AggExpr(MIN, "v1"), AggExpr(MAX, 
            AggExpr(SUM, "v2")]
 
 
+@pytest.fixture(params=["partitioned", "hash"])
+def gctx(request, ctx_gbp, ctx_gbh):
+    return ctx_gbp if request.param == "partitioned" else ctx_gbh
+
+
 @pytest.mark.parametrize("n,nk", [(1000, 100), (300_000, 50_000), (2_000_000, 200_000)])
-def test_groupby_high_cardinality_zipf(ctx, n, nk):
+def test_groupby_high_cardinality_zipf(gctx, n, nk):
+    ctx = gctx
     K = dg.keys(nk, 40)
     cols = dg.c2_table(n, K)
     t = _table(ctx, cols)
     p = col("f") < 40
     g = fa.GroupByExec(["key"], C2_AGGS).execute(ctx, t, p)
+    plan = ctx.last_plan()
+    if ctx.mode == "partitioned":
+        assert plan.startswith("groupby:partitioned") and "fallback" not in plan, plan
+    else:
+        assert plan == "groupby:hash", plan
     o = ora.groupby(cols, p, "key", C2_AGGS)
     assert_batches_equal(g, o.exact, "key", o.reference)
     # reference output typing: SUM/MIN/MAX of Int64 come back as Float64
@@ -178,10 +189,11 @@ def test_groupby_int32_dense_keys(ctx, card):
     assert_batches_equal(g, o.exact, "key", o.reference)
 
 
-def test_groupby_nulls_and_float_minmax(ctx):
+def test_groupby_nulls_and_float_minmax(gctx):
+    ctx = gctx
     n = 50_000
     rng = np.random.default_rng(11)
-    key = rng.integers(-50, 50, n)
+    key = rng.integers(-50, 50, n) * 1_000_003  # sparse: hash path, not dense
     kvalid = rng.random(n) > 0.1
     key[::97] = np.iinfo(np.int64).min  # folds into the NULL group like the reference
     x = rng.normal(size=n) * 1e3
@@ -343,3 +355,34 @@ def test_join_partitioned_matches_direct_at_scale(ctx_part, ctx_direct):
             pt, bt = _table(c, pcols), _table(c, bcols)
             res.append(fa.execute_hash_join(c, pt, bt, "pk", "bk", jt, aggs, filter=col("f") >= 25))
         assert_batches_equal(res[0], res[1], None)
+
+
+def test_groupby_partitioned_overflow_falls_back(ctx_gbp):
+    # A pass-2 table far too small for the partition's keys: the bound is
+    # detected, and the query is answered by the single-table hash path.
+    K = dg.keys(50_000, 40)
+    cols = dg.c2_table(400_000, K)
+    c = fa.Context(0)
+    c.set_tuning("gb_part_min_rows", 1)
+    c.set_tuning("gbp_table_slots", 64)
+    t = _table(c, cols)
+    g = fa.GroupByExec(["key"], C2_AGGS).execute(c, t, col("f") < 40)
+    assert "fallback" in c.last_plan(), c.last_plan()
+    o = ora.groupby(cols, col("f") < 40, "key", C2_AGGS)
+    assert_batches_equal(g, o.exact, "key")
+    c.close()
+
+
+def test_groupby_partitioned_at_scale_matches_hash(ctx_gbp, ctx_gbh):
+    K = dg.keys(1_000_000, 40)
+    cols = dg.c2_table(20_000_000, K)
+    res, plans = [], []
+    for c in (ctx_gbp, ctx_gbh):
+        t = _table(c, cols)
+        res.append(fa.GroupByExec(["key"], C2_AGGS).execute(c, t, col("f") < 40))
+        plans.append(c.last_plan())
+        t.free()
+    assert "partitioned" in plans[0] and "fallback" not in plans[0], plans[0]
+    assert plans[1] == "groupby:hash", plans[1]
+    assert res[0].num_rows > 500_000
+    assert_batches_equal(res[0], res[1], "key")

@@ -8,14 +8,16 @@
 
 namespace fgpu {
 
-constexpr int GBP_UNROLL = 4;
-
 FGPU_DEV int gbp_part(uint64_t h, int log_p) { return log_p ? (int)(h >> (64 - log_p)) : 0; }
 
-// Find-or-insert in an LDS table with linear probing (at most `limit` probes);
-// slot index from hash bits 40.. so it is independent of the partition bits.
-FGPU_DEV int gbp_slot(int64_t *keys, int mask, int limit, int64_t key, uint64_t h) {
-    int i = (int)(h >> 40) & mask;
+// Index in [0, n) from 32 hash bits (multiply-high: n need not be a power of two).
+FGPU_DEV int gbp_range(uint32_t bits, int n) { return (int)(((uint64_t)bits * (uint32_t)n) >> 32); }
+
+// Find-or-insert in an LDS table of n slots with linear probing (at most
+// `limit` probes); the start slot comes from hash bits 0..31, independent of
+// the partition bits at the top.
+FGPU_DEV int gbp_slot(int64_t *keys, int n, int limit, int64_t key, uint64_t h) {
+    int i = gbp_range((uint32_t)h, n);
     for (int p = 0; p < limit; ++p) {
         const int64_t k = __hip_atomic_load(keys + i, __ATOMIC_RELAXED, LDS_SCOPE);
         if (k == key) return i;
@@ -26,214 +28,595 @@ FGPU_DEV int gbp_slot(int64_t *keys, int mask, int limit, int64_t key, uint64_t 
                 return i;
             if (expect == key) return i;
         }
-        i = (i + 1) & mask;
+        i = i + 1 == n ? 0 : i + 1;
     }
     return -1;
 }
 
-// Lookup only (no insertion): slot of `key` within `limit` probes, or -1.
-FGPU_DEV int gbp_find(const int64_t *keys, int mask, int limit, int64_t key, uint64_t h) {
-    int i = (int)(h >> 40) & mask;
-    for (int p = 0; p < limit; ++p) {
-        const int64_t k = __hip_atomic_load(keys + i, __ATOMIC_RELAXED, LDS_SCOPE);
-        if (k == key) return i;
-        if (k == EMPTY_KEY) return -1;
-        i = (i + 1) & mask;
+// ─── pass 1 ─────────────────────────────────────────────────────────────────
+// Each wave streams tiles of 64*U rows through a three-stage pipeline:
+// predicate columns of tile t+2 and the key/inputs of tile t+1 (loaded only
+// for rows that passed) are in flight while tile t is aggregated.  Passing
+// rows are compacted across tiles into dense 64-lane chunks with ds_permute,
+// so the hash / LDS work runs on full waves whatever the selectivity.
+//
+// The hot-key cache is 8-way set associative: one bucket (64 bytes) is read
+// per lookup.  Ways fill in order and are never emptied, so a key occupies at
+// most one way.  A key enters the cache on its second sighting (doorkeeper
+// bitmap), which keeps one-off keys from taking the ways of heavy hitters.
+
+constexpr int GBP_DOOR_WORDS = 2048;  // doorkeeper bitmap: 65,536 bits of LDS (hash bits 0..15)
+constexpr int GBP_WAYS = 8;
+constexpr uint32_t GBP_HAS_KEY = 1u << 30;  // record validity word: key is not NULL
+typedef long long gbp_ll2 __attribute__((ext_vector_type(2)));
+
+// Bit u of the result = operand rows u of the tile satisfy comparison leaf nd.
+template <int U, int NPC>
+FGPU_DEV uint32_t leaf_mask(const DPredNode nd, const int64_t (&v)[U][NPC], const uint32_t (&ok)[U]) {
+    uint32_t m = 0;
+    const int pc = nd.pc;
+    switch (nd.cls) {
+        case CMP_IS_NULL:
+#pragma unroll
+            for (int u = 0; u < U; ++u) m |= ((ok[u] >> pc) & 1u) ? 0u : 1u << u;
+            return m;
+        case CMP_IS_NOT_NULL:
+#pragma unroll
+            for (int u = 0; u < U; ++u) m |= ((ok[u] >> pc) & 1u) << u;
+            return m;
+        case CMP_FALSE: return 0;
+        case CMP_DICT:
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t code = (uint32_t)pickn<NPC>(v[u], pc);
+                m |= (code != NULL_CODE && nd.pass[code]) ? 1u << u : 0u;
+            }
+            return m;
+        default: break;
     }
-    return -1;
-}
-
-constexpr int GBP_DOOR_WORDS = 2048;  // doorkeeper bitmap: 65,536 bits of LDS
-
-FGPU_DEV Inputs gbp_inputs(const GbArgs &a, int64_t r) {
-    Inputs in{0, 0, 0, 0, 0, 0, 0};
-    auto one = [&](int j, int64_t &dst) {
-        if (j < a.n_in) {
-            bool ok;
-            load_cell(a.cols, a.in_col[j], r, dst, ok);
-            if (ok) in.valid |= 1u << j;
+    // (lt, eq, gt) selected by the operator; NE is the complement of EQ, which
+    // keeps NaN semantics (NaN != x holds, every other comparison fails).
+    const int op = nd.op;
+    const bool s_lt = op == FGPU_LT || op == FGPU_LE, s_gt = op == FGPU_GT || op == FGPU_GE;
+    const bool s_eq = op == FGPU_EQ || op == FGPU_NE || op == FGPU_LE || op == FGPU_GE;
+    const bool inv = op == FGPU_NE;
+    if (nd.cls == CMP_F64 || nd.cls == CMP_I64_AS_F64) {
+        const bool bits = nd.cls == CMP_F64;
+        const double lf = nd.lit_f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t x = pickn<NPC>(v[u], pc);
+            const double d = bits ? __builtin_bit_cast(double, x) : (double)x;
+            const bool r = ((s_lt && d < lf) || (s_eq && d == lf) || (s_gt && d > lf)) != inv;
+            m |= (((ok[u] >> pc) & 1u) && r) ? 1u << u : 0u;
         }
-    };
-    one(0, in.v0);
-    one(1, in.v1);
-    one(2, in.v2);
-    one(3, in.v3);
-    one(4, in.v4);
-    one(5, in.v5);
+    } else {  // CMP_I64, CMP_I32 (the literal narrowed with `as i32`)
+        const int64_t li = nd.cls == CMP_I32 ? (int64_t)(int32_t)nd.lit_i : nd.lit_i;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t x = pickn<NPC>(v[u], pc);
+            const bool r = ((s_lt && x < li) || (s_eq && x == li) || (s_gt && x > li)) != inv;
+            m |= (((ok[u] >> pc) & 1u) && r) ? 1u << u : 0u;
+        }
+    }
+    return m;
+}
+
+// Bit u = row u of the tile satisfies the postfix predicate.  Each node is
+// decoded once per tile.  The stack holds U-bit masks, 8 bits per entry: a
+// postfix of at most 16 binary nodes has at most 8 leaves, so depth <= 8.
+template <int U, int NPC>
+FGPU_DEV uint32_t pred_mask_tile(const DPred &p, const int64_t (&v)[U][NPC], const uint32_t (&ok)[U]) {
+    static_assert(U <= 8, "masks are 8 bits wide");
+    if (p.n == 0) return (1u << U) - 1u;
+    if (p.n == 1) return leaf_mask<U, NPC>(p.nodes[0], v, ok);
+    uint64_t st = 0;
+    for (int i = 0; i < p.n; ++i) {
+        const DPredNode nd = p.nodes[i];
+        uint32_t m;
+        if (nd.kind == FGPU_P_AND || nd.kind == FGPU_P_OR) {
+            const uint32_t b = (uint32_t)st & 0xffu;
+            const uint32_t a = (uint32_t)(st >> 8) & 0xffu;
+            st >>= 16;
+            m = nd.kind == FGPU_P_AND ? (a & b) : (a | b);
+        } else {
+            m = leaf_mask<U, NPC>(nd, v, ok);
+        }
+        st = (st << 8) | m;
+    }
+    return (uint32_t)st & 0xffu;
+}
+
+// One lane's row as it travels through the compaction.
+template <int NIN>
+struct GbpRow {
+    int64_t key;
+    int64_t in[NIN > 0 ? NIN : 1];
+    uint32_t vw;  // bit j: input j not NULL; GBP_HAS_KEY: key not NULL
+};
+
+// Columns and parameters of one scan, resolved into registers at kernel start
+// so that the tile loop issues no scalar loads (their waits would also wait
+// for the LDS atomics in flight).
+template <int NPC, int NIN>
+struct GbpScan {
+    ColRef pc[NPC];
+    int n_pcols;
+    DPredNode leaf0;  // the predicate's only node when it has one
+    ColRef key;
+    ColRef in[NIN > 0 ? NIN : 1];
+    OpList ops;
+    int S, H, NB, log_p, Cw, RW, has_valid_word;
+    unsigned long long *flags;
+    int64_t *my;  // this workgroup's cells
+};
+
+template <int U, int NPC>
+struct GbpPredRaw {
+    RawCell c[U][NPC];
+};
+
+template <int U, int NIN>
+struct GbpDataRaw {
+    RawCell key[U];
+    RawCell in[U][NIN > 0 ? NIN : 1];
+    uint32_t pass;
+};
+
+template <int U, int NIN>
+struct GbpDataTile {
+    GbpRow<NIN> r[U];
+    uint32_t pass;  // bit u: row u passed
+};
+
+FGPU_DEV int64_t perm64(int addr, int64_t v) {
+    const int lo = __builtin_amdgcn_ds_permute(addr, (int)(uint32_t)(uint64_t)v);
+    const int hi = __builtin_amdgcn_ds_permute(addr, (int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+FGPU_DEV int64_t gbp_pred_row(int64_t base, int u, int64_t r1) {
+    const int64_t r = base + u * 64 + (threadIdx.x & 63);
+    return r < r1 ? r : r1 - 1;
+}
+
+template <int U, int NPC, int NIN>
+FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1, GbpPredRaw<U, NPC> &t) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t r = gbp_pred_row(base, u, r1);
+#pragma unroll
+        for (int j = 0; j < NPC; ++j)
+            if (j < g.n_pcols) cell_issue(g.pc[j], r, t.c[u][j]);
+    }
+}
+
+// Bit u = row u of the tile exists and satisfies the predicate.
+template <int U, int NPC, int NIN>
+FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &a, const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1,
+                                  const GbpPredRaw<U, NPC> &t) {
+    int64_t v[U][NPC];
+    uint32_t ok[U];
+    uint32_t exists = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t r = gbp_pred_row(base, u, r1);
+        exists |= base + u * 64 + (threadIdx.x & 63) < r1 ? 1u << u : 0u;
+        ok[u] = 0;
+#pragma unroll
+        for (int j = 0; j < NPC; ++j) {
+            v[u][j] = 0;
+            if (j < g.n_pcols) {
+                bool okj;
+                cell_finish(g.pc[j], r, t.c[u][j], v[u][j], okj);
+                ok[u] |= (okj ? 1u : 0u) << j;
+            }
+        }
+    }
+    const uint32_t m = a.pred.n == 1 ? leaf_mask<U, NPC>(g.leaf0, v, ok) : pred_mask_tile<U, NPC>(a.pred, v, ok);
+    return m & exists;
+}
+
+// Key and inputs of the rows in `pass`; a lane whose row did not pass reads
+// the tile's first row instead, so lines holding no passing row stay unread.
+template <int U, int NPC, int NIN>
+FGPU_DEV int64_t gbp_data_row(int64_t base, int u, uint32_t pass) {
+    return ((pass >> u) & 1u) ? base + u * 64 + (threadIdx.x & 63) : base;
+}
+
+template <int U, int NPC, int NIN>
+FGPU_DEV void gbp_issue_data(const GbpScan<NPC, NIN> &g, int64_t base, uint32_t pass, GbpDataRaw<U, NIN> &t) {
+    t.pass = pass;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t r = gbp_data_row<U, NPC, NIN>(base, u, pass);
+        cell_issue(g.key, r, t.key[u]);
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) cell_issue(g.in[j], r, t.in[u][j]);
+    }
+}
+
+template <int U, int NPC, int NIN>
+FGPU_DEV void gbp_finish_data(const GbpScan<NPC, NIN> &g, int64_t base, const GbpDataRaw<U, NIN> &t,
+                              GbpDataTile<U, NIN> &d) {
+    d.pass = t.pass;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t r = gbp_data_row<U, NPC, NIN>(base, u, t.pass);
+        bool ok;
+        cell_finish(g.key, r, t.key[u], d.r[u].key, ok);
+        d.r[u].vw = (ok && d.r[u].key != EMPTY_KEY) ? GBP_HAS_KEY : 0u;
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) {
+            cell_finish(g.in[j], r, t.in[u][j], d.r[u].in[j], ok);
+            d.r[u].vw |= (ok ? 1u : 0u) << j;
+        }
+    }
+}
+
+template <int NIN>
+FGPU_DEV Inputs gbp_inputs(const GbpRow<NIN> &r) {
+    Inputs in{0, 0, 0, 0, 0, 0, r.vw & 0x3fu};
+    if (NIN > 0) in.v0 = r.in[0];
+    if (NIN > 1) in.v1 = r.in[NIN > 1 ? 1 : 0];
+    if (NIN > 2) in.v2 = r.in[NIN > 2 ? 2 : 0];
+    if (NIN > 3) in.v3 = r.in[NIN > 3 ? 3 : 0];
+    if (NIN > 4) in.v4 = r.in[NIN > 4 ? 4 : 0];
+    if (NIN > 5) in.v5 = r.in[NIN > 5 ? 5 : 0];
     return in;
 }
 
-FGPU_DEV bool gbp_key(const GbArgs &a, int64_t r, int64_t &key) {
-    bool ok;
-    load_cell(a.cols, a.key_col, r, key, ok);
-    return ok && key != EMPTY_KEY;  // NULL and the sentinel value form the NULL group
+struct GbpLds {
+    int64_t *keys;
+    uint64_t *state, *null_state;
+    unsigned int *front, *back, *door;
+};
+
+// Aggregates one compacted row (lanes with active == false do nothing).
+template <int NPC, int NIN>
+FGPU_DEV void gbp_row(const GbpScan<NPC, NIN> &g, const GbpLds &L, bool active, const GbpRow<NIN> &row) {
+    const Inputs in = gbp_inputs<NIN>(row);
+    if (!active) return;
+    const int64_t key = row.key;
+    const uint64_t h = hash64((uint64_t)key);
+    uint64_t *st = nullptr;  // the NULL group's or a cached key's state
+    if (!(row.vw & GBP_HAS_KEY)) {
+        st = L.null_state;
+    } else if (g.H) {
+        const int b = gbp_range((uint32_t)(h >> 16), g.NB);  // bits 16..47
+        int64_t *bk = L.keys + b * GBP_WAYS;
+        int64_t k[GBP_WAYS];
+#pragma unroll
+        for (int w = 0; w < GBP_WAYS; w += 2) {
+            const gbp_ll2 kk = *(const gbp_ll2 *)(bk + w);
+            k[w] = kk.x;
+            k[w + 1] = kk.y;
+        }
+        int way = -1, empty = -1;
+#pragma unroll
+        for (int w = GBP_WAYS - 1; w >= 0; --w) {
+            way = k[w] == key ? w : way;
+            empty = k[w] == EMPTY_KEY ? w : empty;
+        }
+        if (way < 0 && empty >= 0) {
+            const unsigned bit = 1u << (h & 31);
+            const unsigned seen = atomicOr(L.door + ((h >> 5) & (GBP_DOOR_WORDS - 1)), bit) & bit;
+            // Insert at the first empty way; a way lost to another key sends the
+            // insert to the next one, so ways still fill strictly in order.
+            for (int w = empty; seen && w < GBP_WAYS; ++w) {
+                int64_t expect = EMPTY_KEY;
+                if (__hip_atomic_compare_exchange_strong(bk + w, &expect, key, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, LDS_SCOPE) ||
+                    expect == key) {
+                    way = w;
+                    break;
+                }
+            }
+        }
+        if (way >= 0) st = L.state + (int64_t)(b * GBP_WAYS + way) * g.S;
+    }
+    if (st) {
+        update_row<LDS_SCOPE>(st, g.ops, in);
+        return;
+    }
+    const int p = gbp_part(h, g.log_p);
+    const unsigned pos = atomicAdd(L.front + p, 1u);
+    if (pos >= (unsigned)g.Cw) {
+        atomicExch(g.flags, 1ull);
+        return;
+    }
+    int64_t *rec = g.my + ((int64_t)p * g.Cw + pos) * g.RW;
+    rec[0] = key;
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) rec[1 + j] = row.in[j];
+    if (g.has_valid_word) rec[1 + NIN] = (int64_t)(row.vw & 0x3fu);
 }
 
-template <int NPC>
-__global__ void k_gbp_scan(GbPartArgs A) {
+template <int U, int NPC, int NIN>
+__global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     extern __shared__ uint64_t lds[];
     const GbArgs &a = A.g;
-    const int S = a.ops.S, H = A.H, P = 1 << A.log_p;
-    int64_t *lkeys = (int64_t *)lds;
-    uint64_t *lstate = lds + H;
-    uint64_t *lnull = lstate + (int64_t)H * S;
-    unsigned int *front = (unsigned int *)(lnull + S);
-    unsigned int *back = front + P;
-    unsigned int *door = back + P;  // a key enters the cache on its second sighting
-    for (int i = threadIdx.x; i < H; i += blockDim.x) lkeys[i] = EMPTY_KEY;
-    for (int i = threadIdx.x; i < H * S + S; i += blockDim.x) lstate[i] = a.ops.ident[i % S];
-    for (int i = threadIdx.x; i < 2 * P + GBP_DOOR_WORDS; i += blockDim.x) front[i] = 0;
+    GbpScan<NPC, NIN> g;
+#pragma unroll
+    for (int j = 0; j < NPC; ++j) g.pc[j] = col_ref(a.cols, j < a.pred.n_pcols ? a.pred.pcol[j] : 0);
+    g.n_pcols = a.pred.n_pcols;
+    g.leaf0 = a.pred.nodes[0];
+    g.key = col_ref(a.cols, a.key_col);
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) g.in[j] = col_ref(a.cols, a.in_col[j]);
+    g.ops = op_list(a.ops);
+    g.S = a.ops.S;
+    g.H = A.H;
+    g.NB = A.H / GBP_WAYS;
+    g.log_p = A.log_p;
+    g.Cw = A.Cw;
+    g.RW = A.RW;
+    g.has_valid_word = A.has_valid_word;
+    g.flags = A.flags;
+    const int S = g.S, H = g.H, P = 1 << g.log_p;
+    GbpLds L;
+    L.keys = (int64_t *)lds;
+    L.state = lds + H;
+    L.null_state = L.state + (int64_t)H * S;
+    L.front = (unsigned int *)(L.null_state + S);
+    L.back = L.front + P;
+    L.door = L.back + P;
+    for (int i = threadIdx.x; i < H; i += blockDim.x) L.keys[i] = EMPTY_KEY;
+    for (int i = threadIdx.x; i < H * S + S; i += blockDim.x) L.state[i] = a.ops.ident[i % S];
+    for (int i = threadIdx.x; i < 2 * P + GBP_DOOR_WORDS; i += blockDim.x) L.front[i] = 0;
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * A.rows_per_wg;
     const int64_t r1 = r0 + A.rows_per_wg < a.n_rows ? r0 + A.rows_per_wg : a.n_rows;
-    int64_t *my = A.cells + (int64_t)blockIdx.x * P * A.Cw * A.RW;
-    for (int64_t base = r0; base < r1; base += (int64_t)blockDim.x * GBP_UNROLL) {
-        int64_t rows[GBP_UNROLL], key[GBP_UNROLL];
-        bool pass[GBP_UNROLL], has_key[GBP_UNROLL];
-        Inputs in[GBP_UNROLL];
+    g.my = A.cells + (int64_t)blockIdx.x * P * g.Cw * g.RW;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr int TR = 64 * U;  // rows per tile
+    const int64_t n_tiles = r1 > r0 ? (r1 - r0 + TR - 1) / TR : 0;
+
+    GbpRow<NIN> cur, nxt;  // the chunk being filled and its overflow
+    cur.key = nxt.key = 0;
+    cur.vw = nxt.vw = 0;
 #pragma unroll
-        for (int u = 0; u < GBP_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
-        pred_tile<GBP_UNROLL, NPC>(a.pred, a.cols, rows, r1, pass);
-#pragma unroll
-        for (int u = 0; u < GBP_UNROLL; ++u) {
-            key[u] = 0;
-            has_key[u] = false;
-            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
-            if (pass[u]) {
-                has_key[u] = gbp_key(a, rows[u], key[u]);
-                in[u] = gbp_inputs(a, rows[u]);
+    for (int j = 0; j < (NIN > 0 ? NIN : 1); ++j) cur.in[j] = nxt.in[j] = 0;
+    int fill = 0;
+
+    // Pipeline: while tile t is aggregated, the data loads of tile t + nw and
+    // the predicate loads of tile t + 2 nw are in flight.
+    int64_t t = wave;
+    GbpPredRaw<U, NPC> pr;
+    GbpDataRaw<U, NIN> draw;
+    draw.pass = 0;
+    if (t < n_tiles) {
+        gbp_issue_pred<U, NPC, NIN>(g, r0 + t * TR, r1, pr);
+        const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + t * TR, r1, pr);
+        gbp_issue_data<U, NPC, NIN>(g, r0 + t * TR, m, draw);
+        if (t + nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (t + nw) * TR, r1, pr);
+    }
+    // One pass per tile, then one more (have == false) that aggregates the last,
+    // partial chunk; gbp_row has a single call site to keep the loop compact.
+    for (;; t += nw) {
+        const bool have = t < n_tiles;
+        GbpDataTile<U, NIN> dt;
+        dt.pass = 0;
+        if (have) {
+            gbp_finish_data<U, NPC, NIN>(g, r0 + t * TR, draw, dt);
+            if (t + nw < n_tiles) {
+                const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + (t + nw) * TR, r1, pr);
+                gbp_issue_data<U, NPC, NIN>(g, r0 + (t + nw) * TR, m, draw);
+                if (t + 2 * nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (t + 2 * nw) * TR, r1, pr);
             }
         }
+        // compact tile t's passing rows into chunks and aggregate full ones
+#pragma unroll 1
+        for (int u = 0; u < U; ++u) {
+            GbpRow<NIN> src = dt.r[0];
 #pragma unroll
-        for (int u = 0; u < GBP_UNROLL; ++u) {
-            if (!pass[u]) continue;
-            if (!has_key[u]) {
-                update_row<LDS_SCOPE>(lnull, a.ops, in[u]);
-                continue;
+            for (int q = 1; q < U; ++q)
+                if (u == q) src = dt.r[q];
+            const bool pass = (dt.pass >> u) & 1u;
+            const uint64_t bal = __ballot(pass);
+            const int pc = __popcll(bal);
+            if (pc) {
+                const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                const int addr = ((fill + (pass ? rank : pc + (lane - rank))) & 63) << 2;
+                const bool recv = ((lane - fill) & 63) < pc;
+                const bool to_cur = recv && lane >= fill, to_nxt = recv && lane < fill;
+                const int64_t k = perm64(addr, src.key);
+                cur.key = to_cur ? k : cur.key;
+                nxt.key = to_nxt ? k : nxt.key;
+#pragma unroll
+                for (int j = 0; j < NIN; ++j) {
+                    const int64_t x = perm64(addr, src.in[j]);
+                    cur.in[j] = to_cur ? x : cur.in[j];
+                    nxt.in[j] = to_nxt ? x : nxt.in[j];
+                }
+                const uint32_t vw = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)src.vw);
+                cur.vw = to_cur ? vw : cur.vw;
+                nxt.vw = to_nxt ? vw : nxt.vw;
+                fill += pc;
             }
-            const uint64_t h = hash64((uint64_t)key[u]);
-            int li = H ? gbp_find(lkeys, H - 1, A.probe_limit, key[u], h) : -1;
-            if (li < 0 && H) {
-                const unsigned bit = 1u << (h & 31);
-                const unsigned seen = atomicOr(door + ((h >> 5) & (GBP_DOOR_WORDS - 1)), bit) & bit;
-                if (seen) li = gbp_slot(lkeys, H - 1, A.probe_limit, key[u], h);
+            if (fill >= 64 || (!have && fill > 0)) {
+                gbp_row<NPC, NIN>(g, L, lane < fill, cur);
+                if (fill >= 64) {
+                    cur = nxt;
+                    fill -= 64;
+                } else {
+                    fill = 0;
+                }
             }
-            if (li >= 0) {
-                update_row<LDS_SCOPE>(lstate + (int64_t)li * S, a.ops, in[u]);
-                continue;
-            }
-            const int p = gbp_part(h, A.log_p);
-            const unsigned pos = atomicAdd(front + p, 1u);
-            if (pos >= (unsigned)A.Cw) {
-                atomicExch(A.flags, 1ull);
-                continue;
-            }
-            int64_t *rec = my + ((int64_t)p * A.Cw + pos) * A.RW;
-            rec[0] = key[u];
-            const int ni = a.n_in;
-            if (ni > 0) rec[1] = in[u].v0;
-            if (ni > 1) rec[2] = in[u].v1;
-            if (ni > 2) rec[3] = in[u].v2;
-            if (ni > 3) rec[4] = in[u].v3;
-            if (ni > 4) rec[5] = in[u].v4;
-            if (ni > 5) rec[6] = in[u].v5;
-            if (A.has_valid_word) rec[1 + ni] = in[u].valid;
         }
+        if (!have) break;
     }
     __syncthreads();
     // flush the cache: state records fill each cell from its back end
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
-        const int64_t k = lkeys[i];
-        if (k == EMPTY_KEY || lstate[(int64_t)i * S] == 0) continue;
-        const int p = gbp_part(hash64((uint64_t)k), A.log_p);
-        const unsigned b = atomicAdd(back + p, 1u);
-        const int64_t slot = (int64_t)A.Cw - (int64_t)(b + 1) * A.SR;
-        const int64_t used = front[p] < (unsigned)A.Cw ? front[p] : A.Cw;
+        const int64_t k = L.keys[i];
+        if (k == EMPTY_KEY || L.state[(int64_t)i * S] == 0) continue;
+        const int p = gbp_part(hash64((uint64_t)k), g.log_p);
+        const unsigned b = atomicAdd(L.back + p, 1u);
+        const int64_t slot = (int64_t)g.Cw - (int64_t)(b + 1) * A.SR;
+        const int64_t used = L.front[p] < (unsigned)g.Cw ? L.front[p] : g.Cw;
         if (slot < used) {
             atomicExch(A.flags, 1ull);
             continue;
         }
-        int64_t *dst = my + ((int64_t)p * A.Cw + slot) * A.RW;
+        int64_t *dst = g.my + ((int64_t)p * g.Cw + slot) * g.RW;
         dst[0] = k;
-        for (int w = 0; w < S; ++w) dst[1 + w] = (int64_t)lstate[(int64_t)i * S + w];
+        for (int w = 0; w < S; ++w) dst[1 + w] = (int64_t)L.state[(int64_t)i * S + w];
     }
     __syncthreads();
     for (int p = threadIdx.x; p < P; p += blockDim.x) {
-        A.cell_cnt[((int64_t)blockIdx.x * P + p) * 2] = front[p];
-        A.cell_cnt[((int64_t)blockIdx.x * P + p) * 2 + 1] = back[p];
+        A.cell_cnt[((int64_t)blockIdx.x * P + p) * 2] = L.front[p];
+        A.cell_cnt[((int64_t)blockIdx.x * P + p) * 2 + 1] = L.back[p];
     }
-    for (int w = threadIdx.x; w < S; w += blockDim.x) A.nullbuf[(int64_t)blockIdx.x * S + w] = lnull[w];
+    for (int w = threadIdx.x; w < S; w += blockDim.x) A.nullbuf[(int64_t)blockIdx.x * S + w] = L.null_state[w];
 }
 
-// One workgroup per partition: LDS table, every cell of the partition, emit.
-__global__ void k_gbp_agg(GbPartArgs A) {
+// ─── pass 2 ─────────────────────────────────────────────────────────────────
+// One workgroup per partition.  The cell counts of the partition are scanned
+// into LDS prefix arrays first, so every record has a flat index and all
+// record loads are independent (U per thread in flight); the records then
+// aggregate into an LDS table that is emitted at the end.
+constexpr int GBP_AGG_UNROLL = 4;
+
+// Largest g in [0, G) with pre[g] <= i (pre[0] = 0 <= i < pre[G]).
+FGPU_DEV int gbp_cell_of(const unsigned *pre, int G, unsigned i) {
+    int lo = 0, hi = G;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     extern __shared__ uint64_t lds[];
     const GbArgs &a = A.g;
-    const int S = a.ops.S, T2 = A.T2, P = 1 << A.log_p;
+    const int S = a.ops.S, T2 = A.T2, P = 1 << A.log_p, G = A.G;
     int64_t *tkeys = (int64_t *)lds;
     uint64_t *tstate = lds + T2;
+    unsigned *pre = (unsigned *)(tstate + (int64_t)T2 * S);  // G + 1: row records
+    unsigned *spre = pre + G + 1;                             // G + 1: state records
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x & 63;
     for (int i = threadIdx.x; i < T2; i += blockDim.x) tkeys[i] = EMPTY_KEY;
     for (int i = threadIdx.x; i < T2 * S; i += blockDim.x) tstate[i] = a.ops.ident[i % S];
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        const int64_t c = (int64_t)g * P + p;
+        const unsigned nf = A.cell_cnt[2 * c];
+        pre[g] = nf < (unsigned)A.Cw ? nf : (unsigned)A.Cw;
+        spre[g] = A.cell_cnt[2 * c + 1];
+    }
     __syncthreads();
-    const int p = blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int ni = a.n_in;
-    for (int w = wave; w < A.G; w += nw) {
-        const int64_t c = (int64_t)w * P + p;
-        unsigned nf = A.cell_cnt[2 * c];
-        nf = nf < (unsigned)A.Cw ? nf : (unsigned)A.Cw;
-        const unsigned nb = A.cell_cnt[2 * c + 1];
-        const int64_t *cell = A.cells + c * A.Cw * A.RW;
-        for (unsigned i = lane; i < nf; i += 64) {
-            const int64_t *rec = cell + (int64_t)i * A.RW;
-            const int64_t key = rec[0];
-            Inputs in{0, 0, 0, 0, 0, 0, 0};
-            if (ni > 0) in.v0 = rec[1];
-            if (ni > 1) in.v1 = rec[2];
-            if (ni > 2) in.v2 = rec[3];
-            if (ni > 3) in.v3 = rec[4];
-            if (ni > 4) in.v4 = rec[5];
-            if (ni > 5) in.v5 = rec[6];
-            in.valid = A.has_valid_word ? (uint32_t)rec[1 + ni] : 0xffffffffu;
-            const int li = gbp_slot(tkeys, T2 - 1, T2, key, hash64((uint64_t)key));
-            if (li < 0) {
-                atomicExch(A.flags + 1, 1ull);
-                continue;
+    if (threadIdx.x < 64) {  // exclusive scans of both count arrays by one wave
+        const int per = (G + 63) / 64;
+        unsigned s1 = 0, s2 = 0;
+        for (int k = 0; k < per; ++k) {
+            const int g = lane * per + k;
+            if (g < G) {
+                s1 += pre[g];
+                s2 += spre[g];
             }
-            update_row<LDS_SCOPE>(tstate + (int64_t)li * S, a.ops, in);
         }
-        for (unsigned k = lane; k < nb; k += 64) {
-            const int64_t slot = (int64_t)A.Cw - (int64_t)(k + 1) * A.SR;
-            if (slot < (int64_t)nf) continue;  // not written (cell full; flagged in pass 1)
-            const int64_t *rec = cell + slot * A.RW;
-            const int64_t key = rec[0];
-            const int li = gbp_slot(tkeys, T2 - 1, T2, key, hash64((uint64_t)key));
-            if (li < 0) {
-                atomicExch(A.flags + 1, 1ull);
-                continue;
+        unsigned i1 = s1, i2 = s2;
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned t1 = __shfl_up(i1, d, 64), t2 = __shfl_up(i2, d, 64);
+            if (lane >= d) {
+                i1 += t1;
+                i2 += t2;
             }
-            merge_state<LDS_SCOPE>(tstate + (int64_t)li * S, (const uint64_t *)(rec + 1), a.ops);
+        }
+        unsigned e1 = i1 - s1, e2 = i2 - s2;
+        for (int k = 0; k < per; ++k) {
+            const int g = lane * per + k;
+            if (g < G) {
+                const unsigned c1 = pre[g], c2 = spre[g];
+                pre[g] = e1;
+                spre[g] = e2;
+                e1 += c1;
+                e2 += c2;
+            }
+        }
+        if (lane == 63) {
+            pre[G] = i1;
+            spre[G] = i2;
         }
     }
     __syncthreads();
-    for (int b = 0; b < T2; b += blockDim.x) {
-        const int i = b + threadIdx.x;
-        const bool have = i < T2 && tkeys[i] != EMPTY_KEY && tstate[(int64_t)i * S] > 0;
-        const uint64_t ballot = __ballot(have);
-        if (ballot == 0) continue;
-        const int leader = __ffsll((unsigned long long)ballot) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(A.out_count, (unsigned long long)__popcll(ballot));
-        base = __shfl(base, leader);
-        if (!have) continue;
-        const int64_t o = (int64_t)base + __popcll(ballot & ((1ull << lane) - 1));
+    const unsigned n_rows = pre[G], n_states = spre[G];
+    const int ni = a.n_in;
+    const int64_t *part_cells = A.cells + (int64_t)p * A.Cw * A.RW;  // cell (g, p) = part_cells + g * P * Cw * RW
+    const int64_t cell_stride = (int64_t)P * A.Cw * A.RW;
+    for (unsigned i0 = threadIdx.x; i0 < n_rows; i0 += blockDim.x * GBP_AGG_UNROLL) {
+        int64_t key[GBP_AGG_UNROLL];
+        Inputs in[GBP_AGG_UNROLL];
+        bool have[GBP_AGG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GBP_AGG_UNROLL; ++u) {
+            const unsigned i = i0 + u * blockDim.x;
+            have[u] = i < n_rows;
+            key[u] = 0;
+            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+            if (have[u]) {
+                const int g = gbp_cell_of(pre, G, i);
+                const int64_t *rec = part_cells + g * cell_stride + (int64_t)(i - pre[g]) * A.RW;
+                key[u] = rec[0];
+                if (ni > 0) in[u].v0 = rec[1];
+                if (ni > 1) in[u].v1 = rec[2];
+                if (ni > 2) in[u].v2 = rec[3];
+                if (ni > 3) in[u].v3 = rec[4];
+                if (ni > 4) in[u].v4 = rec[5];
+                if (ni > 5) in[u].v5 = rec[6];
+                in[u].valid = A.has_valid_word ? (uint32_t)rec[1 + ni] : 0xffffffffu;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GBP_AGG_UNROLL; ++u) {
+            if (!have[u]) continue;
+            const int li = gbp_slot(tkeys, T2, T2, key[u], hash64((uint64_t)key[u]));
+            if (li < 0) {
+                atomicExch(A.flags + 1, 1ull);
+                continue;
+            }
+            update_row<LDS_SCOPE>(tstate + (int64_t)li * S, a.ops, in[u]);
+        }
+    }
+    for (unsigned i = threadIdx.x; i < n_states; i += blockDim.x) {
+        const int g = gbp_cell_of(spre, G, i);
+        const int64_t slot = (int64_t)A.Cw - (int64_t)(i - spre[g] + 1) * A.SR;
+        const int64_t nf = (int64_t)(pre[g + 1] - pre[g]);
+        if (slot < nf) continue;  // not written (cell full; flagged in pass 1)
+        const int64_t *rec = part_cells + g * cell_stride + slot * A.RW;
+        const int64_t key = rec[0];
+        const int li = gbp_slot(tkeys, T2, T2, key, hash64((uint64_t)key));
+        if (li < 0) {
+            atomicExch(A.flags + 1, 1ull);
+            continue;
+        }
+        merge_state<LDS_SCOPE>(tstate + (int64_t)li * S, (const uint64_t *)(rec + 1), a.ops);
+    }
+    __syncthreads();
+    // emit: one output reservation for the whole partition
+    unsigned have = 0, n_have = 0;
+    for (int i = threadIdx.x, k = 0; i < T2; i += blockDim.x, ++k) {
+        const bool h = tkeys[i] != EMPTY_KEY && tstate[(int64_t)i * S] > 0;
+        have |= h ? 1u << k : 0u;
+        n_have += h ? 1u : 0u;
+    }
+    unsigned total;
+    const unsigned mine = block_excl_scan(n_have, total);
+    int64_t o = (int64_t)block_reserve(A.out_count, total) + mine;
+    for (int i = threadIdx.x, k = 0; i < T2; i += blockDim.x, ++k) {
+        if (!((have >> k) & 1u)) continue;
         if (o >= A.out_cap) {
             atomicExch(A.flags + 2, 1ull);
-            continue;
+            break;
         }
         A.out_key[o] = tkeys[i];
         A.out_key_valid[o] = 1;
         emit_row(A.outs, tstate + (int64_t)i * S, o, A.out_val, A.out_valid);
+        ++o;
     }
 }
 
@@ -258,21 +641,38 @@ __global__ void k_gbp_null(GbPartArgs A) {
     }
 }
 
-hipError_t launch_gbp_scan(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
-    hipError_t e;
-    if (a.g.pred.n_pcols <= 2) {
-        e = hipFuncSetAttribute((const void *)k_gbp_scan<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_gbp_scan<2>, dim3(a.G), dim3(block), lds, s, a);
-    } else {
-        e = hipFuncSetAttribute((const void *)k_gbp_scan<MAX_PCOLS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_gbp_scan<MAX_PCOLS>, dim3(a.G), dim3(block), lds, s, a);
-    }
+template <int NPC, int NIN>
+static hipError_t gbp_scan_go(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
+    constexpr int U = 2;  // 128-row tiles: two in flight per wave within 128 VGPRs
+    const void *fn = (const void *)k_gbp_scan<U, NPC, NIN>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gbp_scan<U, NPC, NIN>), dim3(a.G), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 
+template <int NPC>
+static hipError_t gbp_scan_nin(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
+    switch (a.g.n_in) {
+        case 0: return gbp_scan_go<NPC, 0>(s, a, block, lds);
+        case 1: return gbp_scan_go<NPC, 1>(s, a, block, lds);
+        case 2: return gbp_scan_go<NPC, 2>(s, a, block, lds);
+        case 3: return gbp_scan_go<NPC, 3>(s, a, block, lds);
+        case 4: return gbp_scan_go<NPC, 4>(s, a, block, lds);
+        case 5: return gbp_scan_go<NPC, 5>(s, a, block, lds);
+        default: return gbp_scan_go<NPC, 6>(s, a, block, lds);
+    }
+}
+
+hipError_t launch_gbp_scan(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
+    if (a.H % GBP_WAYS) return hipErrorInvalidValue;
+    if (a.g.pred.n_pcols <= 1) return gbp_scan_nin<1>(s, a, block, lds);
+    if (a.g.pred.n_pcols <= 2) return gbp_scan_nin<2>(s, a, block, lds);
+    return gbp_scan_nin<MAX_PCOLS>(s, a, block, lds);
+}
+
 hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
+    if (a.T2 > 32 * block) return hipErrorInvalidValue;  // the emit mask holds 32 slots per thread
     hipError_t e = hipFuncSetAttribute((const void *)k_gbp_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gbp_agg, dim3(1 << a.log_p), dim3(block), lds, s, a);

@@ -301,38 +301,45 @@ __global__ void k_gb_scalar(GbArgs a) {
 }
 
 // ─── emit: occupied groups -> output columns ────────────────────────────────
+// Each workgroup covers EMIT_PER_THREAD * blockDim.x slots and reserves its
+// output rows with one atomic.
+constexpr int EMIT_PER_THREAD = 16;
+
 __global__ void k_gb_emit(EmitArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool have = false;
-    const uint64_t *st = nullptr;
-    if (i < a.n_slots) {
-        st = a.state + i * a.S;
-        if (a.mode == GB_SCALAR) have = true;
-        else if (a.mode == GB_HASH) have = (i == a.n_slots - 1 || a.keys[i] != EMPTY_KEY) && st[0] > 0;
-        else have = st[0] > 0;
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x * EMIT_PER_THREAD + threadIdx.x;
+    uint32_t have = 0;
+#pragma unroll
+    for (int k = 0; k < EMIT_PER_THREAD; ++k) {
+        const int64_t i = first + (int64_t)k * blockDim.x;
+        if (i >= a.n_slots) break;
+        const uint64_t *st = a.state + i * a.S;
+        bool h;
+        if (a.mode == GB_SCALAR) h = true;
+        else if (a.mode == GB_HASH) h = (i == a.n_slots - 1 || a.keys[i] != EMPTY_KEY) && st[0] > 0;
+        else h = st[0] > 0;
+        have |= h ? 1u << k : 0u;
     }
-    // wave-aggregated output slot
-    const uint64_t ballot = __ballot(have);
-    if (ballot == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)ballot) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(a.out_count, (unsigned long long)__popcll(ballot));
-    base = __shfl(base, leader);
-    if (!have) return;
-    const int64_t o = (int64_t)base + __popcll(ballot & ((1ull << lane) - 1));
-    if (a.out_key) {
-        if (a.mode == GB_HASH) {
-            const bool null_group = i == a.n_slots - 1;
-            a.out_key[o] = null_group ? 0 : a.keys[i];
-            a.out_key_valid[o] = null_group ? 0 : 1;
-        } else {
-            const bool null_group = i == a.n_slots - 1 && !a.key_is_utf8;
-            a.out_key[o] = null_group ? 0 : i + a.key_min;
-            a.out_key_valid[o] = null_group ? 0 : 1;
+    unsigned total;
+    const unsigned mine = block_excl_scan((unsigned)__popc(have), total);
+    int64_t o = (int64_t)block_reserve(a.out_count, total) + mine;
+    for (int k = 0; k < EMIT_PER_THREAD; ++k) {
+        if (!((have >> k) & 1u)) continue;
+        const int64_t i = first + (int64_t)k * blockDim.x;
+        const uint64_t *st = a.state + i * a.S;
+        if (a.out_key) {
+            if (a.mode == GB_HASH) {
+                const bool null_group = i == a.n_slots - 1;
+                a.out_key[o] = null_group ? 0 : a.keys[i];
+                a.out_key_valid[o] = null_group ? 0 : 1;
+            } else {
+                const bool null_group = i == a.n_slots - 1 && !a.key_is_utf8;
+                a.out_key[o] = null_group ? 0 : i + a.key_min;
+                a.out_key_valid[o] = null_group ? 0 : 1;
+            }
         }
+        emit_row(a.outs, st, o, a.out_val, a.out_valid);
+        ++o;
     }
-    emit_row(a.outs, st, o, a.out_val, a.out_valid);
 }
 
 __global__ void k_state_init(int64_t *keys, int64_t n_keys, uint64_t *state, int64_t n_words,
@@ -383,7 +390,8 @@ hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, in
 }
 
 hipError_t launch_emit(hipStream_t s, const EmitArgs &a) {
-    const int grid = (int)std::max<int64_t>(1, (a.n_slots + 255) / 256);
+    const int64_t per_wg = 256 * EMIT_PER_THREAD;
+    const int grid = (int)std::max<int64_t>(1, (a.n_slots + per_wg - 1) / per_wg);
     hipLaunchKernelGGL(k_gb_emit, dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }

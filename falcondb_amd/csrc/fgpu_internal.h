@@ -170,7 +170,7 @@ FGPU_HD void two_sum(double a, double b, double &s, double &err) {
 struct DCols {
     const void *v[MAX_COLS];
     const uint64_t *valid[MAX_COLS];
-    int8_t dtype[MAX_COLS];
+    int32_t dtype[MAX_COLS];
     int n;
 };
 
@@ -205,11 +205,11 @@ enum : int8_t {
 constexpr int MAX_PCOLS = 4;  // distinct columns one predicate may read
 
 struct DPredNode {
-    int8_t kind;   // FGPU_P_*
-    int8_t cls;    // CMP_*
-    int8_t op;     // FGPU_EQ..FGPU_GE
-    int8_t col;    // column slot in DCols
-    int8_t pc;     // index of that column in DPred::pcol (tile evaluation)
+    int32_t kind;  // FGPU_P_*
+    int32_t cls;   // CMP_*
+    int32_t op;    // FGPU_EQ..FGPU_GE
+    int32_t col;   // column slot in DCols
+    int32_t pc;    // index of that column in DPred::pcol (tile evaluation)
     int64_t lit_i;
     double lit_f;
     const uint8_t *pass;  // CMP_DICT
@@ -217,7 +217,7 @@ struct DPredNode {
 struct DPred {
     int n;        // 0 = no predicate (all rows pass)
     int n_pcols;  // distinct columns the predicate reads
-    int8_t pcol[MAX_PCOLS];  // their DCols slots
+    int32_t pcol[MAX_PCOLS];  // their DCols slots
     DPredNode nodes[FGPU_MAX_PRED_NODES];
 };
 
@@ -290,6 +290,32 @@ FGPU_DEV void load_cell(const DCols &c, int k, int64_t r, int64_t &v, bool &ok) 
     }
     const uint64_t *vb = c.valid[k];
     ok = vb ? ((vb[r >> 6] >> (r & 63)) & 1) : (c.dtype[k] != FGPU_UTF8 || (uint32_t)v != NULL_CODE);
+}
+
+// ─── two-phase column reads ─────────────────────────────────────────────────
+// A column resolved once per kernel (uniform: lives in SGPRs).
+struct ColRef {
+    const uint8_t *base;
+    const uint32_t *valid;  // validity bits as 32-bit words, or null
+    int dtype;
+};
+FGPU_DEV ColRef col_ref(const DCols &c, int k) {
+    return ColRef{(const uint8_t *)c.v[k], (const uint32_t *)c.valid[k], c.dtype[k]};
+}
+// The raw loads of one cell.  cell_issue only issues them; nothing reads the
+// results until cell_finish, so a caller can keep many loads in flight.
+struct RawCell {
+    uint64_t bits;
+    uint32_t vword;
+};
+FGPU_DEV void cell_issue(const ColRef &c, int64_t r, RawCell &o) {
+    if (c.dtype == FGPU_INT32 || c.dtype == FGPU_UTF8) o.bits = (uint64_t)*(const uint32_t *)(c.base + r * 4);
+    else o.bits = *(const uint64_t *)(c.base + r * 8);
+    o.vword = c.valid ? c.valid[r >> 5] : 0xffffffffu;
+}
+FGPU_DEV void cell_finish(const ColRef &c, int64_t r, const RawCell &o, int64_t &v, bool &ok) {
+    v = c.dtype == FGPU_INT32 ? (int64_t)(int32_t)(uint32_t)o.bits : (int64_t)o.bits;
+    ok = ((o.vword >> (r & 31)) & 1u) && (c.dtype != FGPU_UTF8 || (uint32_t)o.bits != NULL_CODE);
 }
 
 template <int NPC>
@@ -383,19 +409,41 @@ enum : int8_t {
 };
 
 struct DOp {
-    int8_t kind;
-    int8_t col;     // column slot
-    int8_t W;       // OP_FSUM words
-    int8_t has_flags;  // OP_FSUM: column holds NaN/inf, flags word follows the W words
-    int16_t word;   // first state word
-    int16_t L;      // OP_FSUM fixed-point LSB exponent
+    int32_t kind;
+    int32_t col;    // column slot
+    int32_t W;      // OP_FSUM words
+    int32_t has_flags;  // OP_FSUM: column holds NaN/inf, flags word follows the W words
+    int32_t word;   // first state word
+    int32_t L;      // OP_FSUM fixed-point LSB exponent
 };
 struct DOps {
     int n;
     int S;  // words per group state
     DOp op[MAX_OPS];
+    // op[i] packed into one dword (op_pack): the per-row loops unroll over
+    // these with constant indices, so the compiler keeps them in SGPRs.  A
+    // scalar load inside such a loop would need s_waitcnt lgkmcnt(0), which
+    // also waits for every LDS atomic in flight.
+    uint32_t pk[MAX_OPS];
     uint64_t ident[64];  // identity value of each state word (S <= 64)
 };
+
+// Packed op: kind 0..2, input column 3..5, W 6..9, has_flags 10, word 11..17,
+// L (signed) 18..31.
+FGPU_HD uint32_t op_pack(const DOp &o) {
+    return (uint32_t)(o.kind & 7) | (uint32_t)(o.col & 7) << 3 | (uint32_t)(o.W & 15) << 6 |
+           (uint32_t)(o.has_flags & 1) << 10 | (uint32_t)(o.word & 127) << 11 | (uint32_t)o.L << 18;
+}
+FGPU_HD bool op_packable(const DOp &o) {
+    return o.kind >= 0 && o.kind < 8 && o.col >= 0 && o.col < 8 && o.W >= 0 && o.W < 16 && o.word >= 0 &&
+           o.word < 128 && o.L >= -8192 && o.L < 8192;
+}
+FGPU_DEV int opk_kind(uint32_t d) { return (int)(d & 7u); }
+FGPU_DEV int opk_col(uint32_t d) { return (int)((d >> 3) & 7u); }
+FGPU_DEV int opk_W(uint32_t d) { return (int)((d >> 6) & 15u); }
+FGPU_DEV bool opk_flags(uint32_t d) { return (d >> 10) & 1u; }
+FGPU_DEV int opk_word(uint32_t d) { return (int)((d >> 11) & 127u); }
+FGPU_DEV int opk_L(uint32_t d) { return (int32_t)d >> 18; }
 
 // Flags word bits for non-finite float inputs.
 constexpr uint64_t FL_NAN = 1, FL_PINF = 2, FL_NINF = 4;

@@ -22,13 +22,12 @@ enum : int8_t {
     OUT_MAX_F = 9
 };
 struct DOut {
-    int8_t kind;
-    int8_t W;
-    int8_t has_flags;
-    int8_t pad;
-    int16_t word;      // value word (first fixed-point word for float sums)
-    int16_t cnt_word;  // non-NULL count word (0 = row count)
-    int16_t L;
+    int32_t kind;
+    int32_t W;
+    int32_t has_flags;
+    int32_t word;      // value word (first fixed-point word for float sums)
+    int32_t cnt_word;  // non-NULL count word (0 = row count)
+    int32_t L;
 };
 struct DOuts {
     int n;
@@ -43,7 +42,7 @@ struct GbArgs {
     DPred pred;
     DOps ops;
     int n_in;
-    int8_t in_col[MAX_IN];  // column slot of each aggregate input
+    int32_t in_col[MAX_IN];  // column slot of each aggregate input
     int key_col;            // column slot of the key (-1 = scalar)
     int key_is_utf8;
     uint32_t utf8_null_code;  // code NULL Utf8 keys are mapped to
@@ -157,14 +156,13 @@ struct JoinBuildArgs {
 
 // Per-thread register aggregate ops of a join / scalar aggregate.
 struct JoinOp {
-    int8_t kind;     // OP_CNT .. OP_FMAX (OP_CNT with col < 0 = COUNT(*))
-    int8_t side;     // 0 probe, 1 build
-    int8_t col;      // column slot within that side's DCols
-    int8_t pad;
-    int16_t word;    // destination word in the global state row
-    int16_t L;
-    int8_t W;
-    int8_t has_flags;
+    int32_t kind;    // OP_CNT .. OP_FMAX (OP_CNT with col < 0 = COUNT(*))
+    int32_t side;    // 0 probe, 1 build
+    int32_t col;     // column slot within that side's DCols
+    int32_t word;    // destination word in the global state row
+    int32_t L;
+    int32_t W;
+    int32_t has_flags;
 };
 struct JoinProbeArgs {
     DCols pcols;      // probe columns
@@ -207,8 +205,8 @@ struct JoinPartArgs {
     int Cw;                   // records per cell
     int RW;                   // words per record (key + probe payloads)
     int n_pay;                // probe payload columns carried in records (<= 2)
-    int8_t pay_col[2];        // their pcols slots
-    int8_t field_of_op[FGPU_MAX_AGGS];  // probe-side op -> record field (1..n_pay)
+    int32_t pay_col[2];       // their pcols slots
+    int32_t field_of_op[FGPU_MAX_AGGS];  // probe-side op -> record field (1..n_pay)
     int64_t *cells;           // G * P * Cw * RW words
     unsigned int *cell_cnt;   // G * P
     unsigned long long *flags;  // [0] cell overflow, [1] sub-table overflow, [2] side-list rows,
@@ -227,5 +225,11 @@ hipError_t launch_join_pairs(hipStream_t s, const JoinProbeArgs &a, int pass, in
 
 // Fill helpers
 hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v);
+
+// Kernel arguments travel in the 4 KiB kernarg segment; descriptor fields are
+// 32-bit so that the kernels read them with scalar loads.
+static_assert(sizeof(GbPartArgs) <= 4096, "GbPartArgs exceeds the kernarg segment");
+static_assert(sizeof(JoinProbeArgs) <= 4096, "JoinProbeArgs exceeds the kernarg segment");
+static_assert(sizeof(JoinPartArgs) <= 4096, "JoinPartArgs exceeds the kernarg segment");
 
 }  // namespace fgpu

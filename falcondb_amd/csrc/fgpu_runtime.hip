@@ -124,11 +124,13 @@ struct Tuning {
     int join_part_min_rows = 1 << 21;
     int gb_partition = 1;          // radix-partitioned GROUP BY for high-cardinality keys
     int gb_part_min_rows = 1 << 20;
-    int gbp_cache_slots = 1024;    // pass-1 LDS cache slots
+    int gbp_cache_slots = 8192;    // cap on the pass-1 LDS cache slots (8-way buckets)
     int gbp_probe_limit = 8;
-    int gbp_table_slots = 2048;    // pass-2 LDS table slots
+    int gbp_table_slots = 4096;    // cap on the pass-2 LDS table slots
+    int gbp_partitions = 0;        // radix fan-out; 0 = from the key's distinct estimate
+    int gbp_cells_max_mb = 16384;  // bound on the pass-1 cell buffer
     int gbp_scan_block = 1024;
-    int gbp_agg_block = 512;
+    int gbp_agg_block = 1024;
     int gbp_scan_wgs_per_cu = 1;
     int lds_limit = 160 * 1024;    // LDS bytes one workgroup may use
     int join_partitions = 32;      // radix partitions of the build / probe sides
@@ -357,15 +359,15 @@ int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot
         if (L.keys[i].kind == kind && L.keys[i].side == side && L.keys[i].col == col) return (int)i;
     if (L.ops.n >= MAX_OPS) return set_err(FGPU_E_UNSUPPORTED, "too many aggregate states (max %d)", MAX_OPS) * 0 - 1;
     DOp o{};
-    o.kind = (int8_t)kind;
-    o.col = (int8_t)slot;
-    o.word = (int16_t)L.ops.S;
+    o.kind = kind;
+    o.col = slot;
+    o.word = L.ops.S;
     int nwords = 1;
     if (kind == OP_FSUM) {
         int l, w;
         if (fx_words(dc, l, w)) return -1;
-        o.L = (int16_t)l;
-        o.W = (int8_t)w;
+        o.L = l;
+        o.W = w;
         o.has_flags = dc.nonfinite ? 1 : 0;
         nwords = w + o.has_flags;
     }
@@ -380,7 +382,12 @@ int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot
         else if (kind == OP_FMIN) id = ~0ull;
         L.ops.ident[L.ops.S + k] = id;
     }
+    if (!op_packable(o)) {
+        set_err(FGPU_E_UNSUPPORTED, "aggregate state layout out of range");
+        return -1;
+    }
     L.ops.S += nwords;
+    L.ops.pk[L.ops.n] = op_pack(o);
     L.ops.op[L.ops.n++] = o;
     L.keys.push_back({kind, side, col});
     return L.ops.n - 1;
@@ -649,6 +656,8 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "gbp_cache_slots") t.gbp_cache_slots = value;
     else if (n == "gbp_probe_limit") t.gbp_probe_limit = value;
     else if (n == "gbp_table_slots") t.gbp_table_slots = value;
+    else if (n == "gbp_partitions") t.gbp_partitions = value;
+    else if (n == "gbp_cells_max_mb") t.gbp_cells_max_mb = value;
     else if (n == "gbp_scan_block") t.gbp_scan_block = value;
     else if (n == "gbp_agg_block") t.gbp_agg_block = value;
     else if (n == "gbp_scan_wgs_per_cu") t.gbp_scan_wgs_per_cu = value;
@@ -662,8 +671,9 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
         return set_err(FGPU_E_INVALID_QUERY, "block sizes must be multiples of 64 in [64, 1024]");
     if (t.hash_lds_slots & (t.hash_lds_slots - 1))
         return set_err(FGPU_E_INVALID_QUERY, "hash_lds_slots must be a power of two");
-    if ((t.gbp_cache_slots & (t.gbp_cache_slots - 1)) || t.gbp_table_slots < 64 ||
-        (t.gbp_table_slots & (t.gbp_table_slots - 1)) || t.gbp_scan_block % 64 || t.gbp_agg_block % 64 ||
+    if (t.gbp_cache_slots < 0 || t.gbp_cache_slots % 8 || t.gbp_table_slots < 64 ||
+        t.gbp_partitions < 0 || t.gbp_partitions > 16384 || (t.gbp_partitions & (t.gbp_partitions - 1)) ||
+        t.gbp_scan_block % 64 || t.gbp_agg_block % 64 ||
         t.gbp_scan_block < 64 || t.gbp_scan_block > 1024 || t.gbp_agg_block < 64 ||
         t.gbp_agg_block > 1024 || t.gbp_scan_wgs_per_cu < 1 || t.gbp_probe_limit < 1)
         return set_err(FGPU_E_INVALID_QUERY, "partitioned GROUP BY knobs out of range");
@@ -957,22 +967,31 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     A.has_valid_word = nullable_inputs ? 1 : 0;
     A.RW = 1 + a.n_in + A.has_valid_word;
     A.SR = (1 + S + A.RW - 1) / A.RW;
-    // pass-2 table: as many slots as the LDS allows, at most the tuned count
-    int T2 = tu.gbp_table_slots;
-    while (T2 > 64 && (size_t)T2 * (8 + 8 * (size_t)S) > (size_t)tu.lds_limit) T2 >>= 1;
-    A.T2 = T2;
-    const double per_part = std::max(16.0, T2 * 0.45);
-    int64_t P = next_pow2(std::max<int64_t>(8, (int64_t)std::ceil((double)std::max<int64_t>(key->distinct, 1) / per_part)));
+    A.G = c->num_cus * tu.gbp_scan_wgs_per_cu;
+    // Pass-2 table: the most slots (a multiple of 64, at most the tuned cap)
+    // that fit in LDS beside the two cell-count prefix arrays and the emit
+    // scan.  The fan-out keeps a partition's expected keys at <= 0.6 of that,
+    // then the table shrinks to ~2x the expected keys (more workgroups per CU).
+    const int64_t lds_fixed2 = 8 * (int64_t)(A.G + 1) + 256;
+    const int64_t t2max = std::min<int64_t>(tu.gbp_table_slots, ((int64_t)tu.lds_limit - lds_fixed2) / (8 * (S + 1))) / 64 * 64;
+    if (t2max < 64) return -1;
+    const int64_t distinct = std::max<int64_t>(key->distinct, 1);
+    int64_t P = tu.gbp_partitions ? tu.gbp_partitions
+                                  : next_pow2(std::max<int64_t>(8, (int64_t)std::ceil((double)distinct / (0.6 * t2max))));
     if (P > 16384) return -1;
+    const int T2 = (int)std::max<int64_t>(64, std::min<int64_t>(t2max, ((int64_t)std::ceil(2.0 * distinct / P) + 63) / 64 * 64));
+    A.T2 = T2;
+    auto lds2 = [&](int t2) { return (size_t)t2 * (8 + 8 * (size_t)S) + (size_t)lds_fixed2; };
     A.log_p = 0;
     while ((int64_t(1) << A.log_p) < P) ++A.log_p;
-    int H = tu.gbp_cache_slots;
-    auto lds1 = [&](int h) { return (size_t)(h + (size_t)h * S + S) * 8 + (size_t)P * 8 + 2048 * 4; };
-    while (H > 0 && lds1(H) > (size_t)tu.lds_limit) H >>= 1;
+    // Pass-1 cache: the most slots (a multiple of 8, the ways) beside the cell
+    // cursors and the doorkeeper bitmap, at most the tuned cap.
+    auto lds1 = [&](int64_t h) { return (size_t)(h + h * S + S) * 8 + (size_t)P * 8 + 2048 * 4 + 256; };
+    const int64_t hmax = ((int64_t)tu.lds_limit - (int64_t)lds1(0)) / (8 * (S + 1));
+    int H = (int)(std::min<int64_t>(tu.gbp_cache_slots, std::max<int64_t>(hmax, 0)) / 8 * 8);
     if (lds1(H) > (size_t)tu.lds_limit) return -1;
     A.H = H;
     A.probe_limit = tu.gbp_probe_limit;
-    A.G = c->num_cus * tu.gbp_scan_wgs_per_cu;
     A.rows_per_wg = (t->n_rows + A.G - 1) / A.G;
     A.Cw = (int)std::ceil((double)A.rows_per_wg / (double)P * 1.25) +
            A.SR * ((int)((4 * (int64_t)H + P - 1) / P) + 4) + 32;
@@ -980,8 +999,6 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     A.out_cap = out_cap;
     int rc;
     void *p;
-    if ((rc = buf(c, B_CELLS, (size_t)A.G * P * A.Cw * A.RW * 8, &p))) return rc;
-    A.cells = (int64_t *)p;
     if ((rc = buf(c, B_CELLCNT, (size_t)A.G * P * 8, &p))) return rc;
     A.cell_cnt = (unsigned int *)p;
     if ((rc = buf(c, B_NULLBUF, (size_t)A.G * S * 8, &p))) return rc;
@@ -998,21 +1015,58 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
         A.out_valid[i] = ob.valid[i];
     }
     A.out_count = ob.count;
-    mark(c, 0);
-    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_flags, 4, 0));
-    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0));
-    mark(c, 1);
-    HIPCHK(launch_gbp_scan(c->stream, A, tu.gbp_scan_block, lds1(H)));
-    mark(c, 2);
-    HIPCHK(launch_gbp_agg(c->stream, A, tu.gbp_agg_block, (size_t)T2 * (8 + 8 * (size_t)S)));
-    mark(c, 3);
-    unsigned long long h[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(h, d_flags, 24, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
     char msg[256];
-    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d Cw=%d RW=%d", (long long)P, H, T2,
-             A.G, A.Cw, A.RW);
+    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d RW=%d", (long long)P, H, T2, A.G, A.RW);
     c->plan = msg;
+    unsigned long long h[3] = {0, 0, 0};
+    mark(c, 0);
+    // A cell that overflows (a heavy key some workgroup could not cache) is
+    // retried once with cells sized from the counts the first attempt kept.
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if ((size_t)A.G * P * A.Cw * A.RW * 8 > (size_t)tu.gbp_cells_max_mb << 20) {
+            c->plan += " -> fallback: cells over budget";
+            return -1;
+        }
+        if ((rc = buf(c, B_CELLS, (size_t)A.G * P * A.Cw * A.RW * 8, &p))) return rc;
+        A.cells = (int64_t *)p;
+        HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_flags, 4, 0));
+        HIPCHK(launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0));
+        mark(c, 1);
+        HIPCHK(launch_gbp_scan(c->stream, A, tu.gbp_scan_block, lds1(H)));
+        mark(c, 2);
+        HIPCHK(hipMemcpyAsync(h, d_flags, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        snprintf(msg, sizeof msg, " Cw=%d", A.Cw);
+        c->plan += msg;
+        const bool diag = env_int("FGPU_GBP_DIAG", 0) != 0;
+        if (h[0] || diag) {  // cell occupancy: rows that missed the cache, state records
+            std::vector<unsigned int> cc((size_t)A.G * P * 2);
+            HIPCHK(hipMemcpy(cc.data(), A.cell_cnt, cc.size() * 4, hipMemcpyDeviceToHost));
+            uint64_t cold = 0, states = 0, mx = 0, need = 0;
+            for (size_t i = 0; i < cc.size(); i += 2) {
+                cold += cc[i];
+                states += cc[i + 1];
+                mx = std::max<uint64_t>(mx, cc[i]);
+                need = std::max<uint64_t>(need, cc[i] + (uint64_t)cc[i + 1] * A.SR);
+            }
+            if (diag) {
+                snprintf(msg, sizeof msg, " [cold=%llu states=%llu max_cell=%llu]", (unsigned long long)cold,
+                         (unsigned long long)states, (unsigned long long)mx);
+                c->plan += msg;
+            }
+            if (h[0] && attempt == 0) {
+                c->plan += " cell full, resized";
+                A.Cw = (int)std::min<uint64_t>((uint64_t)need * 5 / 4 + 32, (uint64_t)INT32_MAX / 2);
+                continue;
+            }
+        }
+        if (h[0]) break;
+        HIPCHK(launch_gbp_agg(c->stream, A, tu.gbp_agg_block, lds2(T2)));
+        mark(c, 3);
+        HIPCHK(hipMemcpyAsync(h, d_flags, 24, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        break;
+    }
     if (h[0] || h[1] || h[2]) {  // a bound was exceeded: single-table hash path
         c->plan += h[0] ? " -> fallback: cell full" : h[1] ? " -> fallback: pass-2 table full"
                                                        : " -> fallback: output full";

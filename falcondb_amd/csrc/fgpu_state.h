@@ -54,16 +54,33 @@ FGPU_DEV int64_t pick(const Inputs &in, int j) {
     }
 }
 
-// Adds one row (input values iv, validity bits ivalid) into a group state.
+// The packed ops of a kernel, copied into registers once (kernels that run
+// update_row in a loop build it before the loop).
+struct OpList {
+    int n;
+    uint32_t pk[MAX_OPS];
+};
+FGPU_DEV OpList op_list(const DOps &ops) {
+    OpList l;
+    l.n = ops.n;
+#pragma unroll
+    for (int i = 0; i < MAX_OPS; ++i) l.pk[i] = ops.pk[i];
+    return l;
+}
+
+// Adds one row (input values and validity bits in `in`) into a group state.
 template <int SCOPE>
-FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
+FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
     a_add<SCOPE>(st, 1ull);
-    for (int i = 0; i < ops.n; ++i) {
-        const DOp o = ops.op[i];
-        if (!((in.valid >> o.col) & 1)) continue;
-        const int64_t v = pick(in, o.col);
-        uint64_t *w = st + o.word;
-        switch (o.kind) {
+#pragma unroll
+    for (int i = 0; i < MAX_OPS; ++i) {  // no early exit: fully unrolled, pk stays in SGPRs
+        if (i >= ops.n) continue;
+        const uint32_t d = ops.pk[i];
+        const int col = opk_col(d);
+        if (!((in.valid >> col) & 1)) continue;
+        const int64_t v = pick(in, col);
+        uint64_t *w = st + opk_word(d);
+        switch (opk_kind(d)) {
             case OP_CNT: a_add<SCOPE>(w, 1ull); break;
             case OP_ISUM: a_add<SCOPE>(w, (uint64_t)v); break;
             case OP_IMIN: a_smin<SCOPE>(w, v); break;
@@ -74,12 +91,12 @@ FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
                     const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN
                                         : (bits >> 63)                ? FL_NINF
                                                                       : FL_PINF;
-                    a_or<SCOPE>(w + o.W, fl);
+                    a_or<SCOPE>(w + opk_W(d), fl);
                     break;
                 }
                 int w0;
                 int64_t d0, d1, d2;
-                if (!fx_digits(bits, o.L, w0, d0, d1, d2)) break;
+                if (!fx_digits(bits, opk_L(d), w0, d0, d1, d2)) break;
                 if (d0) a_add<SCOPE>(w + w0, (uint64_t)d0);
                 if (d1) a_add<SCOPE>(w + w0 + 1, (uint64_t)d1);
                 if (d2) a_add<SCOPE>(w + w0 + 2, (uint64_t)d2);
@@ -90,7 +107,7 @@ FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
                 const uint64_t bits = (uint64_t)v;
                 if (((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull)) break;  // NaN
                 const uint64_t e = f64_order(bits);
-                if (o.kind == OP_FMIN) a_umin<SCOPE>(w, e);
+                if (opk_kind(d) == OP_FMIN) a_umin<SCOPE>(w, e);
                 else a_umax<SCOPE>(w, e);
                 break;
             }
@@ -98,26 +115,74 @@ FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
     }
 }
 
+// Exclusive prefix of v over the workgroup and the workgroup total.  Every
+// thread of the block calls it (blockDim.x a multiple of 64, at most 1024).
+FGPU_DEV unsigned block_excl_scan(unsigned v, unsigned &total) {
+    __shared__ unsigned wsum[17];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    unsigned x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned t = __shfl_up(x, d, 64);
+        if (lane >= d) x += t;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned run = 0;
+        for (int w = 0; w < nw; ++w) {
+            const unsigned c = wsum[w];
+            wsum[w] = run;
+            run += c;
+        }
+        wsum[16] = run;
+    }
+    __syncthreads();
+    total = wsum[16];
+    const unsigned r = wsum[wave] + x - v;
+    __syncthreads();
+    return r;
+}
+
+// Reserves `total` output rows for the workgroup with one device atomic (one
+// atomic per wave on a shared counter serialises across thousands of waves).
+FGPU_DEV unsigned long long block_reserve(unsigned long long *counter, unsigned total) {
+    __shared__ unsigned long long base;
+    if (threadIdx.x == 0) base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    const unsigned long long b = base;
+    __syncthreads();
+    return b;
+}
+
+template <int SCOPE>
+FGPU_DEV void update_row(uint64_t *st, const DOps &ops, const Inputs &in) {
+    update_row<SCOPE>(st, op_list(ops), in);
+}
+
 // Merges a whole state (src, e.g. an LDS partial) into dst (HBM by default).
 template <int SCOPE = DEV_SCOPE>
 FGPU_DEV void merge_state(uint64_t *dst, const uint64_t *src, const DOps &ops) {
     a_add<SCOPE>(dst, src[0]);
-    for (int i = 0; i < ops.n; ++i) {
-        const DOp o = ops.op[i];
-        uint64_t *w = dst + o.word;
-        const uint64_t *s = src + o.word;
-        switch (o.kind) {
+#pragma unroll
+    for (int i = 0; i < MAX_OPS; ++i) {  // no early exit: fully unrolled, pk stays in SGPRs
+        if (i >= ops.n) continue;
+        const uint32_t d = ops.pk[i];
+        uint64_t *w = dst + opk_word(d);
+        const uint64_t *s = src + opk_word(d);
+        switch (opk_kind(d)) {
             case OP_CNT:
             case OP_ISUM:
                 if (s[0]) a_add<SCOPE>(w, s[0]);
                 break;
             case OP_IMIN: a_smin<SCOPE>(w, (int64_t)s[0]); break;
             case OP_IMAX: a_smax<SCOPE>(w, (int64_t)s[0]); break;
-            case OP_FSUM:
-                for (int k = 0; k < o.W; ++k)
+            case OP_FSUM: {
+                const int W = opk_W(d);
+                for (int k = 0; k < W; ++k)
                     if (s[k]) a_add<SCOPE>(w + k, s[k]);
-                if (o.has_flags && s[o.W]) a_or<SCOPE>(w + o.W, s[o.W]);
+                if (opk_flags(d) && s[W]) a_or<SCOPE>(w + W, s[W]);
                 break;
+            }
             case OP_FMIN: a_umin<SCOPE>(w, s[0]); break;
             case OP_FMAX: a_umax<SCOPE>(w, s[0]); break;
         }

@@ -53,6 +53,19 @@ def ctx_gbp():
 
 
 @pytest.fixture(scope="session")
+def ctx_gbp_small():
+    """Partitioned GROUP BY with an 8-bucket hot-key cache: most rows take the
+    cold path, and inserts race for the few ways."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("gb_part_min_rows", 1)
+    c.set_tuning("gbp_cache_slots", 64)
+    c.mode = "partitioned"
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="session")
 def ctx_gbh():
     """A context that keeps high-cardinality GROUP BY on the single hash table."""
     import falcondb_amd as fa

@@ -153,9 +153,9 @@ C2_AGGS = [AggExpr(COUNT), AggExpr(SUM, "v1"), AggExpr(MIN, "v1"), AggExpr(MAX, 
            AggExpr(SUM, "v2")]
 
 
-@pytest.fixture(params=["partitioned", "hash"])
-def gctx(request, ctx_gbp, ctx_gbh):
-    return ctx_gbp if request.param == "partitioned" else ctx_gbh
+@pytest.fixture(params=["partitioned", "partitioned_small_cache", "hash"])
+def gctx(request, ctx_gbp, ctx_gbp_small, ctx_gbh):
+    return {"partitioned": ctx_gbp, "partitioned_small_cache": ctx_gbp_small, "hash": ctx_gbh}[request.param]
 
 
 @pytest.mark.parametrize("n,nk", [(1000, 100), (300_000, 50_000), (2_000_000, 200_000)])
@@ -358,16 +358,35 @@ def test_join_partitioned_matches_direct_at_scale(ctx_part, ctx_direct):
 
 
 def test_groupby_partitioned_overflow_falls_back(ctx_gbp):
-    # A pass-2 table far too small for the partition's keys: the bound is
+    # A fan-out and pass-2 table far too small for the keys: the bound is
     # detected, and the query is answered by the single-table hash path.
     K = dg.keys(50_000, 40)
     cols = dg.c2_table(400_000, K)
     c = fa.Context(0)
     c.set_tuning("gb_part_min_rows", 1)
     c.set_tuning("gbp_table_slots", 64)
+    c.set_tuning("gbp_partitions", 8)
     t = _table(c, cols)
     g = fa.GroupByExec(["key"], C2_AGGS).execute(c, t, col("f") < 40)
     assert "fallback" in c.last_plan(), c.last_plan()
+    o = ora.groupby(cols, col("f") < 40, "key", C2_AGGS)
+    assert_batches_equal(g, o.exact, "key")
+    c.close()
+
+
+def test_groupby_partitioned_cell_overflow_resizes(ctx_gbp):
+    # No hot-key cache and a fine fan-out: the Zipf head overflows its cells in
+    # pass 1; the retry sizes cells from the counts it kept and is exact.
+    K = dg.keys(50_000, 40)
+    cols = dg.c2_table(400_000, K)
+    c = fa.Context(0)
+    c.set_tuning("gb_part_min_rows", 1)
+    c.set_tuning("gbp_cache_slots", 0)
+    c.set_tuning("gbp_partitions", 256)
+    t = _table(c, cols)
+    g = fa.GroupByExec(["key"], C2_AGGS).execute(c, t, col("f") < 40)
+    plan = c.last_plan()
+    assert "resized" in plan and "fallback" not in plan, plan
     o = ora.groupby(cols, col("f") < 40, "key", C2_AGGS)
     assert_batches_equal(g, o.exact, "key")
     c.close()

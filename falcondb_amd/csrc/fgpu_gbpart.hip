@@ -175,19 +175,24 @@ FGPU_DEV int64_t perm64(int addr, int64_t v) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
-FGPU_DEV int64_t gbp_pred_row(int64_t base, int u, int64_t r1) {
-    const int64_t r = base + u * 64 + (threadIdx.x & 63);
-    return r < r1 ? r : r1 - 1;
+// Tiles start at multiples of 128 rows (workgroup ranges are 256-aligned), so
+// a row's validity bit is bit (lane & 31) of its 32-bit validity word.
+
+// Row index within the tile for predicate loads, clamped into the table.
+FGPU_DEV uint32_t gbp_pred_idx(int64_t base, int u, int64_t r1) {
+    const uint32_t i = (uint32_t)(u * 64 + (threadIdx.x & 63));
+    const uint32_t last = (uint32_t)(r1 - base - 1);
+    return i < last ? i : last;
 }
 
 template <int U, int NPC, int NIN>
 FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1, GbpPredRaw<U, NPC> &t) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int64_t r = gbp_pred_row(base, u, r1);
+        const uint32_t i = gbp_pred_idx(base, u, r1);
 #pragma unroll
         for (int j = 0; j < NPC; ++j)
-            if (j < g.n_pcols) cell_issue(g.pc[j], r, t.c[u][j]);
+            if (j < g.n_pcols) cell_issue_tile(g.pc[j], base, i, t.c[u][j]);
     }
 }
 
@@ -198,9 +203,9 @@ FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &a, const GbpScan<NPC, NIN> &g, i
     int64_t v[U][NPC];
     uint32_t ok[U];
     uint32_t exists = 0;
+    const int64_t r = threadIdx.x & 31;  // validity bit of every row of the lane
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int64_t r = gbp_pred_row(base, u, r1);
         exists |= base + u * 64 + (threadIdx.x & 63) < r1 ? 1u << u : 0u;
         ok[u] = 0;
 #pragma unroll
@@ -220,19 +225,14 @@ FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &a, const GbpScan<NPC, NIN> &g, i
 // Key and inputs of the rows in `pass`; a lane whose row did not pass reads
 // the tile's first row instead, so lines holding no passing row stay unread.
 template <int U, int NPC, int NIN>
-FGPU_DEV int64_t gbp_data_row(int64_t base, int u, uint32_t pass) {
-    return ((pass >> u) & 1u) ? base + u * 64 + (threadIdx.x & 63) : base;
-}
-
-template <int U, int NPC, int NIN>
 FGPU_DEV void gbp_issue_data(const GbpScan<NPC, NIN> &g, int64_t base, uint32_t pass, GbpDataRaw<U, NIN> &t) {
     t.pass = pass;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int64_t r = gbp_data_row<U, NPC, NIN>(base, u, pass);
-        cell_issue(g.key, r, t.key[u]);
+        const uint32_t i = ((pass >> u) & 1u) ? (uint32_t)(u * 64 + (threadIdx.x & 63)) : 0u;
+        cell_issue_tile(g.key, base, i, t.key[u]);
 #pragma unroll
-        for (int j = 0; j < NIN; ++j) cell_issue(g.in[j], r, t.in[u][j]);
+        for (int j = 0; j < NIN; ++j) cell_issue_tile(g.in[j], base, i, t.in[u][j]);
     }
 }
 
@@ -240,9 +240,9 @@ template <int U, int NPC, int NIN>
 FGPU_DEV void gbp_finish_data(const GbpScan<NPC, NIN> &g, int64_t base, const GbpDataRaw<U, NIN> &t,
                               GbpDataTile<U, NIN> &d) {
     d.pass = t.pass;
+    const int64_t r = threadIdx.x & 31;  // validity bit of every passing row of the lane
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int64_t r = gbp_data_row<U, NPC, NIN>(base, u, t.pass);
         bool ok;
         cell_finish(g.key, r, t.key[u], d.r[u].key, ok);
         d.r[u].vw = (ok && d.r[u].key != EMPTY_KEY) ? GBP_HAS_KEY : 0u;
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     g.key = col_ref(a.cols, a.key_col);
 #pragma unroll
     for (int j = 0; j < NIN; ++j) g.in[j] = col_ref(a.cols, a.in_col[j]);
-    g.ops = op_list(a.ops);
+    g.ops = op_list(a.ops, A.has_valid_word != 0);
     g.S = a.ops.S;
     g.H = A.H;
     g.NB = A.H / GBP_WAYS;
@@ -368,9 +368,13 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     const int64_t r0 = (int64_t)blockIdx.x * A.rows_per_wg;
     const int64_t r1 = r0 + A.rows_per_wg < a.n_rows ? r0 + A.rows_per_wg : a.n_rows;
     g.my = A.cells + (int64_t)blockIdx.x * P * g.Cw * g.RW;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // wave, tile and chunk-fill counters are wave-uniform: readfirstlane lets
+    // the compiler keep them in SGPRs and branch on them with scalar branches.
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nw = __builtin_amdgcn_readfirstlane((int)(blockDim.x >> 6));
     constexpr int TR = 64 * U;  // rows per tile
-    const int64_t n_tiles = r1 > r0 ? (r1 - r0 + TR - 1) / TR : 0;
+    const int n_tiles = r1 > r0 ? (int)((r1 - r0 + TR - 1) / TR) : 0;
 
     GbpRow<NIN> cur, nxt;  // the chunk being filled and its overflow
     cur.key = nxt.key = 0;
@@ -381,15 +385,15 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
 
     // Pipeline: while tile t is aggregated, the data loads of tile t + nw and
     // the predicate loads of tile t + 2 nw are in flight.
-    int64_t t = wave;
+    int t = wave;
     GbpPredRaw<U, NPC> pr;
     GbpDataRaw<U, NIN> draw;
     draw.pass = 0;
     if (t < n_tiles) {
-        gbp_issue_pred<U, NPC, NIN>(g, r0 + t * TR, r1, pr);
-        const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + t * TR, r1, pr);
-        gbp_issue_data<U, NPC, NIN>(g, r0 + t * TR, m, draw);
-        if (t + nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (t + nw) * TR, r1, pr);
+        gbp_issue_pred<U, NPC, NIN>(g, r0 + (int64_t)t * TR, r1, pr);
+        const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + (int64_t)t * TR, r1, pr);
+        gbp_issue_data<U, NPC, NIN>(g, r0 + (int64_t)t * TR, m, draw);
+        if (t + nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (int64_t)(t + nw) * TR, r1, pr);
     }
     // One pass per tile, then one more (have == false) that aggregates the last,
     // partial chunk; gbp_row has a single call site to keep the loop compact.
@@ -398,11 +402,11 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
         GbpDataTile<U, NIN> dt;
         dt.pass = 0;
         if (have) {
-            gbp_finish_data<U, NPC, NIN>(g, r0 + t * TR, draw, dt);
+            gbp_finish_data<U, NPC, NIN>(g, r0 + (int64_t)t * TR, draw, dt);
             if (t + nw < n_tiles) {
-                const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + (t + nw) * TR, r1, pr);
-                gbp_issue_data<U, NPC, NIN>(g, r0 + (t + nw) * TR, m, draw);
-                if (t + 2 * nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (t + 2 * nw) * TR, r1, pr);
+                const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + (int64_t)(t + nw) * TR, r1, pr);
+                gbp_issue_data<U, NPC, NIN>(g, r0 + (int64_t)(t + nw) * TR, m, draw);
+                if (t + 2 * nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (int64_t)(t + 2 * nw) * TR, r1, pr);
             }
         }
         // compact tile t's passing rows into chunks and aggregate full ones
@@ -433,16 +437,14 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
                 const uint32_t vw = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)src.vw);
                 cur.vw = to_cur ? vw : cur.vw;
                 nxt.vw = to_nxt ? vw : nxt.vw;
-                fill += pc;
+                fill = __builtin_amdgcn_readfirstlane(fill + pc);
             }
             if (fill >= 64 || (!have && fill > 0)) {
                 gbp_row<NPC, NIN>(g, L, lane < fill, cur);
-                if (fill >= 64) {
-                    cur = nxt;
-                    fill -= 64;
-                } else {
-                    fill = 0;
-                }
+                // a full chunk continues with its overflow; after the final
+                // partial chunk the loop ends, so the copy is harmless there
+                fill = __builtin_amdgcn_readfirstlane(fill >= 64 ? fill - 64 : 0);
+                cur = nxt;
             }
         }
         if (!have) break;
@@ -500,6 +502,8 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     unsigned *spre = pre + G + 1;                             // G + 1: state records
     const int p = blockIdx.x;
     const int lane = threadIdx.x & 63;
+    // pass 1 overflowed a cell: the host resizes and runs both passes again
+    if (__hip_atomic_load(A.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     for (int i = threadIdx.x; i < T2; i += blockDim.x) tkeys[i] = EMPTY_KEY;
     for (int i = threadIdx.x; i < T2 * S; i += blockDim.x) tstate[i] = a.ops.ident[i % S];
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
@@ -623,6 +627,7 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
 // The NULL group: merge every workgroup's partial state, emit one row.
 __global__ void k_gbp_null(GbPartArgs A) {
     __shared__ uint64_t row[64];
+    if (__hip_atomic_load(A.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const DOps &ops = A.g.ops;
     for (int w = threadIdx.x; w < ops.S; w += blockDim.x) row[w] = ops.ident[w];
     __syncthreads();

@@ -60,6 +60,27 @@ FGPU_HD uint64_t f64_unorder(uint64_t enc) {
 // the top one, < 2^22) added at words w0, w0+1, w0+2 of a fixed point number
 // with LSB 2^L.  Returns false for zero (nothing to add).  Requires x to be a
 // multiple of 2^L (true for every value whose mantissa-LSB exponent is >= L).
+// Branch-free form of fx_digits for a finite value (zero yields three zero
+// digits).  The exact-sum layout leaves room for digits w0..w0+2 of every
+// value of the column, so callers add all three without tests.
+FGPU_DEV void fx_digits3(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1, int64_t &d2) {
+    const int be = (int)((bits >> 52) & 0x7ff);
+    uint64_t m = (bits & 0x000fffffffffffffull) | (be ? 0x0010000000000000ull : 0ull);
+    const int e = (be ? be : 1) - 1075;
+    const int s = L - e;  // > 0: the value's low s bits are zero (it is a multiple of 2^L)
+    m = s > 0 ? (s < 64 ? m >> s : 0ull) : m;
+    const int shift = s > 0 ? 0 : -s;
+    w0 = shift >> 5;
+    const int off = shift & 31;
+    const uint64_t lo = m << off;
+    const uint64_t hi = off ? (m >> (64 - off)) : 0ull;
+    const bool neg = bits >> 63;
+    const int64_t a0 = (int64_t)(lo & 0xffffffffull), a1 = (int64_t)(lo >> 32), a2 = (int64_t)hi;
+    d0 = neg ? -a0 : a0;
+    d1 = neg ? -a1 : a1;
+    d2 = neg ? -a2 : a2;
+}
+
 FGPU_HD bool fx_digits(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1, int64_t &d2) {
     uint64_t m = bits & 0x000fffffffffffffull;
     const int be = (int)((bits >> 52) & 0x7ff);
@@ -313,9 +334,17 @@ FGPU_DEV void cell_issue(const ColRef &c, int64_t r, RawCell &o) {
     else o.bits = *(const uint64_t *)(c.base + r * 8);
     o.vword = c.valid ? c.valid[r >> 5] : 0xffffffffu;
 }
+// Issue for row tile0 + idx where tile0 is a multiple of 32: the column and
+// validity bases are uniform (SGPR) and the lane offset is 32-bit.
+FGPU_DEV void cell_issue_tile(const ColRef &c, int64_t tile0, uint32_t idx, RawCell &o) {
+    if (c.dtype == FGPU_INT32 || c.dtype == FGPU_UTF8) o.bits = (uint64_t)((const uint32_t *)c.base + tile0)[idx];
+    else o.bits = ((const uint64_t *)c.base + tile0)[idx];
+    o.vword = c.valid ? (c.valid + (tile0 >> 5))[idx >> 5] : 0xffffffffu;
+}
 FGPU_DEV void cell_finish(const ColRef &c, int64_t r, const RawCell &o, int64_t &v, bool &ok) {
     v = c.dtype == FGPU_INT32 ? (int64_t)(int32_t)(uint32_t)o.bits : (int64_t)o.bits;
-    ok = ((o.vword >> (r & 31)) & 1u) && (c.dtype != FGPU_UTF8 || (uint32_t)o.bits != NULL_CODE);
+    // bitwise, not short-circuit: no branch around the code test
+    ok = (((o.vword >> (r & 31)) & 1u) != 0) & ((c.dtype != FGPU_UTF8) | ((uint32_t)o.bits != NULL_CODE));
 }
 
 template <int NPC>

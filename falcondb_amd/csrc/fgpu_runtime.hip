@@ -992,7 +992,7 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     if (lds1(H) > (size_t)tu.lds_limit) return -1;
     A.H = H;
     A.probe_limit = tu.gbp_probe_limit;
-    A.rows_per_wg = (t->n_rows + A.G - 1) / A.G;
+    A.rows_per_wg = ((t->n_rows + A.G - 1) / A.G + 255) / 256 * 256;  // tiles stay 32-row aligned
     A.Cw = (int)std::ceil((double)A.rows_per_wg / (double)P * 1.25) +
            A.SR * ((int)((4 * (int64_t)H + P - 1) / P) + 4) + 32;
     const int64_t out_cap = std::min<int64_t>(t->n_rows + 1, std::max<int64_t>(65536, 2 * key->distinct + 1024));
@@ -1034,7 +1034,9 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
         mark(c, 1);
         HIPCHK(launch_gbp_scan(c->stream, A, tu.gbp_scan_block, lds1(H)));
         mark(c, 2);
-        HIPCHK(hipMemcpyAsync(h, d_flags, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(launch_gbp_agg(c->stream, A, tu.gbp_agg_block, lds2(T2)));  // returns at once after a cell overflow
+        mark(c, 3);
+        HIPCHK(hipMemcpyAsync(h, d_flags, 24, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         snprintf(msg, sizeof msg, " Cw=%d", A.Cw);
         c->plan += msg;
@@ -1060,11 +1062,6 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
                 continue;
             }
         }
-        if (h[0]) break;
-        HIPCHK(launch_gbp_agg(c->stream, A, tu.gbp_agg_block, lds2(T2)));
-        mark(c, 3);
-        HIPCHK(hipMemcpyAsync(h, d_flags, 24, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
         break;
     }
     if (h[0] || h[1] || h[2]) {  // a bound was exceeded: single-table hash path

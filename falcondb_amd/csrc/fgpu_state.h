@@ -58,11 +58,13 @@ FGPU_DEV int64_t pick(const Inputs &in, int j) {
 // update_row in a loop build it before the loop).
 struct OpList {
     int n;
+    int check_valid;  // some input may be NULL (else the per-op validity test is skipped)
     uint32_t pk[MAX_OPS];
 };
-FGPU_DEV OpList op_list(const DOps &ops) {
+FGPU_DEV OpList op_list(const DOps &ops, bool check_valid = true) {
     OpList l;
     l.n = ops.n;
+    l.check_valid = check_valid;
 #pragma unroll
     for (int i = 0; i < MAX_OPS; ++i) l.pk[i] = ops.pk[i];
     return l;
@@ -77,7 +79,7 @@ FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
         if (i >= ops.n) continue;
         const uint32_t d = ops.pk[i];
         const int col = opk_col(d);
-        if (!((in.valid >> col) & 1)) continue;
+        if (ops.check_valid && !((in.valid >> col) & 1)) continue;
         const int64_t v = pick(in, col);
         uint64_t *w = st + opk_word(d);
         switch (opk_kind(d)) {
@@ -87,7 +89,8 @@ FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
             case OP_IMAX: a_smax<SCOPE>(w, v); break;
             case OP_FSUM: {
                 const uint64_t bits = (uint64_t)v;
-                if (((bits >> 52) & 0x7ff) == 0x7ff) {  // NaN / inf
+                // the flags word exists only for columns holding NaN / inf
+                if (opk_flags(d) && ((bits >> 52) & 0x7ff) == 0x7ff) {
                     const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN
                                         : (bits >> 63)                ? FL_NINF
                                                                       : FL_PINF;
@@ -96,10 +99,10 @@ FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
                 }
                 int w0;
                 int64_t d0, d1, d2;
-                if (!fx_digits(bits, opk_L(d), w0, d0, d1, d2)) break;
-                if (d0) a_add<SCOPE>(w + w0, (uint64_t)d0);
-                if (d1) a_add<SCOPE>(w + w0 + 1, (uint64_t)d1);
-                if (d2) a_add<SCOPE>(w + w0 + 2, (uint64_t)d2);
+                fx_digits3(bits, opk_L(d), w0, d0, d1, d2);
+                a_add<SCOPE>(w + w0, (uint64_t)d0);
+                a_add<SCOPE>(w + w0 + 1, (uint64_t)d1);
+                a_add<SCOPE>(w + w0 + 2, (uint64_t)d2);
                 break;
             }
             case OP_FMIN:

@@ -62,6 +62,74 @@ def join_call(lib, ctx, probe, build, idx, spec):
     lib.fgpu_result_free(h)
 
 
+def bench_batched(ctx, lib, steps):
+    """256 filter + GROUP BY queries over batches of log-uniform size (the C4
+    recipe): the same queries as 256 plain fgpu_groupby calls, then as one
+    prepared batch (one graph replay per run, autotuned)."""
+    import falcondb_amd as fa
+    from falcondb_amd import _abi
+    from falcondb_amd import datagen as dg
+    from falcondb_amd._abi import COUNT, MAX, MIN, SUM
+    from falcondb_amd.operators import AggExpr, _aggs_struct, col
+    qs = dg.c4_queries()
+    tabs = [fa.Table(ctx, dg.c4_batch(q["rows"], q["card"], q["seed"])) for q in qs]
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "val"), AggExpr(MIN, "val"), AggExpr(MAX, "val"),
+            AggExpr(SUM, "score")]
+    queries = [(t, "key", aggs, col("score") < q["theta"]) for q, t in zip(qs, tabs)]
+    rows = sum(q["rows"] for q in qs)
+    # parity: every query, batched against plain (results copied back)
+    ctx.set_tuning("keep_on_device", 0)
+    batch = fa.BatchGroupBy(ctx, queries)
+    same = True
+    for (t, key, ag, f), got in zip(queries, batch.run()):
+        plain = fa.GroupByExec([key], ag).execute(ctx, t, f).sort_by(key)
+        got = got.sort_by(key)
+        same &= got.num_rows == plain.num_rows and all(
+            np.array_equal(a, b) for a, b in zip(got.columns, plain.columns))
+    ctx.set_tuning("keep_on_device", 1)
+    specs = []
+    for t, key, ag, f in queries:
+        p, keep = f._compile(t)
+        arr, keep2 = _aggs_struct(ag, lambda side, name, t=t: t.column_index(name))
+        specs.append((t, p, keep, arr, keep2, _abi.GroupBySpec(t.column_index(key), len(ag),
+                                                               C.cast(arr, C.POINTER(_abi.Agg)))))
+
+    def plain_all():
+        for t, p, keep, arr, keep2, spec in specs:
+            groupby_call(lib, ctx, t, p, spec)
+
+    plain_all()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain_all()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / steps
+    tune_info = batch.tune(18)
+
+    def batched():
+        for h in batch.run_raw():
+            lib.fgpu_result_free(h)
+
+    batched()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched()
+        dev_ms.append(ctx.last_timing()["main_ms"])
+    batch_ms = (time.perf_counter() - t0) * 1e3 / steps
+    alg = rows * (4 + 8 + 8)
+    log(f"[bench] C4: plain {plain_ms:.2f} ms, batched {batch_ms:.2f} ms ({np.mean(dev_ms):.2f} ms on "
+        f"the device), parity {same}; {tune_info}")
+    out = {"queries": len(qs), "rows": rows, "plain_ms": plain_ms, "batched_ms": batch_ms,
+           "batched_device_ms": float(np.mean(dev_ms)), "speedup": plain_ms / batch_ms,
+           "rows_per_s": rows / (batch_ms * 1e-3), "achieved_GBs": alg / (np.mean(dev_ms) * 1e-3) / 1e9,
+           "frac": alg / (np.mean(dev_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, "parity_vs_plain": bool(same),
+           "config": tune_info, "fallback_queries": batch.fallbacks()}
+    batch.free()
+    for t in tabs:
+        t.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +138,7 @@ def main():
     ap.add_argument("--rows", type=int, default=C2_ROWS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-join", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-sample-rows", type=int, default=C2_ROWS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -182,6 +251,9 @@ def main():
         pt.free()
         bt.free()
         del pc, bc
+
+    if not args.no_batch:
+        secondary["c4_batched"] = bench_batched(ctx, lib, max(3, args.steps // 4))
 
     cpu = None
     if not args.no_cpu:

@@ -62,6 +62,10 @@ class ColumnView(C.Structure):
                 ("dict", C.POINTER(C.c_char_p))]
 
 
+class BatchQuery(C.Structure):
+    _fields_ = [("table", C.c_void_p), ("pred", C.c_void_p), ("spec", C.c_void_p)]
+
+
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_float), ("main_ms", C.c_float), ("build_ms", C.c_float),
                 ("finalize_ms", C.c_float), ("bytes_main", C.c_int64)]
@@ -100,6 +104,12 @@ DECLARED = {
     "fgpu_join_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.c_void_p, C.c_void_p,
                                   C.POINTER(JoinSpec), C.c_void_p, C.c_void_p, C.c_int64,
                                   C.POINTER(C.c_int64)]),
+    "fgpu_batch_prepare": (C.c_int, [C.c_void_p, C.POINTER(BatchQuery), C.c_int, C.POINTER(C.c_void_p)]),
+    "fgpu_batch_tune": (C.c_int, [C.c_void_p, C.c_int]),
+    "fgpu_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "fgpu_batch_info": (C.c_char_p, [C.c_void_p]),
+    "fgpu_batch_fallbacks": (C.c_int, [C.c_void_p]),
+    "fgpu_batch_free": (None, [C.c_void_p]),
     "fgpu_result_rows": (C.c_int64, [C.c_void_p]),
     "fgpu_result_cols": (C.c_int, [C.c_void_p]),
     "fgpu_result_column": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ColumnView)]),

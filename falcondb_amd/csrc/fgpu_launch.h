@@ -226,6 +226,65 @@ hipError_t launch_join_pairs(hipStream_t s, const JoinProbeArgs &a, int pass, in
 // Fill helpers
 hipError_t launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v);
 
+// ─── batched GROUP BY ─────────────────────────────────────────────────────
+// Many GROUP BY queries (each over its own table) run as one persistent pass 1
+// over (query, row range) units and one persistent pass 2 over (query, key
+// range) units.  Keys are integers with a dense range: slot = key - key_min.
+enum : int32_t {
+    BQ_DENSE = 0,  // pass 1 aggregates the whole key range in LDS, one state block per unit
+    BQ_PART = 1    // pass 1 writes (key, inputs) records into key-range partitions
+};
+
+struct BatchQuery {
+    GbArgs g;                 // columns, predicate, ops, key slot, n_rows
+    DOuts outs;
+    int mode;                 // BQ_*
+    int64_t key_min;
+    int64_t range;            // dense key slots [0, range); the NULL group is separate
+    int shift;                // BQ_PART: partition = slot >> shift (2^shift slots each)
+    int P;                    // BQ_PART partitions
+    int S, RW, Cw, has_valid_word;
+    int64_t unit0, n_units;   // this query's pass-1 units
+    int64_t *cells;           // BQ_PART: n_units * P * Cw * RW words
+    unsigned int *cell_cnt;   // BQ_PART: n_units * P record counts
+    uint64_t *blocks;         // BQ_DENSE: n_units * range * S words
+    uint64_t *null_blocks;    // n_units * S words: each unit's NULL-key group state
+    int64_t *out_key;
+    uint8_t *out_key_valid;
+    int64_t *out_val[FGPU_MAX_AGGS];
+    uint8_t *out_valid[FGPU_MAX_AGGS];
+    unsigned long long *out_count;
+    int64_t out_cap;
+};
+
+struct BatchUnit {            // pass 1: rows [row0, row1) of query q (unit `local` of it)
+    int32_t q;
+    int32_t local;
+    int64_t row0, row1;
+};
+struct MergeUnit {            // pass 2: dense slots [k0, k1) or partition `part` of query q
+    int32_t q;
+    int32_t part;             // BQ_PART partition, or -1 for a BQ_DENSE slot range
+    int64_t k0, k1;
+    int32_t null_group;       // this unit also merges and emits the NULL-key group
+    int32_t pad;
+};
+
+struct BatchArgs {
+    const BatchUnit *units;
+    int64_t n_units;
+    const MergeUnit *units2;
+    int64_t n_units2;
+    unsigned long long *work;   // [0] pass-1 cursor, [1] pass-2 cursor (zeroed before each run)
+    unsigned long long *flags;  // [0] cell full, [1] output full
+    int lds_slots;              // LDS table slots (pass 1 BQ_DENSE range cap, pass 2 partition size)
+};
+hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int npc, int grid,
+                             int block, size_t lds);
+hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int grid, int block,
+                              size_t lds);
+hipError_t batch_set_lds(size_t lds1, size_t lds2);
+
 // Kernel arguments travel in the 4 KiB kernarg segment; descriptor fields are
 // 32-bit so that the kernels read them with scalar loads.
 static_assert(sizeof(GbPartArgs) <= 4096, "GbPartArgs exceeds the kernarg segment");

@@ -463,12 +463,17 @@ int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const f
 }
 
 // Copies the emitted rows of the last GROUP BY / aggregate into a result.
+// `known` >= 0: the row count is already on the host (a batch copies all of
+// them at once); otherwise it is read from d_count.
 int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key, int key_dtype,
             const std::string &key_name, int64_t *d_key, uint8_t *d_key_valid,
-            int64_t **d_val, uint8_t **d_valid, unsigned long long *d_count, fgpu_result **out) {
-    unsigned long long n = 0;
-    HIPCHK(hipMemcpyAsync(&n, d_count, sizeof n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+            int64_t **d_val, uint8_t **d_valid, unsigned long long *d_count, fgpu_result **out,
+            int64_t known = -1) {
+    unsigned long long n = (unsigned long long)known;
+    if (known < 0) {
+        HIPCHK(hipMemcpyAsync(&n, d_count, sizeof n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     if ((int64_t)n > n_slots) return set_err(FGPU_E_DEVICE, "emit counter %llu exceeds %lld slots", n, (long long)n_slots);
     auto *r = new fgpu_result();
     r->rows = (int64_t)n;
@@ -1709,3 +1714,485 @@ int fgpu_host_exact_sum(const double *x, int64_t n, double *out) {
 }
 
 }  // extern "C"
+
+// ─── batched GROUP BY ───────────────────────────────────────────────────────
+// The knobs the autotuner sweeps: workgroup size, rows per pass-1 unit and the
+// LDS table size of pass 2 (which is also the key-range fan-out of BQ_PART).
+struct BatchCfg {
+    int block = 1024;
+    int64_t unit_rows = int64_t(1) << 18;
+    int slots = 2048;
+    int wgs_per_cu = 1;
+};
+
+struct BatchQ {
+    const fgpu_table *t = nullptr;
+    int fallback = 0;  // a shape the batch kernels do not take: runs through fgpu_groupby
+    std::vector<fgpu_pred_node> nodes;
+    std::vector<std::string> strs;  // literal strings and aliases the copies point at
+    std::vector<fgpu_agg> aggs;
+    fgpu_pred pred{};
+    bool has_pred = false;
+    fgpu_groupby_spec spec{};
+    AggLayout L;
+    const DevCol *key = nullptr;
+    GbArgs g{};
+    int has_valid_word = 0;
+    int64_t Cw_need = 0;  // cell size seen by an overflowing run (0 = from the row count)
+    BatchQuery q{};
+};
+
+struct fgpu_batch {
+    fgpu_ctx *c = nullptr;
+    std::vector<BatchQ> qs;
+    BatchCfg cfg;
+    int npc = 1;
+    void *mem = nullptr;
+    BatchQuery *d_qs = nullptr;
+    BatchUnit *d_units = nullptr;
+    MergeUnit *d_units2 = nullptr;
+    unsigned long long *d_work = nullptr, *d_flags = nullptr, *d_counts = nullptr;
+    int64_t n_units = 0, n_units2 = 0;
+    size_t lds1 = 0, lds2 = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string info;
+    float last_ms = 0;
+};
+
+namespace {
+
+void batch_release(fgpu_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (b->mem) (void)hipFree(b->mem);
+    b->mem = nullptr;
+}
+
+// Lays out every batched query for b->cfg, allocates the plan's device memory
+// in one piece, uploads the descriptors and captures the run as a graph.
+int batch_plan(fgpu_batch *b) {
+    fgpu_ctx *c = b->c;
+    batch_release(b);
+    const BatchCfg &cfg = b->cfg;
+    const size_t lim = (size_t)c->tune.lds_limit - 1024;  // static shared of the kernels
+    struct Piece {
+        size_t off = 0, bytes = 0;
+    };
+    size_t total = 0;
+    auto take = [&](size_t bytes) {
+        Piece pc;
+        pc.off = total;
+        pc.bytes = bytes;
+        total += (bytes + 255) & ~size_t(255);
+        return pc;
+    };
+    std::vector<BatchUnit> units;
+    std::vector<MergeUnit> units2;
+    struct QPieces {
+        Piece cells, cnt, blocks, nulls, key, kvalid, val[FGPU_MAX_AGGS], valid[FGPU_MAX_AGGS];
+    };
+    std::vector<QPieces> pieces(b->qs.size());
+    b->lds1 = b->lds2 = 8;
+    b->npc = 1;
+    for (size_t qi = 0; qi < b->qs.size(); ++qi) {
+        BatchQ &Q = b->qs[qi];
+        if (Q.fallback) continue;
+        BatchQuery &q = Q.q;
+        q = BatchQuery{};
+        q.g = Q.g;
+        q.outs = Q.L.outs;
+        const int S = Q.L.ops.S;
+        q.S = S;
+        q.has_valid_word = Q.has_valid_word;
+        q.RW = 1 + Q.g.n_in + Q.has_valid_word;
+        const DevCol &key = *Q.key;
+        q.key_min = key.imin <= key.imax ? key.imin : 0;
+        q.range = key.imin <= key.imax ? key.imax - key.imin + 1 : 1;
+        const int64_t n_rows = Q.t->n_rows;
+        q.n_units = std::max<int64_t>(1, (n_rows + cfg.unit_rows - 1) / cfg.unit_rows);
+        const int64_t per_unit = (n_rows + q.n_units - 1) / q.n_units;
+        // pass-2 slots: a multiple-of-two table that fits next to the G + 1 prefix
+        int64_t slots = cfg.slots;
+        while (slots > 64 && (size_t)slots * S * 8 + (size_t)(q.n_units + 1) * 4 > lim) slots >>= 1;
+        if ((size_t)(S + q.range * S) * 8 <= std::min<size_t>(lim, 64 * 1024)) {
+            q.mode = BQ_DENSE;
+            b->lds1 = std::max(b->lds1, (size_t)(S + q.range * S) * 8);
+            for (int64_t k0 = 0; k0 < q.range; k0 += slots)
+                units2.push_back(MergeUnit{(int32_t)qi, -1, k0, std::min(k0 + slots, q.range), k0 == 0, 0});
+            b->lds2 = std::max(b->lds2, (size_t)std::min(slots, q.range) * S * 8);
+        } else {
+            q.mode = BQ_PART;
+            int shift = 0;
+            while ((int64_t(2) << shift) <= slots) ++shift;
+            q.shift = shift;
+            q.P = (int)((q.range + (int64_t(1) << shift) - 1) >> shift);
+            if ((size_t)S * 8 + (size_t)q.P * 4 > lim) {  // the pass-1 cursors do not fit
+                Q.fallback = 1;
+                continue;
+            }
+            const int64_t cw_rows = std::min<int64_t>(per_unit, n_rows);
+            q.Cw = (int)std::max<int64_t>(Q.Cw_need, (int64_t)std::ceil((double)cw_rows / q.P * 1.25) + 64);
+            b->lds1 = std::max(b->lds1, (size_t)S * 8 + (size_t)q.P * 4);
+            for (int p = 0; p < q.P; ++p) {
+                const int64_t k0 = (int64_t)p << shift;
+                units2.push_back(MergeUnit{(int32_t)qi, p, k0, std::min(k0 + (int64_t(1) << shift), q.range), p == 0, 0});
+            }
+            b->lds2 = std::max(b->lds2, ((size_t)1 << shift) * S * 8 + (size_t)(q.n_units + 1) * 4);
+        }
+        q.unit0 = (int64_t)units.size();
+        for (int64_t u = 0; u < q.n_units; ++u)
+            units.push_back(BatchUnit{(int32_t)qi, (int32_t)u, std::min(u * per_unit, n_rows),
+                                      std::min((u + 1) * per_unit, n_rows)});
+        q.out_cap = std::min(q.range, n_rows) + 1;
+        QPieces &pc = pieces[qi];
+        if (q.mode == BQ_PART) {
+            pc.cells = take((size_t)q.n_units * q.P * q.Cw * q.RW * 8);
+            pc.cnt = take((size_t)q.n_units * q.P * 4);
+        } else {
+            pc.blocks = take((size_t)q.n_units * q.range * S * 8);
+        }
+        pc.nulls = take((size_t)q.n_units * S * 8);
+        pc.key = take((size_t)q.out_cap * 8);
+        pc.kvalid = take((size_t)q.out_cap);
+        for (int k = 0; k < Q.L.outs.n; ++k) {
+            pc.val[k] = take((size_t)q.out_cap * 8);
+            pc.valid[k] = take((size_t)q.out_cap);
+        }
+        b->npc = std::max(b->npc, Q.g.pred.n_pcols);
+    }
+    // larger units first: the persistent workgroups finish together
+    std::stable_sort(units.begin(), units.end(), [](const BatchUnit &x, const BatchUnit &y) {
+        return x.row1 - x.row0 > y.row1 - y.row0;
+    });
+    std::stable_sort(units2.begin(), units2.end(), [](const MergeUnit &x, const MergeUnit &y) {
+        return x.k1 - x.k0 > y.k1 - y.k0;
+    });
+    b->n_units = (int64_t)units.size();
+    b->n_units2 = (int64_t)units2.size();
+    const Piece p_qs = take(b->qs.size() * sizeof(BatchQuery));
+    const Piece p_u = take(std::max<size_t>(1, units.size()) * sizeof(BatchUnit));
+    const Piece p_u2 = take(std::max<size_t>(1, units2.size()) * sizeof(MergeUnit));
+    const Piece p_ctl = take((4 + b->qs.size()) * 8);
+    if (b->lds1 > lim || b->lds2 > lim) return set_err(FGPU_E_UNSUPPORTED, "batch tables exceed the LDS");
+    if (hipMalloc(&b->mem, std::max<size_t>(total, 256)) != hipSuccess)
+        return set_err(FGPU_E_OUT_OF_MEMORY, "batch plan of %zu bytes", total);
+    char *m = (char *)b->mem;
+    b->d_qs = (BatchQuery *)(m + p_qs.off);
+    b->d_units = (BatchUnit *)(m + p_u.off);
+    b->d_units2 = (MergeUnit *)(m + p_u2.off);
+    b->d_work = (unsigned long long *)(m + p_ctl.off);
+    b->d_flags = b->d_work + 2;
+    b->d_counts = b->d_work + 4;
+    std::vector<BatchQuery> hq(b->qs.size());
+    for (size_t qi = 0; qi < b->qs.size(); ++qi) {
+        BatchQ &Q = b->qs[qi];
+        if (Q.fallback) continue;
+        BatchQuery &q = Q.q;
+        const QPieces &pc = pieces[qi];
+        q.cells = q.mode == BQ_PART ? (int64_t *)(m + pc.cells.off) : nullptr;
+        q.cell_cnt = q.mode == BQ_PART ? (unsigned int *)(m + pc.cnt.off) : nullptr;
+        q.blocks = q.mode == BQ_DENSE ? (uint64_t *)(m + pc.blocks.off) : nullptr;
+        q.null_blocks = (uint64_t *)(m + pc.nulls.off);
+        q.out_key = (int64_t *)(m + pc.key.off);
+        q.out_key_valid = (uint8_t *)(m + pc.kvalid.off);
+        for (int k = 0; k < Q.L.outs.n; ++k) {
+            q.out_val[k] = (int64_t *)(m + pc.val[k].off);
+            q.out_valid[k] = (uint8_t *)(m + pc.valid[k].off);
+        }
+        q.out_count = b->d_counts + qi;
+        hq[qi] = q;
+    }
+    HIPCHK(hipMemcpyAsync(b->d_qs, hq.data(), hq.size() * sizeof(BatchQuery), hipMemcpyHostToDevice, c->stream));
+    if (!units.empty())
+        HIPCHK(hipMemcpyAsync(b->d_units, units.data(), units.size() * sizeof(BatchUnit), hipMemcpyHostToDevice,
+                              c->stream));
+    if (!units2.empty())
+        HIPCHK(hipMemcpyAsync(b->d_units2, units2.data(), units2.size() * sizeof(MergeUnit),
+                              hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(batch_set_lds(b->lds1, b->lds2));
+    // the run: resets by fill kernels (no memset nodes), then the two passes
+    BatchArgs A{};
+    A.units = b->d_units;
+    A.n_units = b->n_units;
+    A.units2 = b->d_units2;
+    A.n_units2 = b->n_units2;
+    A.work = b->d_work;
+    A.flags = b->d_flags;
+    A.lds_slots = cfg.slots;
+    const int grid = c->num_cus * cfg.wgs_per_cu;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_fill_u64(c->stream, (uint64_t *)b->d_work, (int64_t)(4 + b->qs.size()), 0);
+    if (e == hipSuccess && b->n_units) e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, grid, cfg.block, b->lds1);
+    if (e == hipSuccess && b->n_units2) e = launch_batch_merge(c->stream, b->d_qs, A, grid, cfg.block, b->lds2);
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &b->graph);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return set_err(FGPU_E_DEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    char msg[200];
+    snprintf(msg, sizeof msg, "block=%d unit_rows=%lld slots=%d units=%lld/%lld plan_bytes=%zu", cfg.block,
+             (long long)cfg.unit_rows, cfg.slots, (long long)b->n_units, (long long)b->n_units2, total);
+    b->info = msg;
+    return FGPU_OK;
+}
+
+// One replay of the graph; `flags` gets the overflow flags, `counts` the rows
+// each query emitted.
+int batch_launch(fgpu_batch *b, unsigned long long flags[2], std::vector<unsigned long long> &counts) {
+    fgpu_ctx *c = b->c;
+    counts.assign(b->qs.size(), 0);
+    HIPCHK(hipEventRecord(b->ev0, c->stream));
+    HIPCHK(hipGraphLaunch(b->exec, c->stream));
+    HIPCHK(hipEventRecord(b->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(flags, b->d_flags, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(counts.data(), b->d_counts, counts.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
+    return FGPU_OK;
+}
+
+// Runs the graph; a cell overflow re-plans once with cells sized from the
+// counts the run kept.
+int batch_execute(fgpu_batch *b, std::vector<unsigned long long> &counts) {
+    unsigned long long flags[2] = {0, 0};
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (int rc = batch_launch(b, flags, counts)) return rc;
+        if (!flags[0]) break;
+        if (attempt == 1) return set_err(FGPU_E_UNSUPPORTED, "batch cells overflowed after resizing");
+        for (BatchQ &Q : b->qs) {
+            if (Q.fallback || Q.q.mode != BQ_PART) continue;
+            std::vector<unsigned int> cc((size_t)Q.q.n_units * Q.q.P);
+            HIPCHK(hipMemcpy(cc.data(), Q.q.cell_cnt, cc.size() * 4, hipMemcpyDeviceToHost));
+            const unsigned mx = cc.empty() ? 0 : *std::max_element(cc.begin(), cc.end());
+            if ((int64_t)mx > Q.q.Cw) Q.Cw_need = (int64_t)mx * 5 / 4 + 64;
+        }
+        if (int rc = batch_plan(b)) return rc;
+    }
+    if (flags[1]) return set_err(FGPU_E_DEVICE, "batch output overflow");
+    return FGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgpu_batch_prepare(fgpu_ctx *c, const fgpu_batch_query *queries, int n, fgpu_batch **out) {
+    if (!c || !out || n < 0 || (n > 0 && !queries)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    auto *b = new fgpu_batch();
+    b->c = c;
+    b->qs.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const fgpu_batch_query &bq = queries[i];
+        BatchQ &Q = b->qs[(size_t)i];
+        if (!bq.table || !bq.spec || (bq.spec->n_aggs > 0 && !bq.spec->aggs)) {
+            delete b;
+            return set_err(FGPU_E_INVALID_QUERY, "batch query %d: NULL table or spec", i);
+        }
+        Q.t = bq.table;
+        // copies of the query (the caller's arrays need not outlive this call)
+        Q.strs.reserve((size_t)(bq.pred ? bq.pred->n_nodes : 0) + (size_t)bq.spec->n_aggs);
+        if (bq.pred) {
+            Q.nodes.assign(bq.pred->nodes, bq.pred->nodes + bq.pred->n_nodes);
+            for (auto &nd : Q.nodes)
+                if (nd.lit_str) {
+                    Q.strs.emplace_back(nd.lit_str);
+                    nd.lit_str = Q.strs.back().c_str();
+                }
+            Q.pred.n_nodes = (int32_t)Q.nodes.size();
+            Q.pred.nodes = Q.nodes.data();
+            Q.has_pred = true;
+        }
+        Q.aggs.assign(bq.spec->aggs, bq.spec->aggs + bq.spec->n_aggs);
+        for (auto &ag : Q.aggs)
+            if (ag.alias) {
+                Q.strs.emplace_back(ag.alias);
+                ag.alias = Q.strs.back().c_str();
+            }
+        Q.spec = *bq.spec;
+        Q.spec.aggs = Q.aggs.data();
+        // the batch kernels take integer keys with a dense range and no
+        // dictionary predicates; anything else runs through fgpu_groupby
+        const fgpu_table *t = Q.t;
+        const int kc = Q.spec.key_col;
+        if (kc < 0 || kc >= (int)t->cols.size()) {
+            Q.fallback = 1;
+            continue;
+        }
+        const DevCol &key = t->cols[kc];
+        bool dict_pred = false;
+        for (auto &nd : Q.nodes)
+            if (nd.kind == FGPU_P_CMP && nd.col >= 0 && nd.col < (int)t->cols.size() &&
+                t->cols[nd.col].dtype == FGPU_UTF8)
+                dict_pred = true;
+        const bool int_key = key.dtype == FGPU_INT32 || key.dtype == FGPU_INT64;
+        const int64_t range = key.imin <= key.imax ? key.imax - key.imin + 1 : 1;
+        if (!int_key || dict_pred || key.imin == INT64_MIN || range <= 0 || range > (int64_t(1) << 26)) {
+            Q.fallback = 1;
+            continue;
+        }
+        Q.key = &key;
+        Binder bd(t);
+        DPred dp;
+        if (int rc = compile_pred(c, t, Q.has_pred ? &Q.pred : nullptr, bd, dp)) {
+            delete b;
+            return rc;
+        }
+        GbArgs &a = Q.g;
+        a = GbArgs{};
+        a.pred = dp;
+        a.key_col = bd.bind(kc);
+        int in_of_col[256];
+        for (int &x : in_of_col) x = -1;
+        a.n_in = 0;
+        bool too_many = false;
+        auto slot_of = [&](int /*side*/, int col) -> int {
+            if (col < 256 && in_of_col[col] >= 0) return in_of_col[col];
+            if (a.n_in >= MAX_IN) {
+                too_many = true;
+                return -1;
+            }
+            const int sl = bd.bind(col);
+            if (sl < 0) return -1;
+            a.in_col[a.n_in] = sl;
+            if (col < 256) in_of_col[col] = a.n_in;
+            return a.n_in++;
+        };
+        if (a.key_col < 0 || build_layout(Q.spec.aggs, Q.spec.n_aggs, t, nullptr, slot_of, Q.L) || too_many) {
+            g_err.clear();
+            Q.fallback = 1;  // fgpu_groupby reports the error at run time
+            continue;
+        }
+        a.cols = bd.dc;
+        a.ops = Q.L.ops;
+        a.n_rows = t->n_rows;
+        for (int j = 0; j < a.n_in; ++j)
+            for (const DevCol &dc : t->cols)
+                if (dc.d_values == a.cols.v[a.in_col[j]] && dc.null_count) Q.has_valid_word = 1;
+    }
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        delete b;
+        return set_err(FGPU_E_DEVICE, "batch events");
+    }
+    if (int rc = batch_plan(b)) {
+        (void)hipEventDestroy(b->ev0);
+        (void)hipEventDestroy(b->ev1);
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return FGPU_OK;
+}
+
+int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
+    if (!b) return set_err(FGPU_E_INVALID_QUERY, "NULL batch");
+    fgpu_ctx *c = b->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    std::vector<BatchCfg> cands;
+    for (int block : {1024, 512})
+        for (int64_t ur : {int64_t(1) << 18, int64_t(1) << 16, int64_t(1) << 20})
+            for (int slots : {2048, 4096, 1024}) {
+                BatchCfg k;
+                k.block = block;
+                k.unit_rows = ur;
+                k.slots = slots;
+                cands.push_back(k);
+            }
+    if (max_trials > 0 && (int)cands.size() > max_trials) cands.resize((size_t)max_trials);
+    BatchCfg best = b->cfg;
+    float best_ms = 1e30f;
+    std::vector<unsigned long long> counts;
+    int tried = 0;
+    for (const BatchCfg &k : cands) {
+        b->cfg = k;
+        if (batch_plan(b) != FGPU_OK) {
+            g_err.clear();
+            continue;
+        }
+        if (batch_execute(b, counts) != FGPU_OK) {
+            g_err.clear();
+            continue;
+        }
+        float ms = 0;
+        for (int r = 0; r < 2; ++r) {  // after the warm-up above
+            if (int rc = batch_execute(b, counts)) return rc;
+            ms += b->last_ms;
+        }
+        ++tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = b->cfg;
+        }
+    }
+    b->cfg = best;
+    if (int rc = batch_plan(b)) return rc;
+    char msg[96];
+    snprintf(msg, sizeof msg, " tuned=%d trials best=%.3f ms", tried, best_ms / 2);
+    b->info += msg;
+    return FGPU_OK;
+}
+
+int fgpu_batch_run(fgpu_batch *b, fgpu_result **outs) {
+    if (!b || (!outs && !b->qs.empty())) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    fgpu_ctx *c = b->c;
+    std::vector<unsigned long long> counts;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int rc = set_device(c->device)) return rc;
+        if (int rc = batch_execute(b, counts)) return rc;
+        c->last = fgpu_timing{};
+        c->last.total_ms = c->last.main_ms = b->last_ms;
+        for (size_t qi = 0; qi < b->qs.size(); ++qi) {
+            BatchQ &Q = b->qs[qi];
+            outs[qi] = nullptr;
+            if (Q.fallback) continue;
+            const BatchQuery &q = Q.q;
+            int64_t *vals[FGPU_MAX_AGGS];
+            uint8_t *valid[FGPU_MAX_AGGS];
+            for (int k = 0; k < FGPU_MAX_AGGS; ++k) {
+                vals[k] = q.out_val[k];
+                valid[k] = q.out_valid[k];
+            }
+            if (int rc = collect(c, Q.L, q.out_cap, Q.key, Q.key->dtype, Q.key->name, q.out_key, q.out_key_valid,
+                                 vals, valid, q.out_count, &outs[qi], (int64_t)counts[qi]))
+                return rc;
+        }
+    }
+    for (size_t qi = 0; qi < b->qs.size(); ++qi) {  // the rest, one plain call each
+        BatchQ &Q = b->qs[qi];
+        if (!Q.fallback) continue;
+        if (int rc = fgpu_groupby(c, Q.t, Q.has_pred ? &Q.pred : nullptr, &Q.spec, &outs[qi])) return rc;
+    }
+    return FGPU_OK;
+}
+
+const char *fgpu_batch_info(const fgpu_batch *b) { return b ? b->info.c_str() : ""; }
+
+int fgpu_batch_fallbacks(const fgpu_batch *b) {
+    int n = 0;
+    if (b)
+        for (const BatchQ &Q : b->qs) n += Q.fallback;
+    return n;
+}
+
+void fgpu_batch_free(fgpu_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->c->device);
+    (void)hipStreamSynchronize(b->c->stream);
+    batch_release(b);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+}
+
+}  // extern "C"
+

@@ -114,3 +114,22 @@ def uniform_i64(n: int, seed_: int, lo: int, hi: int) -> np.ndarray:
     out = np.empty(n, np.int64)
     _ok(lib().fgen_uniform_i64(n, seed_, lo, hi, out.ctypes.data))
     return out
+
+
+def c4_queries(n_queries: int = 256, min_rows: int = 10_000, max_rows: int = 10_000_000,
+               min_card: int = 8, max_card: int = 1_000_000) -> list[dict]:
+    """Per-query shape of the batched workload: rows log-uniform in
+    [min_rows, max_rows], key cardinality uniform in [min_card, max_card] and
+    the predicate threshold `score < theta` with theta uniform in [0, 100).
+    Batch i's columns are c4_batch(rows, card, seed(4, 16 + i))."""
+    u_rows = uniform_f64(n_queries, seed(4, 0))
+    u_card = uniform_f64(n_queries, seed(4, 1))
+    u_theta = uniform_f64(n_queries, seed(4, 2))
+    lo, hi = np.log(min_rows), np.log(max_rows)
+    out = []
+    for i in range(n_queries):
+        rows = int(np.floor(np.exp(lo + (hi - lo) * u_rows[i])))
+        card = int(min_card + np.floor(u_card[i] * (max_card - min_card + 1)))
+        out.append({"rows": max(rows, 1), "card": min(card, max_card), "theta": float(100.0 * u_theta[i]),
+                    "seed": seed(4, 16 + i)})
+    return out

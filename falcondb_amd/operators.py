@@ -19,7 +19,9 @@ src/query/aggregate.rs:3390-3420).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 from typing import Iterable, Sequence
 
@@ -50,12 +52,28 @@ def _check(rc: int) -> None:
 
 
 # ─── context ─────────────────────────────────────────────────────────────────
+# Handles still open at interpreter exit are released by an atexit hook, while
+# the library is still loaded (module teardown may already have dropped it by
+# the time __del__ runs): batches first, then contexts.
+_LIVE_BATCHES: "weakref.WeakSet" = weakref.WeakSet()
+_LIVE_CONTEXTS: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_at_exit():
+    for b in list(_LIVE_BATCHES):
+        b.free()
+    for c in list(_LIVE_CONTEXTS):
+        c.close()
+
+
 class Context:
     """A device, a HIP stream and a scratch arena (fgpu_open)."""
 
     def __init__(self, device: int = 0, timing: bool = False):
         h = C.c_void_p()
         _check(_lib().fgpu_open(device, C.byref(h)))
+        _LIVE_CONTEXTS.add(self)
         self._h = h
         self.device = device
         if timing:
@@ -471,6 +489,64 @@ class GroupByExec:
 
     def execute(self, ctx: Context, table: Table, filter: Expr | None = None) -> RecordBatch:
         return _groupby(ctx, table, filter, table.column_index(self.key), self.aggs)
+
+
+class BatchGroupBy:
+    """Many GROUP BY queries run together (fgpu_batch_*): prepared once, then
+    each run() is one replay of a captured graph of two persistent kernels.
+    Query i is (table, key column, aggregates, WHERE); its result is the one
+    GroupByExec gives (src/query/aggregate.rs:535).  The prepare / run split
+    follows the reference's streaming operator shape (pipeline::Operator
+    process / flush, src/query/pipeline.rs:46-72)."""
+
+    def __init__(self, ctx: Context, queries: Sequence[tuple]):
+        self.ctx = ctx
+        self._keep = []
+        self._funcs = []
+        qs = (_abi.BatchQuery * max(1, len(queries)))()
+        for i, (table, key, aggs, filt) in enumerate(queries):
+            p, keep = _pred(filt, table)
+            arr, keep2 = _aggs_struct(list(aggs), lambda side, name, t=table: t.column_index(name))
+            spec = _abi.GroupBySpec(table.column_index(key), len(aggs), C.cast(arr, C.POINTER(_abi.Agg)))
+            self._keep += [p, keep, arr, keep2, spec]
+            qs[i].table = table.handle
+            qs[i].pred = C.addressof(keep[0]) if keep else None
+            qs[i].spec = C.cast(C.pointer(spec), C.c_void_p)
+            self._funcs.append([None] + [a.func for a in aggs])
+        self.n = len(queries)
+        self.handle = C.c_void_p()
+        _check(_lib().fgpu_batch_prepare(ctx.handle, qs, self.n, C.byref(self.handle)))
+        _LIVE_BATCHES.add(self)
+
+    def tune(self, max_trials: int = 0) -> str:
+        _check(_lib().fgpu_batch_tune(self.handle, max_trials))
+        return self.info()
+
+    def info(self) -> str:
+        return _lib().fgpu_batch_info(self.handle).decode()
+
+    def fallbacks(self) -> int:
+        return _lib().fgpu_batch_fallbacks(self.handle)
+
+    def run_raw(self) -> list:
+        """Runs the batch and returns the raw result handles (caller frees)."""
+        outs = (C.c_void_p * max(1, self.n))()
+        _check(_lib().fgpu_batch_run(self.handle, outs))
+        return [outs[i] for i in range(self.n)]
+
+    def run(self) -> list[RecordBatch]:
+        return [_read_result(C.c_void_p(h), f) for h, f in zip(self.run_raw(), self._funcs)]
+
+    def free(self):
+        if self.handle:
+            _lib().fgpu_batch_free(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def execute_agg_only(ctx: Context, table: Table, aggs: Sequence[AggExpr],

@@ -235,6 +235,35 @@ int fgpu_join_pairs(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *pro
                     const fgpu_join_spec *spec, uint32_t *probe_rows,
                     uint32_t *build_rows, int64_t capacity, int64_t *n_out);
 
+/* ── batched GROUP BY ─────────────────────────────────────────────────────
+ * Many independent GROUP BY queries, each over its own device table, prepared
+ * once and then run as one replay of a captured hipGraph: a persistent
+ * pass-1 kernel over (query, row range) units and a persistent pass-2 kernel
+ * over (query, key range) units.  Each query's result is the one
+ * fgpu_groupby gives for it (GroupByExec::execute,
+ * src/query/aggregate.rs:535-1420); the prepare / run split follows the
+ * reference's streaming operator shape (pipeline::Operator process / flush,
+ * src/query/pipeline.rs:46-72).  Integer keys with a dense range take the
+ * batch kernels; other queries run as plain fgpu_groupby calls inside
+ * fgpu_batch_run (fgpu_batch_fallbacks counts them).  The batch copies the
+ * predicate and spec arrays; the tables must outlive the batch. */
+typedef struct fgpu_batch fgpu_batch;
+typedef struct {
+    const fgpu_table *table;
+    const fgpu_pred *pred;           /* NULL = no WHERE */
+    const fgpu_groupby_spec *spec;
+} fgpu_batch_query;
+
+int fgpu_batch_prepare(fgpu_ctx *ctx, const fgpu_batch_query *queries, int n, fgpu_batch **out);
+/* Times up to max_trials configurations (workgroup size, rows per unit, LDS
+ * table size = key-range fan-out) on the batch itself and keeps the fastest. */
+int fgpu_batch_tune(fgpu_batch *b, int max_trials);
+/* outs[i] receives query i's result (free each with fgpu_result_free). */
+int fgpu_batch_run(fgpu_batch *b, fgpu_result **outs);
+const char *fgpu_batch_info(const fgpu_batch *b);  /* chosen configuration, as text */
+int fgpu_batch_fallbacks(const fgpu_batch *b);
+void fgpu_batch_free(fgpu_batch *b);
+
 /* ── results (QueryResult{batches}, src/db.rs:25-27) ─────────────────────── */
 int64_t fgpu_result_rows(const fgpu_result *r);
 int fgpu_result_cols(const fgpu_result *r);

@@ -1,0 +1,106 @@
+"""Batched GROUP BY (fgpu_batch_*): every query of a batch must give exactly
+what the oracle and the plain per-query fgpu_groupby give, on every replay of
+the captured graph and after autotuning."""
+import numpy as np
+import pytest
+
+import falcondb_amd as fa
+import oracle as ora
+from falcondb_amd import datagen as dg
+from falcondb_amd._abi import AVG, COUNT, MAX, MIN, SUM
+from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
+
+from test_gpu_parity import _table, assert_batches_equal
+
+pytestmark = pytest.mark.gpu
+
+C4_AGGS = [AggExpr(COUNT), AggExpr(SUM, "val"), AggExpr(MIN, "val"), AggExpr(MAX, "val"),
+           AggExpr(SUM, "score")]
+
+
+def _mixed_batch(ctx):
+    """(table, key, aggs, filter, columns) for a batch that exercises both
+    pass-1 modes, NULL keys and inputs, a skewed key, and a fallback query."""
+    out = []
+    shapes = [(10_000, 8), (50_000, 300), (120_000, 5_000), (300_000, 60_000), (80_000, 250_000),
+              (1_000_000, 900_000), (20_000, 1)]
+    for i, (n, card) in enumerate(shapes):
+        cols = dg.c4_batch(n, card, dg.seed(4, 100 + i))
+        theta = [37.5, 90.0, 12.0, 50.0, 99.9, 63.0, 100.0][i]
+        out.append((cols, "key", C4_AGGS, col("score") < theta))
+    # int64 keys with an offset range, NULL keys and nullable inputs
+    rng = np.random.default_rng(3)
+    n = 60_000
+    k = rng.integers(10**12, 10**12 + 40_000, n)
+    kv = rng.random(n) > 0.05
+    x = rng.normal(size=n)
+    xv = rng.random(n) > 0.2
+    y = rng.integers(-10**9, 10**9, n)
+    out.append(({"k": NullableColumn(k, kv), "x": NullableColumn(x, xv), "y": y}, "k",
+                [AggExpr(COUNT), AggExpr(COUNT, "x"), AggExpr(SUM, "x"), AggExpr(AVG, "x"),
+                 AggExpr(MIN, "y"), AggExpr(MAX, "y"), AggExpr(AVG, "y")], col("y") > -5 * 10**8))
+    # a skewed key: most rows in one partition, so pass 1 overflows a cell first
+    n = 400_000
+    ks = np.where(rng.random(n) < 0.9, 7, rng.integers(0, 200_000, n)).astype(np.int32)
+    out.append(({"key": ks, "val": rng.integers(-10**6, 10**6, n), "score": rng.random(n) * 100}, "key",
+                C4_AGGS, None))
+    # Utf8 key: not a batch-kernel shape, runs as a plain call inside the batch
+    codes = rng.integers(0, 5, 30_000).astype(np.uint32)
+    out.append(({"g": DictColumn(codes, ["a", "b", "", "c", "d"]), "val": rng.integers(0, 100, 30_000)}, "g",
+                [AggExpr(COUNT), AggExpr(SUM, "val")], col("val") > 10))
+    return [(_table(ctx, cols), key, aggs, filt, cols) for cols, key, aggs, filt in out]
+
+
+def _null_safe_key(rb):
+    """NULL keys sort as INT64_MIN (their stored value is arbitrary)."""
+    if rb.validity[0] is not None and rb.dtypes[0] != "Utf8":
+        rb.columns[0] = np.where(rb.validity[0], rb.columns[0], np.iinfo(np.int64).min)
+    return rb
+
+
+def test_batch_matches_oracle_and_plain_calls(ctx):
+    qs = _mixed_batch(ctx)
+    batch = fa.BatchGroupBy(ctx, [(t, k, a, f) for t, k, a, f, _ in qs])
+    assert batch.fallbacks() == 1, batch.info()
+    for rep in range(3):  # graph replays, with eager plain calls in between
+        res = batch.run()
+        for (t, key, aggs, filt, cols), got in zip(qs, res):
+            plain = fa.GroupByExec([key], aggs).execute(ctx, t, filt)
+            o = ora.groupby(cols, filt, key, aggs)
+            exp = o.exact
+            for rb in (got, plain, exp):
+                _null_safe_key(rb)
+            assert_batches_equal(got, exp, key)
+            assert_batches_equal(got, plain, key)
+    batch.free()
+
+
+def test_batch_autotune_keeps_results(ctx):
+    qs = []
+    for i, (n, card) in enumerate([(200_000, 1_000), (500_000, 400_000), (30_000, 20), (700_000, 70_000)]):
+        cols = dg.c4_batch(n, card, dg.seed(4, 200 + i))
+        qs.append((_table(ctx, cols), "key", C4_AGGS, col("score") < 25.0 + 20 * i, cols))
+    batch = fa.BatchGroupBy(ctx, [(t, k, a, f) for t, k, a, f, _ in qs])
+    before = batch.run()
+    info = batch.tune(6)
+    assert "tuned=" in info, info
+    after = batch.run()
+    for (t, key, aggs, filt, cols), b0, b1 in zip(qs, before, after):
+        assert_batches_equal(b1, b0, key)
+        assert_batches_equal(b1, ora.groupby(cols, filt, key, aggs).exact, key)
+    batch.free()
+
+
+def test_batch_of_c4_shapes_small(ctx):
+    """The C4 recipe at a hundredth of its row counts: 64 queries."""
+    qdesc = dg.c4_queries(64, min_rows=1_000, max_rows=100_000)
+    qs = []
+    for q in qdesc:
+        cols = dg.c4_batch(q["rows"], q["card"], q["seed"])
+        qs.append((_table(ctx, cols), "key", C4_AGGS, col("score") < q["theta"], cols))
+    batch = fa.BatchGroupBy(ctx, [(t, k, a, f) for t, k, a, f, _ in qs])
+    assert batch.fallbacks() == 0
+    res = batch.run()
+    for (t, key, aggs, filt, cols), got in zip(qs, res):
+        assert_batches_equal(got, ora.groupby(cols, filt, key, aggs).exact, key)
+    batch.free()

@@ -104,6 +104,8 @@ DECLARED = {
     "fgpu_join_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.c_void_p, C.c_void_p,
                                   C.POINTER(JoinSpec), C.c_void_p, C.c_void_p, C.c_int64,
                                   C.POINTER(C.c_int64)]),
+    "fgpu_topk": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.c_int32, C.c_int, C.c_int32,
+                            C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
     "fgpu_batch_prepare": (C.c_int, [C.c_void_p, C.POINTER(BatchQuery), C.c_int, C.POINTER(C.c_void_p)]),
     "fgpu_batch_tune": (C.c_int, [C.c_void_p, C.c_int]),
     "fgpu_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

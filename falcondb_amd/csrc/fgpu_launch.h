@@ -285,6 +285,11 @@ hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchAr
                               size_t lds);
 hipError_t batch_set_lds(size_t lds1, size_t lds2);
 
+// ─── ORDER BY ... LIMIT k ──────────────────────────────────────────────────
+// cands: grid * K pairs of 16 bytes; out: K pairs (best first).
+hipError_t launch_topk(hipStream_t s, const DCols &c, const DPred &p, int col, int desc, int null_mode, int64_t n,
+                       int K, int grid, void *cands, void *out);
+
 // Kernel arguments travel in the 4 KiB kernarg segment; descriptor fields are
 // 32-bit so that the kernels read them with scalar loads.
 static_assert(sizeof(GbPartArgs) <= 4096, "GbPartArgs exceeds the kernarg segment");

@@ -107,7 +107,7 @@ struct fgpu_result {
 // ─── context ────────────────────────────────────────────────────────────────
 enum BufSlot {
     B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT,
-    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_NSLOTS
+    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_NSLOTS
 };
 
 struct Tuning {
@@ -1980,6 +1980,69 @@ int batch_execute(fgpu_batch *b, std::vector<unsigned long long> &counts) {
 }  // namespace
 
 extern "C" {
+
+int fgpu_topk(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int32_t col, int desc, int32_t k,
+              int64_t *rows_out, int64_t *vals_out, int64_t *n_out) {
+    if (!c || !t || !n_out || (k > 0 && !rows_out)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    if (k < 0) return set_err(FGPU_E_INVALID_QUERY, "negative LIMIT");
+    if (k > 32) return set_err(FGPU_E_UNSUPPORTED, "LIMIT %d: the device top-k takes at most 32 rows", k);
+    if (col < 0 || col >= (int)t->cols.size()) return set_err(FGPU_E_COLUMN_NOT_FOUND, "sort column %d not in table", col);
+    const DevCol &dc = t->cols[col];
+    if (dc.dtype == FGPU_UTF8) return set_err(FGPU_E_UNSUPPORTED, "ORDER BY a Utf8 column");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    *n_out = 0;
+    if (k == 0 || t->n_rows == 0) return FGPU_OK;
+    Binder b(t);
+    DPred dp;
+    if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    const int slot = b.bind(col);
+    if (slot < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads too many columns");
+    const int K = k <= 8 ? 8 : k <= 16 ? 16 : 32;
+    const int grid = c->num_cus * 4;
+    void *mem;
+    const size_t cand_bytes = (size_t)grid * K * 16;
+    if (int rc = buf(c, B_TOPK, 2 * (cand_bytes + (size_t)K * 16), &mem)) return rc;
+    char *m = (char *)mem;
+    void *cands = m, *out_keys = m + cand_bytes, *cands_n = m + cand_bytes + K * 16,
+         *out_nulls = m + 2 * cand_bytes + K * 16;
+    const bool nulls = dc.null_count > 0;
+    mark(c, 0);
+    mark(c, 1);
+    HIPCHK(launch_topk(c->stream, b.dc, dp, slot, desc, 0, t->n_rows, K, grid, cands, out_keys));
+    if (nulls) HIPCHK(launch_topk(c->stream, b.dc, dp, slot, desc, 1, t->n_rows, K, grid, cands_n, out_nulls));
+    mark(c, 2);
+    std::vector<int64_t> h(4 * (size_t)K);  // (key, row) pairs
+    HIPCHK(hipMemcpyAsync(h.data(), out_keys, (size_t)K * 16, hipMemcpyDeviceToHost, c->stream));
+    if (nulls) HIPCHK(hipMemcpyAsync(h.data() + 2 * K, out_nulls, (size_t)K * 16, hipMemcpyDeviceToHost, c->stream));
+    mark(c, 3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // NULL rows last for DESC, first for ASC (SortExpr::desc / asc, src/query/sort.rs:28-34)
+    // a key is a bijection of the value: undo the order transform
+    auto value_of = [&](uint64_t key) -> int64_t {
+        const uint64_t o = desc ? ~key : key;
+        if (dc.dtype == FGPU_FLOAT64) return (int64_t)((o >> 63) ? o ^ (1ull << 63) : ~o);
+        return (int64_t)(o ^ (1ull << 63));
+    };
+    std::vector<std::pair<int64_t, int64_t>> keyed, nullr;  // (row, value)
+    for (int j = 0; j < K; ++j) {
+        if (h[2 * j + 1] != INT64_MAX) keyed.emplace_back(h[2 * j + 1], value_of((uint64_t)h[2 * j]));
+        if (nulls && h[2 * K + 2 * j + 1] != INT64_MAX) nullr.emplace_back(h[2 * K + 2 * j + 1], 0);
+    }
+    std::vector<std::pair<int64_t, int64_t>> rows;
+    if (!desc) rows.insert(rows.end(), nullr.begin(), nullr.end());
+    rows.insert(rows.end(), keyed.begin(), keyed.end());
+    if (desc) rows.insert(rows.end(), nullr.begin(), nullr.end());
+    if ((int)rows.size() > k) rows.resize((size_t)k);
+    for (size_t j = 0; j < rows.size(); ++j) {
+        rows_out[j] = rows[j].first;
+        if (vals_out) vals_out[j] = rows[j].second;
+    }
+    *n_out = (int64_t)rows.size();
+    finish_timing(c, t->n_rows * (int64_t)dc.elem_bytes(), false);
+    c->plan = "topk";
+    return FGPU_OK;
+}
 
 int fgpu_batch_prepare(fgpu_ctx *c, const fgpu_batch_query *queries, int n, fgpu_batch **out) {
     if (!c || !out || n < 0 || (n > 0 && !queries)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");

@@ -549,6 +549,25 @@ class BatchGroupBy:
             pass
 
 
+def execute_sort_limit(ctx: Context, table: Table, column: str, limit: int, desc: bool = False,
+                       filter: Expr | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """ORDER BY column [DESC] LIMIT limit over the rows that pass `filter`
+    (execute_sort_limit, src/query/sort.rs:59-115).  Returns (row indices,
+    values of the column) in output order; values of NULL rows are NaN for
+    Float64 and 0 for integers, and their row indices say which they are."""
+    p, keep = _pred(filter, table)
+    rows = np.zeros(max(limit, 1), np.int64)
+    vals = np.zeros(max(limit, 1), np.int64)
+    n = C.c_int64()
+    ci = table.column_index(column)
+    _check(_lib().fgpu_topk(ctx.handle, table.handle, p, ci, int(desc), limit, rows.ctypes.data,
+                            vals.ctypes.data, C.byref(n)))
+    rows, vals = rows[: n.value], vals[: n.value]
+    if table.dtypes[ci] == FLOAT64:
+        vals = vals.view(np.float64).copy()
+    return rows, vals
+
+
 def execute_agg_only(ctx: Context, table: Table, aggs: Sequence[AggExpr],
                      filter: Expr | None = None) -> RecordBatch:
     """Aggregates without GROUP BY: one row (src/query/aggregate.rs:1394)."""

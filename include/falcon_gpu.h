@@ -235,6 +235,16 @@ int fgpu_join_pairs(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *pro
                     const fgpu_join_spec *spec, uint32_t *probe_rows,
                     uint32_t *build_rows, int64_t capacity, int64_t *n_out);
 
+/* ── ORDER BY col [ASC | DESC] LIMIT k (execute_sort_limit,
+ * src/query/sort.rs:59-115; typed top-k :209-330) ───────────────────────────
+ * rows_out receives the indices of the k rows in output order: the column's
+ * total order (f64_ord_key, :452-454) reversed for DESC, ties by row, NULL
+ * rows last for DESC and first for ASC.  vals_out (may be NULL) receives the
+ * sort column's values as int64 / double bit patterns (0 for NULL rows).
+ * k <= 32 on the device; larger limits return FGPU_E_UNSUPPORTED. */
+int fgpu_topk(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p, int32_t col, int desc,
+              int32_t k, int64_t *rows_out, int64_t *vals_out, int64_t *n_out);
+
 /* ── batched GROUP BY ─────────────────────────────────────────────────────
  * Many independent GROUP BY queries, each over its own device table, prepared
  * once and then run as one replay of a captured hipGraph: a persistent

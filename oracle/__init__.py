@@ -47,6 +47,7 @@ def lib():
         L.ora_sum_lanes.argtypes = [vp, i64]
         L.ora_filter_mask.argtypes = [vp, C.c_int, i64, i32, vp, vp]
         L.ora_filter_count.argtypes = [vp, C.c_int, i64, i32, vp, C.POINTER(i64)]
+        L.ora_topk.argtypes = [vp, C.c_int, i64, i32, vp, C.c_int, C.c_int, i64, vp, C.POINTER(i64)]
         L.ora_filter_select.argtypes = [vp, C.c_int, i64, i32, vp, vp, C.c_int, C.POINTER(vp)]
         L.ora_groupby.argtypes = [vp, C.c_int, i64, i32, vp, vp, C.c_int, C.POINTER(vp)]
         L.ora_join.argtypes = [vp, C.c_int, i64, vp, C.c_int, i64, i32, vp, vp, C.c_int,
@@ -179,6 +180,18 @@ def filter_mask(columns: dict, expr: Expr | None, batch_rows: int = 65536) -> np
     out = np.zeros((c.n + 63) // 64, np.uint64)
     _check(lib().ora_filter_mask(c.descs, len(c.names), c.n, batch_rows, p, out.ctypes.data))
     return out
+
+
+def topk(columns: dict, expr: Expr | None, column: str, k: int, desc: bool = True,
+         batch_rows: int = 65536) -> np.ndarray:
+    """Row indices of ORDER BY column [DESC] LIMIT k over the rows that pass."""
+    c = _Cols(columns)
+    p, keep = c.pred(expr)
+    out = np.zeros(max(k, 1), np.int64)
+    n = C.c_int64()
+    _check(lib().ora_topk(c.descs, len(c.names), c.n, batch_rows, p, c.column_index(column), int(desc), k,
+                          out.ctypes.data, C.byref(n)))
+    return out[: n.value]
 
 
 def filter_count(columns: dict, expr: Expr | None, batch_rows: int = 65536) -> int:

@@ -1234,6 +1234,57 @@ int ora_join(const fgpu_column_desc *pcd, int pncols, int64_t np,
     return FGPU_OK;
 }
 
+// ORDER BY col [ASC | DESC] LIMIT k over the rows that pass p
+// (execute_sort_limit, src/query/sort.rs:59-115; try_typed_sort :209-330).
+// Keys: the column's total order (integers with the sign bit flipped, floats
+// through f64_ord_key :452-454), inverted for DESC; ties by row; NULL rows
+// last for DESC and first for ASC (SortExpr::asc / desc, :28-34), the first
+// NULL rows in row order (NullSink :627-641; assemble :773-790).
+int ora_topk(const fgpu_column_desc *cd, int ncols, int64_t n, int32_t batch_rows, const fgpu_pred *p,
+             int col, int desc, int64_t k, int64_t *rows_out, int64_t *n_out) {
+    if (batch_rows <= 0 || batch_rows % 64) return fail(FGPU_E_INVALID_QUERY, "batch_rows must be a positive multiple of 64");
+    auto cols = make_cols(cd, ncols);
+    if (col < 0 || col >= (int)cols.size()) return fail(FGPU_E_COLUMN_NOT_FOUND, "sort column not found");
+    const Col &c = cols[col];
+    if (c.dtype == FGPU_UTF8) return fail(FGPU_E_UNSUPPORTED, "ORDER BY a Utf8 column");
+    Pred pred;
+    if (!build_pred(p, cols, pred)) return fail(FGPU_E_INVALID_QUERY, "malformed predicate");
+    std::vector<std::pair<uint64_t, int64_t>> pairs;
+    std::vector<int64_t> nulls;
+    std::vector<uint64_t> m;
+    for (int64_t b = 0; b < n; b += batch_rows) {
+        const int64_t e = std::min(n, b + (int64_t)batch_rows);
+        eval_mask(pred, cols, b, e, m);
+        for (int64_t i = b; i < e; ++i) {
+            if (!((m[(i - b) >> 6] >> ((i - b) & 63)) & 1)) continue;
+            if (!c.valid(i)) {
+                nulls.push_back(i);
+                continue;
+            }
+            uint64_t key;
+            if (c.dtype == FGPU_FLOAT64) {
+                uint64_t bits;
+                std::memcpy(&bits, (const double *)c.values + i, 8);
+                key = (bits >> 63) == 0 ? bits ^ (1ull << 63) : ~bits;
+            } else if (c.dtype == FGPU_INT32) {
+                key = (uint64_t)(int64_t)((const int32_t *)c.values)[i] ^ (1ull << 63);
+            } else {
+                key = (uint64_t)((const int64_t *)c.values)[i] ^ (1ull << 63);
+            }
+            pairs.emplace_back(desc ? ~key : key, i);
+        }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    std::vector<int64_t> out;
+    if (!desc) out.insert(out.end(), nulls.begin(), nulls.end());
+    for (const auto &pr : pairs) out.push_back(pr.second);
+    if (desc) out.insert(out.end(), nulls.begin(), nulls.end());
+    if ((int64_t)out.size() > k) out.resize((size_t)k);
+    std::copy(out.begin(), out.end(), rows_out);
+    *n_out = (int64_t)out.size();
+    return FGPU_OK;
+}
+
 void ora_free(void *p) { std::free(p); }
 
 // ─── result accessors ───────────────────────────────────────────────────────

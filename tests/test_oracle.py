@@ -181,3 +181,57 @@ def test_planted_mistake_is_caught():
     good = ora.filter_count(cols, col("n") > 10)
     wrong = ora.filter_count(cols, col("n") >= 10)  # what a GT->GE slip computes
     assert good == closed_form_count("n > 10", 1000) and wrong != good
+
+
+# ─── ORDER BY ... LIMIT k ─────────────────────────────────────────────────────
+def _topk_values(cols, name, rows):
+    v = cols[name]
+    vals = v.values if isinstance(v, NullableColumn) else v
+    valid = v.valid if isinstance(v, NullableColumn) else np.ones(len(vals), bool)
+    return [vals[r] if valid[r] else None for r in rows]
+
+
+def test_topk_reference_known_answers():
+    # tests/sort_and_dedup_paths.rs:107-125: v = 9, 3, 7, 1, 5
+    cols = {"v": np.array([9, 3, 7, 1, 5], np.int64)}
+    assert _topk_values(cols, "v", ora.topk(cols, None, "v", 3, desc=False)) == [1, 3, 5]
+    assert _topk_values(cols, "v", ora.topk(cols, None, "v", 2, desc=True)) == [9, 7]
+    assert sorted(_topk_values(cols, "v", ora.topk(cols, None, "v", 10))) == [1, 3, 5, 7, 9]
+    # tests/top_k_scan.rs:60-95: the limited sort is the head of the full sort
+    n = 1000
+    s = np.array([((i * 7919) % n) + 0.5 for i in range(n)])
+    ok = np.array([i % 97 != 0 for i in range(n)])
+    cols = {"score": NullableColumn(s, ok)}
+    keyed = sorted(s[ok].tolist())
+    for desc in (False, True):
+        full = ([None] * int((~ok).sum()) + keyed) if not desc else (keyed[::-1] + [None] * int((~ok).sum()))
+        for k in (1, 2, 10):
+            assert _topk_values(cols, "score", ora.topk(cols, None, "score", k, desc)) == full[:k]
+    # tests/top_k_scan.rs:124-131: more NULLs than the limit, placed first
+    s = np.arange(n, dtype=np.float64)
+    cols = {"score": NullableColumn(s, np.arange(n) >= n - 3)}
+    assert _topk_values(cols, "score", ora.topk(cols, None, "score", 5, desc=False)) == [None] * 5
+
+
+def test_topk_float_total_order():
+    # f64_ord_key (src/query/sort.rs:452-454) is IEEE 754 totalOrder:
+    # -NaN < -inf < ... < -0.0 < +0.0 < ... < +inf < +NaN
+    neg_nan = np.frombuffer(np.array([0xfff8000000000000], np.uint64).tobytes(), np.float64)[0]
+    v = np.array([1.0, np.nan, -np.inf, np.inf, -0.0, 0.0, neg_nan])
+    order = ora.topk({"v": v}, None, "v", 7, desc=True)
+    assert order.tolist() == [1, 3, 0, 5, 4, 2, 6]
+    assert ora.topk({"v": v}, None, "v", 7, desc=False).tolist() == [6, 2, 4, 5, 0, 3, 1]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_topk_brute_force(seed):
+    rng = np.random.default_rng(400 + seed)
+    n = int(rng.integers(1, 3000))
+    v = rng.integers(-50, 50, n)  # many ties: broken by row
+    f = rng.integers(0, 10, n)
+    cols = {"v": v, "f": f}
+    for desc in (False, True):
+        for k in (1, 7, 32, 5000):
+            rows = [i for i in range(n) if f[i] < 6]
+            exp = sorted(rows, key=lambda i: (-v[i] if desc else v[i], i))[:k]
+            assert ora.topk(cols, col("f") < 6, "v", k, desc).tolist() == exp

@@ -130,6 +130,111 @@ def bench_batched(ctx, lib, steps):
     return out
 
 
+def _serve_stream(srv, reqs, kinds, qps, seed_):
+    """Serves `reqs` on `srv` with Poisson arrivals at `qps` (seeded); returns
+    latency percentiles and throughput.  A waiter thread collects results while
+    the main thread submits on schedule."""
+    import queue as _q
+    import threading
+    rng = np.random.default_rng(seed_)
+    gaps = rng.exponential(1.0 / qps, len(reqs))
+    done, tq = [], _q.Queue()
+    st0 = srv.stats()
+
+    def waiter():
+        while True:
+            t = tq.get()
+            if t is None:
+                return
+            res, st = srv.wait(t)
+            done.append(st)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for r, g in zip(reqs, gaps):
+        due += g
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            time.sleep(min(due - now, 0.002))
+        tq.put(srv.submit(r))
+    tq.put(None)
+    th.join()
+    wall = time.perf_counter() - t0
+    st1 = srv.stats()
+    lat = np.array([d["latency_ms"] for d in done])
+    by_kind = {}
+    for k, d in zip(kinds, done):
+        by_kind.setdefault(k, []).append(d["latency_ms"])
+    return {"qps_offered": qps, "requests": len(reqs), "qps_served": len(reqs) / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "rows_per_s": float(sum(d["rows_in"] for d in done) / wall),
+            "p99_ms_by_kind": {k: float(np.percentile(v, 99)) for k, v in by_kind.items()},
+            "shared_requests": int(st1["shared_requests"] - st0["shared_requests"]),
+            "peak_running": int(st1["peak_running"]), "peak_held_GB": st1["peak_held_bytes"] / 1e9,
+            "failed": int(st1["failed"] - st0["failed"])}
+
+
+def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=16 << 30):
+    """The serving form: a device-resident fact table (the 100M-row group-by
+    table plus a 12-value column g) and the 1M-row dimension table, a request
+    mix of filtered COUNT(*), GROUP BY g / key, INNER / SEMI joins on
+    predicate-selected probe ranges and ORDER BY v2 DESC LIMIT 10, Poisson
+    arrivals at each offered QPS.  The same arrival streams are also served by
+    the plain per-query form: one slot, no shared scans.  Both servers are
+    warmed up first (scratch arenas allocated, request shapes learned)."""
+    import falcondb_amd as fa
+    from falcondb_amd import datagen as dg
+    from falcondb_amd.operators import DictColumn
+    from falcondb_amd.serving import Server, mix_request, run_alone
+    ctx.set_tuning("keep_on_device", 0)  # run_alone reads its results back
+    cols = dg.c5_fact(rows, K)
+    fact = fa.Table(ctx, {"key": cols["key"], "v1": cols["v1"], "v2": cols["v2"], "f": cols["f"],
+                          "g": DictColumn(cols["g"], dg.G_NAMES)})
+    del cols
+    dim = fa.Table(ctx, dg.c3_build(K))
+    idx = fa.JoinIndex(ctx, dim, "bk")
+    served_srv = Server(0, slots=slots, hbm_budget=budget, share_counts=True)
+    plain_srv = Server(0, slots=1, hbm_budget=budget, share_counts=False)
+    # warm-up, and parity: served results equal the requests run alone
+    mix0 = dg.c5_mix(48, seed_=dg.seed(5, 1))
+    reqs0 = [mix_request(sp, fact, dim, idx) for sp in mix0]
+    same = True
+    for srv in (served_srv, plain_srv):
+        ts = [srv.submit(r) for r in reqs0]
+        for sp, r, t in zip(mix0, reqs0, ts):
+            got, _ = srv.wait(t)
+            exp = run_alone(ctx, r)
+            if sp["kind"] == "groupby":
+                got, exp = got.sort_by(sp["key"]), exp.sort_by(sp["key"])
+            same &= got.num_rows == exp.num_rows and all(
+                np.array_equal(a, b) for a, b in zip(got.columns, exp.columns))
+    served, plain = [], []
+    for i, q in enumerate(qps_levels):
+        n = int(min(max(q * 1.5, 60), 600))
+        mix = dg.c5_mix(n, seed_=dg.seed(5, 32 + i))
+        reqs = [mix_request(sp, fact, dim, idx) for sp in mix]
+        kinds = [sp["kind"] for sp in mix]
+        served.append(_serve_stream(served_srv, reqs, kinds, q, 100 + i))
+        plain.append(_serve_stream(plain_srv, reqs, kinds, q, 100 + i))
+        log(f"[bench] C5 qps {q}: served p50 {served[-1]['p50_ms']:.2f} p99 {served[-1]['p99_ms']:.2f} ms "
+            f"{served[-1]['qps_served']:.0f} q/s {served[-1]['rows_per_s'] / 1e9:.1f} G rows/s | plain p50 "
+            f"{plain[-1]['p50_ms']:.2f} p99 {plain[-1]['p99_ms']:.2f} ms {plain[-1]['qps_served']:.0f} q/s "
+            f"{plain[-1]['rows_per_s'] / 1e9:.1f} G rows/s")
+    served_srv.close()
+    plain_srv.close()
+    idx.free()
+    dim.free()
+    fact.free()
+    return {"rows": rows, "dim_rows": int(K.size), "slots": slots, "budget_GB": budget / 1e9,
+            "mix": "40% COUNT(*) WHERE f<t; 30% GROUP BY g|key WHERE f<t; 20% INNER|SEMI join WHERE f<t; "
+                   "10% ORDER BY v2 DESC LIMIT 10",
+            "parity_vs_alone": bool(same), "served": served, "plain_one_slot": plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-join", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--qps", default="100,300,1000,3000")
     ap.add_argument("--cpu-sample-rows", type=int, default=C2_ROWS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -254,6 +361,10 @@ def main():
 
     if not args.no_batch:
         secondary["c4_batched"] = bench_batched(ctx, lib, max(3, args.steps // 4))
+
+    if not args.no_serve:
+        table.free()
+        secondary["c5_serving"] = bench_serving(ctx, K, args.rows, [int(q) for q in args.qps.split(",")])
 
     cpu = None
     if not args.no_cpu:

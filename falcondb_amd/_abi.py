@@ -71,6 +71,29 @@ class Timing(C.Structure):
                 ("finalize_ms", C.c_float), ("bytes_main", C.c_int64)]
 
 
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("table", C.c_void_p), ("pred", C.c_void_p),
+                ("groupby", C.c_void_p), ("build", C.c_void_p), ("index", C.c_void_p),
+                ("join", C.c_void_p), ("topk_col", C.c_int32), ("topk_desc", C.c_int32),
+                ("topk_k", C.c_int32), ("alias", C.c_char_p)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
+                ("rows_in", C.c_int64), ("reserved_bytes", C.c_int64), ("slot", C.c_int32),
+                ("shared", C.c_int32), ("retries", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("shared_launches", C.c_int64), ("shared_requests", C.c_int64),
+                ("retries", C.c_int64), ("trims", C.c_int64), ("peak_held_bytes", C.c_int64),
+                ("peak_running", C.c_int32), ("n_slots", C.c_int32), ("budget_bytes", C.c_int64)]
+
+
+REQ_COUNT, REQ_GROUPBY, REQ_JOIN, REQ_TOPK = range(4)
+
+
 # Every symbol include/falcon_gpu.h declares (the CPU test checks the library
 # exports each of them).
 DECLARED = {
@@ -112,6 +135,13 @@ DECLARED = {
     "fgpu_batch_info": (C.c_char_p, [C.c_void_p]),
     "fgpu_batch_fallbacks": (C.c_int, [C.c_void_p]),
     "fgpu_batch_free": (None, [C.c_void_p]),
+    "fgpu_server_open": (C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
+    "fgpu_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "fgpu_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p),
+                                   C.POINTER(RequestStats)]),
+    "fgpu_server_pause": (C.c_int, [C.c_void_p, C.c_int]),
+    "fgpu_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "fgpu_server_close": (None, [C.c_void_p]),
     "fgpu_result_rows": (C.c_int64, [C.c_void_p]),
     "fgpu_result_cols": (C.c_int, [C.c_void_p]),
     "fgpu_result_column": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ColumnView)]),

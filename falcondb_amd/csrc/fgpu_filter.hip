@@ -185,6 +185,74 @@ __global__ void k_gather(const uint64_t *words, const int64_t *offs, int64_t n_w
     }
 }
 
+// Shared scan: q COUNT(*) ... WHERE <col op literal> requests over one column
+// answered by one pass.  Every loaded value is tested against every request's
+// comparison; per-lane counters are reduced once per workgroup, one atomic per
+// request.  The column is read once for all q requests instead of q times.
+template <int QB>
+__global__ void k_filter_count_shared(SharedCount a) {
+    __shared__ unsigned int part[QB][BLOCK / 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    const int64_t n_words = (a.n + 63) >> 6;
+    const ColRef cr{(const uint8_t *)a.values, a.valid, a.dtype};
+    unsigned int cnt[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) cnt[q] = 0;
+    for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        RawCell raw[FILTER_UNROLL];
+        int64_t v[FILTER_UNROLL];
+        bool ok[FILTER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) {
+            const int64_t r = ((w0 + u) << 6) + lane;
+            raw[u].bits = 0;
+            raw[u].vword = 0;
+            if (r < a.n) cell_issue(cr, r, raw[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) {
+            const int64_t r = ((w0 + u) << 6) + lane;
+            cell_finish(cr, r, raw[u], v[u], ok[u]);
+            ok[u] = ok[u] && r < a.n;
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            if (q < a.q) {
+                const DPredNode nd = a.leaf[q];
+#pragma unroll
+                for (int u = 0; u < FILTER_UNROLL; ++u) {
+                    const int64_t r = ((w0 + u) << 6) + lane;
+                    // IS NULL passes rows whose validity bit is clear; rows past n never pass
+                    cnt[q] += (r < a.n && leaf_reg(nd, v[u], ok[u])) ? 1u : 0u;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        unsigned int c = cnt[q];
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0) part[q][wave] = c;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < a.q) {
+        unsigned long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += part[threadIdx.x][i];
+        if (t) atomicAdd(a.out + threadIdx.x, t);
+    }
+}
+
+hipError_t launch_filter_count_shared(hipStream_t s, const SharedCount &a, int grid) {
+    if (a.q < 1 || a.q > SHARED_MAX_Q) return hipErrorInvalidValue;
+    if (a.q <= 8) hipLaunchKernelGGL(k_filter_count_shared<8>, dim3(grid), dim3(BLOCK), 0, s, a);
+    else if (a.q <= 16) hipLaunchKernelGGL(k_filter_count_shared<16>, dim3(grid), dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL(k_filter_count_shared<SHARED_MAX_Q>, dim3(grid), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_filter_count(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
                                unsigned long long *d_count, int grid) {
     if (p.n_pcols <= 1) hipLaunchKernelGGL(k_filter_count<1>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_count);

@@ -118,6 +118,18 @@ hipError_t launch_filter_count(hipStream_t s, const DCols &c, const DPred &p, in
                                unsigned long long *d_count, int grid);
 hipError_t launch_filter_mask(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
                               uint64_t *d_words, int grid);
+// Shared COUNT(*) scan: q single-comparison predicates on one column.
+constexpr int SHARED_MAX_Q = 32;
+struct SharedCount {
+    int q;
+    int dtype;                 // FGPU_INT32 / FGPU_INT64 / FGPU_FLOAT64
+    const void *values;
+    const uint32_t *valid;     // validity bits as 32-bit words, or null
+    int64_t n;
+    unsigned long long *out;   // q counters (zeroed by the caller)
+    DPredNode leaf[SHARED_MAX_Q];
+};
+hipError_t launch_filter_count_shared(hipStream_t s, const SharedCount &a, int grid);
 // Exclusive prefix sum of per-word popcounts: offs[i] = sum_{j<i} popcount(words[j]);
 // offs[n_words] = total.  `block_tmp` holds ceil(n_words / 4096) + 1 entries.
 hipError_t launch_word_offsets(hipStream_t s, const uint64_t *words, int64_t n_words,

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "fgpu_host.h"
 #include "fgpu_launch.h"
 
 using namespace fgpu;
@@ -152,20 +153,52 @@ struct fgpu_ctx {
     Tuning tune;
     std::string plan;  // strategy of the last call (diagnostics)
     std::mutex mu;
+    // HBM budget of a serving slot: the arena may not grow past arena_limit
+    // bytes (0 = unbounded).  call_req[s] is the largest size buffer s was
+    // asked for since the tally was last reset: the exact scratch a request
+    // needs, which the server learns per request shape.
+    int64_t arena_limit = 0;
+    size_t call_req[B_NSLOTS] = {};
+    bool limit_hit = false;
 };
 
 namespace {
 
+size_t arena_want(size_t bytes) { return std::max<size_t>(bytes + bytes / 4, 4096); }
+
 int buf(fgpu_ctx *c, BufSlot s, size_t bytes, void **out) {
     auto &b = c->bufs[s];
+    c->call_req[s] = std::max(c->call_req[s], bytes);
     if (b.n < bytes) {
+        const size_t want = arena_want(bytes);
+        if (c->arena_limit) {
+            // Buffers this call has not asked for go first; then the limit holds.
+            size_t others = 0;
+            for (int i = 0; i < B_NSLOTS; ++i)
+                if (i != s) others += c->bufs[i].n;
+            if (others + want > (size_t)c->arena_limit) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                for (int i = 0; i < B_NSLOTS; ++i) {
+                    auto &o = c->bufs[i];
+                    if (i == s || c->call_req[i] || !o.p) continue;
+                    HIPCHK(hipFree(o.p));
+                    others -= o.n;
+                    o.p = nullptr;
+                    o.n = 0;
+                }
+            }
+            if (others + want > (size_t)c->arena_limit) {
+                c->limit_hit = true;
+                return set_err(FGPU_E_OUT_OF_MEMORY, "HBM budget: %zu scratch bytes needed, slot limit %lld",
+                               others + want, (long long)c->arena_limit);
+            }
+        }
         if (b.p) {
             HIPCHK(hipStreamSynchronize(c->stream));
             HIPCHK(hipFree(b.p));
             b.p = nullptr;
             b.n = 0;
         }
-        const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
         HIPCHK(hipMalloc(&b.p, want));
         b.n = want;
     }
@@ -2259,3 +2292,132 @@ void fgpu_batch_free(fgpu_batch *b) {
 
 }  // extern "C"
 
+
+// ─── internals shared with the request server (fgpu_host.h) ─────────────────
+namespace fgpu_host {
+
+int set_error(int code, const char *msg) { return set_err(code, "%s", msg); }
+
+void ctx_budget_begin(fgpu_ctx *c, int64_t limit) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->arena_limit = limit;
+    for (auto &r : c->call_req) r = 0;
+    c->limit_hit = false;
+}
+
+int64_t ctx_call_need(fgpu_ctx *c) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    int64_t s = 0;
+    for (size_t r : c->call_req)
+        if (r) s += (int64_t)arena_want(r);
+    return s;
+}
+
+bool ctx_limit_hit(fgpu_ctx *c) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    return c->limit_hit;
+}
+
+int ctx_trim(fgpu_ctx *c) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &b : c->bufs) {
+        if (b.p) HIPCHK(hipFree(b.p));
+        b.p = nullptr;
+        b.n = 0;
+    }
+    return FGPU_OK;
+}
+
+fgpu_result *result_int64(const char *name, int64_t v) {
+    auto *r = new fgpu_result();
+    r->rows = 1;
+    r->cols.emplace_back();
+    r->cols[0].name = name ? name : "col0";
+    r->cols[0].dtype = FGPU_INT64;
+    r->cols[0].i64 = {v};
+    return r;
+}
+
+fgpu_result *result_topk(const int64_t *rows, const int64_t *vals, int64_t n, bool is_f64,
+                         const uint8_t *valid) {
+    auto *r = new fgpu_result();
+    r->rows = n;
+    r->cols.resize(2);
+    r->cols[0].name = "row";
+    r->cols[0].dtype = FGPU_INT64;
+    r->cols[0].i64.assign(rows, rows + n);
+    r->cols[1].name = "value";
+    r->cols[1].dtype = is_f64 ? FGPU_FLOAT64 : FGPU_INT64;
+    if (is_f64) {
+        r->cols[1].f64.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) r->cols[1].f64[i] = __builtin_bit_cast(double, vals[i]);
+    } else {
+        r->cols[1].i64.assign(vals, vals + n);
+    }
+    if (valid) r->cols[1].valid.assign(valid, valid + n);
+    return r;
+}
+
+bool column_is_f64(const fgpu_table *t, int col) {
+    return t && col >= 0 && col < (int)t->cols.size() && t->cols[col].dtype == FGPU_FLOAT64;
+}
+bool column_has_nulls(const fgpu_table *t, int col) {
+    return t && col >= 0 && col < (int)t->cols.size() && t->cols[col].null_count > 0;
+}
+
+int shared_count_column(const fgpu_table *t, const fgpu_pred *p) {
+    if (!t || !p || p->n_nodes != 1 || !p->nodes) return -1;
+    const fgpu_pred_node &nd = p->nodes[0];
+    if (nd.col < 0 || nd.col >= (int)t->cols.size()) return -1;
+    const int dt = t->cols[nd.col].dtype;
+    if (dt != FGPU_INT32 && dt != FGPU_INT64 && dt != FGPU_FLOAT64) return -1;
+    if (nd.kind == FGPU_P_IS_NULL || nd.kind == FGPU_P_IS_NOT_NULL) return nd.col;
+    if (nd.kind != FGPU_P_CMP || nd.lit_type == FGPU_UTF8) return -1;
+    return nd.col;
+}
+
+int filter_count_shared(fgpu_ctx *c, const fgpu_table *t, int col, const fgpu_pred *const *preds, int q,
+                        int64_t *counts) {
+    if (!c || !t || !preds || !counts || q < 1 || q > SHARED_MAX_Q)
+        return set_err(FGPU_E_INVALID_QUERY, "shared COUNT: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    SharedCount a{};
+    a.q = q;
+    for (int i = 0; i < q; ++i) {
+        if (shared_count_column(t, preds[i]) != col)
+            return set_err(FGPU_E_INVALID_QUERY, "shared COUNT: predicate %d does not read column %d alone", i, col);
+        Binder b(t);
+        DPred dp;
+        if (int rc = compile_pred(c, t, preds[i], b, dp)) return rc;
+        a.leaf[i] = dp.nodes[0];
+        a.leaf[i].col = 0;
+        a.leaf[i].pc = 0;
+    }
+    const DevCol &dc = t->cols[col];
+    a.dtype = dc.dtype;
+    a.values = dc.d_values;
+    a.valid = (const uint32_t *)dc.d_valid;
+    a.n = t->n_rows;
+    unsigned long long *d_cnt;
+    if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return rc;
+    a.out = d_cnt;
+    mark(c, 0);
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_cnt, q, 0));
+    mark(c, 1);
+    const int grid = grid_for(c, (t->n_rows + 63) / 64 * 64, BLOCK, c->tune.filter_waves_per_cu);
+    if (t->n_rows) HIPCHK(launch_filter_count_shared(c->stream, a, grid));
+    mark(c, 2);
+    unsigned long long h[SHARED_MAX_Q];
+    HIPCHK(hipMemcpyAsync(h, d_cnt, (size_t)q * 8, hipMemcpyDeviceToHost, c->stream));
+    mark(c, 3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    finish_timing(c, t->n_rows * (int64_t)dc.elem_bytes(), false);
+    for (int i = 0; i < q; ++i) counts[i] = (int64_t)h[i];
+    c->plan = "count:shared q=" + std::to_string(q);
+    return FGPU_OK;
+}
+
+}  // namespace fgpu_host

@@ -133,3 +133,36 @@ def c4_queries(n_queries: int = 256, min_rows: int = 10_000, max_rows: int = 10_
         out.append({"rows": max(rows, 1), "card": min(card, max_card), "theta": float(100.0 * u_theta[i]),
                     "seed": seed(4, 16 + i)})
     return out
+
+
+G_NAMES = [f"g{i:02d}" for i in range(12)]
+
+
+def c5_fact(n: int, K: np.ndarray) -> dict:
+    """Serving fact table: the 100M-row group-by schema (key, v1, v2, f) plus
+    `g`, dictionary codes of 12 values (G_NAMES) drawn uniformly."""
+    cols = c2_table(n, K)
+    cols["g"] = np.minimum(np.floor(uniform_f64(n, seed(5, 4)) * 12), 11).astype(np.uint32)
+    return cols
+
+
+def c5_mix(n_requests: int, seed_: int | None = None) -> list[dict]:
+    """Serving request mix: 40 % `COUNT(*) WHERE f < t`; 30 % GROUP BY g (12
+    groups) or GROUP BY key (high cardinality), each WHERE f < t; 20 % INNER or
+    SEMI join of the fact table (probe, WHERE f < t) with the dimension table;
+    10 % `ORDER BY v2 DESC LIMIT 10`.  t is uniform in [1, 100], so a join
+    probes 1 % .. 100 % of the fact rows."""
+    u = uniform_f64(3 * n_requests, seed(5, 0) if seed_ is None else seed_)
+    out = []
+    for i in range(n_requests):
+        a, b, c = u[3 * i], u[3 * i + 1], u[3 * i + 2]
+        t = 1 + int(np.floor(c * 100))
+        if a < 0.4:
+            out.append({"kind": "count", "t": t})
+        elif a < 0.7:
+            out.append({"kind": "groupby", "key": "g" if b < 0.5 else "key", "t": t})
+        elif a < 0.9:
+            out.append({"kind": "join", "type": "inner" if b < 0.5 else "semi", "t": t})
+        else:
+            out.append({"kind": "topk", "k": 10})
+    return out

@@ -274,6 +274,83 @@ const char *fgpu_batch_info(const fgpu_batch *b);  /* chosen configuration, as t
 int fgpu_batch_fallbacks(const fgpu_batch *b);
 void fgpu_batch_free(fgpu_batch *b);
 
+/* ── serving: continuous batching of mixed requests ───────────────────────
+ * A request server over device-resident tables: the concurrent form of
+ * Database::sql (src/db.rs:242; Database is Sync and the REST server runs one
+ * sql call per worker, src/bin/server.rs:181-203).  The server owns n_slots
+ * contexts (one HIP stream each) and one worker thread per slot.  Requests
+ * queue in arrival order; a free slot admits the oldest request whose scratch
+ * memory fits what the running requests leave of the HBM budget (smaller ones
+ * may pass a waiting large one), so requests that arrive while others run
+ * start as soon as a slot and memory free up, and their kernels overlap on
+ * the device.  COUNT(*) requests that test one numeric column of the same
+ * table with one comparison ride a single shared scan (up to 32 per launch).
+ * A request's scratch need is learned per request shape from its first run
+ * (first guess: a bound from the row counts); a run that outgrows its
+ * reservation is re-queued with a larger one.  Tables and join indexes are
+ * the caller's, outside the budget, and must outlive the requests on them.
+ * Results equal what the plain entry points give for the same request. */
+typedef struct fgpu_server fgpu_server;
+enum {
+    FGPU_REQ_COUNT = 0,    /* COUNT(*) FROM table WHERE pred (src/db.rs:1439-1476) */
+    FGPU_REQ_GROUPBY = 1,  /* fgpu_groupby(table, pred, groupby) */
+    FGPU_REQ_JOIN = 2,     /* fgpu_join_agg(table = probe, pred, build, index, join); index required */
+    FGPU_REQ_TOPK = 3      /* fgpu_topk(table, pred, topk_col, topk_desc, topk_k) */
+};
+typedef struct {
+    int32_t kind;                       /* FGPU_REQ_* */
+    const fgpu_table *table;            /* FROM (probe) table */
+    const fgpu_pred *pred;              /* WHERE on `table`; NULL = none */
+    const fgpu_groupby_spec *groupby;   /* FGPU_REQ_GROUPBY */
+    const fgpu_table *build;            /* FGPU_REQ_JOIN */
+    const fgpu_join_index *index;       /* FGPU_REQ_JOIN */
+    const fgpu_join_spec *join;         /* FGPU_REQ_JOIN */
+    int32_t topk_col, topk_desc, topk_k;  /* FGPU_REQ_TOPK */
+    const char *alias;                  /* FGPU_REQ_COUNT output name; NULL = "col0" */
+} fgpu_request;
+
+/* Per-request record, filled by fgpu_server_wait. */
+typedef struct {
+    double queue_ms;      /* submit -> admitted to a slot */
+    double run_ms;        /* admitted -> result ready */
+    double latency_ms;    /* submit -> result ready */
+    int64_t rows_in;      /* input rows the request reads (probe + build rows for joins) */
+    int64_t reserved_bytes;  /* HBM reserved for it at admission */
+    int32_t slot;         /* slot it ran on */
+    int32_t shared;       /* requests in the launch it rode in (1 = alone) */
+    int32_t retries;      /* re-admissions after outgrowing its reservation */
+} fgpu_request_stats;
+
+/* Server-wide counters. */
+typedef struct {
+    int64_t submitted, completed, failed;
+    int64_t shared_launches;    /* shared COUNT(*) scans */
+    int64_t shared_requests;    /* requests answered by them */
+    int64_t retries;            /* budget re-queues */
+    int64_t trims;              /* idle slot arenas released to admit a request */
+    int64_t peak_held_bytes;    /* most HBM the slots held at once (<= budget) */
+    int32_t peak_running;       /* most requests running at once */
+    int32_t n_slots;
+    int64_t budget_bytes;
+} fgpu_server_stats;
+
+/* share_counts = 0 turns the shared COUNT(*) scan off (every request runs alone). */
+int fgpu_server_open(int device, int n_slots, int64_t hbm_budget_bytes, int share_counts,
+                     fgpu_server **out);
+/* Copies the request (predicate, specs and strings) and queues it; returns at once. */
+int fgpu_server_submit(fgpu_server *s, const fgpu_request *req, uint64_t *ticket);
+/* Blocks until the request is done.  Returns its status; on success *out is the
+ * result (free with fgpu_result_free).  stats may be NULL.  Each ticket is
+ * waited for once. */
+int fgpu_server_wait(fgpu_server *s, uint64_t ticket, fgpu_result **out, fgpu_request_stats *stats);
+/* on = 1 holds admissions (queued requests wait; running ones finish); 0 resumes.
+ * Closing a paused server still finishes its queue. */
+int fgpu_server_pause(fgpu_server *s, int on);
+int fgpu_server_get_stats(fgpu_server *s, fgpu_server_stats *out);
+/* Finishes the queued requests, then stops the workers and frees the slots.
+ * Results nobody waited for are freed. */
+void fgpu_server_close(fgpu_server *s);
+
 /* ── results (QueryResult{batches}, src/db.rs:25-27) ─────────────────────── */
 int64_t fgpu_result_rows(const fgpu_result *r);
 int fgpu_result_cols(const fgpu_result *r);

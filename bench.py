@@ -235,6 +235,25 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=16 << 30):
             "parity_vs_alone": bool(same), "served": served, "plain_one_slot": plain}
 
 
+def calibrate_copy(n_bytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size (the HIP runtime's copy
+    kernel), for scaling FETCH_SIZE / WRITE_SIZE counter readings: profiled
+    beside the operator kernels, it says what the counters read per byte moved."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n_bytes)):
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n_bytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, C.c_size_t(n_bytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"calibration_copy_bytes": n_bytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,7 +266,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--qps", default="100,300,1000,3000")
     ap.add_argument("--cpu-sample-rows", type=int, default=C2_ROWS)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)

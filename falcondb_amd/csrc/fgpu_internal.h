@@ -347,6 +347,30 @@ FGPU_DEV void cell_finish(const ColRef &c, int64_t r, const RawCell &o, int64_t 
     ok = (((o.vword >> (r & 31)) & 1u) != 0) & ((c.dtype != FGPU_UTF8) | ((uint32_t)o.bits != NULL_CODE));
 }
 
+// Branch-free form of cell_issue / cell_finish: exactly three dword loads per
+// cell whatever the column's width and validity (a 4-byte column re-reads its
+// word as the high half, a column without NULLs re-reads the value word as
+// its validity word), so a loop that keeps several cells in flight has a
+// fixed number of loads outstanding and the compiler's waits stay exact.
+struct RawCell3 {
+    uint32_t lo, hi, vw;
+};
+FGPU_DEV void cell_issue3(const ColRef &c, int64_t r, RawCell3 &o) {
+    const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;
+    const uint32_t *lo = (const uint32_t *)c.base + (wide ? 2 * r : r);
+    const uint32_t *vp = c.valid ? c.valid + (r >> 5) : lo;
+    o.lo = lo[0];
+    o.hi = *(wide ? lo + 1 : lo);
+    o.vw = *vp;
+}
+FGPU_DEV void cell_finish3(const ColRef &c, int64_t r, const RawCell3 &o, int64_t &v, bool &ok) {
+    const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;
+    const uint64_t bits = wide ? ((uint64_t)o.hi << 32 | o.lo) : (uint64_t)o.lo;
+    v = c.dtype == FGPU_INT32 ? (int64_t)(int32_t)o.lo : (int64_t)bits;
+    const uint32_t vw = c.valid ? o.vw : 0xffffffffu;
+    ok = (((vw >> (r & 31)) & 1u) != 0) & ((c.dtype != FGPU_UTF8) | ((uint32_t)bits != NULL_CODE));
+}
+
 template <int NPC>
 FGPU_DEV int64_t pickn(const int64_t (&v)[NPC], int j) {
     int64_t r = v[0];

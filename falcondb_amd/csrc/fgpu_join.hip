@@ -19,6 +19,9 @@
 // columns, then the keys, then the first slot of every row are loaded before
 // any of them is examined, so each wave has JOIN_UNROLL independent random
 // slot loads in flight.
+#include <mutex>
+#include <vector>
+
 #include "fgpu_state.h"
 
 namespace fgpu {
@@ -267,31 +270,39 @@ hipError_t launch_join_probe_agg(hipStream_t s, const JoinProbeArgs &a, int grid
 // ─── radix-partitioned join ──────────────────────────────────────────────────
 FGPU_DEV int part_of(uint64_t h, int log_p) { return log_p ? (int)(h >> (64 - log_p)) : 0; }
 
+// The partitioned join works on h = hash64(key) instead of the key: hash64 is
+// a bijection of 64-bit values, so equal hashes are equal keys.  Sub-table
+// slots and pass-1 records hold h; the partition is h's top bits and the slot
+// its low bits.  The one key whose hash is the empty-slot marker EMPTY_KEY
+// (the "sentinel key") is kept out of the sub-tables: its build rows go to a
+// side list and its probe rows to a side partition.
+FGPU_DEV int64_t build_key_at(const JoinBuildArgs &b, int64_t r) {
+    return b.key_dtype == FGPU_INT32 ? (int64_t)((const int32_t *)b.keys_col)[r] : ((const int64_t *)b.keys_col)[r];
+}
+
 __global__ void k_jp_build(JoinPartArgs a, JoinBuildArgs b) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const uint64_t tmask = (uint64_t)a.T - 1;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.n; r += stride) {
         if (b.key_valid && !((b.key_valid[r >> 6] >> (r & 63)) & 1)) continue;  // NULL joins nothing
-        const int64_t key = b.key_dtype == FGPU_INT32 ? (int64_t)((const int32_t *)b.keys_col)[r]
-                                                      : ((const int64_t *)b.keys_col)[r];
-        if (key == EMPTY_KEY) {
+        const int64_t h = (int64_t)hash64((uint64_t)build_key_at(b, r));
+        if (h == EMPTY_KEY) {
             const unsigned long long k = atomicAdd(a.flags + 2, 1ull);
             if ((int64_t)k < a.min_key_cap) a.min_key_rows[k] = (uint32_t)r;
             continue;
         }
-        const uint64_t h = hash64((uint64_t)key);
-        const int p = part_of(h, a.log_p);  // k_jp_count checked every sub-table has room
+        const int p = part_of((uint64_t)h, a.log_p);  // k_jp_count checked every sub-table has room
         const int64_t pay = a.build_pay_col >= 0 ? load_raw(a.pa.bcols, a.build_pay_col, r) : r;
         int64_t *sub = a.sub + (int64_t)p * a.T * 2;
-        uint64_t slot = h & tmask;
+        uint64_t slot = (uint64_t)h & tmask;
         bool done = false;
         for (int q = 0; q < a.T && !done; ++q) {
             int64_t expect = EMPTY_KEY;
-            if (__hip_atomic_compare_exchange_strong(sub + 2 * slot, &expect, key, __ATOMIC_RELAXED,
+            if (__hip_atomic_compare_exchange_strong(sub + 2 * slot, &expect, h, __ATOMIC_RELAXED,
                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 sub[2 * slot + 1] = pay;
                 done = true;
-            } else if (expect == key) {
+            } else if (expect == h) {
                 atomicExch(a.flags + 3, 1ull);  // duplicate build key
             }
             slot = (slot + 1) & tmask;
@@ -309,10 +320,9 @@ __global__ void k_jp_count(JoinPartArgs a, JoinBuildArgs b) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.n; r += stride) {
         if (b.key_valid && !((b.key_valid[r >> 6] >> (r & 63)) & 1)) continue;
-        const int64_t key = b.key_dtype == FGPU_INT32 ? (int64_t)((const int32_t *)b.keys_col)[r]
-                                                      : ((const int64_t *)b.keys_col)[r];
-        if (key == EMPTY_KEY) continue;
-        atomicAdd(hist + part_of(hash64((uint64_t)key), a.log_p), 1u);
+        const uint64_t h = hash64((uint64_t)build_key_at(b, r));
+        if ((int64_t)h == EMPTY_KEY) continue;
+        atomicAdd(hist + part_of(h, a.log_p), 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < P; i += blockDim.x)
@@ -329,76 +339,164 @@ hipError_t launch_join_part_build(hipStream_t s, const JoinPartArgs &a, const Jo
     return hipGetLastError();
 }
 
-// Pass 1: rows with a NULL key or the sentinel key are joined here directly;
-// every other selected row becomes a record in cell (workgroup, partition).
-template <int TYPE, int NPC>
-__global__ void k_jp_scatter(JoinPartArgs a) {
-    extern __shared__ unsigned int cur[];  // one cursor per partition
-    __shared__ uint64_t lrow[64];
+// Block-wide exclusive scan of hist[0..n) into off[0..n); returns the total.
+// Thread t sums a run of ceil(n / blockDim) entries, the runs are scanned
+// across the block, then written back.  `tmp` holds blockDim / 64 words of
+// LDS.  Every thread must call it.
+FGPU_DEV unsigned block_scan_hist(const unsigned *hist, unsigned *off, int n, unsigned *tmp) {
+    const int nt = blockDim.x, t = threadIdx.x;
+    const int per = (n + nt - 1) / nt;
+    const int first = t * per, last = first + per < n ? first + per : n;
+    unsigned run = 0;
+    for (int i = first; i < last; ++i) run += hist[i];
+    const int lane = t & 63, wave = t >> 6, nw = nt >> 6;
+    unsigned incl = run;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) tmp[wave] = incl;
+    __syncthreads();
+    unsigned before = 0, total = 0;
+    for (int i = 0; i < nw; ++i) {
+        before += i < wave ? tmp[i] : 0u;
+        total += tmp[i];
+    }
+    unsigned x = before + incl - run;
+    for (int i = first; i < last; ++i) {
+        off[i] = x;
+        x += hist[i];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Pass 1: pure partitioning.  Every selected probe row becomes a record (h =
+// hash64(key), then the probe payloads) of partition part_of(h); rows with a
+// NULL key go to partition P and rows with the sentinel key to partition
+// P + 1 (pass 2 gives them their join semantics).  A tile of JS_TILE rows is
+// counted per partition in LDS, staged in LDS in partition order and written
+// to the workgroup's cells in runs: consecutive threads write consecutive
+// records of one partition, so stores leave in whole cache lines.  The loads
+// of tile t + 1 are issued before tile t is staged (every load is issued, a
+// row past the range re-reads the range's last row), so they overlap the
+// LDS phases and the stores of tile t.
+template <int NPC>
+struct ScatterRaw {
+    RawCell3 pc[JS_U][NPC > 0 ? NPC : 1];
+    RawCell3 key[JS_U];
+    RawCell3 f1[JS_U], f2[JS_U];
+};
+
+template <int NPC, int RW>
+__global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
+    extern __shared__ uint64_t js_lds[];
     const JoinProbeArgs &pa = a.pa;
-    const int P = 1 << a.log_p;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) cur[i] = 0;
-    JoinAcc j;
-    acc_init(pa, j, lrow);  // synchronises the workgroup
+    const int P2 = (1 << a.log_p) + 2;
+    const int Cw = a.Cw, log_p = a.log_p;
+    int64_t *stage = (int64_t *)js_lds;                           // JS_TILE * RW words
+    unsigned *hist = (unsigned *)(stage + (size_t)JS_TILE * RW);  // P2
+    unsigned *off = hist + P2;                                    // P2
+    unsigned *cur = off + P2;                                     // P2: records in this workgroup's cells
+    unsigned *tmp = cur + P2;                                     // 16
+    uint16_t *spart = (uint16_t *)(tmp + 16);                     // JS_TILE
+    const ColRef kc = col_ref(pa.pcols, pa.pkey);
+    ColRef pc[NPC > 0 ? NPC : 1];
+#pragma unroll
+    for (int j = 0; j < (NPC > 0 ? NPC : 1); ++j)
+        pc[j] = col_ref(pa.pcols, pa.pred.pcol[j < pa.pred.n_pcols ? j : 0]);
+    // payload columns have no NULLs (the host checks), so no validity word is read
+    const ColRef pay0 = col_ref(pa.pcols, a.pay_col[0]), pay1 = col_ref(pa.pcols, a.pay_col[1]);
+    for (int i = threadIdx.x; i < P2; i += blockDim.x) hist[i] = cur[i] = 0;
+    __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_wg;
     const int64_t r1 = r0 + a.rows_per_wg < pa.n_probe ? r0 + a.rows_per_wg : pa.n_probe;
-    int64_t *cells = a.cells + (int64_t)blockIdx.x * P * a.Cw * a.RW;
-    for (int64_t base = r0; base < r1; base += (int64_t)blockDim.x * JOIN_UNROLL) {
-        int64_t rows[JOIN_UNROLL], key[JOIN_UNROLL], f1[JOIN_UNROLL], f2[JOIN_UNROLL];
-        bool sel[JOIN_UNROLL], has[JOIN_UNROLL];
+    // cells are partition-major: cell (w, p) holds Cw records at ((p * G + w) * Cw) * RW
+    const int64_t G = a.G;
+    auto issue = [&](int64_t base, ScatterRaw<NPC> &t) {
 #pragma unroll
-        for (int u = 0; u < JOIN_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
-        pred_tile<JOIN_UNROLL, NPC>(pa.pred, pa.pcols, rows, r1, sel);
+        for (int u = 0; u < JS_U; ++u) {
+            int64_t r = base + u * JS_TPB + threadIdx.x;
+            r = r < r1 ? r : r1 - 1;
 #pragma unroll
-        for (int u = 0; u < JOIN_UNROLL; ++u) {
-            key[u] = f1[u] = f2[u] = 0;
-            bool ok = false;
-            if (sel[u]) {
-                load_cell(pa.pcols, pa.pkey, rows[u], key[u], ok);
-                if (a.n_pay > 0) f1[u] = load_raw(pa.pcols, a.pay_col[0], rows[u]);
-                if (a.n_pay > 1) f2[u] = load_raw(pa.pcols, a.pay_col[1], rows[u]);
-            }
-            has[u] = sel[u] && ok;
+            for (int j = 0; j < NPC; ++j) cell_issue3(pc[j], r, t.pc[u][j]);
+            cell_issue3(kc, r, t.key[u]);
+            if (RW > 1) cell_issue3(pay0, r, t.f1[u]);
+            if (RW > 2) cell_issue3(pay1, r, t.f2[u]);
         }
+    };
+    ScatterRaw<NPC> raw;
+    if (r0 < r1) issue(r0, raw);
+    for (int64_t base = r0; base < r1; base += JS_TILE) {
+        ScatterRaw<NPC> nxt;
+        issue(base + JS_TILE, nxt);
+        int part[JS_U];
+        int64_t h[JS_U], f1[JS_U], f2[JS_U];
 #pragma unroll
-        for (int u = 0; u < JOIN_UNROLL; ++u) {
-            if (!sel[u]) continue;
-            const int64_t r = rows[u];
-            if (!has[u]) {  // NULL key: matches nothing
-                if (TYPE == FGPU_LEFT || TYPE == FGPU_ANTI) join_emit(pa, j, r, NO_ROW);
-                continue;
-            }
-            if (key[u] == EMPTY_KEY) {  // sentinel-valued key: the build side list
-                int64_t m = 0;
-                for (int64_t q = 0; q < pa.n_min_key_rows; ++q) {
-                    ++m;
-                    if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) join_emit(pa, j, r, pa.min_key_rows[q]);
-                    else break;
+        for (int u = 0; u < JS_U; ++u) {
+            const int64_t r = base + u * JS_TPB + threadIdx.x;
+            f1[u] = f2[u] = 0;
+            bool sel = r < r1;
+            if (NPC > 0 && pa.pred.n > 0) {
+                int64_t v[NPC > 0 ? NPC : 1];
+                uint32_t ok = 0;
+#pragma unroll
+                for (int j = 0; j < NPC; ++j) {
+                    bool okj;
+                    cell_finish3(pc[j], r, raw.pc[u][j], v[j], okj);
+                    ok |= (okj ? 1u : 0u) << j;
                 }
-                if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) ||
-                    (TYPE == FGPU_ANTI && m == 0))
-                    join_emit(pa, j, r, NO_ROW);
-                continue;
+                sel = sel && eval_reg<(NPC > 0 ? NPC : 1)>(pa.pred, v, ok);
             }
-            const int p = part_of(hash64((uint64_t)key[u]), a.log_p);
-            const unsigned pos = atomicAdd(cur + p, 1u);
-            if (pos < (unsigned)a.Cw) {
-                int64_t *rec = cells + ((int64_t)p * a.Cw + pos) * a.RW;
-                rec[0] = key[u];
-                if (a.n_pay > 0) rec[1] = f1[u];
-                if (a.n_pay > 1) rec[2] = f2[u];
+            int64_t key;
+            bool ok, pok;
+            cell_finish3(kc, r, raw.key[u], key, ok);
+            if (RW > 1) cell_finish3(pay0, r, raw.f1[u], f1[u], pok);
+            if (RW > 2) cell_finish3(pay1, r, raw.f2[u], f2[u], pok);
+            h[u] = (int64_t)hash64((uint64_t)key);
+            part[u] = !sel ? -1 : !ok ? P2 - 2 : h[u] == EMPTY_KEY ? P2 - 1 : part_of((uint64_t)h[u], log_p);
+        }
+        unsigned rank[JS_U];
+#pragma unroll
+        for (int u = 0; u < JS_U; ++u) rank[u] = part[u] >= 0 ? atomicAdd(hist + part[u], 1u) : 0u;
+        __syncthreads();
+        const unsigned total = block_scan_hist(hist, off, P2, tmp);
+#pragma unroll
+        for (int u = 0; u < JS_U; ++u) {
+            if (part[u] < 0) continue;
+            const unsigned i = off[part[u]] + rank[u];
+            stage[(size_t)i * RW] = h[u];
+            if (RW > 1) stage[(size_t)i * RW + 1] = f1[u];
+            if (RW > 2) stage[(size_t)i * RW + 2] = f2[u];
+            spart[i] = (uint16_t)part[u];
+        }
+        __syncthreads();
+        for (unsigned i = threadIdx.x; i < total; i += blockDim.x) {
+            const int pp = spart[i];
+            const unsigned pos = cur[pp] + (i - off[pp]);
+            if (pos < (unsigned)Cw) {
+                int64_t *rec = a.cells + (((int64_t)pp * G + blockIdx.x) * Cw + pos) * RW;
+                rec[0] = stage[(size_t)i * RW];
+                if (RW > 1) rec[1] = stage[(size_t)i * RW + 1];
+                if (RW > 2) rec[2] = stage[(size_t)i * RW + 2];
             } else {
                 atomicExch(a.flags, 1ull);  // cell full: the host reruns on the direct path
             }
         }
+        __syncthreads();
+        for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+            cur[i] += hist[i];
+            hist[i] = 0;
+        }
+        __syncthreads();
+        raw = nxt;
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < P; i += blockDim.x) a.cell_cnt[(int64_t)blockIdx.x * P + i] = cur[i];
-    acc_finish(pa, j, lrow);
+    for (int i = threadIdx.x; i < P2; i += blockDim.x) a.cell_cnt[(int64_t)blockIdx.x * P2 + i] = cur[i];
 }
 
-// One output row of the partitioned join: probe payloads f1, f2 come from the
-// record, the build value from the sub-table slot (`present` = a match).
+// One output row of the partitioned join, general form: probe payloads f1, f2
+// from the record, the build value from the sub-table slot (`present` = a
+// match; the slot holds the build column itself or the build row).
 FGPU_DEV void part_emit(const JoinPartArgs &a, JoinAcc &j, int64_t f1, int64_t f2, bool present,
                         int64_t pay) {
     ++j.rows;
@@ -421,104 +519,335 @@ FGPU_DEV void part_emit(const JoinPartArgs &a, JoinAcc &j, int64_t f1, int64_t f
     }
 }
 
-// Pass 2: XCD-aware.  Workgroups are dispatched round-robin over the 8 XCDs
-// (workgroup b runs on XCD b % 8), so XCD x owns partitions x, x+8, x+16, ...
-// and all of its workgroups walk those partitions in the same order: while
-// they probe partition p, p's table (T 16-byte slots, about 1 MB) is the hot
-// set of that XCD's L2.  Workgroup b covers cells [l*G/K, (l+1)*G/K) of each
-// partition, l = b / 8, K = gridDim.x / 8; one cell per wave at a time.
+// The aggregate ops of a join, resolved into (wave-uniform) registers once
+// per workgroup: per op its kind, and where its value comes from (0 = none,
+// 1 / 2 = probe payload field, 3 = the build value in the slot).
+struct EmitOps {
+    int n;
+    int kind[JOIN_MAX_OPS];
+    int src[JOIN_MAX_OPS];
+};
+FGPU_DEV EmitOps emit_ops(const JoinPartArgs &a) {
+    EmitOps e;
+    e.n = __builtin_amdgcn_readfirstlane(a.pa.n_ops);
+#pragma unroll
+    for (int i = 0; i < JOIN_MAX_OPS; ++i) {
+        const JoinOp o = a.pa.ops[i];
+        e.kind[i] = __builtin_amdgcn_readfirstlane(o.kind);
+        const int src = i >= a.pa.n_ops || o.col < 0 ? 0 : o.side == 0 ? (a.field_of_op[i] == 2 ? 2 : 1) : 3;
+        e.src[i] = __builtin_amdgcn_readfirstlane(src);
+    }
+    return e;
+}
+// One output row, fast form: build values are inlined in the slots.
+FGPU_DEV void emit_fast(const EmitOps &e, JoinAcc &j, int64_t f1, int64_t f2, bool present, int64_t pay) {
+    ++j.rows;
+#pragma unroll
+    for (int i = 0; i < JOIN_MAX_OPS; ++i) {
+        const int src = e.src[i];
+        if (i >= e.n || src == 0 || (src == 3 && !present)) continue;
+        const int64_t v = src == 1 ? f1 : src == 2 ? f2 : pay;
+        reg_add(e.kind[i], v, j.acc[i], j.hi[i], j.lo[i]);
+    }
+}
+
+// Records of the two side partitions: NULL keys (matching nothing) and keys
+// equal to the sentinel INT64_MIN (matching the build side list).
 template <int TYPE>
-__global__ void k_jp_probe(JoinPartArgs a) {
-    __shared__ uint64_t lrow[64];
-    JoinAcc j;
-    acc_init(a.pa, j, lrow);  // synchronises the workgroup
-    const int P = 1 << a.log_p;
-    const int x = blockIdx.x & 7, l = blockIdx.x >> 3, K = gridDim.x >> 3;
-    const uint64_t tmask = (uint64_t)a.T - 1;
+FGPU_DEV void probe_side_partitions(const JoinPartArgs &a, JoinAcc &j) {
+    const int P = 1 << a.log_p, P2 = P + 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int w0 = (int)((int64_t)a.G * l / K), w1 = (int)((int64_t)a.G * (l + 1) / K);
-    for (int p = x; p < P; p += 8) {
-        const int64_t *tab = a.sub + (int64_t)p * a.T * 2;
-        for (int w = w0 + wave; w < w1; w += nw) {
-            unsigned n = a.cell_cnt[(int64_t)w * P + p];
+    for (int w = wave; w < a.G; w += nw) {
+        for (int sp = 0; sp < 2; ++sp) {
+            unsigned n = a.cell_cnt[(int64_t)w * P2 + P + sp];
             n = n < (unsigned)a.Cw ? n : (unsigned)a.Cw;
-            const int64_t *cell = a.cells + ((int64_t)w * P + p) * a.Cw * a.RW;
-            constexpr int R = 4;  // records per lane in flight
-            for (unsigned i0 = lane; i0 < n; i0 += 64 * R) {
-                int64_t key[R], f1[R], f2[R];
-                uint64_t slot[R];
-                longlong2 sl[R];
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const unsigned i = i0 + 64 * q;
-                    key[q] = f1[q] = f2[q] = 0;
-                    if (i < n) {
-                        const int64_t *rec = cell + (int64_t)i * a.RW;
-                        key[q] = rec[0];
-                        if (a.RW > 1) f1[q] = rec[1];
-                        if (a.RW > 2) f2[q] = rec[2];
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    slot[q] = hash64((uint64_t)key[q]) & tmask;
-                    sl[q] = make_longlong2(EMPTY_KEY, 0);
-                    if (i0 + 64 * q < n) sl[q] = load_slot(tab, slot[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    if (i0 + 64 * q >= n) continue;
-                    int64_t m = 0;
-                    longlong2 cur = sl[q];
-                    uint64_t si = slot[q];
-                    while (cur.x != EMPTY_KEY) {
-                        if (cur.x == key[q]) {
-                            ++m;
-                            if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
-                                part_emit(a, j, f1[q], f2[q], true, cur.y);
-                                if (a.pa.unique_keys) break;
-                            } else {
-                                break;
-                            }
+            const int64_t *cell = a.cells + ((int64_t)(P + sp) * a.G + w) * a.Cw * a.RW;
+            for (unsigned i = lane; i < n; i += 64) {
+                const int64_t *rec = cell + (int64_t)i * a.RW;
+                const int64_t f1 = a.RW > 1 ? rec[1] : 0, f2 = a.RW > 2 ? rec[2] : 0;
+                int64_t m = 0;
+                if (sp == 1) {  // sentinel key: every build row in the side list matches
+                    for (int64_t q = 0; q < a.pa.n_min_key_rows; ++q) {
+                        ++m;
+                        if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
+                            int64_t pay = a.pa.min_key_rows[q];
+                            if (a.build_pay_col >= 0) pay = load_raw(a.pa.bcols, a.build_pay_col, pay);
+                            part_emit(a, j, f1, f2, true, pay);
+                        } else {
+                            break;
                         }
-                        si = (si + 1) & tmask;
-                        cur = load_slot(tab, si);
                     }
-                    if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) ||
-                        (TYPE == FGPU_ANTI && m == 0))
-                        part_emit(a, j, f1[q], f2[q], false, 0);
                 }
+                if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) || (TYPE == FGPU_ANTI && m == 0))
+                    part_emit(a, j, f1, f2, false, 0);
             }
         }
     }
+}
+
+// Pass 2.  Workgroup b serves partition p = b % P: it copies the partition's
+// sub-table into LDS (slot hashes and build values in two arrays) and the
+// record counts of the partition's cells, then streams the records of its
+// share of the cells and probes the LDS table.  The only HBM traffic is the
+// records, read once and coalesced; the loads of the next group of records
+// are in flight while the current one is probed.
+//
+// FAST (unique build keys, the build value inlined in the slot or not
+// needed): each lane first resolves all R of its records -- the first probe
+// of every record is issued at once, then only records whose chain goes on
+// take further steps -- and folds each record once afterwards, so the fold
+// code runs R times per group instead of once per probe step.
+template <int TYPE, bool FAST, int RW>
+__global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
+    extern __shared__ uint64_t jp_lds[];
+    const int P = 1 << a.log_p, P2 = P + 2;
+    const int T = a.T, Cw = a.Cw, G = a.G;
+    const bool vals = a.need_vals != 0;                     // an aggregate reads the build side
+    int64_t *tk = (int64_t *)jp_lds;                        // T slot hashes
+    int64_t *tv = tk + T;                                   // T build values / rows (if vals)
+    uint64_t *lrow = jp_lds + (vals ? 2 : 1) * (size_t)T;   // 64 words
+    unsigned *cnt = (unsigned *)(lrow + 64);                // G: records of each cell of partition p
+    const int p = blockIdx.x & (P - 1);
+    const int k = blockIdx.x >> a.log_p, K = gridDim.x >> a.log_p;
+    const longlong2 *src = (const longlong2 *)(a.sub + (int64_t)p * T * 2);
+    for (int i = threadIdx.x; i < T; i += blockDim.x) {
+        const longlong2 sl = src[i];
+        tk[i] = sl.x;
+        if (vals) tv[i] = sl.y;
+    }
+    for (int w = threadIdx.x; w < G; w += blockDim.x) {
+        const unsigned c = a.cell_cnt[(int64_t)w * P2 + p];
+        cnt[w] = c < (unsigned)Cw ? c : (unsigned)Cw;
+    }
+    JoinAcc j;
+    acc_init(a.pa, j, lrow);  // synchronises the workgroup (table and counts are in place)
+    const EmitOps e = emit_ops(a);
+    const bool unique = a.pa.unique_keys != 0;
+    const uint32_t tmask = (uint32_t)T - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int w0 = (int)((int64_t)G * k / K), w1 = (int)((int64_t)G * (k + 1) / K);
+    constexpr int R = 4;  // records per lane per group
+    // (w, i0): the wave's current cell and first record of the group
+    int w = w0 + wave;
+    unsigned i0 = 0, n = w < w1 ? cnt[w] : 0;
+    while (w < w1 && n == 0) {
+        w += nw;
+        n = w < w1 ? cnt[w] : 0;
+    }
+    int64_t key[R], f1[R], f2[R];
+    // Loads are issued unconditionally (a lane past the cell's end, or a wave
+    // past its last cell, reads record 0 of cell (0, p), which always exists),
+    // so the number of loads in flight is fixed and the wait before probing
+    // a group does not also wait for the next group's loads.
+    const int64_t *cell0 = a.cells + (int64_t)p * G * Cw * RW;  // cell (0, p): partition-major layout
+    auto load_group = [&](int cw, unsigned ci0, unsigned cn, int64_t (&kk)[R], int64_t (&g1)[R], int64_t (&g2)[R]) {
+        const int64_t *cell = cw < w1 ? cell0 + (int64_t)cw * Cw * RW : cell0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const unsigned i = ci0 + 64 * q + lane;
+            const int64_t *rec = i < cn ? cell + (int64_t)i * RW : cell0;
+            if (RW == 1) {
+                kk[q] = rec[0];
+                g1[q] = g2[q] = 0;
+            } else if (RW == 2) {
+                const longlong2 v = *(const longlong2 *)rec;
+                kk[q] = v.x;
+                g1[q] = v.y;
+                g2[q] = 0;
+            } else {
+                kk[q] = rec[0];
+                g1[q] = rec[1];
+                g2[q] = rec[2];
+            }
+        }
+    };
+    load_group(w, i0, n, key, f1, f2);
+    while (w < w1) {
+        // next group: the rest of this cell, or the next non-empty cell
+        int wn = w;
+        unsigned in = i0 + 64 * R, nn = n;
+        while (wn < w1 && in >= nn) {
+            wn += nw;
+            in = 0;
+            nn = wn < w1 ? cnt[wn] : 0;
+        }
+        int64_t key2[R], g1[R], g2[R];
+        load_group(wn, in, nn, key2, g1, g2);
+        // keep every load of the next group ahead of the first use of this one
+        __builtin_amdgcn_sched_barrier(0);
+        bool live[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) live[q] = i0 + 64 * q + lane < n;
+        if (FAST) {
+            uint32_t si[R];
+            int64_t sk[R];
+            bool hit[R], more[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                si[q] = (uint32_t)key[q] & tmask;
+                sk[q] = tk[si[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                hit[q] = sk[q] == key[q];
+                more[q] = live[q] && !hit[q] && sk[q] != EMPTY_KEY;
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                while (__ballot(more[q])) {  // lockstep walk of the records still unresolved
+                    if (more[q]) {
+                        si[q] = (si[q] + 1) & tmask;
+                        const int64_t x = tk[si[q]];
+                        hit[q] = x == key[q];
+                        more[q] = !hit[q] && x != EMPTY_KEY;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (!live[q]) continue;
+                if (TYPE == FGPU_INNER) {
+                    if (hit[q]) emit_fast(e, j, f1[q], f2[q], true, vals ? tv[si[q]] : 0);
+                } else if (TYPE == FGPU_LEFT) {
+                    emit_fast(e, j, f1[q], f2[q], hit[q], hit[q] && vals ? tv[si[q]] : 0);
+                } else if ((TYPE == FGPU_SEMI) == hit[q]) {
+                    emit_fast(e, j, f1[q], f2[q], false, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (!live[q]) continue;
+                uint32_t si = (uint32_t)key[q] & tmask;
+                int64_t m = 0;
+                for (;;) {
+                    const int64_t sk = tk[si];
+                    if (sk == EMPTY_KEY) break;
+                    if (sk == key[q]) {
+                        ++m;
+                        if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
+                            part_emit(a, j, f1[q], f2[q], true, vals ? tv[si] : 0);
+                            if (unique) break;
+                        } else {
+                            break;
+                        }
+                    }
+                    si = (si + 1) & tmask;
+                }
+                if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) || (TYPE == FGPU_ANTI && m == 0))
+                    part_emit(a, j, f1[q], f2[q], false, 0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            key[q] = key2[q];
+            f1[q] = g1[q];
+            f2[q] = g2[q];
+        }
+        w = wn;
+        i0 = in;
+        n = nn;
+    }
+    if (blockIdx.x == 0) probe_side_partitions<TYPE>(a, j);
     acc_finish(a.pa, j, lrow);
 }
 
+size_t join_scatter_lds(const JoinPartArgs &a) {
+    const size_t P2 = ((size_t)1 << a.log_p) + 2;
+    return (size_t)JS_TILE * a.RW * 8 + 3 * P2 * 4 + 16 * 4 + (size_t)JS_TILE * 2;
+}
+size_t join_probe_lds(const JoinPartArgs &a) {
+    return (size_t)a.T * (a.need_vals ? 16 : 8) + 64 * 8 + (size_t)a.G * 4;
+}
+
+// Lets every pass-1 / pass-2 instantiation take up to JOIN_LDS_MAX bytes of
+// dynamic LDS; done once per process (the attribute is per function).
+template <bool FAST, int RW>
+static void probe_fns(std::vector<const void *> &v) {
+    v.push_back((const void *)k_jp_probe<FGPU_INNER, FAST, RW>);
+    v.push_back((const void *)k_jp_probe<FGPU_LEFT, FAST, RW>);
+    v.push_back((const void *)k_jp_probe<FGPU_SEMI, FAST, RW>);
+    v.push_back((const void *)k_jp_probe<FGPU_ANTI, FAST, RW>);
+}
 template <int NPC>
-static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, int block, size_t lds) {
-    switch (a.pa.type) {
-        case FGPU_INNER: hipLaunchKernelGGL((k_jp_scatter<FGPU_INNER, NPC>), dim3(a.G), dim3(block), lds, s, a); break;
-        case FGPU_LEFT: hipLaunchKernelGGL((k_jp_scatter<FGPU_LEFT, NPC>), dim3(a.G), dim3(block), lds, s, a); break;
-        case FGPU_SEMI: hipLaunchKernelGGL((k_jp_scatter<FGPU_SEMI, NPC>), dim3(a.G), dim3(block), lds, s, a); break;
-        default: hipLaunchKernelGGL((k_jp_scatter<FGPU_ANTI, NPC>), dim3(a.G), dim3(block), lds, s, a); break;
-    }
+static void scatter_fns(std::vector<const void *> &v) {
+    v.push_back((const void *)k_jp_scatter<NPC, 1>);
+    v.push_back((const void *)k_jp_scatter<NPC, 2>);
+    v.push_back((const void *)k_jp_scatter<NPC, 3>);
+}
+static hipError_t join_allow_lds_once() {
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        std::vector<const void *> fns;
+        scatter_fns<0>(fns);
+        scatter_fns<1>(fns);
+        scatter_fns<2>(fns);
+        scatter_fns<MAX_PCOLS>(fns);
+        probe_fns<true, 1>(fns);
+        probe_fns<true, 2>(fns);
+        probe_fns<true, 3>(fns);
+        probe_fns<false, 1>(fns);
+        probe_fns<false, 2>(fns);
+        probe_fns<false, 3>(fns);
+        for (const void *f : fns) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)JOIN_LDS_MAX);
+            if (e != hipSuccess && err == hipSuccess) err = e;
+        }
+    });
+    return err;
+}
+
+template <int NPC>
+static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, size_t lds) {
+    const dim3 g((unsigned)a.G), b(JS_TPB);
+    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1>), g, b, lds, s, a);
+    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3>), g, b, lds, s, a);
 }
 
 hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int block) {
-    const size_t lds = (size_t)(1 << a.log_p) * 4;
-    if (a.pa.pred.n_pcols <= 2) launch_scatter_npc<2>(s, a, block, lds);
-    else launch_scatter_npc<MAX_PCOLS>(s, a, block, lds);
+    (void)block;
+    const size_t lds = join_scatter_lds(a);
+    if (lds > JOIN_LDS_MAX || a.log_p > 14 || a.RW < 1 || a.RW > 3) return hipErrorInvalidValue;
+    if (hipError_t e = join_allow_lds_once()) return e;
+    const int n = a.pa.pred.n == 0 ? 0 : a.pa.pred.n_pcols;
+    if (n == 0) launch_scatter_npc<0>(s, a, lds);
+    else if (n == 1) launch_scatter_npc<1>(s, a, lds);
+    else if (n == 2) launch_scatter_npc<2>(s, a, lds);
+    else launch_scatter_npc<MAX_PCOLS>(s, a, lds);
     return hipGetLastError();
 }
 
-hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int block) {
-    const dim3 grid((unsigned)(8 * a.split));  // a.split = workgroups per XCD
+template <bool FAST, int RW>
+static void launch_probe_rw(hipStream_t s, const JoinPartArgs &a, dim3 grid, size_t lds) {
+    const dim3 b(1024);
     switch (a.pa.type) {
-        case FGPU_INNER: hipLaunchKernelGGL(k_jp_probe<FGPU_INNER>, grid, dim3(block), 0, s, a); break;
-        case FGPU_LEFT: hipLaunchKernelGGL(k_jp_probe<FGPU_LEFT>, grid, dim3(block), 0, s, a); break;
-        case FGPU_SEMI: hipLaunchKernelGGL(k_jp_probe<FGPU_SEMI>, grid, dim3(block), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_jp_probe<FGPU_ANTI>, grid, dim3(block), 0, s, a); break;
+        case FGPU_INNER: hipLaunchKernelGGL((k_jp_probe<FGPU_INNER, FAST, RW>), grid, b, lds, s, a); break;
+        case FGPU_LEFT: hipLaunchKernelGGL((k_jp_probe<FGPU_LEFT, FAST, RW>), grid, b, lds, s, a); break;
+        case FGPU_SEMI: hipLaunchKernelGGL((k_jp_probe<FGPU_SEMI, FAST, RW>), grid, b, lds, s, a); break;
+        default: hipLaunchKernelGGL((k_jp_probe<FGPU_ANTI, FAST, RW>), grid, b, lds, s, a); break;
     }
+}
+template <bool FAST>
+static void launch_probe_fast(hipStream_t s, const JoinPartArgs &a, dim3 grid, size_t lds) {
+    if (a.RW == 1) launch_probe_rw<FAST, 1>(s, a, grid, lds);
+    else if (a.RW == 2) launch_probe_rw<FAST, 2>(s, a, grid, lds);
+    else launch_probe_rw<FAST, 3>(s, a, grid, lds);
+}
+
+hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int block) {
+    (void)block;
+    const size_t lds = join_probe_lds(a);
+    if (lds > JOIN_LDS_MAX) return hipErrorInvalidValue;
+    if (hipError_t e = join_allow_lds_once()) return e;
+    const dim3 grid((unsigned)(a.split << a.log_p));  // a.split = workgroups per partition
+    // fast form: unique build keys, and the build value (if any) inlined in the slots
+    bool build_rows = false;
+    for (int i = 0; i < a.pa.n_ops; ++i) build_rows |= a.pa.ops[i].side == 1 && a.pa.ops[i].col >= 0;
+    const bool fast = a.pa.unique_keys && (!build_rows || a.build_pay_col >= 0);
+    if (fast) launch_probe_fast<true>(s, a, grid, lds);
+    else launch_probe_fast<false>(s, a, grid, lds);
     return hipGetLastError();
 }
 

@@ -91,8 +91,9 @@ struct GbPartArgs {
     int RW;                   // words per record slot
     int has_valid_word;       // records carry the inputs' validity bits
     int SR;                   // record slots per state record
-    int64_t *cells;           // G * P * Cw * RW words
-    unsigned int *cell_cnt;   // G * P * 2: row records, state records
+    int64_t *cells;           // (P + 2) * G * Cw * RW words, partition-major: partitions 0..P-1,
+                              // NULL keys, sentinel keys; cell (w, p) at ((p * G + w) * Cw) * RW
+    unsigned int *cell_cnt;   // G * (P + 2) * 2: row records, state records
     uint64_t *nullbuf;        // G * S
     unsigned long long *flags;  // [0] cell full, [1] pass-2 table full, [2] output full
     int T2;                   // pass-2 LDS table slots (power of two)
@@ -225,8 +226,17 @@ struct JoinPartArgs {
                                 // [3] duplicate build keys seen
     uint32_t *min_key_rows;   // build rows with key INT64_MIN (pass-1 build output)
     int64_t min_key_cap;
-    int split;                // pass-2 workgroups per XCD
+    int split;                // pass-2 workgroups per partition
+    int need_vals;            // an aggregate reads a build column (pass 2 keeps slot values in LDS)
 };
+// Partitioned join pass 1: JS_TPB threads per workgroup, JS_U rows each per tile.
+constexpr int JS_TPB = 1024;
+constexpr int JS_U = 4;
+constexpr int JS_TILE = JS_TPB * JS_U;
+constexpr size_t JOIN_LDS_MAX = 160 * 1024;  // dynamic LDS a join workgroup may take
+// LDS bytes of one pass-1 / pass-2 workgroup for these arguments.
+size_t join_scatter_lds(const JoinPartArgs &a);
+size_t join_probe_lds(const JoinPartArgs &a);
 hipError_t launch_join_part_count(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);
 hipError_t launch_join_part_build(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);
 hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int block);

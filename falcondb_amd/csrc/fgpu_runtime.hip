@@ -45,6 +45,22 @@ int64_t next_pow2(int64_t x) {
     return p;
 }
 
+// Inverse of hash64 (each step of the finaliser is invertible: xorshifts by
+// 33 >= 32 bits undo themselves, and odd multipliers have inverses mod 2^64).
+uint64_t inv_odd(uint64_t c) {
+    uint64_t x = c;  // Newton's iteration doubles the correct low bits each step
+    for (int i = 0; i < 6; ++i) x *= 2 - c * x;
+    return x;
+}
+uint64_t unhash64(uint64_t h) {
+    h ^= h >> 33;
+    h *= inv_odd(0xc4ceb9fe1a85ec53ull);
+    h ^= h >> 33;
+    h *= inv_odd(0xff51afd7ed558ccdull);
+    h ^= h >> 33;
+    return h;
+}
+
 int env_int(const char *name, int dflt) {
     const char *v = std::getenv(name);
     return v && *v ? std::atoi(v) : dflt;
@@ -134,9 +150,10 @@ struct Tuning {
     int gbp_agg_block = 1024;
     int gbp_scan_wgs_per_cu = 1;
     int lds_limit = 160 * 1024;    // LDS bytes one workgroup may use
-    int join_partitions = 32;      // radix partitions of the build / probe sides
-    int join_part_wgs_per_cu = 4;  // pass-1 workgroups per CU
+    int join_partitions = 0;       // least radix partitions of the build / probe sides (0: from the build rows)
+    int join_part_wgs_per_cu = 2;  // pass-1 workgroups per CU (two fit by registers)
     int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
+    int join_slots_per_key = 2;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
 };
 
 struct fgpu_ctx {
@@ -636,6 +653,9 @@ int fgpu_open(int device, fgpu_ctx **out) {
     t.filter_waves_per_cu = env_int("FGPU_FILTER_WAVES", t.filter_waves_per_cu);
     t.join_partition = env_int("FGPU_JOIN_PARTITION", t.join_partition);
     t.join_partitions = env_int("FGPU_JOIN_PARTITIONS", t.join_partitions);
+    t.join_slots_per_key = env_int("FGPU_JOIN_SLOTS_PER_KEY", t.join_slots_per_key);
+    t.join_part_wgs_per_cu = env_int("FGPU_JOIN_PART_WGS", t.join_part_wgs_per_cu);
+    t.join_probe_wgs_per_cu = env_int("FGPU_JOIN_PROBE_WGS", t.join_probe_wgs_per_cu);
     t.gb_partition = env_int("FGPU_GB_PARTITION", t.gb_partition);
     *out = c;
     return FGPU_OK;
@@ -703,6 +723,7 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_partitions") t.join_partitions = value;
     else if (n == "join_part_wgs_per_cu") t.join_part_wgs_per_cu = value;
     else if (n == "join_probe_wgs_per_cu") t.join_probe_wgs_per_cu = value;
+    else if (n == "join_slots_per_key") t.join_slots_per_key = value;
     else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
     if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
         t.join_block > 1024 || t.join_block % 64)
@@ -715,8 +736,9 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
         t.gbp_scan_block < 64 || t.gbp_scan_block > 1024 || t.gbp_agg_block < 64 ||
         t.gbp_agg_block > 1024 || t.gbp_scan_wgs_per_cu < 1 || t.gbp_probe_limit < 1)
         return set_err(FGPU_E_INVALID_QUERY, "partitioned GROUP BY knobs out of range");
-    if (t.join_partitions < 8 || t.join_partitions > 1024 || (t.join_partitions & (t.join_partitions - 1)))
-        return set_err(FGPU_E_INVALID_QUERY, "join_partitions must be a power of two in [8, 1024]");
+    if (t.join_partitions && (t.join_partitions < 8 || t.join_partitions > 4096 ||
+                              (t.join_partitions & (t.join_partitions - 1))))
+        return set_err(FGPU_E_INVALID_QUERY, "join_partitions must be 0 or a power of two in [8, 4096]");
     if (t.join_part_wgs_per_cu < 1 || t.join_probe_wgs_per_cu < 1)
         return set_err(FGPU_E_INVALID_QUERY, "join workgroup knobs out of range");
     return FGPU_OK;
@@ -1459,23 +1481,37 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     }
     const DevCol &bkc = build->cols[spec->build_key];
     const int64_t nb = build->n_rows;
-    // P partitions (>= 8: one or more per XCD), each with a table at load <= 0.5
-    const int64_t P = std::max<int64_t>(8, next_pow2(tu.join_partitions));
-    jp.T = (int)std::max<int64_t>(1024, next_pow2(4 * ((nb + P - 1) / P)));  // load <= 0.25
-    if (jp.T > (1 << 24)) return -1;
+    // P partitions, each with a sub-table that pass 2 holds in LDS: at most
+    // tmax slots, at load <= 0.5 on average (checked <= 0.75 after counting).
+    const int64_t tmax = std::min<int64_t>(8192, next_pow2(((int64_t)tu.lds_limit - 1024) / 16 + 1) / 2);
+    const int64_t spk = std::max(2, tu.join_slots_per_key);  // slots per build key: load <= 1 / spk
+    int64_t P = std::max<int64_t>(8, next_pow2((spk * nb + tmax - 1) / tmax));
+    if (tu.join_partitions) P = std::max<int64_t>(P, next_pow2(tu.join_partitions));
+    if (P > 4096) return -1;
+    jp.T = (int)std::max<int64_t>(256, std::min<int64_t>(tmax, next_pow2((spk * nb + P - 1) / P)));
     jp.log_p = 0;
     while ((int64_t(1) << jp.log_p) < P) ++jp.log_p;
-    jp.G = c->num_cus * tu.join_part_wgs_per_cu;
-    jp.rows_per_wg = (probe->n_rows + jp.G - 1) / jp.G;
-    jp.Cw = (int)std::ceil((double)jp.rows_per_wg / (double)P * 1.25) + 32;
     jp.RW = 1 + jp.n_pay;
-    jp.split = std::max(1, c->num_cus / 8 * tu.join_probe_wgs_per_cu);
-    const size_t cell_words = (size_t)jp.G * P * jp.Cw * jp.RW;
+    jp.need_vals = bcols_used.empty() ? 0 : 1;
+    // pass 1: as many workgroups per CU as LDS allows (at most the tuned count);
+    // cells of P partitions plus the NULL-key and sentinel-key side partitions
+    if (join_scatter_lds(jp) > (size_t)tu.lds_limit) return -1;
+    const int64_t fit1 = std::max<int64_t>(1, std::min<int64_t>(tu.join_part_wgs_per_cu,
+                                                                 (int64_t)tu.lds_limit / (int64_t)join_scatter_lds(jp)));
+    jp.G = (int)(c->num_cus * fit1);
+    if (join_probe_lds(jp) > (size_t)tu.lds_limit) return -1;
+    jp.rows_per_wg = (probe->n_rows + jp.G - 1) / jp.G;
+    jp.Cw = ((int)std::ceil((double)jp.rows_per_wg / (double)P * 1.15) + 64 + 15) / 16 * 16;  // whole lines
+    const int64_t fit2 = std::max<int64_t>(1, std::min<int64_t>(tu.join_probe_wgs_per_cu,
+                                                                 (int64_t)tu.lds_limit / (int64_t)join_probe_lds(jp)));
+    jp.split = (int)std::max<int64_t>(1, (int64_t)c->num_cus * fit2 / P);
+    const int64_t P2 = P + 2;
+    const size_t cell_words = (size_t)jp.G * P2 * jp.Cw * jp.RW;
     int rc = FGPU_OK;
     void *p;
     if ((rc = buf(c, B_CELLS, cell_words * 8, &p))) return rc;
     jp.cells = (int64_t *)p;
-    if ((rc = buf(c, B_CELLCNT, (size_t)jp.G * P * 4, &p))) return rc;
+    if ((rc = buf(c, B_CELLCNT, (size_t)jp.G * P2 * 4, &p))) return rc;
     jp.cell_cnt = (unsigned int *)p;
     if ((rc = buf(c, B_SUB, (size_t)P * jp.T * 16, &p))) return rc;
     jp.sub = (int64_t *)p;
@@ -1484,7 +1520,10 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     unsigned long long *d_flags;
     if ((rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_flags))) return rc;
     jp.flags = d_flags;
-    const bool has_min = bkc.dtype == FGPU_INT64 && bkc.imin == INT64_MIN;
+    // the partitioned path keys on hash64(key): the one key whose hash is the
+    // empty-slot marker keeps its build rows in a side list
+    const int64_t sentinel = (int64_t)unhash64((uint64_t)EMPTY_KEY);
+    const bool has_min = bkc.imin <= sentinel && sentinel <= bkc.imax;
     jp.min_key_cap = has_min ? std::max<int64_t>(nb, 1) : 0;
     jp.min_key_rows = nullptr;
     if (has_min) {
@@ -1506,7 +1545,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
         HIPCHK(hipMemcpyAsync(counts.data(), jp.sub_cnt, P * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         const unsigned mx = *std::max_element(counts.begin(), counts.end());
-        if (mx > (unsigned)(jp.T - jp.T / 8)) return -1;  // skewed keys: direct probe
+        if (mx > (unsigned)(jp.T - jp.T / 4)) return -1;  // skewed keys: direct probe
         HIPCHK(launch_join_part_build(c->stream, jp, ba, grid_for(c, nb, BLOCK, 16)));
     }
     mark(c, 5);

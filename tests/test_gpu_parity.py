@@ -303,15 +303,33 @@ def test_join_stride_keys_closed_form(jctx):
         assert r.columns[0][0] == n
 
 
+# The key whose 64-bit finaliser hash (MurmurHash3 fmix64) is 0x8000000000000000,
+# the empty-slot marker of the partitioned join's hash-keyed sub-tables.
+HASH_SENTINEL_KEY = 1401494638771588894
+
+
+def test_hash_sentinel_key_is_the_marker_preimage():
+    m = (1 << 64) - 1
+    h = HASH_SENTINEL_KEY
+    h ^= h >> 33
+    h = h * 0xff51afd7ed558ccd & m
+    h ^= h >> 33
+    h = h * 0xc4ceb9fe1a85ec53 & m
+    h ^= h >> 33
+    assert h == 1 << 63
+
+
 def test_join_duplicates_nulls_and_sentinel_keys(jctx):
     ctx = jctx
     rng = np.random.default_rng(21)
     nb, np_ = 3000, 20000
     bk = rng.integers(0, 500, nb)
-    bk[::101] = np.iinfo(np.int64).min  # a real key equal to the empty-slot sentinel
+    bk[::101] = np.iinfo(np.int64).min  # a real key equal to the direct index's empty-slot marker
+    bk[7::211] = HASH_SENTINEL_KEY       # a real key whose hash is the partitioned tables' marker
     bvalid = rng.random(nb) > 0.05
     pk = rng.integers(0, 700, np_)
     pk[::203] = np.iinfo(np.int64).min
+    pk[5::307] = HASH_SENTINEL_KEY
     pvalid = rng.random(np_) > 0.05
     bcols = {"bk": NullableColumn(bk, bvalid), "bx": rng.normal(size=nb)}
     pcols = {"pk": NullableColumn(pk, pvalid), "px": rng.integers(-50, 50, np_)}

@@ -272,8 +272,12 @@ FGPU_DEV int part_of(uint64_t h, int log_p) { return log_p ? (int)(h >> (64 - lo
 
 // The partitioned join works on h = hash64(key) instead of the key: hash64 is
 // a bijection of 64-bit values, so equal hashes are equal keys.  Sub-table
-// slots and pass-1 records hold h; the partition is h's top bits and the slot
-// its low bits.  The one key whose hash is the empty-slot marker EMPTY_KEY
+// slots and pass-1 records hold h; the partition is h's top bits.  A sub-table
+// is T / 4 buckets of 4 slots; a key's home bucket is given by h's low bits and
+// it is stored in the first free slot from the start of that bucket on
+// (linear probing that spills into the next buckets only when its own is
+// full).  Slots are never freed, so a bucket with a free slot proves a key
+// that is not in it is absent.  The one key whose hash is the empty-slot marker EMPTY_KEY
 // (the "sentinel key") is kept out of the sub-tables: its build rows go to a
 // side list and its probe rows to a side partition.
 FGPU_DEV int64_t build_key_at(const JoinBuildArgs &b, int64_t r) {
@@ -294,7 +298,7 @@ __global__ void k_jp_build(JoinPartArgs a, JoinBuildArgs b) {
         const int p = part_of((uint64_t)h, a.log_p);  // k_jp_count checked every sub-table has room
         const int64_t pay = a.build_pay_col >= 0 ? load_raw(a.pa.bcols, a.build_pay_col, r) : r;
         int64_t *sub = a.sub + (int64_t)p * a.T * 2;
-        uint64_t slot = (uint64_t)h & tmask;
+        uint64_t slot = ((uint64_t)h << 2) & tmask;  // first slot of the home bucket (4 slots)
         bool done = false;
         for (int q = 0; q < a.T && !done; ++q) {
             int64_t expect = EMPTY_KEY;
@@ -374,32 +378,33 @@ FGPU_DEV unsigned block_scan_hist(const unsigned *hist, unsigned *off, int n, un
 // Pass 1: pure partitioning.  Every selected probe row becomes a record (h =
 // hash64(key), then the probe payloads) of partition part_of(h); rows with a
 // NULL key go to partition P and rows with the sentinel key to partition
-// P + 1 (pass 2 gives them their join semantics).  A tile of JS_TILE rows is
+// P + 1 (pass 2 gives them their join semantics).  A tile of JS_TPB * U rows is
 // counted per partition in LDS, staged in LDS in partition order and written
 // to the workgroup's cells in runs: consecutive threads write consecutive
 // records of one partition, so stores leave in whole cache lines.  The loads
 // of tile t + 1 are issued before tile t is staged (every load is issued, a
 // row past the range re-reads the range's last row), so they overlap the
 // LDS phases and the stores of tile t.
-template <int NPC>
+template <int NPC, int U>
 struct ScatterRaw {
-    RawCell3 pc[JS_U][NPC > 0 ? NPC : 1];
-    RawCell3 key[JS_U];
-    RawCell3 f1[JS_U], f2[JS_U];
+    RawCell3 pc[U][NPC > 0 ? NPC : 1];
+    RawCell3 key[U];
+    RawCell3 f1[U], f2[U];
 };
 
-template <int NPC, int RW>
+template <int NPC, int RW, int U>
 __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     extern __shared__ uint64_t js_lds[];
+    constexpr int TILE = JS_TPB * U;
     const JoinProbeArgs &pa = a.pa;
     const int P2 = (1 << a.log_p) + 2;
     const int Cw = a.Cw, log_p = a.log_p;
-    int64_t *stage = (int64_t *)js_lds;                           // JS_TILE * RW words
-    unsigned *hist = (unsigned *)(stage + (size_t)JS_TILE * RW);  // P2
-    unsigned *off = hist + P2;                                    // P2
-    unsigned *cur = off + P2;                                     // P2: records in this workgroup's cells
-    unsigned *tmp = cur + P2;                                     // 16
-    uint16_t *spart = (uint16_t *)(tmp + 16);                     // JS_TILE
+    int64_t *stage = (int64_t *)js_lds;                        // TILE * RW words
+    uint32_t *dest = (uint32_t *)(stage + (size_t)TILE * RW);  // TILE: partition << 17 | position in its cell
+    unsigned *hist = dest + TILE;                              // P2
+    unsigned *off = hist + P2;                                 // P2
+    unsigned *cur = off + P2;                                  // P2: records in this workgroup's cells
+    unsigned *tmp = cur + P2;                                  // 16
     const ColRef kc = col_ref(pa.pcols, pa.pkey);
     ColRef pc[NPC > 0 ? NPC : 1];
 #pragma unroll
@@ -413,9 +418,9 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     const int64_t r1 = r0 + a.rows_per_wg < pa.n_probe ? r0 + a.rows_per_wg : pa.n_probe;
     // cells are partition-major: cell (w, p) holds Cw records at ((p * G + w) * Cw) * RW
     const int64_t G = a.G;
-    auto issue = [&](int64_t base, ScatterRaw<NPC> &t) {
+    auto issue = [&](int64_t base, ScatterRaw<NPC, U> &t) {
 #pragma unroll
-        for (int u = 0; u < JS_U; ++u) {
+        for (int u = 0; u < U; ++u) {
             int64_t r = base + u * JS_TPB + threadIdx.x;
             r = r < r1 ? r : r1 - 1;
 #pragma unroll
@@ -425,15 +430,15 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             if (RW > 2) cell_issue3(pay1, r, t.f2[u]);
         }
     };
-    ScatterRaw<NPC> raw;
+    ScatterRaw<NPC, U> raw;
     if (r0 < r1) issue(r0, raw);
-    for (int64_t base = r0; base < r1; base += JS_TILE) {
-        ScatterRaw<NPC> nxt;
-        issue(base + JS_TILE, nxt);
-        int part[JS_U];
-        int64_t h[JS_U], f1[JS_U], f2[JS_U];
+    for (int64_t base = r0; base < r1; base += TILE) {
+        ScatterRaw<NPC, U> nxt;
+        issue(base + TILE, nxt);
+        int part[U];
+        int64_t h[U], f1[U], f2[U];
 #pragma unroll
-        for (int u = 0; u < JS_U; ++u) {
+        for (int u = 0; u < U; ++u) {
             const int64_t r = base + u * JS_TPB + threadIdx.x;
             f1[u] = f2[u] = 0;
             bool sel = r < r1;
@@ -456,26 +461,27 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             h[u] = (int64_t)hash64((uint64_t)key);
             part[u] = !sel ? -1 : !ok ? P2 - 2 : h[u] == EMPTY_KEY ? P2 - 1 : part_of((uint64_t)h[u], log_p);
         }
-        unsigned rank[JS_U];
+        unsigned rank[U];
 #pragma unroll
-        for (int u = 0; u < JS_U; ++u) rank[u] = part[u] >= 0 ? atomicAdd(hist + part[u], 1u) : 0u;
+        for (int u = 0; u < U; ++u) rank[u] = part[u] >= 0 ? atomicAdd(hist + part[u], 1u) : 0u;
         __syncthreads();
         const unsigned total = block_scan_hist(hist, off, P2, tmp);
 #pragma unroll
-        for (int u = 0; u < JS_U; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (part[u] < 0) continue;
             const unsigned i = off[part[u]] + rank[u];
             stage[(size_t)i * RW] = h[u];
             if (RW > 1) stage[(size_t)i * RW + 1] = f1[u];
             if (RW > 2) stage[(size_t)i * RW + 2] = f2[u];
-            spart[i] = (uint16_t)part[u];
+            dest[i] = (uint32_t)part[u] << 17 | (cur[part[u]] + rank[u]);
         }
         __syncthreads();
         for (unsigned i = threadIdx.x; i < total; i += blockDim.x) {
-            const int pp = spart[i];
-            const unsigned pos = cur[pp] + (i - off[pp]);
+            const uint32_t d = dest[i];
+            const int64_t pp = d >> 17;
+            const unsigned pos = d & 0x1ffffu;
             if (pos < (unsigned)Cw) {
-                int64_t *rec = a.cells + (((int64_t)pp * G + blockIdx.x) * Cw + pos) * RW;
+                int64_t *rec = a.cells + ((pp * G + blockIdx.x) * Cw + pos) * RW;
                 rec[0] = stage[(size_t)i * RW];
                 if (RW > 1) rec[1] = stage[(size_t)i * RW + 1];
                 if (RW > 2) rec[2] = stage[(size_t)i * RW + 2];
@@ -485,7 +491,7 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
         }
         __syncthreads();
         for (int i = threadIdx.x; i < P2; i += blockDim.x) {
-            cur[i] += hist[i];
+            cur[i] = min(cur[i] + hist[i], (unsigned)Cw);  // positions stay below 2^17 (Cw < 2^17)
             hist[i] = 0;
         }
         __syncthreads();
@@ -679,22 +685,32 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
 #pragma unroll
         for (int q = 0; q < R; ++q) live[q] = i0 + 64 * q + lane < n;
         if (FAST) {
+            // one 32-byte bucket per record, read as two 16-byte halves in an
+            // order that alternates with bucket bit 3, so the lanes of a wave
+            // spread over all sixteen 16-byte LDS bank groups
             uint32_t si[R];
-            int64_t sk[R];
             bool hit[R], more[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                si[q] = (uint32_t)key[q] & tmask;
-                sk[q] = tk[si[q]];
+                const uint32_t b0 = ((uint32_t)key[q] << 2) & tmask;  // first slot of the bucket
+                const uint32_t rot = (b0 >> 5) & 1;
+                longlong2 kv[2];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) kv[c] = *(const longlong2 *)(tk + b0 + 2 * ((c + rot) & 1));
+                uint32_t match = 0, empty = 0;
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const uint32_t sh = 2 * ((c + rot) & 1);
+                    match |= (kv[c].x == key[q] ? 1u : 0u) << sh | (kv[c].y == key[q] ? 1u : 0u) << (sh + 1);
+                    empty |= (kv[c].x == EMPTY_KEY ? 1u : 0u) << sh | (kv[c].y == EMPTY_KEY ? 1u : 0u) << (sh + 1);
+                }
+                hit[q] = match != 0;
+                si[q] = b0 + (hit[q] ? (uint32_t)__builtin_ctz(match) : 3u);
+                more[q] = live[q] && !hit[q] && empty == 0;  // full bucket without the key: read on
             }
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                hit[q] = sk[q] == key[q];
-                more[q] = live[q] && !hit[q] && sk[q] != EMPTY_KEY;
-            }
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                while (__ballot(more[q])) {  // lockstep walk of the records still unresolved
+                while (__ballot(more[q])) {  // rare: walk on past a full bucket
                     if (more[q]) {
                         si[q] = (si[q] + 1) & tmask;
                         const int64_t x = tk[si[q]];
@@ -718,7 +734,7 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 if (!live[q]) continue;
-                uint32_t si = (uint32_t)key[q] & tmask;
+                uint32_t si = ((uint32_t)key[q] << 2) & tmask;  // first slot of the home bucket
                 int64_t m = 0;
                 for (;;) {
                     const int64_t sk = tk[si];
@@ -752,9 +768,14 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     acc_finish(a.pa, j, lrow);
 }
 
+static int scatter_npc(const JoinPartArgs &a) { return a.pa.pred.n == 0 ? 0 : a.pa.pred.n_pcols; }
+// Rows per thread per tile: 8 when a record is one word and the predicate reads
+// at most one column (the double-buffered loads then fit the registers), else 4.
+int join_scatter_rows(const JoinPartArgs &a) { return a.RW == 1 && scatter_npc(a) <= 1 ? 8 : 4; }
 size_t join_scatter_lds(const JoinPartArgs &a) {
     const size_t P2 = ((size_t)1 << a.log_p) + 2;
-    return (size_t)JS_TILE * a.RW * 8 + 3 * P2 * 4 + 16 * 4 + (size_t)JS_TILE * 2;
+    const size_t tile = (size_t)JS_TPB * join_scatter_rows(a);
+    return tile * a.RW * 8 + tile * 4 + 3 * P2 * 4 + 16 * 4;
 }
 size_t join_probe_lds(const JoinPartArgs &a) {
     return (size_t)a.T * (a.need_vals ? 16 : 8) + 64 * 8 + (size_t)a.G * 4;
@@ -771,9 +792,9 @@ static void probe_fns(std::vector<const void *> &v) {
 }
 template <int NPC>
 static void scatter_fns(std::vector<const void *> &v) {
-    v.push_back((const void *)k_jp_scatter<NPC, 1>);
-    v.push_back((const void *)k_jp_scatter<NPC, 2>);
-    v.push_back((const void *)k_jp_scatter<NPC, 3>);
+    v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4>);
+    v.push_back((const void *)k_jp_scatter<NPC, 2, 4>);
+    v.push_back((const void *)k_jp_scatter<NPC, 3, 4>);
 }
 static hipError_t join_allow_lds_once() {
     static std::once_flag once;
@@ -801,17 +822,18 @@ static hipError_t join_allow_lds_once() {
 template <int NPC>
 static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, size_t lds) {
     const dim3 g((unsigned)a.G), b(JS_TPB);
-    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1>), g, b, lds, s, a);
-    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3>), g, b, lds, s, a);
+    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4>), g, b, lds, s, a);
+    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 4>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3, 4>), g, b, lds, s, a);
 }
 
 hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int block) {
     (void)block;
     const size_t lds = join_scatter_lds(a);
-    if (lds > JOIN_LDS_MAX || a.log_p > 14 || a.RW < 1 || a.RW > 3) return hipErrorInvalidValue;
+    if (lds > JOIN_LDS_MAX || a.log_p > 14 || a.RW < 1 || a.RW > 3 || a.Cw >= (1 << 17))
+        return hipErrorInvalidValue;
     if (hipError_t e = join_allow_lds_once()) return e;
-    const int n = a.pa.pred.n == 0 ? 0 : a.pa.pred.n_pcols;
+    const int n = scatter_npc(a);
     if (n == 0) launch_scatter_npc<0>(s, a, lds);
     else if (n == 1) launch_scatter_npc<1>(s, a, lds);
     else if (n == 2) launch_scatter_npc<2>(s, a, lds);

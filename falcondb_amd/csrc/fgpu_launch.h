@@ -229,12 +229,12 @@ struct JoinPartArgs {
     int split;                // pass-2 workgroups per partition
     int need_vals;            // an aggregate reads a build column (pass 2 keeps slot values in LDS)
 };
-// Partitioned join pass 1: JS_TPB threads per workgroup, JS_U rows each per tile.
+// Partitioned join pass 1: JS_TPB threads per workgroup, join_scatter_rows()
+// rows each per tile.
 constexpr int JS_TPB = 1024;
-constexpr int JS_U = 4;
-constexpr int JS_TILE = JS_TPB * JS_U;
 constexpr size_t JOIN_LDS_MAX = 160 * 1024;  // dynamic LDS a join workgroup may take
 // LDS bytes of one pass-1 / pass-2 workgroup for these arguments.
+int join_scatter_rows(const JoinPartArgs &a);
 size_t join_scatter_lds(const JoinPartArgs &a);
 size_t join_probe_lds(const JoinPartArgs &a);
 hipError_t launch_join_part_count(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);

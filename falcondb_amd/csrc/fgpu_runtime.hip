@@ -153,7 +153,7 @@ struct Tuning {
     int join_partitions = 0;       // least radix partitions of the build / probe sides (0: from the build rows)
     int join_part_wgs_per_cu = 2;  // pass-1 workgroups per CU (two fit by registers)
     int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
-    int join_slots_per_key = 2;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
+    int join_slots_per_key = 4;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
 };
 
 struct fgpu_ctx {
@@ -1502,6 +1502,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     if (join_probe_lds(jp) > (size_t)tu.lds_limit) return -1;
     jp.rows_per_wg = (probe->n_rows + jp.G - 1) / jp.G;
     jp.Cw = ((int)std::ceil((double)jp.rows_per_wg / (double)P * 1.15) + 64 + 15) / 16 * 16;  // whole lines
+    if (jp.Cw >= (1 << 17)) return -1;  // a record's position in its cell is packed in 17 bits
     const int64_t fit2 = std::max<int64_t>(1, std::min<int64_t>(tu.join_probe_wgs_per_cu,
                                                                  (int64_t)tu.lds_limit / (int64_t)join_probe_lds(jp)));
     jp.split = (int)std::max<int64_t>(1, (int64_t)c->num_cus * fit2 / P);

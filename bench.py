@@ -178,7 +178,7 @@ def _serve_stream(srv, reqs, kinds, qps, seed_):
             "failed": int(st1["failed"] - st0["failed"])}
 
 
-def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=16 << 30):
+def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
     """The serving form: a device-resident fact table (the 100M-row group-by
     table plus a 12-value column g) and the 1M-row dimension table, a request
     mix of filtered COUNT(*), GROUP BY g / key, INNER / SEMI joins on

@@ -153,13 +153,13 @@ struct GbpScan {
 
 template <int U, int NPC>
 struct GbpPredRaw {
-    RawCell c[U][NPC];
+    RawCell3 c[U][NPC];
 };
 
 template <int U, int NIN>
 struct GbpDataRaw {
-    RawCell key[U];
-    RawCell in[U][NIN > 0 ? NIN : 1];
+    RawCell3 key[U];
+    RawCell3 in[U][NIN > 0 ? NIN : 1];
     uint32_t pass;
 };
 
@@ -192,7 +192,7 @@ FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r
         const uint32_t i = gbp_pred_idx(base, u, r1);
 #pragma unroll
         for (int j = 0; j < NPC; ++j)
-            if (j < g.n_pcols) cell_issue_tile(g.pc[j], base, i, t.c[u][j]);
+            cell_issue3_tile(g.pc[j], base, i, t.c[u][j]);  // unused slots re-read column 0: loads stay fixed
     }
 }
 
@@ -213,7 +213,7 @@ FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &a, const GbpScan<NPC, NIN> &g, i
             v[u][j] = 0;
             if (j < g.n_pcols) {
                 bool okj;
-                cell_finish(g.pc[j], r, t.c[u][j], v[u][j], okj);
+                cell_finish3(g.pc[j], r, t.c[u][j], v[u][j], okj);
                 ok[u] |= (okj ? 1u : 0u) << j;
             }
         }
@@ -230,9 +230,9 @@ FGPU_DEV void gbp_issue_data(const GbpScan<NPC, NIN> &g, int64_t base, uint32_t 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t i = ((pass >> u) & 1u) ? (uint32_t)(u * 64 + (threadIdx.x & 63)) : 0u;
-        cell_issue_tile(g.key, base, i, t.key[u]);
+        cell_issue3_tile(g.key, base, i, t.key[u]);
 #pragma unroll
-        for (int j = 0; j < NIN; ++j) cell_issue_tile(g.in[j], base, i, t.in[u][j]);
+        for (int j = 0; j < NIN; ++j) cell_issue3_tile(g.in[j], base, i, t.in[u][j]);
     }
 }
 
@@ -244,11 +244,11 @@ FGPU_DEV void gbp_finish_data(const GbpScan<NPC, NIN> &g, int64_t base, const Gb
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         bool ok;
-        cell_finish(g.key, r, t.key[u], d.r[u].key, ok);
+        cell_finish3(g.key, r, t.key[u], d.r[u].key, ok);
         d.r[u].vw = (ok && d.r[u].key != EMPTY_KEY) ? GBP_HAS_KEY : 0u;
 #pragma unroll
         for (int j = 0; j < NIN; ++j) {
-            cell_finish(g.in[j], r, t.in[u][j], d.r[u].in[j], ok);
+            cell_finish3(g.in[j], r, t.in[u][j], d.r[u].in[j], ok);
             d.r[u].vw |= (ok ? 1u : 0u) << j;
         }
     }

@@ -363,6 +363,17 @@ FGPU_DEV void cell_issue3(const ColRef &c, int64_t r, RawCell3 &o) {
     o.hi = *(wide ? lo + 1 : lo);
     o.vw = *vp;
 }
+// Row tile0 + idx, tile0 a multiple of 32: the bases stay uniform (SGPR) and
+// the lane offsets 32-bit.
+FGPU_DEV void cell_issue3_tile(const ColRef &c, int64_t tile0, uint32_t idx, RawCell3 &o) {
+    const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;
+    const uint32_t *b = (const uint32_t *)c.base + (wide ? 2 * tile0 : tile0);
+    const uint32_t off = wide ? 2 * idx : idx;
+    const uint32_t *vb = c.valid ? c.valid + (tile0 >> 5) : b;
+    o.lo = b[off];
+    o.hi = b[wide ? off + 1 : off];
+    o.vw = vb[c.valid ? (idx >> 5) : off];
+}
 FGPU_DEV void cell_finish3(const ColRef &c, int64_t r, const RawCell3 &o, int64_t &v, bool &ok) {
     const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;
     const uint64_t bits = wide ? ((uint64_t)o.hi << 32 | o.lo) : (uint64_t)o.lo;

@@ -392,19 +392,29 @@ struct ScatterRaw {
     RawCell3 f1[U], f2[U];
 };
 
-template <int NPC, int RW, int U>
+// MODE selects where records go.  JS_CELLS: cell (w, p) of fixed capacity Cw,
+// partition-major; a full cell sets flags[0] (once per workgroup) and the host
+// redoes the partitioning exactly.  JS_COUNT: nothing is written; the number
+// of records of (w, p) goes to cell_cnt[p * G + w].  JS_EXACT: records of
+// (w, p) go to a flat array at xoff[p * G + w] (the exclusive prefix sum of
+// the JS_COUNT pass), so each partition's records are contiguous.
+enum { JS_CELLS = 0, JS_COUNT = 1, JS_EXACT = 2 };
+
+template <int NPC, int RW, int U, int MODE>
 __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     extern __shared__ uint64_t js_lds[];
+    __shared__ int overflow;
     constexpr int TILE = JS_TPB * U;
+    constexpr bool STAGE = MODE != JS_COUNT;
     const JoinProbeArgs &pa = a.pa;
     const int P2 = (1 << a.log_p) + 2;
     const int Cw = a.Cw, log_p = a.log_p;
-    int64_t *stage = (int64_t *)js_lds;                        // TILE * RW words
-    uint32_t *dest = (uint32_t *)(stage + (size_t)TILE * RW);  // TILE: partition << 17 | position in its cell
-    unsigned *hist = dest + TILE;                              // P2
-    unsigned *off = hist + P2;                                 // P2
-    unsigned *cur = off + P2;                                  // P2: records in this workgroup's cells
-    unsigned *tmp = cur + P2;                                  // 16
+    int64_t *stage = (int64_t *)js_lds;                                     // TILE * RW words (staging modes)
+    uint32_t *dest = (uint32_t *)(stage + (STAGE ? (size_t)TILE * RW : 0));  // TILE: partition << 17 | position
+    unsigned *hist = dest + (STAGE ? TILE : 0);                             // P2
+    unsigned *off = hist + P2;                                              // P2
+    unsigned *cur = off + P2;                                               // P2: records of this workgroup so far
+    unsigned *tmp = cur + P2;                                               // 16
     const ColRef kc = col_ref(pa.pcols, pa.pkey);
     ColRef pc[NPC > 0 ? NPC : 1];
 #pragma unroll
@@ -413,10 +423,10 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     // payload columns have no NULLs (the host checks), so no validity word is read
     const ColRef pay0 = col_ref(pa.pcols, a.pay_col[0]), pay1 = col_ref(pa.pcols, a.pay_col[1]);
     for (int i = threadIdx.x; i < P2; i += blockDim.x) hist[i] = cur[i] = 0;
+    if (threadIdx.x == 0) overflow = 0;
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_wg;
     const int64_t r1 = r0 + a.rows_per_wg < pa.n_probe ? r0 + a.rows_per_wg : pa.n_probe;
-    // cells are partition-major: cell (w, p) holds Cw records at ((p * G + w) * Cw) * RW
     const int64_t G = a.G;
     auto issue = [&](int64_t base, ScatterRaw<NPC, U> &t) {
 #pragma unroll
@@ -426,8 +436,8 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
 #pragma unroll
             for (int j = 0; j < NPC; ++j) cell_issue3(pc[j], r, t.pc[u][j]);
             cell_issue3(kc, r, t.key[u]);
-            if (RW > 1) cell_issue3(pay0, r, t.f1[u]);
-            if (RW > 2) cell_issue3(pay1, r, t.f2[u]);
+            if (STAGE && RW > 1) cell_issue3(pay0, r, t.f1[u]);
+            if (STAGE && RW > 2) cell_issue3(pay1, r, t.f2[u]);
         }
     };
     ScatterRaw<NPC, U> raw;
@@ -456,10 +466,17 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             int64_t key;
             bool ok, pok;
             cell_finish3(kc, r, raw.key[u], key, ok);
-            if (RW > 1) cell_finish3(pay0, r, raw.f1[u], f1[u], pok);
-            if (RW > 2) cell_finish3(pay1, r, raw.f2[u], f2[u], pok);
+            if (STAGE && RW > 1) cell_finish3(pay0, r, raw.f1[u], f1[u], pok);
+            if (STAGE && RW > 2) cell_finish3(pay1, r, raw.f2[u], f2[u], pok);
             h[u] = (int64_t)hash64((uint64_t)key);
             part[u] = !sel ? -1 : !ok ? P2 - 2 : h[u] == EMPTY_KEY ? P2 - 1 : part_of((uint64_t)h[u], log_p);
+        }
+        if (!STAGE) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (part[u] >= 0) atomicAdd(cur + part[u], 1u);
+            raw = nxt;
+            continue;
         }
         unsigned rank[U];
 #pragma unroll
@@ -473,31 +490,44 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             stage[(size_t)i * RW] = h[u];
             if (RW > 1) stage[(size_t)i * RW + 1] = f1[u];
             if (RW > 2) stage[(size_t)i * RW + 2] = f2[u];
-            dest[i] = (uint32_t)part[u] << 17 | (cur[part[u]] + rank[u]);
+            // cells: the position is packed beside the partition (Cw < 2^17);
+            // exact mode keeps the partition only (a hot partition can hold
+            // more than 2^17 of a workgroup's records)
+            dest[i] = MODE == JS_EXACT ? (uint32_t)part[u] : (uint32_t)part[u] << 17 | (cur[part[u]] + rank[u]);
         }
         __syncthreads();
         for (unsigned i = threadIdx.x; i < total; i += blockDim.x) {
             const uint32_t d = dest[i];
-            const int64_t pp = d >> 17;
-            const unsigned pos = d & 0x1ffffu;
-            if (pos < (unsigned)Cw) {
-                int64_t *rec = a.cells + ((pp * G + blockIdx.x) * Cw + pos) * RW;
-                rec[0] = stage[(size_t)i * RW];
-                if (RW > 1) rec[1] = stage[(size_t)i * RW + 1];
-                if (RW > 2) rec[2] = stage[(size_t)i * RW + 2];
+            int64_t *rec;
+            if (MODE == JS_EXACT) {
+                const int64_t pos = (int64_t)cur[d] + (i - off[d]);
+                rec = a.cells + (a.xoff[(int64_t)d * G + blockIdx.x] + pos) * RW;
+            } else if ((d & 0x1ffffu) < (unsigned)Cw) {
+                rec = a.cells + (((int64_t)(d >> 17) * G + blockIdx.x) * Cw + (d & 0x1ffffu)) * RW;
             } else {
-                atomicExch(a.flags, 1ull);  // cell full: the host reruns on the direct path
+                overflow = 1;  // cell full: the host redoes the partitioning exactly
+                continue;
             }
+            rec[0] = stage[(size_t)i * RW];
+            if (RW > 1) rec[1] = stage[(size_t)i * RW + 1];
+            if (RW > 2) rec[2] = stage[(size_t)i * RW + 2];
         }
         __syncthreads();
         for (int i = threadIdx.x; i < P2; i += blockDim.x) {
-            cur[i] = min(cur[i] + hist[i], (unsigned)Cw);  // positions stay below 2^17 (Cw < 2^17)
+            // cells: positions stay below Cw < 2^17 (they are packed in 17 bits)
+            cur[i] = MODE == JS_EXACT ? cur[i] + hist[i] : min(cur[i] + hist[i], (unsigned)Cw);
             hist[i] = 0;
         }
         __syncthreads();
         raw = nxt;
     }
-    for (int i = threadIdx.x; i < P2; i += blockDim.x) a.cell_cnt[(int64_t)blockIdx.x * P2 + i] = cur[i];
+    __syncthreads();
+    if (MODE == JS_COUNT) {
+        for (int i = threadIdx.x; i < P2; i += blockDim.x) a.cell_cnt[(int64_t)i * G + blockIdx.x] = cur[i];
+    } else if (MODE == JS_CELLS) {
+        for (int i = threadIdx.x; i < P2; i += blockDim.x) a.cell_cnt[(int64_t)blockIdx.x * P2 + i] = cur[i];
+        if (threadIdx.x == 0 && overflow) atomicExch(a.flags, 1ull);
+    }
 }
 
 // One output row of the partitioned join, general form: probe payloads f1, f2
@@ -591,6 +621,31 @@ FGPU_DEV void probe_side_partitions(const JoinPartArgs &a, JoinAcc &j) {
     }
 }
 
+// Side-partition records recs[0..n) (sp 0: NULL keys, 1: the sentinel key),
+// shared by all threads of the workgroup.
+template <int TYPE>
+FGPU_DEV void side_records(const JoinPartArgs &a, JoinAcc &j, const int64_t *recs, int64_t n, int sp) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int64_t *rec = recs + i * a.RW;
+        const int64_t f1 = a.RW > 1 ? rec[1] : 0, f2 = a.RW > 2 ? rec[2] : 0;
+        int64_t m = 0;
+        if (sp == 1) {
+            for (int64_t q = 0; q < a.pa.n_min_key_rows; ++q) {
+                ++m;
+                if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
+                    int64_t pay = a.pa.min_key_rows[q];
+                    if (a.build_pay_col >= 0) pay = load_raw(a.pa.bcols, a.build_pay_col, pay);
+                    part_emit(a, j, f1, f2, true, pay);
+                } else {
+                    break;
+                }
+            }
+        }
+        if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) || (TYPE == FGPU_ANTI && m == 0))
+            part_emit(a, j, f1, f2, false, 0);
+    }
+}
+
 // Pass 2.  Workgroup b serves partition p = b % P: it copies the partition's
 // sub-table into LDS (slot hashes and build values in two arrays) and the
 // record counts of the partition's cells, then streams the records of its
@@ -612,26 +667,56 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     int64_t *tk = (int64_t *)jp_lds;                        // T slot hashes
     int64_t *tv = tk + T;                                   // T build values / rows (if vals)
     uint64_t *lrow = jp_lds + (vals ? 2 : 1) * (size_t)T;   // 64 words
-    unsigned *cnt = (unsigned *)(lrow + 64);                // G: records of each cell of partition p
-    const int p = blockIdx.x & (P - 1);
-    const int k = blockIdx.x >> a.log_p, K = gridDim.x >> a.log_p;
-    const longlong2 *src = (const longlong2 *)(a.sub + (int64_t)p * T * 2);
-    for (int i = threadIdx.x; i < T; i += blockDim.x) {
-        const longlong2 sl = src[i];
-        tk[i] = sl.x;
-        if (vals) tv[i] = sl.y;
+    unsigned *cnt = (unsigned *)(lrow + 64);                // G: records of each cell (or wave slice)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // Cell mode: workgroup b serves partition b % P, cells [w0, w1) of it.
+    // Chunk mode (exact partitioning): workgroup b serves chunks[b] = {p, lo,
+    // n}, records lo .. lo + n of the flat array; wave w takes slice w of it.
+    const bool chunked = a.chunks != nullptr;
+    int p, w0, w1;
+    int64_t lo = 0, cn = 0, S = 0;
+    if (chunked) {
+        p = (int)a.chunks[3 * (int64_t)blockIdx.x];
+        lo = a.chunks[3 * (int64_t)blockIdx.x + 1];
+        cn = a.chunks[3 * (int64_t)blockIdx.x + 2];
+        S = (cn + 64 * nw - 1) / (64 * nw) * 64;  // slice length, a multiple of 64
+        w0 = 0;
+        w1 = nw;
+    } else {
+        p = blockIdx.x & (P - 1);
+        const int k = blockIdx.x >> a.log_p, K = gridDim.x >> a.log_p;
+        w0 = (int)((int64_t)G * k / K);
+        w1 = (int)((int64_t)G * (k + 1) / K);
     }
-    for (int w = threadIdx.x; w < G; w += blockDim.x) {
-        const unsigned c = a.cell_cnt[(int64_t)w * P2 + p];
-        cnt[w] = c < (unsigned)Cw ? c : (unsigned)Cw;
+    if (p < P) {
+        const longlong2 *src = (const longlong2 *)(a.sub + (int64_t)p * T * 2);
+        for (int i = threadIdx.x; i < T; i += blockDim.x) {
+            const longlong2 sl = src[i];
+            tk[i] = sl.x;
+            if (vals) tv[i] = sl.y;
+        }
+    }
+    if (chunked) {
+        for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+            const int64_t rest = cn - (int64_t)w * S;
+            cnt[w] = (unsigned)(rest <= 0 ? 0 : rest < S ? rest : S);
+        }
+    } else {
+        for (int w = threadIdx.x; w < G; w += blockDim.x) {
+            const unsigned c = a.cell_cnt[(int64_t)w * P2 + p];
+            cnt[w] = c < (unsigned)Cw ? c : (unsigned)Cw;
+        }
     }
     JoinAcc j;
     acc_init(a.pa, j, lrow);  // synchronises the workgroup (table and counts are in place)
+    if (p >= P) {  // chunk of the NULL-key or sentinel-key side partition
+        side_records<TYPE>(a, j, a.cells + lo * RW, cn, p - P);
+        acc_finish(a.pa, j, lrow);
+        return;
+    }
     const EmitOps e = emit_ops(a);
     const bool unique = a.pa.unique_keys != 0;
     const uint32_t tmask = (uint32_t)T - 1;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int w0 = (int)((int64_t)G * k / K), w1 = (int)((int64_t)G * (k + 1) / K);
     constexpr int R = 4;  // records per lane per group
     // (w, i0): the wave's current cell and first record of the group
     int w = w0 + wave;
@@ -642,12 +727,13 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     }
     int64_t key[R], f1[R], f2[R];
     // Loads are issued unconditionally (a lane past the cell's end, or a wave
-    // past its last cell, reads record 0 of cell (0, p), which always exists),
-    // so the number of loads in flight is fixed and the wait before probing
-    // a group does not also wait for the next group's loads.
-    const int64_t *cell0 = a.cells + (int64_t)p * G * Cw * RW;  // cell (0, p): partition-major layout
+    // past its last cell, reads the first record of the partition's cells or
+    // chunk, which always exists), so the number of loads in flight is fixed
+    // and the wait before probing a group does not also wait for the next
+    // group's loads.
+    const int64_t *cell0 = chunked ? a.cells + lo * RW : a.cells + (int64_t)p * G * Cw * RW;
     auto load_group = [&](int cw, unsigned ci0, unsigned cn, int64_t (&kk)[R], int64_t (&g1)[R], int64_t (&g2)[R]) {
-        const int64_t *cell = cw < w1 ? cell0 + (int64_t)cw * Cw * RW : cell0;
+        const int64_t *cell = cw >= w1 ? cell0 : chunked ? cell0 + (int64_t)cw * S * RW : cell0 + (int64_t)cw * Cw * RW;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const unsigned i = ci0 + 64 * q + lane;
@@ -764,21 +850,25 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
         i0 = in;
         n = nn;
     }
-    if (blockIdx.x == 0) probe_side_partitions<TYPE>(a, j);
+    if (!chunked && blockIdx.x == 0) probe_side_partitions<TYPE>(a, j);
     acc_finish(a.pa, j, lrow);
 }
 
 static int scatter_npc(const JoinPartArgs &a) { return a.pa.pred.n == 0 ? 0 : a.pa.pred.n_pcols; }
-// Rows per thread per tile: 8 when a record is one word and the predicate reads
-// at most one column (the double-buffered loads then fit the registers), else 4.
-int join_scatter_rows(const JoinPartArgs &a) { return a.RW == 1 && scatter_npc(a) <= 1 ? 8 : 4; }
-size_t join_scatter_lds(const JoinPartArgs &a) {
+// Rows per thread per tile: 8 when a record is one word (or nothing is staged)
+// and the predicate reads at most one column -- the double-buffered loads then
+// fit the registers -- else 4.
+int join_scatter_rows(const JoinPartArgs &a, int mode) {
+    return (a.RW == 1 || mode == JS_COUNT) && scatter_npc(a) <= 1 ? 8 : 4;
+}
+size_t join_scatter_lds(const JoinPartArgs &a, int mode) {
     const size_t P2 = ((size_t)1 << a.log_p) + 2;
-    const size_t tile = (size_t)JS_TPB * join_scatter_rows(a);
-    return tile * a.RW * 8 + tile * 4 + 3 * P2 * 4 + 16 * 4;
+    const size_t tile = (size_t)JS_TPB * join_scatter_rows(a, mode);
+    const size_t staged = mode == JS_COUNT ? 0 : tile * a.RW * 8 + tile * 4;
+    return staged + 3 * P2 * 4 + 16 * 4;
 }
 size_t join_probe_lds(const JoinPartArgs &a) {
-    return (size_t)a.T * (a.need_vals ? 16 : 8) + 64 * 8 + (size_t)a.G * 4;
+    return (size_t)a.T * (a.need_vals ? 16 : 8) + 64 * 8 + (size_t)(a.G > 16 ? a.G : 16) * 4;
 }
 
 // Lets every pass-1 / pass-2 instantiation take up to JOIN_LDS_MAX bytes of
@@ -790,21 +880,29 @@ static void probe_fns(std::vector<const void *> &v) {
     v.push_back((const void *)k_jp_probe<FGPU_SEMI, FAST, RW>);
     v.push_back((const void *)k_jp_probe<FGPU_ANTI, FAST, RW>);
 }
-template <int NPC>
+template <int NPC, int MODE>
 static void scatter_fns(std::vector<const void *> &v) {
-    v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4>);
-    v.push_back((const void *)k_jp_scatter<NPC, 2, 4>);
-    v.push_back((const void *)k_jp_scatter<NPC, 3, 4>);
+    v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>);
+    v.push_back((const void *)k_jp_scatter<NPC, 2, 4, MODE>);
+    v.push_back((const void *)k_jp_scatter<NPC, 3, 4, MODE>);
 }
 static hipError_t join_allow_lds_once() {
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
         std::vector<const void *> fns;
-        scatter_fns<0>(fns);
-        scatter_fns<1>(fns);
-        scatter_fns<2>(fns);
-        scatter_fns<MAX_PCOLS>(fns);
+        scatter_fns<0, JS_CELLS>(fns);
+        scatter_fns<1, JS_CELLS>(fns);
+        scatter_fns<2, JS_CELLS>(fns);
+        scatter_fns<MAX_PCOLS, JS_CELLS>(fns);
+        scatter_fns<0, JS_EXACT>(fns);
+        scatter_fns<1, JS_EXACT>(fns);
+        scatter_fns<2, JS_EXACT>(fns);
+        scatter_fns<MAX_PCOLS, JS_EXACT>(fns);
+        fns.push_back((const void *)k_jp_scatter<0, 1, 8, JS_COUNT>);
+        fns.push_back((const void *)k_jp_scatter<1, 1, 8, JS_COUNT>);
+        fns.push_back((const void *)k_jp_scatter<2, 1, 4, JS_COUNT>);
+        fns.push_back((const void *)k_jp_scatter<MAX_PCOLS, 1, 4, JS_COUNT>);
         probe_fns<true, 1>(fns);
         probe_fns<true, 2>(fns);
         probe_fns<true, 3>(fns);
@@ -819,25 +917,35 @@ static hipError_t join_allow_lds_once() {
     return err;
 }
 
-template <int NPC>
+template <int NPC, int MODE>
 static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, size_t lds) {
     const dim3 g((unsigned)a.G), b(JS_TPB);
-    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4>), g, b, lds, s, a);
-    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 4>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3, 4>), g, b, lds, s, a);
+    if (MODE == JS_COUNT) {
+        hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, JS_COUNT>), g, b, lds, s, a);
+        return;
+    }
+    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>), g, b, lds, s, a);
+    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 4, MODE>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3, 4, MODE>), g, b, lds, s, a);
+}
+template <int MODE>
+static void launch_scatter_mode(hipStream_t s, const JoinPartArgs &a, size_t lds) {
+    const int n = scatter_npc(a);
+    if (n == 0) launch_scatter_npc<0, MODE>(s, a, lds);
+    else if (n == 1) launch_scatter_npc<1, MODE>(s, a, lds);
+    else if (n == 2) launch_scatter_npc<2, MODE>(s, a, lds);
+    else launch_scatter_npc<MAX_PCOLS, MODE>(s, a, lds);
 }
 
-hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int block) {
-    (void)block;
-    const size_t lds = join_scatter_lds(a);
-    if (lds > JOIN_LDS_MAX || a.log_p > 14 || a.RW < 1 || a.RW > 3 || a.Cw >= (1 << 17))
+hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int mode) {
+    const size_t lds = join_scatter_lds(a, mode);
+    if (lds > JOIN_LDS_MAX || a.log_p > 14 || a.RW < 1 || a.RW > 3 || a.Cw >= (1 << 17) ||
+        (mode == JS_EXACT && !a.xoff))
         return hipErrorInvalidValue;
     if (hipError_t e = join_allow_lds_once()) return e;
-    const int n = scatter_npc(a);
-    if (n == 0) launch_scatter_npc<0>(s, a, lds);
-    else if (n == 1) launch_scatter_npc<1>(s, a, lds);
-    else if (n == 2) launch_scatter_npc<2>(s, a, lds);
-    else launch_scatter_npc<MAX_PCOLS>(s, a, lds);
+    if (mode == JS_COUNT) launch_scatter_mode<JS_COUNT>(s, a, lds);
+    else if (mode == JS_EXACT) launch_scatter_mode<JS_EXACT>(s, a, lds);
+    else launch_scatter_mode<JS_CELLS>(s, a, lds);
     return hipGetLastError();
 }
 
@@ -858,12 +966,12 @@ static void launch_probe_fast(hipStream_t s, const JoinPartArgs &a, dim3 grid, s
     else launch_probe_rw<FAST, 3>(s, a, grid, lds);
 }
 
-hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int block) {
-    (void)block;
+hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int n_chunks) {
     const size_t lds = join_probe_lds(a);
-    if (lds > JOIN_LDS_MAX) return hipErrorInvalidValue;
+    if (lds > JOIN_LDS_MAX || (n_chunks > 0 && !a.chunks) || (n_chunks == 0 && a.chunks)) return hipErrorInvalidValue;
     if (hipError_t e = join_allow_lds_once()) return e;
-    const dim3 grid((unsigned)(a.split << a.log_p));  // a.split = workgroups per partition
+    // cell mode: a.split workgroups per partition; chunk mode: one per chunk
+    const dim3 grid(n_chunks > 0 ? (unsigned)n_chunks : (unsigned)(a.split << a.log_p));
     // fast form: unique build keys, and the build value (if any) inlined in the slots
     bool build_rows = false;
     for (int i = 0; i < a.pa.n_ops; ++i) build_rows |= a.pa.ops[i].side == 1 && a.pa.ops[i].col >= 0;

@@ -228,19 +228,23 @@ struct JoinPartArgs {
     int64_t min_key_cap;
     int split;                // pass-2 workgroups per partition
     int need_vals;            // an aggregate reads a build column (pass 2 keeps slot values in LDS)
+    const int64_t *xoff;      // exact partitioning: record offset of (w, p) at [p * G + w]
+    const int64_t *chunks;    // exact partitioning: pass-2 work items {p, first record, records}
 };
 // Partitioned join pass 1: JS_TPB threads per workgroup, join_scatter_rows()
 // rows each per tile.
 constexpr int JS_TPB = 1024;
-constexpr size_t JOIN_LDS_MAX = 160 * 1024;  // dynamic LDS a join workgroup may take
+constexpr size_t JOIN_LDS_MAX = 159 * 1024;  // dynamic LDS a join workgroup may take (static use < 1 KB)
 // LDS bytes of one pass-1 / pass-2 workgroup for these arguments.
-int join_scatter_rows(const JoinPartArgs &a);
-size_t join_scatter_lds(const JoinPartArgs &a);
+int join_scatter_rows(const JoinPartArgs &a, int mode);
+size_t join_scatter_lds(const JoinPartArgs &a, int mode);
 size_t join_probe_lds(const JoinPartArgs &a);
 hipError_t launch_join_part_count(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);
 hipError_t launch_join_part_build(hipStream_t s, const JoinPartArgs &a, const JoinBuildArgs &b, int grid);
-hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int block);
-hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int block);
+// mode: 0 cells (optimistic), 1 count, 2 exact (see k_jp_scatter)
+hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int mode);
+// n_chunks > 0: chunk mode over a.chunks (grid = n_chunks); else cell mode.
+hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int n_chunks);
 hipError_t launch_join_build(hipStream_t s, const JoinBuildArgs &a, int grid);
 hipError_t launch_join_probe_agg(hipStream_t s, const JoinProbeArgs &a, int grid, int block);
 hipError_t launch_join_pairs(hipStream_t s, const JoinProbeArgs &a, int pass, int grid);

@@ -1495,9 +1495,9 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     jp.need_vals = bcols_used.empty() ? 0 : 1;
     // pass 1: as many workgroups per CU as LDS allows (at most the tuned count);
     // cells of P partitions plus the NULL-key and sentinel-key side partitions
-    if (join_scatter_lds(jp) > (size_t)tu.lds_limit) return -1;
+    if (join_scatter_lds(jp, 0) > (size_t)tu.lds_limit) return -1;
     const int64_t fit1 = std::max<int64_t>(1, std::min<int64_t>(tu.join_part_wgs_per_cu,
-                                                                 (int64_t)tu.lds_limit / (int64_t)join_scatter_lds(jp)));
+                                                                 (int64_t)tu.lds_limit / (int64_t)join_scatter_lds(jp, 0)));
     jp.G = (int)(c->num_cus * fit1);
     if (join_probe_lds(jp) > (size_t)tu.lds_limit) return -1;
     jp.rows_per_wg = (probe->n_rows + jp.G - 1) / jp.G;
@@ -1558,16 +1558,59 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     jp.pa.n_min_key_rows = (int64_t)h[2];
     jp.pa.unique_keys = !h[3] && h[2] <= 1;
     HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
+    jp.xoff = nullptr;
+    jp.chunks = nullptr;
     mark(c, 1);
-    HIPCHK(launch_join_part_scatter(c->stream, jp, c->tune.join_block));
-    HIPCHK(launch_join_part_probe(c->stream, jp, c->tune.join_block));
+    HIPCHK(launch_join_part_scatter(c->stream, jp, 0));
+    HIPCHK(launch_join_part_probe(c->stream, jp, 0));
     mark(c, 2);
     HIPCHK(hipMemcpyAsync(h, d_flags, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (h[0]) {  // a cell overflowed (skewed keys): redo on the direct path
-        HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
-        return -1;
+    if (!h[0]) return FGPU_OK;
+    // A cell overflowed: the probe keys are skewed (a hot key fills its
+    // partition).  Partition exactly instead: count the records of every
+    // (workgroup, partition), prefix-sum them into partition-major offsets,
+    // scatter to those offsets (each partition's records contiguous), and run
+    // pass 2 over chunks of at most `chunk` records, so a hot partition is
+    // shared by many workgroups.
+    c->plan += " (skewed probe keys: exact partitions)";
+    HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
+    const int64_t nc = (int64_t)jp.G * P2;
+    if ((rc = buf(c, B_CELLCNT, (size_t)nc * 4, &p))) return rc;
+    jp.cell_cnt = (unsigned int *)p;
+    int64_t *d_off, *d_tmp;
+    if ((rc = buf(c, B_OFFS, (size_t)(nc + 1) * 8, (void **)&d_off))) return rc;
+    if ((rc = buf(c, B_TMP, (size_t)((nc + 4095) / 4096 + 1) * 8, (void **)&d_tmp))) return rc;
+    HIPCHK(launch_join_part_scatter(c->stream, jp, 1));
+    HIPCHK(launch_scan_u32(c->stream, jp.cell_cnt, nc, d_off, d_tmp));
+    std::vector<int64_t> offs((size_t)nc + 1);
+    HIPCHK(hipMemcpyAsync(offs.data(), d_off, (size_t)(nc + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t total = offs[(size_t)nc];
+    if ((rc = buf(c, B_CELLS, (size_t)std::max<int64_t>(total, 1) * jp.RW * 8, &p))) return rc;
+    jp.cells = (int64_t *)p;
+    jp.xoff = d_off;
+    HIPCHK(launch_join_part_scatter(c->stream, jp, 2));
+    const int64_t chunk = std::max<int64_t>(16384, (total + 2 * c->num_cus - 1) / (2 * c->num_cus));
+    std::vector<int64_t> ch;
+    for (int64_t q = 0; q < P2; ++q) {
+        const int64_t base = offs[(size_t)(q * jp.G)], tot = offs[(size_t)((q + 1) * jp.G)] - base;
+        for (int64_t i = 0; i < tot; i += chunk) {
+            ch.push_back(q);
+            ch.push_back(base + i);
+            ch.push_back(std::min(chunk, tot - i));
+        }
     }
+    const int n_chunks = (int)(ch.size() / 3);
+    if (n_chunks) {
+        int64_t *d_ch;
+        if ((rc = buf(c, B_PAIRS, ch.size() * 8, (void **)&d_ch))) return rc;
+        HIPCHK(hipMemcpyAsync(d_ch, ch.data(), ch.size() * 8, hipMemcpyHostToDevice, c->stream));
+        jp.chunks = d_ch;
+        HIPCHK(launch_join_part_probe(c->stream, jp, n_chunks));
+        HIPCHK(hipStreamSynchronize(c->stream));  // ch is a local
+    }
+    mark(c, 2);
     return FGPU_OK;
 }
 

@@ -360,6 +360,28 @@ def test_join_with_probe_predicate(jctx):
         assert_batches_equal(g, o.exact, None)
 
 
+def test_join_skewed_probe_keys_exact_partitions(ctx_part):
+    # Zipf(1.1) probe keys: the hottest key alone fills its partition's cells,
+    # so the partitioned join redoes the partitioning exactly (count, offsets,
+    # chunked pass 2); results must still equal the oracle's
+    K = dg.keys(20_000, 40, seed_=dg.seed(3, 9))
+    probe = dg.c2_table(3_000_000, K)
+    pcols = {"pk": probe["key"], "ppay": probe["v1"], "f": probe["f"]}
+    bcols = dg.c3_build(K)
+    pt, bt = _table(ctx_part, pcols), _table(ctx_part, bcols)
+    idx = fa.JoinIndex(ctx_part, bt, "bk")
+    p = col("f") < 60
+    for jt in (INNER, LEFT, SEMI, ANTI):
+        aggs = [AggExpr(COUNT), AggExpr(SUM, "ppay"), AggExpr(MAX, "ppay")]
+        if jt in (INNER, LEFT):
+            aggs += [AggExpr(SUM, "bpay", side="build")]
+        g = fa.execute_hash_join(ctx_part, pt, bt, "pk", "bk", jt, aggs, filter=p, index=idx)
+        assert "exact partitions" in ctx_part.last_plan(), ctx_part.last_plan()
+        o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
+        assert_batches_equal(g, o.exact, None)
+    idx.free()
+
+
 def test_join_partitioned_matches_direct_at_scale(ctx_part, ctx_direct):
     # 4M probe rows over a 200k build: both paths, several aggregates per side
     pcols, bcols = _c3(200_000, 4_000_000)

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -107,18 +108,88 @@ struct fgpu_join_index {
 };
 
 // ─── results ────────────────────────────────────────────────────────────────
+// Vectors that leave new elements uninitialised: result columns are filled by
+// copies right away, so value-initialising them first is wasted host time.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... Args>
+    void construct(U *p, Args &&...args) {
+        ::new ((void *)p) U(std::forward<Args>(args)...);
+    }
+};
+template <class T>
+using RVec = std::vector<T, NoInitAlloc<T>>;
+
 struct RCol {
     std::string name;
     int dtype = FGPU_INT64;
-    std::vector<int64_t> i64;
-    std::vector<double> f64;
-    std::vector<uint32_t> u32;
-    std::vector<uint8_t> valid;
+    RVec<int64_t> i64;
+    RVec<double> f64;
+    RVec<uint32_t> u32;
+    RVec<uint8_t> valid;
     const DevCol *dict_src = nullptr;
+    // large results: values and validity live in the result's pinned block
+    const void *xv = nullptr;
+    const uint8_t *xvalid = nullptr;
 };
+// Pinned host blocks for result columns, kept by the process after a result
+// is freed: the next result of a similar size is copied by DMA into memory
+// that is already pinned and faulted in (a fresh allocation costs a page fault
+// per 4 KB on first touch, more than the copy itself).
+namespace {
+std::mutex g_pin_mu;
+std::multimap<size_t, void *> g_pin_free;
+size_t g_pin_free_bytes = 0;
+constexpr size_t PIN_KEEP_BYTES = size_t(1) << 30;  // at most this much is kept unused
+
+void *pin_get(size_t bytes, size_t *got) {
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        auto it = g_pin_free.lower_bound(bytes);
+        if (it != g_pin_free.end() && it->first <= 2 * bytes + (size_t(4) << 20)) {
+            void *p = it->second;
+            *got = it->first;
+            g_pin_free_bytes -= it->first;
+            g_pin_free.erase(it);
+            return p;
+        }
+    }
+    const size_t n = (bytes + (size_t(2) << 20) - 1) / (size_t(2) << 20) * (size_t(2) << 20);
+    void *p = nullptr;
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) return nullptr;
+    *got = n;
+    return p;
+}
+void pin_put(void *p, size_t n) {
+    if (!p) return;
+    std::unique_lock<std::mutex> lk(g_pin_mu);
+    if (g_pin_free_bytes + n > PIN_KEEP_BYTES) {
+        lk.unlock();
+        (void)hipHostFree(p);
+        return;
+    }
+    g_pin_free.emplace(n, p);
+    g_pin_free_bytes += n;
+}
+}  // namespace
+
 struct fgpu_result {
     int64_t rows = 0;
     std::vector<RCol> cols;
+    void *block = nullptr;  // pinned storage of the columns that point into it
+    size_t block_n = 0;
+    ~fgpu_result() { pin_put(block, block_n); }
 };
 
 // ─── context ────────────────────────────────────────────────────────────────
@@ -515,6 +586,9 @@ int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const f
 // Copies the emitted rows of the last GROUP BY / aggregate into a result.
 // `known` >= 0: the row count is already on the host (a batch copies all of
 // them at once); otherwise it is read from d_count.
+// Copies the emitted groups into a result.  The columns' values and validity
+// bytes are copied by DMA straight into the result's pinned block (see
+// pin_get); Utf8 key codes are narrowed to u32 on the host.
 int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key, int key_dtype,
             const std::string &key_name, int64_t *d_key, uint8_t *d_key_valid,
             int64_t **d_val, uint8_t **d_valid, unsigned long long *d_count, fgpu_result **out,
@@ -528,15 +602,44 @@ int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key,
     auto *r = new fgpu_result();
     r->rows = (int64_t)n;
     if (c->tune.keep_on_device) n = 0;  // columns stay in device scratch
+    const int n_cols = (key ? 1 : 0) + L.outs.n;
+    char *blk = nullptr;
+    if (n) {
+        blk = (char *)pin_get((size_t)n * 9 * n_cols, &r->block_n);
+        if (!blk) {
+            delete r;
+            return set_err(FGPU_E_OUT_OF_MEMORY, "pinned host memory for %llu result rows", n);
+        }
+        r->block = blk;
+    }
+    auto col_at = [&](int ci) { return blk + (size_t)ci * n * 9; };
+    hipError_t e = hipSuccess;
+    auto copy_in = [&](int ci, const void *d_v, const void *d_ok) {
+        if (!n || e != hipSuccess) return;
+        e = hipMemcpyAsync(col_at(ci), d_v, n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(col_at(ci) + n * 8, d_ok, n, hipMemcpyDeviceToHost, c->stream);
+    };
+    int ci = 0;
+    if (key) copy_in(ci++, d_key, d_key_valid);
+    for (int i = 0; i < L.outs.n; ++i) copy_in(ci++, d_val[i], d_valid[i]);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        delete r;
+        return set_err(FGPU_E_DEVICE, "result copy: %s", hipGetErrorString(e));
+    }
+    ci = 0;
     if (key) {
         RCol kc;
         kc.name = key_name;
         kc.dtype = key_dtype == FGPU_UTF8 ? FGPU_UTF8 : FGPU_INT64;
-        kc.i64.resize(n);
-        kc.valid.resize(n);
-        if (n) {
-            (void)hipMemcpyAsync(kc.i64.data(), d_key, n * 8, hipMemcpyDeviceToHost, c->stream);
-            (void)hipMemcpyAsync(kc.valid.data(), d_key_valid, n, hipMemcpyDeviceToHost, c->stream);
+        const char *v = col_at(ci++);
+        if (key_dtype == FGPU_UTF8) {  // Utf8 keys: codes as u32
+            kc.u32.resize(n);
+            for (size_t i = 0; i < n; ++i) kc.u32[i] = (uint32_t)((const int64_t *)v)[i];
+            kc.valid.assign((const uint8_t *)v + n * 8, (const uint8_t *)v + n * 9);
+        } else {
+            kc.xv = v;
+            kc.xvalid = (const uint8_t *)v + n * 8;
         }
         kc.dict_src = key_dtype == FGPU_UTF8 ? key : nullptr;
         r->cols.push_back(std::move(kc));
@@ -547,28 +650,10 @@ int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key,
         const int k = L.outs.o[i].kind;
         const bool is_f = k == OUT_AVG_I || k == OUT_SUM_F || k == OUT_AVG_F || k == OUT_MIN_F || k == OUT_MAX_F;
         rc.dtype = is_f ? FGPU_FLOAT64 : FGPU_INT64;
-        rc.valid.resize(n);
-        if (is_f) rc.f64.resize(n);
-        else rc.i64.resize(n);
-        if (n) {
-            void *dst = is_f ? (void *)rc.f64.data() : (void *)rc.i64.data();
-            (void)hipMemcpyAsync(dst, d_val[i], n * 8, hipMemcpyDeviceToHost, c->stream);
-            (void)hipMemcpyAsync(rc.valid.data(), d_valid[i], n, hipMemcpyDeviceToHost, c->stream);
-        }
+        const char *v = col_at(ci++);
+        rc.xv = v;
+        rc.xvalid = (const uint8_t *)v + n * 8;
         r->cols.push_back(std::move(rc));
-    }
-    const hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        delete r;
-        return set_err(FGPU_E_DEVICE, "result copy: %s", hipGetErrorString(e));
-    }
-    // Utf8 keys: codes as u32
-    if (key && key_dtype == FGPU_UTF8) {
-        RCol &kc = r->cols[0];
-        kc.u32.resize(n);
-        for (size_t i = 0; i < n; ++i) kc.u32[i] = (uint32_t)kc.i64[i];
-        kc.i64.clear();
-        kc.i64.shrink_to_fit();
     }
     *out = r;
     return FGPU_OK;
@@ -1778,10 +1863,15 @@ int fgpu_result_column(const fgpu_result *r, int i, fgpu_column_view *v) {
     const RCol &c = r->cols[i];
     v->name = c.name.c_str();
     v->dtype = c.dtype;
-    v->values = c.dtype == FGPU_FLOAT64 ? (const void *)c.f64.data()
-              : c.dtype == FGPU_UTF8   ? (const void *)c.u32.data()
-                                       : (const void *)c.i64.data();
-    v->valid = c.valid.empty() ? nullptr : c.valid.data();
+    if (c.xv) {
+        v->values = c.xv;
+        v->valid = c.xvalid;
+    } else {
+        v->values = c.dtype == FGPU_FLOAT64 ? (const void *)c.f64.data()
+                  : c.dtype == FGPU_UTF8   ? (const void *)c.u32.data()
+                                           : (const void *)c.i64.data();
+        v->valid = c.valid.empty() ? nullptr : c.valid.data();
+    }
     v->dict_len = c.dict_src ? (int32_t)c.dict_src->dictp.size() : 0;
     v->dict = c.dict_src ? c.dict_src->dictp.data() : nullptr;
     return FGPU_OK;

@@ -11,7 +11,9 @@ of the whole operator with the table already in HBM.  The 1M x 100M join
 (INNER / LEFT / SEMI / ANTI fused with SUM + COUNT) is reported beside it.
 
 The JSON line carries a roofline block for the dominant kernel (algorithmic
-bytes / its average duration from HIP events on the stream it runs on) and a
+bytes / its average duration from HIP events on the stream it runs on; traffic
+= its measured HBM bytes per launch, read from the rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary committed under profiles/ for this same command) and a
 CPU baseline: the oracle's restatement of the reference's rayon operators on
 the GPU box's host cores over a bounded sample.
 """
@@ -235,6 +237,24 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
             "parity_vs_alone": bool(same), "served": served, "plain_one_slot": plain}
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_c2_groupby_100M_and_c3_join.txt"
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, each scaled by
+    what the counter reads for a plain copy of known size), from the committed
+    rocprofv3 counter summary of this same bench command; None if absent."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            f = line.split()
+            if len(f) > 2 and f[0] == "traffic" and f[1] == kernel:
+                kv = dict(x.split("=", 1) for x in f[2:])
+                return int(kv["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def calibrate_copy(n_bytes=1 << 30, reps=3):
     """A plain device-to-device copy of known size (the HIP runtime's copy
     kernel), for scaling FETCH_SIZE / WRITE_SIZE counter readings: profiled
@@ -371,9 +391,9 @@ def main():
             km = float(np.mean(jms))
             secondary[f"join_{name}"] = {
                 "rows_per_s": (C3_PROBE + C3_BUILD) / jwall, "ms_per_step": jwall * 1e3,
-                "probe_kernel_ms": km, "achieved_GBs": jb / (km * 1e-3) / 1e9,
+                "kernels_ms": km, "achieved_GBs": jb / (km * 1e-3) / 1e9,
                 "frac": jb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS}
-            log(f"[bench] join {name}: {jwall * 1e3:.3f} ms/step, probe {km:.3f} ms")
+            log(f"[bench] join {name}: {jwall * 1e3:.3f} ms/step, partition + probe kernels {km:.3f} ms")
         # index build time
         t0 = time.perf_counter()
         idx2 = fa.JoinIndex(ctx, bt, "bk")
@@ -422,7 +442,8 @@ def main():
                    "aggregates": "COUNT(*), SUM(v1), MIN(v1), MAX(v1), SUM(v2)",
                    "groups_out": int(n_groups), "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel),
+                     "traffic_source": TRAFFIC_FILE.name,
                      "kernel": kernel, "kernel_ms": kern_ms, "algorithmic_bytes": scan_bytes,
                      "pass2_kernel": pass2, "pass2_ms": pass2_ms,
                      "query_achieved": query_gbs, "query_frac": query_gbs / HBM_PEAK_GBS,

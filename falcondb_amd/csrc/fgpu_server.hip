@@ -11,8 +11,10 @@
 // the budget.
 //
 // Invariant: the bytes all slots hold (an idle slot its arena, a running one
-// its reservation) never exceed the budget.  A running slot's arena is capped
-// at its reservation (the runtime refuses to grow past it), so a request that
+// its reservation) never exceed the budget.  A reservation is the larger of
+// the request's learned need and the slot's fair share of the budget.  A
+// running slot's arena is capped at its reservation (the runtime refuses to
+// grow past it), so a request that
 // needs more than its reservation fails inside the call and is re-queued with
 // a larger one; what a request of each shape needs is learned from its runs.
 #include <hip/hip_runtime.h>
@@ -295,8 +297,16 @@ void fgpu_server::worker(int s) {
             cv_work.wait(lk);
             continue;
         }
+        // Beyond its learned need, a slot may keep a fair share of the budget
+        // (budget / slots, less what the other slots hold beyond theirs): the
+        // request kinds use different buffers and their sizes vary with the
+        // predicate, and capping a slot at each request's own need would free
+        // and re-allocate those buffers on every call (each hipFree also
+        // waits for the other slots' streams).
         const int64_t need = batch[0]->need;
-        const int64_t mine = std::max(held[s], need);
+        const int64_t others = held_total() - held[s];
+        const int64_t share = std::min(budget / n_slots, budget - others);
+        const int64_t mine = std::max({held[s], need, share});
         held[s] = mine;
         busy[s] = 1;
         ++running;

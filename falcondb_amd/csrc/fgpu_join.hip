@@ -587,8 +587,8 @@ FGPU_DEV void emit_fast(const EmitOps &e, JoinAcc &j, int64_t f1, int64_t f2, bo
     }
 }
 
-// Records of the two side partitions: NULL keys (matching nothing) and keys
-// equal to the sentinel INT64_MIN (matching the build side list).
+// Records of the two side partitions: NULL keys (matching nothing) and the
+// sentinel key, whose hash is EMPTY_KEY (matching the build side list).
 template <int TYPE>
 FGPU_DEV void probe_side_partitions(const JoinPartArgs &a, JoinAcc &j) {
     const int P = 1 << a.log_p, P2 = P + 2;

@@ -4,6 +4,8 @@
 //
 // Traffic per input row: the scanned columns once; per cold row (a key the
 // LDS cache did not hold) one record written in pass 1 and read in pass 2.
+#include <type_traits>
+
 #include "fgpu_state.h"
 
 namespace fgpu {
@@ -254,6 +256,80 @@ FGPU_DEV void gbp_finish_data(const GbpScan<NPC, NIN> &g, int64_t base, const Gb
     }
 }
 
+// ─── specialised tile loads ──────────────────────────────────────────────────
+// The common shape -- one comparison on a column without NULLs, an INT64 key
+// and 64-bit inputs, none with NULLs -- needs none of the generic cell
+// machinery: each cell is one load from a wave-uniform base plus a 32-bit
+// lane offset (the scalar-base addressing form), with no per-dtype finish.
+// FP = the predicate column's width in bytes (4: INT32, 8: INT64 / FLOAT64).
+template <int U, int FP>
+struct GbpPredFast {
+    uint64_t x[U];  // INT32 values sign-extended, 64-bit values as bits
+};
+
+template <int U, int NIN>
+struct GbpDataFast {
+    uint64_t key[U];
+    uint64_t in[U][NIN > 0 ? NIN : 1];
+    uint32_t pass;
+};
+
+template <class T>
+FGPU_DEV T gbp_ld(const uint8_t *base, uint32_t off) {
+    return *(const T *)(base + off);
+}
+
+template <int U, int FP, int NPC, int NIN>
+FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1, GbpPredFast<U, FP> &t) {
+    const uint8_t *b = g.pc[0].base + base * FP;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t off = gbp_pred_idx(base, u, r1) * (uint32_t)FP;
+        t.x[u] = FP == 4 ? (uint64_t)(int64_t)gbp_ld<int32_t>(b, off) : gbp_ld<uint64_t>(b, off);
+    }
+}
+
+template <int U, int FP, int NPC, int NIN>
+FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &, const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1,
+                                  const GbpPredFast<U, FP> &t) {
+    int64_t v[U][1];
+    uint32_t ok[U];
+    uint32_t exists = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        exists |= base + u * 64 + (threadIdx.x & 63) < r1 ? 1u << u : 0u;
+        v[u][0] = (int64_t)t.x[u];
+        ok[u] = 1u;
+    }
+    return leaf_mask<U, 1>(g.leaf0, v, ok) & exists;
+}
+
+template <int U, int NPC, int NIN>
+FGPU_DEV void gbp_issue_data(const GbpScan<NPC, NIN> &g, int64_t base, uint32_t pass, GbpDataFast<U, NIN> &t) {
+    t.pass = pass;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t off = ((pass >> u) & 1u) ? (uint32_t)(u * 64 + (threadIdx.x & 63)) * 8u : 0u;
+        t.key[u] = gbp_ld<uint64_t>(g.key.base + base * 8, off);
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) t.in[u][j] = gbp_ld<uint64_t>(g.in[j].base + base * 8, off);
+    }
+}
+
+// The validity word is rebuilt after compaction from the key (FP != 0).
+template <int U, int NPC, int NIN>
+FGPU_DEV void gbp_finish_data(const GbpScan<NPC, NIN> &, int64_t, const GbpDataFast<U, NIN> &t,
+                              GbpDataTile<U, NIN> &d) {
+    d.pass = t.pass;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        d.r[u].key = (int64_t)t.key[u];
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) d.r[u].in[j] = (int64_t)t.in[u][j];
+        d.r[u].vw = 0;
+    }
+}
+
 template <int NIN>
 FGPU_DEV Inputs gbp_inputs(const GbpRow<NIN> &r) {
     Inputs in{0, 0, 0, 0, 0, 0, r.vw & 0x3fu};
@@ -332,7 +408,7 @@ FGPU_DEV void gbp_row(const GbpScan<NPC, NIN> &g, const GbpLds &L, bool active, 
     if (g.has_valid_word) rec[1 + NIN] = (int64_t)(row.vw & 0x3fu);
 }
 
-template <int U, int NPC, int NIN>
+template <int U, int NPC, int NIN, int FP>
 __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     extern __shared__ uint64_t lds[];
     const GbArgs &a = A.g;
@@ -386,14 +462,14 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     // Pipeline: while tile t is aggregated, the data loads of tile t + nw and
     // the predicate loads of tile t + 2 nw are in flight.
     int t = wave;
-    GbpPredRaw<U, NPC> pr;
-    GbpDataRaw<U, NIN> draw;
+    std::conditional_t<FP != 0, GbpPredFast<U, FP>, GbpPredRaw<U, NPC>> pr;
+    std::conditional_t<FP != 0, GbpDataFast<U, NIN>, GbpDataRaw<U, NIN>> draw;
     draw.pass = 0;
     if (t < n_tiles) {
-        gbp_issue_pred<U, NPC, NIN>(g, r0 + (int64_t)t * TR, r1, pr);
-        const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + (int64_t)t * TR, r1, pr);
-        gbp_issue_data<U, NPC, NIN>(g, r0 + (int64_t)t * TR, m, draw);
-        if (t + nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (int64_t)(t + nw) * TR, r1, pr);
+        gbp_issue_pred(g, r0 + (int64_t)t * TR, r1, pr);
+        const uint32_t m = gbp_finish_pred(a, g, r0 + (int64_t)t * TR, r1, pr);
+        gbp_issue_data(g, r0 + (int64_t)t * TR, m, draw);
+        if (t + nw < n_tiles) gbp_issue_pred(g, r0 + (int64_t)(t + nw) * TR, r1, pr);
     }
     // One pass per tile, then one more (have == false) that aggregates the last,
     // partial chunk; gbp_row has a single call site to keep the loop compact.
@@ -402,11 +478,11 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
         GbpDataTile<U, NIN> dt;
         dt.pass = 0;
         if (have) {
-            gbp_finish_data<U, NPC, NIN>(g, r0 + (int64_t)t * TR, draw, dt);
+            gbp_finish_data(g, r0 + (int64_t)t * TR, draw, dt);
             if (t + nw < n_tiles) {
-                const uint32_t m = gbp_finish_pred<U, NPC, NIN>(a, g, r0 + (int64_t)(t + nw) * TR, r1, pr);
-                gbp_issue_data<U, NPC, NIN>(g, r0 + (int64_t)(t + nw) * TR, m, draw);
-                if (t + 2 * nw < n_tiles) gbp_issue_pred<U, NPC, NIN>(g, r0 + (int64_t)(t + 2 * nw) * TR, r1, pr);
+                const uint32_t m = gbp_finish_pred(a, g, r0 + (int64_t)(t + nw) * TR, r1, pr);
+                gbp_issue_data(g, r0 + (int64_t)(t + nw) * TR, m, draw);
+                if (t + 2 * nw < n_tiles) gbp_issue_pred(g, r0 + (int64_t)(t + 2 * nw) * TR, r1, pr);
             }
         }
         // compact tile t's passing rows into chunks and aggregate full ones
@@ -434,7 +510,12 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
                     cur.in[j] = to_cur ? x : cur.in[j];
                     nxt.in[j] = to_nxt ? x : nxt.in[j];
                 }
-                const uint32_t vw = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)src.vw);
+                uint32_t vw;
+                if constexpr (FP != 0) {
+                    vw = (k != EMPTY_KEY ? GBP_HAS_KEY : 0u) | ((1u << NIN) - 1u);
+                } else {
+                    vw = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)src.vw);
+                }
                 cur.vw = to_cur ? vw : cur.vw;
                 nxt.vw = to_nxt ? vw : nxt.vw;
                 fill = __builtin_amdgcn_readfirstlane(fill + pc);
@@ -646,34 +727,59 @@ __global__ void k_gbp_null(GbPartArgs A) {
     }
 }
 
-template <int NPC, int NIN>
+template <int NPC, int NIN, int FP>
 static hipError_t gbp_scan_go(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
     constexpr int U = 2;  // 128-row tiles: two in flight per wave within 128 VGPRs
-    const void *fn = (const void *)k_gbp_scan<U, NPC, NIN>;
+    const void *fn = (const void *)k_gbp_scan<U, NPC, NIN, FP>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_gbp_scan<U, NPC, NIN>), dim3(a.G), dim3(block), lds, s, a);
+    hipLaunchKernelGGL((k_gbp_scan<U, NPC, NIN, FP>), dim3(a.G), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 
-template <int NPC>
+template <int NPC, int FP>
 static hipError_t gbp_scan_nin(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
     switch (a.g.n_in) {
-        case 0: return gbp_scan_go<NPC, 0>(s, a, block, lds);
-        case 1: return gbp_scan_go<NPC, 1>(s, a, block, lds);
-        case 2: return gbp_scan_go<NPC, 2>(s, a, block, lds);
-        case 3: return gbp_scan_go<NPC, 3>(s, a, block, lds);
-        case 4: return gbp_scan_go<NPC, 4>(s, a, block, lds);
-        case 5: return gbp_scan_go<NPC, 5>(s, a, block, lds);
-        default: return gbp_scan_go<NPC, 6>(s, a, block, lds);
+        case 0: return gbp_scan_go<NPC, 0, FP>(s, a, block, lds);
+        case 1: return gbp_scan_go<NPC, 1, FP>(s, a, block, lds);
+        case 2: return gbp_scan_go<NPC, 2, FP>(s, a, block, lds);
+        case 3: return gbp_scan_go<NPC, 3, FP>(s, a, block, lds);
+        case 4: return gbp_scan_go<NPC, 4, FP>(s, a, block, lds);
+        case 5: return gbp_scan_go<NPC, 5, FP>(s, a, block, lds);
+        default: return gbp_scan_go<NPC, 6, FP>(s, a, block, lds);
     }
+}
+
+// Width in bytes of the predicate column when the scan can take the
+// specialised tile loads (see GbpPredFast), else 0.
+int gbp_fast_width(const GbPartArgs &a) {
+    const GbArgs &g = a.g;
+    if (a.has_valid_word || g.pred.n != 1 || g.pred.n_pcols != 1) return 0;
+    const DPredNode &nd = g.pred.nodes[0];
+    if (nd.kind != FGPU_P_CMP || nd.pc != 0) return 0;
+    if (nd.cls != CMP_F64 && nd.cls != CMP_I64 && nd.cls != CMP_I64_AS_F64 && nd.cls != CMP_I32) return 0;
+    const int pcol = g.pred.pcol[0];
+    if (g.cols.valid[pcol] || g.cols.valid[g.key_col] || g.cols.dtype[g.key_col] != FGPU_INT64) return 0;
+    for (int j = 0; j < g.n_in; ++j) {
+        const int c = g.in_col[j];
+        if (g.cols.valid[c] || (g.cols.dtype[c] != FGPU_INT64 && g.cols.dtype[c] != FGPU_FLOAT64)) return 0;
+    }
+    const int dt = g.cols.dtype[pcol];
+    if (dt == FGPU_INT32 && (nd.cls == CMP_I32 || nd.cls == CMP_I64_AS_F64)) return 4;
+    if ((dt == FGPU_INT64 || dt == FGPU_FLOAT64) && nd.cls != CMP_I32) return 8;
+    return 0;
 }
 
 hipError_t launch_gbp_scan(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
     if (a.H % GBP_WAYS) return hipErrorInvalidValue;
-    if (a.g.pred.n_pcols <= 1) return gbp_scan_nin<1>(s, a, block, lds);
-    if (a.g.pred.n_pcols <= 2) return gbp_scan_nin<2>(s, a, block, lds);
-    return gbp_scan_nin<MAX_PCOLS>(s, a, block, lds);
+    switch (gbp_fast_width(a)) {
+        case 4: return gbp_scan_nin<1, 4>(s, a, block, lds);
+        case 8: return gbp_scan_nin<1, 8>(s, a, block, lds);
+        default: break;
+    }
+    if (a.g.pred.n_pcols <= 1) return gbp_scan_nin<1, 0>(s, a, block, lds);
+    if (a.g.pred.n_pcols <= 2) return gbp_scan_nin<2, 0>(s, a, block, lds);
+    return gbp_scan_nin<MAX_PCOLS, 0>(s, a, block, lds);
 }
 
 hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {

@@ -106,6 +106,8 @@ struct GbPartArgs {
     unsigned long long *out_count;
 };
 hipError_t launch_gbp_scan(hipStream_t s, const GbPartArgs &a, int block, size_t lds);
+// Predicate width (4 / 8) when the scan takes its specialised tile loads, else 0.
+int gbp_fast_width(const GbPartArgs &a);
 hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t lds);
 
 hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint64_t *state,

@@ -1161,7 +1161,9 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     }
     A.out_count = ob.count;
     char msg[256];
-    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d RW=%d", (long long)P, H, T2, A.G, A.RW);
+    const int fw = gbp_fast_width(A);
+    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d RW=%d tiles=%s", (long long)P, H, T2, A.G,
+             A.RW, fw == 4 ? "fast4" : fw == 8 ? "fast8" : "generic");
     c->plan = msg;
     unsigned long long h[3] = {0, 0, 0};
     mark(c, 0);

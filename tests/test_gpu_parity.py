@@ -177,6 +177,33 @@ def test_groupby_high_cardinality_zipf(gctx, n, nk):
     assert g.to_reference_types().dtypes == ["Int64", "Int64", "Float64", "Float64", "Float64", "Float64"]
 
 
+@pytest.mark.parametrize("pred,tiles", [
+    (col("f") < 40, "fast4"), (col("f") != 7, "fast4"), (col("f") >= -3, "fast4"),
+    (col("v2") < 0.5, "fast8"), (col("v2") < 1, "fast8"), (col("v1") >= 0, "fast8"),
+    (col("v1") < 0.5, "fast8"), (col("key") > 2**39, "fast8"),
+    (col("f") < 40.5, "generic"),  # int32 column vs float literal: no row passes
+    ((col("f") < 40) & (col("v1") > 0), "generic"),
+])
+def test_groupby_partitioned_tile_paths(ctx_gbp, pred, tiles):
+    # One comparison on a column without NULLs, an INT64 key and 64-bit inputs
+    # take the scan's specialised tile loads; INT64_MIN keys still fold into
+    # the NULL group there.
+    K = dg.keys(40_000, 40)
+    cols = dg.c2_table(300_000, K)
+    cols["key"][::101] = np.iinfo(np.int64).min
+    t = _table(ctx_gbp, cols)
+    g = fa.GroupByExec(["key"], C2_AGGS).execute(ctx_gbp, t, pred)
+    plan = ctx_gbp.last_plan()
+    assert plan.startswith("groupby:partitioned") and f"tiles={tiles}" in plan, plan
+    o = ora.groupby(cols, pred, "key", C2_AGGS)
+    imin = np.iinfo(np.int64).min
+    for r in (g, o.exact, o.reference):
+        if r.validity[0] is not None:
+            r.columns[0] = np.where(r.validity[0], r.columns[0], imin)
+            r.validity[0] = None
+    assert_batches_equal(g, o.exact, "key", o.reference)
+
+
 @pytest.mark.parametrize("card", [8, 1000, 70_000, 1_000_000])
 def test_groupby_int32_dense_keys(ctx, card):
     cols = dg.c4_batch(400_000, card, 1234 + card)

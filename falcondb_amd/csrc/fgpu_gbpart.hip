@@ -558,7 +558,8 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
 // ─── pass 2 ─────────────────────────────────────────────────────────────────
 // One workgroup per partition.  The cell counts of the partition are scanned
 // into LDS prefix arrays first, so every record has a flat index and all
-// record loads are independent (U per thread in flight); the records then
+// record loads are independent: each thread keeps two batches of U records in
+// flight (the next batch loads while the current one is folded).  The records
 // aggregate into an LDS table that is emitted at the end.
 constexpr int GBP_AGG_UNROLL = 4;
 
@@ -573,6 +574,34 @@ FGPU_DEV int gbp_cell_of(const unsigned *pre, int G, unsigned i) {
     return lo;
 }
 
+// Records i0, i0 + blockDim.x, ... (U of them) of a partition: the cell of
+// each flat index, then its loads; have[u] = the index exists.
+template <int NIN, int U>
+FGPU_DEV void gbp_agg_fetch(const GbPartArgs &A, const unsigned *pre, int G, const int64_t *part_cells,
+                            int64_t cell_stride, unsigned n_rows, unsigned i0, int64_t (&key)[U], Inputs (&in)[U],
+                            bool (&have)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const unsigned i = i0 + u * blockDim.x;
+        have[u] = i < n_rows;
+        key[u] = 0;
+        in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+        if (have[u]) {
+            const int g = gbp_cell_of(pre, G, i);
+            const int64_t *rec = part_cells + g * cell_stride + (int64_t)(i - pre[g]) * A.RW;
+            key[u] = rec[0];
+            if (NIN > 0) in[u].v0 = rec[1];
+            if (NIN > 1) in[u].v1 = rec[2];
+            if (NIN > 2) in[u].v2 = rec[3];
+            if (NIN > 3) in[u].v3 = rec[4];
+            if (NIN > 4) in[u].v4 = rec[5];
+            if (NIN > 5) in[u].v5 = rec[6];
+            in[u].valid = A.has_valid_word ? (uint32_t)rec[1 + NIN] : 0xffffffffu;
+        }
+    }
+}
+
+template <int NIN>
 __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     extern __shared__ uint64_t lds[];
     const GbArgs &a = A.g;
@@ -630,34 +659,21 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     }
     __syncthreads();
     const unsigned n_rows = pre[G], n_states = spre[G];
-    const int ni = a.n_in;
     const int64_t *part_cells = A.cells + (int64_t)p * A.Cw * A.RW;  // cell (g, p) = part_cells + g * P * Cw * RW
     const int64_t cell_stride = (int64_t)P * A.Cw * A.RW;
-    for (unsigned i0 = threadIdx.x; i0 < n_rows; i0 += blockDim.x * GBP_AGG_UNROLL) {
-        int64_t key[GBP_AGG_UNROLL];
-        Inputs in[GBP_AGG_UNROLL];
-        bool have[GBP_AGG_UNROLL];
+    {
+    // Two batches in flight: the cell searches and record loads of batch
+    // i0 + step are issued before batch i0 is folded into the table.
+    constexpr int U = NIN >= 5 ? 2 : GBP_AGG_UNROLL;  // two batches of U records within 128 VGPRs
+    const unsigned step = blockDim.x * U;
+    int64_t key[U], nkey[U];
+    Inputs in[U], nin[U];
+    bool have[U], nhave[U];
+    gbp_agg_fetch<NIN, U>(A, pre, G, part_cells, cell_stride, n_rows, threadIdx.x, key, in, have);
+    for (unsigned i0 = threadIdx.x; i0 < n_rows; i0 += step) {
+        gbp_agg_fetch<NIN, U>(A, pre, G, part_cells, cell_stride, n_rows, i0 + step, nkey, nin, nhave);
 #pragma unroll
-        for (int u = 0; u < GBP_AGG_UNROLL; ++u) {
-            const unsigned i = i0 + u * blockDim.x;
-            have[u] = i < n_rows;
-            key[u] = 0;
-            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
-            if (have[u]) {
-                const int g = gbp_cell_of(pre, G, i);
-                const int64_t *rec = part_cells + g * cell_stride + (int64_t)(i - pre[g]) * A.RW;
-                key[u] = rec[0];
-                if (ni > 0) in[u].v0 = rec[1];
-                if (ni > 1) in[u].v1 = rec[2];
-                if (ni > 2) in[u].v2 = rec[3];
-                if (ni > 3) in[u].v3 = rec[4];
-                if (ni > 4) in[u].v4 = rec[5];
-                if (ni > 5) in[u].v5 = rec[6];
-                in[u].valid = A.has_valid_word ? (uint32_t)rec[1 + ni] : 0xffffffffu;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < GBP_AGG_UNROLL; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (!have[u]) continue;
             const int li = gbp_slot(tkeys, T2, T2, key[u], hash64((uint64_t)key[u]));
             if (li < 0) {
@@ -666,6 +682,13 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
             }
             update_row<LDS_SCOPE>(tstate + (int64_t)li * S, a.ops, in[u]);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            key[u] = nkey[u];
+            in[u] = nin[u];
+            have[u] = nhave[u];
+        }
+    }
     }
     for (unsigned i = threadIdx.x; i < n_states; i += blockDim.x) {
         const int g = gbp_cell_of(spre, G, i);
@@ -782,12 +805,27 @@ hipError_t launch_gbp_scan(hipStream_t s, const GbPartArgs &a, int block, size_t
     return gbp_scan_nin<MAX_PCOLS, 0>(s, a, block, lds);
 }
 
+template <int NIN>
+static hipError_t gbp_agg_go(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_gbp_agg<NIN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gbp_agg<NIN>), dim3(1 << a.log_p), dim3(block), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t lds) {
     if (a.T2 > 32 * block) return hipErrorInvalidValue;  // the emit mask holds 32 slots per thread
-    hipError_t e = hipFuncSetAttribute((const void *)k_gbp_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e;
+    switch (a.g.n_in) {
+        case 0: e = gbp_agg_go<0>(s, a, block, lds); break;
+        case 1: e = gbp_agg_go<1>(s, a, block, lds); break;
+        case 2: e = gbp_agg_go<2>(s, a, block, lds); break;
+        case 3: e = gbp_agg_go<3>(s, a, block, lds); break;
+        case 4: e = gbp_agg_go<4>(s, a, block, lds); break;
+        case 5: e = gbp_agg_go<5>(s, a, block, lds); break;
+        default: e = gbp_agg_go<6>(s, a, block, lds); break;
+    }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gbp_agg, dim3(1 << a.log_p), dim3(block), lds, s, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_gbp_null, dim3(1), dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -5,10 +5,10 @@
 // :452-454), inverted for DESC, ties by row.  NULL rows are selected by a
 // second scan (by row) and placed by the host.
 //
-// Each thread keeps its K best (key, row) pairs sorted in registers behind a
-// threshold test, so almost every row costs one comparison; a workgroup then
-// takes K rounds of a block-wide argmin over the threads' heads, and one final
-// workgroup repeats that over every workgroup's K candidates.
+// Each wave keeps its K best (key, row) pairs sorted across its lanes behind a
+// threshold test, so almost every row costs one comparison; a workgroup's
+// waves then merge into one list, and one final workgroup merges every
+// workgroup's K candidates the same way.
 #include "fgpu_state.h"
 
 namespace fgpu {
@@ -22,71 +22,62 @@ FGPU_DEV bool top_less(const TopPair &a, const TopPair &b) {
     return a.key < b.key || (a.key == b.key && a.row < b.row);
 }
 
-template <int K>
-struct TopList {
-    TopPair a[K];  // ascending; unused entries are (~0, INT64_MAX)
-    FGPU_DEV void init() {
-#pragma unroll
-        for (int j = 0; j < K; ++j) a[j] = TopPair{~0ull, INT64_MAX};
-    }
-    FGPU_DEV void offer(const TopPair &c) {
-        if (!top_less(c, a[K - 1])) return;
-        a[K - 1] = c;
-#pragma unroll
-        for (int j = K - 1; j > 0; --j) {
-            if (top_less(a[j], a[j - 1])) {
-                const TopPair t = a[j];
-                a[j] = a[j - 1];
-                a[j - 1] = t;
-            }
-        }
-    }
-    FGPU_DEV void pop() {
-#pragma unroll
-        for (int j = 0; j < K - 1; ++j) a[j] = a[j + 1];
-        a[K - 1] = TopPair{~0ull, INT64_MAX};
-    }
-};
+// ─── one wave's best K ───────────────────────────────────────────────────────
+// The list is spread over the lanes, sorted: lane i holds the i-th best pair
+// (lanes past the filled part hold the sentinel (~0, INT64_MAX)).  Lane K - 1
+// holds the threshold a row must beat.  Rows are tested against it, and the
+// few that beat it are inserted one at a time: the position is the number of
+// lanes holding better pairs, and the lanes from there on shift up by one.
+FGPU_DEV TopPair top_sentinel() { return TopPair{~0ull, INT64_MAX}; }
 
-// K rounds of a block-wide argmin over the threads' list heads; out[r] = r-th.
+// Lane l's pair, in every lane (l is wave-uniform).
+FGPU_DEV TopPair lane_pair(const TopPair &p, int l) {
+    const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)p.key, l);
+    const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(p.key >> 32), l);
+    const uint32_t rlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)p.row, l);
+    const uint32_t rhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)p.row >> 32), l);
+    return TopPair{(uint64_t)khi << 32 | klo, (int64_t)((uint64_t)rhi << 32 | rlo)};
+}
+
+// Inserts the wave-uniform pair c into the lane list `my`.
+FGPU_DEV void wave_insert(TopPair &my, const TopPair &c) {
+    const int pos = __popcll(__ballot(top_less(my, c)));
+    const int lane = threadIdx.x & 63;
+    TopPair prev;
+    prev.key = __shfl_up(my.key, 1, 64);
+    prev.row = __shfl_up(my.row, 1, 64);
+    if (lane == pos) my = c;
+    else if (lane > pos) my = prev;
+}
+
+// Offers each active lane's pair c to the wave's list; thr tracks lane K - 1.
+// Every lane of the wave must call it (the ballots span the wave).
 template <int K>
-FGPU_DEV void top_block_merge(TopList<K> &l, TopPair *out) {
-    __shared__ TopPair wbest[16];
-    __shared__ int wtid[16];
-    __shared__ int winner;
+FGPU_DEV void wave_offer(TopPair &my, TopPair &thr, const TopPair &c, bool active) {
+    uint64_t m = __ballot(active && top_less(c, thr));
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        wave_insert(my, lane_pair(c, l));
+        thr = lane_pair(my, K - 1);
+        m &= ~(1ull << l);
+        m &= __ballot(active && top_less(c, thr));
+    }
+}
+
+// The block's best K into out[0..K): every wave's list meets in LDS and
+// wave 0 takes the others' pairs into its own.  Every thread must call it.
+template <int K>
+FGPU_DEV void block_best(TopPair &my, TopPair &thr, TopPair *out) {
+    __shared__ TopPair lists[16][K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int r = 0; r < K; ++r) {
-        TopPair b = l.a[0];
-        int tid = threadIdx.x;
-        for (int d = 32; d > 0; d >>= 1) {
-            TopPair o;
-            o.key = __shfl_xor(b.key, d, 64);
-            o.row = __shfl_xor(b.row, d, 64);
-            const int ot = __shfl_xor(tid, d, 64);
-            if (top_less(o, b) || (!top_less(b, o) && ot < tid)) {
-                b = o;
-                tid = ot;
-            }
+    if (lane < K) lists[wave][lane] = my;
+    __syncthreads();
+    if (wave == 0) {
+        for (int w = 1; w < nw; ++w) {
+            const TopPair c = lane < K ? lists[w][lane] : top_sentinel();
+            wave_offer<K>(my, thr, c, lane < K);
         }
-        if (lane == 0) {
-            wbest[wave] = b;
-            wtid[wave] = tid;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            TopPair m = wbest[0];
-            int mt = wtid[0];
-            for (int w = 1; w < nw; ++w)
-                if (top_less(wbest[w], m)) {
-                    m = wbest[w];
-                    mt = wtid[w];
-                }
-            out[r] = m;
-            winner = mt;
-        }
-        __syncthreads();
-        if (threadIdx.x == winner) l.pop();
-        __syncthreads();
+        if (lane < K) out[lane] = my;
     }
 }
 
@@ -103,40 +94,51 @@ FGPU_DEV uint64_t top_key(const DCols &c, int col, int64_t r, int desc) {
     return desc ? ~k : k;
 }
 
-constexpr int TOPK_UNROLL = 4;
+// Rows per thread in flight: 8, or 4 when the predicate reads several columns
+// (their values for every row of the tile are held at once).
+template <int NPC>
+constexpr int topk_unroll() { return NPC > 1 ? 4 : 8; }
 
 // null_mode 0: passing non-NULL rows by key; 1: passing NULL rows by row.
+// A wave walks tiles of U * blockDim.x rows (its lanes' rows are
+// u * blockDim.x + wave * 64 + lane), so the loop is uniform per wave.
 template <int K, int NPC>
 __global__ __launch_bounds__(256) void k_topk_scan(DCols cols, DPred pred, int col, int desc, int null_mode,
                                                    int64_t n, TopPair *cands) {
-    TopList<K> l;
-    l.init();
+    constexpr int TOPK_UNROLL = topk_unroll<NPC>();
+    TopPair my = top_sentinel(), thr = top_sentinel();
+    const int lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x * TOPK_UNROLL;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x * TOPK_UNROLL + threadIdx.x; base < n; base += stride) {
+    for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x * TOPK_UNROLL + (threadIdx.x & ~63); w0 < n; w0 += stride) {
         int64_t rows[TOPK_UNROLL];
-        bool pass[TOPK_UNROLL];
 #pragma unroll
-        for (int u = 0; u < TOPK_UNROLL; ++u) rows[u] = base + (int64_t)u * blockDim.x;
+        for (int u = 0; u < TOPK_UNROLL; ++u) rows[u] = w0 + (int64_t)u * blockDim.x + lane;
+        bool pass[TOPK_UNROLL];
         pred_tile<TOPK_UNROLL, NPC>(pred, cols, rows, n, pass);
+        uint64_t key[TOPK_UNROLL];
+        bool take[TOPK_UNROLL];
 #pragma unroll
         for (int u = 0; u < TOPK_UNROLL; ++u) {
-            if (!pass[u]) continue;
-            const bool ok = load_valid(cols, col, rows[u]);
-            if (null_mode ? ok : !ok) continue;
-            l.offer(TopPair{null_mode ? 0ull : top_key(cols, col, rows[u], desc), rows[u]});
+            const int64_t r = rows[u] < n ? rows[u] : n - 1;
+            key[u] = null_mode ? 0ull : top_key(cols, col, r, desc);
+            take[u] = pass[u] && (load_valid(cols, col, r) != (null_mode != 0));
         }
+#pragma unroll
+        for (int u = 0; u < TOPK_UNROLL; ++u) wave_offer<K>(my, thr, TopPair{key[u], rows[u]}, take[u]);
     }
-    top_block_merge<K>(l, cands + (int64_t)blockIdx.x * K);
+    block_best<K>(my, thr, cands + (int64_t)blockIdx.x * K);
 }
 
 template <int K>
 __global__ __launch_bounds__(256) void k_topk_final(const TopPair *cands, int64_t n_cands, TopPair *out) {
-    TopList<K> l;
-    l.init();
-    for (int64_t i = threadIdx.x; i < n_cands; i += blockDim.x) l.offer(cands[i]);
-    __shared__ TopPair best[K];
-    top_block_merge<K>(l, best);
-    if (threadIdx.x < K) out[threadIdx.x] = best[threadIdx.x];
+    TopPair my = top_sentinel(), thr = top_sentinel();
+    const int lane = threadIdx.x & 63;
+    for (int64_t i0 = threadIdx.x & ~63; i0 < n_cands; i0 += blockDim.x) {
+        const int64_t i = i0 + lane;
+        const bool a = i < n_cands;
+        wave_offer<K>(my, thr, a ? cands[i] : top_sentinel(), a);
+    }
+    block_best<K>(my, thr, out);
 }
 
 template <int K>

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -95,6 +96,19 @@ struct fgpu_table {
     int64_t bytes = 0;
 };
 
+// Radix-partitioned sub-tables of the build side for one (fan-out, table
+// size, build column held in the slots), built by the first partitioned join
+// that needs them and kept by the index.  sub == nullptr records that these
+// keys cannot be partitioned so (a sub-table would overflow).
+struct JoinSubTables {
+    int log_p = 0, T = 0;
+    const void *val_col = nullptr;  // build column whose values the slots hold; nullptr: build rows
+    int64_t *sub = nullptr;         // P * T * 2 words: {hash, value or build row}
+    uint32_t *min_key_rows = nullptr;
+    int64_t n_min_key_rows = 0;
+    int unique_keys = 0;
+};
+
 struct fgpu_join_index {
     int device = 0;
     const fgpu_table *build = nullptr;
@@ -104,7 +118,9 @@ struct fgpu_join_index {
     uint32_t *min_key_rows = nullptr;
     int64_t n_min_key_rows = 0;
     int unique_keys = 0;
-    int64_t bytes = 0;
+    mutable int64_t bytes = 0;
+    mutable std::mutex sub_mu;                 // guards subs (joins on many contexts share an index)
+    mutable std::list<JoinSubTables> subs;     // a list: entries stay put while others are added
 };
 
 // ─── results ────────────────────────────────────────────────────────────────
@@ -1400,6 +1416,10 @@ void fgpu_join_index_free(fgpu_join_index *idx) {
     (void)hipSetDevice(idx->device);
     if (idx->slots) (void)hipFree(idx->slots);
     if (idx->min_key_rows) (void)hipFree(idx->min_key_rows);
+    for (auto &st : idx->subs) {
+        if (st.sub) (void)hipFree(st.sub);
+        if (st.min_key_rows) (void)hipFree(st.min_key_rows);
+    }
     delete idx;
 }
 
@@ -1526,8 +1546,40 @@ static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *p
 // Radix-partitioned probe (see JoinPartArgs).  Returns FGPU_OK with the state
 // row filled, a negative value when the inputs do not suit it (the caller then
 // runs the direct probe), or an error code.
+// Builds the P sub-tables into jp.sub (and the sentinel key's build rows into
+// jp.min_key_rows) on the context's stream; fits = false when a sub-table
+// would be too full (skewed keys) or an insert found no slot.  Uses jp.flags
+// and jp.sub_cnt as scratch; st receives the side-list length and whether the
+// build keys are unique.  Returns with the stream synchronised.
+static int join_build_sub_tables(fgpu_ctx *c, JoinPartArgs &jp, const JoinBuildArgs &ba, JoinSubTables &st,
+                                 bool &fits) {
+    fits = false;
+    const int64_t P = int64_t(1) << jp.log_p;
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.flags, 8, 0));
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.sub, P * jp.T * 2, (uint64_t)EMPTY_KEY));
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.sub_cnt, (P + 1) / 2, 0));
+    if (ba.n) {
+        std::vector<unsigned int> counts((size_t)P);
+        HIPCHK(launch_join_part_count(c->stream, jp, ba, grid_for(c, ba.n, BLOCK, 8)));
+        HIPCHK(hipMemcpyAsync(counts.data(), jp.sub_cnt, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const unsigned mx = *std::max_element(counts.begin(), counts.end());
+        if (mx > (unsigned)(jp.T - jp.T / 4)) return FGPU_OK;  // skewed keys
+        HIPCHK(launch_join_part_build(c->stream, jp, ba, grid_for(c, ba.n, BLOCK, 16)));
+    }
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, jp.flags, 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[1]) return FGPU_OK;  // an insert found no slot
+    st.n_min_key_rows = (int64_t)h[2];
+    st.unique_keys = !h[3] && h[2] <= 1;
+    fits = true;
+    return FGPU_OK;
+}
+
 static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu_table *build,
-                                const fgpu_join_spec *spec, JoinProbeArgs &a, const AggLayout &L) {
+                                const fgpu_join_index *idx, const fgpu_join_spec *spec, JoinProbeArgs &a,
+                                const AggLayout &L) {
     const Tuning &tu = c->tune;
     if (!tu.join_partition || probe->n_rows < tu.join_part_min_rows) return -1;
     // probe payload columns (non-NULL, at most 2) and build columns the aggregates read
@@ -1601,49 +1653,78 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     jp.cells = (int64_t *)p;
     if ((rc = buf(c, B_CELLCNT, (size_t)jp.G * P2 * 4, &p))) return rc;
     jp.cell_cnt = (unsigned int *)p;
-    if ((rc = buf(c, B_SUB, (size_t)P * jp.T * 16, &p))) return rc;
-    jp.sub = (int64_t *)p;
-    if ((rc = buf(c, B_SUBCNT, (size_t)P * 4 + 64, &p))) return rc;
-    jp.sub_cnt = (unsigned int *)p;
     unsigned long long *d_flags;
     if ((rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_flags))) return rc;
     jp.flags = d_flags;
+    if ((rc = buf(c, B_SUBCNT, (size_t)P * 4 + 64, &p))) return rc;
+    jp.sub_cnt = (unsigned int *)p;
     // the partitioned path keys on hash64(key): the one key whose hash is the
     // empty-slot marker keeps its build rows in a side list
     const int64_t sentinel = (int64_t)unhash64((uint64_t)EMPTY_KEY);
     const bool has_min = bkc.imin <= sentinel && sentinel <= bkc.imax;
     jp.min_key_cap = has_min ? std::max<int64_t>(nb, 1) : 0;
-    jp.min_key_rows = nullptr;
-    if (has_min) {
-        if ((rc = buf(c, B_MINKEY, (size_t)jp.min_key_cap * 4, &p))) return rc;
-        jp.min_key_rows = (uint32_t *)p;
-    }
     JoinBuildArgs ba{};
     ba.keys_col = bkc.d_values;
     ba.key_dtype = bkc.dtype;
     ba.key_valid = bkc.d_valid;
     ba.n = nb;
+    const void *val_col = jp.build_pay_col >= 0 ? a.bcols.v[jp.build_pay_col] : nullptr;
     mark(c, 4);
-    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_flags, 8, 0));
-    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.sub, P * jp.T * 2, (uint64_t)EMPTY_KEY));
-    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)jp.sub_cnt, (P + 1) / 2, 0));
-    std::vector<unsigned int> counts((size_t)P);
-    if (nb) {
-        HIPCHK(launch_join_part_count(c->stream, jp, ba, grid_for(c, nb, BLOCK, 8)));
-        HIPCHK(hipMemcpyAsync(counts.data(), jp.sub_cnt, P * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        const unsigned mx = *std::max_element(counts.begin(), counts.end());
-        if (mx > (unsigned)(jp.T - jp.T / 4)) return -1;  // skewed keys: direct probe
-        HIPCHK(launch_join_part_build(c->stream, jp, ba, grid_for(c, nb, BLOCK, 16)));
+    JoinSubTables built;
+    bool ok = false;
+    if (idx) {  // built once per shape and kept by the index
+        std::lock_guard<std::mutex> lk(idx->sub_mu);
+        const JoinSubTables *hit = nullptr;
+        for (const auto &st : idx->subs)
+            if (st.log_p == jp.log_p && st.T == jp.T && st.val_col == val_col) hit = &st;
+        if (!hit) {
+            JoinSubTables st;
+            st.log_p = jp.log_p;
+            st.T = jp.T;
+            st.val_col = val_col;
+            const size_t sub_bytes = (size_t)P * jp.T * 16, min_bytes = (size_t)jp.min_key_cap * 4;
+            HIPCHK(hipMalloc(&st.sub, sub_bytes));
+            if (min_bytes && hipMalloc(&st.min_key_rows, min_bytes) != hipSuccess) {
+                (void)hipFree(st.sub);
+                return set_err(FGPU_E_OUT_OF_MEMORY, "join sub-tables: %zu bytes", min_bytes);
+            }
+            jp.sub = st.sub;
+            jp.min_key_rows = st.min_key_rows;
+            bool fits = false;
+            rc = join_build_sub_tables(c, jp, ba, st, fits);
+            if (rc || !fits) {
+                (void)hipFree(st.sub);
+                if (st.min_key_rows) (void)hipFree(st.min_key_rows);
+                st.sub = nullptr;
+                st.min_key_rows = nullptr;
+                if (rc) return rc;
+            } else {
+                idx->bytes += (int64_t)(sub_bytes + min_bytes);
+            }
+            idx->subs.push_back(st);
+            hit = &idx->subs.back();
+        }
+        ok = hit->sub != nullptr;
+        built = *hit;
+    } else {  // no index: the context's scratch, rebuilt per call
+        if ((rc = buf(c, B_SUB, (size_t)P * jp.T * 16, &p))) return rc;
+        built.sub = (int64_t *)p;
+        if (jp.min_key_cap) {
+            if ((rc = buf(c, B_MINKEY, (size_t)jp.min_key_cap * 4, &p))) return rc;
+            built.min_key_rows = (uint32_t *)p;
+        }
+        jp.sub = built.sub;
+        jp.min_key_rows = built.min_key_rows;
+        if ((rc = join_build_sub_tables(c, jp, ba, built, ok))) return rc;
     }
     mark(c, 5);
+    if (!ok) return -1;  // a sub-table would overflow: direct probe
+    jp.sub = built.sub;
+    jp.min_key_rows = built.min_key_rows;
+    jp.pa.min_key_rows = built.min_key_rows;
+    jp.pa.n_min_key_rows = built.n_min_key_rows;
+    jp.pa.unique_keys = built.unique_keys;
     unsigned long long h[4] = {0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(h, d_flags, 32, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (h[1]) return -1;  // an insert found no slot: direct probe
-    jp.pa.min_key_rows = jp.min_key_rows;
-    jp.pa.n_min_key_rows = (int64_t)h[2];
-    jp.pa.unique_keys = !h[3] && h[2] <= 1;
     HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
     jp.xoff = nullptr;
     jp.chunks = nullptr;
@@ -1719,7 +1800,7 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     a.state = d_state;
     hipError_t e = hipSuccess;
     c->plan.clear();
-    const int prc = join_agg_partitioned(c, probe, build, spec, a, L);
+    const int prc = join_agg_partitioned(c, probe, build, idx, spec, a, L);
     if (prc > 0) return prc;
     c->plan += prc == 0 ? "join:partitioned" : "join:direct";
     if (prc < 0) {  // direct probe of the open-addressing index

@@ -209,7 +209,11 @@ int fgpu_groupby(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
 /* ── hash join ───────────────────────────────────────────────────────────── */
 /* Builds a device hash index over `key_col` of the build table
  * (build_hash_table / IntHashTable::insert, src/query/join.rs:607-645,
- *  :1054-1107, :1318-1324).  Reusable across probes and contexts. */
+ *  :1054-1107, :1318-1324).  Reusable across probes and contexts (also
+ *  concurrently).  The first partitioned join through an index adds the
+ *  build side's partitioned sub-tables to it (P * T * 16 bytes, 64 MB for
+ *  1M build keys), so later joins skip that build;
+ *  fgpu_join_index_device_bytes counts them. */
 int fgpu_join_build(fgpu_ctx *ctx, const fgpu_table *build, int32_t key_col,
                     fgpu_join_index **out);
 void fgpu_join_index_free(fgpu_join_index *idx);

@@ -424,6 +424,33 @@ def test_join_partitioned_matches_direct_at_scale(ctx_part, ctx_direct):
         assert_batches_equal(res[0], res[1], None)
 
 
+def test_join_index_keeps_partitioned_sub_tables(ctx_part, ctx_direct):
+    # The first partitioned join through an index builds the build side's
+    # sub-tables into it; joins holding another build column in the slots add
+    # their own; repeats reuse them and give the same results as the direct path.
+    pcols, bcols = _c3(150_000, 3_000_000)
+    pcols["f"] = (np.arange(3_000_000) % 100).astype(np.int32)
+    bcols["bpay2"] = np.arange(bcols["bk"].size, dtype=np.float64) * 0.5
+    pt, bt = _table(ctx_part, pcols), _table(ctx_part, bcols)
+    idx = fa.JoinIndex(ctx_part, bt, "bk")
+    a1 = [AggExpr(COUNT), AggExpr(SUM, "bpay", side="build")]
+    a2 = [AggExpr(COUNT), AggExpr(MAX, "bpay2", side="build"), AggExpr(SUM, "ppay")]
+    sizes, got = [idx.device_bytes()], []
+    for jt, aggs in ((INNER, a1), (LEFT, a1), (INNER, a2), (INNER, a1), (SEMI, a2[:1] + a2[2:]), (LEFT, a2)):
+        got.append(fa.execute_hash_join(ctx_part, pt, bt, "pk", "bk", jt, aggs, filter=col("f") < 70, index=idx))
+        assert ctx_part.last_plan().startswith("join:partitioned"), ctx_part.last_plan()
+        sizes.append(idx.device_bytes())
+    # new entries: bpay slots (1st join), bpay2 slots (3rd), build-row slots (5th)
+    assert sizes[1] > sizes[0] and sizes[2] == sizes[1] and sizes[3] > sizes[2]
+    assert sizes[4] == sizes[3] and sizes[5] > sizes[4] and sizes[6] == sizes[5]
+    pd, bd = _table(ctx_direct, pcols), _table(ctx_direct, bcols)
+    for (jt, aggs), g in zip(((INNER, a1), (LEFT, a1), (INNER, a2), (INNER, a1), (SEMI, a2[:1] + a2[2:]), (LEFT, a2)),
+                             got):
+        e = fa.execute_hash_join(ctx_direct, pd, bd, "pk", "bk", jt, aggs, filter=col("f") < 70)
+        assert_batches_equal(g, e, None)
+    idx.free()
+
+
 def test_groupby_partitioned_overflow_falls_back(ctx_gbp):
     # A fan-out and pass-2 table far too small for the keys: the bound is
     # detected, and the query is answered by the single-table hash path.

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchQuery *__restric
         unsigned int *cursor = (unsigned int *)(lds + S);   // BQ_PART: P cursors
         for (int i = threadIdx.x; i < S; i += blockDim.x) null_st[i] = a.ops.ident[i];
         if (dense) {
-            for (int64_t i = threadIdx.x; i < Q.range * S; i += blockDim.x) table[i] = a.ops.ident[i % S];
+            fill_ident(table, Q.range, S, a.ops);
         } else {
             for (int i = threadIdx.x; i < Q.P; i += blockDim.x) cursor[i] = 0;
         }
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(1024) void k_batch_merge(const BatchQuery *__restri
         const int G = (int)Q.n_units;
         uint64_t *table = lds;                                   // n * S words
         unsigned int *pre = (unsigned int *)(lds + (int64_t)n * S);  // BQ_PART: G + 1 prefix
-        for (int64_t i = threadIdx.x; i < (int64_t)n * S; i += blockDim.x) table[i] = a.ops.ident[i % S];
+        fill_ident(table, n, S, a.ops);
         if (M.part >= 0)
             for (int g = threadIdx.x; g < G; g += blockDim.x) {
                 const unsigned c = Q.cell_cnt[(int64_t)g * Q.P + M.part];

@@ -438,7 +438,7 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     L.back = L.front + P;
     L.door = L.back + P;
     for (int i = threadIdx.x; i < H; i += blockDim.x) L.keys[i] = EMPTY_KEY;
-    for (int i = threadIdx.x; i < H * S + S; i += blockDim.x) L.state[i] = a.ops.ident[i % S];
+    fill_ident(L.state, (int64_t)H + 1, S, a.ops);  // H cache rows, then the NULL group's
     for (int i = threadIdx.x; i < 2 * P + GBP_DOOR_WORDS; i += blockDim.x) L.front[i] = 0;
     __syncthreads();
     const int64_t r0 = (int64_t)blockIdx.x * A.rows_per_wg;
@@ -614,13 +614,23 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     const int lane = threadIdx.x & 63;
     // pass 1 overflowed a cell: the host resizes and runs both passes again
     if (__hip_atomic_load(A.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    // the cell counts' loads are issued first, so they land during the table init
+    unsigned nf = 0, ns = 0;
+    if ((int)threadIdx.x < G) {
+        const int64_t c = (int64_t)threadIdx.x * P + p;
+        nf = A.cell_cnt[2 * c];
+        ns = A.cell_cnt[2 * c + 1];
+    }
     for (int i = threadIdx.x; i < T2; i += blockDim.x) tkeys[i] = EMPTY_KEY;
-    for (int i = threadIdx.x; i < T2 * S; i += blockDim.x) tstate[i] = a.ops.ident[i % S];
+    fill_ident(tstate, T2, S, a.ops);
     for (int g = threadIdx.x; g < G; g += blockDim.x) {
-        const int64_t c = (int64_t)g * P + p;
-        const unsigned nf = A.cell_cnt[2 * c];
+        if (g != (int)threadIdx.x) {  // G > blockDim.x
+            const int64_t c = (int64_t)g * P + p;
+            nf = A.cell_cnt[2 * c];
+            ns = A.cell_cnt[2 * c + 1];
+        }
         pre[g] = nf < (unsigned)A.Cw ? nf : (unsigned)A.Cw;
-        spre[g] = A.cell_cnt[2 * c + 1];
+        spre[g] = ns;
     }
     __syncthreads();
     if (threadIdx.x < 64) {  // exclusive scans of both count arrays by one wave

@@ -101,7 +101,7 @@ __global__ void k_gb_dense_lds(GbArgs a) {
     extern __shared__ uint64_t lds[];
     const int S = a.ops.S;
     const int64_t groups = a.range + 1;  // + NULL group
-    for (int64_t i = threadIdx.x; i < groups * S; i += blockDim.x) lds[i] = a.ops.ident[i % S];
+    fill_ident(lds, groups, S, a.ops);
     __syncthreads();
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
@@ -179,7 +179,7 @@ __global__ void k_gb_hash(GbArgs a) {
     int64_t *lkeys = (int64_t *)lds;
     uint64_t *lstate = lds + ls;
     for (int i = threadIdx.x; i < ls; i += blockDim.x) lkeys[i] = EMPTY_KEY;
-    for (int i = threadIdx.x; i < ls * S; i += blockDim.x) lstate[i] = a.ops.ident[i % S];
+    fill_ident(lstate, ls, S, a.ops);
     __syncthreads();
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {

@@ -118,6 +118,20 @@ FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
     }
 }
 
+// Fills n_rows group states of S words with the ops' identity values.
+// Consecutive threads write consecutive words; each thread steps its word
+// index instead of dividing per word (a 64-bit modulo is a long sequence).
+FGPU_DEV void fill_ident(uint64_t *dst, int64_t n_rows, int S, const DOps &ops) {
+    const int64_t total = n_rows * S;
+    const int step = (int)(blockDim.x % (unsigned)S);
+    int w = (int)(threadIdx.x % (unsigned)S);
+    for (int64_t i = threadIdx.x; i < total; i += blockDim.x) {
+        dst[i] = ops.ident[w];
+        w += step;
+        if (w >= S) w -= S;
+    }
+}
+
 // Exclusive prefix of v over the workgroup and the workgroup total.  Every
 // thread of the block calls it (blockDim.x a multiple of 64, at most 1024).
 FGPU_DEV unsigned block_excl_scan(unsigned v, unsigned &total) {

@@ -83,9 +83,11 @@ def bench_batched(ctx, lib, steps):
     ctx.set_tuning("keep_on_device", 0)
     batch = fa.BatchGroupBy(ctx, queries)
     same = True
+    groups = 0
     for (t, key, ag, f), got in zip(queries, batch.run()):
         plain = fa.GroupByExec([key], ag).execute(ctx, t, f).sort_by(key)
         got = got.sort_by(key)
+        groups += got.num_rows
         same &= got.num_rows == plain.num_rows and all(
             np.array_equal(a, b) for a, b in zip(got.columns, plain.columns))
     ctx.set_tuning("keep_on_device", 1)
@@ -118,10 +120,11 @@ def bench_batched(ctx, lib, steps):
         batched()
         dev_ms.append(ctx.last_timing()["main_ms"])
     batch_ms = (time.perf_counter() - t0) * 1e3 / steps
-    alg = rows * (4 + 8 + 8)
+    alg = rows * (4 + 8 + 8) + groups * 48  # key, val, score read once; every group row written
     log(f"[bench] C4: plain {plain_ms:.2f} ms, batched {batch_ms:.2f} ms ({np.mean(dev_ms):.2f} ms on "
         f"the device), parity {same}; {tune_info}")
-    out = {"queries": len(qs), "rows": rows, "plain_ms": plain_ms, "batched_ms": batch_ms,
+    out = {"queries": len(qs), "rows": rows, "groups": groups, "algorithmic_bytes": alg,
+           "plain_ms": plain_ms, "batched_ms": batch_ms,
            "batched_device_ms": float(np.mean(dev_ms)), "speedup": plain_ms / batch_ms,
            "rows_per_s": rows / (batch_ms * 1e-3), "achieved_GBs": alg / (np.mean(dev_ms) * 1e-3) / 1e9,
            "frac": alg / (np.mean(dev_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, "parity_vs_plain": bool(same),

@@ -2450,9 +2450,11 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
     std::lock_guard<std::mutex> lk(c->mu);
     if (int rc = set_device(c->device)) return rc;
     std::vector<BatchCfg> cands;
+    // slots: 4096 is capped at the largest table the state width lets fit
+    // (one pass-2 workgroup per CU); 512 and 256 let several share a CU
     for (int block : {1024, 512})
         for (int64_t ur : {int64_t(1) << 18, int64_t(1) << 16, int64_t(1) << 20})
-            for (int slots : {2048, 4096, 1024}) {
+            for (int slots : {4096, 512, 256}) {
                 BatchCfg k;
                 k.block = block;
                 k.unit_rows = ur;

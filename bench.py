@@ -289,12 +289,20 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--qps", default="100,300,1000,3000")
     ap.add_argument("--cpu-sample-rows", type=int, default=C2_ROWS)
+    ap.add_argument("--ceiling", action="store_true",
+                    help="only run the STREAM-like read probe (the scan's four columns, its tiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.calibrate:
         calibrate_copy()
         return
+    if args.ceiling:  # a child process: this one has not touched the GPU
+        import subprocess
+        exe = ROOT / "falcondb_amd" / "fgpu_stream_read"
+        if not exe.exists():
+            sys.exit(f"{exe} is not built (python -m falcondb_amd.build)")
+        sys.exit(subprocess.run([str(exe)]).returncode)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)

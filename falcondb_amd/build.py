@@ -68,8 +68,19 @@ def build_datagen_lib() -> Path:
     return GEN_LIB
 
 
+STREAM_READ = PKG / "fgpu_stream_read"
+
+
+def build_stream_read() -> Path:
+    """The standalone read-ceiling probe (bench.py --ceiling)."""
+    tmp = STREAM_READ.with_suffix(".tmp")
+    _run([HIPCC, *HIP_FLAGS, "-o", str(tmp), str(CSRC / "fgpu_stream_read.hip")])
+    os.replace(tmp, STREAM_READ)
+    return STREAM_READ
+
+
 def build_all() -> list[Path]:
-    return [build_gpu_lib(), build_datagen_lib()]
+    return [build_gpu_lib(), build_datagen_lib(), build_stream_read()]
 
 
 if __name__ == "__main__":

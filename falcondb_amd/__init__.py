@@ -7,14 +7,14 @@ Native pieces (built by `python -m falcondb_amd.build`):
 
 The Python layer mirrors the reference's operator names (see operators.py).
 """
-from .operators import (AggExpr, AggFunc, BatchGroupBy, ColumnRef, Context, DictColumn, Expr, FalconError,
-                        GroupByExec, JoinIndex, JoinType, NullableColumn, RecordBatch, Table,
+from .operators import (AggExpr, AggFunc, BatchDictColumn, BatchGroupBy, ColumnRef, Context, DictColumn, Expr,
+                        FalconError, GroupByExec, JoinIndex, JoinType, NullableColumn, RecordBatch, Table,
                         col, cmp, count_star, execute_agg_only, execute_hash_join,
                         execute_hash_semi_anti_join, execute_select, host_exact_sum, join_pairs,
                         lit_cmp, pack_validity)
 
 __all__ = [
-    "AggExpr", "AggFunc", "BatchGroupBy", "ColumnRef", "Context", "DictColumn", "Expr", "FalconError",
+    "AggExpr", "AggFunc", "BatchDictColumn", "BatchGroupBy", "ColumnRef", "Context", "DictColumn", "Expr", "FalconError",
     "GroupByExec", "JoinIndex", "JoinType", "NullableColumn", "RecordBatch", "Table", "col",
     "cmp", "count_star", "execute_agg_only", "execute_hash_join", "execute_hash_semi_anti_join",
     "execute_select", "host_exact_sum", "join_pairs", "lit_cmp", "pack_validity",

@@ -32,6 +32,11 @@ class ColumnDesc(C.Structure):
                 ("dict", C.POINTER(C.c_char_p))]
 
 
+class Utf8Batches(C.Structure):
+    _fields_ = [("n_batches", C.c_int32), ("batch_rows", C.POINTER(C.c_int64)),
+                ("dict_len", C.POINTER(C.c_int32)), ("dicts", C.POINTER(C.POINTER(C.c_char_p)))]
+
+
 class PredNode(C.Structure):
     _fields_ = [("kind", C.c_int32), ("col", C.c_int32), ("op", C.c_int32),
                 ("lit_type", C.c_int32), ("lit_i64", C.c_int64), ("lit_f64", C.c_double),
@@ -107,6 +112,8 @@ DECLARED = {
     "fgpu_scratch_bytes": (C.c_int64, [C.c_void_p]),
     "fgpu_table_upload": (C.c_int, [C.c_void_p, C.POINTER(ColumnDesc), C.c_int, C.c_int64,
                                     C.c_int32, C.POINTER(C.c_void_p)]),
+    "fgpu_table_upload_batches": (C.c_int, [C.c_void_p, C.POINTER(ColumnDesc), C.POINTER(Utf8Batches), C.c_int,
+                                            C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "fgpu_table_free": (None, [C.c_void_p]),
     "fgpu_table_rows": (C.c_int64, [C.c_void_p]),
     "fgpu_table_device_bytes": (C.c_int64, [C.c_void_p]),

@@ -340,6 +340,38 @@ __global__ void k_col_stats(const void *values, int dtype, const uint64_t *valid
         if (hll[i]) atomicMax(&st->hll[i], hll[i]);
 }
 
+// Batch-local dictionary codes to a table-wide dictionary: row r of batch b
+// (row_off[b] <= r < row_off[b + 1]) maps code c to lut[lut_off[b] + c]
+// (Utf8Column::concat_all's per-part remap, src/storage/column.rs:1576-1589).
+// A code past its batch's dictionary becomes NULL and sets *bad.
+__global__ void k_dict_remap(uint32_t *codes, int64_t n, const int64_t *row_off, int n_batches,
+                             const uint32_t *lut, const int64_t *lut_off, unsigned long long *bad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        const uint32_t c = codes[r];
+        if (c == NULL_CODE) continue;
+        int lo = 0, hi = n_batches;  // the last batch whose first row is <= r
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (row_off[mid] <= r) lo = mid;
+            else hi = mid;
+        }
+        const int64_t base = lut_off[lo], len = lut_off[lo + 1] - base;
+        if ((int64_t)c >= len) {
+            atomicExch(bad, 1ull);
+            codes[r] = NULL_CODE;
+            continue;
+        }
+        codes[r] = lut[base + c];
+    }
+}
+
+hipError_t launch_dict_remap(hipStream_t s, uint32_t *codes, int64_t n, const int64_t *row_off, int n_batches,
+                             const uint32_t *lut, const int64_t *lut_off, unsigned long long *bad, int grid) {
+    hipLaunchKernelGGL(k_dict_remap, dim3(grid), dim3(BLOCK), 0, s, codes, n, row_off, n_batches, lut, lut_off, bad);
+    return hipGetLastError();
+}
+
 hipError_t launch_col_stats(hipStream_t s, const void *values, int dtype, const uint64_t *valid,
                             int64_t n, ColStatsDev *d_stats, int grid) {
     hipLaunchKernelGGL(k_col_stats, dim3(grid), dim3(BLOCK), 0, s, values, dtype, valid, n, d_stats);

@@ -11,6 +11,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "fgpu_host.h"
@@ -856,8 +857,94 @@ void fgpu_table_free(fgpu_table *t) {
     delete t;
 }
 
-int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols, int64_t n_rows,
-                      int32_t batch_rows, fgpu_table **out) {
+// A Utf8 column's per-batch dictionaries merged into one, following
+// Utf8Column::concat_all (src/storage/column.rs:1539-1595).
+struct DictMerge {
+    std::vector<std::string> dict;  // the table-wide dictionary
+    bool keep = false;              // every batch has the first batch's dictionary: codes stay
+    std::vector<uint32_t> lut;      // local code -> global code, batch after batch
+    std::vector<int64_t> lut_off;   // n_batches + 1
+    std::vector<int64_t> row_off;   // n_batches + 1
+};
+
+static int merge_dicts(const fgpu_utf8_batches &ub, int64_t n_rows, const std::string &name, DictMerge &m) {
+    const int nb = ub.n_batches;
+    if (!ub.batch_rows || !ub.dict_len || !ub.dicts)
+        return set_err(FGPU_E_INVALID_QUERY, "column '%s': batch arrays are NULL", name.c_str());
+    m.row_off.assign((size_t)nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        if (ub.batch_rows[b] < 0 || ub.dict_len[b] < 0 || (ub.dict_len[b] > 0 && !ub.dicts[b]))
+            return set_err(FGPU_E_INVALID_QUERY, "column '%s': bad batch %d", name.c_str(), b);
+        m.row_off[(size_t)b + 1] = m.row_off[(size_t)b] + ub.batch_rows[b];
+    }
+    if (m.row_off[(size_t)nb] != n_rows)
+        return set_err(FGPU_E_INVALID_QUERY, "column '%s': batches hold %lld rows, the table %lld", name.c_str(),
+                       (long long)m.row_off[(size_t)nb], (long long)n_rows);
+    auto str = [&](int b, int k) { return std::string(ub.dicts[b][k] ? ub.dicts[b][k] : ""); };
+    // fast paths: one shared dictionary array, or equal entries everywhere
+    bool same = true;
+    for (int b = 1; b < nb && same; ++b) {
+        if (ub.dict_len[b] != ub.dict_len[0]) same = false;
+        else if (ub.dicts[b] != ub.dicts[0])
+            for (int k = 0; k < ub.dict_len[0] && same; ++k) same = str(b, k) == str(0, k);
+    }
+    if (same) {
+        m.keep = true;
+        for (int k = 0; k < ub.dict_len[0]; ++k) m.dict.push_back(str(0, k));
+        return FGPU_OK;
+    }
+    // merge: first-seen order over the dictionaries in batch order
+    std::unordered_map<std::string, uint32_t> seen;
+    m.lut_off.assign((size_t)nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        for (int k = 0; k < ub.dict_len[b]; ++k) {
+            std::string t = str(b, k);
+            auto it = seen.find(t);
+            uint32_t g;
+            if (it != seen.end()) {
+                g = it->second;
+            } else {
+                g = (uint32_t)m.dict.size();
+                if (g == NULL_CODE) return set_err(FGPU_E_UNSUPPORTED, "column '%s': dictionary too large", name.c_str());
+                seen.emplace(t, g);
+                m.dict.push_back(std::move(t));
+            }
+            m.lut.push_back(g);
+        }
+        m.lut_off[(size_t)b + 1] = (int64_t)m.lut.size();
+    }
+    return FGPU_OK;
+}
+
+// Applies a merge's local-to-global maps to the column's device codes.
+static int remap_codes(fgpu_ctx *ctx, DevCol &c, const DictMerge &m) {
+    const int nb = (int)m.row_off.size() - 1;
+    const size_t lut_bytes = std::max<size_t>(m.lut.size(), 1) * 4, off_bytes = m.row_off.size() * 8;
+    char *mem = nullptr;
+    if (hipMalloc(&mem, lut_bytes + 2 * off_bytes + 8) != hipSuccess)
+        return set_err(FGPU_E_OUT_OF_MEMORY, "column '%s': dictionary maps", c.name.c_str());
+    int64_t *d_row_off = (int64_t *)mem, *d_lut_off = (int64_t *)(mem + off_bytes);
+    unsigned long long *d_bad = (unsigned long long *)(mem + 2 * off_bytes);
+    uint32_t *d_lut = (uint32_t *)(mem + 2 * off_bytes + 8);
+    unsigned long long bad = 0;
+    hipError_t e = hipMemcpy(d_row_off, m.row_off.data(), off_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_lut_off, m.lut_off.data(), off_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_bad, &bad, 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !m.lut.empty()) e = hipMemcpy(d_lut, m.lut.data(), m.lut.size() * 4, hipMemcpyHostToDevice);
+    const int64_t n = m.row_off.back();
+    if (e == hipSuccess)
+        e = launch_dict_remap(nullptr, (uint32_t *)c.d_values, n, d_row_off, nb, d_lut, d_lut_off, d_bad,
+                              grid_for(ctx, n, BLOCK, 8));
+    if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(mem);
+    if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "dictionary remap: %s", hipGetErrorString(e));
+    if (bad) return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "column '%s': a code is past its batch's dictionary",
+                            c.name.c_str());
+    return FGPU_OK;
+}
+
+static int table_upload_impl(fgpu_ctx *ctx, const fgpu_column_desc *cols, const fgpu_utf8_batches *utf8,
+                             int n_cols, int64_t n_rows, int32_t batch_rows, fgpu_table **out) {
     if (!ctx || !out || (n_cols > 0 && !cols)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
     if (n_rows < 0 || n_cols < 0) return set_err(FGPU_E_INVALID_QUERY, "negative size");
     if (n_rows >= (int64_t(1) << 31))
@@ -891,25 +978,35 @@ int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols, i
             rc = set_err(FGPU_E_INVALID_QUERY, "column '%s': values is NULL", c.name.c_str());
             break;
         }
+        DictMerge merge;
+        const bool batched = d.dtype == FGPU_UTF8 && utf8 && utf8[i].n_batches > 0;
+        if (batched && (rc = merge_dicts(utf8[i], n_rows, c.name, merge))) break;
         if (d.dtype == FGPU_UTF8) {
-            if (d.dict_len < 0 || (d.dict_len > 0 && !d.dict)) {
-                rc = set_err(FGPU_E_INVALID_QUERY, "column '%s': bad dictionary", c.name.c_str());
-                break;
-            }
-            for (int k = 0; k < d.dict_len; ++k) {
-                c.dict.push_back(d.dict[k] ? d.dict[k] : "");
-                if (c.empty_code < 0 && c.dict.back().empty()) c.empty_code = k;
-            }
-            for (auto &s : c.dict) c.dictp.push_back(s.c_str());
-            // every non-NULL code must index the dictionary
-            const uint32_t *codes = (const uint32_t *)d.values;
-            for (int64_t r = 0; r < n_rows; ++r)
-                if (codes[r] != NULL_CODE && codes[r] >= (uint32_t)d.dict_len) {
-                    rc = set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "column '%s': code %u at row %lld >= dictionary size %d",
-                                 c.name.c_str(), codes[r], (long long)r, d.dict_len);
+            if (batched) {
+                c.dict = merge.dict;
+            } else {
+                if (d.dict_len < 0 || (d.dict_len > 0 && !d.dict)) {
+                    rc = set_err(FGPU_E_INVALID_QUERY, "column '%s': bad dictionary", c.name.c_str());
                     break;
                 }
-            if (rc) break;
+                for (int k = 0; k < d.dict_len; ++k) c.dict.push_back(d.dict[k] ? d.dict[k] : "");
+            }
+            for (size_t k = 0; k < c.dict.size(); ++k)
+                if (c.empty_code < 0 && c.dict[k].empty()) c.empty_code = (int)k;
+            for (auto &s : c.dict) c.dictp.push_back(s.c_str());
+            // every non-NULL code must index the dictionary (merged columns: on the device)
+            if (!batched || merge.keep) {
+                const uint32_t *codes = (const uint32_t *)d.values;
+                const uint32_t len = (uint32_t)c.dict.size();
+                for (int64_t r = 0; r < n_rows; ++r)
+                    if (codes[r] != NULL_CODE && codes[r] >= len) {
+                        rc = set_err(FGPU_E_INDEX_OUT_OF_BOUNDS,
+                                     "column '%s': code %u at row %lld >= dictionary size %u", c.name.c_str(),
+                                     codes[r], (long long)r, len);
+                        break;
+                    }
+                if (rc) break;
+            }
         }
         const size_t bytes = (size_t)std::max<int64_t>(n_rows, 1) * c.elem_bytes();
         hipError_t e = hipMalloc(&c.d_values, bytes);
@@ -917,6 +1014,7 @@ int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols, i
         t->bytes += (int64_t)bytes;
         if (n_rows) e = hipMemcpy(c.d_values, d.values, (size_t)n_rows * c.elem_bytes(), hipMemcpyHostToDevice);
         if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "upload: %s", hipGetErrorString(e)); break; }
+        if (batched && !merge.keep && n_rows && (rc = remap_codes(ctx, c, merge))) break;
         if (d.validity && d.dtype != FGPU_UTF8) {
             const size_t vb = (size_t)((n_rows + 63) / 64) * 8;
             e = hipMalloc(&c.d_valid, std::max<size_t>(vb, 8));
@@ -953,6 +1051,16 @@ int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols, i
     if (rc) return fail(rc);
     *out = t;
     return FGPU_OK;
+}
+
+int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols, int64_t n_rows,
+                      int32_t batch_rows, fgpu_table **out) {
+    return table_upload_impl(ctx, cols, nullptr, n_cols, n_rows, batch_rows, out);
+}
+
+int fgpu_table_upload_batches(fgpu_ctx *ctx, const fgpu_column_desc *cols, const fgpu_utf8_batches *utf8,
+                              int n_cols, int64_t n_rows, int32_t batch_rows, fgpu_table **out) {
+    return table_upload_impl(ctx, cols, utf8, n_cols, n_rows, batch_rows, out);
 }
 
 int64_t fgpu_table_rows(const fgpu_table *t) { return t ? t->n_rows : -1; }

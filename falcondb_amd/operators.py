@@ -124,6 +124,17 @@ class DictColumn:
 
 
 @dataclass
+class BatchDictColumn:
+    """A text column as record batches with one dictionary each (what the CSV /
+    Parquet loaders build): batch-local u32 codes of every batch in row order,
+    the batches' row counts and their dictionaries.  Upload merges the
+    dictionaries like Utf8Column::concat_all (src/storage/column.rs:1539-1595)."""
+    codes: np.ndarray
+    batch_rows: Sequence[int]
+    dictionaries: Sequence[Sequence[str]]
+
+
+@dataclass
 class NullableColumn:
     """A primitive column with a validity mask (True = valid)."""
     values: np.ndarray
@@ -160,12 +171,32 @@ class Table:
         self.dicts: list[list[str] | None] = []
         keep = []  # host buffers alive for the duration of the upload
         descs = (_abi.ColumnDesc * max(len(columns), 1))()
+        utf8 = (_abi.Utf8Batches * max(len(columns), 1))()
+        batched = False
         n_rows = None
         for i, (name, col) in enumerate(columns.items()):
             d = descs[i]
             d.name = name.encode()
             valid_words = None
-            if isinstance(col, DictColumn):
+            if isinstance(col, BatchDictColumn):
+                codes = np.ascontiguousarray(col.codes, dtype=np.uint32)
+                rows = np.ascontiguousarray(col.batch_rows, dtype=np.int64)
+                if len(col.dictionaries) != rows.size:
+                    raise FalconError(6, f"column '{name}': {rows.size} batches, {len(col.dictionaries)} dictionaries")
+                lens = np.array([len(dd) for dd in col.dictionaries], dtype=np.int32)
+                arrs = [(C.c_char_p * max(len(dd), 1))(*[x.encode() for x in dd]) for dd in col.dictionaries]
+                parr = (C.POINTER(C.c_char_p) * max(len(arrs), 1))(*[C.cast(a, C.POINTER(C.c_char_p)) for a in arrs])
+                keep += [codes, rows, lens, arrs, parr]
+                u = utf8[i]
+                u.n_batches = rows.size
+                u.batch_rows = rows.ctypes.data_as(C.POINTER(C.c_int64))
+                u.dict_len = lens.ctypes.data_as(C.POINTER(C.c_int32))
+                u.dicts = C.cast(parr, C.POINTER(C.POINTER(C.c_char_p)))
+                d.dtype, d.values = UTF8, codes.ctypes.data
+                n = codes.size
+                batched = True
+                self.dicts.append(None)  # merged on upload
+            elif isinstance(col, DictColumn):
                 codes = np.ascontiguousarray(col.codes, dtype=np.uint32)
                 strs = [s.encode() for s in col.dictionary]
                 arr = (C.c_char_p * max(len(strs), 1))(*strs)
@@ -196,8 +227,12 @@ class Table:
             elif n != n_rows:
                 raise FalconError(6, f"column '{name}' has {n} rows, expected {n_rows}")
         h = C.c_void_p()
-        _check(_lib().fgpu_table_upload(ctx.handle, descs, len(columns), int(n_rows or 0),
-                                        batch_rows, C.byref(h)))
+        if batched:
+            _check(_lib().fgpu_table_upload_batches(ctx.handle, descs, utf8, len(columns), int(n_rows or 0),
+                                                    batch_rows, C.byref(h)))
+        else:
+            _check(_lib().fgpu_table_upload(ctx.handle, descs, len(columns), int(n_rows or 0),
+                                            batch_rows, C.byref(h)))
         self._h = h
         self.n_rows = int(n_rows or 0)
         self.batch_rows = batch_rows

@@ -172,6 +172,28 @@ int64_t fgpu_scratch_bytes(const fgpu_ctx *ctx);
  * approximate distinct count) used to choose kernels. */
 int fgpu_table_upload(fgpu_ctx *ctx, const fgpu_column_desc *cols, int n_cols,
                       int64_t n_rows, int32_t batch_rows, fgpu_table **out);
+/* A Utf8 column delivered as record batches with one dictionary each, as the
+ * CSV and Parquet loaders build them (src/storage/csv.rs:232-237,
+ * src/storage/parquet_io.rs:194-207). */
+typedef struct {
+    int32_t n_batches;
+    const int64_t *batch_rows;        /* rows of each batch, in row order; they sum to n_rows */
+    const int32_t *dict_len;          /* entries of each batch's dictionary */
+    const char *const *const *dicts;  /* each batch's dict_len[b] strings */
+} fgpu_utf8_batches;
+
+/* fgpu_table_upload for tables whose Utf8 columns carry one dictionary per
+ * batch.  For a FGPU_UTF8 column i with utf8[i].n_batches > 0, cols[i].values
+ * holds every batch's local codes in row order and cols[i].dict is ignored.
+ * The dictionaries are merged the way Utf8Column::concat_all merges them
+ * (src/storage/column.rs:1539-1595): batches that share one dictionary (the
+ * same array, or equal entries) keep it; otherwise entries are numbered in
+ * first-seen order over the batches' dictionaries in batch order, and each
+ * batch's codes are remapped through its local-to-global map on the device.
+ * A local code past its batch's dictionary fails with
+ * FGPU_E_INDEX_OUT_OF_BOUNDS.  utf8 == NULL makes this fgpu_table_upload. */
+int fgpu_table_upload_batches(fgpu_ctx *ctx, const fgpu_column_desc *cols, const fgpu_utf8_batches *utf8,
+                              int n_cols, int64_t n_rows, int32_t batch_rows, fgpu_table **out);
 void fgpu_table_free(fgpu_table *t);
 int64_t fgpu_table_rows(const fgpu_table *t);
 int64_t fgpu_table_device_bytes(const fgpu_table *t);

@@ -253,3 +253,34 @@ def join(probe: dict, build: dict, probe_key: str, build_key: str, join_type: in
     lib().ora_free(pl)
     lib().ora_free(pr)
     return res, l, r
+
+
+def concat_dicts(codes: np.ndarray, batch_rows, dictionaries) -> DictColumn:
+    """Utf8Column::concat_all (src/storage/column.rs:1539-1595) over batches
+    with one dictionary each: batches sharing one dictionary (equal entries,
+    :1549-1565) keep it; otherwise the merged dictionary takes each batch's
+    entries in first-seen order, batch after batch (:1567-1585), and every
+    batch's codes go through its local-to-global map (:1587-1589).  NULL
+    (0xFFFFFFFF) stays NULL.  Plain Python: batches and dictionaries are small
+    next to the codes, which numpy maps."""
+    codes = np.asarray(codes, dtype=np.uint32)
+    dicts = [list(d) for d in dictionaries]
+    if all(d == dicts[0] for d in dicts):
+        return DictColumn(codes.copy(), dicts[0] if dicts else [])
+    merged, seen, out = [], {}, codes.copy()
+    r0 = 0
+    for n, d in zip(batch_rows, dicts):
+        remap = []
+        for s in d:
+            if s not in seen:
+                seen[s] = len(merged)
+                merged.append(s)
+            remap.append(seen[s])
+        part = codes[r0:r0 + n]
+        lut = np.array(remap + [0], dtype=np.uint32)  # slot len(d) is never read for valid codes
+        valid = part != 0xFFFFFFFF
+        if np.any(part[valid] >= len(d)):
+            raise IndexError("a code is past its batch's dictionary")
+        out[r0:r0 + n] = np.where(valid, lut[np.where(valid, part, len(d))], part)
+        r0 += n
+    return DictColumn(out, merged)

@@ -239,6 +239,52 @@ def test_groupby_nulls_and_float_minmax(gctx):
     assert_batches_equal(g, o.exact, "k")
 
 
+def test_upload_batches_merges_dictionaries(ctx):
+    # A loader-shaped text column: one dictionary per 65,536-row batch.  Upload
+    # merges them like concat_all (remapping the codes on the device); every
+    # query then equals the same strings uploaded with one table-wide dictionary.
+    rng = np.random.default_rng(21)
+    vocab = [f"dept{i:02d}" for i in range(20)] + [""]
+    rows = [65536] * 30 + [12345]
+    dicts, parts = [], []
+    for n in rows:
+        d = [str(s) for s in rng.permutation(vocab)[: int(rng.integers(1, len(vocab) + 1))]]
+        c = rng.integers(0, len(d), n).astype(np.uint32)
+        c[rng.random(n) < 0.05] = 0xFFFFFFFF
+        dicts.append(d)
+        parts.append(c)
+    codes = np.concatenate(parts)
+    x = rng.normal(size=codes.size)
+    m = ora.concat_dicts(codes, rows, dicts)
+    tb = _table(ctx, {"d": fa.BatchDictColumn(codes, rows, dicts), "x": x})
+    tm = _table(ctx, {"d": DictColumn(m.codes, m.dictionary), "x": x})
+    # row by row: the strings the batches' own dictionaries give
+    sel = fa.execute_select(ctx, tb, None, ["d"]).columns[0]
+    expect = [None if c == 0xFFFFFFFF else m.dictionary[c] for c in m.codes.tolist()]
+    assert list(sel) == expect
+    # the device codes are the merged numbering itself
+    assert list(fa.execute_select(ctx, tm, None, ["d"]).columns[0]) == expect
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "x")]
+    gb = fa.GroupByExec(["d"], aggs).execute(ctx, tb, col("x") > 0)
+    gm = fa.GroupByExec(["d"], aggs).execute(ctx, tm, col("x") > 0)
+    assert_batches_equal(gb, gm, "d")
+    for lit in ("dept03", "", "zzz"):
+        assert fa.count_star(ctx, tb, col("d") == lit) == fa.count_star(ctx, tm, col("d") == lit)
+    assert fa.count_star(ctx, tb, col("d") < "dept10") == fa.count_star(ctx, tm, col("d") < "dept10")
+    # batches sharing one dictionary keep their codes
+    same = _table(ctx, {"d": fa.BatchDictColumn(parts[0][:300], [100, 200], [dicts[0], list(dicts[0])])})
+    assert list(fa.execute_select(ctx, same, None, ["d"]).columns[0]) == \
+        [None if c == 0xFFFFFFFF else dicts[0][c] for c in parts[0][:300].tolist()]
+    # a code past its batch's dictionary is an error, not a read past the map
+    bad = codes.copy()
+    bad[65536 + 7] = len(dicts[1])
+    with pytest.raises(fa.FalconError) as e:
+        _table(ctx, {"d": fa.BatchDictColumn(bad, rows, dicts)})
+    assert e.value.code == 3  # IndexOutOfBounds
+    for t in (tb, tm, same):
+        t.free()
+
+
 def test_groupby_utf8_with_null_keys(ctx):
     n = 30_000
     rng = np.random.default_rng(5)

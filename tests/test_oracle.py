@@ -9,10 +9,15 @@ Parity status of the oracle's exported functions:
       tests/count_star_filter.rs:95-117) + brute force + math.fsum
   ora_groupby (reference columns)      pinned to within 1e-12 of the exact sum;
       their last bits follow the restated lane / batch order, which nothing
-      outside the Rust binary can confirm (SURVEY Q12)
+      outside the Rust binary can confirm
   ora_join                             pinned: reference known answers
       (src/query/join.rs:1747-1848, :2607-2676) + nested-loop brute force
   ora_exact_sum                        pinned: math.fsum, bit for bit
+  concat_dicts (Python)                pinned: closed form of the merged dictionary
+      (ordered union of the batches' dictionaries), every row decoding to the
+      same string, the shared-dictionary fast path, and a planted row-order
+      numbering that the checks reject.  The reference's own concat_all tests
+      cover primitive columns only (src/storage/column.rs:549-590).
 """
 import math
 
@@ -235,3 +240,63 @@ def test_topk_brute_force(seed):
             rows = [i for i in range(n) if f[i] < 6]
             exp = sorted(rows, key=lambda i: (-v[i] if desc else v[i], i))[:k]
             assert ora.topk(cols, col("f") < 6, "v", k, desc).tolist() == exp
+
+
+def _batched_dicts(seed):
+    rng = np.random.default_rng(seed)
+    vocab = [f"w{i:02d}" for i in range(40)]
+    rows, dicts, codes = [], [], []
+    for b in range(7):
+        d = list(rng.permutation(vocab)[: int(rng.integers(0, 15))])
+        n = int(rng.integers(0, 300))
+        c = rng.integers(0, max(len(d), 1), n).astype(np.uint32)
+        if not d:
+            c[:] = 0xFFFFFFFF
+        c[rng.random(n) < 0.1] = 0xFFFFFFFF
+        # rows reach the dictionary's entries in another order than the dictionary's
+        if len(d) > 1 and n > 1:
+            c[0] = len(d) - 1
+        rows.append(n)
+        dicts.append(d)
+        codes.append(c)
+    return np.concatenate(codes) if codes else np.zeros(0, np.uint32), rows, dicts
+
+
+def _decode(codes, rows, dicts):
+    out, r0 = [], 0
+    for n, d in zip(rows, dicts):
+        out += [None if c == 0xFFFFFFFF else d[c] for c in codes[r0:r0 + n].tolist()]
+        r0 += n
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_concat_dicts_merge(seed):
+    codes, rows, dicts = _batched_dicts(seed)
+    m = ora.concat_dicts(codes, rows, dicts)
+    # closed form: the ordered union of the dictionaries, batch after batch
+    assert m.dictionary == list(dict.fromkeys(s for d in dicts for s in d))
+    # every row decodes to the string it had
+    got = [None if c == 0xFFFFFFFF else m.dictionary[c] for c in m.codes.tolist()]
+    assert got == _decode(codes, rows, dicts)
+
+
+def test_concat_dicts_shared_dictionary_keeps_codes():
+    d = ["x", "y", "z"]
+    codes = np.array([2, 0, 0xFFFFFFFF, 1, 2], np.uint32)
+    m = ora.concat_dicts(codes, [2, 3], [d, list(d)])
+    assert m.dictionary == d and np.array_equal(m.codes, codes)
+
+
+def test_concat_dicts_planted_row_order_numbering_is_caught():
+    # Numbering entries in the order rows first use them (instead of the
+    # dictionaries' order) decodes every row correctly but breaks the closed form.
+    codes, rows, dicts = _batched_dicts(3)
+    used = []
+    for s in _decode(codes, rows, dicts):
+        if s is not None and s not in used:
+            used.append(s)
+    merged = used + [s for d in dicts for s in d if s not in used]
+    merged = list(dict.fromkeys(merged))
+    assert merged != list(dict.fromkeys(s for d in dicts for s in d))
+    assert ora.concat_dicts(codes, rows, dicts).dictionary != merged

@@ -1,0 +1,498 @@
+"""The SQL subset in front of the device operators: the statements of the
+filter / GROUP BY / join / ORDER BY ... LIMIT workloads as a drop-in for
+Database::sql (src/db.rs:242).
+
+Plans follow the reference's step order (src/db.rs:1226-1744): FROM and JOIN,
+WHERE on the FROM table (the reference's scan-time filter), GROUP BY, ORDER BY,
+LIMIT, then projection in SELECT-list order.  The COUNT(*)-only query takes the
+count fast path (src/db.rs:1439-1476), and a correlated [NOT] EXISTS becomes a
+SEMI / ANTI join (src/query/sql.rs:586-897).  Output columns are named the
+reference's way (expr_alias, src/query/sql.rs:780-788): a column by its name, a
+qualified column by its last part, `AS` by the alias, anything else `col<i>`
+for its position in the SELECT list.
+
+Accepted:
+    SELECT item, ... FROM t [alias] [[INNER | LEFT [OUTER]] JOIN b [alias] ON x.k = y.k]
+        [WHERE pred] [GROUP BY col] [ORDER BY col [ASC | DESC]] [LIMIT n]
+    item := [t.]col [AS a] | COUNT(*) | AGG([t.]col) [AS a]      AGG: COUNT SUM AVG MIN MAX
+    pred := [t.]col op literal | literal op [t.]col | [t.]col IS [NOT] NULL
+            | pred AND pred | pred OR pred | (pred)
+            | [NOT] EXISTS (SELECT 1 FROM b [alias] WHERE x.k = y.k)   (a top-level conjunct)
+    op   := = | != | <> | < | <= | > | >=
+Anything else raises FalconError (InvalidQuery, or Unsupported for shapes the
+device path does not plan).
+"""
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .operators import (AggExpr, Context, Expr, FalconError, GroupByExec, JoinIndex, RecordBatch, Table, cmp,
+                        col, count_star, execute_agg_only, execute_hash_join, execute_select,
+                        execute_sort_limit, lit_cmp)
+
+PARSE, INVALID_QUERY, UNSUPPORTED = 4, 6, 102  # FalconError codes
+
+_TOKEN = re.compile(r"""
+    \s*(?:
+      (?P<num>-?\d+\.\d*(?:[eE][-+]?\d+)?|-?\d+[eE][-+]?\d+|-?\.\d+(?:[eE][-+]?\d+)?|-?\d+)
+    | (?P<str>'(?:[^']|'')*')
+    | (?P<op><>|!=|<=|>=|=|<|>)
+    | (?P<punct>[(),.*;])
+    | (?P<name>[A-Za-z_][A-Za-z0-9_]*|"[^"]+")
+    )""", re.X)
+
+_KEYWORDS = {"SELECT", "FROM", "WHERE", "GROUP", "BY", "ORDER", "LIMIT", "ASC", "DESC", "AND", "OR",
+             "NOT", "IS", "NULL", "JOIN", "INNER", "LEFT", "OUTER", "ON", "AS", "EXISTS"}
+_AGGS = {"COUNT": _abi.COUNT, "SUM": _abi.SUM, "AVG": _abi.AVG, "MIN": _abi.MIN, "MAX": _abi.MAX}
+
+
+def _tokens(text: str) -> list[tuple[str, str]]:
+    out, pos, text = [], 0, text.strip()
+    while pos < len(text):
+        m = _TOKEN.match(text, pos)
+        if not m or m.end() == pos:
+            raise FalconError(PARSE, f"SQL: cannot read '{text[pos:pos + 20]}'")
+        pos = m.end()
+        kind = m.lastgroup
+        val = m.group(kind)
+        if kind == "name" and not val.startswith('"') and val.upper() in _KEYWORDS:
+            kind, val = "kw", val.upper()
+        elif kind == "name" and val.startswith('"'):
+            val = val[1:-1]
+        out.append((kind, val))
+    while out and out[-1] == ("punct", ";"):
+        out.pop()
+    return out
+
+
+@dataclass
+class ColName:
+    table: str | None
+    name: str
+
+
+@dataclass
+class Item:
+    func: int | None          # None: a plain column
+    arg: ColName | None       # None with func: COUNT(*)
+    alias: str | None
+
+
+@dataclass
+class Exists:
+    negated: bool
+    table: str
+    alias: str | None
+    left: ColName
+    right: ColName
+
+
+@dataclass
+class Query:
+    items: list[Item]
+    table: str
+    alias: str | None
+    join: tuple | None = None          # (type, table, alias, left ColName, right ColName)
+    where: object = None               # predicate tree
+    group_by: ColName | None = None
+    order_by: tuple | None = None      # (ColName, desc)
+    limit: int | None = None
+    exists: list = field(default_factory=list)
+
+
+class _Parser:
+    def __init__(self, text: str):
+        self.t = _tokens(text)
+        self.i = 0
+
+    def peek(self, k: int = 0):
+        return self.t[self.i + k] if self.i + k < len(self.t) else ("end", "")
+
+    def take(self):
+        tok = self.peek()
+        self.i += 1
+        return tok
+
+    def accept(self, kind: str, val: str | None = None) -> bool:
+        k, v = self.peek()
+        if k == kind and (val is None or v == val):
+            self.i += 1
+            return True
+        return False
+
+    def expect(self, kind: str, val: str | None = None):
+        k, v = self.take()
+        if k != kind or (val is not None and v != val):
+            raise FalconError(PARSE, f"SQL: expected {val or kind}, found '{v or 'end'}'")
+        return v
+
+    def colname(self) -> ColName:
+        a = self.expect("name")
+        if self.accept("punct", "."):
+            return ColName(a, self.expect("name"))
+        return ColName(None, a)
+
+    def alias(self) -> str | None:
+        if self.accept("kw", "AS"):
+            return self.expect("name")
+        if self.peek()[0] == "name":
+            return self.take()[1]
+        return None
+
+    def item(self) -> Item:
+        k, v = self.peek()
+        if k == "name" and v.upper() in _AGGS and self.peek(1) == ("punct", "("):
+            self.i += 2
+            if self.accept("punct", "*"):
+                if v.upper() != "COUNT":
+                    raise FalconError(PARSE, f"SQL: {v}(*)")
+                arg = None
+            else:
+                arg = self.colname()
+            self.expect("punct", ")")
+            return Item(_AGGS[v.upper()], arg, self.alias())
+        return Item(None, self.colname(), self.alias())
+
+    def literal(self):
+        k, v = self.take()
+        if k == "num":
+            return float(v) if any(c in v for c in ".eE") else int(v)
+        if k == "str":
+            return v[1:-1].replace("''", "'")
+        raise FalconError(PARSE, f"SQL: expected a literal, found '{v or 'end'}'")
+
+    def operand_is_literal(self) -> bool:
+        return self.peek()[0] in ("num", "str")
+
+    def primary(self):
+        if self.accept("punct", "("):
+            e = self.disjunction()
+            self.expect("punct", ")")
+            return e
+        negated = False
+        if self.peek() == ("kw", "NOT") and self.peek(1) == ("kw", "EXISTS"):
+            self.i += 1
+            negated = True
+        if self.accept("kw", "EXISTS"):
+            self.expect("punct", "(")
+            self.expect("kw", "SELECT")
+            self.literal()
+            self.expect("kw", "FROM")
+            t = self.expect("name")
+            a = self.alias() if self.peek() != ("kw", "WHERE") else None
+            self.expect("kw", "WHERE")
+            left = self.colname()
+            if self.expect("op") != "=":
+                raise FalconError(UNSUPPORTED, "SQL: EXISTS needs an equality correlation")
+            right = self.colname()
+            self.expect("punct", ")")
+            return ("exists", Exists(negated, t, a, left, right))
+        if self.operand_is_literal():
+            lit = self.literal()
+            op = self.expect("op")
+            return ("cmp", self.colname(), op, lit, True)
+        c = self.colname()
+        if self.accept("kw", "IS"):
+            neg = self.accept("kw", "NOT")
+            self.expect("kw", "NULL")
+            return ("isnull", c, neg)
+        op = self.expect("op")
+        return ("cmp", c, op, self.literal(), False)
+
+    def conjunction(self):
+        e = self.primary()
+        while self.accept("kw", "AND"):
+            e = ("and", e, self.primary())
+        return e
+
+    def disjunction(self):
+        e = self.conjunction()
+        while self.accept("kw", "OR"):
+            e = ("or", e, self.conjunction())
+        return e
+
+    def query(self) -> Query:
+        self.expect("kw", "SELECT")
+        items = [self.item()]
+        while self.accept("punct", ","):
+            items.append(self.item())
+        self.expect("kw", "FROM")
+        table = self.expect("name")
+        alias = self.alias() if self.peek()[0] == "name" or self.peek() == ("kw", "AS") else None
+        q = Query(items, table, alias)
+        jtype = None
+        if self.accept("kw", "JOIN"):
+            jtype = _abi.INNER
+        elif self.accept("kw", "INNER"):
+            self.expect("kw", "JOIN")
+            jtype = _abi.INNER
+        elif self.accept("kw", "LEFT"):
+            self.accept("kw", "OUTER")
+            self.expect("kw", "JOIN")
+            jtype = _abi.LEFT
+        if jtype is not None:
+            bt = self.expect("name")
+            ba = self.alias() if self.peek()[0] == "name" or self.peek() == ("kw", "AS") else None
+            self.expect("kw", "ON")
+            left = self.colname()
+            if self.expect("op") != "=":
+                raise FalconError(UNSUPPORTED, "SQL: joins on an equality only")
+            q.join = (jtype, bt, ba, left, self.colname())
+        if self.accept("kw", "WHERE"):
+            q.where = self.disjunction()
+        if self.accept("kw", "GROUP"):
+            self.expect("kw", "BY")
+            q.group_by = self.colname()
+            if self.peek() == ("punct", ","):
+                raise FalconError(UNSUPPORTED, "SQL: GROUP BY over one column only")
+        if self.accept("kw", "ORDER"):
+            self.expect("kw", "BY")
+            c = self.colname()
+            desc = False
+            if self.accept("kw", "DESC"):
+                desc = True
+            else:
+                self.accept("kw", "ASC")
+            q.order_by = (c, desc)
+        if self.accept("kw", "LIMIT"):
+            n = self.literal()
+            if not isinstance(n, int) or n < 0:
+                raise FalconError(PARSE, "SQL: LIMIT takes a non-negative integer")
+            q.limit = n
+        if self.peek()[0] != "end":
+            raise FalconError(UNSUPPORTED, f"SQL: unsupported clause at '{self.peek()[1]}'")
+        return q
+
+
+def parse(text: str) -> Query:
+    """Parses one statement of the subset; [NOT] EXISTS conjuncts of WHERE are
+    moved to Query.exists (they plan as SEMI / ANTI joins)."""
+    q = _Parser(text).query()
+
+    def split(e):
+        if e is None:
+            return None
+        if e[0] == "exists":
+            q.exists.append(e[1])
+            return None
+        if e[0] == "and":
+            a, b = split(e[1]), split(e[2])
+            return b if a is None else a if b is None else ("and", a, b)
+        if _has_exists(e):
+            raise FalconError(UNSUPPORTED, "SQL: EXISTS only as a top-level AND term")
+        return e
+    q.where = split(q.where)
+    if len(q.exists) > 1 or (q.exists and q.join):
+        raise FalconError(UNSUPPORTED, "SQL: one join or one EXISTS per query")
+    return q
+
+
+def _has_exists(e) -> bool:
+    if e[0] == "exists":
+        return True
+    if e[0] in ("and", "or"):
+        return _has_exists(e[1]) or _has_exists(e[2])
+    return False
+
+
+def _output_name(item: Item, i: int) -> str:
+    if item.alias:
+        return item.alias
+    if item.func is None:
+        return item.arg.name
+    return f"col{i}"
+
+
+class Database:
+    """Named device tables and the SQL entry point (Database::sql,
+    src/db.rs:242).  Join indexes are built once per (build table, key)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.tables: dict[str, Table] = {}
+        self._indexes: dict[tuple[str, str], JoinIndex] = {}
+
+    def create_table(self, name: str, columns: dict, batch_rows: int = 65536) -> Table:
+        t = Table(self.ctx, columns, batch_rows=batch_rows)
+        self.register(name, t)
+        return t
+
+    def register(self, name: str, table: Table) -> None:
+        self.tables[name.lower()] = table
+
+    def drop(self, name: str) -> None:
+        t = self.tables.pop(name.lower())
+        for k in [k for k in self._indexes if k[0] == name.lower()]:
+            self._indexes.pop(k).free()
+        t.free()
+
+    def table(self, name: str) -> Table:
+        try:
+            return self.tables[name.lower()]
+        except KeyError:
+            raise FalconError(INVALID_QUERY, f"table '{name}' not found") from None
+
+    def index(self, name: str, key: str) -> JoinIndex:
+        k = (name.lower(), key.lower())
+        if k not in self._indexes:
+            self._indexes[k] = JoinIndex(self.ctx, self.table(name), key)
+        return self._indexes[k]
+
+    # ── planning ──────────────────────────────────────────────────────────
+    def sql(self, text: str) -> RecordBatch:
+        q = parse(text)
+        left = self.table(q.table)
+        names_left = {q.table.lower()} | ({q.alias.lower()} if q.alias else set())
+        right_name = right = None
+        names_right: set[str] = set()
+        jtype = None
+        if q.join:
+            jtype, right_name, ralias, jl, jr = q.join
+        elif q.exists:
+            ex = q.exists[0]
+            jtype = _abi.ANTI if ex.negated else _abi.SEMI
+            right_name, ralias, jl, jr = ex.table, ex.alias, ex.left, ex.right
+        if right_name is not None:
+            right = self.table(right_name)
+            names_right = {right_name.lower()} | ({ralias.lower()} if ralias else set())
+
+        def side(c: ColName) -> str:
+            """'probe' or 'build' for a column reference."""
+            if c.table is not None:
+                t = c.table.lower()
+                if t in names_left:
+                    return "probe"
+                if t in names_right:
+                    return "build"
+                raise FalconError(1, f"table or alias '{c.table}' not in the query")  # ColumnNotFound
+            in_left = _has_col(left, c.name)
+            in_right = right is not None and _has_col(right, c.name)
+            if in_left and in_right:
+                raise FalconError(INVALID_QUERY, f"column '{c.name}' is ambiguous")
+            if in_left:
+                return "probe"
+            if in_right:
+                return "build"
+            raise FalconError(1, f"column '{c.name}' not found")
+
+        where = self._predicate(q.where, side) if q.where is not None else None
+        names = [_output_name(it, i) for i, it in enumerate(q.items)]
+        has_agg = any(it.func is not None for it in q.items)
+
+        if right is not None:
+            return self._plan_join(q, left, right, right_name, jtype, jl, jr, where, side, names)
+        if q.group_by is not None or has_agg:
+            return self._plan_aggregate(q, left, where, side, names)
+        if q.order_by is not None:
+            return self._plan_sort_limit(q, left, where, names)
+        if q.limit is not None:
+            raise FalconError(UNSUPPORTED, "SQL: LIMIT without ORDER BY")
+        return self._plan_select(q, left, where, names)
+
+    def _predicate(self, e, side) -> Expr:
+        kind = e[0]
+        if kind == "and":
+            return self._predicate(e[1], side) & self._predicate(e[2], side)
+        if kind == "or":
+            return self._predicate(e[1], side) | self._predicate(e[2], side)
+        c = e[1]
+        if side(c) != "probe":  # the reference filters the FROM table at scan time
+            raise FalconError(UNSUPPORTED, f"SQL: WHERE on '{c.name}' of the joined table")
+        if kind == "isnull":
+            return col(c.name).is_not_null() if e[2] else col(c.name).is_null()
+        _, _, op, lit, lit_first = e
+        op = "!=" if op == "<>" else op
+        return lit_cmp(lit, op, c.name) if lit_first else cmp(c.name, op, lit)
+
+    def _aggs(self, q: Query, side, names, skip_plain: bool = True) -> list[AggExpr]:
+        out = []
+        for it, nm in zip(q.items, names):
+            if it.func is None:
+                continue
+            if it.arg is None:
+                out.append(AggExpr(_abi.COUNT, None, nm))
+            else:
+                out.append(AggExpr(it.func, it.arg.name, nm, side=side(it.arg)))
+        return out
+
+    def _plan_aggregate(self, q, t, where, side, names) -> RecordBatch:
+        if q.order_by is not None or q.limit is not None:
+            raise FalconError(UNSUPPORTED, "SQL: ORDER BY / LIMIT over an aggregate")
+        aggs = self._aggs(q, side, names)
+        plain = [(it, nm) for it, nm in zip(q.items, names) if it.func is None]
+        if q.group_by is None:
+            if plain:
+                raise FalconError(INVALID_QUERY, f"column '{plain[0][0].arg.name}' must appear in GROUP BY")
+            if len(aggs) == 1 and aggs[0].func == _abi.COUNT and aggs[0].input is None:
+                n = count_star(self.ctx, t, where)  # the COUNT(*) fast path
+                return RecordBatch([names[0]], ["Int64"], [np.array([n], np.int64)], [None], [_abi.COUNT])
+            return execute_agg_only(self.ctx, t, aggs, where)
+        key = q.group_by.name
+        for it, _ in plain:
+            if it.arg.name.lower() != key.lower():
+                raise FalconError(INVALID_QUERY, f"column '{it.arg.name}' must appear in GROUP BY")
+        res = GroupByExec([key], aggs).execute(self.ctx, t, where)
+        # operator order: key, then the aggregates; SELECT order may differ
+        cols, dts, valid, funcs, ai = [], [], [], [], 1
+        for it, nm in zip(q.items, names):
+            j = 0 if it.func is None else ai
+            if it.func is not None:
+                ai += 1
+            cols.append(res.columns[j])
+            dts.append(res.dtypes[j])
+            valid.append(res.validity[j])
+            funcs.append(res.agg_funcs[j] if j < len(res.agg_funcs) else None)
+        return RecordBatch(names, dts, cols, valid, funcs)
+
+    def _plan_join(self, q, left, right, right_name, jtype, jl, jr, where, side, names) -> RecordBatch:
+        if q.group_by is not None or q.order_by is not None or q.limit is not None:
+            raise FalconError(UNSUPPORTED, "SQL: GROUP BY / ORDER BY / LIMIT over a join")
+        if any(it.func is None for it in q.items):
+            raise FalconError(UNSUPPORTED, "SQL: a join's SELECT list takes aggregates only")
+        sl, sr = side(jl), side(jr)
+        if {sl, sr} != {"probe", "build"}:
+            raise FalconError(INVALID_QUERY, "SQL: the join condition must relate the two tables")
+        pk, bk = (jl.name, jr.name) if sl == "probe" else (jr.name, jl.name)
+        aggs = self._aggs(q, side, names)
+        if jtype in (_abi.SEMI, _abi.ANTI) and any(a.side == "build" for a in aggs):
+            raise FalconError(INVALID_QUERY, "SQL: EXISTS exposes no column of its subquery")
+        return execute_hash_join(self.ctx, left, right, pk, bk, jtype, aggs, filter=where,
+                                 index=self.index(right_name, bk))
+
+    def _plan_sort_limit(self, q, t, where, names) -> RecordBatch:
+        c, desc = q.order_by
+        if q.limit is None:
+            raise FalconError(UNSUPPORTED, "SQL: ORDER BY without LIMIT")
+        if len(q.items) != 1 or q.items[0].func is not None or q.items[0].arg.name.lower() != c.name.lower():
+            raise FalconError(UNSUPPORTED, "SQL: ORDER BY ... LIMIT selects the sort column only")
+        rows, vals = execute_sort_limit(self.ctx, t, c.name, q.limit, desc=desc, filter=where)
+        dt = "Float64" if vals.dtype == np.float64 else "Int64"
+        valid = None
+        if t.column_stats(c.name)["nulls"]:
+            # NULL rows follow the others for DESC and lead for ASC (the reference's placement)
+            i = np.arange(vals.size)
+            if desc:
+                nn = col(c.name).is_not_null()
+                valid = i < count_star(self.ctx, t, nn if where is None else where & nn)
+            else:
+                isn = col(c.name).is_null()
+                valid = i >= min(vals.size, count_star(self.ctx, t, isn if where is None else where & isn))
+        return RecordBatch([names[0]], [dt], [vals], [valid], [None])
+
+    def _plan_select(self, q, t, where, names) -> RecordBatch:
+        if any(it.func is not None for it in q.items):
+            raise FalconError(INVALID_QUERY, "SQL: aggregates need GROUP BY or no plain columns")
+        res = execute_select(self.ctx, t, where, [it.arg.name for it in q.items])
+        return RecordBatch(names, res.dtypes, res.columns, res.validity, [None] * len(names))
+
+
+def _has_col(t: Table, name: str) -> bool:
+    try:
+        t.column_index(name)
+        return True
+    except FalconError:
+        return False

@@ -368,9 +368,26 @@ def main():
     for _ in range(3):
         groupby_call(lib, ctx, table, pred, spec)
     host_ms = (time.perf_counter() - t0) * 1e3 / 3
+    # the same query as SQL text (Database.sql) against the operator call, both
+    # reading the result back: the difference is the SQL layer's parse and plan
+    from falcondb_amd.sql import Database
+    db = Database(ctx)
+    db.register("t", table)
+    c2_sql = "SELECT key, COUNT(*), SUM(v1), MIN(v1), MAX(v1), SUM(v2) FROM t WHERE f < 40 GROUP BY key"
+    db.sql(c2_sql)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        db.sql(c2_sql)
+    sql_ms = (time.perf_counter() - t0) * 1e3 / 3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        fa.GroupByExec(["key"], aggs).execute(ctx, table, col("f") < 40)
+    op_ms = (time.perf_counter() - t0) * 1e3 / 3
     ctx.set_tuning("keep_on_device", 1)
+    log(f"[bench] C2 as SQL: {sql_ms:.3f} ms per query, operator call {op_ms:.3f} ms (both read the result back)")
 
-    secondary = {}
+    secondary = {"c2_sql": {"statement": c2_sql, "sql_ms": sql_ms, "operator_ms": op_ms,
+                            "parse_plan_ms": sql_ms - op_ms}}
     if not args.no_join:
         del c2
         Kb = dg.keys(C3_BUILD, 40, seed_=dg.seed(3, 15))

@@ -15,6 +15,16 @@ namespace fgpu {
 
 constexpr int FILTER_UNROLL = 8;
 
+// Is any batch touched by mask words [w0, w0 + FILTER_UNROLL) live?  (A chunk
+// lies inside one batch when the batch size is a multiple of its 512 rows.)
+FGPU_DEV bool chunk_live(const DPred &p, int64_t w0, int64_t n) {
+    const int64_t r0 = w0 << 6;
+    const int64_t r1 = ((w0 + FILTER_UNROLL) << 6) < n ? ((w0 + FILTER_UNROLL) << 6) : n;
+    for (int64_t b = r0 / p.batch_rows; b <= (r1 - 1) / p.batch_rows; ++b)
+        if (p.live[b]) return true;
+    return false;
+}
+
 template <int NPC>
 __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *out) {
     __shared__ unsigned long long part[BLOCK / 64];
@@ -25,6 +35,7 @@ __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *
     const int64_t n_words = (n + 63) >> 6;
     unsigned int cnt = 0;
     for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        if (p.live && !chunk_live(p, w0, n)) continue;
         int64_t rows[FILTER_UNROLL];
         bool pass[FILTER_UNROLL];
 #pragma unroll
@@ -51,6 +62,10 @@ __global__ void k_filter_mask(DCols c, DPred p, int64_t n, uint64_t *words) {
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t n_words = (n + 63) >> 6;
     for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
+        if (p.live && !chunk_live(p, w0, n)) {  // no row of these words can pass
+            if (lane < FILTER_UNROLL && w0 + lane < n_words) words[w0 + lane] = 0;
+            continue;
+        }
         int64_t rows[FILTER_UNROLL];
         bool pass[FILTER_UNROLL];
 #pragma unroll
@@ -369,6 +384,81 @@ __global__ void k_dict_remap(uint32_t *codes, int64_t n, const int64_t *row_off,
 hipError_t launch_dict_remap(hipStream_t s, uint32_t *codes, int64_t n, const int64_t *row_off, int n_batches,
                              const uint32_t *lut, const int64_t *lut_off, unsigned long long *bad, int grid) {
     hipLaunchKernelGGL(k_dict_remap, dim3(grid), dim3(BLOCK), 0, s, codes, n, row_off, n_batches, lut, lut_off, bad);
+    return hipGetLastError();
+}
+
+// Zone map, one batch per workgroup (BatchStats::compute / column_stats,
+// src/storage/stats.rs:44-51, :143-178): bounds of the batch's valid values and
+// its valid / NaN counts.  Float bounds skip NaN, as f64::min / f64::max do.
+__global__ void k_batch_zones(const void *values, int dtype, const uint64_t *valid, int64_t n,
+                              int64_t batch_rows, BatchZone *out) {
+    __shared__ int64_t s_lo[BLOCK / 64], s_hi[BLOCK / 64];
+    __shared__ unsigned long long s_nv[BLOCK / 64], s_nn[BLOCK / 64];
+    const int64_t r0 = (int64_t)blockIdx.x * batch_rows;
+    const int64_t r1 = r0 + batch_rows < n ? r0 + batch_rows : n;
+    const bool is_f = dtype == FGPU_FLOAT64;
+    // integers: signed bounds; floats: f64_order keys, compared as unsigned
+    uint64_t lo = is_f ? ~0ull : (uint64_t)INT64_MAX, hi = is_f ? 0ull : (uint64_t)INT64_MIN;
+    unsigned long long nv = 0, nn = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        if (valid && !((valid[r >> 6] >> (r & 63)) & 1)) continue;
+        ++nv;
+        if (is_f) {
+            const uint64_t bits = ((const uint64_t *)values)[r];
+            if (((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull)) {
+                ++nn;
+                continue;
+            }
+            const uint64_t k = f64_order(bits);
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        } else {
+            const int64_t v = dtype == FGPU_INT32 ? (int64_t)((const int32_t *)values)[r] : ((const int64_t *)values)[r];
+            lo = v < (int64_t)lo ? (uint64_t)v : lo;
+            hi = v > (int64_t)hi ? (uint64_t)v : hi;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint64_t l2 = __shfl_xor(lo, d, 64), h2 = __shfl_xor(hi, d, 64);
+        if (is_f) {
+            lo = l2 < lo ? l2 : lo;
+            hi = h2 > hi ? h2 : hi;
+        } else {
+            lo = (int64_t)l2 < (int64_t)lo ? l2 : lo;
+            hi = (int64_t)h2 > (int64_t)hi ? h2 : hi;
+        }
+        nv += __shfl_xor(nv, d, 64);
+        nn += __shfl_xor(nn, d, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[wave] = (int64_t)lo;
+        s_hi[wave] = (int64_t)hi;
+        s_nv[wave] = nv;
+        s_nn[wave] = nn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BatchZone z{s_lo[0], s_hi[0], (int64_t)s_nv[0], (int64_t)s_nn[0]};
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            if (is_f) {
+                z.lo = (uint64_t)s_lo[w] < (uint64_t)z.lo ? s_lo[w] : z.lo;
+                z.hi = (uint64_t)s_hi[w] > (uint64_t)z.hi ? s_hi[w] : z.hi;
+            } else {
+                z.lo = s_lo[w] < z.lo ? s_lo[w] : z.lo;
+                z.hi = s_hi[w] > z.hi ? s_hi[w] : z.hi;
+            }
+            z.n_valid += (int64_t)s_nv[w];
+            z.n_nan += (int64_t)s_nn[w];
+        }
+        out[blockIdx.x] = z;
+    }
+}
+
+hipError_t launch_batch_zones(hipStream_t s, const void *values, int dtype, const uint64_t *valid, int64_t n,
+                              int64_t batch_rows, BatchZone *out) {
+    const int64_t nb = (n + batch_rows - 1) / batch_rows;
+    hipLaunchKernelGGL(k_batch_zones, dim3((unsigned)nb), dim3(BLOCK), 0, s, values, dtype, valid, n, batch_rows, out);
     return hipGetLastError();
 }
 

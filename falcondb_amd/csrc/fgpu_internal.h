@@ -240,6 +240,17 @@ struct DPred {
     int n_pcols;  // distinct columns the predicate reads
     int32_t pcol[MAX_PCOLS];  // their DCols slots
     DPredNode nodes[FGPU_MAX_PRED_NODES];
+    // zone-map pruning (the filter kernels): live[b] = 0 when batch b's bounds
+    // prove no row passes; nullptr = every batch is read
+    const uint8_t *live;
+    int64_t batch_rows;
+};
+
+// One batch's bounds for one column (upload-time zone map): integers as int64;
+// floats as f64_order keys over the non-NaN values; valid and NaN counts.
+struct BatchZone {
+    int64_t lo, hi;
+    int64_t n_valid, n_nan;
 };
 
 template <class T>

@@ -155,6 +155,8 @@ struct ColStatsDev {
 };
 hipError_t launch_dict_remap(hipStream_t s, uint32_t *codes, int64_t n, const int64_t *row_off, int n_batches,
                              const uint32_t *lut, const int64_t *lut_off, unsigned long long *bad, int grid);
+hipError_t launch_batch_zones(hipStream_t s, const void *values, int dtype, const uint64_t *valid, int64_t n,
+                              int64_t batch_rows, BatchZone *out);
 hipError_t launch_col_stats(hipStream_t s, const void *values, int dtype, const uint64_t *valid,
                             int64_t n, ColStatsDev *d_stats, int grid);
 

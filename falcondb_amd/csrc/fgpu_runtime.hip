@@ -86,6 +86,7 @@ struct DevCol {
     int emin = 0, emax = -1;  // float mantissa-LSB exponents (emin > emax: no finite nonzero)
     uint64_t nonfinite = 0;
     int64_t distinct = 0;
+    std::vector<BatchZone> zones;  // per batch_rows batch (numeric columns): the zone map
     size_t elem_bytes() const { return dtype == FGPU_INT64 || dtype == FGPU_FLOAT64 ? 8 : 4; }
 };
 
@@ -212,10 +213,11 @@ struct fgpu_result {
 // ─── context ────────────────────────────────────────────────────────────────
 enum BufSlot {
     B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT,
-    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_NSLOTS
+    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_LIVE, B_NSLOTS
 };
 
 struct Tuning {
+    int zone_maps = 1;             // WHERE scans skip batches whose bounds rule them out
     int gb_block = 256;
     int gb_waves_per_cu = 16;      // target resident waves per CU for streaming kernels
     int hash_lds_slots = 1024;
@@ -257,6 +259,7 @@ struct fgpu_ctx {
     fgpu_timing last{};
     Tuning tune;
     std::string plan;  // strategy of the last call (diagnostics)
+    std::vector<uint8_t> live_host;  // zone-map flags being copied to B_LIVE (calls end synchronised)
     std::mutex mu;
     // HBM budget of a serving slot: the arena may not grow past arena_limit
     // bytes (0 = unbounded).  call_req[s] is the largest size buffer s was
@@ -810,6 +813,7 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_waves_per_cu") t.join_waves_per_cu = value;
     else if (n == "filter_waves_per_cu") t.filter_waves_per_cu = value;
     else if (n == "keep_on_device") t.keep_on_device = value;
+    else if (n == "zone_maps") t.zone_maps = value;
     else if (n == "join_partition") t.join_partition = value;
     else if (n == "gb_partition") t.gb_partition = value;
     else if (n == "gb_part_min_rows") t.gb_part_min_rows = value;
@@ -1046,6 +1050,18 @@ static int table_upload_impl(fgpu_ctx *ctx, const fgpu_column_desc *cols, const 
         c.emax = h.emax;
         c.nonfinite = h.nonfinite;
         c.distinct = (int64_t)std::llround(hll_estimate(h.hll));
+        if (n_rows && c.dtype != FGPU_UTF8) {  // zone map
+            const int64_t nb = (n_rows + batch_rows - 1) / batch_rows;
+            BatchZone *d_z = nullptr;
+            e = hipMalloc(&d_z, (size_t)nb * sizeof(BatchZone));
+            if (e == hipSuccess) e = launch_batch_zones(nullptr, c.d_values, c.dtype, c.d_valid, n_rows, batch_rows, d_z);
+            if (e == hipSuccess) {
+                c.zones.resize((size_t)nb);
+                e = hipMemcpy(c.zones.data(), d_z, (size_t)nb * sizeof(BatchZone), hipMemcpyDeviceToHost);
+            }
+            if (d_z) (void)hipFree(d_z);
+            if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "zone map: %s", hipGetErrorString(e)); break; }
+        }
     }
     (void)hipFree(d_st);
     if (rc) return fail(rc);
@@ -1088,6 +1104,96 @@ int fgpu_table_float_range(const fgpu_table *t, int col, int *emin, int *emax, u
 }
 
 // ─── WHERE ──────────────────────────────────────────────────────────────────
+// Zone-map pruning (BatchStats::can_skip, src/storage/stats.rs:57-113): may a
+// row of batch b pass leaf i of the predicate?  The reference compares f64
+// images of the bounds; here each comparison class uses the evaluator's own
+// arithmetic (int64 / int32 exactly, (double)v for an integer column against a
+// float literal, doubles with NaN apart), so a batch is skipped exactly when no
+// row of it can pass.  IS [NOT] NULL prunes on the counts; dictionary
+// comparisons never prune.
+static bool zone_may(const DevCol &col, const BatchZone &z, int64_t rows, const DPredNode &nd) {
+    const int op = nd.op;
+    auto test = [&](double lo, double hi, double lit) {
+        switch (op) {
+            case FGPU_LT: return lo < lit;
+            case FGPU_LE: return lo <= lit;
+            case FGPU_GT: return hi > lit;
+            case FGPU_GE: return hi >= lit;
+            case FGPU_EQ: return lo <= lit && lit <= hi;
+            default: return !(lo == hi && lo == lit);  // NE
+        }
+    };
+    switch (nd.cls) {
+        case CMP_FALSE: return false;
+        case CMP_IS_NULL: return rows - z.n_valid > 0;
+        case CMP_IS_NOT_NULL: return z.n_valid > 0;
+        case CMP_DICT: return true;
+        case CMP_I64:
+        case CMP_I32: {
+            if (z.n_valid == 0) return false;
+            const int64_t lo = z.lo, hi = z.hi, lit = nd.lit_i;
+            switch (op) {
+                case FGPU_LT: return lo < lit;
+                case FGPU_LE: return lo <= lit;
+                case FGPU_GT: return hi > lit;
+                case FGPU_GE: return hi >= lit;
+                case FGPU_EQ: return lo <= lit && lit <= hi;
+                default: return !(lo == hi && lo == lit);
+            }
+        }
+        case CMP_I64_AS_F64:
+            if (z.n_valid == 0) return false;
+            return test((double)z.lo, (double)z.hi, nd.lit_f);
+        case CMP_F64: {
+            if (op == FGPU_NE && z.n_nan > 0) return true;  // NaN != x holds
+            if (z.n_valid - z.n_nan == 0) return false;
+            const double lo = __builtin_bit_cast(double, f64_unorder((uint64_t)z.lo));
+            const double hi = __builtin_bit_cast(double, f64_unorder((uint64_t)z.hi));
+            return test(lo, hi, nd.lit_f);
+        }
+        default: return true;
+    }
+    (void)col;
+}
+
+// Sets dp.live to the batches that may hold a passing row, when the zone maps
+// rule some out (uploaded into the context's scratch); else leaves it null.
+static int prune_batches(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, DPred &dp) {
+    dp.live = nullptr;
+    dp.batch_rows = t->batch_rows;
+    if (!c->tune.zone_maps || !p || dp.n == 0 || t->n_rows == 0) return FGPU_OK;
+    const int64_t nb = (t->n_rows + t->batch_rows - 1) / t->batch_rows;
+    std::vector<uint8_t> &live = c->live_host;
+    live.assign((size_t)nb, 0);
+    int64_t n_live = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t rows = std::min<int64_t>(t->batch_rows, t->n_rows - b * t->batch_rows);
+        bool st[FGPU_MAX_PRED_NODES];
+        int sp = 0;
+        for (int i = 0; i < dp.n; ++i) {
+            const fgpu_pred_node &nd = p->nodes[i];
+            if (nd.kind == FGPU_P_AND || nd.kind == FGPU_P_OR) {
+                const bool y = st[--sp], x = st[--sp];
+                st[sp++] = nd.kind == FGPU_P_AND ? x && y : x || y;
+                continue;
+            }
+            const DevCol &col = t->cols[nd.col];
+            st[sp++] = col.zones.empty() ? true : zone_may(col, col.zones[(size_t)b], rows, dp.nodes[i]);
+        }
+        live[(size_t)b] = st[0] ? 1 : 0;
+        n_live += live[(size_t)b];
+    }
+    char msg[96];
+    snprintf(msg, sizeof msg, " zone maps: %lld of %lld batches live", (long long)n_live, (long long)nb);
+    c->plan += msg;
+    if (n_live == nb) return FGPU_OK;
+    void *d;
+    if (int rc = buf(c, B_LIVE, (size_t)nb, &d)) return rc;
+    HIPCHK(hipMemcpyAsync(d, live.data(), (size_t)nb, hipMemcpyHostToDevice, c->stream));
+    dp.live = (const uint8_t *)d;
+    return FGPU_OK;
+}
+
 int fgpu_filter_count(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int64_t *out_count) {
     if (!c || !t || !out_count) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
@@ -1095,6 +1201,8 @@ int fgpu_filter_count(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int6
     Binder b(t);
     DPred dp;
     if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    c->plan = "filter:count";
+    if (int rc = prune_batches(c, t, p, dp)) return rc;
     unsigned long long *d_cnt;
     if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return rc;
     mark(c, 0);
@@ -1118,6 +1226,8 @@ static int filter_mask_dev(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
     Binder b(t);
     DPred dp;
     if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    c->plan = "filter:mask";
+    if (int rc = prune_batches(c, t, p, dp)) return rc;
     const int64_t nw = (t->n_rows + 63) / 64;
     if (int rc = buf(c, B_MASK, (size_t)std::max<int64_t>(nw, 1) * 8, (void **)d_words)) return rc;
     const int grid = grid_for(c, nw * 64, BLOCK, c->tune.filter_waves_per_cu);

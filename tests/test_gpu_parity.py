@@ -120,6 +120,48 @@ def test_filter_nulls_and_nan(ctx):
         assert fa.count_star(ctx, t, p) == ora.filter_count(cols, p)
 
 
+def test_zone_maps_prune_exactly(ctx):
+    # Clustered columns: a sorted timestamp, a float drifting by batch with NaN
+    # in one batch, an int32 constant per batch, NULLs in three batches.  Every
+    # predicate gives the same mask, count and selection with zone maps on and
+    # off, equal to the oracle; batches are skipped where the bounds allow.
+    br, nb = 4096, 41
+    n = (nb - 1) * br + 123
+    rng = np.random.default_rng(17)
+    bi = np.arange(n) // br
+    drift = bi.astype(np.float64) + rng.random(n) * 0.5
+    drift[5 * br: 5 * br + 300] = np.nan
+    gvalid = ~((bi >= 10) & (bi <= 12) & (rng.random(n) < 0.5))
+    cols = {"ts": np.arange(n, dtype=np.int64) * 3, "drift": drift, "k": bi.astype(np.int32),
+            "g": NullableColumn(rng.integers(-50, 50, n), gvalid)}
+    t = _table(ctx, cols, batch_rows=br)
+    off = fa.Context(0)
+    off.set_tuning("zone_maps", 0)
+    cases = [  # (predicate, live batches expected when pruned, or None for all live)
+        (col("ts") < 3 * br * 2, 2), (col("ts") >= 3 * (n - 100), 1),
+        ((col("ts") > 1000) & (col("ts") < 50_000), 5), ((col("ts") < 5000) | (col("ts") > 3 * (n - 5000)), 4),
+        (col("drift") < 2.0, 2), (col("drift") > 39.2, 2), (col("drift") != 5.25, None),
+        (col("drift") == 7.3, 1), (col("k") != 7, nb - 1), (col("k") == 3, 1), (col("k") >= 38, 3),
+        (col("g").is_null(), 3), (col("g").is_not_null(), None), (col("ts") < 1.5e5, 13),
+        (col("k") < 3.5, 0), ((col("k") == 3) | col("g").is_null(), 4),
+    ]
+    for p, expect_live in cases:
+        m_on = p.eval_mask_bits(ctx, t)
+        plan = ctx.last_plan()
+        m_off = p.eval_mask_bits(off, t)
+        assert np.array_equal(m_on, m_off), (p.nodes, plan)
+        assert np.array_equal(m_on, ora.filter_mask(cols, p, br)), p.nodes
+        c_on = fa.count_star(ctx, t, p)
+        assert c_on == fa.count_star(off, t, p) == ora.filter_count(cols, p, br)
+        live = nb if expect_live is None else expect_live
+        assert f"zone maps: {live} of {nb} batches live" in plan, (plan, p.nodes)
+        g_on = fa.execute_select(ctx, t, p, ["ts", "g"])
+        g_off = fa.execute_select(off, t, p, ["ts", "g"])
+        assert_batches_equal(g_on, g_off, None)
+    off.close()
+    t.free()
+
+
 def test_filter_select_materialises_in_row_order(ctx):
     n = 200_003
     t1 = dg.c1_table(n)

@@ -135,6 +135,31 @@ def bench_batched(ctx, lib, steps):
     return out
 
 
+def bench_zone_maps(ctx, rows, reps=10):
+    """COUNT(*) WHERE ts < X over a sorted int64 column (a clustered column such
+    as a load timestamp; 2 % of its batches hold qualifying rows), with the zone
+    maps on and off: pruned batches are not read."""
+    import falcondb_amd as fa
+    from falcondb_amd.operators import col
+    t = fa.Table(ctx, {"ts": np.arange(rows, dtype=np.int64)})
+    pred = col("ts") < rows // 50
+    out = {}
+    for name, on in (("on", 1), ("off", 0)):
+        ctx.set_tuning("zone_maps", on)
+        n = fa.count_star(ctx, t, pred)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fa.count_star(ctx, t, pred)
+        out[f"{name}_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+        out["count"] = int(n)
+    ctx.set_tuning("zone_maps", 1)
+    out["plan"] = ctx.last_plan()
+    t.free()
+    log(f"[bench] zone maps: COUNT(*) WHERE ts < {rows // 50} over {rows} sorted rows: "
+        f"{out['on_ms']:.3f} ms pruned, {out['off_ms']:.3f} ms full scan")
+    return out
+
+
 def _serve_stream(srv, reqs, kinds, qps, seed_):
     """Serves `reqs` on `srv` with Poisson arrivals at `qps` (seeded); returns
     latency percentiles and throughput.  A waiter thread collects results while
@@ -430,6 +455,9 @@ def main():
         pt.free()
         bt.free()
         del pc, bc
+
+    if not args.no_join:
+        secondary["zone_maps"] = bench_zone_maps(ctx, args.rows)
 
     if not args.no_batch:
         secondary["c4_batched"] = bench_batched(ctx, lib, max(3, args.steps // 4))

@@ -575,15 +575,78 @@ FGPU_DEV EmitOps emit_ops(const JoinPartArgs &a) {
     }
     return e;
 }
-// One output row, fast form: build values are inlined in the slots.
-FGPU_DEV void emit_fast(const EmitOps &e, JoinAcc &j, int64_t f1, int64_t f2, bool present, int64_t pay) {
-    ++j.rows;
+// Folds a group of R records per lane, fast form.  The op loop is outside, so
+// each (wave-uniform) op kind is dispatched once per group rather than once per
+// record, and records enter through selects: one that is not an output row
+// adds the identity (0, or +0.0 to the float sum: the high word is never -0.0,
+// so hi + 0.0 == hi and the error term is +0.0).  Every accumulator sees its
+// records in the order q = 0 .. R-1.
+template <int TYPE, int R>
+FGPU_DEV void fold_group(const EmitOps &e, JoinAcc &j, const bool (&live)[R], const bool (&hit)[R],
+                         const int64_t (&f1)[R], const int64_t (&f2)[R], const int64_t (&pay)[R]) {
+    bool row[R], has_pay[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        row[q] = live[q] && (TYPE == FGPU_INNER ? hit[q] : TYPE == FGPU_LEFT ? true : (TYPE == FGPU_SEMI) == hit[q]);
+        has_pay[q] = row[q] && (TYPE == FGPU_INNER || (TYPE == FGPU_LEFT && hit[q]));
+        j.rows += row[q] ? 1 : 0;
+    }
 #pragma unroll
     for (int i = 0; i < JOIN_MAX_OPS; ++i) {
         const int src = e.src[i];
-        if (i >= e.n || src == 0 || (src == 3 && !present)) continue;
-        const int64_t v = src == 1 ? f1 : src == 2 ? f2 : pay;
-        reg_add(e.kind[i], v, j.acc[i], j.hi[i], j.lo[i]);
+        if (i >= e.n || src == 0) continue;
+        bool on[R];
+        int64_t v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            on[q] = src == 3 ? has_pay[q] : row[q];
+            v[q] = src == 1 ? f1[q] : src == 2 ? f2[q] : pay[q];
+        }
+        int64_t &acc = j.acc[i];
+        switch (e.kind[i]) {
+            case OP_CNT:
+#pragma unroll
+                for (int q = 0; q < R; ++q) acc += on[q] ? 1 : 0;
+                break;
+            case OP_ISUM:
+#pragma unroll
+                for (int q = 0; q < R; ++q) acc = (int64_t)((uint64_t)acc + (on[q] ? (uint64_t)v[q] : 0ull));
+                break;
+            case OP_IMIN:
+#pragma unroll
+                for (int q = 0; q < R; ++q) acc = on[q] && v[q] < acc ? v[q] : acc;
+                break;
+            case OP_IMAX:
+#pragma unroll
+                for (int q = 0; q < R; ++q) acc = on[q] && v[q] > acc ? v[q] : acc;
+                break;
+            case OP_FSUM:
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const uint64_t bits = (uint64_t)v[q];
+                    const bool special = ((bits >> 52) & 0x7ff) == 0x7ff;
+                    const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
+                    acc |= on[q] && special ? (int64_t)fl : 0;
+                    const double x = on[q] && !special ? __longlong_as_double(v[q]) : 0.0;
+                    double sum, err;
+                    two_sum(j.hi[i], x, sum, err);
+                    j.hi[i] = sum;
+                    j.lo[i] += err;
+                }
+                break;
+            case OP_FMIN:
+            case OP_FMAX:
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const uint64_t bits = (uint64_t)v[q];
+                    const bool nan = ((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull);
+                    const uint64_t o = f64_order(bits);
+                    const bool take = on[q] && !nan &&
+                                      (e.kind[i] == OP_FMIN ? o < (uint64_t)acc : o > (uint64_t)acc);
+                    acc = take ? (int64_t)o : acc;
+                }
+                break;
+        }
     }
 }
 
@@ -656,8 +719,8 @@ FGPU_DEV void side_records(const JoinPartArgs &a, JoinAcc &j, const int64_t *rec
 // FAST (unique build keys, the build value inlined in the slot or not
 // needed): each lane first resolves all R of its records -- the first probe
 // of every record is issued at once, then only records whose chain goes on
-// take further steps -- and folds each record once afterwards, so the fold
-// code runs R times per group instead of once per probe step.
+// take further steps -- and folds the group afterwards, one dispatch per
+// aggregate op for all R records (fold_group).
 template <int TYPE, bool FAST, int RW>
 __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     extern __shared__ uint64_t jp_lds[];
@@ -805,17 +868,12 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
                     }
                 }
             }
+            // the build values of all R records are read before any is folded,
+            // so their LDS latencies overlap (si is a slot of the table, hit or not)
+            int64_t pay[R];
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-                if (!live[q]) continue;
-                if (TYPE == FGPU_INNER) {
-                    if (hit[q]) emit_fast(e, j, f1[q], f2[q], true, vals ? tv[si[q]] : 0);
-                } else if (TYPE == FGPU_LEFT) {
-                    emit_fast(e, j, f1[q], f2[q], hit[q], hit[q] && vals ? tv[si[q]] : 0);
-                } else if ((TYPE == FGPU_SEMI) == hit[q]) {
-                    emit_fast(e, j, f1[q], f2[q], false, 0);
-                }
-            }
+            for (int q = 0; q < R; ++q) pay[q] = (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) && vals ? tv[si[q]] : 0;
+            fold_group<TYPE, R>(e, j, live, hit, f1, f2, pay);
         } else {
 #pragma unroll
             for (int q = 0; q < R; ++q) {

@@ -836,6 +836,7 @@ hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t 
         default: e = gbp_agg_go<6>(s, a, block, lds); break;
     }
     if (e != hipSuccess) return e;
+    if (!a.null_group) return hipGetLastError();  // no row can be in the NULL group
     hipLaunchKernelGGL(k_gbp_null, dim3(1), dim3(256), 0, s, a);
     return hipGetLastError();
 }

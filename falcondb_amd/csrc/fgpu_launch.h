@@ -96,6 +96,7 @@ struct GbPartArgs {
     unsigned int *cell_cnt;   // G * (P + 2) * 2: row records, state records
     uint64_t *nullbuf;        // G * S
     unsigned long long *flags;  // [0] cell full, [1] pass-2 table full, [2] output full
+    int null_group;           // the key column holds NULL or INT64_MIN rows: merge the NULL group
     int T2;                   // pass-2 LDS table slots (power of two)
     DOuts outs;
     int64_t out_cap;

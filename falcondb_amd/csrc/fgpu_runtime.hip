@@ -260,6 +260,7 @@ struct fgpu_ctx {
     Tuning tune;
     std::string plan;  // strategy of the last call (diagnostics)
     std::vector<uint8_t> live_host;  // zone-map flags being copied to B_LIVE (calls end synchronised)
+    uint64_t *mail = nullptr;        // MAIL_BYTES of pinned host memory for small read-backs
     std::mutex mu;
     // HBM budget of a serving slot: the arena may not grow past arena_limit
     // bytes (0 = unbounded).  call_req[s] is the largest size buffer s was
@@ -269,6 +270,21 @@ struct fgpu_ctx {
     size_t call_req[B_NSLOTS] = {};
     bool limit_hit = false;
 };
+
+// Small device-to-host reads (flags, counts) go through the context's pinned
+// mailbox: a copy into pageable memory is staged by the runtime, one into
+// pinned memory is a direct DMA.  mail_copy issues the copy on the context's
+// stream; mail_take waits for it and hands the bytes over.
+constexpr size_t MAIL_BYTES = 512;
+static hipError_t mail_copy(fgpu_ctx *c, const void *src, size_t bytes) {
+    if (bytes > MAIL_BYTES) return hipErrorInvalidValue;
+    return hipMemcpyAsync(c->mail, src, bytes, hipMemcpyDeviceToHost, c->stream);
+}
+static hipError_t mail_take(fgpu_ctx *c, void *dst, size_t bytes) {
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) memcpy(dst, c->mail, bytes);
+    return e;
+}
 
 namespace {
 
@@ -615,8 +631,8 @@ int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key,
             int64_t known = -1) {
     unsigned long long n = (unsigned long long)known;
     if (known < 0) {
-        HIPCHK(hipMemcpyAsync(&n, d_count, sizeof n, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(mail_copy(c, d_count, sizeof n));
+        HIPCHK(mail_take(c, &n, sizeof n));
     }
     if ((int64_t)n > n_slots) return set_err(FGPU_E_DEVICE, "emit counter %llu exceeds %lld slots", n, (long long)n_slots);
     auto *r = new fgpu_result();
@@ -747,6 +763,11 @@ int fgpu_open(int device, fgpu_ctx **out) {
         delete c;
         return set_err(FGPU_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    if (hipHostMalloc((void **)&c->mail, MAIL_BYTES, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return set_err(FGPU_E_OUT_OF_MEMORY, "pinned host memory for the context");
+    }
     for (auto &ev : c->ev) (void)hipEventCreate(&ev);
     Tuning &t = c->tune;
     t.gb_block = env_int("FGPU_GB_BLOCK", t.gb_block);
@@ -774,6 +795,7 @@ void fgpu_close(fgpu_ctx *c) {
         if (b.p) (void)hipFree(b.p);
     for (auto &ev : c->ev) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(c->stream);
+    if (c->mail) (void)hipHostFree(c->mail);
     delete c;
 }
 
@@ -1212,9 +1234,9 @@ int fgpu_filter_count(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int6
     if (t->n_rows) HIPCHK(launch_filter_count(c->stream, b.dc, dp, t->n_rows, d_cnt, grid));
     mark(c, 2);
     unsigned long long n = 0;
-    HIPCHK(hipMemcpyAsync(&n, d_cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(mail_copy(c, d_cnt, 8));
     mark(c, 3);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(mail_take(c, &n, 8));
     int64_t bytes = 0;
     for (int s = 0; s < b.dc.n; ++s) bytes += t->n_rows * (b.dc.dtype[s] == FGPU_INT64 || b.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
     finish_timing(c, bytes, false);
@@ -1385,6 +1407,8 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     unsigned long long *d_flags;
     if ((rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_flags))) return rc;
     A.flags = d_flags;
+    // rows with a NULL key, or the INT64_MIN key that shares the NULL group
+    A.null_group = key->null_count > 0 || key->imin == INT64_MIN ? 1 : 0;
     OutBufs ob;
     if ((rc = out_bufs(c, out_cap, L.outs.n, ob))) return rc;
     A.out_key = ob.key;
@@ -1393,13 +1417,13 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
         A.out_val[i] = ob.val[i];
         A.out_valid[i] = ob.valid[i];
     }
-    A.out_count = ob.count;
+    A.out_count = d_flags + 3;  // zeroed and read back with the flags
     char msg[256];
     const int fw = gbp_fast_width(A);
     snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d RW=%d tiles=%s", (long long)P, H, T2, A.G,
              A.RW, fw == 4 ? "fast4" : fw == 8 ? "fast8" : "generic");
     c->plan = msg;
-    unsigned long long h[3] = {0, 0, 0};
+    unsigned long long h[4] = {0, 0, 0, 0};  // the three flags, then the output row count
     mark(c, 0);
     // A cell that overflows (a heavy key some workgroup could not cache) is
     // retried once with cells sized from the counts the first attempt kept.
@@ -1411,14 +1435,13 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
         if ((rc = buf(c, B_CELLS, (size_t)A.G * P * A.Cw * A.RW * 8, &p))) return rc;
         A.cells = (int64_t *)p;
         HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_flags, 4, 0));
-        HIPCHK(launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0));
         mark(c, 1);
         HIPCHK(launch_gbp_scan(c->stream, A, tu.gbp_scan_block, lds1(H)));
         mark(c, 2);
         HIPCHK(launch_gbp_agg(c->stream, A, tu.gbp_agg_block, lds2(T2)));  // returns at once after a cell overflow
         mark(c, 3);
-        HIPCHK(hipMemcpyAsync(h, d_flags, 24, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(mail_copy(c, d_flags, sizeof h));
+        HIPCHK(mail_take(c, h, sizeof h));
         snprintf(msg, sizeof msg, " Cw=%d", A.Cw);
         c->plan += msg;
         const bool diag = env_int("FGPU_GBP_DIAG", 0) != 0;
@@ -1451,7 +1474,7 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
         return -1;
     }
     if ((rc = collect(c, L, out_cap, key, key->dtype, key->name, ob.key, ob.key_valid, ob.val, ob.valid,
-                      ob.count, out)))
+                      ob.count, out, (int64_t)h[3])))
         return rc;
     int64_t bytes = 0;
     for (int s2 = 0; s2 < b.dc.n; ++s2)
@@ -1588,8 +1611,8 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
         mark(c, 2);
         unsigned long long h_cnt[2] = {0, 0};
         if (mode == GB_HASH) {
-            HIPCHK(hipMemcpyAsync(h_cnt, d_cnt, 16, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(mail_copy(c, d_cnt, 16));
+            HIPCHK(mail_take(c, h_cnt, 16));
             if (h_cnt[1]) {  // table overflowed: grow and rerun
                 cap *= 4;
                 continue;
@@ -1786,8 +1809,8 @@ static int join_build_sub_tables(fgpu_ctx *c, JoinPartArgs &jp, const JoinBuildA
         HIPCHK(launch_join_part_build(c->stream, jp, ba, grid_for(c, ba.n, BLOCK, 16)));
     }
     unsigned long long h[4] = {0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(h, jp.flags, 32, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(mail_copy(c, jp.flags, 32));
+    HIPCHK(mail_take(c, h, 32));
     if (h[1]) return FGPU_OK;  // an insert found no slot
     st.n_min_key_rows = (int64_t)h[2];
     st.unique_keys = !h[3] && h[2] <= 1;
@@ -1950,8 +1973,8 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     HIPCHK(launch_join_part_scatter(c->stream, jp, 0));
     HIPCHK(launch_join_part_probe(c->stream, jp, 0));
     mark(c, 2);
-    HIPCHK(hipMemcpyAsync(h, d_flags, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(mail_copy(c, d_flags, 8));
+    HIPCHK(mail_take(c, h, 8));
     if (!h[0]) return FGPU_OK;
     // A cell overflowed: the probe keys are skewed (a hot key fills its
     // partition).  Partition exactly instead: count the records of every
@@ -2887,9 +2910,9 @@ int filter_count_shared(fgpu_ctx *c, const fgpu_table *t, int col, const fgpu_pr
     if (t->n_rows) HIPCHK(launch_filter_count_shared(c->stream, a, grid));
     mark(c, 2);
     unsigned long long h[SHARED_MAX_Q];
-    HIPCHK(hipMemcpyAsync(h, d_cnt, (size_t)q * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(mail_copy(c, d_cnt, (size_t)q * 8));
     mark(c, 3);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(mail_take(c, h, (size_t)q * 8));
     finish_timing(c, t->n_rows * (int64_t)dc.elem_bytes(), false);
     for (int i = 0; i < q; ++i) counts[i] = (int64_t)h[i];
     c->plan = "count:shared q=" + std::to_string(q);

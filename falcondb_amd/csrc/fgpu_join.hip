@@ -562,10 +562,15 @@ struct EmitOps {
     int n;
     int kind[JOIN_MAX_OPS];
     int src[JOIN_MAX_OPS];
+    int special;  // bit i: op i's column holds Inf / NaN (has_flags), so values are classified
 };
 FGPU_DEV EmitOps emit_ops(const JoinPartArgs &a) {
     EmitOps e;
     e.n = __builtin_amdgcn_readfirstlane(a.pa.n_ops);
+    int special = 0;
+#pragma unroll
+    for (int i = 0; i < JOIN_MAX_OPS; ++i) special |= (i < a.pa.n_ops && a.pa.ops[i].has_flags ? 1 : 0) << i;
+    e.special = __builtin_amdgcn_readfirstlane(special);
 #pragma unroll
     for (int i = 0; i < JOIN_MAX_OPS; ++i) {
         const JoinOp o = a.pa.ops[i];
@@ -621,17 +626,28 @@ FGPU_DEV void fold_group(const EmitOps &e, JoinAcc &j, const bool (&live)[R], co
                 for (int q = 0; q < R; ++q) acc = on[q] && v[q] > acc ? v[q] : acc;
                 break;
             case OP_FSUM:
+                if ((e.special >> i) & 1) {
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const uint64_t bits = (uint64_t)v[q];
-                    const bool special = ((bits >> 52) & 0x7ff) == 0x7ff;
-                    const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
-                    acc |= on[q] && special ? (int64_t)fl : 0;
-                    const double x = on[q] && !special ? __longlong_as_double(v[q]) : 0.0;
-                    double sum, err;
-                    two_sum(j.hi[i], x, sum, err);
-                    j.hi[i] = sum;
-                    j.lo[i] += err;
+                    for (int q = 0; q < R; ++q) {
+                        const uint64_t bits = (uint64_t)v[q];
+                        const bool special = ((bits >> 52) & 0x7ff) == 0x7ff;
+                        const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
+                        acc |= on[q] && special ? (int64_t)fl : 0;
+                        const double x = on[q] && !special ? __longlong_as_double(v[q]) : 0.0;
+                        double sum, err;
+                        two_sum(j.hi[i], x, sum, err);
+                        j.hi[i] = sum;
+                        j.lo[i] += err;
+                    }
+                } else {  // the column has no Inf / NaN: every value is finite
+#pragma unroll
+                    for (int q = 0; q < R; ++q) {
+                        const double x = on[q] ? __longlong_as_double(v[q]) : 0.0;
+                        double sum, err;
+                        two_sum(j.hi[i], x, sum, err);
+                        j.hi[i] = sum;
+                        j.lo[i] += err;
+                    }
                 }
                 break;
             case OP_FMIN:

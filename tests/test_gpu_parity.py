@@ -44,7 +44,7 @@ def assert_batches_equal(gpu: fa.RecordBatch, exp: fa.RecordBatch, key: str | No
             continue
         if gpu.dtypes[i] == "Float64":
             ge, ee = g[gv], e[ev].astype(np.float64)
-            assert np.array_equal(ge.view(np.int64), ee.view(np.int64)) or np.array_equal(ge, ee), \
+            assert np.array_equal(ge.view(np.int64), ee.view(np.int64)) or np.array_equal(ge, ee, equal_nan=True), \
                 f"{name}: float results differ: max |d| {np.max(np.abs(ge - ee)) if ge.size else 0}"
             if ref is not None and ref.dtypes[i] == "Float64":
                 r = ref.columns[i][ev]
@@ -244,6 +244,45 @@ def test_groupby_partitioned_tile_paths(ctx_gbp, pred, tiles):
             r.columns[0] = np.where(r.validity[0], r.columns[0], imin)
             r.validity[0] = None
     assert_batches_equal(g, o.exact, "key", o.reference)
+
+
+def _with_specials(x, rng, pinf=0, ninf=0, nan=0):
+    """x with `pinf` +inf, `ninf` -inf and `nan` NaN values at distinct random rows."""
+    x = x.copy()
+    rows = rng.choice(x.size, pinf + ninf + nan, replace=False)
+    x[rows[:pinf]] = np.inf
+    x[rows[pinf:pinf + ninf]] = -np.inf
+    x[rows[pinf + ninf:]] = np.nan
+    return x
+
+
+@pytest.mark.parametrize("path", ["hash", "partitioned"])
+def test_groupby_float_sums_with_inf_and_nan(ctx, ctx_gbp, path):
+    # a group whose summands hold +inf sums to +inf, one with both infinities or
+    # a NaN to NaN (IEEE addition, in any order); the other groups keep their
+    # exact sums
+    c = ctx if path == "hash" else ctx_gbp
+    K = dg.keys(40_000, 40)
+    cols = dg.c2_table(300_000, K)
+    v2 = cols["v2"].copy()
+    key = cols["key"]
+    groups, counts = np.unique(key, return_counts=True)
+    g_pinf, g_both, g_nan = groups[counts >= 2][:3]
+    v2[np.flatnonzero(key == g_pinf)[:2]] = np.inf
+    both = np.flatnonzero(key == g_both)
+    v2[both[0]], v2[both[-1]] = np.inf, -np.inf
+    v2[np.flatnonzero(key == g_nan)[0]] = np.nan
+    cols["v2"] = v2
+    t = _table(c, cols)
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "v2"), AggExpr(SUM, "v1")]
+    g = fa.GroupByExec(["key"], aggs).execute(c, t, None)
+    if path == "partitioned":
+        assert c.last_plan().startswith("groupby:partitioned"), c.last_plan()
+    o = ora.groupby(cols, None, "key", aggs)
+    assert_batches_equal(g, o.exact, "key")
+    res = dict(zip(g.columns[0].tolist(), g.columns[2].tolist()))
+    assert res[int(g_pinf)] == np.inf and np.isnan(res[int(g_both)]) and np.isnan(res[int(g_nan)])
+    assert sum(np.isfinite(x) for x in res.values()) == len(res) - 3
 
 
 @pytest.mark.parametrize("card", [8, 1000, 70_000, 1_000_000])
@@ -473,6 +512,33 @@ def test_join_with_probe_predicate(jctx):
         g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs, filter=p)
         o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
         assert_batches_equal(g, o.exact, None)
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT, SEMI, ANTI])
+def test_join_float_sums_with_inf_and_nan(jctx, jt):
+    # non-finite summands on both sides: +inf alone gives +inf, both infinities
+    # or a NaN give NaN (IEEE addition, in any order); columns without them
+    # keep their exact sums
+    ctx = jctx
+    rng = np.random.default_rng(41)
+    pcols, bcols = _c3(20_000, 400_000)
+    bcols["b_pinf"] = _with_specials(bcols["bpay"], rng, pinf=3)
+    bcols["b_both"] = _with_specials(bcols["bpay"], rng, pinf=2, ninf=2)
+    bcols["b_nan"] = _with_specials(bcols["bpay"], rng, nan=1)
+    pcols["pf"] = _with_specials(rng.normal(size=pcols["pk"].size), rng, ninf=40)
+    pt, bt = _table(ctx, pcols), _table(ctx, bcols)
+    cases = [("pf", [AggExpr(SUM, "pf"), AggExpr(COUNT)])]
+    if jt in (INNER, LEFT):
+        cases += [(c, [AggExpr(SUM, c, side="build"), AggExpr(COUNT)]) for c in ("bpay", "b_pinf", "b_both", "b_nan")]
+    for name, aggs in cases:
+        g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs)
+        o = ora.join(pcols, bcols, "pk", "bk", jt, aggs)
+        assert_batches_equal(g, o.exact, None)
+        s = g.columns[0][0]
+        if jt == INNER:  # every build row has ~18 probe matches, so each special is summed
+            expect = {"bpay": np.isfinite, "b_pinf": lambda x: x == np.inf, "b_both": np.isnan,
+                      "b_nan": np.isnan, "pf": lambda x: x == -np.inf}[name]
+            assert expect(s), (name, s)
 
 
 def test_join_skewed_probe_keys_exact_partitions(ctx_part):

@@ -929,17 +929,31 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
 }
 
 static int scatter_npc(const JoinPartArgs &a) { return a.pa.pred.n == 0 ? 0 : a.pa.pred.n_pcols; }
-// Rows per thread per tile: 8 when a record is one word (or nothing is staged)
-// and the predicate reads at most one column -- the double-buffered loads then
-// fit the registers -- else 4.
-int join_scatter_rows(const JoinPartArgs &a, int mode) {
-    return (a.RW == 1 || mode == JS_COUNT) && scatter_npc(a) <= 1 ? 8 : 4;
-}
-size_t join_scatter_lds(const JoinPartArgs &a, int mode) {
+// LDS of pass 1 with `rows` rows per thread: the staged tile (records and
+// their destinations) and three per-partition counters.
+static size_t scatter_lds_rows(const JoinPartArgs &a, int mode, int rows) {
     const size_t P2 = ((size_t)1 << a.log_p) + 2;
-    const size_t tile = (size_t)JS_TPB * join_scatter_rows(a, mode);
+    const size_t tile = (size_t)JS_TPB * rows;
     const size_t staged = mode == JS_COUNT ? 0 : tile * a.RW * 8 + tile * 4;
     return staged + 3 * P2 * 4 + 16 * 4;
+}
+// Rows per thread per tile.  Counting: 8 when the predicate reads at most one
+// column (the double-buffered loads then fit the registers), else 4.  Staging:
+// the largest tile whose staging fits in LDS -- a larger tile gives each
+// partition longer runs of consecutive records, so more stores leave in whole
+// cache lines -- with 12 or 8 rows for one-word records and 6 or 4 for
+// two-word records when no predicate column is read, else 8 / 4 as counting.
+int join_scatter_rows(const JoinPartArgs &a, int mode) {
+    const int npc = scatter_npc(a);
+    if (mode == JS_COUNT) return npc <= 1 ? 8 : 4;
+    if (npc == 0 && a.RW <= 2) {
+        const int big = a.RW == 1 ? 12 : 6;
+        if (scatter_lds_rows(a, mode, big) <= JOIN_LDS_MAX) return big;
+    }
+    return a.RW == 1 && npc <= 1 ? 8 : 4;
+}
+size_t join_scatter_lds(const JoinPartArgs &a, int mode) {
+    return scatter_lds_rows(a, mode, join_scatter_rows(a, mode));
 }
 size_t join_probe_lds(const JoinPartArgs &a) {
     return (size_t)a.T * (a.need_vals ? 16 : 8) + 64 * 8 + (size_t)(a.G > 16 ? a.G : 16) * 4;
@@ -959,6 +973,10 @@ static void scatter_fns(std::vector<const void *> &v) {
     v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>);
     v.push_back((const void *)k_jp_scatter<NPC, 2, 4, MODE>);
     v.push_back((const void *)k_jp_scatter<NPC, 3, 4, MODE>);
+    if constexpr (NPC == 0) {
+        v.push_back((const void *)k_jp_scatter<0, 1, 12, MODE>);
+        v.push_back((const void *)k_jp_scatter<0, 2, 6, MODE>);
+    }
 }
 static hipError_t join_allow_lds_once() {
     static std::once_flag once;
@@ -994,9 +1012,20 @@ static hipError_t join_allow_lds_once() {
 template <int NPC, int MODE>
 static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, size_t lds) {
     const dim3 g((unsigned)a.G), b(JS_TPB);
-    if (MODE == JS_COUNT) {
+    if constexpr (MODE == JS_COUNT) {
         hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, JS_COUNT>), g, b, lds, s, a);
         return;
+    }
+    const int rows = join_scatter_rows(a, MODE);
+    if constexpr (NPC == 0 && MODE != JS_COUNT) {
+        if (a.RW == 1 && rows == 12) {
+            hipLaunchKernelGGL((k_jp_scatter<0, 1, 12, MODE>), g, b, lds, s, a);
+            return;
+        }
+        if (a.RW == 2 && rows == 6) {
+            hipLaunchKernelGGL((k_jp_scatter<0, 2, 6, MODE>), g, b, lds, s, a);
+            return;
+        }
     }
     if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>), g, b, lds, s, a);
     else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 4, MODE>), g, b, lds, s, a);

@@ -551,15 +551,16 @@ def test_join_skewed_probe_keys_exact_partitions(ctx_part):
     bcols = dg.c3_build(K)
     pt, bt = _table(ctx_part, pcols), _table(ctx_part, bcols)
     idx = fa.JoinIndex(ctx_part, bt, "bk")
-    p = col("f") < 60
-    for jt in (INNER, LEFT, SEMI, ANTI):
-        aggs = [AggExpr(COUNT), AggExpr(SUM, "ppay"), AggExpr(MAX, "ppay")]
-        if jt in (INNER, LEFT):
-            aggs += [AggExpr(SUM, "bpay", side="build")]
-        g = fa.execute_hash_join(ctx_part, pt, bt, "pk", "bk", jt, aggs, filter=p, index=idx)
-        assert "exact partitions" in ctx_part.last_plan(), ctx_part.last_plan()
-        o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
-        assert_batches_equal(g, o.exact, None)
+    # with a predicate column and without one (the larger staging tiles)
+    for p in (col("f") < 60, None):
+        for jt in (INNER, LEFT, SEMI, ANTI):
+            aggs = [AggExpr(COUNT), AggExpr(SUM, "ppay"), AggExpr(MAX, "ppay")]
+            if jt in (INNER, LEFT):
+                aggs += [AggExpr(SUM, "bpay", side="build")]
+            g = fa.execute_hash_join(ctx_part, pt, bt, "pk", "bk", jt, aggs, filter=p, index=idx)
+            assert "exact partitions" in ctx_part.last_plan(), ctx_part.last_plan()
+            o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
+            assert_batches_equal(g, o.exact, None)
     idx.free()
 
 

@@ -1862,8 +1862,12 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     const DevCol &bkc = build->cols[spec->build_key];
     const int64_t nb = build->n_rows;
     // P partitions, each with a sub-table that pass 2 holds in LDS: at most
-    // tmax slots, at load <= 0.5 on average (checked <= 0.75 after counting).
-    const int64_t tmax = std::min<int64_t>(8192, next_pow2(((int64_t)tu.lds_limit - 1024) / 16 + 1) / 2);
+    // tmax slots (8,192 of hash + build value, 16,384 of hash alone, so the
+    // hash-only form needs half the partitions and pass 1 writes longer runs),
+    // at load <= 0.5 on average (checked <= 0.75 after counting).
+    const int64_t slot_bytes = bcols_used.empty() ? 8 : 16;
+    const int64_t tmax = std::min<int64_t>(131072 / slot_bytes,
+                                           next_pow2(((int64_t)tu.lds_limit - 1024) / slot_bytes + 1) / 2);
     const int64_t spk = std::max(2, tu.join_slots_per_key);  // slots per build key: load <= 1 / spk
     int64_t P = std::max<int64_t>(8, next_pow2((spk * nb + tmax - 1) / tmax));
     if (tu.join_partitions) P = std::max<int64_t>(P, next_pow2(tu.join_partitions));

@@ -941,12 +941,13 @@ static size_t scatter_lds_rows(const JoinPartArgs &a, int mode, int rows) {
 // column (the double-buffered loads then fit the registers), else 4.  Staging:
 // the largest tile whose staging fits in LDS -- a larger tile gives each
 // partition longer runs of consecutive records, so more stores leave in whole
-// cache lines -- with 12 or 8 rows for one-word records and 6 or 4 for
-// two-word records when no predicate column is read, else 8 / 4 as counting.
+// cache lines -- with 12 or 8 rows for one-word records when no predicate
+// column is read and 6 or 4 for two-word records when at most one is (the
+// registers hold no more), else 8 / 4 as counting.
 int join_scatter_rows(const JoinPartArgs &a, int mode) {
     const int npc = scatter_npc(a);
     if (mode == JS_COUNT) return npc <= 1 ? 8 : 4;
-    if (npc == 0 && a.RW <= 2) {
+    if ((npc == 0 && a.RW == 1) || (npc <= 1 && a.RW == 2)) {
         const int big = a.RW == 1 ? 12 : 6;
         if (scatter_lds_rows(a, mode, big) <= JOIN_LDS_MAX) return big;
     }
@@ -973,10 +974,8 @@ static void scatter_fns(std::vector<const void *> &v) {
     v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>);
     v.push_back((const void *)k_jp_scatter<NPC, 2, 4, MODE>);
     v.push_back((const void *)k_jp_scatter<NPC, 3, 4, MODE>);
-    if constexpr (NPC == 0) {
-        v.push_back((const void *)k_jp_scatter<0, 1, 12, MODE>);
-        v.push_back((const void *)k_jp_scatter<0, 2, 6, MODE>);
-    }
+    if constexpr (NPC == 0) v.push_back((const void *)k_jp_scatter<0, 1, 12, MODE>);
+    if constexpr (NPC <= 1) v.push_back((const void *)k_jp_scatter<NPC, 2, 6, MODE>);
 }
 static hipError_t join_allow_lds_once() {
     static std::once_flag once;
@@ -1022,8 +1021,10 @@ static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, size_t lds)
             hipLaunchKernelGGL((k_jp_scatter<0, 1, 12, MODE>), g, b, lds, s, a);
             return;
         }
+    }
+    if constexpr (NPC <= 1 && MODE != JS_COUNT) {
         if (a.RW == 2 && rows == 6) {
-            hipLaunchKernelGGL((k_jp_scatter<0, 2, 6, MODE>), g, b, lds, s, a);
+            hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 6, MODE>), g, b, lds, s, a);
             return;
         }
     }

@@ -274,9 +274,12 @@ struct GbpDataFast {
     uint32_t pass;
 };
 
+// Column loads of the specialised tiles are non-temporal: every column word is
+// read once, and keeping the stream out of L2 leaves it to the cold-key
+// records, whose partly written lines otherwise leave early.
 template <class T>
 FGPU_DEV T gbp_ld(const uint8_t *base, uint32_t off) {
-    return *(const T *)(base + off);
+    return __builtin_nontemporal_load((const T *)(base + off));
 }
 
 template <int U, int FP, int NPC, int NIN>

@@ -25,6 +25,53 @@ FGPU_DEV bool chunk_live(const DPred &p, int64_t w0, int64_t n) {
     return false;
 }
 
+// The predicate's columns, resolved once per kernel (uniform bases).
+template <int NPC>
+FGPU_DEV void pred_cols(const DCols &c, const DPred &p, ColRef (&pc)[NPC]) {
+#pragma unroll
+    for (int j = 0; j < NPC; ++j) pc[j] = col_ref(c, j < p.n_pcols ? p.pcol[j] : 0);
+}
+
+// The predicate on rows tile0 + 64 * u + lane (u < FILTER_UNROLL), tile0 a
+// multiple of 64 below n.  Every cell of the step is issued -- three loads
+// from uniform bases, rows past n re-reading row n - 1 -- before any is
+// examined, so the loads of a step are all in flight together.
+template <int NPC>
+FGPU_DEV void pred_step(const DPred &p, const ColRef (&pc)[NPC], int64_t tile0, int64_t n, int lane,
+                        bool (&pass)[FILTER_UNROLL]) {
+    if (p.n == 0) {
+#pragma unroll
+        for (int u = 0; u < FILTER_UNROLL; ++u) pass[u] = tile0 + 64 * u + lane < n;
+        return;
+    }
+    const int64_t last = n - 1 - tile0;
+    RawCell3 raw[FILTER_UNROLL][NPC];
+#pragma unroll
+    for (int u = 0; u < FILTER_UNROLL; ++u) {
+        const int64_t i = 64 * u + lane;
+        const uint32_t idx = (uint32_t)(i < last ? i : last);
+#pragma unroll
+        for (int j = 0; j < NPC; ++j)
+            if (j < p.n_pcols) cell_issue3_tile(pc[j], tile0, idx, raw[u][j]);
+    }
+#pragma unroll
+    for (int u = 0; u < FILTER_UNROLL; ++u) {
+        const int64_t r = tile0 + 64 * u + lane;
+        int64_t v[NPC];
+        uint32_t ok = 0;
+#pragma unroll
+        for (int j = 0; j < NPC; ++j) {
+            v[j] = 0;
+            if (j < p.n_pcols) {
+                bool okj;
+                cell_finish3(pc[j], r, raw[u][j], v[j], okj);
+                ok |= (okj ? 1u : 0u) << j;
+            }
+        }
+        pass[u] = r < n && eval_reg<NPC>(p, v, ok);
+    }
+}
+
 template <int NPC>
 __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *out) {
     __shared__ unsigned long long part[BLOCK / 64];
@@ -34,13 +81,12 @@ __global__ void k_filter_count(DCols c, DPred p, int64_t n, unsigned long long *
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     const int64_t n_words = (n + 63) >> 6;
     unsigned int cnt = 0;
+    ColRef pc[NPC];
+    pred_cols<NPC>(c, p, pc);
     for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
         if (p.live && !chunk_live(p, w0, n)) continue;
-        int64_t rows[FILTER_UNROLL];
         bool pass[FILTER_UNROLL];
-#pragma unroll
-        for (int u = 0; u < FILTER_UNROLL; ++u) rows[u] = ((w0 + u) << 6) + lane;
-        pred_tile<FILTER_UNROLL, NPC>(p, c, rows, n, pass);
+        pred_step<NPC>(p, pc, w0 << 6, n, lane, pass);
 #pragma unroll
         for (int u = 0; u < FILTER_UNROLL; ++u) cnt += pass[u] ? 1u : 0u;
     }
@@ -61,16 +107,15 @@ __global__ void k_filter_mask(DCols c, DPred p, int64_t n, uint64_t *words) {
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t n_words = (n + 63) >> 6;
+    ColRef pc[NPC];
+    pred_cols<NPC>(c, p, pc);
     for (int64_t w0 = gw * FILTER_UNROLL; w0 < n_words; w0 += waves * FILTER_UNROLL) {
         if (p.live && !chunk_live(p, w0, n)) {  // no row of these words can pass
             if (lane < FILTER_UNROLL && w0 + lane < n_words) words[w0 + lane] = 0;
             continue;
         }
-        int64_t rows[FILTER_UNROLL];
         bool pass[FILTER_UNROLL];
-#pragma unroll
-        for (int u = 0; u < FILTER_UNROLL; ++u) rows[u] = ((w0 + u) << 6) + lane;
-        pred_tile<FILTER_UNROLL, NPC>(p, c, rows, n, pass);
+        pred_step<NPC>(p, pc, w0 << 6, n, lane, pass);
 #pragma unroll
         for (int u = 0; u < FILTER_UNROLL; ++u) {
             const uint64_t b = __ballot(pass[u]);

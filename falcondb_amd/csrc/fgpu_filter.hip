@@ -32,10 +32,24 @@ FGPU_DEV void pred_cols(const DCols &c, const DPred &p, ColRef (&pc)[NPC]) {
     for (int j = 0; j < NPC; ++j) pc[j] = col_ref(c, j < p.n_pcols ? p.pcol[j] : 0);
 }
 
+// Issues the loads of one cell of row tile0 + idx: one 64-bit or 32-bit load
+// of the value from a uniform base (non-temporal: every value is read once),
+// and the validity word when the column has NULLs.
+FGPU_DEV void step_issue(const ColRef &c, int64_t tile0, uint32_t idx, RawCell3 &o) {
+    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) {
+        const uint64_t x = __builtin_nontemporal_load((const uint64_t *)c.base + tile0 + idx);
+        o.lo = (uint32_t)x;
+        o.hi = (uint32_t)(x >> 32);
+    } else {
+        o.lo = o.hi = __builtin_nontemporal_load((const uint32_t *)c.base + tile0 + idx);
+    }
+    o.vw = c.valid ? c.valid[(tile0 >> 5) + (idx >> 5)] : 0xffffffffu;
+}
+
 // The predicate on rows tile0 + 64 * u + lane (u < FILTER_UNROLL), tile0 a
-// multiple of 64 below n.  Every cell of the step is issued -- three loads
-// from uniform bases, rows past n re-reading row n - 1 -- before any is
-// examined, so the loads of a step are all in flight together.
+// multiple of 64 below n.  Every cell of the step is issued (rows past n
+// re-read row n - 1) before any is examined, so the loads of a step are all
+// in flight together.
 template <int NPC>
 FGPU_DEV void pred_step(const DPred &p, const ColRef (&pc)[NPC], int64_t tile0, int64_t n, int lane,
                         bool (&pass)[FILTER_UNROLL]) {
@@ -52,7 +66,7 @@ FGPU_DEV void pred_step(const DPred &p, const ColRef (&pc)[NPC], int64_t tile0, 
         const uint32_t idx = (uint32_t)(i < last ? i : last);
 #pragma unroll
         for (int j = 0; j < NPC; ++j)
-            if (j < p.n_pcols) cell_issue3_tile(pc[j], tile0, idx, raw[u][j]);
+            if (j < p.n_pcols) step_issue(pc[j], tile0, idx, raw[u][j]);
     }
 #pragma unroll
     for (int u = 0; u < FILTER_UNROLL; ++u) {

@@ -358,32 +358,35 @@ FGPU_DEV void cell_finish(const ColRef &c, int64_t r, const RawCell &o, int64_t 
     ok = (((o.vword >> (r & 31)) & 1u) != 0) & ((c.dtype != FGPU_UTF8) | ((uint32_t)o.bits != NULL_CODE));
 }
 
-// Branch-free form of cell_issue / cell_finish: exactly three dword loads per
-// cell whatever the column's width and validity (a 4-byte column re-reads its
-// word as the high half, a column without NULLs re-reads the value word as
-// its validity word), so a loop that keeps several cells in flight has a
+// Two-phase cell reads with a fixed shape per column: one value load (64-bit
+// for 8-byte columns, 32-bit otherwise; hi repeats lo for 4-byte columns) and,
+// only for a column with NULLs, its validity word.  The column's type and
+// validity are uniform, so a loop that keeps several cells in flight has a
 // fixed number of loads outstanding and the compiler's waits stay exact.
 struct RawCell3 {
     uint32_t lo, hi, vw;
 };
 FGPU_DEV void cell_issue3(const ColRef &c, int64_t r, RawCell3 &o) {
-    const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;
-    const uint32_t *lo = (const uint32_t *)c.base + (wide ? 2 * r : r);
-    const uint32_t *vp = c.valid ? c.valid + (r >> 5) : lo;
-    o.lo = lo[0];
-    o.hi = *(wide ? lo + 1 : lo);
-    o.vw = *vp;
+    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) {
+        const uint64_t x = ((const uint64_t *)c.base)[r];
+        o.lo = (uint32_t)x;
+        o.hi = (uint32_t)(x >> 32);
+    } else {
+        o.lo = o.hi = ((const uint32_t *)c.base)[r];
+    }
+    o.vw = c.valid ? c.valid[r >> 5] : 0xffffffffu;
 }
 // Row tile0 + idx, tile0 a multiple of 32: the bases stay uniform (SGPR) and
 // the lane offsets 32-bit.
 FGPU_DEV void cell_issue3_tile(const ColRef &c, int64_t tile0, uint32_t idx, RawCell3 &o) {
-    const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;
-    const uint32_t *b = (const uint32_t *)c.base + (wide ? 2 * tile0 : tile0);
-    const uint32_t off = wide ? 2 * idx : idx;
-    const uint32_t *vb = c.valid ? c.valid + (tile0 >> 5) : b;
-    o.lo = b[off];
-    o.hi = b[wide ? off + 1 : off];
-    o.vw = vb[c.valid ? (idx >> 5) : off];
+    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) {
+        const uint64_t x = ((const uint64_t *)c.base + tile0)[idx];
+        o.lo = (uint32_t)x;
+        o.hi = (uint32_t)(x >> 32);
+    } else {
+        o.lo = o.hi = ((const uint32_t *)c.base + tile0)[idx];
+    }
+    o.vw = c.valid ? (c.valid + (tile0 >> 5))[idx >> 5] : 0xffffffffu;
 }
 FGPU_DEV void cell_finish3(const ColRef &c, int64_t r, const RawCell3 &o, int64_t &v, bool &ok) {
     const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;

@@ -450,28 +450,39 @@ FGPU_DEV bool eval_reg(const DPred &p, const int64_t (&v)[NPC], uint32_t ok) {
 template <int U, int NPC = MAX_PCOLS>
 FGPU_DEV void pred_tile(const DPred &p, const DCols &c, const int64_t (&rows)[U], int64_t n,
                         bool (&pass)[U]) {
-    if (p.n == 0) {
+    if (p.n == 0 || n <= 0) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) pass[u] = rows[u] < n;
+        for (int u = 0; u < U; ++u) pass[u] = p.n == 0 && rows[u] < n;
         return;
     }
-    int64_t v[U][NPC];
-    uint32_t ok[U];
+    // uniform column bases; rows past n re-read row n - 1
+    ColRef pc[NPC];
+#pragma unroll
+    for (int j = 0; j < NPC; ++j) pc[j] = col_ref(c, j < p.n_pcols ? p.pcol[j] : 0);
+    RawCell3 raw[U][NPC];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        ok[u] = 0;
+        const int64_t r = rows[u] < n ? rows[u] : n - 1;
 #pragma unroll
-        for (int j = 0; j < NPC; ++j) {
-            v[u][j] = 0;
-            if (j < p.n_pcols && rows[u] < n) {
-                bool okj;
-                load_cell(c, p.pcol[j], rows[u], v[u][j], okj);
-                ok[u] |= (okj ? 1u : 0u) << j;
-            }
-        }
+        for (int j = 0; j < NPC; ++j)
+            if (j < p.n_pcols) cell_issue3(pc[j], r, raw[u][j]);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) pass[u] = rows[u] < n && eval_reg<NPC>(p, v[u], ok[u]);
+    for (int u = 0; u < U; ++u) {
+        const int64_t r = rows[u] < n ? rows[u] : n - 1;
+        int64_t v[NPC];
+        uint32_t ok = 0;
+#pragma unroll
+        for (int j = 0; j < NPC; ++j) {
+            v[j] = 0;
+            if (j < p.n_pcols) {
+                bool okj;
+                cell_finish3(pc[j], r, raw[u][j], v[j], okj);
+                ok |= (okj ? 1u : 0u) << j;
+            }
+        }
+        pass[u] = rows[u] < n && eval_reg<NPC>(p, v, ok);
+    }
 }
 
 // ─── group state operations ─────────────────────────────────────────────────

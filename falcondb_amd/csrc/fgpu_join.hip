@@ -737,6 +737,9 @@ FGPU_DEV void side_records(const JoinPartArgs &a, JoinAcc &j, const int64_t *rec
 // of every record is issued at once, then only records whose chain goes on
 // take further steps -- and folds the group afterwards, one dispatch per
 // aggregate op for all R records (fold_group).
+// A two-word record as one 16-byte vector (records are read once, non-temporal).
+typedef long long RecPair __attribute__((ext_vector_type(2)));
+
 template <int TYPE, bool FAST, int RW>
 __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     extern __shared__ uint64_t jp_lds[];
@@ -818,10 +821,11 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
             const unsigned i = ci0 + 64 * q + lane;
             const int64_t *rec = i < cn ? cell + (int64_t)i * RW : cell0;
             if (RW == 1) {
-                kk[q] = rec[0];
+                kk[q] = __builtin_nontemporal_load(rec);
                 g1[q] = g2[q] = 0;
             } else if (RW == 2) {
-                const longlong2 v = *(const longlong2 *)rec;
+                const RecPair w2 = __builtin_nontemporal_load((const RecPair *)rec);
+                const longlong2 v = make_longlong2(w2.x, w2.y);
                 kk[q] = v.x;
                 g1[q] = v.y;
                 g2[q] = 0;

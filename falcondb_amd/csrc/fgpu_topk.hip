@@ -81,11 +81,11 @@ FGPU_DEV void block_best(TopPair &my, TopPair &thr, TopPair *out) {
     }
 }
 
-// Sort key of row r of the column (ascending order for the wanted direction).
-FGPU_DEV uint64_t top_key(const DCols &c, int col, int64_t r, int desc) {
-    const int64_t v = load_raw(c, col, r);
+// Sort key of a value v of a column of type dtype (ascending order for the
+// wanted direction).
+FGPU_DEV uint64_t top_key_of(int dtype, int64_t v, int desc) {
     uint64_t k;
-    if (c.dtype[col] == FGPU_FLOAT64) {
+    if (dtype == FGPU_FLOAT64) {
         const uint64_t bits = (uint64_t)v;
         k = (bits >> 63) == 0 ? bits ^ (1ull << 63) : ~bits;
     } else {
@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256) void k_topk_scan(DCols cols, DPred pred, int c
     constexpr int TOPK_UNROLL = topk_unroll<NPC>();
     TopPair my = top_sentinel(), thr = top_sentinel();
     const int lane = threadIdx.x & 63;
+    const ColRef kc = col_ref(cols, col);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x * TOPK_UNROLL;
     for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x * TOPK_UNROLL + (threadIdx.x & ~63); w0 < n; w0 += stride) {
         int64_t rows[TOPK_UNROLL];
@@ -117,11 +118,18 @@ __global__ __launch_bounds__(256) void k_topk_scan(DCols cols, DPred pred, int c
         pred_tile<TOPK_UNROLL, NPC>(pred, cols, rows, n, pass);
         uint64_t key[TOPK_UNROLL];
         bool take[TOPK_UNROLL];
+        // the key cells of every row of the tile are issued before any is examined
+        RawCell3 raw[TOPK_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TOPK_UNROLL; ++u) cell_issue3(kc, rows[u] < n ? rows[u] : n - 1, raw[u]);
 #pragma unroll
         for (int u = 0; u < TOPK_UNROLL; ++u) {
             const int64_t r = rows[u] < n ? rows[u] : n - 1;
-            key[u] = null_mode ? 0ull : top_key(cols, col, r, desc);
-            take[u] = pass[u] && (load_valid(cols, col, r) != (null_mode != 0));
+            int64_t v;
+            bool ok;
+            cell_finish3(kc, r, raw[u], v, ok);
+            key[u] = null_mode ? 0ull : top_key_of(kc.dtype, v, desc);
+            take[u] = pass[u] && (ok != (null_mode != 0));
         }
 #pragma unroll
         for (int u = 0; u < TOPK_UNROLL; ++u) wave_offer<K>(my, thr, TopPair{key[u], rows[u]}, take[u]);

@@ -57,6 +57,81 @@ FGPU_DEV bool load_key(const GbArgs &a, int64_t r, int64_t &key) {
     return key != EMPTY_KEY;
 }
 
+// Two-phase reads of a row's key and inputs: gb_issue issues the loads of
+// row r from uniform column bases (one value load per cell, a validity word
+// only for a column with NULLs) into a GbRow; gb_finish turns them into what
+// load_key / load_inputs return, so the loads of several rows are in flight
+// together.
+struct GbCols {
+    ColRef key;
+    ColRef in[MAX_IN];
+};
+FGPU_DEV GbCols gb_cols(const GbArgs &a) {
+    GbCols g;
+    g.key = col_ref(a.cols, a.key_col >= 0 ? a.key_col : 0);
+#pragma unroll
+    for (int j = 0; j < MAX_IN; ++j) g.in[j] = col_ref(a.cols, j < a.n_in ? a.in_col[j] : 0);
+    return g;
+}
+struct GbRow {
+    int64_t key;
+    uint32_t kvw;
+    int64_t v[MAX_IN];
+    uint32_t vw[MAX_IN];
+};
+FGPU_DEV void gb_cell_issue(const ColRef &c, int64_t r, int64_t &v, uint32_t &vw) {
+    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) v = ((const int64_t *)c.base)[r];
+    else v = (int64_t)((const uint32_t *)c.base)[r];
+    vw = c.valid ? c.valid[r >> 5] : 0xffffffffu;
+}
+// load_raw / load_valid of an issued cell
+FGPU_DEV bool gb_cell_finish(const ColRef &c, int64_t r, int64_t &v, uint32_t vw) {
+    if (c.dtype == FGPU_INT32) v = (int64_t)(int32_t)(uint32_t)v;
+    return (((vw >> (r & 31)) & 1u) != 0) && (c.dtype != FGPU_UTF8 || (uint32_t)v != NULL_CODE);
+}
+FGPU_DEV void gb_issue(const GbArgs &a, const GbCols &g, int64_t r, GbRow &x) {
+    x.key = 0;
+    x.kvw = 0;
+    if (a.key_col >= 0) gb_cell_issue(g.key, r, x.key, x.kvw);
+#pragma unroll
+    for (int j = 0; j < MAX_IN; ++j) {
+        x.v[j] = 0;
+        x.vw[j] = 0;
+        if (j < a.n_in) gb_cell_issue(g.in[j], r, x.v[j], x.vw[j]);
+    }
+}
+// Returns has_key (load_key's result; false without a key column).
+FGPU_DEV bool gb_finish(const GbArgs &a, const GbCols &g, int64_t r, GbRow &x, int64_t &key, Inputs &in) {
+    bool has_key = false;
+    key = 0;
+    if (a.key_col >= 0) {
+        if (a.key_is_utf8) {
+            const uint32_t code = (uint32_t)x.key;
+            key = code == NULL_CODE ? (int64_t)a.utf8_null_code : (int64_t)code;
+            has_key = true;
+        } else {
+            int64_t v = x.key;
+            const bool ok = gb_cell_finish(g.key, r, v, x.kvw);
+            key = v;
+            has_key = ok && v != EMPTY_KEY;
+        }
+    }
+    in = Inputs{0, 0, 0, 0, 0, 0, 0};
+    int64_t v[MAX_IN];
+#pragma unroll
+    for (int j = 0; j < MAX_IN; ++j) {
+        v[j] = x.v[j];
+        if (j < a.n_in && gb_cell_finish(g.in[j], r, v[j], x.vw[j])) in.valid |= 1u << j;
+    }
+    in.v0 = v[0];
+    in.v1 = v[1];
+    in.v2 = v[2];
+    in.v3 = v[3];
+    in.v4 = v[4];
+    in.v5 = v[5];
+    return has_key;
+}
+
 // Find-or-insert in the HBM table; -1 if the table is full.
 FGPU_DEV int64_t hbm_slot(const GbArgs &a, int64_t key, uint64_t h) {
     uint64_t i = h & a.cap_mask;
@@ -103,6 +178,7 @@ __global__ void k_gb_dense_lds(GbArgs a) {
     const int64_t groups = a.range + 1;  // + NULL group
     fill_ident(lds, groups, S, a.ops);
     __syncthreads();
+    const GbCols gc = gb_cols(a);
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
         int64_t rows[GB_UNROLL];
@@ -110,25 +186,22 @@ __global__ void k_gb_dense_lds(GbArgs a) {
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) rows[u] = base + u * blockDim.x + threadIdx.x;
         pred_tile<GB_UNROLL, NPC>(a.pred, a.cols, rows, a.n_rows, pass);
-        // stage B: key and aggregate inputs of every passing row, loads first
-        int64_t key[GB_UNROLL];
-        bool has_key[GB_UNROLL];
-        Inputs in[GB_UNROLL];
+        // stage B: key and aggregate inputs of every passing row, all issued
+        // before any is examined
+        GbRow x[GB_UNROLL];
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
-            key[u] = 0;
-            has_key[u] = false;
-            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
-            if (pass[u]) {
-                if (a.key_col >= 0) has_key[u] = load_key(a, rows[u], key[u]);
-                in[u] = load_inputs(a, rows[u]);
-            }
+            x[u] = GbRow{};
+            if (pass[u]) gb_issue(a, gc, rows[u], x[u]);
         }
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
             if (!pass[u]) continue;
-            const int64_t g = has_key[u] ? (key[u] - a.key_min) : a.range;
-            update_row<LDS_SCOPE>(lds + g * S, a.ops, in[u]);
+            int64_t key;
+            Inputs in;
+            const bool has_key = gb_finish(a, gc, rows[u], x[u], key, in);
+            const int64_t g = has_key ? (key - a.key_min) : a.range;
+            update_row<LDS_SCOPE>(lds + g * S, a.ops, in);
         }
     }
     __syncthreads();
@@ -384,7 +457,8 @@ static void launch_gb_npc(hipStream_t s, int mode, const GbArgs &a, int grid, in
 
 hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,
                           size_t lds_bytes) {
-    if (a.pred.n_pcols <= 2) launch_gb_npc<2>(s, mode, a, grid, block, lds_bytes);
+    if (a.pred.n_pcols <= 1) launch_gb_npc<1>(s, mode, a, grid, block, lds_bytes);
+    else if (a.pred.n_pcols == 2) launch_gb_npc<2>(s, mode, a, grid, block, lds_bytes);
     else launch_gb_npc<MAX_PCOLS>(s, mode, a, grid, block, lds_bytes);
     return hipGetLastError();
 }

@@ -57,81 +57,6 @@ FGPU_DEV bool load_key(const GbArgs &a, int64_t r, int64_t &key) {
     return key != EMPTY_KEY;
 }
 
-// Two-phase reads of a row's key and inputs: gb_issue issues the loads of
-// row r from uniform column bases (one value load per cell, a validity word
-// only for a column with NULLs) into a GbRow; gb_finish turns them into what
-// load_key / load_inputs return, so the loads of several rows are in flight
-// together.
-struct GbCols {
-    ColRef key;
-    ColRef in[MAX_IN];
-};
-FGPU_DEV GbCols gb_cols(const GbArgs &a) {
-    GbCols g;
-    g.key = col_ref(a.cols, a.key_col >= 0 ? a.key_col : 0);
-#pragma unroll
-    for (int j = 0; j < MAX_IN; ++j) g.in[j] = col_ref(a.cols, j < a.n_in ? a.in_col[j] : 0);
-    return g;
-}
-struct GbRow {
-    int64_t key;
-    uint32_t kvw;
-    int64_t v[MAX_IN];
-    uint32_t vw[MAX_IN];
-};
-FGPU_DEV void gb_cell_issue(const ColRef &c, int64_t r, int64_t &v, uint32_t &vw) {
-    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) v = ((const int64_t *)c.base)[r];
-    else v = (int64_t)((const uint32_t *)c.base)[r];
-    vw = c.valid ? c.valid[r >> 5] : 0xffffffffu;
-}
-// load_raw / load_valid of an issued cell
-FGPU_DEV bool gb_cell_finish(const ColRef &c, int64_t r, int64_t &v, uint32_t vw) {
-    if (c.dtype == FGPU_INT32) v = (int64_t)(int32_t)(uint32_t)v;
-    return (((vw >> (r & 31)) & 1u) != 0) && (c.dtype != FGPU_UTF8 || (uint32_t)v != NULL_CODE);
-}
-FGPU_DEV void gb_issue(const GbArgs &a, const GbCols &g, int64_t r, GbRow &x) {
-    x.key = 0;
-    x.kvw = 0;
-    if (a.key_col >= 0) gb_cell_issue(g.key, r, x.key, x.kvw);
-#pragma unroll
-    for (int j = 0; j < MAX_IN; ++j) {
-        x.v[j] = 0;
-        x.vw[j] = 0;
-        if (j < a.n_in) gb_cell_issue(g.in[j], r, x.v[j], x.vw[j]);
-    }
-}
-// Returns has_key (load_key's result; false without a key column).
-FGPU_DEV bool gb_finish(const GbArgs &a, const GbCols &g, int64_t r, GbRow &x, int64_t &key, Inputs &in) {
-    bool has_key = false;
-    key = 0;
-    if (a.key_col >= 0) {
-        if (a.key_is_utf8) {
-            const uint32_t code = (uint32_t)x.key;
-            key = code == NULL_CODE ? (int64_t)a.utf8_null_code : (int64_t)code;
-            has_key = true;
-        } else {
-            int64_t v = x.key;
-            const bool ok = gb_cell_finish(g.key, r, v, x.kvw);
-            key = v;
-            has_key = ok && v != EMPTY_KEY;
-        }
-    }
-    in = Inputs{0, 0, 0, 0, 0, 0, 0};
-    int64_t v[MAX_IN];
-#pragma unroll
-    for (int j = 0; j < MAX_IN; ++j) {
-        v[j] = x.v[j];
-        if (j < a.n_in && gb_cell_finish(g.in[j], r, v[j], x.vw[j])) in.valid |= 1u << j;
-    }
-    in.v0 = v[0];
-    in.v1 = v[1];
-    in.v2 = v[2];
-    in.v3 = v[3];
-    in.v4 = v[4];
-    in.v5 = v[5];
-    return has_key;
-}
-
 // Find-or-insert in the HBM table; -1 if the table is full.
 FGPU_DEV int64_t hbm_slot(const GbArgs &a, int64_t key, uint64_t h) {
     uint64_t i = h & a.cap_mask;

@@ -235,9 +235,9 @@ __global__ void k_gb_hash(GbArgs a) {
 }
 
 // ─── no GROUP BY: register accumulators ─────────────────────────────────────
-// Each thread folds its rows into registers (exact int64; double-double for
-// float sums), then the workgroup merges them into an LDS state row, which one
-// pass of atomics adds to the HBM state row.
+// Each thread folds its rows into registers (exact integer sums; double-double
+// for float sums), then the workgroup merges them exactly into an LDS state
+// row, which one pass of atomics adds to the HBM state row.
 constexpr int SCALAR_OPS = 8;  // register accumulators per thread (the host checks)
 
 template <int NPC>
@@ -284,7 +284,7 @@ __global__ void k_gb_scalar(GbArgs a) {
             for (int i = 0; i < SCALAR_OPS; ++i) {
                 if (i < a.ops.n) {
                     const DOp o = a.ops.op[i];
-                    if ((in[u].valid >> o.col) & 1) reg_add(o.kind, pick(in[u], o.col), acc[i], hi[i], lo[i]);
+                    if ((in[u].valid >> o.col) & 1) reg_add(o, pick(in[u], o.col), acc[i], hi[i], lo[i]);
                 }
             }
         }

@@ -489,7 +489,7 @@ FGPU_DEV void pred_tile(const DPred &p, const DCols &c, const int64_t (&rows)[U]
 // Word 0 of every group state is its row count.
 enum : int8_t {
     OP_CNT = 1,    // non-NULL count of a column
-    OP_ISUM = 2,   // exact int64 sum (wrapping)
+    OP_ISUM = 2,   // exact integer sum, W words (see isum_total)
     OP_IMIN = 3,
     OP_IMAX = 4,
     OP_FSUM = 5,   // exact fixed-point sum: W words at `word`, then a flags word
@@ -504,6 +504,10 @@ struct DOp {
     int32_t has_flags;  // OP_FSUM: column holds NaN/inf, flags word follows the W words
     int32_t word;   // first state word
     int32_t L;      // OP_FSUM fixed-point LSB exponent
+    // Register accumulators only (scalar aggregates, joins): OP_FSUM sums the
+    // values times rscale = 2^-sc, so that no partial sum overflows.
+    int32_t sc;
+    double rscale;
 };
 struct DOps {
     int n;
@@ -536,6 +540,38 @@ FGPU_DEV int opk_L(uint32_t d) { return (int32_t)d >> 18; }
 
 // Flags word bits for non-finite float inputs.
 constexpr uint64_t FL_NAN = 1, FL_PINF = 2, FL_NINF = 4;
+
+// ─── exact integer sums ─────────────────────────────────────────────────────
+// An OP_ISUM state has W = 1 or 2 words.
+//   W = 1: one word of wrapping 64-bit adds.  The host picks it only when the
+//          column's bounds prove |total| <= rows * max|v| < 2^63; the sum mod
+//          2^64 is then the total itself.
+//   W = 2: carry-save in two words: word 0 adds the low 32 bits of each value
+//          (unsigned), word 1 adds the high 32 bits (arithmetic shift).  For up
+//          to 2^32 - 1 values neither word can overflow, and the total is
+//          word1 * 2^32 + word0 exactly.
+// isum_digits gives the two addends of a value for W = 2.
+FGPU_HD void isum_digits(int64_t v, uint64_t &lo, int64_t &hi) {
+    lo = (uint64_t)(uint32_t)(uint64_t)v;
+    hi = v >> 32;
+}
+// The 128-bit total (t_hi * 2^64 + t_lo) of a W = 2 state.
+FGPU_HD void isum_total(uint64_t w0, int64_t w1, uint64_t &t_lo, int64_t &t_hi) {
+    const uint64_t s = (uint64_t)w1 << 32;
+    t_lo = w0 + s;
+    t_hi = (w1 >> 32) + (t_lo < w0 ? 1 : 0);
+}
+// True when the 128-bit total fits int64 (its value is then (int64_t)t_lo).
+FGPU_HD bool isum_fits(uint64_t t_lo, int64_t t_hi) {
+    return t_hi == ((int64_t)t_lo >> 63);
+}
+// The correctly rounded double (round half to even) of a W = 2 state.
+FGPU_HD double isum_to_double(uint64_t w0, int64_t w1) {
+    // three base-2^32 words: w0 low, w1 low + w0 high, w1 high (signed)
+    const int64_t w[3] = {(int64_t)(w0 & 0xffffffffull),
+                          (int64_t)((uint64_t)w1 & 0xffffffffull) + (int64_t)(w0 >> 32), w1 >> 32};
+    return fx_to_double(w, 3, 0);
+}
 
 FGPU_HD double fx_finish(const int64_t *w, int W, int L, uint64_t flags) {
     if (flags) {

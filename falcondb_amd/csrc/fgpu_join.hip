@@ -93,7 +93,7 @@ FGPU_DEV void join_emit(const JoinProbeArgs &a, JoinAcc &j, int64_t pr, uint32_t
         if (i < a.n_ops) {
             const JoinOp o = a.ops[i];
             int64_t v;
-            if (o.col >= 0 && side_value(a, o, pr, br, v)) reg_add(o.kind, v, j.acc[i], j.hi[i], j.lo[i]);
+            if (o.col >= 0 && side_value(a, o, pr, br, v)) reg_add(o.kind, o.W, o.rscale, v, j.acc[i], j.hi[i], j.lo[i]);
         }
     }
 }
@@ -160,6 +160,8 @@ FGPU_DEV void acc_finish(const JoinProbeArgs &a, JoinAcc &j, uint64_t *lrow) {
                 o.has_flags = jo.has_flags;
                 o.word = jo.word;
                 o.L = jo.L;
+                o.sc = jo.sc;
+                o.rscale = jo.rscale;
                 fold_reg(lrow, o, j.acc[i], j.hi[i], j.lo[i]);
             }
         }
@@ -550,7 +552,7 @@ FGPU_DEV void part_emit(const JoinPartArgs &a, JoinAcc &j, int64_t f1, int64_t f
                 if (a.build_pay_col >= 0) v = pay;
                 else load_cell(a.pa.bcols, o.col, (int64_t)(uint32_t)pay, v, ok);
             }
-            if (ok) reg_add(o.kind, v, j.acc[i], j.hi[i], j.lo[i]);
+            if (ok) reg_add(o.kind, o.W, o.rscale, v, j.acc[i], j.hi[i], j.lo[i]);
         }
     }
 }
@@ -563,6 +565,9 @@ struct EmitOps {
     int kind[JOIN_MAX_OPS];
     int src[JOIN_MAX_OPS];
     int special;  // bit i: op i's column holds Inf / NaN (has_flags), so values are classified
+    int wide;     // bit i: op i is a two-word integer sum
+    int scaled;   // bit i: op i's float sum scales its values by rscale[i] (sc > 0)
+    double rscale[JOIN_MAX_OPS];
 };
 FGPU_DEV EmitOps emit_ops(const JoinPartArgs &a) {
     EmitOps e;
@@ -571,6 +576,15 @@ FGPU_DEV EmitOps emit_ops(const JoinPartArgs &a) {
 #pragma unroll
     for (int i = 0; i < JOIN_MAX_OPS; ++i) special |= (i < a.pa.n_ops && a.pa.ops[i].has_flags ? 1 : 0) << i;
     e.special = __builtin_amdgcn_readfirstlane(special);
+    int wide = 0, scaled = 0;
+#pragma unroll
+    for (int i = 0; i < JOIN_MAX_OPS; ++i) {
+        wide |= (i < a.pa.n_ops && a.pa.ops[i].kind == OP_ISUM && a.pa.ops[i].W == 2 ? 1 : 0) << i;
+        scaled |= (i < a.pa.n_ops && a.pa.ops[i].kind == OP_FSUM && a.pa.ops[i].sc != 0 ? 1 : 0) << i;
+        e.rscale[i] = a.pa.ops[i].rscale;
+    }
+    e.wide = __builtin_amdgcn_readfirstlane(wide);
+    e.scaled = __builtin_amdgcn_readfirstlane(scaled);
 #pragma unroll
     for (int i = 0; i < JOIN_MAX_OPS; ++i) {
         const JoinOp o = a.pa.ops[i];
@@ -614,8 +628,21 @@ FGPU_DEV void fold_group(const EmitOps &e, JoinAcc &j, const bool (&live)[R], co
                 for (int q = 0; q < R; ++q) acc += on[q] ? 1 : 0;
                 break;
             case OP_ISUM:
+                if ((e.wide >> i) & 1) {  // two words: low halves in acc, high halves in hi's bits
+                    int64_t h = __double_as_longlong(j.hi[i]);
 #pragma unroll
-                for (int q = 0; q < R; ++q) acc = (int64_t)((uint64_t)acc + (on[q] ? (uint64_t)v[q] : 0ull));
+                    for (int q = 0; q < R; ++q) {
+                        uint64_t dl;
+                        int64_t dh;
+                        isum_digits(on[q] ? v[q] : 0, dl, dh);
+                        acc = (int64_t)((uint64_t)acc + dl);
+                        h += dh;
+                    }
+                    j.hi[i] = __longlong_as_double(h);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < R; ++q) acc = (int64_t)((uint64_t)acc + (on[q] ? (uint64_t)v[q] : 0ull));
+                }
                 break;
             case OP_IMIN:
 #pragma unroll
@@ -626,7 +653,20 @@ FGPU_DEV void fold_group(const EmitOps &e, JoinAcc &j, const bool (&live)[R], co
                 for (int q = 0; q < R; ++q) acc = on[q] && v[q] > acc ? v[q] : acc;
                 break;
             case OP_FSUM:
-                if ((e.special >> i) & 1) {
+                if ((e.scaled >> i) & 1) {  // values near the top of the double range
+#pragma unroll
+                    for (int q = 0; q < R; ++q) {
+                        const uint64_t bits = (uint64_t)v[q];
+                        const bool special = ((bits >> 52) & 0x7ff) == 0x7ff;
+                        const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
+                        acc |= on[q] && special ? (int64_t)fl : 0;
+                        const double x = on[q] && !special ? __longlong_as_double(v[q]) * e.rscale[i] : 0.0;
+                        double sum, err;
+                        two_sum(j.hi[i], x, sum, err);
+                        j.hi[i] = sum;
+                        j.lo[i] += err;
+                    }
+                } else if ((e.special >> i) & 1) {
 #pragma unroll
                     for (int q = 0; q < R; ++q) {
                         const uint64_t bits = (uint64_t)v[q];

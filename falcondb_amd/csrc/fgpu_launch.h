@@ -183,6 +183,8 @@ struct JoinOp {
     int32_t L;
     int32_t W;
     int32_t has_flags;
+    int32_t sc;      // OP_FSUM: values are summed times rscale = 2^-sc
+    double rscale;
 };
 struct JoinProbeArgs {
     DCols pcols;      // probe columns

@@ -510,8 +510,35 @@ int fx_words(const DevCol &c, int &L, int &W) {
     return FGPU_OK;
 }
 
-// Adds (or finds) an op; returns its index.  `col` is a table column index.
-int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot) {
+// ceil(log2(n)) for n >= 1.
+int ceil_log2(int64_t n) {
+    int b = 0;
+    while (b < 63 && (int64_t(1) << b) < n) ++b;
+    return b;
+}
+
+// Words of an exact integer SUM of at most `rows` values of column c: one
+// wrapping word when rows * max|v| < 2^63 proves that the total fits int64,
+// else the two-word carry-save form (isum_total).
+int isum_words(const DevCol &c, int64_t rows) {
+    if (c.imin > c.imax || rows <= 0) return 1;  // no valid value
+    const unsigned __int128 mag = (unsigned __int128)std::max<__int128>(-(__int128)c.imin, (__int128)c.imax);
+    const unsigned __int128 bound = mag * (unsigned __int128)(uint64_t)rows;
+    return bound < ((unsigned __int128)1 << 63) ? 1 : 2;
+}
+
+// Power-of-two prescale of a register float sum over at most `rows` values of
+// column c: every value is below 2^(emax + 53), so the partial sums stay below
+// 2^(emax + 53 + ceil(log2(rows))), which must stay below 2^1023.
+int fsum_scale(const DevCol &c, int64_t rows) {
+    if (c.emin > c.emax) return 0;
+    const int top = c.emax + 53 + ceil_log2(std::max<int64_t>(rows, 1)) + 1;
+    return top > 1023 ? top - 1023 : 0;
+}
+
+// Adds (or finds) an op; returns its index.  `col` is a table column index;
+// `rows` bounds the number of values any one group's aggregate takes in.
+int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot, int64_t rows) {
     for (size_t i = 0; i < L.keys.size(); ++i)
         if (L.keys[i].kind == kind && L.keys[i].side == side && L.keys[i].col == col) return (int)i;
     if (L.ops.n >= MAX_OPS) return set_err(FGPU_E_UNSUPPORTED, "too many aggregate states (max %d)", MAX_OPS) * 0 - 1;
@@ -520,13 +547,24 @@ int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot
     o.col = slot;
     o.word = L.ops.S;
     int nwords = 1;
+    o.rscale = 1.0;
     if (kind == OP_FSUM) {
         int l, w;
         if (fx_words(dc, l, w)) return -1;
         o.L = l;
         o.W = w;
         o.has_flags = dc.nonfinite ? 1 : 0;
+        o.sc = fsum_scale(dc, rows);
+        o.rscale = std::ldexp(1.0, -o.sc);
         nwords = w + o.has_flags;
+    } else if (kind == OP_ISUM) {
+        o.W = isum_words(dc, rows);
+        if (o.W == 2 && rows >= (int64_t(1) << 32)) {
+            set_err(FGPU_E_UNSUPPORTED, "exact SUM of '%s' over %lld rows (the two-word sum takes < 2^32)",
+                    dc.name.c_str(), (long long)rows);
+            return -1;
+        }
+        nwords = o.W;
     }
     if (L.ops.S + nwords > 64) {
         set_err(FGPU_E_UNSUPPORTED, "aggregate state exceeds 64 words");
@@ -557,9 +595,12 @@ std::string alias_of(const fgpu_agg &a, int i) {
 
 // Builds ops + outputs for aggregates over one or two tables.  `slot_of(side,
 // col)` binds a column and returns its slot (or <0).
+// `rows` bounds the number of values one group's aggregate takes in (-1: the
+// rows of t0).
 template <class SlotFn>
 int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const fgpu_table *t1,
-                 SlotFn slot_of, AggLayout &L) {
+                 SlotFn slot_of, AggLayout &L, int64_t rows = -1) {
+    if (rows < 0) rows = t0->n_rows;
     L = AggLayout{};
     L.ops.n = 0;
     L.ops.S = 1;  // word 0 = rows
@@ -592,7 +633,7 @@ int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const f
         // A NULL build side (LEFT) makes every build column nullable.
         const bool need_cnt = nullable || side == 1 || a.func == FGPU_AVG || a.func == FGPU_COUNT;
         if (need_cnt && (nullable || side == 1)) {
-            const int ci = add_op(L, OP_CNT, side, a.col, dc, slot);
+            const int ci = add_op(L, OP_CNT, side, a.col, dc, slot, rows);
             if (ci < 0) return FGPU_E_UNSUPPORTED;
             cnt_word = L.ops.op[ci].word;
         }
@@ -608,7 +649,7 @@ int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const f
             case FGPU_MIN: kind = isf ? OP_FMIN : OP_IMIN; o.kind = isf ? OUT_MIN_F : OUT_MIN_I; break;
             default: kind = isf ? OP_FMAX : OP_IMAX; o.kind = isf ? OUT_MAX_F : OUT_MAX_I; break;
         }
-        const int oi = add_op(L, kind, side, a.col, dc, slot);
+        const int oi = add_op(L, kind, side, a.col, dc, slot, rows);
         if (oi < 0) return g_err.empty() ? set_err(FGPU_E_UNSUPPORTED, "aggregate layout") : FGPU_E_UNSUPPORTED;
         const DOp &op = L.ops.op[oi];
         o.word = op.word;
@@ -686,9 +727,30 @@ int collect(fgpu_ctx *c, const AggLayout &L, int64_t n_slots, const DevCol *key,
         const int k = L.outs.o[i].kind;
         const bool is_f = k == OUT_AVG_I || k == OUT_SUM_F || k == OUT_AVG_F || k == OUT_MIN_F || k == OUT_MAX_F;
         rc.dtype = is_f ? FGPU_FLOAT64 : FGPU_INT64;
-        const char *v = col_at(ci++);
+        char *v = col_at(ci++);
         rc.xv = v;
         rc.xvalid = (const uint8_t *)v + n * 8;
+        if (k == OUT_SUM_I && L.outs.o[i].W == 2 && n) {
+            // A two-word integer sum emits the exact int64 total, or, where the
+            // total leaves int64, its correctly rounded double with validity 2.
+            // A column with any such group is returned as Float64 (the type the
+            // reference gives every SUM, src/query/aggregate.rs:3396-3413).
+            int64_t *vals = (int64_t *)v;
+            uint8_t *ok = (uint8_t *)v + n * 8;
+            bool any_wide = false;
+            for (size_t g = 0; g < n && !any_wide; ++g) any_wide = ok[g] == 2;
+            if (any_wide) {
+                for (size_t g = 0; g < n; ++g) {
+                    if (ok[g] == 1) {
+                        const double d = (double)vals[g];
+                        std::memcpy(&vals[g], &d, 8);
+                    } else if (ok[g] == 2) {
+                        ok[g] = 1;
+                    }
+                }
+                rc.dtype = FGPU_FLOAT64;
+            }
+        }
         r->cols.push_back(std::move(rc));
     }
     *out = r;
@@ -1764,7 +1826,14 @@ static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *p
     a.type = spec->type;
     a.n_probe = probe->n_rows;
     auto slot_of = [&](int side, int col) -> int { return side == 1 ? bb.bind(col) : pb.bind(col); };
-    if (int rc = build_layout(spec->aggs, spec->n_aggs, probe, build, slot_of, L)) return rc;
+    // Output rows of the join: at most one per probe row when the build keys are
+    // unique (SEMI / ANTI always), else up to one per (probe, build) pair.
+    int64_t out_rows = probe->n_rows;
+    if ((spec->type == FGPU_INNER || spec->type == FGPU_LEFT) && !(idx && idx->unique_keys)) {
+        const __int128 pairs = (__int128)probe->n_rows * std::max<int64_t>(build->n_rows, 1);
+        out_rows = pairs > (__int128)INT64_MAX ? INT64_MAX : (int64_t)pairs;
+    }
+    if (int rc = build_layout(spec->aggs, spec->n_aggs, probe, build, slot_of, L, out_rows)) return rc;
     if (L.ops.n > 4) return set_err(FGPU_E_UNSUPPORTED, "join aggregates need %d accumulators (max 4)", L.ops.n);
     a.pcols = pb.dc;
     a.bcols = bb.dc;
@@ -1779,6 +1848,8 @@ static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *p
         j.L = o.L;
         j.W = o.W;
         j.has_flags = o.has_flags;
+        j.sc = o.sc;
+        j.rscale = o.rscale;
     }
     a.S = L.ops.S;
     return FGPU_OK;

@@ -84,7 +84,17 @@ FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
         uint64_t *w = st + opk_word(d);
         switch (opk_kind(d)) {
             case OP_CNT: a_add<SCOPE>(w, 1ull); break;
-            case OP_ISUM: a_add<SCOPE>(w, (uint64_t)v); break;
+            case OP_ISUM:
+                if (opk_W(d) == 2) {
+                    uint64_t lo;
+                    int64_t hi;
+                    isum_digits(v, lo, hi);
+                    a_add<SCOPE>(w, lo);
+                    a_add<SCOPE>(w + 1, (uint64_t)hi);
+                } else {
+                    a_add<SCOPE>(w, (uint64_t)v);
+                }
+                break;
             case OP_IMIN: a_smin<SCOPE>(w, v); break;
             case OP_IMAX: a_smax<SCOPE>(w, v); break;
             case OP_FSUM: {
@@ -268,8 +278,11 @@ FGPU_DEV void merge_state(uint64_t *dst, const uint64_t *src, const DOps &ops) {
         const uint64_t *s = src + opk_word(d);
         switch (opk_kind(d)) {
             case OP_CNT:
+                if (s[0]) a_add<SCOPE>(w, s[0]);
+                break;
             case OP_ISUM:
                 if (s[0]) a_add<SCOPE>(w, s[0]);
+                if (opk_W(d) == 2 && s[1]) a_add<SCOPE>(w + 1, s[1]);
                 break;
             case OP_IMIN: a_smin<SCOPE>(w, (int64_t)s[0]); break;
             case OP_IMAX: a_smax<SCOPE>(w, (int64_t)s[0]); break;
@@ -287,46 +300,44 @@ FGPU_DEV void merge_state(uint64_t *dst, const uint64_t *src, const DOps &ops) {
 }
 
 
-// Folds one thread's register accumulators (acc: integer / encoded word,
-// hi+lo: double-double float sum) for op `o` into an LDS state row.
-FGPU_DEV void fold_reg(uint64_t *row, const DOp &o, int64_t acc, double hi, double lo) {
-    uint64_t *w = row + o.word;
-    switch (o.kind) {
-        case OP_CNT:
-        case OP_ISUM:
-            if (acc) a_add<LDS_SCOPE>(w, (uint64_t)acc);
-            break;
-        case OP_IMIN: a_smin<LDS_SCOPE>(w, acc); break;
-        case OP_IMAX: a_smax<LDS_SCOPE>(w, acc); break;
-        case OP_FMIN: a_umin<LDS_SCOPE>(w, (uint64_t)acc); break;
-        case OP_FMAX: a_umax<LDS_SCOPE>(w, (uint64_t)acc); break;
-        case OP_FSUM: {
-            if (acc) a_or<LDS_SCOPE>(w + o.W, (uint64_t)acc);
-            const double parts[2] = {hi, lo};
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                int w0;
-                int64_t d0, d1, d2;
-                if (!fx_digits((uint64_t)__double_as_longlong(parts[q]), o.L, w0, d0, d1, d2)) continue;
-                if (d0) a_add<LDS_SCOPE>(w + w0, (uint64_t)d0);
-                if (d1) a_add<LDS_SCOPE>(w + w0 + 1, (uint64_t)d1);
-                if (d2) a_add<LDS_SCOPE>(w + w0 + 2, (uint64_t)d2);
-            }
-            break;
-        }
-    }
-}
+// ─── per-thread register accumulators (scalar aggregates, joins) ────────────
+// One op's accumulator is an int64 `acc` and a double pair hi + lo:
+//   COUNT, MIN, MAX   acc (MIN/MAX of doubles as f64_order keys; NaN skipped)
+//   ISUM, W = 1       acc (wrapping; the host proved the total fits)
+//   ISUM, W = 2       acc = sum of the values' low 32 bits, and the bits of
+//                     `hi` hold the sum of their high 32 bits (isum_digits)
+//   FSUM              hi + lo: a double-double sum of the values times
+//                     2^-sc (sc > 0 only for columns near the top of the double
+//                     range, so that no partial sum can overflow); acc ORs the
+//                     FL_* flags of non-finite inputs
+// A thread's register sums are then folded exactly into the workgroup's state
+// row.  The double-double sum is exact while each error term adds into `lo`
+// without rounding; otherwise its error is below 2^-104 of the sum of the
+// thread's absolute values.
+FGPU_DEV int64_t dbits(double x) { return __double_as_longlong(x); }
+FGPU_DEV double bitsd(int64_t x) { return __longlong_as_double(x); }
 
 // Identity of a register accumulator for op kind k.
 FGPU_DEV int64_t reg_ident(int k) {
     return k == OP_IMIN ? INT64_MAX : k == OP_IMAX ? INT64_MIN : k == OP_FMIN ? (int64_t)~0ull : 0;
 }
 
-// Adds value v (raw 64-bit pattern) into a register accumulator of op kind k.
-FGPU_DEV void reg_add(int k, int64_t v, int64_t &acc, double &hi, double &lo) {
+// Adds value v (raw 64-bit pattern) into the register accumulator of an op of
+// kind k (W words; rscale = 2^-sc for FSUM).
+FGPU_DEV void reg_add(int k, int W, double rscale, int64_t v, int64_t &acc, double &hi, double &lo) {
     switch (k) {
         case OP_CNT: acc += 1; break;
-        case OP_ISUM: acc = (int64_t)((uint64_t)acc + (uint64_t)v); break;
+        case OP_ISUM:
+            if (W == 2) {
+                uint64_t dl;
+                int64_t dh;
+                isum_digits(v, dl, dh);
+                acc = (int64_t)((uint64_t)acc + dl);
+                hi = bitsd(dbits(hi) + dh);
+            } else {
+                acc = (int64_t)((uint64_t)acc + (uint64_t)v);
+            }
+            break;
         case OP_IMIN: acc = v < acc ? v : acc; break;
         case OP_IMAX: acc = v > acc ? v : acc; break;
         case OP_FSUM: {
@@ -335,7 +346,7 @@ FGPU_DEV void reg_add(int k, int64_t v, int64_t &acc, double &hi, double &lo) {
                 acc |= (bits & 0x000fffffffffffffull) ? FL_NAN : (bits >> 63) ? FL_NINF : FL_PINF;
             } else {
                 double s, e;
-                two_sum(hi, __longlong_as_double(v), s, e);
+                two_sum(hi, bitsd(v) * rscale, s, e);
                 hi = s;
                 lo += e;
             }
@@ -348,6 +359,43 @@ FGPU_DEV void reg_add(int k, int64_t v, int64_t &acc, double &hi, double &lo) {
             const uint64_t e = f64_order(bits);
             if (k == OP_FMIN) acc = (int64_t)(e < (uint64_t)acc ? e : (uint64_t)acc);
             else acc = (int64_t)(e > (uint64_t)acc ? e : (uint64_t)acc);
+            break;
+        }
+    }
+}
+FGPU_DEV void reg_add(const DOp &o, int64_t v, int64_t &acc, double &hi, double &lo) {
+    reg_add(o.kind, o.W, o.rscale, v, acc, hi, lo);
+}
+
+// Folds one thread's register accumulators for op `o` into an LDS state row.
+FGPU_DEV void fold_reg(uint64_t *row, const DOp &o, int64_t acc, double hi, double lo) {
+    uint64_t *w = row + o.word;
+    switch (o.kind) {
+        case OP_CNT:
+            if (acc) a_add<LDS_SCOPE>(w, (uint64_t)acc);
+            break;
+        case OP_ISUM:
+            if (acc) a_add<LDS_SCOPE>(w, (uint64_t)acc);
+            if (o.W == 2 && dbits(hi)) a_add<LDS_SCOPE>(w + 1, (uint64_t)dbits(hi));
+            break;
+        case OP_IMIN: a_smin<LDS_SCOPE>(w, acc); break;
+        case OP_IMAX: a_smax<LDS_SCOPE>(w, acc); break;
+        case OP_FMIN: a_umin<LDS_SCOPE>(w, (uint64_t)acc); break;
+        case OP_FMAX: a_umax<LDS_SCOPE>(w, (uint64_t)acc); break;
+        case OP_FSUM: {
+            if (acc) a_or<LDS_SCOPE>(w + o.W, (uint64_t)acc);
+            // hi and lo are finite multiples of 2^(L - sc): digits relative to
+            // 2^(L - sc) of the scaled sums are digits relative to 2^L of the sums
+            const double parts[2] = {hi, lo};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                int w0;
+                int64_t d0, d1, d2;
+                if (!fx_digits((uint64_t)dbits(parts[q]), o.L - o.sc, w0, d0, d1, d2)) continue;
+                if (d0) a_add<LDS_SCOPE>(w + w0, (uint64_t)d0);
+                if (d1) a_add<LDS_SCOPE>(w + w0 + 1, (uint64_t)d1);
+                if (d2) a_add<LDS_SCOPE>(w + w0 + 2, (uint64_t)d2);
+            }
             break;
         }
     }
@@ -367,9 +415,25 @@ FGPU_DEV void emit_row(const DOuts &outs, const uint64_t *st, int64_t o, int64_t
             case OUT_COUNT_ROWS: out = (int64_t)st[0]; valid = 1; break;
             case OUT_COUNT: out = cnt; valid = 1; break;
             case OUT_SUM_I:
+                out = (int64_t)w;
+                if (d.W == 2) {  // the exact total, or (valid = 2) its rounded double
+                    uint64_t t_lo;
+                    int64_t t_hi;
+                    isum_total(w, (int64_t)st[d.word + 1], t_lo, t_hi);
+                    out = (int64_t)t_lo;
+                    if (!isum_fits(t_lo, t_hi)) {
+                        out = dbits(isum_to_double(w, (int64_t)st[d.word + 1]));
+                        valid = valid ? 2 : 0;
+                    }
+                }
+                break;
             case OUT_MIN_I:
             case OUT_MAX_I: out = (int64_t)w; break;
-            case OUT_AVG_I: out = __double_as_longlong(cnt ? (double)(int64_t)w / (double)cnt : 0.0); break;
+            case OUT_AVG_I: {
+                const double s2 = d.W == 2 ? isum_to_double(w, (int64_t)st[d.word + 1]) : (double)(int64_t)w;
+                out = dbits(cnt ? s2 / (double)cnt : 0.0);
+                break;
+            }
             case OUT_SUM_F:
             case OUT_AVG_F: {
                 double s2 = fx_finish((const int64_t *)(st + d.word), d.W, d.L,

@@ -45,6 +45,10 @@ def lib():
         L.ora_exact_sum.argtypes = [vp, i64]
         L.ora_sum_lanes.restype = C.c_double
         L.ora_sum_lanes.argtypes = [vp, i64]
+        L.ora_sum_i64.restype = C.c_double
+        L.ora_sum_i64.argtypes = [vp, i64]
+        L.ora_checked_sum_i64.restype = C.c_int
+        L.ora_checked_sum_i64.argtypes = [vp, i64, C.POINTER(i64)]
         L.ora_filter_mask.argtypes = [vp, C.c_int, i64, i32, vp, vp]
         L.ora_filter_count.argtypes = [vp, C.c_int, i64, i32, vp, C.POINTER(i64)]
         L.ora_topk.argtypes = [vp, C.c_int, i64, i32, vp, C.c_int, C.c_int, i64, vp, C.POINTER(i64)]
@@ -90,6 +94,21 @@ def exact_sum(x: np.ndarray) -> float:
 def sum_lanes(x: np.ndarray) -> float:
     x = np.ascontiguousarray(x, np.float64)
     return float(lib().ora_sum_lanes(x.ctypes.data, x.size))
+
+
+def sum_i64(x: np.ndarray) -> float:
+    """sum_i64 (src/query/aggregate.rs:3039-3044): checked i64 total as f64, or
+    the f64 lanes on overflow."""
+    x = np.ascontiguousarray(x, np.int64)
+    return float(lib().ora_sum_i64(x.ctypes.data, x.size))
+
+
+def checked_sum_i64(x: np.ndarray):
+    """checked_sum_i64 (src/query/aggregate.rs:3087-3110): the total, or None on
+    overflow."""
+    x = np.ascontiguousarray(x, np.int64)
+    out = C.c_int64()
+    return None if lib().ora_checked_sum_i64(x.ctypes.data, x.size, C.byref(out)) else int(out.value)
 
 
 class _Cols:

@@ -9,8 +9,9 @@
 // packed LSB-first masks, materialised filter output, per-batch hash maps with a
 // serial merge in batch order, a serial chained-table join build with a Bloom
 // filter and parallel probe units) so that its timing stands for the reference's
-// CPU path, and it adds exact results (integer sums in int64, float sums exactly
-// rounded by Shewchuk's algorithm) that the parity tests compare against.
+// CPU path, and it adds exact results (integer sums in 128 bits, float sums
+// exactly rounded from a wide integer accumulator) that the parity tests compare
+// against.
 //
 // Reference files followed (paths relative to the FalconDB tree):
 //   filter       src/query/expr.rs:635-1175, src/storage/table.rs:154-242,
@@ -136,53 +137,93 @@ std::vector<Col> make_cols(const fgpu_column_desc *d, int n) {
     return v;
 }
 
-// ─── exact float summation (Shewchuk 1997, "Adaptive precision floating-point
-// arithmetic", grow-expansion with the round-half-even correction of the final
-// compression) ───────────────────────────────────────────────────────────────
+// ─── exact float summation ──────────────────────────────────────────────────
+// Every finite double is an integer multiple of 2^-1074.  The sum is kept as
+// two unsigned big integers (positive and negative terms) in units of 2^-1074,
+// 34 limbs of 64 bits each (2176 bits: the largest double is below 2^2098 in
+// these units, which leaves 78 bits of room for the number of terms).  Nothing
+// can overflow or round until result(), which rounds the exact difference to
+// nearest-even once.
 struct ExactSum {
-    std::vector<double> p;  // non-overlapping partials, increasing magnitude
+    static constexpr int NL = 34;
+    uint64_t pos[NL] = {}, neg[NL] = {};
     double special = 0.0;   // running sum of non-finite inputs (IEEE semantics)
     bool any_special = false;
 
+    static void add_at(uint64_t *a, uint64_t m, int bit) {
+        int l = bit >> 6, s = bit & 63;
+        const uint64_t lo = m << s, hi = s ? (m >> (64 - s)) : 0;
+        unsigned __int128 t = (unsigned __int128)a[l] + lo;
+        a[l] = (uint64_t)t;
+        uint64_t carry = (uint64_t)(t >> 64) + hi;
+        while (carry && ++l < NL) {
+            t = (unsigned __int128)a[l] + carry;
+            a[l] = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+    }
     void add(double x) {
         if (!std::isfinite(x)) {
             special += x;
             any_special = true;
             return;
         }
-        size_t i = 0;
-        for (size_t j = 0; j < p.size(); ++j) {
-            double y = p[j];
-            if (std::fabs(x) < std::fabs(y)) std::swap(x, y);
-            const double hi = x + y;
-            const double lo = y - (hi - x);
-            if (lo != 0.0) p[i++] = lo;
-            x = hi;
-        }
-        p.resize(i);
-        p.push_back(x);
+        uint64_t bits;
+        std::memcpy(&bits, &x, 8);
+        const int be = (int)((bits >> 52) & 0x7ff);
+        uint64_t m = bits & 0x000fffffffffffffull;
+        if (be) m |= 0x0010000000000000ull;
+        if (!m) return;
+        const int bit = be ? be - 1 : 0;  // x = m * 2^(bit - 1074)
+        add_at((bits >> 63) ? neg : pos, m, bit);
+    }
+    // a >= b over NL limbs
+    static bool ge(const uint64_t *a, const uint64_t *b) {
+        for (int l = NL - 1; l >= 0; --l)
+            if (a[l] != b[l]) return a[l] > b[l];
+        return true;
     }
     double result() const {
         if (any_special) return special;  // NaN / +-inf dominate
-        size_t n = p.size();
-        if (n == 0) return 0.0;
-        double hi = p[--n], lo = 0.0;
-        while (n > 0) {
-            const double x = hi, y = p[--n];
-            hi = x + y;
-            const double yr = hi - x;
-            lo = y - yr;
-            if (lo != 0.0) break;
+        const bool negative = !ge(pos, neg);
+        const uint64_t *a = negative ? neg : pos, *b = negative ? pos : neg;
+        uint64_t d[NL];
+        uint64_t borrow = 0;
+        for (int l = 0; l < NL; ++l) {
+            const unsigned __int128 t = (unsigned __int128)a[l] - b[l] - borrow;
+            d[l] = (uint64_t)t;
+            borrow = (uint64_t)(t >> 64) ? 1 : 0;
         }
-        // Round half to even across the remaining partials: if the discarded
-        // tail has the same sign as lo, the true sum is beyond the halfway point.
-        if (n > 0 && ((lo < 0.0 && p[n - 1] < 0.0) || (lo > 0.0 && p[n - 1] > 0.0))) {
-            const double y = lo * 2.0;
-            const double x = hi + y;
-            const double yr = x - hi;
-            if (y == yr) hi = x;
+        int top = NL - 1;
+        while (top >= 0 && d[top] == 0) --top;
+        if (top < 0) return 0.0;
+        const int msb = top * 64 + 63 - __builtin_clzll(d[top]);
+        auto bit_at = [&](int i) -> uint64_t { return (d[i >> 6] >> (i & 63)) & 1; };
+        double r;
+        if (msb <= 52) {  // below 2^53 units: exact (a subnormal or small normal)
+            r = std::ldexp((double)d[0], -1074);
+        } else {
+            const int shift = msb - 52;  // drop `shift` low bits
+            // bits [shift, shift + 52] of d
+            const int l = shift >> 6, o = shift & 63;
+            uint64_t mant = d[l] >> o;
+            if (o && l + 1 < NL) mant |= d[l + 1] << (64 - o);
+            mant &= (1ull << 53) - 1;
+            const bool round = bit_at(shift - 1);
+            // sticky: any set bit below shift - 1
+            const int sb = shift - 1;
+            bool sticky = (d[sb >> 6] & ((1ull << (sb & 63)) - 1)) != 0;
+            for (int i = 0; i < (sb >> 6) && !sticky; ++i) sticky = d[i] != 0;
+            int e = shift;
+            if (round && (sticky || (mant & 1))) {
+                if (++mant == (1ull << 53)) {
+                    mant >>= 1;
+                    ++e;
+                }
+            }
+            r = std::ldexp((double)mant, e - 1074);  // inf when it leaves the range
         }
-        return hi;
+        return negative ? -r : r;
     }
 };
 
@@ -545,6 +586,30 @@ struct I64Map {
     }
 };
 
+// Exact integer totals.  The reference never wraps: its GROUP BY sums `v as
+// f64` (src/query/aggregate.rs:1020-1036) and its scalar SUM falls back to f64
+// lanes when a checked i64 sum overflows (checked_sum_i64 / sum_i64, :3039-3110;
+// known answer falls_back_instead_of_wrapping_on_overflow, :1975-1982).  The
+// exact form keeps the total in 128 bits: an int64 when it fits, else its
+// correctly rounded double (`is_more_accurate_past_the_float_limit`, :1959-1970).
+inline bool i128_fits(__int128 t) { return t >= (__int128)INT64_MIN && t <= (__int128)INT64_MAX; }
+
+// Fills an exact SUM column from 128-bit totals: Int64 when every total fits,
+// else Float64 of every total.
+void int_sum_column(const std::vector<__int128> &tot, OCol &oc) {
+    bool all_fit = true;
+    for (auto t : tot) all_fit = all_fit && i128_fits(t);
+    if (all_fit) {
+        oc.dtype = FGPU_INT64;
+        oc.i64.resize(tot.size());
+        for (size_t g = 0; g < tot.size(); ++g) oc.i64[g] = (int64_t)tot[g];
+    } else {
+        oc.dtype = FGPU_FLOAT64;
+        oc.f64.resize(tot.size());
+        for (size_t g = 0; g < tot.size(); ++g) oc.f64[g] = (double)tot[g];
+    }
+}
+
 // Exact per-group values, computed by a separate pass over the filtered rows.
 void exact_values(const std::vector<AggPlan> &plan, const std::vector<FBatch> &fbs,
                   const std::vector<std::vector<uint32_t>> &batch_gids, size_t n_groups,
@@ -553,8 +618,8 @@ void exact_values(const std::vector<AggPlan> &plan, const std::vector<FBatch> &f
         const AggPlan &ap = plan[a];
         OCol &oc = res.cols[first_agg_col + a];
         const bool isf = ap.dtype == FGPU_FLOAT64;
-        std::vector<int64_t> cnt(n_groups, 0), isum(n_groups, 0), mn(n_groups, INT64_MAX),
-            mx(n_groups, INT64_MIN);
+        std::vector<int64_t> cnt(n_groups, 0), mn(n_groups, INT64_MAX), mx(n_groups, INT64_MIN);
+        std::vector<__int128> isum(n_groups, 0);
         std::vector<double> fmn(n_groups, 0.0), fmx(n_groups, 0.0);
         std::vector<uint8_t> fhas(n_groups, 0);
         std::vector<ExactSum> fs;
@@ -578,7 +643,7 @@ void exact_values(const std::vector<AggPlan> &plan, const std::vector<FBatch> &f
                         fhas[g] = 1;
                     }
                 } else {
-                    isum[g] = (int64_t)((uint64_t)isum[g] + (uint64_t)raw);
+                    isum[g] += raw;
                     mn[g] = std::min(mn[g], raw);
                     mx[g] = std::max(mx[g], raw);
                 }
@@ -595,7 +660,7 @@ void exact_values(const std::vector<AggPlan> &plan, const std::vector<FBatch> &f
             oc.dtype = FGPU_FLOAT64;
             oc.f64.resize(n_groups);
             for (size_t g = 0; g < n_groups; ++g)
-                oc.f64[g] = cnt[g] ? (isf ? fs[g].result() : (double)isum[g]) / (double)cnt[g] : 0.0;
+                oc.f64[g] = cnt[g] ? (isf ? fs[g].result() : (double)isum[g]) / (double)cnt[g] : 0.0;  // rounded total / count
             continue;
         }
         if (isf) {
@@ -608,11 +673,12 @@ void exact_values(const std::vector<AggPlan> &plan, const std::vector<FBatch> &f
                     oc.valid[g] = fhas[g];
                 }
             }
+        } else if (ap.func == FGPU_SUM) {
+            int_sum_column(isum, oc);
         } else {
             oc.dtype = FGPU_INT64;
             oc.i64.resize(n_groups);
-            for (size_t g = 0; g < n_groups; ++g)
-                oc.i64[g] = ap.func == FGPU_SUM ? isum[g] : ap.func == FGPU_MIN ? mn[g] : mx[g];
+            for (size_t g = 0; g < n_groups; ++g) oc.i64[g] = ap.func == FGPU_MIN ? mn[g] : mx[g];
         }
     }
 }
@@ -661,6 +727,34 @@ double ora_sum_lanes(const double *x, int64_t n) {
     for (int k = 0; k < 8; ++k) t += acc[k];
     for (; i < n; ++i) t += x[i];
     return t;
+}
+
+// checked_sum_i64 (src/query/aggregate.rs:3087-3110): four lanes of checked
+// adds, then the lanes, then the tail.  Returns 0 with *out set, or 1 when an
+// add leaves i64 (the reference's None).
+int ora_checked_sum_i64(const int64_t *x, int64_t n, int64_t *out) {
+    int64_t acc[4] = {0, 0, 0, 0};
+    int64_t i = 0;
+    for (; i + 4 <= n; i += 4)
+        for (int k = 0; k < 4; ++k)
+            if (__builtin_add_overflow(acc[k], x[i + k], &acc[k])) return 1;
+    int64_t total = 0;
+    for (int k = 0; k < 4; ++k)
+        if (__builtin_add_overflow(total, acc[k], &total)) return 1;
+    for (; i < n; ++i)
+        if (__builtin_add_overflow(total, x[i], &total)) return 1;
+    *out = total;
+    return 0;
+}
+
+// sum_i64 (src/query/aggregate.rs:3039-3044): the checked total as f64, else
+// the f64 lanes (sum_lanes over the values widened with `as f64`).
+double ora_sum_i64(const int64_t *x, int64_t n) {
+    int64_t t;
+    if (ora_checked_sum_i64(x, n, &t) == 0) return (double)t;
+    std::vector<double> v((size_t)n);
+    for (int64_t i = 0; i < n; ++i) v[i] = (double)x[i];
+    return ora_sum_lanes(v.data(), n);
 }
 
 // Packed mask, one batch of `batch_rows` at a time, concatenated (batch_rows is
@@ -1172,10 +1266,13 @@ int ora_join(const fgpu_column_desc *pcd, int pncols, int64_t np,
         if (g.func == FGPU_SUM || g.func == FGPU_AVG) {
             double t = 0.0;
             if (!isf && g.func == FGPU_SUM) {
+                // par_sum_i64: per 16,384-value chunk a checked i64 sum, or the f64
+                // lanes when it overflows; chunks added in f64 (:3087-3119)
                 for (int64_t s = 0; s < nv; s += 16384) {
+                    const int64_t e = std::min(nv, s + 16384);
                     __int128 acc = 0;
-                    for (int64_t i = s; i < std::min(nv, s + 16384); ++i) acc += ivals[i];
-                    t += (double)(int64_t)acc;
+                    for (int64_t i = s; i < e; ++i) acc += ivals[i];
+                    t += i128_fits(acc) ? (double)(int64_t)acc : ora_sum_lanes(vals.data() + s, e - s);
                 }
             } else {
                 for (int64_t s = 0; s < nv; s += 16384)
@@ -1194,6 +1291,10 @@ int ora_join(const fgpu_column_desc *pcd, int pncols, int64_t np,
         if (g.func == FGPU_COUNT) {
             oc.dtype = FGPU_INT64;
             oc.i64.assign(1, nv);
+        } else if (!isf && g.func == FGPU_SUM) {
+            __int128 s = 0;
+            for (int64_t x : ivals) s += x;
+            int_sum_column({s}, oc);
         } else if (isf || g.func == FGPU_AVG) {
             oc.dtype = FGPU_FLOAT64;
             double v = 0.0;
@@ -1203,8 +1304,8 @@ int ora_join(const fgpu_column_desc *pcd, int pncols, int64_t np,
                     for (double x : vals) es.add(x);
                     v = es.result();
                 } else {
-                    int64_t s = 0;
-                    for (int64_t x : ivals) s = (int64_t)((uint64_t)s + (uint64_t)x);
+                    __int128 s = 0;
+                    for (int64_t x : ivals) s += x;
                     v = (double)s;
                 }
                 if (g.func == FGPU_AVG) v = nv ? v / (double)nv : 0.0;
@@ -1216,10 +1317,8 @@ int ora_join(const fgpu_column_desc *pcd, int pncols, int64_t np,
             oc.f64.assign(1, v);
         } else {
             oc.dtype = FGPU_INT64;
-            int64_t v = g.func == FGPU_SUM ? 0 : (g.func == FGPU_MIN ? INT64_MAX : INT64_MIN);
-            for (int64_t x : ivals)
-                v = g.func == FGPU_SUM ? (int64_t)((uint64_t)v + (uint64_t)x)
-                                       : (g.func == FGPU_MIN ? std::min(v, x) : std::max(v, x));
+            int64_t v = g.func == FGPU_MIN ? INT64_MAX : INT64_MIN;
+            for (int64_t x : ivals) v = g.func == FGPU_MIN ? std::min(v, x) : std::max(v, x);
             oc.i64.assign(1, v);
         }
     }

@@ -150,3 +150,99 @@ def check_all(engine) -> int:
     sl, sr = side("k", g["self_join_keys"]), side("rk", g["self_join_keys"])
     assert scalar(engine.join(sl, sr, "k", "rk", INNER, [AggExpr(COUNT)])) == g["self_inner"]
     return checks + 1
+
+
+# ─── integer SUM past 2^53 and past int64 (src/query/aggregate.rs:1936-1993) ──
+I64_MAX = 2**63 - 1
+
+
+def _int_sum_cases():
+    """(name, values, exact integer total) for the reference's int_sum tests."""
+    g = GOLDEN["int_sum"]
+    out = []
+    for n in g["ordinary"]["rows"]:
+        v = [(i % 997) - 498 for i in range(n)]
+        out.append((f"ordinary_{n}", v, sum(v)))
+    v = [(1 << 53) + i for i in range(8)]
+    assert sum(v) == g["past_float_limit"]["exact"]
+    out.append(("past_float_limit", v, sum(v)))
+    v = list(g["overflow"]["data"])
+    out.append(("overflow", v, sum(v)))
+    out.append(("overflow_negative", [-x for x in v], -sum(v)))
+    # partial sums leave int64 but the total fits: the exact int64 comes back
+    v = [I64_MAX, I64_MAX, -I64_MAX, -I64_MAX, 5, -(2**63), 2**62, 2**62]
+    out.append(("wide_but_fits", v, sum(v)))
+    for n in g["ragged_tail"]["n"]:
+        v = list(range(1, n + 1))
+        out.append((f"ragged_{n}", v, n * (n + 1) // 2))
+    return out
+
+
+def _as_total(x):
+    """A returned SUM as an exact Python number (int for Int64, float for Float64)."""
+    return x.item() if hasattr(x, "item") else x
+
+
+def _check_total(name, got, exact):
+    """The exact total when it fits int64, else its correctly rounded double
+    (Python's int -> float conversion rounds half to even)."""
+    if -2**63 <= exact <= I64_MAX:
+        assert isinstance(got, int) and got == exact, (name, got, exact)
+        # the reference's Float64 result is the total as f64 (sum_i64)
+        assert float(got) == float(exact)
+    else:
+        assert isinstance(got, float) and got == float(exact), (name, got, exact)
+    if name.startswith("overflow"):
+        assert abs(got) > GOLDEN["int_sum"]["overflow"]["greater_than"], (name, got)
+        assert (got > 0) == (exact > 0), (name, got)
+
+
+def check_int_sum(engine, key_rows: int = 3) -> int:
+    """The int_sum known answers through every aggregate path `engine` offers:
+    scalar SUM / AVG, GROUP BY where one group holds the test vector and other
+    groups hold small values, and SUM over a join (probe side via SEMI, build
+    side via INNER).  Returns the number of checks."""
+    checks = 0
+    for name, vals, exact in _int_sum_cases():
+        v = np.array(vals, dtype=np.int64)
+        n = v.size
+        # scalar
+        r = engine.groupby({"v": v}, None, None, [AggExpr(SUM, "v"), AggExpr(COUNT)])
+        if n:
+            _check_total(name, _as_total(r.columns[0][0]), exact)
+        else:
+            assert r.validity[0] is not None and not r.validity[0][0], name  # SUM of nothing is NULL
+        checks += 1
+        if n:
+            r = engine.groupby({"v": v}, None, None, [AggExpr(AVG, "v")])
+            assert r.columns[0][0] == float(exact) / n, (name, r.columns[0][0])
+            checks += 1
+        # GROUP BY: key 0 holds the vector, keys 1..key_rows one small value each
+        k = np.concatenate([np.zeros(n, np.int64), np.arange(1, key_rows + 1, dtype=np.int64)])
+        w = np.concatenate([v, np.arange(1, key_rows + 1, dtype=np.int64) * 7])
+        for kname, kcol in (("int64 key", k), ("wide key", k * 1_000_003 + 5)):
+            r = engine.groupby({"k": kcol, "w": w}, None, "k", [AggExpr(SUM, "w"), AggExpr(COUNT)])
+            r = r.sort_by("k")
+            sums = [_as_total(x) for x in r.columns[1]]
+            if n:
+                _check_total(f"{name} / {kname}", sums[0], exact)
+                rest = sums[1:]
+            else:
+                rest = sums
+            overflowed = not (-2**63 <= exact <= I64_MAX)
+            for j, s in enumerate(rest):
+                # the column is Float64 when any group left int64
+                assert s == (float(7 * (j + 1)) if overflowed else 7 * (j + 1)), (name, kname, j, s)
+                assert isinstance(s, float) == overflowed, (name, kname, s)
+            checks += 1
+        if not n:
+            continue
+        # joins: SEMI sums the probe column; INNER sums the build column
+        probe = {"pk": np.arange(n, dtype=np.int64), "pv": v}
+        build = {"bk": np.arange(n, dtype=np.int64), "bv": v}
+        r = engine.join(probe, build, "pk", "bk", SEMI, [AggExpr(SUM, "pv")])
+        _check_total(f"{name} / semi", _as_total(r.columns[0][0]), exact)
+        r = engine.join(probe, build, "pk", "bk", INNER, [AggExpr(SUM, "bv", side="build")])
+        _check_total(f"{name} / inner", _as_total(r.columns[0][0]), exact)
+        checks += 2
+    return checks

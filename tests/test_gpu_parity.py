@@ -654,3 +654,73 @@ def test_groupby_partitioned_at_scale_matches_hash(ctx_gbp, ctx_gbh):
     assert plans[1] == "groupby:hash", plans[1]
     assert res[0].num_rows > 500_000
     assert_batches_equal(res[0], res[1], "key")
+
+
+# ─── register float sums near the top of the double range ───────────────────
+def _fraction_sum(x):
+    """The correctly rounded sum by exact rational arithmetic."""
+    from fractions import Fraction
+    v, c = np.unique(x, return_counts=True)
+    return float(sum(Fraction(float(a)) * int(n) for a, n in zip(v, c)))
+
+
+def _huge_column(n):
+    # +/-1.7e308 alternating: every thread of a strided loop sees one sign, so
+    # its partial sum would overflow without the power-of-two prescale; every
+    # 1000th pair is 1e300, and the column is balanced so that the exact total
+    # is one 1.7e308 plus the 1e300 terms
+    x = np.where(np.arange(n) % 2 == 0, 1.7e308, -1.7e308)
+    x[::1000] = 1e300
+    x[1::1000] = 1e300
+    extra = int((x == 1.7e308).sum() - (x == -1.7e308).sum()) - 1
+    idx = np.flatnonzero(x == (1.7e308 if extra > 0 else -1.7e308))[:abs(extra)]
+    x[idx] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("n", [10_000, 1_000_003])
+def test_scalar_float_sum_near_dbl_max(ctx, n):
+    x = _huge_column(n)
+    t = _table(ctx, {"x": x})
+    g = fa.execute_agg_only(ctx, t, [AggExpr(SUM, "x"), AggExpr(AVG, "x")])
+    exact = _fraction_sum(x)  # math.fsum raises on the overflowing partials
+    assert math.isfinite(exact)
+    assert g.columns[0][0] == exact == ora.exact_sum(x)
+    assert g.columns[1][0] == exact / n
+
+
+@pytest.mark.parametrize("jt", [INNER, SEMI])
+def test_join_float_sum_near_dbl_max(jctx, jt):
+    n = 300_000
+    x = _huge_column(n)
+    probe = {"pk": np.arange(n, dtype=np.int64), "px": x}
+    build = {"bk": np.arange(n, dtype=np.int64), "bx": x}
+    side = "build" if jt == INNER else "probe"
+    aggs = [AggExpr(SUM, "bx" if jt == INNER else "px", side=side), AggExpr(COUNT)]
+    g = fa.execute_hash_join(jctx, _table(jctx, probe), _table(jctx, build), "pk", "bk", jt, aggs)
+    assert g.columns[0][0] == _fraction_sum(x)
+    assert g.columns[1][0] == n
+
+
+# ─── MIN / MAX over NaN ──────────────────────────────────────────────────────
+def test_float_minmax_skip_nan_even_when_first(ctx, ctx_gbp, ctx_gbh):
+    # A group whose first value is NaN: MIN / MAX skip NaN on every path, the
+    # way the reference's execute_agg_only does (f64::min / f64::max); the
+    # reference's GROUP BY fast path would keep the leading NaN (its
+    # `v < state.sum` never replaces it), which the oracle's reference-order
+    # columns restate.
+    k = np.array([0, 0, 0, 1, 1, 2, 2], np.int64)
+    x = np.array([np.nan, 1.0, -2.0, 3.0, np.nan, np.nan, np.nan])
+    cols = {"k": k, "x": x}
+    aggs = [AggExpr(MIN, "x"), AggExpr(MAX, "x"), AggExpr(COUNT, "x")]
+    o = ora.groupby(cols, None, "k", aggs)
+    exp = o.exact.sort_by("k")
+    assert list(exp.columns[1][:2]) == [-2.0, 3.0] and list(exp.columns[2][:2]) == [1.0, 3.0]
+    assert not exp.validity[1][2] and not exp.validity[2][2]  # all-NaN group: NULL
+    ref = o.reference.sort_by("k")
+    assert math.isnan(ref.columns[1][0]) and math.isnan(ref.columns[2][0])
+    for c in (ctx, ctx_gbp, ctx_gbh):
+        t = _table(c, cols)
+        assert_batches_equal(fa.GroupByExec(["k"], aggs).execute(c, t), o.exact, "k")
+        s = fa.execute_agg_only(c, t, aggs)
+        assert s.columns[0][0] == -2.0 and s.columns[1][0] == 3.0

@@ -12,7 +12,14 @@ Parity status of the oracle's exported functions:
       outside the Rust binary can confirm
   ora_join                             pinned: reference known answers
       (src/query/join.rs:1747-1848, :2607-2676) + nested-loop brute force
-  ora_exact_sum                        pinned: math.fsum, bit for bit
+  ora_exact_sum                        pinned: math.fsum, bit for bit, and exact
+      rational sums (fractions.Fraction) over subnormals, ties and partial sums
+      beyond the double range (where math.fsum raises)
+  ora_sum_i64 / ora_checked_sum_i64    pinned: the reference's int_sum known
+      answers (src/query/aggregate.rs:1936-1993)
+  exact integer SUM columns            pinned: int_sum known answers through
+      scalar, GROUP BY and join (exact int64, or the correctly rounded double
+      past int64, checked against Python's exact integers)
   concat_dicts (Python)                pinned: closed form of the merged dictionary
       (ordered union of the batches' dictionaries), every row decoding to the
       same string, the shared-dictionary fast path, and a planted row-order
@@ -28,7 +35,7 @@ import oracle as ora
 from falcondb_amd._abi import ANTI, AVG, COUNT, INNER, LEFT, MAX, MIN, NO_ROW, SEMI, SUM
 from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
 
-from known_answers import check_all
+from known_answers import GOLDEN, check_all, check_int_sum
 
 
 class OracleEngine:
@@ -44,6 +51,33 @@ class OracleEngine:
 
 def test_reference_known_answers():
     assert check_all(OracleEngine()) > 100
+
+
+def test_sum_i64_restatement_known_answers():
+    # aggregate.rs:1936-1993, the reference's int_sum tests, on the oracle's
+    # restatement of sum_i64 / checked_sum_i64 / sum_lanes
+    g = GOLDEN["int_sum"]
+    for n in g["ordinary"]["rows"]:
+        d = np.array([(i % 997) - 498 for i in range(n)], np.int64)
+        assert ora.sum_i64(d) == ora.sum_lanes(d.astype(np.float64)) == float(sum(d.tolist()))
+    d = np.array([(1 << 53) + i for i in range(8)], np.int64)
+    exact = g["past_float_limit"]["exact"]
+    new, old = ora.sum_i64(d), ora.sum_lanes(d.astype(np.float64))
+    assert int(new) == int(float(exact))  # the correctly rounded total
+    assert abs(int(new) - exact) < abs(int(old) - exact)
+    d = np.array(g["overflow"]["data"], np.int64)
+    assert ora.checked_sum_i64(d) is None
+    got = ora.sum_i64(d)
+    assert got == ora.sum_lanes(d.astype(np.float64)) and got > g["overflow"]["greater_than"]
+    for n in g["ragged_tail"]["n"]:
+        assert ora.sum_i64(np.arange(1, n + 1, dtype=np.int64)) == n * (n + 1) // 2
+    # a planted wrapping sum is caught: it gives the wrapped (negative) total
+    wrapped = int(np.sum(d, dtype=np.int64))
+    assert not wrapped > g["overflow"]["greater_than"]
+
+
+def test_int_sum_known_answers_every_path():
+    assert check_int_sum(OracleEngine()) > 100
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -300,3 +334,65 @@ def test_concat_dicts_planted_row_order_numbering_is_caught():
     merged = list(dict.fromkeys(merged))
     assert merged != list(dict.fromkeys(s for d in dicts for s in d))
     assert ora.concat_dicts(codes, rows, dicts).dictionary != merged
+
+
+def _fraction_sum(x):
+    """The correctly rounded sum by exact rational arithmetic."""
+    from fractions import Fraction
+    v, c = np.unique(x, return_counts=True)
+    return float(sum(Fraction(float(a)) * int(n) for a, n in zip(v, c)))
+
+
+def _huge_column(n):
+    """+/-1.7e308 alternating (a strided per-thread partial overflows), every
+    1000th pair 1e300, balanced so that the total is 1.7e308 plus the 1e300s."""
+    x = np.where(np.arange(n) % 2 == 0, 1.7e308, -1.7e308)
+    x[::1000] = 1e300
+    x[1::1000] = 1e300
+    extra = int((x == 1.7e308).sum() - (x == -1.7e308).sum()) - 1
+    idx = np.flatnonzero(x == (1.7e308 if extra > 0 else -1.7e308))[:abs(extra)]
+    x[idx] = 0.0
+    return x
+
+
+def test_float_minmax_nan_semantics():
+    # Exact columns skip NaN in MIN / MAX, as the reference's execute_agg_only
+    # does (f64::min / f64::max, src/query/aggregate.rs:3296-3330); the
+    # reference-order columns restate the GROUP BY fast path, whose
+    # `!has_val || v < sum` keeps a NaN that comes first (:1093-1101).
+    k = np.array([0, 0, 0, 1, 1], np.int64)
+    x = np.array([np.nan, 1.0, -2.0, 3.0, np.nan])
+    o = ora.groupby({"k": k, "x": x}, None, "k", [AggExpr(MIN, "x"), AggExpr(MAX, "x")])
+    e, r = o.exact.sort_by("k"), o.reference.sort_by("k")
+    assert list(e.columns[1]) == [-2.0, 3.0] and list(e.columns[2]) == [1.0, 3.0]
+    assert math.isnan(r.columns[1][0]) and math.isnan(r.columns[2][0])
+    assert list(r.columns[1][1:]) == [3.0] and list(r.columns[2][1:]) == [3.0]
+    s = ora.groupby({"x": x}, None, None, [AggExpr(MIN, "x"), AggExpr(MAX, "x")]).exact
+    assert [s.columns[0][0], s.columns[1][0]] == [-2.0, 3.0]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_exact_sum_is_the_rational_sum(seed):
+    from fractions import Fraction
+    rng = np.random.default_rng(50 + seed)
+    for t in range(400):
+        n = int(rng.integers(1, 40))
+        x = rng.normal(size=n) * 10.0 ** rng.integers(-320, 300, size=n)
+        if t % 3 == 0:
+            x = np.concatenate([x, -x[: n // 2]])
+        if t % 7 == 0:
+            x = np.concatenate([x, [5e-324, -1e-310, 2.2250738585072014e-308]])
+        assert ora.exact_sum(x) == float(sum(Fraction(float(v)) for v in x))
+    # round half to even at the last place, and a sticky bit far below it
+    assert ora.exact_sum(np.array([1.0, 2.0 ** -53])) == 1.0
+    assert ora.exact_sum(np.array([1.0, 2.0 ** -53, 2.0 ** -1000])) == 1.0 + 2 ** -52
+    assert ora.exact_sum(np.array([1.0 + 2 ** -52, 2.0 ** -53])) == 1.0 + 2 ** -51
+    assert ora.exact_sum(np.array([1.7976931348623157e308, 1e292])) == math.inf
+
+
+def test_exact_sum_near_dbl_max():
+    x = _huge_column(10_000)
+    exact = _fraction_sum(x)  # math.fsum raises on the overflowing partials
+    assert math.isfinite(exact) and ora.exact_sum(x) == exact
+    r = ora.groupby({"x": x}, None, None, [AggExpr(SUM, "x")]).exact
+    assert r.columns[0][0] == exact

@@ -17,6 +17,8 @@ ERROR_NAMES = {
     9: "ConstraintViolation", 100: "Device", 101: "OutOfMemory", 102: "Unsupported",
 }
 
+E_INDEX_OUT_OF_BOUNDS, E_UNSUPPORTED = 3, 102
+
 INT32, INT64, FLOAT64, UTF8 = 1, 2, 3, 4
 EQ, NE, LT, LE, GT, GE = range(6)
 P_CMP, P_AND, P_OR, P_IS_NULL, P_IS_NOT_NULL = range(5)
@@ -126,6 +128,9 @@ DECLARED = {
                                      C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_void_p)]),
     "fgpu_groupby": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(GroupBySpec),
                                C.POINTER(C.c_void_p)]),
+    "fgpu_groupby_static": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(GroupBySpec),
+                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p), C.c_int64, C.POINTER(C.c_int64)]),
     "fgpu_join_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
     "fgpu_join_index_free": (None, [C.c_void_p]),
     "fgpu_join_index_device_bytes": (C.c_int64, [C.c_void_p]),

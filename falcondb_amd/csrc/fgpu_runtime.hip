@@ -19,6 +19,41 @@
 
 using namespace fgpu;
 
+// ─── static scratch of fgpu_groupby_static ──────────────────────────────────
+// fgpu_groupby_static keeps every buffer its kernels need beyond the inputs
+// and outputs (counters, the output row count, Utf8 predicate pass tables and
+// the group state table) in this array, so that it allocates nothing.  Each
+// device holds its own copy for the life of the process.  In host code the
+// name `scratch` is a stand-in, not a device address: host code only ever
+// uses the address hipGetSymbolAddress returns for the current device
+// (g_static[device].ptr, looked up once per device by fgpu_open).
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(16) << 20;
+static __device__ __align__(256) uint64_t scratch[STATIC_SCRATCH_BYTES / 8];
+
+namespace {
+constexpr int STATIC_MAX_DEVICES = 64;
+struct StaticScratch {
+    void *ptr = nullptr;       // this device's copy of `scratch`
+    hipError_t lookup = hipSuccess;
+    bool looked_up = false;
+    std::mutex use;            // held for a whole fgpu_groupby_static call
+};
+StaticScratch g_static[STATIC_MAX_DEVICES];
+std::mutex g_static_init;
+
+// Looks up the device address of `scratch` for `device` (current) once.
+void static_scratch_init(int device) {
+    if (device < 0 || device >= STATIC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_static_init);
+    StaticScratch &st = g_static[device];
+    if (st.looked_up) return;
+    void *ptr = nullptr;
+    st.lookup = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    st.ptr = st.lookup == hipSuccess ? ptr : nullptr;
+    st.looked_up = true;
+}
+}  // namespace
+
 // ─── errors ─────────────────────────────────────────────────────────────────
 namespace {
 thread_local std::string g_err;
@@ -380,8 +415,12 @@ struct Binder {
     }
 };
 
-// Compiles a predicate (postfix program) into device form.
-int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b, DPred &out) {
+// Compiles a predicate (postfix program) into device form.  Utf8 pass tables
+// go to the context's arena, or, when pass_dst is given, to pass_dst (at most
+// pass_cap bytes; *pass_used receives the bytes taken).
+int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b, DPred &out,
+                 uint8_t *pass_dst = nullptr, size_t pass_cap = 0, size_t *pass_used = nullptr) {
+    if (pass_used) *pass_used = 0;
     out = DPred{};
     out.n = 0;
     out.n_pcols = 0;
@@ -398,7 +437,13 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
     }
     uint8_t *d_pass = nullptr;
     std::vector<uint8_t> h_pass(pass_bytes);
-    if (pass_bytes) {
+    if (pass_bytes && pass_dst) {
+        if (pass_bytes > pass_cap)
+            return set_err(FGPU_E_UNSUPPORTED, "predicate pass tables need %zu bytes (%zu available)", pass_bytes,
+                           pass_cap);
+        d_pass = pass_dst;
+        if (pass_used) *pass_used = pass_bytes;
+    } else if (pass_bytes) {
         int rc = buf(c, B_PASS, pass_bytes, (void **)&d_pass);
         if (rc) return rc;
     }
@@ -831,6 +876,7 @@ int fgpu_open(int device, fgpu_ctx **out) {
         return set_err(FGPU_E_OUT_OF_MEMORY, "pinned host memory for the context");
     }
     for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+    static_scratch_init(device);
     Tuning &t = c->tune;
     t.gb_block = env_int("FGPU_GB_BLOCK", t.gb_block);
     t.gb_waves_per_cu = env_int("FGPU_GB_WAVES", t.gb_waves_per_cu);
@@ -1545,21 +1591,19 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     return FGPU_OK;
 }
 
-int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
-                 const fgpu_groupby_spec *spec, fgpu_result **out) {
-    if (!c || !t || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
-        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (int rc = set_device(c->device)) return rc;
+// Resolves a GROUP BY call: the key column, the compiled predicate (Utf8 pass
+// tables in the arena, or at pass_dst when given), the bound columns and the
+// aggregate state layout.
+static int gb_prepare(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const fgpu_groupby_spec *spec,
+                      Binder &b, GbArgs &a, AggLayout &L, const DevCol *&key, uint8_t *pass_dst = nullptr,
+                      size_t pass_cap = 0, size_t *pass_used = nullptr) {
     const int kc = spec->key_col;
     if (kc >= (int)t->cols.size()) return set_err(FGPU_E_COLUMN_NOT_FOUND, "group key column %d not in table", kc);
-    const DevCol *key = kc >= 0 ? &t->cols[kc] : nullptr;
+    key = kc >= 0 ? &t->cols[kc] : nullptr;
     if (key && key->dtype == FGPU_FLOAT64)
         return set_err(FGPU_E_INVALID_QUERY, "GROUP BY on a Float64 column is not supported");
-    Binder b(t);
     DPred dp;
-    if (int rc = compile_pred(c, t, p, b, dp)) return rc;
-    GbArgs a{};
+    if (int rc = compile_pred(c, t, p, b, dp, pass_dst, pass_cap, pass_used)) return rc;
     a.pred = dp;
     a.key_col = key ? b.bind(kc) : -1;
     if (key && a.key_col < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads too many columns");
@@ -1577,16 +1621,22 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
         if (col < 256) in_of_col[col] = a.n_in;
         return a.n_in++;
     };
-    AggLayout L;
     if (int rc = build_layout(spec->aggs, spec->n_aggs, t, nullptr, slot_of, L))
         return too_many ? set_err(FGPU_E_UNSUPPORTED, "at most %d distinct aggregate input columns", MAX_IN) : rc;
     a.cols = b.dc;
     a.ops = L.ops;
     a.n_rows = t->n_rows;
+    return FGPU_OK;
+}
+
+// Picks the GROUP BY strategy from the key's statistics: scalar registers (no
+// key), a dense state table in LDS or in HBM (slot = key - key_min; Utf8 keys
+// by dictionary code), or a hash table.  Fills a.key_min / range / key_is_utf8
+// / utf8_null_code and the state rows a dense or scalar mode needs.
+static int gb_pick_mode(const fgpu_ctx *c, const DevCol *key, const AggLayout &L, GbArgs &a,
+                        int64_t &n_state_groups) {
     const int S = L.ops.S;
-    // strategy
     int mode;
-    int64_t n_state_groups;
     if (!key && L.ops.n <= 8) {  // register accumulators of the scalar kernel
         mode = GB_SCALAR;
         a.range = 0;
@@ -1624,6 +1674,23 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
             n_state_groups = 0;
         }
     }
+    return mode;
+}
+
+int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
+                 const fgpu_groupby_spec *spec, fgpu_result **out) {
+    if (!c || !t || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    const DevCol *key = nullptr;
+    Binder b(t);
+    GbArgs a{};
+    AggLayout L;
+    if (int rc = gb_prepare(c, t, p, spec, b, a, L, key)) return rc;
+    const int S = L.ops.S;
+    int64_t n_state_groups;
+    const int mode = gb_pick_mode(c, key, L, a, n_state_groups);
     c->plan.clear();
     if (mode == GB_HASH && !a.key_is_utf8) {
         const int prc = groupby_partitioned(c, t, key, a, L, b, out);
@@ -1711,6 +1778,94 @@ int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
         return FGPU_OK;
     }
     return set_err(FGPU_E_OUT_OF_MEMORY, "group table kept overflowing (last capacity %lld)", (long long)cap);
+}
+
+int fgpu_groupby_static(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const fgpu_groupby_spec *spec,
+                        int64_t *out_key, uint8_t *out_key_valid, int64_t *const *out_vals,
+                        uint8_t *const *out_valid, int64_t capacity, int64_t *n_out) {
+    if (!c || !t || !spec || !n_out || (spec->n_aggs > 0 && (!spec->aggs || !out_vals || !out_valid)))
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    *n_out = 0;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    if (c->device >= STATIC_MAX_DEVICES) return set_err(FGPU_E_UNSUPPORTED, "device %d has no static scratch", c->device);
+    StaticScratch &st = g_static[c->device];
+    if (!st.ptr)
+        return set_err(FGPU_E_DEVICE, "static scratch of device %d: %s", c->device,
+                       st.looked_up ? hipGetErrorString(st.lookup) : "not looked up");
+    // One call at a time per device: every call shares this device's array.
+    std::lock_guard<std::mutex> use(st.use);
+    // scratch layout: [0, 512) counters, [512, 520) output row count,
+    // [4096, 4096 + pass) Utf8 pass tables, then the state table (256-aligned)
+    uint8_t *base = (uint8_t *)st.ptr;
+    unsigned long long *d_cnt = (unsigned long long *)base;
+    unsigned long long *d_rows = (unsigned long long *)(base + 512);
+    const size_t pass_off = 4096;
+    size_t pass_used = 0;
+    const DevCol *key = nullptr;
+    Binder b(t);
+    GbArgs a{};
+    AggLayout L;
+    if (int rc = gb_prepare(c, t, p, spec, b, a, L, key, base + pass_off, STATIC_SCRATCH_BYTES - pass_off,
+                            &pass_used))
+        return rc;
+    const int S = L.ops.S;
+    int64_t n_state_groups;
+    const int mode = gb_pick_mode(c, key, L, a, n_state_groups);
+    if (mode == GB_HASH)
+        return set_err(FGPU_E_UNSUPPORTED, "key '%s' spans %lld values: a hash GROUP BY allocates (use fgpu_groupby)",
+                       key->name.c_str(), (long long)a.range);
+    const size_t state_off = (pass_off + pass_used + 255) & ~size_t(255);
+    const size_t state_bytes = (size_t)n_state_groups * S * 8;
+    if (state_off + state_bytes > STATIC_SCRATCH_BYTES)
+        return set_err(FGPU_E_UNSUPPORTED, "group state of %lld slots x %d words needs %zu bytes; the static scratch "
+                       "holds %zu", (long long)n_state_groups, S, state_off + state_bytes, STATIC_SCRATCH_BYTES);
+    const int64_t max_rows = key ? n_state_groups : 1;
+    if (capacity < max_rows) {
+        *n_out = max_rows;
+        return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "output capacity %lld below the %lld rows this GROUP BY may emit",
+                       (long long)capacity, (long long)max_rows);
+    }
+    if (key && (!out_key || !out_key_valid)) return set_err(FGPU_E_INVALID_QUERY, "NULL key output");
+    for (int i = 0; i < spec->n_aggs; ++i)
+        if (!out_vals[i] || !out_valid[i]) return set_err(FGPU_E_INVALID_QUERY, "NULL output of aggregate %d", i);
+    uint64_t *d_state = (uint64_t *)(base + state_off);
+    a.keys = nullptr;
+    a.state = d_state;
+    a.cap_mask = 0;
+    a.counters = d_cnt;
+    // the array keeps the last call's contents: clear counters and the row count
+    HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_cnt, 65, 0));
+    HIPCHK(launch_state_init(c->stream, nullptr, 0, d_state, n_state_groups, L.ops));
+    const int block = mode == GB_SCALAR ? 256 : c->tune.gb_block;
+    const size_t lds = mode == GB_DENSE_LDS ? (size_t)n_state_groups * S * 8 : 0;
+    const int grid = grid_for(c, (t->n_rows + 3) / 4, block, c->tune.gb_waves_per_cu);
+    if (t->n_rows) HIPCHK(launch_groupby(c->stream, mode, a, grid, block, lds));
+    EmitArgs e{};
+    e.outs = L.outs;
+    e.keys = nullptr;
+    e.state = d_state;
+    e.S = S;
+    e.n_slots = n_state_groups;
+    e.mode = key ? mode : GB_SCALAR;
+    e.key_min = a.key_min;
+    e.key_is_utf8 = a.key_is_utf8;
+    e.out_key = key ? out_key : nullptr;
+    e.out_key_valid = key ? out_key_valid : nullptr;
+    for (int i = 0; i < FGPU_MAX_AGGS; ++i) {
+        e.out_val[i] = i < spec->n_aggs ? out_vals[i] : nullptr;
+        e.out_valid[i] = i < spec->n_aggs ? out_valid[i] : nullptr;
+    }
+    e.out_count = d_rows;
+    HIPCHK(launch_emit(c->stream, e));
+    unsigned long long rows = 0;
+    HIPCHK(mail_copy(c, d_rows, sizeof rows));
+    HIPCHK(mail_take(c, &rows, sizeof rows));
+    if ((int64_t)rows > max_rows) return set_err(FGPU_E_DEVICE, "emit counter %llu exceeds %lld", rows, (long long)max_rows);
+    *n_out = (int64_t)rows;
+    c->plan = std::string("groupby_static:") + (mode == GB_SCALAR ? "scalar" : mode == GB_DENSE_LDS ? "dense_lds"
+                                                                                                  : "dense_global");
+    return FGPU_OK;
 }
 
 // ─── hash join ──────────────────────────────────────────────────────────────

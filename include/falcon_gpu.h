@@ -228,6 +228,31 @@ int fgpu_filter_select(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
 int fgpu_groupby(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
                  const fgpu_groupby_spec *spec, fgpu_result **out);
 
+/* GROUP BY that allocates nothing.  No counterpart in the reference: this is
+ * for callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result as fgpu_groupby for the strategies that need a bounded state:
+ * no GROUP BY, or a key whose range takes a dense table (Utf8 keys by
+ * dictionary code).  Every scratch buffer (counters, the row count, Utf8
+ * predicate pass tables, the group state) lives in a static device array of
+ * FGPU_STATIC_SCRATCH_BYTES per device; a larger state, or a key that needs a
+ * hash table, fails with FGPU_E_UNSUPPORTED.  The outputs are the caller's
+ * DEVICE buffers of `capacity` rows: out_key / out_key_valid (int64 keys,
+ * Utf8 keys as codes; unused without a key) and, per aggregate i,
+ * out_vals[i] (int64, or the bits of a double for SUM / AVG / MIN / MAX of a
+ * Float64 column and AVG of an integer column) and out_valid[i] (1 = valid,
+ * 0 = NULL; 2 marks an integer SUM whose total left int64, its value then
+ * being the bits of the correctly rounded double).  If `capacity` is below
+ * the rows the call may emit, *n_out receives that bound and
+ * FGPU_E_INDEX_OUT_OF_BOUNDS is returned.  Calls on one device share its
+ * array and are serialised by the library; the call returns with its stream
+ * synchronised and *n_out set to the rows written. */
+#define FGPU_STATIC_SCRATCH_BYTES (16 << 20)
+int fgpu_groupby_static(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
+                        const fgpu_groupby_spec *spec, int64_t *out_key, uint8_t *out_key_valid,
+                        int64_t *const *out_vals, uint8_t *const *out_valid, int64_t capacity,
+                        int64_t *n_out);
+
 /* ── hash join ───────────────────────────────────────────────────────────── */
 /* Builds a device hash index over `key_col` of the build table
  * (build_hash_table / IntHashTable::insert, src/query/join.rs:607-645,

@@ -265,6 +265,114 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
             "parity_vs_alone": bool(same), "served": served, "plain_one_slot": plain}
 
 
+def cpu_info():
+    """CPU model, logical CPUs and the oracle's thread count on this host."""
+    import oracle as ora
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "threads": ora.threads()}
+
+
+def timed_runs(fn, runs):
+    """Median and min wall time (s) of `runs` calls after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), float(np.min(ts))
+
+
+def cpu_baselines(c2, K, gpu_c2, aggs, runs=5, c3_probe_rows=20_000_000):
+    """The reference's CPU operators (the oracle's restatement, on this host's
+    cores): C1 at 1M rows and C2 at full size, median (and min) of `runs`
+    runs each; C3 INNER and SEMI over the first `c3_probe_rows` probe rows
+    against the full 1M-row build side.  Also checks every output column of
+    this run's GPU C2 result against the oracle's exact result."""
+    import oracle as ora
+    from falcondb_amd import datagen as dg
+    from falcondb_amd._abi import AVG, COUNT, INNER, SEMI, SUM
+    from falcondb_amd.operators import AggExpr, DictColumn, col
+    info = cpu_info()
+    out = {"host": info, "runs": runs}
+    c1 = dg.c1_table(1_000_000)
+    c1cols = {"id": c1["id"], "dept": DictColumn(c1["dept"], dg.DEPTS), "score": c1["score"],
+              "salary": c1["salary"]}
+    c1aggs = [AggExpr(COUNT), AggExpr(SUM, "score"), AggExpr(AVG, "salary")]
+    med, mn = timed_runs(lambda: ora.groupby(c1cols, col("score") > 50, "dept", c1aggs, exact=False), runs)
+    out["c1"] = {"rows": 1_000_000, "median_s": med, "min_s": mn, "rows_per_s": 1_000_000 / med}
+    n = len(c2["key"])
+    med, mn = timed_runs(lambda: ora.groupby(c2, col("f") < 40, "key", aggs, exact=False), runs)
+    out["c2"] = {"rows": n, "median_s": med, "min_s": mn, "rows_per_s": n / med}
+    # parity of this run: every column of the GPU result against the exact oracle
+    e = ora.groupby(c2, col("f") < 40, "key", aggs).exact.sort_by("key")
+    g = gpu_c2.sort_by("key")
+    cols_ok = {}
+    for i, name in enumerate(e.names):
+        a, b = g.columns[i], e.columns[i]
+        va = g.validity[i] if g.validity[i] is not None else np.ones(a.size, bool)
+        vb = e.validity[i] if e.validity[i] is not None else np.ones(b.size, bool)
+        same = g.num_rows == e.num_rows and np.array_equal(va, vb)
+        if same and e.dtypes[i] == "Float64":
+            same = np.array_equal(a[va].astype(np.float64).view(np.int64), b[vb].view(np.int64))
+        elif same:
+            same = np.array_equal(a[va], b[vb].astype(a.dtype))
+        cols_ok[name] = bool(same)
+    out["c2_parity_by_column"] = cols_ok
+    Kb = dg.keys(C3_BUILD, 40, seed_=dg.seed(3, 15))
+    bc = dg.c3_build(Kb)
+    pc = dg.c3_probe(c3_probe_rows, Kb)
+    for name, jt, jaggs in (("inner", INNER, [AggExpr(SUM, "bpay", side="build"), AggExpr(COUNT)]),
+                            ("semi", SEMI, [AggExpr(SUM, "ppay"), AggExpr(COUNT)])):
+        med, mn = timed_runs(lambda: ora.join(pc, bc, "pk", "bk", jt, jaggs, exact=False), runs)
+        out[f"c3_{name}"] = {"probe_rows": c3_probe_rows, "build_rows": C3_BUILD, "median_s": med, "min_s": mn,
+                             "rows_per_s": (c3_probe_rows + C3_BUILD) / med}
+    log(f"[bench] CPU ({info['threads']} threads, {info['cpu_model']}): C1 {out['c1']['rows_per_s'] / 1e6:.0f} M rows/s, "
+        f"C2 {out['c2']['rows_per_s'] / 1e6:.0f} M rows/s, C3 inner {out['c3_inner']['rows_per_s'] / 1e6:.0f} "
+        f"semi {out['c3_semi']['rows_per_s'] / 1e6:.0f} M rows/s; C2 parity {cols_ok}")
+    return out
+
+
+def bench_c1(ctx, reps=50):
+    """C1 on the GPU: SELECT dept, COUNT(*), SUM(score), AVG(salary) FROM t
+    WHERE score > 50 GROUP BY dept over 1M rows.  Latency is the whole call
+    with the 12 result rows copied back; the kernel time comes from HIP events
+    on the context's stream."""
+    import falcondb_amd as fa
+    from falcondb_amd import datagen as dg
+    from falcondb_amd._abi import AVG, COUNT, SUM
+    from falcondb_amd.operators import AggExpr, DictColumn, col
+    c1 = dg.c1_table(1_000_000)
+    t = fa.Table(ctx, {"id": c1["id"], "dept": DictColumn(c1["dept"], dg.DEPTS), "score": c1["score"],
+                       "salary": c1["salary"]})
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "score"), AggExpr(AVG, "salary")]
+    gb = fa.GroupByExec(["dept"], aggs)
+    ctx.set_tuning("keep_on_device", 0)  # the caller restores it
+    for _ in range(5):
+        r = gb.execute(ctx, t, col("score") > 50)
+    lat, kern = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = gb.execute(ctx, t, col("score") > 50)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        kern.append(ctx.last_timing()["total_ms"])
+    med = float(np.median(lat))
+    out = {"rows": 1_000_000, "groups": int(r.num_rows), "latency_ms_p50": med,
+           "latency_ms_min": float(np.min(lat)), "kernels_ms": float(np.median(kern)),
+           "rows_per_s": 1_000_000 / (med * 1e-3), "plan": ctx.last_plan()}
+    log(f"[bench] C1: {med:.3f} ms per query with copy-back ({out['kernels_ms']:.3f} ms of kernels), "
+        f"{out['rows_per_s'] / 1e9:.2f} G rows/s")
+    t.free()
+    return out
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_c2_groupby_100M_and_c3_join.txt"
 
 
@@ -313,7 +421,6 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--qps", default="100,300,1000,3000")
-    ap.add_argument("--cpu-sample-rows", type=int, default=C2_ROWS)
     ap.add_argument("--ceiling", action="store_true",
                     help="only run the STREAM-like read probe (the scan's four columns, its tiling)")
     ap.add_argument("--calibrate", action="store_true",
@@ -413,6 +520,8 @@ def main():
 
     secondary = {"c2_sql": {"statement": c2_sql, "sql_ms": sql_ms, "operator_ms": op_ms,
                             "parse_plan_ms": sql_ms - op_ms}}
+    secondary["c1"] = bench_c1(ctx)
+    ctx.set_tuning("keep_on_device", 1)
     if not args.no_join:
         del c2
         Kb = dg.keys(C3_BUILD, 40, seed_=dg.seed(3, 15))
@@ -447,11 +556,29 @@ def main():
                 "kernels_ms": km, "achieved_GBs": jb / (km * 1e-3) / 1e9,
                 "frac": jb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS}
             log(f"[bench] join {name}: {jwall * 1e3:.3f} ms/step, partition + probe kernels {km:.3f} ms")
-        # index build time
+        # index build time, and the cold join: no prebuilt index, so the call
+        # builds its index and partitioned sub-tables per query, as the
+        # reference builds its hash table per query (src/query/join.rs:607-645)
         t0 = time.perf_counter()
         idx2 = fa.JoinIndex(ctx, bt, "bk")
         secondary["join_build_ms"] = (time.perf_counter() - t0) * 1e3
         idx2.free()
+        jaggs = [AggExpr(SUM, "bpay", side="build"), AggExpr(COUNT)]
+        jarr, jkeep = _aggs_struct(jaggs, lambda side, nm: (bt if side == 1 else pt).column_index(nm))
+        jspec = _abi.JoinSpec(INNER, pt.column_index("pk"), bt.column_index("bk"), len(jaggs),
+                              C.cast(jarr, C.POINTER(_abi.Agg)))
+
+        class _NoIndex:
+            handle = None
+        for _ in range(2):
+            join_call(lib, ctx, pt, bt, _NoIndex, jspec)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            join_call(lib, ctx, pt, bt, _NoIndex, jspec)
+        cold = (time.perf_counter() - t0) / 5
+        secondary["join_inner_cold"] = {"ms_per_step": cold * 1e3, "rows_per_s": (C3_PROBE + C3_BUILD) / cold,
+                                        "plan": ctx.last_plan()}
+        log(f"[bench] join inner, cold (index and sub-tables built per query): {cold * 1e3:.3f} ms/step")
         pt.free()
         bt.free()
         del pc, bc
@@ -468,25 +595,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        import oracle as ora
-        sample = min(args.cpu_sample_rows, args.rows)
         c2 = dg.c2_table(args.rows, K) if "c2" not in locals() else c2
-        cols = {k: v[:sample] for k, v in c2.items()}
-        threads = ora.threads()
-        t0 = time.perf_counter()
-        o = ora.groupby(cols, col("f") < 40, "key", aggs, exact=False)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": sample / cpu_s, "unit": "rows/s", "cores": threads, "kind": "port",
-               "sample": f"first {sample} rows of the same table, one run of the restated "
-                         f"reference operators ({cpu_s:.2f} s)"}
-        log(f"[bench] CPU oracle: {cpu_s:.2f} s on {threads} threads")
-        if sample == args.rows:  # parity of this very run (integers exact, by value)
-            e = o.reference.sort_by("key")
-            g = gpu_res.sort_by("key")
-            same = (g.num_rows == e.num_rows and np.array_equal(g.columns[0], e.columns[0])
-                    and np.array_equal(g.columns[1], e.columns[1])
-                    and np.array_equal(g.columns[2].astype(np.float64), e.columns[2]))
-            secondary["parity_vs_cpu"] = bool(same)
+        base = cpu_baselines(c2, K, gpu_res, aggs)
+        secondary["cpu_baselines"] = base
+        info = base["host"]
+        cpu = {"value": base["c2"]["rows_per_s"], "unit": "rows/s", "cores": info["threads"], "kind": "port",
+               "sample": f"the full {args.rows}-row C2 query, median of {base['runs']} runs of the oracle's "
+                         f"restatement of the reference operators ({base['c2']['median_s']:.2f} s; min "
+                         f"{base['c2']['min_s']:.2f} s) on {info['threads']} threads of {info['nproc']} "
+                         f"({info['cpu_model']})",
+               "median_s": base["c2"]["median_s"], "min_s": base["c2"]["min_s"],
+               "cpu_model": info["cpu_model"], "nproc": info["nproc"]}
+        secondary["parity_vs_cpu"] = all(base["c2_parity_by_column"].values())
 
     out = {
         "metric": METRIC, "value": rows_per_s, "unit": "rows/s", "n_gpus": 1,

@@ -131,6 +131,11 @@ DECLARED = {
     "fgpu_groupby_static": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(GroupBySpec),
                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p), C.c_int64, C.POINTER(C.c_int64)]),
+    "fgpu_join_materialize": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.c_void_p, C.c_void_p,
+                                        C.POINTER(JoinSpec), C.POINTER(C.c_int32), C.c_int,
+                                        C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_void_p)]),
+    "fgpu_table_cols": (C.c_int, [C.c_void_p]),
+    "fgpu_table_column_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ColumnView)]),
     "fgpu_join_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
     "fgpu_join_index_free": (None, [C.c_void_p]),
     "fgpu_join_index_device_bytes": (C.c_int64, [C.c_void_p]),

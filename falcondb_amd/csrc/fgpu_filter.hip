@@ -341,6 +341,45 @@ hipError_t launch_filter_mask(hipStream_t s, const DCols &c, const DPred &p, int
     else hipLaunchKernelGGL(k_filter_mask<MAX_PCOLS>, dim3(grid), dim3(BLOCK), 0, s, c, p, n, d_words);
     return hipGetLastError();
 }
+// Gathers a column by row index (a materialised join): dst[i] = src[idx[i]]
+// for 4- or 8-byte values; row i is NULL where idx[i] is NO_ROW (the
+// NULL-extended side of a LEFT join) or the source row is NULL.  Each wave
+// takes 64 consecutive rows, so the output validity word of those rows is one
+// __ballot (dst_valid may be null when no row can be NULL).  A Utf8 column
+// (utf8 != 0) writes the NULL code for its NULL rows.
+__global__ void k_gather_idx(const void *src, int elem_bytes, int utf8, const uint64_t *src_valid,
+                             const uint32_t *idx, int64_t n, void *dst, uint64_t *dst_valid) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t c = wave; c * 64 < n; c += nwaves) {
+        const int64_t i = c * 64 + lane;
+        bool ok = false;
+        if (i < n) {
+            const uint32_t r = idx[i];
+            uint64_t v = utf8 ? NULL_CODE : 0;
+            if (r != NO_ROW) {
+                ok = src_valid ? ((src_valid[r >> 6] >> (r & 63)) & 1) != 0 : true;
+                const uint64_t x = elem_bytes == 8 ? ((const uint64_t *)src)[r] : ((const uint32_t *)src)[r];
+                if (utf8) ok = ok && (uint32_t)x != NULL_CODE;
+                if (ok || !utf8) v = x;
+            }
+            if (elem_bytes == 8) ((uint64_t *)dst)[i] = v;
+            else ((uint32_t *)dst)[i] = (uint32_t)v;
+        }
+        const uint64_t word = __ballot(ok);
+        if (dst_valid && lane == 0) dst_valid[c] = word;
+    }
+}
+
+hipError_t launch_gather_idx(hipStream_t s, const void *src, int elem_bytes, int utf8, const uint64_t *src_valid,
+                             const uint32_t *idx, int64_t n, void *dst, uint64_t *dst_valid, int grid) {
+    if (elem_bytes != 4 && elem_bytes != 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather_idx, dim3(grid), dim3(BLOCK), 0, s, src, elem_bytes, utf8, src_valid, idx, n, dst,
+                       dst_valid);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather(hipStream_t s, const uint64_t *words, const int64_t *offs,
                          int64_t n_words, const void *src, int elem_bytes,
                          const uint64_t *src_valid, void *dst, uint8_t *dst_valid, int grid) {

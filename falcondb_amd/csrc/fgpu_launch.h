@@ -141,6 +141,9 @@ hipError_t launch_word_offsets(hipStream_t s, const uint64_t *words, int64_t n_w
 hipError_t launch_gather(hipStream_t s, const uint64_t *words, const int64_t *offs,
                          int64_t n_words, const void *src, int elem_bytes,
                          const uint64_t *src_valid, void *dst, uint8_t *dst_valid, int grid);
+// dst[i] = src[idx[i]] with validity (NO_ROW = NULL); see k_gather_idx.
+hipError_t launch_gather_idx(hipStream_t s, const void *src, int elem_bytes, int utf8, const uint64_t *src_valid,
+                             const uint32_t *idx, int64_t n, void *dst, uint64_t *dst_valid, int grid);
 // Exclusive prefix sum of u32 counts into int64 offsets (offs[n] = total).
 hipError_t launch_scan_u32(hipStream_t s, const uint32_t *cnt, int64_t n, int64_t *offs,
                            int64_t *block_tmp);

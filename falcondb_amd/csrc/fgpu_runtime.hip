@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <strings.h>
 #include <list>
 #include <map>
 #include <mutex>
@@ -1077,6 +1078,51 @@ static int remap_codes(fgpu_ctx *ctx, DevCol &c, const DictMerge &m) {
     return FGPU_OK;
 }
 
+// Statistics of a device column (upload, materialised joins): min / max, NULL
+// count, float exponent range and non-finite flags, the distinct estimate and
+// the per-batch zone map.  d_st is scratch for one ColStatsDev.
+static int column_stats(fgpu_ctx *ctx, DevCol &c, int64_t n_rows, int32_t batch_rows, ColStatsDev *d_st) {
+    ColStatsDev h{};
+    h.imin = INT64_MAX;
+    h.imax = INT64_MIN;
+    h.fmin_enc = ~0ull;
+    h.fmax_enc = 0;
+    h.emin = 1 << 20;
+    h.emax = -(1 << 20);
+    hipError_t e = hipMemcpyAsync(d_st, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && n_rows) {
+        const int grid = grid_for(ctx, n_rows, BLOCK, 8);
+        e = launch_col_stats(ctx->stream, c.d_values, c.dtype, c.d_valid, n_rows, d_st, grid);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d_st, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "column stats: %s", hipGetErrorString(e));
+    c.null_count = (int64_t)h.nulls;
+    c.imin = h.imin;
+    c.imax = h.imax;
+    c.fmin = __builtin_bit_cast(double, f64_unorder(h.fmin_enc));
+    c.fmax = __builtin_bit_cast(double, f64_unorder(h.fmax_enc));
+    c.emin = h.emin;
+    c.emax = h.emax;
+    c.nonfinite = h.nonfinite;
+    c.distinct = (int64_t)std::llround(hll_estimate(h.hll));
+    c.zones.clear();
+    if (n_rows && c.dtype != FGPU_UTF8) {  // zone map
+        const int64_t nb = (n_rows + batch_rows - 1) / batch_rows;
+        BatchZone *d_z = nullptr;
+        e = hipMalloc(&d_z, (size_t)nb * sizeof(BatchZone));
+        if (e == hipSuccess) e = launch_batch_zones(ctx->stream, c.d_values, c.dtype, c.d_valid, n_rows, batch_rows, d_z);
+        if (e == hipSuccess) {
+            c.zones.resize((size_t)nb);
+            e = hipMemcpyAsync(c.zones.data(), d_z, (size_t)nb * sizeof(BatchZone), hipMemcpyDeviceToHost, ctx->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (d_z) (void)hipFree(d_z);
+        if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "zone map: %s", hipGetErrorString(e));
+    }
+    return FGPU_OK;
+}
+
 static int table_upload_impl(fgpu_ctx *ctx, const fgpu_column_desc *cols, const fgpu_utf8_batches *utf8,
                              int n_cols, int64_t n_rows, int32_t batch_rows, fgpu_table **out) {
     if (!ctx || !out || (n_cols > 0 && !cols)) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
@@ -1156,42 +1202,7 @@ static int table_upload_impl(fgpu_ctx *ctx, const fgpu_column_desc *cols, const 
             if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "validity upload: %s", hipGetErrorString(e)); break; }
             t->bytes += (int64_t)vb;
         }
-        // statistics
-        ColStatsDev h{};
-        h.imin = INT64_MAX;
-        h.imax = INT64_MIN;
-        h.fmin_enc = ~0ull;
-        h.fmax_enc = 0;
-        h.emin = 1 << 20;
-        h.emax = -(1 << 20);
-        e = hipMemcpy(d_st, &h, sizeof h, hipMemcpyHostToDevice);
-        if (e == hipSuccess && n_rows) {
-            const int grid = grid_for(ctx, n_rows, BLOCK, 8);
-            e = launch_col_stats(nullptr, c.d_values, c.dtype, c.d_valid, n_rows, d_st, grid);
-        }
-        if (e == hipSuccess) e = hipMemcpy(&h, d_st, sizeof h, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "column stats: %s", hipGetErrorString(e)); break; }
-        c.null_count = (int64_t)h.nulls;
-        c.imin = h.imin;
-        c.imax = h.imax;
-        c.fmin = __builtin_bit_cast(double, f64_unorder(h.fmin_enc));
-        c.fmax = __builtin_bit_cast(double, f64_unorder(h.fmax_enc));
-        c.emin = h.emin;
-        c.emax = h.emax;
-        c.nonfinite = h.nonfinite;
-        c.distinct = (int64_t)std::llround(hll_estimate(h.hll));
-        if (n_rows && c.dtype != FGPU_UTF8) {  // zone map
-            const int64_t nb = (n_rows + batch_rows - 1) / batch_rows;
-            BatchZone *d_z = nullptr;
-            e = hipMalloc(&d_z, (size_t)nb * sizeof(BatchZone));
-            if (e == hipSuccess) e = launch_batch_zones(nullptr, c.d_values, c.dtype, c.d_valid, n_rows, batch_rows, d_z);
-            if (e == hipSuccess) {
-                c.zones.resize((size_t)nb);
-                e = hipMemcpy(c.zones.data(), d_z, (size_t)nb * sizeof(BatchZone), hipMemcpyDeviceToHost);
-            }
-            if (d_z) (void)hipFree(d_z);
-            if (e != hipSuccess) { rc = set_err(FGPU_E_DEVICE, "zone map: %s", hipGetErrorString(e)); break; }
-        }
+        if ((rc = column_stats(ctx, c, n_rows, batch_rows, d_st))) break;
     }
     (void)hipFree(d_st);
     if (rc) return fail(rc);
@@ -1210,6 +1221,20 @@ int fgpu_table_upload_batches(fgpu_ctx *ctx, const fgpu_column_desc *cols, const
 }
 
 int64_t fgpu_table_rows(const fgpu_table *t) { return t ? t->n_rows : -1; }
+int fgpu_table_cols(const fgpu_table *t) { return t ? (int)t->cols.size() : -1; }
+int fgpu_table_column_info(const fgpu_table *t, int col, fgpu_column_view *out) {
+    if (!t || !out) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    if (col < 0 || col >= (int)t->cols.size())
+        return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "column %d of %zu", col, t->cols.size());
+    const DevCol &c = t->cols[col];
+    out->name = c.name.c_str();
+    out->dtype = c.dtype;
+    out->values = nullptr;
+    out->valid = nullptr;
+    out->dict_len = (int32_t)c.dict.size();
+    out->dict = c.dictp.empty() ? nullptr : c.dictp.data();
+    return FGPU_OK;
+}
 int64_t fgpu_table_device_bytes(const fgpu_table *t) { return t ? t->bytes : 0; }
 
 int fgpu_table_column_stats(const fgpu_table *t, int col, int64_t *min_i64, int64_t *max_i64,
@@ -2328,12 +2353,12 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     return rc;
 }
 
-int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
-                    const fgpu_join_index *idx, const fgpu_join_spec *spec, uint32_t *probe_rows,
-                    uint32_t *build_rows, int64_t capacity, int64_t *n_out) {
-    if (!c || !probe || !build || !spec || !n_out) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (int rc = set_device(c->device)) return rc;
+// The (probe row, build row) pairs of a join on the device, in the
+// context's B_PAIRS slot: *d_l and *d_r hold *total rows each (NO_ROW for the
+// NULL-extended side).  Rows come in probe order.  The caller holds c->mu.
+static int join_pairs_dev(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                          const fgpu_join_index *idx, const fgpu_join_spec *spec, uint32_t **d_l, uint32_t **d_r,
+                          int64_t *total_out) {
     Binder pb(probe), bb(build);
     JoinProbeArgs a;
     AggLayout L;
@@ -2365,17 +2390,11 @@ int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
     hipError_t e = n ? launch_join_pairs(c->stream, a, 0, grid) : hipSuccess;
     if (e == hipSuccess) e = launch_scan_u32(c->stream, d_cnt, n, d_offs, d_tmp);
     int64_t total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_offs + n, 8, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = mail_copy(c, d_offs + n, 8);
+    if (e == hipSuccess) e = mail_take(c, &total, 8);
     if (e != hipSuccess) {
         fgpu_join_index_free(own);
         return set_err(FGPU_E_DEVICE, "join pairs: %s", hipGetErrorString(e));
-    }
-    *n_out = total;
-    if (total > capacity || (total > 0 && (!probe_rows || !build_rows))) {
-        fgpu_join_index_free(own);
-        return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "join produces %lld pairs, capacity %lld",
-                       (long long)total, (long long)capacity);
     }
     uint32_t *d_pairs;
     rc = buf(c, B_PAIRS, (size_t)std::max<int64_t>(total, 1) * 8, (void **)&d_pairs);
@@ -2387,11 +2406,33 @@ int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
     a.out_l = d_pairs;
     a.out_r = d_pairs + std::max<int64_t>(total, 1);
     e = (n && total) ? launch_join_pairs(c->stream, a, 1, grid) : hipSuccess;
-    std::vector<uint32_t> l(total), r(total);
-    if (e == hipSuccess && total) e = hipMemcpyAsync(l.data(), a.out_l, total * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && total) e = hipMemcpyAsync(r.data(), a.out_r, total * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     fgpu_join_index_free(own);
+    if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "join pairs: %s", hipGetErrorString(e));
+    *d_l = a.out_l;
+    *d_r = a.out_r;
+    *total_out = total;
+    return FGPU_OK;
+}
+
+int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                    const fgpu_join_index *idx, const fgpu_join_spec *spec, uint32_t *probe_rows,
+                    uint32_t *build_rows, int64_t capacity, int64_t *n_out) {
+    if (!c || !probe || !build || !spec || !n_out) return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    uint32_t *d_l = nullptr, *d_r = nullptr;
+    int64_t total = 0;
+    if (int rc = join_pairs_dev(c, probe, pred, build, idx, spec, &d_l, &d_r, &total)) return rc;
+    *n_out = total;
+    if (total > capacity || (total > 0 && (!probe_rows || !build_rows)))
+        return set_err(FGPU_E_INDEX_OUT_OF_BOUNDS, "join produces %lld pairs, capacity %lld", (long long)total,
+                       (long long)capacity);
+    std::vector<uint32_t> l(total), r(total);
+    hipError_t e = hipSuccess;
+    if (total) e = hipMemcpyAsync(l.data(), d_l, total * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && total) e = hipMemcpyAsync(r.data(), d_r, total * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "join pairs: %s", hipGetErrorString(e));
     // within one probe row, list build rows in ascending order
     int64_t i = 0;
@@ -2405,6 +2446,92 @@ int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
         std::memcpy(probe_rows, l.data(), total * 4);
         std::memcpy(build_rows, r.data(), total * 4);
     }
+    return FGPU_OK;
+}
+
+int fgpu_join_materialize(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                          const fgpu_join_index *idx, const fgpu_join_spec *spec, const int32_t *probe_cols,
+                          int n_probe_cols, const int32_t *build_cols, int n_build_cols, fgpu_table **out) {
+    if (!c || !probe || !build || !spec || !out || (n_probe_cols > 0 && !probe_cols) ||
+        (n_build_cols > 0 && !build_cols) || n_probe_cols < 0 || n_build_cols < 0)
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    for (int i = 0; i < n_probe_cols; ++i)
+        if (probe_cols[i] < 0 || probe_cols[i] >= (int)probe->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "probe column %d not in table", probe_cols[i]);
+    for (int i = 0; i < n_build_cols; ++i)
+        if (build_cols[i] < 0 || build_cols[i] >= (int)build->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "build column %d not in table", build_cols[i]);
+    if (n_build_cols && (spec->type == FGPU_SEMI || spec->type == FGPU_ANTI))
+        return set_err(FGPU_E_COLUMN_NOT_FOUND, "SEMI/ANTI output has only probe columns");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    uint32_t *d_l = nullptr, *d_r = nullptr;
+    int64_t total = 0;
+    if (int rc = join_pairs_dev(c, probe, pred, build, idx, spec, &d_l, &d_r, &total)) return rc;
+    if (total >= (int64_t(1) << 31))
+        return set_err(FGPU_E_UNSUPPORTED, "join output of %lld rows (tables are limited to 2^31-1 rows)",
+                       (long long)total);
+    auto *t = new fgpu_table();
+    t->device = c->device;
+    t->n_rows = total;
+    t->batch_rows = probe->batch_rows;
+    t->cols.reserve((size_t)(n_probe_cols + n_build_cols));
+    auto fail = [&](int rc) {
+        fgpu_table_free(t);
+        return rc;
+    };
+    ColStatsDev *d_st = nullptr;
+    if (hipMalloc(&d_st, sizeof(ColStatsDev)) != hipSuccess) return fail(set_err(FGPU_E_OUT_OF_MEMORY, "stats buffer"));
+    const int grid = grid_for(c, (total + 63) / 64 * 64, BLOCK, 8);
+    int rc = FGPU_OK;
+    for (int k = 0; k < n_probe_cols + n_build_cols && rc == FGPU_OK; ++k) {
+        const bool right = k >= n_probe_cols;
+        const DevCol &src = right ? build->cols[build_cols[k - n_probe_cols]] : probe->cols[probe_cols[k]];
+        const uint32_t *rows = right ? d_r : d_l;
+        DevCol col;
+        col.name = src.name;
+        if (right)  // output_plan: a right column whose name the left side has becomes right.<name>
+            for (const DevCol &lc : probe->cols)
+                if (strcasecmp(lc.name.c_str(), src.name.c_str()) == 0) col.name = "right." + src.name;
+        col.dtype = src.dtype;
+        col.dict = src.dict;
+        col.empty_code = src.empty_code;
+        for (auto &str : col.dict) col.dictp.push_back(str.c_str());
+        t->cols.push_back(std::move(col));
+        DevCol &d = t->cols.back();
+        const size_t bytes = (size_t)std::max<int64_t>(total, 1) * d.elem_bytes();
+        if (hipMalloc(&d.d_values, bytes) != hipSuccess) {
+            rc = set_err(FGPU_E_OUT_OF_MEMORY, "join output column '%s': %zu bytes", d.name.c_str(), bytes);
+            break;
+        }
+        t->bytes += (int64_t)bytes;
+        // a build column of a LEFT join is NULL where no build row matched
+        const bool nullable = src.dtype != FGPU_UTF8 && (src.d_valid || (right && spec->type == FGPU_LEFT));
+        if (nullable) {
+            const size_t vb = (size_t)std::max<int64_t>((total + 63) / 64, 1) * 8;
+            if (hipMalloc(&d.d_valid, vb) != hipSuccess) {
+                rc = set_err(FGPU_E_OUT_OF_MEMORY, "join output validity");
+                break;
+            }
+            t->bytes += (int64_t)vb;
+        }
+        hipError_t e = total ? launch_gather_idx(c->stream, src.d_values, (int)src.elem_bytes(), src.dtype == FGPU_UTF8,
+                                                 src.d_valid, rows, total, d.d_values, d.d_valid, grid)
+                             : hipSuccess;
+        if (e != hipSuccess) {
+            rc = set_err(FGPU_E_DEVICE, "join output gather: %s", hipGetErrorString(e));
+            break;
+        }
+        rc = column_stats(c, d, total, t->batch_rows, d_st);
+    }
+    (void)hipFree(d_st);
+    if (rc) return fail(rc);
+    for (DevCol &d : t->cols) {  // dictionary views of the columns in their final place
+        d.dictp.clear();
+        for (auto &str : d.dict) d.dictp.push_back(str.c_str());
+    }
+    c->plan = "join:materialize";
+    *out = t;
     return FGPU_OK;
 }
 

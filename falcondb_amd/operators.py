@@ -237,18 +237,49 @@ class Table:
         self.n_rows = int(n_rows or 0)
         self.batch_rows = batch_rows
 
+    @classmethod
+    def from_handle(cls, ctx: Context, h, batch_rows: int = 65536) -> "Table":
+        """Wraps a device table the library built (a materialised join); the
+        schema comes from the library."""
+        t = cls.__new__(cls)
+        t.ctx = ctx
+        t._h = h
+        lib = _lib()
+        t.n_rows = int(lib.fgpu_table_rows(h))
+        t.batch_rows = batch_rows
+        t.names, t.dtypes, t.dicts = [], [], []
+        for i in range(int(lib.fgpu_table_cols(h))):
+            v = _abi.ColumnView()
+            _check(lib.fgpu_table_column_info(h, i, C.byref(v)))
+            t.names.append(v.name.decode())
+            t.dtypes.append(int(v.dtype))
+            t.dicts.append([v.dict[k].decode() for k in range(v.dict_len)] if v.dtype == UTF8 else None)
+        return t
+
     @property
     def handle(self):
         return self._h
 
     def column_index(self, name: str) -> int:
         low = name.lower()
+        for i, n in enumerate(self.names):  # an exact name first ("right.v" is a column name)
+            if n.lower() == low:
+                return i
         if "." in low:  # qualified t.col: resolve by the bare name
             low = low.split(".", 1)[1]
         for i, n in enumerate(self.names):
             if n.lower() == low:
                 return i
         raise FalconError(1, f"column '{name}' not found")
+
+    def read(self, ctx: Context | None = None) -> "RecordBatch":
+        """Every row and column, copied back to the host."""
+        ctx = ctx or self.ctx
+        idx = list(range(len(self.names)))
+        arr = (C.c_int32 * max(len(idx), 1))(*idx)
+        h = C.c_void_p()
+        _check(_lib().fgpu_filter_select(ctx.handle, self._h, None, arr, len(idx), C.byref(h)))
+        return _read_result(h, [None] * len(idx))
 
     def device_bytes(self) -> int:
         return int(_lib().fgpu_table_device_bytes(self._h))
@@ -676,6 +707,31 @@ def execute_hash_join(ctx: Context, probe: Table, build: Table, probe_key: str, 
     _check(_lib().fgpu_join_agg(ctx.handle, probe.handle, p, build.handle,
                                 index.handle if index else None, C.byref(spec), C.byref(h)))
     return _read_result(h, [a.func for a in aggs])
+
+
+def execute_hash_join_projected(ctx: Context, probe: Table, build: Table, probe_key: str, build_key: str,
+                                join_type: int, probe_columns: Sequence[str] | None = None,
+                                build_columns: Sequence[str] | None = None, filter: Expr | None = None,
+                                index: JoinIndex | None = None) -> Table:
+    """The join's output as a new device table: the kept probe columns, then
+    the kept build columns (a build column whose name the probe side has is
+    named right.<name>; NULL where a LEFT row found no match), rows in probe
+    order (execute_hash_join_projected, src/query/join.rs:184-282).  None keeps
+    every column of that side; SEMI / ANTI keep probe columns only."""
+    spec, keep = _join_spec(probe, build, probe_key, build_key, join_type, [])
+    p, keep2 = _pred(filter, probe)
+    if probe_columns is None:
+        probe_columns = list(probe.names)
+    if build_columns is None:
+        build_columns = [] if join_type in (SEMI, ANTI) else list(build.names)
+    pi = [probe.column_index(c) for c in probe_columns]
+    bi = [build.column_index(c) for c in build_columns]
+    pa = (C.c_int32 * max(len(pi), 1))(*pi)
+    ba = (C.c_int32 * max(len(bi), 1))(*bi)
+    h = C.c_void_p()
+    _check(_lib().fgpu_join_materialize(ctx.handle, probe.handle, p, build.handle, index.handle if index else None,
+                                        C.byref(spec), pa, len(pi), ba, len(bi), C.byref(h)))
+    return Table.from_handle(ctx, h, probe.batch_rows)
 
 
 def execute_hash_semi_anti_join(ctx: Context, probe: Table, build: Table, probe_key: str,

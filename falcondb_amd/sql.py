@@ -31,7 +31,8 @@ import numpy as np
 
 from . import _abi
 from .operators import (AggExpr, Context, Expr, FalconError, GroupByExec, JoinIndex, RecordBatch, Table, cmp,
-                        col, count_star, execute_agg_only, execute_hash_join, execute_select,
+                        col, count_star, execute_agg_only, execute_hash_join,
+                        execute_hash_join_projected, execute_select,
                         execute_sort_limit, lit_cmp)
 
 PARSE, INVALID_QUERY, UNSUPPORTED = 4, 6, 102  # FalconError codes
@@ -144,6 +145,8 @@ class _Parser:
         return None
 
     def item(self) -> Item:
+        if self.accept("punct", "*"):  # SELECT *: every column (expanded by the planner)
+            return Item(None, ColName(None, "*"), None)
         k, v = self.peek()
         if k == "name" and v.upper() in _AGGS and self.peek(1) == ("punct", "("):
             self.i += 2
@@ -380,6 +383,9 @@ class Database:
             raise FalconError(1, f"column '{c.name}' not found")
 
         where = self._predicate(q.where, side) if q.where is not None else None
+        star = any(it.func is None and it.arg.name == "*" for it in q.items)
+        if star and right is None:
+            q.items = _expand_star(q.items, left.names)
         names = [_output_name(it, i) for i, it in enumerate(q.items)]
         has_agg = any(it.func is not None for it in q.items)
 
@@ -449,19 +455,66 @@ class Database:
         return RecordBatch(names, dts, cols, valid, funcs)
 
     def _plan_join(self, q, left, right, right_name, jtype, jl, jr, where, side, names) -> RecordBatch:
-        if q.group_by is not None or q.order_by is not None or q.limit is not None:
-            raise FalconError(UNSUPPORTED, "SQL: GROUP BY / ORDER BY / LIMIT over a join")
-        if any(it.func is None for it in q.items):
-            raise FalconError(UNSUPPORTED, "SQL: a join's SELECT list takes aggregates only")
         sl, sr = side(jl), side(jr)
         if {sl, sr} != {"probe", "build"}:
             raise FalconError(INVALID_QUERY, "SQL: the join condition must relate the two tables")
         pk, bk = (jl.name, jr.name) if sl == "probe" else (jr.name, jl.name)
-        aggs = self._aggs(q, side, names)
-        if jtype in (_abi.SEMI, _abi.ANTI) and any(a.side == "build" for a in aggs):
-            raise FalconError(INVALID_QUERY, "SQL: EXISTS exposes no column of its subquery")
-        return execute_hash_join(self.ctx, left, right, pk, bk, jtype, aggs, filter=where,
-                                 index=self.index(right_name, bk))
+        semi = jtype in (_abi.SEMI, _abi.ANTI)
+        index = self.index(right_name, bk)
+        star = any(it.func is None and it.arg.name == "*" for it in q.items)
+        if (q.group_by is None and q.order_by is None and q.limit is None and not star
+                and all(it.func is not None for it in q.items)):
+            # aggregates only: the join is fused with them, nothing is materialised
+            aggs = self._aggs(q, side, names)
+            if semi and any(a.side == "build" for a in aggs):
+                raise FalconError(INVALID_QUERY, "SQL: EXISTS exposes no column of its subquery")
+            return execute_hash_join(self.ctx, left, right, pk, bk, jtype, aggs, filter=where, index=index)
+        # Otherwise the join's output is materialised with the columns the rest
+        # of the plan reads (execute_hash_join_projected's keep set), and the
+        # plan's remaining steps run over it.  A build column the probe table
+        # also names is called right.<name> there (src/query/join.rs:877-886).
+        def out_name(c: ColName) -> str:
+            if side(c) == "probe":
+                return left.names[left.column_index(c.name)]
+            bname = right.names[right.column_index(c.name)]
+            return "right." + bname if _has_col(left, bname) else bname
+        refs = [it.arg for it in q.items if it.arg is not None and it.arg.name != "*"]
+        refs += [c for c in (q.group_by, q.order_by[0] if q.order_by else None) if c is not None]
+        probe_cols, build_cols = [], []
+        for c in refs:
+            lst, nm = (probe_cols, left.names[left.column_index(c.name)]) if side(c) == "probe" else \
+                (build_cols, right.names[right.column_index(c.name)])
+            if semi and lst is build_cols:
+                raise FalconError(INVALID_QUERY, "SQL: EXISTS exposes no column of its subquery")
+            if nm not in lst:
+                lst.append(nm)
+        if star:
+            probe_cols = list(left.names)
+            build_cols = [] if semi else list(right.names)
+        jt = execute_hash_join_projected(self.ctx, left, right, pk, bk, jtype, probe_cols, build_cols,
+                                         filter=where, index=index)
+        try:
+            def rename(c):
+                return None if c is None else ColName(None, out_name(c))
+            items = []
+            for it in q.items:
+                if it.func is None and it.arg.name == "*":
+                    items += [Item(None, ColName(None, n), None) for n in jt.names]
+                else:
+                    items.append(Item(it.func, rename(it.arg), it.alias))
+            q2 = Query(items, "", None, group_by=rename(q.group_by),
+                       order_by=(rename(q.order_by[0]), q.order_by[1]) if q.order_by else None, limit=q.limit)
+            names2 = [_output_name(it, i) for i, it in enumerate(items)] if star else names
+
+            def side2(c):
+                return "probe"
+            if q2.group_by is not None or any(it.func is not None for it in items):
+                return self._plan_aggregate(q2, jt, None, side2, names2)
+            if q2.order_by is not None:
+                return self._plan_sort_limit(q2, jt, None, names2)
+            return self._plan_select(q2, jt, None, names2)
+        finally:
+            jt.free()
 
     def _plan_sort_limit(self, q, t, where, names) -> RecordBatch:
         c, desc = q.order_by
@@ -488,6 +541,16 @@ class Database:
             raise FalconError(INVALID_QUERY, "SQL: aggregates need GROUP BY or no plain columns")
         res = execute_select(self.ctx, t, where, [it.arg.name for it in q.items])
         return RecordBatch(names, res.dtypes, res.columns, res.validity, [None] * len(names))
+
+
+def _expand_star(items: list, columns: list[str]) -> list:
+    out = []
+    for it in items:
+        if it.func is None and it.arg.name == "*":
+            out += [Item(None, ColName(None, n), None) for n in columns]
+        else:
+            out.append(it)
+    return out
 
 
 def _has_col(t: Table, name: str) -> bool:

@@ -196,6 +196,10 @@ int fgpu_table_upload_batches(fgpu_ctx *ctx, const fgpu_column_desc *cols, const
                               int n_cols, int64_t n_rows, int32_t batch_rows, fgpu_table **out);
 void fgpu_table_free(fgpu_table *t);
 int64_t fgpu_table_rows(const fgpu_table *t);
+/* The table's schema (Schema, src/storage/schema.rs): column count, and per
+ * column its name, dtype and (Utf8) dictionary; values / valid are NULL. */
+int fgpu_table_cols(const fgpu_table *t);
+int fgpu_table_column_info(const fgpu_table *t, int col, fgpu_column_view *out);
 int64_t fgpu_table_device_bytes(const fgpu_table *t);
 /* Column statistics computed at upload: min/max as int64 (integers, codes) or
  * double bits (floats); approximate distinct count. */
@@ -285,6 +289,24 @@ int fgpu_join_pairs(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *pro
                     const fgpu_table *build, const fgpu_join_index *idx,
                     const fgpu_join_spec *spec, uint32_t *probe_rows,
                     uint32_t *build_rows, int64_t capacity, int64_t *n_out);
+
+/* The join's output itself: SELECT <probe_cols>, <build_cols> FROM probe
+ * [LEFT] JOIN build ON probe.k = build.k [WHERE probe_pred], or for SEMI /
+ * ANTI the kept probe rows (probe columns only), materialised as a new
+ * device-resident table (execute_hash_join_projected, src/query/join.rs:184-282;
+ * output_plan / build_output_dense :828-899, :403-502).  Columns are the kept
+ * probe columns, then the kept build columns; a build column whose name the
+ * probe table also has is named "right.<name>" (src/query/join.rs:877-886);
+ * build columns of a LEFT join are NULL where no build row matched (NO_ROW).
+ * With no columns kept the table still has the join's row count, so
+ * COUNT(*) over it works (:891-897).  Rows come in probe order.  The table is
+ * an ordinary fgpu_table (GROUP BY, ORDER BY, WHERE over the join output);
+ * the caller frees it with fgpu_table_free.  The spec's aggregates are
+ * ignored. */
+int fgpu_join_materialize(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *probe_pred,
+                          const fgpu_table *build, const fgpu_join_index *idx, const fgpu_join_spec *spec,
+                          const int32_t *probe_cols, int n_probe_cols, const int32_t *build_cols,
+                          int n_build_cols, fgpu_table **out);
 
 /* ── ORDER BY col [ASC | DESC] LIMIT k (execute_sort_limit,
  * src/query/sort.rs:59-115; typed top-k :209-330) ───────────────────────────

@@ -149,7 +149,16 @@ def check_all(engine) -> int:
         checks += 1
     sl, sr = side("k", g["self_join_keys"]), side("rk", g["self_join_keys"])
     assert scalar(engine.join(sl, sr, "k", "rk", INNER, [AggExpr(COUNT)])) == g["self_inner"]
-    return checks + 1
+    checks += 1
+
+    # an empty right side: SEMI keeps nothing, ANTI everything (src/query/join.rs:60-67)
+    n = GOLDEN["join_empty_build"]["left_rows"]
+    left = {"k": np.arange(n, dtype=np.int64)}
+    right = {"rk": np.zeros(0, np.int64)}
+    for jt, exp in ((SEMI, 0), (ANTI, n), (INNER, 0), (LEFT, n)):
+        assert scalar(engine.join(left, right, "k", "rk", jt, [AggExpr(COUNT)])) == exp, jt
+        checks += 1
+    return checks
 
 
 # ─── integer SUM past 2^53 and past int64 (src/query/aggregate.rs:1936-1993) ──

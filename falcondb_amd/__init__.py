@@ -10,7 +10,7 @@ The Python layer mirrors the reference's operator names (see operators.py).
 from .operators import (AggExpr, AggFunc, BatchDictColumn, BatchGroupBy, ColumnRef, Context, DictColumn, Expr,
                         FalconError, GroupByExec, JoinIndex, JoinType, NullableColumn, RecordBatch, Table,
                         col, cmp, count_star, execute_agg_only, execute_hash_join,
-                        execute_hash_join_projected, execute_hash_semi_anti_join, execute_select,
+                        execute_hash_join_projected, execute_hash_semi_anti_join, execute_select, execute_sort,
                         host_exact_sum, join_pairs,
                         lit_cmp, pack_validity)
 
@@ -19,5 +19,5 @@ __all__ = [
     "GroupByExec", "JoinIndex", "JoinType", "NullableColumn", "RecordBatch", "Table", "col",
     "cmp", "count_star", "execute_agg_only", "execute_hash_join", "execute_hash_join_projected",
     "execute_hash_semi_anti_join",
-    "execute_select", "host_exact_sum", "join_pairs", "lit_cmp", "pack_validity",
+    "execute_select", "execute_sort", "host_exact_sum", "join_pairs", "lit_cmp", "pack_validity",
 ]

@@ -63,6 +63,10 @@ class JoinSpec(C.Structure):
                 ("n_aggs", C.c_int32), ("aggs", C.POINTER(Agg))]
 
 
+class SortKey(C.Structure):
+    _fields_ = [("col", C.c_int32), ("desc", C.c_int32), ("nulls_last", C.c_int32)]
+
+
 class ColumnView(C.Structure):
     _fields_ = [("name", C.c_char_p), ("dtype", C.c_int32), ("values", C.c_void_p),
                 ("valid", C.POINTER(C.c_uint8)), ("dict_len", C.c_int32),
@@ -134,6 +138,8 @@ DECLARED = {
     "fgpu_join_materialize": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.c_void_p, C.c_void_p,
                                         C.POINTER(JoinSpec), C.POINTER(C.c_int32), C.c_int,
                                         C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_void_p)]),
+    "fgpu_sort_limit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(SortKey), C.c_int,
+                                  C.c_int64, C.POINTER(C.c_int32), C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "fgpu_table_cols": (C.c_int, [C.c_void_p]),
     "fgpu_table_column_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ColumnView)]),
     "fgpu_join_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wno-pass-failed", f"-I{ROOT / 'include'}",
 ]
 HIP_SOURCES = ["fgpu_runtime.hip", "fgpu_filter.hip", "fgpu_groupby.hip", "fgpu_gbpart.hip",
-               "fgpu_join.hip", "fgpu_batch.hip", "fgpu_topk.hip", "fgpu_server.hip"]
+               "fgpu_join.hip", "fgpu_batch.hip", "fgpu_topk.hip", "fgpu_sort.hip", "fgpu_server.hip"]
 
 GPU_LIB = PKG / "libfalcon_gpu.so"
 GEN_LIB = PKG / "libfgpu_datagen.so"

@@ -323,6 +323,20 @@ hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchAr
                               size_t lds);
 hipError_t batch_set_lds(size_t lds1, size_t lds2);
 
+// ─── ORDER BY with any LIMIT and any number of keys (fgpu_sort.hip) ────────
+hipError_t launch_mask_rows(hipStream_t s, const uint64_t *words, const int64_t *offs, int64_t n_words,
+                            uint32_t *out);
+hipError_t launch_sort_key(hipStream_t s, const void *values, int dtype, const uint64_t *valid, const uint32_t *rank,
+                           const uint32_t *rows, int64_t n, int desc, int nulls_last, uint64_t *key, uint8_t *flag);
+hipError_t launch_take_u64(hipStream_t s, const uint64_t *src, const uint32_t *idx, int64_t n, uint64_t *dst);
+hipError_t launch_take_u8(hipStream_t s, const uint8_t *src, const uint32_t *idx, int64_t n, uint8_t *dst);
+hipError_t launch_take_u32(hipStream_t s, const uint32_t *src, const uint32_t *idx, int64_t n, uint32_t *dst);
+hipError_t launch_iota_u32(hipStream_t s, uint32_t *p, int64_t n);
+hipError_t sort_pairs_u64(void *temp, size_t *temp_bytes, const uint64_t *k_in, uint64_t *k_out, const uint32_t *v_in,
+                          uint32_t *v_out, int64_t n, int end_bit, hipStream_t s);
+hipError_t sort_pairs_u8(void *temp, size_t *temp_bytes, const uint8_t *k_in, uint8_t *k_out, const uint32_t *v_in,
+                         uint32_t *v_out, int64_t n, hipStream_t s);
+
 // ─── ORDER BY ... LIMIT k ──────────────────────────────────────────────────
 // cands: grid * K pairs of 16 bytes; out: K pairs (best first).
 hipError_t launch_topk(hipStream_t s, const DCols &c, const DPred &p, int col, int desc, int null_mode, int64_t n,

@@ -249,7 +249,7 @@ struct fgpu_result {
 // ─── context ────────────────────────────────────────────────────────────────
 enum BufSlot {
     B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT,
-    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_LIVE, B_NSLOTS
+    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_LIVE, B_SORT, B_NSLOTS
 };
 
 struct Tuning {
@@ -2447,6 +2447,169 @@ int fgpu_join_pairs(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
         std::memcpy(build_rows, r.data(), total * 4);
     }
     return FGPU_OK;
+}
+
+// Copies `m` rows of table columns, gathered at device row numbers d_rows,
+// into a host result (values, and validity bytes from validity words).
+static int result_from_rows(fgpu_ctx *c, const fgpu_table *t, const uint32_t *d_rows, int64_t m, const int32_t *cols,
+                            int n_cols, fgpu_result **out) {
+    void *d_out;
+    const int64_t nw = (m + 63) / 64;
+    if (int rc = buf(c, B_OUT, (size_t)std::max<int64_t>(m, 1) * 8 + (size_t)std::max<int64_t>(nw, 1) * 8 + 64, &d_out))
+        return rc;
+    uint64_t *d_vw = (uint64_t *)((char *)d_out + (size_t)std::max<int64_t>(m, 1) * 8);
+    auto *r = new fgpu_result();
+    r->rows = m;
+    const int grid = grid_for(c, (m + 63) / 64 * 64, BLOCK, 8);
+    std::vector<uint64_t> vw((size_t)nw);
+    for (int i = 0; i < n_cols; ++i) {
+        const DevCol &dc = t->cols[cols[i]];
+        RCol rc2;
+        rc2.name = dc.name;
+        rc2.dtype = dc.dtype == FGPU_FLOAT64 ? FGPU_FLOAT64 : dc.dtype == FGPU_UTF8 ? FGPU_UTF8 : FGPU_INT64;
+        const int eb = (int)dc.elem_bytes();
+        hipError_t e = m ? launch_gather_idx(c->stream, dc.d_values, eb, dc.dtype == FGPU_UTF8, dc.d_valid, d_rows, m,
+                                             d_out, d_vw, grid)
+                         : hipSuccess;
+        std::vector<uint32_t> tmp32;
+        if (e == hipSuccess && m) {
+            if (eb == 8) {
+                if (rc2.dtype == FGPU_FLOAT64) {
+                    rc2.f64.resize(m);
+                    e = hipMemcpyAsync(rc2.f64.data(), d_out, m * 8, hipMemcpyDeviceToHost, c->stream);
+                } else {
+                    rc2.i64.resize(m);
+                    e = hipMemcpyAsync(rc2.i64.data(), d_out, m * 8, hipMemcpyDeviceToHost, c->stream);
+                }
+            } else {
+                tmp32.resize(m);
+                e = hipMemcpyAsync(tmp32.data(), d_out, m * 4, hipMemcpyDeviceToHost, c->stream);
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(vw.data(), d_vw, nw * 8, hipMemcpyDeviceToHost, c->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            delete r;
+            return set_err(FGPU_E_DEVICE, "row gather: %s", hipGetErrorString(e));
+        }
+        rc2.valid.resize(m);
+        for (int64_t k = 0; k < m; ++k) rc2.valid[k] = (vw[k >> 6] >> (k & 63)) & 1;
+        if (eb == 4) {
+            if (dc.dtype == FGPU_UTF8) {
+                rc2.u32.resize(m);
+                for (int64_t k = 0; k < m; ++k) rc2.u32[k] = tmp32[k];
+                rc2.dict_src = &dc;
+            } else {
+                rc2.i64.resize(m);
+                for (int64_t k = 0; k < m; ++k) rc2.i64[k] = (int32_t)tmp32[k];
+            }
+        }
+        r->cols.push_back(std::move(rc2));
+    }
+    *out = r;
+    return FGPU_OK;
+}
+
+int fgpu_sort_limit(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const fgpu_sort_key *keys, int n_keys,
+                    int64_t limit, const int32_t *cols, int n_cols, int64_t *rows_out, fgpu_result **out) {
+    if (!c || !t || !out || n_keys < 1 || !keys || (n_cols > 0 && !cols) || n_cols < 0)
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    for (int j = 0; j < n_keys; ++j)
+        if (keys[j].col < 0 || keys[j].col >= (int)t->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "sort column %d not in table", keys[j].col);
+    for (int i = 0; i < n_cols; ++i)
+        if (cols[i] < 0 || cols[i] >= (int)t->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "column %d not in table", cols[i]);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    mark(c, 0);
+    // the rows that pass p, in row order
+    uint64_t *d_words;
+    if (int rc = filter_mask_dev(c, t, p, &d_words)) return rc;
+    const int64_t nw = (t->n_rows + 63) / 64;
+    int64_t *d_offs, *d_tmp;
+    if (int rc = buf(c, B_OFFS, (size_t)(nw + 1) * 8, (void **)&d_offs)) return rc;
+    if (int rc = buf(c, B_TMP, (size_t)(nw / 4096 + 2) * 8, (void **)&d_tmp)) return rc;
+    HIPCHK(launch_word_offsets(c->stream, d_words, nw, d_offs, d_tmp));
+    int64_t n = 0;
+    HIPCHK(mail_copy(c, d_offs + nw, 8));
+    HIPCHK(mail_take(c, &n, 8));
+    // Utf8 sort keys order by string: each code's rank in the sorted dictionary
+    std::vector<uint32_t> ranks;
+    std::vector<size_t> rank_off((size_t)n_keys, 0);
+    for (int j = 0; j < n_keys; ++j) {
+        const DevCol &dc = t->cols[keys[j].col];
+        if (dc.dtype != FGPU_UTF8) continue;
+        std::vector<uint32_t> order(dc.dict.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (uint32_t)k;
+        std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return dc.dict[x] < dc.dict[y]; });
+        rank_off[j] = ranks.size();
+        ranks.resize(ranks.size() + std::max<size_t>(order.size(), 1));
+        uint32_t rk = 0;  // equal strings (a dictionary may repeat one) share a rank
+        for (size_t pos = 0; pos < order.size(); ++pos) {
+            if (pos && dc.dict[order[pos]] != dc.dict[order[pos - 1]]) rk = (uint32_t)pos;
+            ranks[rank_off[j] + order[pos]] = rk;
+        }
+    }
+    // scratch: rows, two orders, two key arrays, two flag arrays, ranks, sort temp
+    size_t temp_u64 = 0, temp_u32 = 0, temp_u8 = 0;
+    HIPCHK(sort_pairs_u64(nullptr, &temp_u64, nullptr, nullptr, nullptr, nullptr, std::max<int64_t>(n, 1), 64, c->stream));
+    HIPCHK(sort_pairs_u64(nullptr, &temp_u32, nullptr, nullptr, nullptr, nullptr, std::max<int64_t>(n, 1), 32, c->stream));
+    temp_u64 = std::max(temp_u64, temp_u32);
+    HIPCHK(sort_pairs_u8(nullptr, &temp_u8, nullptr, nullptr, nullptr, nullptr, std::max<int64_t>(n, 1), c->stream));
+    const size_t N = (size_t)std::max<int64_t>(n, 1);
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_rows = 0, o_ia = up(o_rows + N * 4), o_ib = up(o_ia + N * 4), o_ka = up(o_ib + N * 4),
+                 o_kb = up(o_ka + N * 8), o_fa = up(o_kb + N * 8), o_fb = up(o_fa + N), o_rank = up(o_fb + N),
+                 o_temp = up(o_rank + std::max<size_t>(ranks.size(), 1) * 4),
+                 total = up(o_temp + std::max(temp_u64, temp_u8));
+    char *base;
+    if (int rc = buf(c, B_SORT, total, (void **)&base)) return rc;
+    uint32_t *rows = (uint32_t *)(base + o_rows), *ia = (uint32_t *)(base + o_ia), *ib = (uint32_t *)(base + o_ib);
+    uint64_t *ka = (uint64_t *)(base + o_ka), *kb = (uint64_t *)(base + o_kb);
+    uint8_t *fa = (uint8_t *)(base + o_fa), *fb = (uint8_t *)(base + o_fb);
+    uint32_t *d_rank = (uint32_t *)(base + o_rank);
+    void *temp = base + o_temp;
+    if (!ranks.empty())
+        HIPCHK(hipMemcpyAsync(d_rank, ranks.data(), ranks.size() * 4, hipMemcpyHostToDevice, c->stream));
+    mark(c, 1);
+    if (n) {
+        HIPCHK(launch_mask_rows(c->stream, d_words, d_offs, nw, rows));
+        HIPCHK(launch_iota_u32(c->stream, ia, n));
+        for (int j = n_keys - 1; j >= 0; --j) {  // stable passes, last key first
+            const fgpu_sort_key &sk = keys[j];
+            const DevCol &dc = t->cols[sk.col];
+            HIPCHK(launch_sort_key(c->stream, dc.d_values, dc.dtype, dc.d_valid, d_rank + rank_off[j], rows, n,
+                                   sk.desc != 0, sk.nulls_last != 0, ka, fa));
+            HIPCHK(launch_take_u64(c->stream, ka, ia, n, kb));
+            size_t tb = temp_u64;
+            const int bits = (dc.dtype == FGPU_UTF8 || dc.dtype == FGPU_INT32) ? 32 : 64;  // see k_sort_key
+            HIPCHK(sort_pairs_u64(temp, &tb, kb, ka, ia, ib, n, bits, c->stream));
+            std::swap(ia, ib);
+            if (dc.null_count > 0) {  // NULL rows first or last
+                HIPCHK(launch_take_u8(c->stream, fa, ia, n, fb));
+                tb = temp_u8;
+                HIPCHK(sort_pairs_u8(temp, &tb, fb, fa, ia, ib, n, c->stream));
+                std::swap(ia, ib);
+            }
+        }
+        HIPCHK(launch_take_u32(c->stream, rows, ia, n, ib));  // row numbers in sorted order
+    }
+    mark(c, 2);
+    const int64_t m = limit < 0 ? n : std::min(limit, n);
+    if (rows_out && m) {
+        std::vector<uint32_t> h((size_t)m);
+        HIPCHK(hipMemcpyAsync(h.data(), ib, (size_t)m * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < m; ++i) rows_out[i] = h[i];
+    }
+    int rc = result_from_rows(c, t, ib, m, cols, n_cols, out);
+    mark(c, 3);
+    char msg[96];
+    snprintf(msg, sizeof msg, "sort:radix keys=%d rows=%lld", n_keys, (long long)n);
+    c->plan = msg;
+    finish_timing(c, 0, false);
+    return rc;
 }
 
 int fgpu_join_materialize(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,

@@ -634,6 +634,34 @@ def execute_sort_limit(ctx: Context, table: Table, column: str, limit: int, desc
     return rows, vals
 
 
+def execute_sort(ctx: Context, table: Table, keys: Sequence, limit: int | None = None,
+                 filter: Expr | None = None, columns: Sequence[str] | None = None
+                 ) -> tuple[np.ndarray, RecordBatch]:
+    """ORDER BY keys [LIMIT limit] for any limit and number of keys
+    (execute_sort_limit, src/query/sort.rs:59-115).  keys are (column, desc)
+    or (column, desc, nulls_last); nulls_last defaults to desc (SortExpr::desc
+    / ::asc).  Returns the row numbers in output order and the rows' `columns`
+    (default: the sort columns)."""
+    p, keep = _pred(filter, table)
+    ks = (_abi.SortKey * max(len(keys), 1))()
+    for i, k in enumerate(keys):
+        ks[i].col = table.column_index(k[0])
+        ks[i].desc = int(k[1])
+        ks[i].nulls_last = int(k[2]) if len(k) > 2 else int(k[1])
+    if columns is None:
+        columns = list(dict.fromkeys(k[0] for k in keys))
+    ci = [table.column_index(c) for c in columns]
+    arr = (C.c_int32 * max(len(ci), 1))(*ci)
+    lim = -1 if limit is None else int(limit)
+    cap = table.n_rows if lim < 0 else min(lim, table.n_rows)
+    rows = np.zeros(max(cap, 1), np.int64)
+    h = C.c_void_p()
+    _check(_lib().fgpu_sort_limit(ctx.handle, table.handle, p, ks, len(keys), lim, arr, len(ci),
+                                  rows.ctypes.data, C.byref(h)))
+    m = int(_lib().fgpu_result_rows(h))
+    return rows[:m], _read_result(h, [None] * len(ci))
+
+
 def execute_agg_only(ctx: Context, table: Table, aggs: Sequence[AggExpr],
                      filter: Expr | None = None) -> RecordBatch:
     """Aggregates without GROUP BY: one row (src/query/aggregate.rs:1394)."""

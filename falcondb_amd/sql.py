@@ -33,7 +33,7 @@ from . import _abi
 from .operators import (AggExpr, Context, Expr, FalconError, GroupByExec, JoinIndex, RecordBatch, Table, cmp,
                         col, count_star, execute_agg_only, execute_hash_join,
                         execute_hash_join_projected, execute_select,
-                        execute_sort_limit, lit_cmp)
+                        execute_sort, execute_sort_limit, lit_cmp)
 
 PARSE, INVALID_QUERY, UNSUPPORTED = 4, 6, 102  # FalconError codes
 
@@ -100,8 +100,9 @@ class Query:
     join: tuple | None = None          # (type, table, alias, left ColName, right ColName)
     where: object = None               # predicate tree
     group_by: ColName | None = None
-    order_by: tuple | None = None      # (ColName, desc)
+    order_by: list | None = None       # [(ColName, desc, nulls_last)]
     limit: int | None = None
+    offset: int = 0
     exists: list = field(default_factory=list)
 
 
@@ -121,6 +122,15 @@ class _Parser:
     def accept(self, kind: str, val: str | None = None) -> bool:
         k, v = self.peek()
         if k == kind and (val is None or v == val):
+            self.i += 1
+            return True
+        return False
+
+    def accept_word(self, word: str) -> bool:
+        """Takes an unquoted name spelling `word` (a keyword only where it
+        is read, so columns may still be called first, last ...)."""
+        k, v = self.peek()
+        if k == "name" and not v.startswith('"') and v.upper() == word:
             self.i += 1
             return True
         return False
@@ -254,18 +264,37 @@ class _Parser:
                 raise FalconError(UNSUPPORTED, "SQL: GROUP BY over one column only")
         if self.accept("kw", "ORDER"):
             self.expect("kw", "BY")
-            c = self.colname()
-            desc = False
-            if self.accept("kw", "DESC"):
-                desc = True
-            else:
-                self.accept("kw", "ASC")
-            q.order_by = (c, desc)
+            q.order_by = []
+            while True:
+                c = self.colname()
+                desc = False
+                if self.accept("kw", "DESC"):
+                    desc = True
+                else:
+                    self.accept("kw", "ASC")
+                # NULLS FIRST / LAST; the default puts NULLs last for DESC only
+                # (translate_order_by, src/query/sql.rs:1480-1492)
+                nulls_last = desc
+                if self.accept_word("NULLS"):
+                    if self.accept_word("FIRST"):
+                        nulls_last = False
+                    elif self.accept_word("LAST"):
+                        nulls_last = True
+                    else:
+                        raise FalconError(PARSE, "SQL: NULLS takes FIRST or LAST")
+                q.order_by.append((c, desc, nulls_last))
+                if not self.accept("punct", ","):
+                    break
         if self.accept("kw", "LIMIT"):
             n = self.literal()
             if not isinstance(n, int) or n < 0:
                 raise FalconError(PARSE, "SQL: LIMIT takes a non-negative integer")
             q.limit = n
+        if self.accept_word("OFFSET"):
+            n = self.literal()
+            if not isinstance(n, int) or n < 0:
+                raise FalconError(PARSE, "SQL: OFFSET takes a non-negative integer")
+            q.offset = n
         if self.peek()[0] != "end":
             raise FalconError(UNSUPPORTED, f"SQL: unsupported clause at '{self.peek()[1]}'")
         return q
@@ -372,12 +401,12 @@ class Database:
                 if t in names_right:
                     return "build"
                 raise FalconError(1, f"table or alias '{c.table}' not in the query")  # ColumnNotFound
-            in_left = _has_col(left, c.name)
-            in_right = right is not None and _has_col(right, c.name)
-            if in_left and in_right:
-                raise FalconError(INVALID_QUERY, f"column '{c.name}' is ambiguous")
-            if in_left:
+            # A name both tables carry means the FROM table's column: the
+            # join output keeps that one as <name> and calls the other
+            # right.<name> (src/query/join.rs:877-886; tests/join_projection.rs:145-175).
+            if _has_col(left, c.name):
                 return "probe"
+            in_right = right is not None and _has_col(right, c.name)
             if in_right:
                 return "build"
             raise FalconError(1, f"column '{c.name}' not found")
@@ -395,8 +424,8 @@ class Database:
             return self._plan_aggregate(q, left, where, side, names)
         if q.order_by is not None:
             return self._plan_sort_limit(q, left, where, names)
-        if q.limit is not None:
-            raise FalconError(UNSUPPORTED, "SQL: LIMIT without ORDER BY")
+        if q.limit is not None or q.offset:
+            raise FalconError(UNSUPPORTED, "SQL: LIMIT / OFFSET without ORDER BY")
         return self._plan_select(q, left, where, names)
 
     def _predicate(self, e, side) -> Expr:
@@ -426,7 +455,7 @@ class Database:
         return out
 
     def _plan_aggregate(self, q, t, where, side, names) -> RecordBatch:
-        if q.order_by is not None or q.limit is not None:
+        if q.order_by is not None or q.limit is not None or q.offset:
             raise FalconError(UNSUPPORTED, "SQL: ORDER BY / LIMIT over an aggregate")
         aggs = self._aggs(q, side, names)
         plain = [(it, nm) for it, nm in zip(q.items, names) if it.func is None]
@@ -462,7 +491,7 @@ class Database:
         semi = jtype in (_abi.SEMI, _abi.ANTI)
         index = self.index(right_name, bk)
         star = any(it.func is None and it.arg.name == "*" for it in q.items)
-        if (q.group_by is None and q.order_by is None and q.limit is None and not star
+        if (q.group_by is None and q.order_by is None and q.limit is None and not q.offset and not star
                 and all(it.func is not None for it in q.items)):
             # aggregates only: the join is fused with them, nothing is materialised
             aggs = self._aggs(q, side, names)
@@ -479,7 +508,7 @@ class Database:
             bname = right.names[right.column_index(c.name)]
             return "right." + bname if _has_col(left, bname) else bname
         refs = [it.arg for it in q.items if it.arg is not None and it.arg.name != "*"]
-        refs += [c for c in (q.group_by, q.order_by[0] if q.order_by else None) if c is not None]
+        refs += [c for c in [q.group_by] + [k[0] for k in q.order_by or []] if c is not None]
         probe_cols, build_cols = [], []
         for c in refs:
             lst, nm = (probe_cols, left.names[left.column_index(c.name)]) if side(c) == "probe" else \
@@ -503,7 +532,8 @@ class Database:
                 else:
                     items.append(Item(it.func, rename(it.arg), it.alias))
             q2 = Query(items, "", None, group_by=rename(q.group_by),
-                       order_by=(rename(q.order_by[0]), q.order_by[1]) if q.order_by else None, limit=q.limit)
+                       order_by=[(rename(k[0]),) + tuple(k[1:]) for k in q.order_by] if q.order_by else None,
+                       limit=q.limit, offset=q.offset)
             names2 = [_output_name(it, i) for i, it in enumerate(items)] if star else names
 
             def side2(c):
@@ -517,24 +547,40 @@ class Database:
             jt.free()
 
     def _plan_sort_limit(self, q, t, where, names) -> RecordBatch:
-        c, desc = q.order_by
-        if q.limit is None:
-            raise FalconError(UNSUPPORTED, "SQL: ORDER BY without LIMIT")
-        if len(q.items) != 1 or q.items[0].func is not None or q.items[0].arg.name.lower() != c.name.lower():
-            raise FalconError(UNSUPPORTED, "SQL: ORDER BY ... LIMIT selects the sort column only")
-        rows, vals = execute_sort_limit(self.ctx, t, c.name, q.limit, desc=desc, filter=where)
-        dt = "Float64" if vals.dtype == np.float64 else "Int64"
-        valid = None
-        if t.column_stats(c.name)["nulls"]:
-            # NULL rows follow the others for DESC and lead for ASC (the reference's placement)
-            i = np.arange(vals.size)
-            if desc:
-                nn = col(c.name).is_not_null()
-                valid = i < count_star(self.ctx, t, nn if where is None else where & nn)
-            else:
-                isn = col(c.name).is_null()
-                valid = i >= min(vals.size, count_star(self.ctx, t, isn if where is None else where & isn))
-        return RecordBatch([names[0]], [dt], [vals], [valid], [None])
+        """ORDER BY k1, k2 ... [LIMIT n [OFFSET m]] (execute_sort_limit,
+        src/query/sort.rs:59-115).  One key selected alone with LIMIT ≤ 32
+        takes the fused top-k (fgpu_topk); every other shape sorts the
+        passing rows on the device (fgpu_sort_limit) and gathers the selected
+        columns in output order."""
+        if any(it.func is not None for it in q.items):
+            raise FalconError(INVALID_QUERY, "SQL: aggregates need GROUP BY or no plain columns")
+        keys = [(k[0].name, k[1], k[2]) for k in q.order_by]
+        lim = None if q.limit is None else q.limit + q.offset
+        c, desc, nulls_last = keys[0]
+        if (len(keys) == 1 and nulls_last == desc and lim is not None and lim <= 32 and len(q.items) == 1
+                and q.items[0].arg.name.lower() == c.lower()
+                and t.dtypes[t.column_index(c)] != _abi.UTF8):
+            rows, vals = execute_sort_limit(self.ctx, t, c, lim, desc=desc, filter=where)
+            dt = "Float64" if vals.dtype == np.float64 else "Int64"
+            valid = None
+            if t.column_stats(c)["nulls"]:
+                # NULL rows follow the others for DESC and lead for ASC (the reference's placement)
+                i = np.arange(vals.size)
+                if desc:
+                    nn = col(c).is_not_null()
+                    valid = i < count_star(self.ctx, t, nn if where is None else where & nn)
+                else:
+                    isn = col(c).is_null()
+                    valid = i >= min(vals.size, count_star(self.ctx, t, isn if where is None else where & isn))
+            res = RecordBatch([names[0]], [dt], [vals], [valid], [None])
+        else:
+            _, res = execute_sort(self.ctx, t, keys, limit=lim, filter=where,
+                                  columns=[it.arg.name for it in q.items])
+            res = RecordBatch(names, res.dtypes, res.columns, res.validity, [None] * len(names))
+        if q.offset:
+            res = RecordBatch(res.names, res.dtypes, [a[q.offset:] for a in res.columns],
+                              [None if v is None else v[q.offset:] for v in res.validity], res.agg_funcs)
+        return res
 
     def _plan_select(self, q, t, where, names) -> RecordBatch:
         if any(it.func is not None for it in q.items):

@@ -318,6 +318,25 @@ int fgpu_join_materialize(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pre
 int fgpu_topk(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p, int32_t col, int desc,
               int32_t k, int64_t *rows_out, int64_t *vals_out, int64_t *n_out);
 
+/* ORDER BY key [ASC | DESC], ... [LIMIT limit] for any limit and any number
+ * of keys (execute_sort_limit, src/query/sort.rs:59-115; try_typed_sort
+ * :209-330; compare_scalar :807-840).  Each key orders by its column's total
+ * order (integers, doubles by f64_ord_key :451-454, Utf8 by string, i.e. by
+ * the code's rank in the sorted dictionary), reversed for DESC, with NULL rows
+ * last when nulls_last is set and first otherwise (SortExpr::desc sets it,
+ * ::asc does not, :28-34).  Rows equal on every key keep row order.  The
+ * first `limit` rows (all when limit < 0) of the rows passing p come back
+ * with columns `cols` in order; rows_out (may be NULL) receives their row
+ * numbers.  Single-key LIMIT k <= 32 is also served by fgpu_topk. */
+typedef struct {
+    int32_t col;
+    int32_t desc;
+    int32_t nulls_last;
+} fgpu_sort_key;
+int fgpu_sort_limit(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p, const fgpu_sort_key *keys,
+                    int n_keys, int64_t limit, const int32_t *cols, int n_cols, int64_t *rows_out,
+                    fgpu_result **out);
+
 /* ── batched GROUP BY ─────────────────────────────────────────────────────
  * Many independent GROUP BY queries, each over its own device table, prepared
  * once and then run as one replay of a captured hipGraph: a persistent

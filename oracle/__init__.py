@@ -52,6 +52,8 @@ def lib():
         L.ora_filter_mask.argtypes = [vp, C.c_int, i64, i32, vp, vp]
         L.ora_filter_count.argtypes = [vp, C.c_int, i64, i32, vp, C.POINTER(i64)]
         L.ora_topk.argtypes = [vp, C.c_int, i64, i32, vp, C.c_int, C.c_int, i64, vp, C.POINTER(i64)]
+        L.ora_sort_limit.argtypes = [vp, C.c_int, i64, i32, vp, vp, vp, vp, C.c_int, i64, vp, C.POINTER(i64)]
+        L.ora_sort_limit.restype = C.c_int
         L.ora_filter_select.argtypes = [vp, C.c_int, i64, i32, vp, vp, C.c_int, C.POINTER(vp)]
         L.ora_groupby.argtypes = [vp, C.c_int, i64, i32, vp, vp, C.c_int, C.POINTER(vp)]
         L.ora_join.argtypes = [vp, C.c_int, i64, vp, C.c_int, i64, i32, vp, vp, C.c_int,
@@ -210,6 +212,23 @@ def topk(columns: dict, expr: Expr | None, column: str, k: int, desc: bool = Tru
     n = C.c_int64()
     _check(lib().ora_topk(c.descs, len(c.names), c.n, batch_rows, p, c.column_index(column), int(desc), k,
                           out.ctypes.data, C.byref(n)))
+    return out[: n.value]
+
+
+def sort_limit(columns: dict, expr: Expr | None, keys: list, limit: int | None = None,
+               batch_rows: int = 65536) -> np.ndarray:
+    """Row indices of ORDER BY keys [LIMIT limit]; keys are (column, desc) or
+    (column, desc, nulls_last) with nulls_last defaulting to desc."""
+    c = _Cols(columns)
+    p, keep = c.pred(expr)
+    kc = np.array([c.column_index(k[0]) for k in keys], np.int32)
+    dk = np.array([int(k[1]) for k in keys], np.int32)
+    nl = np.array([int(k[2]) if len(k) > 2 else int(k[1]) for k in keys], np.int32)
+    out = np.zeros(max(c.n, 1), np.int64)
+    n = C.c_int64()
+    _check(lib().ora_sort_limit(c.descs, len(c.names), c.n, batch_rows, p, kc.ctypes.data, dk.ctypes.data,
+                                nl.ctypes.data, len(keys), -1 if limit is None else limit, out.ctypes.data,
+                                C.byref(n)))
     return out[: n.value]
 
 

@@ -1384,6 +1384,70 @@ int ora_topk(const fgpu_column_desc *cd, int ncols, int64_t n, int32_t batch_row
     return FGPU_OK;
 }
 
+// ORDER BY k1 [ASC | DESC], k2 ... [LIMIT limit] over the rows that pass p
+// (execute_sort_limit, src/query/sort.rs:59-115): each key compares rows by
+// its column's total order -- integers exactly, doubles by f64_ord_key
+// (:451-454), Utf8 by the string (Rust's byte-wise String order) -- reversed
+// for DESC, with NULL rows after the others when nulls_last and before them
+// otherwise (compare_scalar :807-815); rows equal on every key keep row order
+// (the typed single-key path's (key, row) pairs, :209-330).
+int ora_sort_limit(const fgpu_column_desc *cd, int ncols, int64_t n, int32_t batch_rows, const fgpu_pred *p,
+                   const int32_t *key_cols, const int32_t *desc, const int32_t *nulls_last, int n_keys, int64_t limit,
+                   int64_t *rows_out, int64_t *n_out) {
+    if (batch_rows <= 0 || batch_rows % 64) return fail(FGPU_E_INVALID_QUERY, "batch_rows must be a positive multiple of 64");
+    auto cols = make_cols(cd, ncols);
+    for (int j = 0; j < n_keys; ++j)
+        if (key_cols[j] < 0 || key_cols[j] >= ncols) return fail(FGPU_E_COLUMN_NOT_FOUND, "sort column not found");
+    Pred pred;
+    if (!build_pred(p, cols, pred)) return fail(FGPU_E_INVALID_QUERY, "malformed predicate");
+    std::vector<int64_t> rows;
+    std::vector<uint64_t> m;
+    for (int64_t b = 0; b < n; b += batch_rows) {
+        const int64_t e = std::min(n, b + (int64_t)batch_rows);
+        eval_mask(pred, cols, b, e, m);
+        for (int64_t i = b; i < e; ++i)
+            if ((m[(i - b) >> 6] >> ((i - b) & 63)) & 1) rows.push_back(i);
+    }
+    // compares row a with row b on key j: <0, 0, >0 before DESC / NULL placement
+    auto cmp_key = [&](int j, int64_t a, int64_t b) -> int {
+        const Col &c = cols[key_cols[j]];
+        const bool na = c.is_null(a), nb = c.is_null(b);
+        if (na || nb) {
+            if (na && nb) return 0;
+            const int r = na ? 1 : -1;  // NULL after the other row
+            return nulls_last[j] ? r : -r;
+        }
+        int r = 0;
+        if (c.dtype == FGPU_UTF8) {
+            const uint32_t ca = ((const uint32_t *)c.values)[a], cb = ((const uint32_t *)c.values)[b];
+            r = std::strcmp(c.dict[ca], c.dict[cb]);  // byte order: strcmp compares as unsigned char
+            r = r < 0 ? -1 : r > 0 ? 1 : 0;
+        } else if (c.dtype == FGPU_FLOAT64) {
+            uint64_t xa, xb;
+            std::memcpy(&xa, (const double *)c.values + a, 8);
+            std::memcpy(&xb, (const double *)c.values + b, 8);
+            xa = (xa >> 63) == 0 ? xa ^ (1ull << 63) : ~xa;
+            xb = (xb >> 63) == 0 ? xb ^ (1ull << 63) : ~xb;
+            r = xa < xb ? -1 : xa > xb ? 1 : 0;
+        } else {
+            const int64_t va = c.i64(a), vb = c.i64(b);
+            r = va < vb ? -1 : va > vb ? 1 : 0;
+        }
+        return desc[j] ? -r : r;
+    };
+    std::stable_sort(rows.begin(), rows.end(), [&](int64_t a, int64_t b) {
+        for (int j = 0; j < n_keys; ++j) {
+            const int r = cmp_key(j, a, b);
+            if (r) return r < 0;
+        }
+        return false;
+    });
+    const int64_t k = limit < 0 ? (int64_t)rows.size() : std::min<int64_t>(limit, (int64_t)rows.size());
+    std::copy(rows.begin(), rows.begin() + k, rows_out);
+    *n_out = k;
+    return FGPU_OK;
+}
+
 void ora_free(void *p) { std::free(p); }
 
 // ─── result accessors ───────────────────────────────────────────────────────

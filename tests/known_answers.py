@@ -255,3 +255,73 @@ def check_int_sum(engine, key_rows: int = 3) -> int:
         _check_total(f"{name} / inner", _as_total(r.columns[0][0]), exact)
         checks += 2
     return checks
+
+
+# ─── ORDER BY [LIMIT [OFFSET]] ────────────────────────────────────────────────
+def sort_column(spec: dict):
+    """A golden column spec ({"dtype", "values"} with None for NULL) as an
+    upload-ready column; Utf8 dictionaries are in first-seen order, which
+    is not sorted, so a sort by code instead of by string shows."""
+    vals, dt = spec["values"], spec["dtype"]
+    if dt == "utf8":
+        dic = list(dict.fromkeys(v for v in vals if v is not None))
+        codes = np.array([0xFFFFFFFF if v is None else dic.index(v) for v in vals], np.uint32)
+        return DictColumn(codes, dic)
+    np_dt = np.int64 if dt == "int64" else np.float64
+    arr = np.array([0 if v is None else v for v in vals], np_dt)
+    if any(v is None for v in vals):
+        return NullableColumn(arr, np.array([v is not None for v in vals]))
+    return arr
+
+
+def column_values(c, rows) -> list:
+    """Python values (None for NULL) of column c at the given rows."""
+    rows = np.asarray(rows, np.int64)
+    if isinstance(c, DictColumn):
+        return [None if c.codes[r] == 0xFFFFFFFF else c.dictionary[c.codes[r]] for r in rows]
+    if isinstance(c, NullableColumn):
+        return [c.values[r].item() if c.valid[r] else None for r in rows]
+    return [c[r].item() for r in rows]
+
+
+def check_sort(engine) -> int:
+    """The reference's ORDER BY known answers through engine.sort(cols, pred,
+    keys, limit) -> row numbers in output order; OFFSET drops the first rows
+    of a sort kept to limit + offset (tests/sort_and_dedup_paths.rs:116-121)."""
+    g = GOLDEN["sort"]
+    n = 0
+    for case in g["cases"]:
+        cols = {k: sort_column(v) for k, v in case["cols"].items()}
+        keys = [tuple(k) for k in case["keys"]]
+        lim, off = case.get("limit"), case.get("offset", 0)
+        rows = engine.sort(cols, None, keys, None if lim is None else lim + off)[off:]
+        for name, want in case.get("out", {}).items():
+            assert column_values(cols[name], rows) == want, case["src"]
+        for name, at in case.get("out_at", {}).items():
+            got = column_values(cols[name], rows)
+            for i, want in at.items():
+                assert got[int(i)] == want, case["src"]
+        n += 1
+    cf = g["closed_forms"]
+    v = np.arange(80000, dtype=np.int64)[::-1] % 1000
+    got = column_values(v, engine.sort({"v": v}, None, [("v", False)], None))
+    assert len(got) == cf["large_sort"]["rows"] and got[0] == cf["large_sort"]["first"]
+    assert got[-1] == cf["large_sort"]["last"] and got == sorted(got)
+    v = (np.arange(5000, dtype=np.int64) * 7919) % 1000
+    full = engine.sort({"v": v}, None, [("v", False)], None)
+    top = engine.sort({"v": v}, None, [("v", False)], cf["top_n_matches_full"]["limit"])
+    assert top.tolist() == full[:25].tolist()
+    s = [f"{(i * 37) % 64:03d}" for i in range(64)]
+    sc = sort_column({"dtype": "utf8", "values": s})
+    assert column_values(sc, engine.sort({"s": sc}, None, [("s", False)], None)) == sorted(s)
+    m = 2 * 8192
+    f = ((np.arange(m) * 7919) % m).astype(np.float64) - m / 2
+    got = column_values(f, engine.sort({"score": f}, None, [("score", False)], None))
+    assert all(a <= b for a, b in zip(got, got[1:]))
+    i = np.arange(1, 21, dtype=np.int64)
+    wide = {"a": i, "b": 2 * i, "c": 3 * i}
+    got = column_values(i, engine.sort(wide, col("b") > 20, [("a", False)], None))
+    assert len(got) == cf["wide_filter"]["rows"] and got[0] == cf["wide_filter"]["first"]
+    got = column_values(i, engine.sort(wide, None, [("c", True)], None))
+    assert len(got) == cf["wide_order_by_unselected"]["rows"] and got[0] == cf["wide_order_by_unselected"]["first"]
+    return n + 6

@@ -116,8 +116,9 @@ def test_join_projection_known_answers(ctx):
         d.sql("SELECT COUNT(*) AS n FROM l JOIN r ON l.k = r.rk WHERE rv > 1000")
     res = d.sql("SELECT tag, COUNT(*) AS n FROM l JOIN r ON l.k = r.rk GROUP BY tag")
     assert res.num_rows == g["group_by_tag_groups"] and int(res.columns[1].sum()) == g["group_by_tag_total"]
-    res = d.sql("SELECT lv FROM l JOIN r ON l.k = r.rk ORDER BY rv DESC LIMIT 10")
+    res = d.sql("SELECT lv FROM l JOIN r ON l.k = r.rk ORDER BY rv DESC")
     assert res.num_rows == 10 and res.columns[0][0] == g["order_by_rv_desc_first_lv"]
+    assert res.columns[0].tolist() == list(range(360, -1, -40))
     res = d.sql("SELECT * FROM l JOIN r ON l.k = r.rk")
     assert res.num_rows == 10 and len(res.names) == g["select_star_columns"]
     sh = g["shared_names"]

@@ -20,6 +20,10 @@ Parity status of the oracle's exported functions:
   exact integer SUM columns            pinned: int_sum known answers through
       scalar, GROUP BY and join (exact int64, or the correctly rounded double
       past int64, checked against Python's exact integers)
+  ora_sort_limit                       pinned: the reference's ORDER BY known
+      answers (tests/sort_and_dedup_paths.rs, src/query/sort.rs unit tests,
+      tests/projection_pushdown.rs:42-56), Python's stable sorted() with the
+      same keys on random multi-key inputs, and the float total order
   concat_dicts (Python)                pinned: closed form of the merged dictionary
       (ordered union of the batches' dictionaries), every row decoding to the
       same string, the shared-dictionary fast path, and a planted row-order
@@ -35,7 +39,7 @@ import oracle as ora
 from falcondb_amd._abi import ANTI, AVG, COUNT, INNER, LEFT, MAX, MIN, NO_ROW, SEMI, SUM
 from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
 
-from known_answers import GOLDEN, check_all, check_int_sum
+from known_answers import GOLDEN, check_all, check_int_sum, check_sort, column_values
 
 
 class OracleEngine:
@@ -47,6 +51,9 @@ class OracleEngine:
 
     def join(self, pcols, bcols, pk, bk, jt, aggs):
         return ora.join(pcols, bcols, pk, bk, jt, aggs).exact
+
+    def sort(self, cols, pred, keys, limit):
+        return ora.sort_limit(cols, pred, keys, limit)
 
 
 def test_reference_known_answers():
@@ -274,6 +281,74 @@ def test_topk_brute_force(seed):
             rows = [i for i in range(n) if f[i] < 6]
             exp = sorted(rows, key=lambda i: (-v[i] if desc else v[i], i))[:k]
             assert ora.topk(cols, col("f") < 6, "v", k, desc).tolist() == exp
+
+
+def test_sort_reference_known_answers():
+    assert check_sort(OracleEngine()) >= 25
+
+
+def _sort_key_fn(cols, name, desc, nulls_last):
+    """Python's sort key for one ORDER BY key: NULLs placed by nulls_last,
+    values by their total order (f64_ord_key for doubles, byte order for
+    strings), negated or inverted for DESC so that one ascending stable
+    sorted() reproduces the reference's comparator."""
+    c = cols[name]
+    vals = column_values(c, np.arange(len(c.codes) if isinstance(c, DictColumn) else
+                                      len(c.values) if isinstance(c, NullableColumn) else len(c)))
+
+    def ordkey(v):
+        if isinstance(v, float):
+            b = int(np.array([v]).view(np.uint64)[0])
+            return (b ^ (1 << 63)) if b >> 63 == 0 else (~b & ((1 << 64) - 1))
+        if isinstance(v, str):
+            return v.encode()
+        return v
+
+    def key(r):
+        v = vals[r]
+        if v is None:
+            return (0 if not nulls_last else 2, 0)
+        k = ordkey(v)
+        if desc:
+            k = bytes(255 - x for x in k) + b"\xff" if isinstance(k, bytes) else -k
+        return (1, k)
+    return key
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_sort_brute_force(seed):
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(1, 2500))
+    words = ["", "a", "ab", "b", "ba", "zz", "Z", "é", "a\x7f"]
+    cols = {
+        "i": NullableColumn(rng.integers(-5, 5, n), rng.random(n) > 0.2),
+        "f": NullableColumn(rng.choice([-np.inf, -1.5, -0.0, 0.0, 2.0, np.inf, np.nan], n), rng.random(n) > 0.1),
+        "s": DictColumn(np.where(rng.random(n) < 0.15, 0xFFFFFFFF, rng.integers(0, len(words), n)).astype(np.uint32),
+                        words),
+        "w": rng.integers(0, 10, n).astype(np.int32),
+    }
+    for trial in range(12):
+        names = list(rng.permutation(["i", "f", "s", "w"])[: int(rng.integers(1, 4))])
+        keys = [(c, bool(rng.integers(0, 2)), bool(rng.integers(0, 2))) for c in names]
+        limit = [None, 0, 1, 17, n + 5][int(rng.integers(0, 5))]
+        pred = col("w") < 7 if trial % 2 else None
+        rows = [r for r in range(n) if pred is None or cols["w"][r] < 7]
+        fns = [_sort_key_fn(cols, *k) for k in keys]
+        exp = sorted(rows, key=lambda r: tuple(f(r) for f in fns))
+        exp = exp if limit is None else exp[:limit]
+        assert ora.sort_limit(cols, pred, keys, limit).tolist() == exp, (keys, limit)
+
+
+def test_sort_second_key_breaks_ties():
+    # a comparator that ignored the second key would still pass one-key checks
+    cols = {"d": sort_column_utf8(["eng", "hr", "eng"]), "s": np.array([7.0, 9.0, 8.0])}
+    assert ora.sort_limit(cols, None, [("d", False), ("s", True)]).tolist() == [2, 0, 1]
+    assert ora.sort_limit(cols, None, [("d", False)]).tolist() == [0, 2, 1]
+
+
+def sort_column_utf8(vals):
+    dic = list(dict.fromkeys(vals))
+    return DictColumn(np.array([dic.index(v) for v in vals], np.uint32), dic)
 
 
 def _batched_dicts(seed):

@@ -35,11 +35,23 @@ def test_exists_becomes_a_join_term():
 
 def test_order_by_limit_and_literal_first():
     q = parse("select v2 from fact where 40 > f order by v2 desc limit 10")
-    assert q.order_by == (ColName(None, "v2"), True) and q.limit == 10
+    assert q.order_by == [(ColName(None, "v2"), True, True)] and q.limit == 10
     assert q.where == ("cmp", ColName(None, "f"), ">", 40, True)
 
 
+def test_order_by_several_keys_nulls_offset():
+    q = parse("SELECT a, b FROM t ORDER BY b DESC NULLS FIRST, a, c ASC NULLS LAST LIMIT 5 OFFSET 2")
+    assert q.order_by == [(ColName(None, "b"), True, False), (ColName(None, "a"), False, False),
+                          (ColName(None, "c"), False, True)]
+    assert q.limit == 5 and q.offset == 2
+    # first / last / nulls stay usable as column names
+    q = parse("SELECT first, last FROM t ORDER BY last, nulls DESC")
+    assert q.order_by == [(ColName(None, "last"), False, False), (ColName(None, "nulls"), True, True)]
+
+
 @pytest.mark.parametrize("text,code", [
+    ("SELECT a FROM t ORDER BY a NULLS MIDDLE", 4),
+    ("SELECT a FROM t ORDER BY a LIMIT 3 OFFSET -1", 4),
     ("SELECT FROM t", 4),
     ("SELECT a FROM t WHERE a ~ 3", 4),
     ("SELECT SUM(*) FROM t", 4),

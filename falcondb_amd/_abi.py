@@ -21,7 +21,7 @@ E_INDEX_OUT_OF_BOUNDS, E_UNSUPPORTED = 3, 102
 
 INT32, INT64, FLOAT64, UTF8 = 1, 2, 3, 4
 EQ, NE, LT, LE, GT, GE = range(6)
-P_CMP, P_AND, P_OR, P_IS_NULL, P_IS_NOT_NULL = range(5)
+P_CMP, P_AND, P_OR, P_IS_NULL, P_IS_NOT_NULL, P_IN, P_LIKE = range(7)
 COUNT, SUM, AVG, MIN, MAX = range(5)
 INNER, LEFT, SEMI, ANTI = range(4)
 NO_ROW = 0xFFFFFFFF
@@ -42,7 +42,7 @@ class Utf8Batches(C.Structure):
 class PredNode(C.Structure):
     _fields_ = [("kind", C.c_int32), ("col", C.c_int32), ("op", C.c_int32),
                 ("lit_type", C.c_int32), ("lit_i64", C.c_int64), ("lit_f64", C.c_double),
-                ("lit_str", C.c_char_p)]
+                ("lit_str", C.c_char_p), ("n_list", C.c_int32), ("list", C.c_void_p)]
 
 
 class Pred(C.Structure):

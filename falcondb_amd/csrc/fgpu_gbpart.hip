@@ -71,8 +71,12 @@ FGPU_DEV uint32_t leaf_mask(const DPredNode nd, const int64_t (&v)[U][NPC], cons
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t code = (uint32_t)pickn<NPC>(v[u], pc);
-                m |= (code != NULL_CODE && nd.pass[code]) ? 1u << u : 0u;
+                m |= (code != NULL_CODE ? nd.pass[code] != 0 : nd.op != 0) ? 1u << u : 0u;
             }
+            return m;
+        case CMP_IN:
+#pragma unroll
+            for (int u = 0; u < U; ++u) m |= in_leaf(nd, pickn<NPC>(v[u], pc), (ok[u] >> pc) & 1u) ? 1u << u : 0u;
             return m;
         default: break;
     }

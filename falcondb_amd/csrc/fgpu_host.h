@@ -4,10 +4,41 @@
 #define FGPU_HOST_H
 
 #include <cstdint>
+#include <deque>
+#include <string>
+#include <vector>
 
 #include "falcon_gpu.h"
 
 namespace fgpu_host {
+
+// Deep copies of what predicate nodes point at (literal strings, IN lists),
+// for a query that outlives its caller's arrays.  Deques keep element
+// addresses stable as copies are added.
+struct PredCopies {
+    std::deque<std::string> strs;
+    std::deque<std::vector<int64_t>> nums;
+    std::deque<std::vector<const char *>> ptrs;
+    const char *str(const char *p) {
+        if (!p) return nullptr;
+        strs.emplace_back(p);
+        return strs.back().c_str();
+    }
+    void keep(fgpu_pred_node &nd) {
+        nd.lit_str = str(nd.lit_str);
+        if (nd.kind != FGPU_P_IN || nd.n_list <= 0 || !nd.list) return;
+        if (nd.lit_type == FGPU_UTF8) {
+            const char *const *src = (const char *const *)nd.list;
+            ptrs.emplace_back();
+            for (int i = 0; i < nd.n_list; ++i) ptrs.back().push_back(str(src[i]));
+            nd.list = ptrs.back().data();
+        } else {  // int64 or double literals, copied as 8-byte patterns
+            const int64_t *src = (const int64_t *)nd.list;
+            nums.emplace_back(src, src + nd.n_list);
+            nd.list = nums.back().data();
+        }
+    }
+};
 
 // Sets the calling thread's fgpu_last_error() message; returns `code`.
 int set_error(int code, const char *msg);

@@ -220,8 +220,13 @@ enum : int8_t {
     CMP_DICT = 4,    // Utf8 code -> per-dictionary pass table
     CMP_FALSE = 5,   // unsupported pairing: no row passes
     CMP_IS_NULL = 6,
-    CMP_IS_NOT_NULL = 7
+    CMP_IS_NOT_NULL = 7,
+    CMP_IN = 8       // [NOT] IN over a sorted list of lit_i values at `pass`
 };
+// CMP_IN: op = flavor | IN_NOT.  The list holds int64 values (IN_I64, and
+// IN_I32 with the literals narrowed `as i32`) or doubles (IN_F64 for a double
+// column, IN_I64_AS_F64 for an int64 column widened to double).
+enum : int32_t { IN_I64 = 0, IN_F64 = 1, IN_I64_AS_F64 = 2, IN_I32 = 3, IN_NOT = 4 };
 
 constexpr int MAX_PCOLS = 4;  // distinct columns one predicate may read
 
@@ -233,7 +238,7 @@ struct DPredNode {
     int32_t pc;    // index of that column in DPred::pcol (tile evaluation)
     int64_t lit_i;
     double lit_f;
-    const uint8_t *pass;  // CMP_DICT
+    const uint8_t *pass;  // CMP_DICT pass table (op = 1: a NULL code passes); CMP_IN list
 };
 struct DPred {
     int n;        // 0 = no predicate (all rows pass)
@@ -265,6 +270,35 @@ FGPU_DEV bool cmp_op(T x, T v, int op) {
     }
 }
 
+// Whether value v (an integer, or a double's bits for IN_F64) is in the
+// sorted list of a CMP_IN node: a lower-bound binary search.
+FGPU_DEV bool in_list(const DPredNode &nd, int64_t v) {
+    const int fl = nd.op & 3;
+    const int n = (int)nd.lit_i;
+    int lo = 0, hi = n;
+    if (fl == IN_I64 || fl == IN_I32) {
+        const int64_t *a = (const int64_t *)nd.pass;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo < n && a[lo] == v;
+    }
+    const double d = fl == IN_F64 ? __builtin_bit_cast(double, v) : (double)v;
+    const double *a = (const double *)nd.pass;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < d) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && a[lo] == d;
+}
+// A CMP_IN leaf for a row whose value is v (ok = not NULL).
+FGPU_DEV bool in_leaf(const DPredNode &nd, int64_t v, bool ok) {
+    return ok && (in_list(nd, v) != ((nd.op & IN_NOT) != 0));
+}
+
 FGPU_DEV bool eval_leaf(const DPredNode &nd, const DCols &c, int64_t r) {
     const int k = nd.col;
     if (nd.cls == CMP_IS_NULL) return !load_valid(c, k, r);
@@ -280,9 +314,14 @@ FGPU_DEV bool eval_leaf(const DPredNode &nd, const DCols &c, int64_t r) {
         case CMP_I64: h = cmp_op(((const int64_t *)c.v[k])[r], nd.lit_i, nd.op); break;
         case CMP_I64_AS_F64: h = cmp_op((double)((const int64_t *)c.v[k])[r], nd.lit_f, nd.op); break;
         case CMP_I32: h = cmp_op(((const int32_t *)c.v[k])[r], (int32_t)nd.lit_i, nd.op); break;
-        default: {  // CMP_DICT: NULL code fails; the pass table has one byte per entry
+        case CMP_IN: {
+            const int64_t v = c.dtype[k] == FGPU_INT32 ? (int64_t)((const int32_t *)c.v[k])[r]
+                                                      : ((const int64_t *)c.v[k])[r];
+            return in_leaf(nd, v, load_valid(c, k, r));
+        }
+        default: {  // CMP_DICT: the pass table has one byte per entry
             const uint32_t code = ((const uint32_t *)c.v[k])[r];
-            return code != NULL_CODE && nd.pass[code];
+            return code != NULL_CODE ? nd.pass[code] != 0 : nd.op != 0;
         }
     }
     // clear_null_bits: a comparison against NULL is never true
@@ -413,9 +452,10 @@ FGPU_DEV bool leaf_reg(const DPredNode &nd, int64_t v, bool ok) {
         case CMP_I64: return ok && cmp_op(v, nd.lit_i, nd.op);
         case CMP_I64_AS_F64: return ok && cmp_op((double)v, nd.lit_f, nd.op);
         case CMP_I32: return ok && cmp_op((int32_t)v, (int32_t)nd.lit_i, nd.op);
+        case CMP_IN: return in_leaf(nd, v, ok);
         default: {
             const uint32_t code = (uint32_t)v;
-            return code != NULL_CODE && nd.pass[code];
+            return code != NULL_CODE ? nd.pass[code] != 0 : nd.op != 0;
         }
     }
 }

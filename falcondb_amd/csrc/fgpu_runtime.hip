@@ -416,9 +416,53 @@ struct Binder {
     }
 };
 
+// SQL LIKE over bytes: '%' matches any run of bytes, '_' exactly one byte,
+// every other byte itself; no escape character (like_matches,
+// src/query/expr.rs:1180-1199).  Greedy with one backtrack point, which
+// decides the same language as the recursive definition.
+static bool like_match(const std::string &s, const std::string &p) {
+    size_t i = 0, j = 0, star = std::string::npos, mark = 0;
+    while (i < s.size()) {
+        if (j < p.size() && p[j] == '%') {
+            star = j++;
+            mark = i;
+        } else if (j < p.size() && (p[j] == '_' || p[j] == s[i])) {
+            ++i;
+            ++j;
+        } else if (star != std::string::npos) {
+            j = star + 1;
+            i = ++mark;
+        } else {
+            return false;
+        }
+    }
+    while (j < p.size() && p[j] == '%') ++j;
+    return j == p.size();
+}
+
+static std::string ascii_lower(std::string s) {
+    for (char &ch : s)
+        if (ch >= 'A' && ch <= 'Z') ch = (char)(ch + 32);
+    return s;
+}
+
+// Bytes a predicate node's device table takes (a Utf8 pass table, one byte
+// per dictionary entry, or a numeric IN list, eight bytes per literal), or 0.
+static size_t pred_table_bytes(const fgpu_table *t, const fgpu_pred_node &nd) {
+    if (nd.col < 0 || nd.col >= (int)t->cols.size()) return 0;
+    const DevCol &col = t->cols[nd.col];
+    size_t b = 0;
+    if (col.dtype == FGPU_UTF8 && ((nd.kind == FGPU_P_CMP && nd.lit_type == FGPU_UTF8) || nd.kind == FGPU_P_LIKE ||
+                                   (nd.kind == FGPU_P_IN && nd.lit_type == FGPU_UTF8)))
+        b = col.dict.size();
+    else if (nd.kind == FGPU_P_IN && col.dtype != FGPU_UTF8 && nd.n_list > 0)
+        b = (size_t)nd.n_list * 8;
+    return (b + 255) & ~size_t(255);
+}
+
 // Compiles a predicate (postfix program) into device form.  Utf8 pass tables
-// go to the context's arena, or, when pass_dst is given, to pass_dst (at most
-// pass_cap bytes; *pass_used receives the bytes taken).
+// and IN lists go to the context's arena, or, when pass_dst is given, to
+// pass_dst (at most pass_cap bytes; *pass_used receives the bytes taken).
 int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b, DPred &out,
                  uint8_t *pass_dst = nullptr, size_t pass_cap = 0, size_t *pass_used = nullptr) {
     if (pass_used) *pass_used = 0;
@@ -430,12 +474,7 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
         return set_err(FGPU_E_INVALID_QUERY, "predicate has %d nodes (max %d)", p->n_nodes, FGPU_MAX_PRED_NODES);
     // dictionary pass tables live in one device buffer
     size_t pass_bytes = 0;
-    for (int i = 0; i < p->n_nodes; ++i) {
-        const auto &nd = p->nodes[i];
-        if (nd.kind == FGPU_P_CMP && nd.col >= 0 && nd.col < (int)t->cols.size() &&
-            t->cols[nd.col].dtype == FGPU_UTF8 && nd.lit_type == FGPU_UTF8)
-            pass_bytes += (t->cols[nd.col].dict.size() + 255) & ~size_t(255);
-    }
+    for (int i = 0; i < p->n_nodes; ++i) pass_bytes += pred_table_bytes(t, p->nodes[i]);
     uint8_t *d_pass = nullptr;
     std::vector<uint8_t> h_pass(pass_bytes);
     if (pass_bytes && pass_dst) {
@@ -460,7 +499,8 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
             --depth;
             continue;
         }
-        if (nd.kind != FGPU_P_CMP && nd.kind != FGPU_P_IS_NULL && nd.kind != FGPU_P_IS_NOT_NULL)
+        if (nd.kind != FGPU_P_CMP && nd.kind != FGPU_P_IS_NULL && nd.kind != FGPU_P_IS_NOT_NULL &&
+            nd.kind != FGPU_P_IN && nd.kind != FGPU_P_LIKE)
             return set_err(FGPU_E_INVALID_QUERY, "unknown predicate node kind %d", nd.kind);
         if (nd.col < 0 || nd.col >= (int)t->cols.size())
             return set_err(FGPU_E_COLUMN_NOT_FOUND, "predicate column %d not in table", nd.col);
@@ -478,6 +518,87 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
         ++depth;
         if (nd.kind == FGPU_P_IS_NULL) { o.cls = CMP_IS_NULL; continue; }
         if (nd.kind == FGPU_P_IS_NOT_NULL) { o.cls = CMP_IS_NOT_NULL; continue; }
+        if (nd.kind == FGPU_P_LIKE) {
+            const DevCol &col = t->cols[nd.col];
+            if (col.dtype != FGPU_UTF8) return set_err(FGPU_E_UNSUPPORTED, "LIKE on a column that is not Utf8");
+            if (nd.op < 0 || nd.op > 3) return set_err(FGPU_E_INVALID_QUERY, "bad LIKE flags %d", nd.op);
+            const bool neg = nd.op & 1, fold = nd.op & 2;
+            const std::string pat = fold ? ascii_lower(nd.lit_str ? nd.lit_str : "") : std::string(nd.lit_str ? nd.lit_str : "");
+            for (size_t k = 0; k < col.dict.size(); ++k)
+                h_pass[pass_off + k] = like_match(fold ? ascii_lower(col.dict[k]) : col.dict[k], pat) != neg;
+            o.cls = CMP_DICT;
+            o.op = neg ? 1 : 0;  // a NULL row passes NOT LIKE (src/query/expr.rs:732-736)
+            o.pass = d_pass + pass_off;
+            pass_off += pred_table_bytes(t, nd);
+            continue;
+        }
+        if (nd.kind == FGPU_P_IN) {
+            if (nd.op != 0 && nd.op != 1) return set_err(FGPU_E_INVALID_QUERY, "bad IN flags %d", nd.op);
+            if (nd.n_list < 0 || (nd.n_list > 0 && !nd.list))
+                return set_err(FGPU_E_INVALID_QUERY, "IN list of %d literals at NULL", nd.n_list);
+            const bool neg = nd.op == 1;
+            const DevCol &col = t->cols[nd.col];
+            const int lt = nd.lit_type;
+            const bool lit_int = lt == FGPU_INT64 || lt == FGPU_INT32;
+            if (col.dtype == FGPU_UTF8 && lt == FGPU_UTF8) {
+                std::vector<uint8_t> member(col.dict.size(), 0);
+                const char *const *lits = (const char *const *)nd.list;
+                for (size_t k = 0; k < col.dict.size(); ++k)
+                    for (int m = 0; m < nd.n_list && !member[k]; ++m)
+                        member[k] = std::strcmp(col.dict[k].c_str(), lits[m] ? lits[m] : "") == 0;
+                for (size_t k = 0; k < col.dict.size(); ++k) h_pass[pass_off + k] = member[k] != neg;
+                o.cls = CMP_DICT;
+                o.op = 0;  // NULL never passes IN, and NOT IN drops the NULL rows
+                o.pass = d_pass + pass_off;
+                pass_off += pred_table_bytes(t, nd);
+                continue;
+            }
+            // the `=` pairing of each literal with the column (fast_cmp_mask)
+            int flavor = -1;
+            if (col.dtype == FGPU_FLOAT64 && (lt == FGPU_FLOAT64 || lit_int)) flavor = IN_F64;
+            else if (col.dtype == FGPU_INT64 && lit_int) flavor = IN_I64;
+            else if (col.dtype == FGPU_INT64 && lt == FGPU_FLOAT64) flavor = IN_I64_AS_F64;
+            else if (col.dtype == FGPU_INT32 && lit_int) flavor = IN_I32;
+            if (flavor < 0 || nd.n_list == 0) {
+                // no member can match: IN passes nothing, NOT IN every non-NULL row
+                o.cls = neg ? CMP_IS_NOT_NULL : CMP_FALSE;
+                pass_off += pred_table_bytes(t, nd);
+                continue;
+            }
+            int64_t n_vals = 0;
+            int64_t *dst = (int64_t *)(h_pass.data() + pass_off);
+            if (flavor == IN_I64 || flavor == IN_I32) {
+                std::vector<int64_t> v((size_t)nd.n_list);
+                for (int m = 0; m < nd.n_list; ++m) {
+                    const int64_t x = ((const int64_t *)nd.list)[m];
+                    v[m] = flavor == IN_I32 ? (int64_t)(int32_t)x : x;  // `as i32`
+                }
+                std::sort(v.begin(), v.end());
+                v.erase(std::unique(v.begin(), v.end()), v.end());
+                n_vals = (int64_t)v.size();
+                std::memcpy(dst, v.data(), v.size() * 8);
+            } else {
+                std::vector<double> v;
+                for (int m = 0; m < nd.n_list; ++m) {
+                    const double x = lit_int ? (double)((const int64_t *)nd.list)[m] : ((const double *)nd.list)[m];
+                    if (x == x) v.push_back(x == 0.0 ? 0.0 : x);  // NaN equals nothing; -0.0 == 0.0
+                }
+                std::sort(v.begin(), v.end());
+                v.erase(std::unique(v.begin(), v.end()), v.end());
+                n_vals = (int64_t)v.size();
+                std::memcpy(dst, v.data(), v.size() * 8);
+            }
+            if (n_vals == 0) {
+                o.cls = neg ? CMP_IS_NOT_NULL : CMP_FALSE;
+            } else {
+                o.cls = CMP_IN;
+                o.op = flavor | (neg ? IN_NOT : 0);
+                o.lit_i = n_vals;
+                o.pass = d_pass + pass_off;
+            }
+            pass_off += pred_table_bytes(t, nd);
+            continue;
+        }
         if (nd.op < FGPU_EQ || nd.op > FGPU_GE) return set_err(FGPU_E_INVALID_QUERY, "bad comparison op %d", nd.op);
         o.op = (int8_t)nd.op;
         const DevCol &col = t->cols[nd.col];
@@ -512,8 +633,9 @@ int compile_pred(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, Binder &b
                 }
                 h_pass[pass_off + k] = h;
             }
+            o.op = 0;  // a NULL code fails
             o.pass = d_pass + pass_off;
-            pass_off += (col.dict.size() + 255) & ~size_t(255);
+            pass_off += pred_table_bytes(t, nd);
         } else {
             o.cls = CMP_FALSE;  // unsupported pairing: no rows pass
         }
@@ -2779,7 +2901,8 @@ struct BatchQ {
     const fgpu_table *t = nullptr;
     int fallback = 0;  // a shape the batch kernels do not take: runs through fgpu_groupby
     std::vector<fgpu_pred_node> nodes;
-    std::vector<std::string> strs;  // literal strings and aliases the copies point at
+    std::vector<std::string> strs;  // aliases the copies point at
+    fgpu_host::PredCopies pred_copies;  // literal strings and IN lists
     std::vector<fgpu_agg> aggs;
     fgpu_pred pred{};
     bool has_pred = false;
@@ -3113,11 +3236,7 @@ int fgpu_batch_prepare(fgpu_ctx *c, const fgpu_batch_query *queries, int n, fgpu
         Q.strs.reserve((size_t)(bq.pred ? bq.pred->n_nodes : 0) + (size_t)bq.spec->n_aggs);
         if (bq.pred) {
             Q.nodes.assign(bq.pred->nodes, bq.pred->nodes + bq.pred->n_nodes);
-            for (auto &nd : Q.nodes)
-                if (nd.lit_str) {
-                    Q.strs.emplace_back(nd.lit_str);
-                    nd.lit_str = Q.strs.back().c_str();
-                }
+            for (auto &nd : Q.nodes) Q.pred_copies.keep(nd);
             Q.pred.n_nodes = (int32_t)Q.nodes.size();
             Q.pred.nodes = Q.nodes.data();
             Q.has_pred = true;
@@ -3141,8 +3260,9 @@ int fgpu_batch_prepare(fgpu_ctx *c, const fgpu_batch_query *queries, int n, fgpu
         const DevCol &key = t->cols[kc];
         bool dict_pred = false;
         for (auto &nd : Q.nodes)
-            if (nd.kind == FGPU_P_CMP && nd.col >= 0 && nd.col < (int)t->cols.size() &&
-                t->cols[nd.col].dtype == FGPU_UTF8)
+            if ((nd.kind == FGPU_P_CMP && nd.col >= 0 && nd.col < (int)t->cols.size() &&
+                 t->cols[nd.col].dtype == FGPU_UTF8) ||
+                nd.kind == FGPU_P_IN || nd.kind == FGPU_P_LIKE)
                 dict_pred = true;
         const bool int_key = key.dtype == FGPU_INT32 || key.dtype == FGPU_INT64;
         const int64_t range = key.imin <= key.imax ? key.imax - key.imin + 1 : 1;

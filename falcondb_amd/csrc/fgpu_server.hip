@@ -51,6 +51,7 @@ struct Req {
     // deep copies the request's pointers refer to
     std::vector<fgpu_pred_node> nodes;
     std::deque<std::string> strs;  // deque: element addresses stay put
+    fgpu_host::PredCopies pred_copies;
     fgpu_pred pred{};
     std::vector<fgpu_agg> aggs;
     fgpu_groupby_spec gs{};
@@ -120,7 +121,7 @@ int copy_request(Req &q, const fgpu_request &in) {
         if (!in.pred->nodes || in.pred->n_nodes > FGPU_MAX_PRED_NODES)
             return fgpu_host::set_error(FGPU_E_INVALID_QUERY, "request predicate: bad node array");
         q.nodes.assign(in.pred->nodes, in.pred->nodes + in.pred->n_nodes);
-        for (auto &n : q.nodes) n.lit_str = keep(n.lit_str);
+        for (auto &n : q.nodes) q.pred_copies.keep(n);
         q.pred.n_nodes = in.pred->n_nodes;
         q.pred.nodes = q.nodes.data();
         q.r.pred = &q.pred;

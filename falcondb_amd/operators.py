@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (ANTI, AVG, COUNT, EQ, FLOAT64, GE, GT, INNER, INT32, INT64, LE, LEFT, LT,
-                   MAX, MIN, NE, NO_ROW, P_AND, P_CMP, P_IS_NOT_NULL, P_IS_NULL, P_OR, SEMI,
+                   MAX, MIN, NE, NO_ROW, P_AND, P_CMP, P_IN, P_IS_NOT_NULL, P_IS_NULL, P_LIKE, P_OR, SEMI,
                    SUM, UTF8)
 
 
@@ -312,7 +312,7 @@ class _Node:
     kind: int
     col: str | None = None
     op: int = 0
-    lit: object = None
+    lit: object = None       # comparison literal, LIKE pattern, or the IN list
     lit_type: int = INT64
 
 
@@ -340,7 +340,20 @@ class Expr:
             d.col = table.column_index(n.col)
             d.op = n.op
             d.lit_type = n.lit_type
-            if n.lit_type == UTF8:
+            if n.kind == P_IN:
+                vals = list(n.lit)
+                if n.lit_type == UTF8:
+                    bs = [str(v).encode() for v in vals]
+                    arr = (C.c_char_p * max(len(bs), 1))(*bs)
+                    keep.append(bs)
+                elif n.lit_type == FLOAT64:
+                    arr = (C.c_double * max(len(vals), 1))(*[float(v) for v in vals])
+                else:
+                    arr = (C.c_int64 * max(len(vals), 1))(*[int(v) for v in vals])
+                keep.append(arr)
+                d.n_list = len(vals)
+                d.list = C.cast(arr, C.c_void_p)
+            elif n.lit_type == UTF8:
                 b = str(n.lit).encode()
                 keep.append(b)
                 d.lit_str = b
@@ -388,6 +401,37 @@ class ColumnRef:
 
     def is_null(self) -> Expr:
         return Expr([_Node(P_IS_NULL, self.name)])
+
+    def like(self, pattern: str, negated: bool = False, case_insensitive: bool = False) -> Expr:
+        """`col [NOT] [I]LIKE pattern` on a Utf8 column: '%' any bytes, '_'
+        one byte, no escape (Expr::Like, src/query/expr.rs:697-754)."""
+        return Expr([_Node(P_LIKE, self.name, int(negated) | 2 * int(case_insensitive), str(pattern), UTF8)])
+
+    def not_like(self, pattern: str) -> Expr:
+        return self.like(pattern, negated=True)
+
+    def ilike(self, pattern: str) -> Expr:
+        return self.like(pattern, case_insensitive=True)
+
+    def isin(self, values, negated: bool = False) -> Expr:
+        """`col [NOT] IN (values)`: the union of the values' `=` comparisons;
+        NOT IN drops the NULL rows (Expr::InList, src/query/expr.rs:756-803).
+        Literals of different types become one list node each."""
+        groups: dict[int, list] = {}
+        for v in values:
+            lt = _lit_type(v)
+            groups.setdefault(INT64 if lt == INT32 else lt, []).append(v)
+        if not groups:
+            groups[INT64] = []
+        nodes: list[_Node] = []
+        for i, (lt, vals) in enumerate(groups.items()):
+            nodes.append(_Node(P_IN, self.name, int(negated), vals, lt))
+            if i:
+                nodes.append(_Node(P_AND if negated else P_OR))
+        return Expr(nodes)
+
+    def not_in(self, values) -> Expr:
+        return self.isin(values, negated=True)
 
     def is_not_null(self) -> Expr:
         return Expr([_Node(P_IS_NOT_NULL, self.name)])

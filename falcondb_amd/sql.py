@@ -47,7 +47,8 @@ _TOKEN = re.compile(r"""
     )""", re.X)
 
 _KEYWORDS = {"SELECT", "FROM", "WHERE", "GROUP", "BY", "ORDER", "LIMIT", "ASC", "DESC", "AND", "OR",
-             "NOT", "IS", "NULL", "JOIN", "INNER", "LEFT", "OUTER", "ON", "AS", "EXISTS"}
+             "NOT", "IS", "NULL", "JOIN", "INNER", "LEFT", "OUTER", "ON", "AS", "EXISTS", "LIKE", "ILIKE",
+             "IN"}
 _AGGS = {"COUNT": _abi.COUNT, "SUM": _abi.SUM, "AVG": _abi.AVG, "MIN": _abi.MIN, "MAX": _abi.MAX}
 
 
@@ -213,6 +214,25 @@ class _Parser:
             neg = self.accept("kw", "NOT")
             self.expect("kw", "NULL")
             return ("isnull", c, neg)
+        neg = False
+        if self.peek() == ("kw", "NOT") and self.peek(1) in (("kw", "LIKE"), ("kw", "ILIKE"), ("kw", "IN")):
+            self.i += 1
+            neg = True
+        if self.peek() in (("kw", "LIKE"), ("kw", "ILIKE")):
+            ci = self.take()[1] == "ILIKE"
+            pat = self.literal()
+            if not isinstance(pat, str):
+                raise FalconError(PARSE, "SQL: LIKE takes a string pattern")
+            return ("like", c, pat, neg, ci)
+        if self.accept("kw", "IN"):
+            self.expect("punct", "(")
+            lits = [self.literal()]
+            while self.accept("punct", ","):
+                lits.append(self.literal())
+            self.expect("punct", ")")
+            return ("in", c, lits, neg)
+        if neg:
+            raise FalconError(PARSE, "SQL: NOT takes LIKE, ILIKE or IN here")
         op = self.expect("op")
         return ("cmp", c, op, self.literal(), False)
 
@@ -429,19 +449,7 @@ class Database:
         return self._plan_select(q, left, where, names)
 
     def _predicate(self, e, side) -> Expr:
-        kind = e[0]
-        if kind == "and":
-            return self._predicate(e[1], side) & self._predicate(e[2], side)
-        if kind == "or":
-            return self._predicate(e[1], side) | self._predicate(e[2], side)
-        c = e[1]
-        if side(c) != "probe":  # the reference filters the FROM table at scan time
-            raise FalconError(UNSUPPORTED, f"SQL: WHERE on '{c.name}' of the joined table")
-        if kind == "isnull":
-            return col(c.name).is_not_null() if e[2] else col(c.name).is_null()
-        _, _, op, lit, lit_first = e
-        op = "!=" if op == "<>" else op
-        return lit_cmp(lit, op, c.name) if lit_first else cmp(c.name, op, lit)
+        return _to_expr(e, side)
 
     def _aggs(self, q: Query, side, names, skip_plain: bool = True) -> list[AggExpr]:
         out = []
@@ -587,6 +595,37 @@ class Database:
             raise FalconError(INVALID_QUERY, "SQL: aggregates need GROUP BY or no plain columns")
         res = execute_select(self.ctx, t, where, [it.arg.name for it in q.items])
         return RecordBatch(names, res.dtypes, res.columns, res.validity, [None] * len(names))
+
+
+def _to_expr(e, side=None) -> Expr:
+    """A parsed WHERE tree as an Expr; with `side`, every column must belong
+    to the FROM table (the reference filters it at scan time)."""
+    kind = e[0]
+    if kind == "and":
+        return _to_expr(e[1], side) & _to_expr(e[2], side)
+    if kind == "or":
+        return _to_expr(e[1], side) | _to_expr(e[2], side)
+    c = e[1]
+    if side is not None and side(c) != "probe":
+        raise FalconError(UNSUPPORTED, f"SQL: WHERE on '{c.name}' of the joined table")
+    if kind == "isnull":
+        return col(c.name).is_not_null() if e[2] else col(c.name).is_null()
+    if kind == "like":
+        return col(c.name).like(e[2], negated=e[3], case_insensitive=e[4])
+    if kind == "in":
+        return col(c.name).isin(e[2], negated=e[3])
+    _, _, op, lit, lit_first = e
+    op = "!=" if op == "<>" else op
+    return lit_cmp(lit, op, c.name) if lit_first else cmp(c.name, op, lit)
+
+
+def where_expr(text: str) -> Expr:
+    """The Expr of a WHERE clause over one table, e.g.
+    `where_expr("s LIKE 'b%' AND n NOT IN (1, 2)")`."""
+    q = parse(f"SELECT COUNT(*) FROM t WHERE {text}")
+    if q.exists:
+        raise FalconError(UNSUPPORTED, "SQL: EXISTS needs a query")
+    return _to_expr(q.where)
 
 
 def _expand_star(items: list, columns: list[str]) -> list:

@@ -86,7 +86,19 @@ enum {
     FGPU_P_AND = 1,         /* pops two, pushes a AND b */
     FGPU_P_OR = 2,          /* pops two, pushes a OR b */
     FGPU_P_IS_NULL = 3,
-    FGPU_P_IS_NOT_NULL = 4
+    FGPU_P_IS_NOT_NULL = 4,
+    /* col [NOT] IN (list): the union of the list's `=` comparisons, each
+     * paired with the column as `=` is; NOT IN is its complement without the
+     * NULL rows (Expr::InList in try_fast_mask, src/query/expr.rs:756-803).
+     * op = 0 (IN) or 1 (NOT IN); the n_list literals at `list` are all of
+     * lit_type: int64_t, double, or const char * for a Utf8 column. */
+    FGPU_P_IN = 5,
+    /* Utf8 col [NOT] [I]LIKE lit_str, decided once per dictionary entry with
+     * '%' (any bytes) and '_' (one byte), no escape character; a NULL row
+     * passes NOT LIKE only (Expr::Like in try_fast_mask, src/query/expr.rs:
+     * 697-754; like_matches :1180-1199).  op bit 0: NOT; bit 1: ILIKE
+     * (ASCII letters fold to lower case). */
+    FGPU_P_LIKE = 6
 };
 /* A predicate is a postfix program of at most FGPU_MAX_PRED_NODES nodes. */
 #define FGPU_MAX_PRED_NODES 16
@@ -97,7 +109,9 @@ typedef struct {
     int32_t lit_type;    /* CMP: FGPU_INT64, FGPU_INT32, FGPU_FLOAT64 or FGPU_UTF8 */
     int64_t lit_i64;
     double lit_f64;
-    const char *lit_str; /* FGPU_UTF8 literal */
+    const char *lit_str; /* FGPU_UTF8 literal; LIKE pattern */
+    int32_t n_list;      /* IN: number of literals */
+    const void *list;    /* IN: n_list literals of lit_type */
 } fgpu_pred_node;
 typedef struct {
     int32_t n_nodes;

@@ -235,6 +235,36 @@ struct Pred {
     std::vector<std::vector<uint8_t>> dict_pass;
 };
 
+// like_recursive (src/query/expr.rs:1186-1199) over bytes, memoised:
+// m[i][j] = s[i..] matches p[j..].  '%' matches zero or more bytes, '_' one
+// byte, any other byte itself.
+bool like_matches(const std::string &s, const std::string &p) {
+    const size_t ns = s.size(), np = p.size();
+    std::vector<uint8_t> m((ns + 1) * (np + 1), 0);
+    auto at = [&](size_t i, size_t j) -> uint8_t & { return m[i * (np + 1) + j]; };
+    at(ns, np) = 1;
+    for (size_t j = np; j-- > 0;) {
+        for (size_t i = ns + 1; i-- > 0;) {
+            bool r;
+            if (p[j] == '%') r = at(i, j + 1) || (i < ns && at(i + 1, j));
+            else if (i == ns) r = false;
+            else if (p[j] == '_') r = at(i + 1, j + 1);
+            else r = s[i] == p[j] && at(i + 1, j + 1);
+            at(i, j) = r;
+        }
+    }
+    return at(0, 0);
+}
+
+// ILIKE's lower-casing, ASCII letters only (str::to_lowercase folds every
+// Unicode letter; non-ASCII letters are compared as they are here).
+std::string lower_ascii(const char *x) {
+    std::string s = x ? x : "";
+    for (char &ch : s)
+        if (ch >= 'A' && ch <= 'Z') ch = (char)(ch + 32);
+    return s;
+}
+
 bool build_pred(const fgpu_pred *p, const std::vector<Col> &cols, Pred &out) {
     out.nodes.clear();
     out.dict_pass.clear();
@@ -251,6 +281,21 @@ bool build_pred(const fgpu_pred *p, const std::vector<Col> &cols, Pred &out) {
         }
         if (nd.col < 0 || nd.col >= (int)cols.size()) return false;
         ++depth;
+        if (nd.kind == FGPU_P_LIKE) {  // decided once per dictionary entry
+            const Col &c = cols[nd.col];
+            if (c.dtype != FGPU_UTF8 || nd.op < 0 || nd.op > 3) return false;
+            const bool neg = nd.op & 1, fold = nd.op & 2;
+            const std::string pat = fold ? lower_ascii(nd.lit_str) : std::string(nd.lit_str ? nd.lit_str : "");
+            auto &pass = out.dict_pass[k];
+            pass.resize((size_t)c.dict_len);
+            for (int i = 0; i < c.dict_len; ++i) {
+                const bool hit = like_matches(fold ? lower_ascii(c.dict[i]) : std::string(c.dict[i]), pat);
+                pass[i] = hit != neg;
+            }
+            continue;
+        }
+        if (nd.kind == FGPU_P_IN && (nd.op < 0 || nd.op > 1 || nd.n_list < 0 || (nd.n_list > 0 && !nd.list)))
+            return false;
         if (nd.kind == FGPU_P_CMP && cols[nd.col].dtype == FGPU_UTF8 &&
             nd.lit_type == FGPU_UTF8) {
             const Col &c = cols[nd.col];
@@ -291,8 +336,44 @@ inline bool cmp(T x, T v, int op) {
 void leaf_mask(const fgpu_pred_node &nd, const Col &c, const std::vector<uint8_t> &pass,
                int64_t b, int64_t e, uint64_t *w) {
     const int64_t n = e - b;
-    std::memset(w, 0, sizeof(uint64_t) * (size_t)((n + 63) / 64));
+    const int64_t nw = (n + 63) / 64;
+    std::memset(w, 0, sizeof(uint64_t) * (size_t)nw);
     auto set = [&](int64_t i) { w[i >> 6] |= 1ull << (i & 63); };
+    if (nd.kind == FGPU_P_LIKE) {
+        // a NULL row never matched, and NOT LIKE negates that (src/query/expr.rs:732-736)
+        const bool null_passes = nd.op & 1;
+        const uint32_t *x = (const uint32_t *)c.values + b;
+        for (int64_t i = 0; i < n; ++i)
+            if (x[i] == NULL_CODE ? null_passes : pass[x[i]] != 0) set(i);
+        return;
+    }
+    if (nd.kind == FGPU_P_IN) {
+        // the union of each literal's `=` mask (src/query/expr.rs:776-789)
+        std::vector<uint64_t> one((size_t)nw);
+        for (int m = 0; m < nd.n_list; ++m) {
+            fgpu_pred_node eq = nd;
+            eq.kind = FGPU_P_CMP;
+            eq.op = FGPU_EQ;
+            if (nd.lit_type == FGPU_UTF8) eq.lit_str = ((const char *const *)nd.list)[m];
+            else if (nd.lit_type == FGPU_FLOAT64) eq.lit_f64 = ((const double *)nd.list)[m];
+            else eq.lit_i64 = ((const int64_t *)nd.list)[m];
+            std::vector<uint8_t> eq_pass;
+            if (c.dtype == FGPU_UTF8 && nd.lit_type == FGPU_UTF8) {
+                eq_pass.resize((size_t)c.dict_len);
+                for (int i = 0; i < c.dict_len; ++i)
+                    eq_pass[i] = std::strcmp(c.dict[i], eq.lit_str ? eq.lit_str : "") == 0;
+            }
+            leaf_mask(eq, c, eq_pass, b, e, one.data());
+            for (int64_t i = 0; i < nw; ++i) w[i] |= one[i];
+        }
+        if (nd.op == 1) {  // NOT IN: the complement, without padding bits or NULL rows (:790-800)
+            for (int64_t i = 0; i < nw; ++i) w[i] = ~w[i];
+            if (n & 63) w[nw - 1] &= (1ull << (n & 63)) - 1;
+            for (int64_t i = 0; i < n; ++i)
+                if (c.is_null(b + i)) w[i >> 6] &= ~(1ull << (i & 63));
+        }
+        return;
+    }
     if (nd.kind == FGPU_P_IS_NULL || nd.kind == FGPU_P_IS_NOT_NULL) {
         const bool want_null = nd.kind == FGPU_P_IS_NULL;
         for (int64_t i = 0; i < n; ++i)

@@ -13,8 +13,7 @@ from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
 
 GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_known_answers.json").read_text())
 
-# The count_star_filter predicates as Expr builders (IN / NOT IN spelled as
-# OR / AND of comparisons, the expansion the reference's IN mask computes).
+# The count_star_filter predicates as Expr builders.
 PRED = {
     "n >= 0": lambda: col("n") >= 0,
     "n < 0": lambda: col("n") < 0,
@@ -24,8 +23,9 @@ PRED = {
     "maybe IS NOT NULL": lambda: col("maybe").is_not_null(),
     "s = 'a'": lambda: col("s") == "a",
     "s <> 'a'": lambda: col("s") != "a",
-    "n IN (1, 2, 3)": lambda: ((col("n") == 1) | (col("n") == 2)) | (col("n") == 3),
-    "n NOT IN (1, 2, 3)": lambda: ((col("n") != 1) & (col("n") != 2)) & (col("n") != 3),
+    "s LIKE 'b%'": lambda: col("s").like("b%"),
+    "n IN (1, 2, 3)": lambda: col("n").isin([1, 2, 3]),
+    "n NOT IN (1, 2, 3)": lambda: col("n").not_in([1, 2, 3]),
     "n > 5 AND n < 20": lambda: (col("n") > 5) & (col("n") < 20),
     "n < 2 OR n > 9995": lambda: (col("n") < 2) | (col("n") > 9995),
 }
@@ -52,6 +52,7 @@ def closed_form_count(pred: str, rows: int) -> int:
         "maybe IS NOT NULL": n % 3 != 0,
         "s = 'a'": n % 3 == 0,
         "s <> 'a'": n % 3 != 0,
+        "s LIKE 'b%'": n % 3 == 1,
         "n IN (1, 2, 3)": (n >= 1) & (n <= 3),
         "n NOT IN (1, 2, 3)": ~((n >= 1) & (n <= 3)),
         "n > 5 AND n < 20": (n > 5) & (n < 20),
@@ -325,3 +326,16 @@ def check_sort(engine) -> int:
     got = column_values(i, engine.sort(wide, None, [("c", True)], None))
     assert len(got) == cf["wide_order_by_unselected"]["rows"] and got[0] == cf["wide_order_by_unselected"]["first"]
     return n + 6
+
+
+# ─── LIKE / ILIKE / IN / NOT IN ───────────────────────────────────────────────
+def check_like_in(engine) -> int:
+    """The reference's LIKE and IN row counts through engine.filter_count."""
+    from falcondb_amd.sql import where_expr
+    n = 0
+    for case in GOLDEN["like_in"]["cases"]:
+        cols = {k: sort_column(v) for k, v in case["cols"].items()}
+        for text, want in case["where"].items():
+            assert engine.filter_count(cols, where_expr(text)) == want, (case["src"], text)
+            n += 1
+    return n

@@ -24,6 +24,11 @@ Parity status of the oracle's exported functions:
       answers (tests/sort_and_dedup_paths.rs, src/query/sort.rs unit tests,
       tests/projection_pushdown.rs:42-56), Python's stable sorted() with the
       same keys on random multi-key inputs, and the float total order
+  LIKE / ILIKE / IN / NOT IN leaves     pinned: the reference's LIKE and IN
+      row counts (src/query/expr.rs:2050-2082, :2415-2462, :2700-2739;
+      tests/sql_integration.rs:1189-1238; tests/count_star_filter.rs:67-69),
+      and brute force against Python's regular expressions (LIKE) and
+      membership tests (IN); ILIKE folds ASCII letters only
   concat_dicts (Python)                pinned: closed form of the merged dictionary
       (ordered union of the batches' dictionaries), every row decoding to the
       same string, the shared-dictionary fast path, and a planted row-order
@@ -39,7 +44,7 @@ import oracle as ora
 from falcondb_amd._abi import ANTI, AVG, COUNT, INNER, LEFT, MAX, MIN, NO_ROW, SEMI, SUM
 from falcondb_amd.operators import AggExpr, DictColumn, NullableColumn, col
 
-from known_answers import GOLDEN, check_all, check_int_sum, check_sort, column_values
+from known_answers import GOLDEN, check_all, check_int_sum, check_like_in, check_sort, column_values
 
 
 class OracleEngine:
@@ -281,6 +286,63 @@ def test_topk_brute_force(seed):
             rows = [i for i in range(n) if f[i] < 6]
             exp = sorted(rows, key=lambda i: (-v[i] if desc else v[i], i))[:k]
             assert ora.topk(cols, col("f") < 6, "v", k, desc).tolist() == exp
+
+
+def test_like_in_reference_known_answers():
+    assert check_like_in(OracleEngine()) >= 35
+
+
+def _like_re(pattern: str, fold: bool):
+    import re
+    pat = pattern.encode()
+    if fold:
+        pat = pat.lower()  # bytes.lower folds ASCII letters only
+    body = b"".join(b".*" if ch == ord("%") else b"." if ch == ord("_") else re.escape(bytes([ch])) for ch in pat)
+    return re.compile(body, re.DOTALL)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_like_in_brute_force(seed):
+    rng = np.random.default_rng(700 + seed)
+    words = ["", "a", "ab", "abc", "Abc", "ba", "b%a", "x_y", "xay", "é", "éa", "aaa", "banana", "BANANA"]
+    n = 3000
+    codes = rng.integers(0, len(words), n).astype(np.uint32)
+    codes[rng.random(n) < 0.1] = 0xFFFFFFFF
+    iv = rng.integers(-5, 6, n)
+    ok = rng.random(n) > 0.1
+    fv = rng.choice([-1.5, -0.0, 0.0, 1.0, 2.0, 2.5, np.nan, np.inf], n)
+    cols = {"s": DictColumn(codes, words), "i": NullableColumn(iv, ok), "f": fv,
+            "j": rng.integers(-3, 4, n).astype(np.int32)}
+    s_vals = [None if c == 0xFFFFFFFF else words[c] for c in codes]
+    for pat in ["%", "", "a%", "%a", "_", "__", "%a%a%", "b%a", "_a%", "x_y", "%é%", "A%", "%%", "a_c"]:
+        for neg in (False, True):
+            for fold in (False, True):
+                r = _like_re(pat, fold)
+                exp = sum(1 for v in s_vals if (neg if v is None else
+                          (r.fullmatch(v.encode().lower() if fold else v.encode()) is not None) != neg))
+                got = ora.filter_count(cols, col("s").like(pat, negated=neg, case_insensitive=fold))
+                assert got == exp, (pat, neg, fold)
+    for lst in ([1, 3], [-5, 5, 0], [2.0, 7], [2.5], [], ["a"], [1, 2.5]):
+        for neg in (False, True):
+            for name in ("i", "f", "j"):
+                c = cols[name]
+                vals = c.values if isinstance(c, NullableColumn) else c
+                valid = c.valid if isinstance(c, NullableColumn) else np.ones(n, bool)
+                def member(x):
+                    for v in lst:
+                        if isinstance(v, str):
+                            continue  # a text literal matches no number (an unsupported pairing)
+                        if isinstance(v, float) and name == "j":
+                            continue  # Int32 against a float: unsupported pairing
+                        if isinstance(v, int) and name == "j":
+                            if np.int32(x) == np.int32(np.int64(v).astype(np.int32)):
+                                return True
+                        elif float(x) == float(v):
+                            return True
+                    return False
+                exp = sum(1 for x, okr in zip(vals.tolist(), valid.tolist()) if okr and member(x) != neg)
+                got = ora.filter_count(cols, col(name).isin(lst, negated=neg))
+                assert got == exp, (name, lst, neg)
 
 
 def test_sort_reference_known_answers():

@@ -47,6 +47,20 @@ FGPU_HD uint64_t hash64(uint64_t h) {
     return h;
 }
 
+// Bijective mix of b-bit values (1 <= b <= 63; mask = 2^b - 1): a xorshift, a
+// multiplication by an odd constant modulo 2^b and a xorshift, each a
+// bijection of [0, 2^b).  The compact join records key on cmix(key - key_min)
+// over the build key range, so equal mixes are equal keys within that range.
+// Up to 32 bits the product needs only its low word.
+FGPU_HD uint64_t cmix(uint64_t x, int b, uint64_t mask) {
+    const int s = (b + 1) >> 1;
+    x ^= x >> s;
+    if (b <= 32) x = (uint64_t)((uint32_t)x * 0x7F4A7C15u) & mask;
+    else x = (x * 0x9E3779B97F4A7C15ull) & mask;
+    x ^= x >> s;
+    return x;
+}
+
 // ─── order-preserving encoding of doubles for atomic min / max ──────────────
 FGPU_HD uint64_t f64_order(uint64_t bits) {
     return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);

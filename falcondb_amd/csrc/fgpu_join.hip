@@ -286,18 +286,34 @@ FGPU_DEV int64_t build_key_at(const JoinBuildArgs &b, int64_t r) {
     return b.key_dtype == FGPU_INT32 ? (int64_t)((const int32_t *)b.keys_col)[r] : ((const int64_t *)b.keys_col)[r];
 }
 
+// The partition and record key of build key `key` (not NULL): false for the
+// sentinel key of the hash64 form (its rows go to the side list).  In the
+// compact form every build key is inside [kmin, kmax] by construction.
+FGPU_DEV bool build_part(const JoinPartArgs &a, int64_t key, int &p, int64_t &h) {
+    if (a.kw == 4) {
+        const uint64_t m = cmix((uint64_t)key - (uint64_t)a.kmin, a.kbits, (1ull << a.kbits) - 1);
+        p = (int)(m >> a.rbits);
+        h = (int64_t)(m & ((1ull << a.rbits) - 1));
+        return true;
+    }
+    h = (int64_t)hash64((uint64_t)key);
+    p = part_of((uint64_t)h, a.log_p);
+    return h != EMPTY_KEY;
+}
+
 __global__ void k_jp_build(JoinPartArgs a, JoinBuildArgs b) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const uint64_t tmask = (uint64_t)a.T - 1;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.n; r += stride) {
         if (b.key_valid && !((b.key_valid[r >> 6] >> (r & 63)) & 1)) continue;  // NULL joins nothing
-        const int64_t h = (int64_t)hash64((uint64_t)build_key_at(b, r));
-        if (h == EMPTY_KEY) {
+        int p;
+        int64_t h;
+        if (!build_part(a, build_key_at(b, r), p, h)) {
             const unsigned long long k = atomicAdd(a.flags + 2, 1ull);
             if ((int64_t)k < a.min_key_cap) a.min_key_rows[k] = (uint32_t)r;
             continue;
         }
-        const int p = part_of((uint64_t)h, a.log_p);  // k_jp_count checked every sub-table has room
+        // k_jp_count checked every sub-table has room
         const int64_t pay = a.build_pay_col >= 0 ? load_raw(a.pa.bcols, a.build_pay_col, r) : r;
         int64_t *sub = a.sub + (int64_t)p * a.T * 2;
         uint64_t slot = ((uint64_t)h << 2) & tmask;  // first slot of the home bucket (4 slots)
@@ -326,9 +342,10 @@ __global__ void k_jp_count(JoinPartArgs a, JoinBuildArgs b) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.n; r += stride) {
         if (b.key_valid && !((b.key_valid[r >> 6] >> (r & 63)) & 1)) continue;
-        const uint64_t h = hash64((uint64_t)build_key_at(b, r));
-        if ((int64_t)h == EMPTY_KEY) continue;
-        atomicAdd(hist + part_of(h, a.log_p), 1u);
+        int p;
+        int64_t h;
+        if (!build_part(a, build_key_at(b, r), p, h)) continue;
+        atomicAdd(hist + p, 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < P; i += blockDim.x)
@@ -402,7 +419,7 @@ struct ScatterRaw {
 // the JS_COUNT pass), so each partition's records are contiguous.
 enum { JS_CELLS = 0, JS_COUNT = 1, JS_EXACT = 2 };
 
-template <int NPC, int RW, int U, int MODE>
+template <int NPC, int RW, int U, int MODE, int KW>
 __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     extern __shared__ uint64_t js_lds[];
     __shared__ int overflow;
@@ -430,6 +447,11 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_wg;
     const int64_t r1 = r0 + a.rows_per_wg < pa.n_probe ? r0 + a.rows_per_wg : pa.n_probe;
     const int64_t G = a.G;
+    // compact records: the mix's width and the record key's
+    const uint64_t kmask = KW == 4 ? (1ull << a.kbits) - 1 : 0, rmask = KW == 4 ? (1ull << a.rbits) - 1 : 0;
+    // a row whose key matches nothing (NULL, or outside the compact range) is
+    // an output row only for LEFT and ANTI; INNER and SEMI drop it here
+    const int nomatch = a.pa.type == FGPU_LEFT || a.pa.type == FGPU_ANTI ? P2 - 2 : -1;
     auto issue = [&](int64_t base, ScatterRaw<NPC, U> &t) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -470,8 +492,15 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             cell_finish3(kc, r, raw.key[u], key, ok);
             if (STAGE && RW > 1) cell_finish3(pay0, r, raw.f1[u], f1[u], pok);
             if (STAGE && RW > 2) cell_finish3(pay1, r, raw.f2[u], f2[u], pok);
-            h[u] = (int64_t)hash64((uint64_t)key);
-            part[u] = !sel ? -1 : !ok ? P2 - 2 : h[u] == EMPTY_KEY ? P2 - 1 : part_of((uint64_t)h[u], log_p);
+            if (KW == 4) {  // keys outside the build range match nothing: the NULL-key partition
+                const bool inr = key >= a.kmin && key <= a.kmax;
+                const uint64_t m = cmix((uint64_t)key - (uint64_t)a.kmin, a.kbits, kmask);
+                h[u] = (int64_t)(m & rmask);
+                part[u] = !sel ? -1 : !ok || !inr ? nomatch : (int)(m >> a.rbits);
+            } else {
+                h[u] = (int64_t)hash64((uint64_t)key);
+                part[u] = !sel ? -1 : !ok ? nomatch : h[u] == EMPTY_KEY ? P2 - 1 : part_of((uint64_t)h[u], log_p);
+            }
         }
         if (!STAGE) {
 #pragma unroll
@@ -500,19 +529,30 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
         __syncthreads();
         for (unsigned i = threadIdx.x; i < total; i += blockDim.x) {
             const uint32_t d = dest[i];
-            int64_t *rec;
+            int64_t ri;  // record index
             if (MODE == JS_EXACT) {
-                const int64_t pos = (int64_t)cur[d] + (i - off[d]);
-                rec = a.cells + (a.xoff[(int64_t)d * G + blockIdx.x] + pos) * RW;
+                ri = a.xoff[(int64_t)d * G + blockIdx.x] + (int64_t)cur[d] + (i - off[d]);
             } else if ((d & 0x1ffffu) < (unsigned)Cw) {
-                rec = a.cells + (((int64_t)(d >> 17) * G + blockIdx.x) * Cw + (d & 0x1ffffu)) * RW;
+                ri = ((int64_t)(d >> 17) * G + blockIdx.x) * Cw + (d & 0x1ffffu);
             } else {
                 overflow = 1;  // cell full: the host redoes the partitioning exactly
                 continue;
             }
-            rec[0] = stage[(size_t)i * RW];
-            if (RW > 1) rec[1] = stage[(size_t)i * RW + 1];
-            if (RW > 2) rec[2] = stage[(size_t)i * RW + 2];
+            if (KW == 4) {
+                uint32_t *rec = (uint32_t *)a.cells + ri * (2 * RW - 1);
+                rec[0] = (uint32_t)stage[(size_t)i * RW];
+#pragma unroll
+                for (int f = 1; f < RW; ++f) {
+                    const uint64_t x = (uint64_t)stage[(size_t)i * RW + f];
+                    rec[2 * f - 1] = (uint32_t)x;
+                    rec[2 * f] = (uint32_t)(x >> 32);
+                }
+            } else {
+                int64_t *rec = a.cells + ri * RW;
+                rec[0] = stage[(size_t)i * RW];
+                if (RW > 1) rec[1] = stage[(size_t)i * RW + 1];
+                if (RW > 2) rec[2] = stage[(size_t)i * RW + 2];
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < P2; i += blockDim.x) {
@@ -706,8 +746,48 @@ FGPU_DEV void fold_group(const EmitOps &e, JoinAcc &j, const bool (&live)[R], co
     }
 }
 
-// Records of the two side partitions: NULL keys (matching nothing) and the
-// sentinel key, whose hash is EMPTY_KEY (matching the build side list).
+// The probe payloads of record ri (either record format).
+// A compact record's payload f (0-based) from its two uint32 words.
+FGPU_DEV int64_t crec_pay(const uint32_t *rec, int f) {
+    return (int64_t)((uint64_t)rec[2 * f + 1] | (uint64_t)rec[2 * f + 2] << 32);
+}
+FGPU_DEV void rec_pay(const JoinPartArgs &a, int64_t ri, int64_t &f1, int64_t &f2) {
+    if (a.kw == 4) {
+        const uint32_t *rec = (const uint32_t *)a.cells + ri * (2 * a.RW - 1);
+        f1 = a.RW > 1 ? crec_pay(rec, 0) : 0;
+        f2 = a.RW > 2 ? crec_pay(rec, 1) : 0;
+        return;
+    }
+    const int64_t *pp = a.cells + ri * a.RW + 1;
+    f1 = a.RW > 1 ? pp[0] : 0;
+    f2 = a.RW > 2 ? pp[1] : 0;
+}
+
+// One record of a side partition: sp 0 holds NULL keys (and, for compact
+// records, keys outside the build range), which match nothing; sp 1 holds the
+// sentinel key, whose hash is EMPTY_KEY, which matches the build side list.
+template <int TYPE>
+FGPU_DEV void side_record(const JoinPartArgs &a, JoinAcc &j, int64_t ri, int sp) {
+    int64_t f1, f2;
+    rec_pay(a, ri, f1, f2);
+    int64_t m = 0;
+    if (sp == 1) {  // sentinel key: every build row in the side list matches
+        for (int64_t q = 0; q < a.pa.n_min_key_rows; ++q) {
+            ++m;
+            if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
+                int64_t pay = a.pa.min_key_rows[q];
+                if (a.build_pay_col >= 0) pay = load_raw(a.pa.bcols, a.build_pay_col, pay);
+                part_emit(a, j, f1, f2, true, pay);
+            } else {
+                break;
+            }
+        }
+    }
+    if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) || (TYPE == FGPU_ANTI && m == 0))
+        part_emit(a, j, f1, f2, false, 0);
+}
+
+// Records of the two side partitions, cell mode (one workgroup takes them all).
 template <int TYPE>
 FGPU_DEV void probe_side_partitions(const JoinPartArgs &a, JoinAcc &j) {
     const int P = 1 << a.log_p, P2 = P + 2;
@@ -716,53 +796,17 @@ FGPU_DEV void probe_side_partitions(const JoinPartArgs &a, JoinAcc &j) {
         for (int sp = 0; sp < 2; ++sp) {
             unsigned n = a.cell_cnt[(int64_t)w * P2 + P + sp];
             n = n < (unsigned)a.Cw ? n : (unsigned)a.Cw;
-            const int64_t *cell = a.cells + ((int64_t)(P + sp) * a.G + w) * a.Cw * a.RW;
-            for (unsigned i = lane; i < n; i += 64) {
-                const int64_t *rec = cell + (int64_t)i * a.RW;
-                const int64_t f1 = a.RW > 1 ? rec[1] : 0, f2 = a.RW > 2 ? rec[2] : 0;
-                int64_t m = 0;
-                if (sp == 1) {  // sentinel key: every build row in the side list matches
-                    for (int64_t q = 0; q < a.pa.n_min_key_rows; ++q) {
-                        ++m;
-                        if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
-                            int64_t pay = a.pa.min_key_rows[q];
-                            if (a.build_pay_col >= 0) pay = load_raw(a.pa.bcols, a.build_pay_col, pay);
-                            part_emit(a, j, f1, f2, true, pay);
-                        } else {
-                            break;
-                        }
-                    }
-                }
-                if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) || (TYPE == FGPU_ANTI && m == 0))
-                    part_emit(a, j, f1, f2, false, 0);
-            }
+            const int64_t cell = ((int64_t)(P + sp) * a.G + w) * a.Cw;  // first record index
+            for (unsigned i = lane; i < n; i += 64) side_record<TYPE>(a, j, cell + i, sp);
         }
     }
 }
 
-// Side-partition records recs[0..n) (sp 0: NULL keys, 1: the sentinel key),
-// shared by all threads of the workgroup.
+// Side-partition records lo .. lo + n (sp 0: NULL keys, 1: the sentinel key),
+// chunk mode, shared by all threads of the workgroup.
 template <int TYPE>
-FGPU_DEV void side_records(const JoinPartArgs &a, JoinAcc &j, const int64_t *recs, int64_t n, int sp) {
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const int64_t *rec = recs + i * a.RW;
-        const int64_t f1 = a.RW > 1 ? rec[1] : 0, f2 = a.RW > 2 ? rec[2] : 0;
-        int64_t m = 0;
-        if (sp == 1) {
-            for (int64_t q = 0; q < a.pa.n_min_key_rows; ++q) {
-                ++m;
-                if (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) {
-                    int64_t pay = a.pa.min_key_rows[q];
-                    if (a.build_pay_col >= 0) pay = load_raw(a.pa.bcols, a.build_pay_col, pay);
-                    part_emit(a, j, f1, f2, true, pay);
-                } else {
-                    break;
-                }
-            }
-        }
-        if ((TYPE == FGPU_LEFT && m == 0) || (TYPE == FGPU_SEMI && m > 0) || (TYPE == FGPU_ANTI && m == 0))
-            part_emit(a, j, f1, f2, false, 0);
-    }
+FGPU_DEV void side_records(const JoinPartArgs &a, JoinAcc &j, int64_t lo, int64_t n, int sp) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) side_record<TYPE>(a, j, lo + i, sp);
 }
 
 // Pass 2.  Workgroup b serves partition p = b % P: it copies the partition's
@@ -780,16 +824,19 @@ FGPU_DEV void side_records(const JoinPartArgs &a, JoinAcc &j, const int64_t *rec
 // A two-word record as one 16-byte vector (records are read once, non-temporal).
 typedef long long RecPair __attribute__((ext_vector_type(2)));
 
-template <int TYPE, bool FAST, int RW>
+template <int TYPE, bool FAST, int RW, int KW>
 __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     extern __shared__ uint64_t jp_lds[];
     const int P = 1 << a.log_p, P2 = P + 2;
     const int T = a.T, Cw = a.Cw, G = a.G;
-    const bool vals = a.need_vals != 0;                     // an aggregate reads the build side
-    int64_t *tk = (int64_t *)jp_lds;                        // T slot hashes
-    int64_t *tv = tk + T;                                   // T build values / rows (if vals)
-    uint64_t *lrow = jp_lds + (vals ? 2 : 1) * (size_t)T;   // 64 words
-    unsigned *cnt = (unsigned *)(lrow + 64);                // G: records of each cell (or wave slice)
+    const bool vals = a.need_vals != 0;  // an aggregate reads the build side
+    // LDS: the T slot keys (int64 hashes, or uint32 compact record keys), then
+    // the T build values / rows (if vals), a 64-word state row and the counts
+    int64_t *tk = (int64_t *)jp_lds;
+    uint32_t *tk32 = (uint32_t *)jp_lds;
+    int64_t *tv = KW == 4 ? (int64_t *)(tk32 + T) : tk + T;
+    uint64_t *lrow = (uint64_t *)(tv + (vals ? T : 0));
+    unsigned *cnt = (unsigned *)(lrow + 64);  // G: records of each cell (or wave slice)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     // Cell mode: workgroup b serves partition b % P, cells [w0, w1) of it.
     // Chunk mode (exact partitioning): workgroup b serves chunks[b] = {p, lo,
@@ -814,7 +861,8 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
         const longlong2 *src = (const longlong2 *)(a.sub + (int64_t)p * T * 2);
         for (int i = threadIdx.x; i < T; i += blockDim.x) {
             const longlong2 sl = src[i];
-            tk[i] = sl.x;
+            if (KW == 4) tk32[i] = sl.x == EMPTY_KEY ? EMPTY32 : (uint32_t)sl.x;
+            else tk[i] = sl.x;
             if (vals) tv[i] = sl.y;
         }
     }
@@ -832,14 +880,16 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     JoinAcc j;
     acc_init(a.pa, j, lrow);  // synchronises the workgroup (table and counts are in place)
     if (p >= P) {  // chunk of the NULL-key or sentinel-key side partition
-        side_records<TYPE>(a, j, a.cells + lo * RW, cn, p - P);
+        side_records<TYPE>(a, j, lo, cn, p - P);
         acc_finish(a.pa, j, lrow);
         return;
     }
     const EmitOps e = emit_ops(a);
     const bool unique = a.pa.unique_keys != 0;
     const uint32_t tmask = (uint32_t)T - 1;
-    constexpr int R = 4;  // records per lane per group
+    // records per lane per group: compact records are 4 (+ 8 per payload)
+    // bytes, so twice as many are kept in flight
+    constexpr int R = KW == 4 ? 8 : 4;
     // (w, i0): the wave's current cell and first record of the group
     int w = w0 + wave;
     unsigned i0 = 0, n = w < w1 ? cnt[w] : 0;
@@ -853,23 +903,29 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
     // chunk, which always exists), so the number of loads in flight is fixed
     // and the wait before probing a group does not also wait for the next
     // group's loads.
-    const int64_t *cell0 = chunked ? a.cells + lo * RW : a.cells + (int64_t)p * G * Cw * RW;
+    const int64_t cell0 = chunked ? lo : (int64_t)p * G * Cw;  // record index of the first record
+    const uint32_t *k32 = (const uint32_t *)a.cells;
     auto load_group = [&](int cw, unsigned ci0, unsigned cn, int64_t (&kk)[R], int64_t (&g1)[R], int64_t (&g2)[R]) {
-        const int64_t *cell = cw >= w1 ? cell0 : chunked ? cell0 + (int64_t)cw * S * RW : cell0 + (int64_t)cw * Cw * RW;
+        const int64_t cell = cw >= w1 ? cell0 : chunked ? cell0 + (int64_t)cw * S : cell0 + (int64_t)cw * Cw;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const unsigned i = ci0 + 64 * q + lane;
-            const int64_t *rec = i < cn ? cell + (int64_t)i * RW : cell0;
-            if (RW == 1) {
-                kk[q] = __builtin_nontemporal_load(rec);
+            const int64_t ri = i < cn ? cell + i : cell0;
+            if (KW == 4) {
+                const uint32_t *rec = k32 + ri * (2 * RW - 1);
+                kk[q] = (int64_t)__builtin_nontemporal_load(rec);
+                g1[q] = RW > 1 ? crec_pay(rec, 0) : 0;
+                g2[q] = RW > 2 ? crec_pay(rec, 1) : 0;
+            } else if (RW == 1) {
+                kk[q] = __builtin_nontemporal_load(a.cells + ri);
                 g1[q] = g2[q] = 0;
             } else if (RW == 2) {
-                const RecPair w2 = __builtin_nontemporal_load((const RecPair *)rec);
-                const longlong2 v = make_longlong2(w2.x, w2.y);
-                kk[q] = v.x;
-                g1[q] = v.y;
+                const RecPair w2 = __builtin_nontemporal_load((const RecPair *)(a.cells + ri * 2));
+                kk[q] = w2.x;
+                g1[q] = w2.y;
                 g2[q] = 0;
             } else {
+                const int64_t *rec = a.cells + ri * 3;
                 kk[q] = rec[0];
                 g1[q] = rec[1];
                 g2[q] = rec[2];
@@ -894,37 +950,64 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
 #pragma unroll
         for (int q = 0; q < R; ++q) live[q] = i0 + 64 * q + lane < n;
         if (FAST) {
-            // one 32-byte bucket per record, read as two 16-byte halves in an
-            // order that alternates with bucket bit 3, so the lanes of a wave
-            // spread over all sixteen 16-byte LDS bank groups
             uint32_t si[R];
             bool hit[R], more[R];
+            if (KW == 4) {
+                // one 16-byte bucket of four uint32 keys per record
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const uint32_t b0 = ((uint32_t)key[q] << 2) & tmask;  // first slot of the bucket
-                const uint32_t rot = (b0 >> 5) & 1;
-                longlong2 kv[2];
-#pragma unroll
-                for (int c = 0; c < 2; ++c) kv[c] = *(const longlong2 *)(tk + b0 + 2 * ((c + rot) & 1));
-                uint32_t match = 0, empty = 0;
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const uint32_t sh = 2 * ((c + rot) & 1);
-                    match |= (kv[c].x == key[q] ? 1u : 0u) << sh | (kv[c].y == key[q] ? 1u : 0u) << (sh + 1);
-                    empty |= (kv[c].x == EMPTY_KEY ? 1u : 0u) << sh | (kv[c].y == EMPTY_KEY ? 1u : 0u) << (sh + 1);
+                for (int q = 0; q < R; ++q) {
+                    const uint32_t rk = (uint32_t)key[q];
+                    const uint32_t b0 = (rk << 2) & tmask;  // first slot of the bucket
+                    const uint4 kv = *(const uint4 *)(tk32 + b0);
+                    const uint32_t match = (kv.x == rk ? 1u : 0u) | (kv.y == rk ? 2u : 0u) |
+                                           (kv.z == rk ? 4u : 0u) | (kv.w == rk ? 8u : 0u);
+                    const bool full = kv.x != EMPTY32 && kv.y != EMPTY32 && kv.z != EMPTY32 && kv.w != EMPTY32;
+                    hit[q] = match != 0;
+                    si[q] = b0 + (hit[q] ? (uint32_t)__builtin_ctz(match) : 3u);
+                    more[q] = live[q] && !hit[q] && full;  // full bucket without the key: read on
                 }
-                hit[q] = match != 0;
-                si[q] = b0 + (hit[q] ? (uint32_t)__builtin_ctz(match) : 3u);
-                more[q] = live[q] && !hit[q] && empty == 0;  // full bucket without the key: read on
-            }
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-                while (__ballot(more[q])) {  // rare: walk on past a full bucket
-                    if (more[q]) {
-                        si[q] = (si[q] + 1) & tmask;
-                        const int64_t x = tk[si[q]];
-                        hit[q] = x == key[q];
-                        more[q] = !hit[q] && x != EMPTY_KEY;
+                for (int q = 0; q < R; ++q) {
+                    while (__ballot(more[q])) {  // rare: walk on past a full bucket
+                        if (more[q]) {
+                            si[q] = (si[q] + 1) & tmask;
+                            const uint32_t x = tk32[si[q]];
+                            hit[q] = x == (uint32_t)key[q];
+                            more[q] = !hit[q] && x != EMPTY32;
+                        }
+                    }
+                }
+            } else {
+                // one 32-byte bucket per record, read as two 16-byte halves in an
+                // order that alternates with bucket bit 3, so the lanes of a wave
+                // spread over all sixteen 16-byte LDS bank groups
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const uint32_t b0 = ((uint32_t)key[q] << 2) & tmask;  // first slot of the bucket
+                    const uint32_t rot = (b0 >> 5) & 1;
+                    longlong2 kv[2];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) kv[c] = *(const longlong2 *)(tk + b0 + 2 * ((c + rot) & 1));
+                    uint32_t match = 0, empty = 0;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const uint32_t sh = 2 * ((c + rot) & 1);
+                        match |= (kv[c].x == key[q] ? 1u : 0u) << sh | (kv[c].y == key[q] ? 1u : 0u) << (sh + 1);
+                        empty |= (kv[c].x == EMPTY_KEY ? 1u : 0u) << sh | (kv[c].y == EMPTY_KEY ? 1u : 0u) << (sh + 1);
+                    }
+                    hit[q] = match != 0;
+                    si[q] = b0 + (hit[q] ? (uint32_t)__builtin_ctz(match) : 3u);
+                    more[q] = live[q] && !hit[q] && empty == 0;  // full bucket without the key: read on
+                }
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    while (__ballot(more[q])) {  // rare: walk on past a full bucket
+                        if (more[q]) {
+                            si[q] = (si[q] + 1) & tmask;
+                            const int64_t x = tk[si[q]];
+                            hit[q] = x == key[q];
+                            more[q] = !hit[q] && x != EMPTY_KEY;
+                        }
                     }
                 }
             }
@@ -941,7 +1024,7 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
                 uint32_t si = ((uint32_t)key[q] << 2) & tmask;  // first slot of the home bucket
                 int64_t m = 0;
                 for (;;) {
-                    const int64_t sk = tk[si];
+                    const int64_t sk = KW == 4 ? (tk32[si] == EMPTY32 ? EMPTY_KEY : (int64_t)tk32[si]) : tk[si];
                     if (sk == EMPTY_KEY) break;
                     if (sk == key[q]) {
                         ++m;
@@ -969,6 +1052,275 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
         n = nn;
     }
     if (!chunked && blockIdx.x == 0) probe_side_partitions<TYPE>(a, j);
+    acc_finish(a.pa, j, lrow);
+}
+
+// ─── compact records in cells: lean pass 1 and pass 2 ──────────────────────
+// The common shape -- compact record keys, a probe key column without NULLs,
+// at most one probe payload and one predicate column, cells rather than exact
+// partitioning, record indices below 2^32 -- runs these kernels.  Pass 1
+// stages 4-byte record keys (so a tile holds twice the rows, and each
+// partition's runs are twice as long) and keeps, per partition, the 32-bit
+// index of its next record in this workgroup's cell, so a staged record is
+// stored with no address arithmetic.  Pass 2 reads four records per lane with
+// one 16-byte load, in groups of 256 records (cells hold several hundred) with
+// two groups in flight, and looks each record up with one 16-byte LDS read of
+// its bucket.
+typedef unsigned int JcU4 __attribute__((ext_vector_type(4)));
+
+template <int RW, int NPC, int U, bool K32>
+__global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
+    extern __shared__ uint64_t jc_lds[];
+    __shared__ int overflow;
+    constexpr int TILE = JS_TPB * U;
+    const JoinProbeArgs &pa = a.pa;
+    const int P2 = (1 << a.log_p) + 2;
+    const unsigned Cw = (unsigned)a.Cw;
+    int64_t *spay = (int64_t *)jc_lds;                              // TILE * (RW - 1) payloads
+    uint32_t *skey = (uint32_t *)(spay + (size_t)TILE * (RW - 1));  // TILE record keys
+    uint32_t *dest = skey + TILE;                                   // TILE record indices (~0: cell full)
+    unsigned *hist = dest + TILE;                                   // P2
+    unsigned *off = hist + P2;                                      // P2
+    unsigned *cur = off + P2;                                       // P2: records of this workgroup so far
+    unsigned *cbase = cur + P2;                                     // P2: first record of this workgroup's cell
+    unsigned *tmp = cbase + P2;                                     // 16
+    // a row whose key matches nothing is an output row only for LEFT and ANTI
+    const int nomatch = pa.type == FGPU_LEFT || pa.type == FGPU_ANTI ? P2 - 2 : -1;
+    for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+        hist[i] = cur[i] = 0;
+        cbase[i] = (unsigned)(((int64_t)i * a.G + blockIdx.x) * a.Cw);
+    }
+    if (threadIdx.x == 0) overflow = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_wg;
+    const int64_t r1 = r0 + a.rows_per_wg < pa.n_probe ? r0 + a.rows_per_wg : pa.n_probe;
+    const void *kcol = pa.pcols.v[pa.pkey];
+    const int64_t *pcol = (const int64_t *)pa.pcols.v[a.pay_col[0]];
+    const ColRef pc = col_ref(pa.pcols, pa.pred.pcol[0]);
+    const uint64_t kmask = (1ull << a.kbits) - 1, rmask = (1ull << a.rbits) - 1;
+    const int64_t kmin = a.kmin, kmax = a.kmax;
+    const int kbits = a.kbits, rbits = a.rbits;
+    char *recs = (char *)a.cells;  // packed records of (2 * RW - 1) uint32 words
+    struct Raw {
+        int64_t key[U];
+        int64_t pay[U];
+        RawCell3 pc[U];
+    };
+    // rows base + u * JS_TPB + tid through uniform bases and 32-bit offsets;
+    // a row past the range re-reads the range's last row (every load issued)
+    auto issue = [&](int64_t base, Raw &t) {
+        const int64_t last = r1 - 1 - base;  // may be negative past the end: clamped below
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t want = (int64_t)(u * JS_TPB + threadIdx.x);
+            const int64_t r = base + (want < last ? want : last);
+            if (K32) t.key[u] = (int64_t)((const int32_t *)kcol)[r];
+            else t.key[u] = ((const int64_t *)kcol)[r];
+            if (RW > 1) t.pay[u] = pcol[r];
+            if (NPC > 0) cell_issue3(pc, r, t.pc[u]);
+        }
+    };
+    Raw raw;
+    if (r0 < r1) issue(r0, raw);
+    for (int64_t base = r0; base < r1; base += TILE) {
+        Raw nxt;
+        issue(base + TILE, nxt);
+        int part[U];
+        uint32_t rec[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = base + u * JS_TPB + threadIdx.x;
+            bool sel = r < r1;
+            if (NPC > 0 && pa.pred.n > 0) {
+                int64_t v[1];
+                bool okj;
+                cell_finish3(pc, r, raw.pc[u], v[0], okj);
+                sel = sel && eval_reg<1>(pa.pred, v, okj ? 1u : 0u);
+            }
+            const int64_t key = raw.key[u];
+            const bool inr = key >= kmin && key <= kmax;
+            const uint64_t m = cmix((uint64_t)key - (uint64_t)kmin, kbits, kmask);
+            rec[u] = (uint32_t)(m & rmask);
+            part[u] = !sel ? -1 : !inr ? nomatch : (int)(m >> rbits);
+        }
+        unsigned rank[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) rank[u] = part[u] >= 0 ? atomicAdd(hist + part[u], 1u) : 0u;
+        __syncthreads();
+        const unsigned total = block_scan_hist(hist, off, P2, tmp);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (part[u] < 0) continue;
+            const unsigned i = off[part[u]] + rank[u];
+            const unsigned pos = cur[part[u]] + rank[u];
+            skey[i] = rec[u];
+            if (RW > 1) spay[i] = raw.pay[u];
+            dest[i] = pos < Cw ? cbase[part[u]] + pos : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (unsigned i = threadIdx.x; i < total; i += JS_TPB) {
+            const uint32_t d = dest[i];
+            if (d == 0xFFFFFFFFu) {
+                overflow = 1;  // cell full: the host redoes the partitioning exactly
+                continue;
+            }
+            // byte offsets stay below 2^32 (checked by the host)
+            uint32_t *rec = (uint32_t *)(recs + d * (uint32_t)(4 * (2 * RW - 1)));
+            rec[0] = skey[i];
+            if (RW > 1) {
+                const uint64_t x = (uint64_t)spay[i];
+                rec[1] = (uint32_t)x;
+                rec[2] = (uint32_t)(x >> 32);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < P2; i += blockDim.x) {
+            cur[i] = min(cur[i] + hist[i], Cw);
+            hist[i] = 0;
+        }
+        __syncthreads();
+        raw = nxt;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P2; i += blockDim.x) a.cell_cnt[(int64_t)blockIdx.x * P2 + i] = cur[i];
+    if (threadIdx.x == 0 && overflow) atomicExch(a.flags, 1ull);
+}
+
+template <int TYPE, int RW>
+__global__ __launch_bounds__(1024) void k_jc_probe(JoinPartArgs a) {
+    extern __shared__ uint64_t jp_lds[];
+    const int P = 1 << a.log_p, P2 = P + 2;
+    const int T = a.T, G = a.G;
+    const unsigned Cw = (unsigned)a.Cw;
+    const bool vals = a.need_vals != 0;
+    uint32_t *tk32 = (uint32_t *)jp_lds;                  // T record keys
+    int64_t *tv = (int64_t *)(tk32 + T);                  // T build values / rows (if vals)
+    uint64_t *lrow = (uint64_t *)(tv + (vals ? T : 0));   // 64-word state row
+    unsigned *cnt = (unsigned *)(lrow + 64);              // G record counts
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int p = blockIdx.x & (P - 1);
+    const int k = blockIdx.x >> a.log_p, K = gridDim.x >> a.log_p;
+    const int w0 = (int)((int64_t)G * k / K), w1 = (int)((int64_t)G * (k + 1) / K);
+    {
+        const longlong2 *src = (const longlong2 *)(a.sub + (int64_t)p * T * 2);
+        for (int i = threadIdx.x; i < T; i += blockDim.x) {
+            const longlong2 sl = src[i];
+            tk32[i] = sl.x == EMPTY_KEY ? EMPTY32 : (uint32_t)sl.x;
+            if (vals) tv[i] = sl.y;
+        }
+        for (int w = threadIdx.x; w < G; w += blockDim.x) {
+            const unsigned c = a.cell_cnt[(int64_t)w * P2 + p];
+            cnt[w] = c < Cw ? c : Cw;
+        }
+    }
+    JoinAcc j;
+    acc_init(a.pa, j, lrow);  // synchronises the workgroup (table and counts are in place)
+    const EmitOps e = emit_ops(a);
+    const uint32_t tmask = (uint32_t)T - 1;
+    const unsigned pbase = (unsigned)((int64_t)p * G) * Cw;  // first record of the partition's cells
+    const char *recs = (const char *)a.cells;  // packed records of (2 * RW - 1) uint32 words
+    // A wave's groups: cell w (w0 + wave, then every nw-th), records i0 .. i0 + 256
+    // of it; lane l takes records i0 + 4l .. i0 + 4l + 3.
+    struct Cur {
+        int w;
+        unsigned i0, n;
+    };
+    auto advance = [&](Cur c) {
+        c.i0 += 256;
+        while (c.w < w1 && c.i0 >= c.n) {
+            c.w += nw;
+            c.i0 = 0;
+            c.n = c.w < w1 ? cnt[c.w] : 0;
+        }
+        return c;
+    };
+    struct Grp {
+        JcU4 k;
+        int64_t f[4];
+    };
+    // Loads are issued for every group slot (a lane past its cell's end, or
+    // past the wave's last cell, reads the partition's first four records,
+    // which exist), so the count of loads in flight is fixed.
+    auto load = [&](const Cur &c, Grp &g) {
+        const unsigned start = c.i0 + 4 * lane;
+        const unsigned idx = c.w < w1 && start < c.n ? pbase + (unsigned)c.w * Cw + start : pbase;
+        // four records: 16 bytes (keys only) or 48 bytes (keys and one payload),
+        // 16-byte aligned as idx is a multiple of 4; byte offsets stay below
+        // 2^32 (checked by the host)
+        const JcU4 *v = (const JcU4 *)(recs + idx * (uint32_t)(4 * (2 * RW - 1)));
+        if (RW == 1) {
+            g.k = __builtin_nontemporal_load(v);
+            g.f[0] = g.f[1] = g.f[2] = g.f[3] = 0;
+        } else {
+            const JcU4 x = __builtin_nontemporal_load(v), y = __builtin_nontemporal_load(v + 1),
+                       z = __builtin_nontemporal_load(v + 2);
+            // records (key, pay_lo, pay_hi) x 4: x = k0 l0 h0 k1, y = l1 h1 k2 l2, z = h2 k3 l3 h3
+            g.k = JcU4{x.x, x.w, y.z, z.y};
+            g.f[0] = (int64_t)((uint64_t)x.y | (uint64_t)x.z << 32);
+            g.f[1] = (int64_t)((uint64_t)y.x | (uint64_t)y.y << 32);
+            g.f[2] = (int64_t)((uint64_t)y.w | (uint64_t)z.x << 32);
+            g.f[3] = (int64_t)((uint64_t)z.z | (uint64_t)z.w << 32);
+        }
+    };
+    // advancing from "before the wave's first cell" finds its first non-empty cell
+    Cur c0 = advance(Cur{w0 + wave - nw, 0u, 0u});
+    Cur c1 = advance(c0), c2 = advance(c1);
+    Grp g0, g1, g2;
+    load(c0, g0);
+    load(c1, g1);
+    load(c2, g2);
+    while (c0.w < w1) {
+        const Cur c3 = advance(c2);
+        Grp g3;
+        load(c3, g3);
+        // keep the loads of the three groups ahead in flight while this one is probed
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t rk[4] = {g0.k.x, g0.k.y, g0.k.z, g0.k.w};
+        bool live[4], hit[4], more[4];
+        uint32_t si[4];
+        bool any_more = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            live[q] = c0.i0 + 4 * lane + q < c0.n;
+            const uint32_t b0 = (rk[q] << 2) & tmask;  // first slot of the bucket
+            const JcU4 kv = *(const JcU4 *)(tk32 + b0);
+            const bool h0 = kv.x == rk[q], h1 = kv.y == rk[q], h2 = kv.z == rk[q], h3 = kv.w == rk[q];
+            hit[q] = h0 || h1 || h2 || h3;
+            si[q] = b0 + (h0 ? 0u : h1 ? 1u : h2 ? 2u : 3u);
+            const bool full = kv.x != EMPTY32 && kv.y != EMPTY32 && kv.z != EMPTY32 && kv.w != EMPTY32;
+            more[q] = live[q] && !hit[q] && full;  // full bucket without the key: read on
+            any_more = any_more || more[q];
+        }
+        if (__ballot(any_more)) {  // rare: walk on past a full bucket
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                while (__ballot(more[q])) {
+                    if (more[q]) {
+                        si[q] = (si[q] + 1) & tmask;
+                        const uint32_t x = tk32[si[q]];
+                        hit[q] = x == rk[q];
+                        more[q] = !hit[q] && x != EMPTY32;
+                    }
+                }
+            }
+        }
+        // plain local arrays: fold_group picks among them per op
+        int64_t pay[4], f1[4], f2[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pay[q] = (TYPE == FGPU_INNER || TYPE == FGPU_LEFT) && vals ? tv[si[q]] : 0;
+            f1[q] = g0.f[q];
+            f2[q] = 0;
+        }
+        fold_group<TYPE, 4>(e, j, live, hit, f1, f2, pay);
+        c0 = c1;
+        c1 = c2;
+        c2 = c3;
+        g0 = g1;
+        g1 = g2;
+        g2 = g3;
+    }
+    if (blockIdx.x == 0) probe_side_partitions<TYPE>(a, j);
     acc_finish(a.pa, j, lrow);
 }
 
@@ -1001,49 +1353,77 @@ size_t join_scatter_lds(const JoinPartArgs &a, int mode) {
     return scatter_lds_rows(a, mode, join_scatter_rows(a, mode));
 }
 size_t join_probe_lds(const JoinPartArgs &a) {
-    return (size_t)a.T * (a.need_vals ? 16 : 8) + 64 * 8 + (size_t)(a.G > 16 ? a.G : 16) * 4;
+    const size_t slot = (a.kw == 4 ? 4 : 8) + (a.need_vals ? 8 : 0);
+    return (size_t)a.T * slot + 64 * 8 + (size_t)(a.G > 16 ? a.G : 16) * 4;
 }
 
 // Lets every pass-1 / pass-2 instantiation take up to JOIN_LDS_MAX bytes of
 // dynamic LDS; done once per process (the attribute is per function).
-template <bool FAST, int RW>
+template <bool FAST, int RW, int KW>
 static void probe_fns(std::vector<const void *> &v) {
-    v.push_back((const void *)k_jp_probe<FGPU_INNER, FAST, RW>);
-    v.push_back((const void *)k_jp_probe<FGPU_LEFT, FAST, RW>);
-    v.push_back((const void *)k_jp_probe<FGPU_SEMI, FAST, RW>);
-    v.push_back((const void *)k_jp_probe<FGPU_ANTI, FAST, RW>);
+    v.push_back((const void *)k_jp_probe<FGPU_INNER, FAST, RW, KW>);
+    v.push_back((const void *)k_jp_probe<FGPU_LEFT, FAST, RW, KW>);
+    v.push_back((const void *)k_jp_probe<FGPU_SEMI, FAST, RW, KW>);
+    v.push_back((const void *)k_jp_probe<FGPU_ANTI, FAST, RW, KW>);
 }
-template <int NPC, int MODE>
+template <int NPC, int MODE, int KW>
 static void scatter_fns(std::vector<const void *> &v) {
-    v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>);
-    v.push_back((const void *)k_jp_scatter<NPC, 2, 4, MODE>);
-    v.push_back((const void *)k_jp_scatter<NPC, 3, 4, MODE>);
-    if constexpr (NPC == 0) v.push_back((const void *)k_jp_scatter<0, 1, 12, MODE>);
-    if constexpr (NPC <= 1) v.push_back((const void *)k_jp_scatter<NPC, 2, 6, MODE>);
+    v.push_back((const void *)k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE, KW>);
+    v.push_back((const void *)k_jp_scatter<NPC, 2, 4, MODE, KW>);
+    v.push_back((const void *)k_jp_scatter<NPC, 3, 4, MODE, KW>);
+    if constexpr (NPC == 0) v.push_back((const void *)k_jp_scatter<0, 1, 12, MODE, KW>);
+    if constexpr (NPC <= 1) v.push_back((const void *)k_jp_scatter<NPC, 2, 6, MODE, KW>);
+}
+template <int KW>
+static void join_fns(std::vector<const void *> &fns) {
+    scatter_fns<0, JS_CELLS, KW>(fns);
+    scatter_fns<1, JS_CELLS, KW>(fns);
+    scatter_fns<2, JS_CELLS, KW>(fns);
+    scatter_fns<MAX_PCOLS, JS_CELLS, KW>(fns);
+    scatter_fns<0, JS_EXACT, KW>(fns);
+    scatter_fns<1, JS_EXACT, KW>(fns);
+    scatter_fns<2, JS_EXACT, KW>(fns);
+    scatter_fns<MAX_PCOLS, JS_EXACT, KW>(fns);
+    fns.push_back((const void *)k_jp_scatter<0, 1, 8, JS_COUNT, KW>);
+    fns.push_back((const void *)k_jp_scatter<1, 1, 8, JS_COUNT, KW>);
+    fns.push_back((const void *)k_jp_scatter<2, 1, 4, JS_COUNT, KW>);
+    fns.push_back((const void *)k_jp_scatter<MAX_PCOLS, 1, 4, JS_COUNT, KW>);
+    probe_fns<true, 1, KW>(fns);
+    probe_fns<true, 2, KW>(fns);
+    probe_fns<true, 3, KW>(fns);
+    probe_fns<false, 1, KW>(fns);
+    probe_fns<false, 2, KW>(fns);
+    probe_fns<false, 3, KW>(fns);
+}
+template <bool K32>
+static void jc_fns(std::vector<const void *> &v) {
+    v.push_back((const void *)k_jc_scatter<1, 0, 8, K32>);
+    v.push_back((const void *)k_jc_scatter<1, 0, 4, K32>);
+    v.push_back((const void *)k_jc_scatter<1, 1, 8, K32>);
+    v.push_back((const void *)k_jc_scatter<1, 1, 4, K32>);
+    v.push_back((const void *)k_jc_scatter<2, 0, 8, K32>);
+    v.push_back((const void *)k_jc_scatter<2, 0, 4, K32>);
+    v.push_back((const void *)k_jc_scatter<2, 1, 6, K32>);
+    v.push_back((const void *)k_jc_scatter<2, 1, 4, K32>);
+}
+template <int RW>
+static void jc_probe_fns(std::vector<const void *> &v) {
+    v.push_back((const void *)k_jc_probe<FGPU_INNER, RW>);
+    v.push_back((const void *)k_jc_probe<FGPU_LEFT, RW>);
+    v.push_back((const void *)k_jc_probe<FGPU_SEMI, RW>);
+    v.push_back((const void *)k_jc_probe<FGPU_ANTI, RW>);
 }
 static hipError_t join_allow_lds_once() {
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
         std::vector<const void *> fns;
-        scatter_fns<0, JS_CELLS>(fns);
-        scatter_fns<1, JS_CELLS>(fns);
-        scatter_fns<2, JS_CELLS>(fns);
-        scatter_fns<MAX_PCOLS, JS_CELLS>(fns);
-        scatter_fns<0, JS_EXACT>(fns);
-        scatter_fns<1, JS_EXACT>(fns);
-        scatter_fns<2, JS_EXACT>(fns);
-        scatter_fns<MAX_PCOLS, JS_EXACT>(fns);
-        fns.push_back((const void *)k_jp_scatter<0, 1, 8, JS_COUNT>);
-        fns.push_back((const void *)k_jp_scatter<1, 1, 8, JS_COUNT>);
-        fns.push_back((const void *)k_jp_scatter<2, 1, 4, JS_COUNT>);
-        fns.push_back((const void *)k_jp_scatter<MAX_PCOLS, 1, 4, JS_COUNT>);
-        probe_fns<true, 1>(fns);
-        probe_fns<true, 2>(fns);
-        probe_fns<true, 3>(fns);
-        probe_fns<false, 1>(fns);
-        probe_fns<false, 2>(fns);
-        probe_fns<false, 3>(fns);
+        join_fns<8>(fns);
+        join_fns<4>(fns);
+        jc_fns<false>(fns);
+        jc_fns<true>(fns);
+        jc_probe_fns<1>(fns);
+        jc_probe_fns<2>(fns);
         for (const void *f : fns) {
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)JOIN_LDS_MAX);
             if (e != hipSuccess && err == hipSuccess) err = e;
@@ -1052,71 +1432,141 @@ static hipError_t join_allow_lds_once() {
     return err;
 }
 
-template <int NPC, int MODE>
+template <int NPC, int MODE, int KW>
 static void launch_scatter_npc(hipStream_t s, const JoinPartArgs &a, size_t lds) {
     const dim3 g((unsigned)a.G), b(JS_TPB);
     if constexpr (MODE == JS_COUNT) {
-        hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, JS_COUNT>), g, b, lds, s, a);
+        hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, JS_COUNT, KW>), g, b, lds, s, a);
         return;
     }
     const int rows = join_scatter_rows(a, MODE);
     if constexpr (NPC == 0 && MODE != JS_COUNT) {
         if (a.RW == 1 && rows == 12) {
-            hipLaunchKernelGGL((k_jp_scatter<0, 1, 12, MODE>), g, b, lds, s, a);
+            hipLaunchKernelGGL((k_jp_scatter<0, 1, 12, MODE, KW>), g, b, lds, s, a);
             return;
         }
     }
     if constexpr (NPC <= 1 && MODE != JS_COUNT) {
         if (a.RW == 2 && rows == 6) {
-            hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 6, MODE>), g, b, lds, s, a);
+            hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 6, MODE, KW>), g, b, lds, s, a);
             return;
         }
     }
-    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE>), g, b, lds, s, a);
-    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 4, MODE>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3, 4, MODE>), g, b, lds, s, a);
+    if (a.RW == 1) hipLaunchKernelGGL((k_jp_scatter<NPC, 1, NPC <= 1 ? 8 : 4, MODE, KW>), g, b, lds, s, a);
+    else if (a.RW == 2) hipLaunchKernelGGL((k_jp_scatter<NPC, 2, 4, MODE, KW>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_jp_scatter<NPC, 3, 4, MODE, KW>), g, b, lds, s, a);
 }
-template <int MODE>
+template <int MODE, int KW>
 static void launch_scatter_mode(hipStream_t s, const JoinPartArgs &a, size_t lds) {
     const int n = scatter_npc(a);
-    if (n == 0) launch_scatter_npc<0, MODE>(s, a, lds);
-    else if (n == 1) launch_scatter_npc<1, MODE>(s, a, lds);
-    else if (n == 2) launch_scatter_npc<2, MODE>(s, a, lds);
-    else launch_scatter_npc<MAX_PCOLS, MODE>(s, a, lds);
+    if (n == 0) launch_scatter_npc<0, MODE, KW>(s, a, lds);
+    else if (n == 1) launch_scatter_npc<1, MODE, KW>(s, a, lds);
+    else if (n == 2) launch_scatter_npc<2, MODE, KW>(s, a, lds);
+    else launch_scatter_npc<MAX_PCOLS, MODE, KW>(s, a, lds);
+}
+template <int KW>
+static void launch_scatter_kw(hipStream_t s, const JoinPartArgs &a, int mode, size_t lds) {
+    if (mode == JS_COUNT) launch_scatter_mode<JS_COUNT, KW>(s, a, lds);
+    else if (mode == JS_EXACT) launch_scatter_mode<JS_EXACT, KW>(s, a, lds);
+    else launch_scatter_mode<JS_CELLS, KW>(s, a, lds);
+}
+
+// A compact record's key must leave the empty marker EMPTY32 free and its
+// partition must take at least one bit of the mix.
+static bool join_format_ok(const JoinPartArgs &a) {
+    if (a.kw != 4) return true;  // hash64 records
+    return a.kbits >= 1 && a.kbits <= 62 && a.rbits >= 1 && a.rbits <= 31 && a.rbits + a.log_p == a.kbits;
+}
+
+// The lean compact kernels' shape (see k_jc_scatter): compact records, a key
+// column without NULLs, at most one 8-byte payload and one predicate column,
+// and record byte offsets below 2^32.
+static bool jc_records_ok(const JoinPartArgs &a) {
+    const int64_t recs = (int64_t)a.G * ((1 << a.log_p) + 2) * a.Cw;
+    return a.kw == 4 && a.RW <= 2 && recs * 4 * (2 * a.RW - 1) < (int64_t(1) << 32);
+}
+static size_t jc_scatter_lds(const JoinPartArgs &a, int u) {
+    const size_t tile = (size_t)JS_TPB * u, P2 = ((size_t)1 << a.log_p) + 2;
+    return tile * (a.RW - 1) * 8 + tile * 8 + 4 * P2 * 4 + 16 * 4;
+}
+// Rows per thread of the lean pass 1 (0: the general kernel runs): the largest
+// tile the registers hold (8 rows, 6 with both a payload and a predicate
+// column) whose staging fits the LDS budget.
+static int jc_scatter_rows(const JoinPartArgs &a) {
+    if (!jc_records_ok(a) || scatter_npc(a) > 1) return 0;
+    const JoinProbeArgs &pa = a.pa;
+    const int kd = pa.pcols.dtype[pa.pkey];
+    if (pa.pcols.valid[pa.pkey] || (kd != FGPU_INT64 && kd != FGPU_INT32)) return 0;
+    if (a.RW > 1) {
+        const int pd = pa.pcols.dtype[a.pay_col[0]];
+        if (pd != FGPU_INT64 && pd != FGPU_FLOAT64) return 0;
+    }
+    const size_t budget = std::min<size_t>(JOIN_LDS_MAX, a.lds_budget > 0 ? (size_t)a.lds_budget : JOIN_LDS_MAX);
+    const int big = a.RW > 1 && scatter_npc(a) > 0 ? 6 : 8;
+    for (int u : {big, 4})
+        if (jc_scatter_lds(a, u) <= budget) return u;
+    return 0;
+}
+template <int RW, int NPC, bool K32>
+static void launch_jc_scatter_u(hipStream_t s, const JoinPartArgs &a, int u) {
+    const dim3 g((unsigned)a.G), b(JS_TPB);
+    const size_t lds = jc_scatter_lds(a, u);
+    constexpr int BIG = RW > 1 && NPC > 0 ? 6 : 8;
+    if (u == BIG) hipLaunchKernelGGL((k_jc_scatter<RW, NPC, BIG, K32>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_jc_scatter<RW, NPC, 4, K32>), g, b, lds, s, a);
+}
+template <bool K32>
+static void launch_jc_scatter(hipStream_t s, const JoinPartArgs &a, int u) {
+    const int npc = scatter_npc(a);
+    if (a.RW == 1) {
+        if (npc == 0) launch_jc_scatter_u<1, 0, K32>(s, a, u);
+        else launch_jc_scatter_u<1, 1, K32>(s, a, u);
+    } else {
+        if (npc == 0) launch_jc_scatter_u<2, 0, K32>(s, a, u);
+        else launch_jc_scatter_u<2, 1, K32>(s, a, u);
+    }
 }
 
 hipError_t launch_join_part_scatter(hipStream_t s, const JoinPartArgs &a, int mode) {
+    if (mode == JS_CELLS && a.kw == 4 && a.Cw < (1 << 17) && join_format_ok(a)) {
+        if (const int u = jc_scatter_rows(a)) {
+            if (hipError_t e = join_allow_lds_once()) return e;
+            if (a.pa.pcols.dtype[a.pa.pkey] == FGPU_INT32) launch_jc_scatter<true>(s, a, u);
+            else launch_jc_scatter<false>(s, a, u);
+            return hipGetLastError();
+        }
+    }
     const size_t lds = join_scatter_lds(a, mode);
     if (lds > JOIN_LDS_MAX || a.log_p > 14 || a.RW < 1 || a.RW > 3 || a.Cw >= (1 << 17) ||
-        (mode == JS_EXACT && !a.xoff))
+        (mode == JS_EXACT && !a.xoff) || !join_format_ok(a))
         return hipErrorInvalidValue;
     if (hipError_t e = join_allow_lds_once()) return e;
-    if (mode == JS_COUNT) launch_scatter_mode<JS_COUNT>(s, a, lds);
-    else if (mode == JS_EXACT) launch_scatter_mode<JS_EXACT>(s, a, lds);
-    else launch_scatter_mode<JS_CELLS>(s, a, lds);
+    if (a.kw == 4) launch_scatter_kw<4>(s, a, mode, lds);
+    else launch_scatter_kw<8>(s, a, mode, lds);
     return hipGetLastError();
 }
 
-template <bool FAST, int RW>
+template <bool FAST, int RW, int KW>
 static void launch_probe_rw(hipStream_t s, const JoinPartArgs &a, dim3 grid, size_t lds) {
     const dim3 b(1024);
     switch (a.pa.type) {
-        case FGPU_INNER: hipLaunchKernelGGL((k_jp_probe<FGPU_INNER, FAST, RW>), grid, b, lds, s, a); break;
-        case FGPU_LEFT: hipLaunchKernelGGL((k_jp_probe<FGPU_LEFT, FAST, RW>), grid, b, lds, s, a); break;
-        case FGPU_SEMI: hipLaunchKernelGGL((k_jp_probe<FGPU_SEMI, FAST, RW>), grid, b, lds, s, a); break;
-        default: hipLaunchKernelGGL((k_jp_probe<FGPU_ANTI, FAST, RW>), grid, b, lds, s, a); break;
+        case FGPU_INNER: hipLaunchKernelGGL((k_jp_probe<FGPU_INNER, FAST, RW, KW>), grid, b, lds, s, a); break;
+        case FGPU_LEFT: hipLaunchKernelGGL((k_jp_probe<FGPU_LEFT, FAST, RW, KW>), grid, b, lds, s, a); break;
+        case FGPU_SEMI: hipLaunchKernelGGL((k_jp_probe<FGPU_SEMI, FAST, RW, KW>), grid, b, lds, s, a); break;
+        default: hipLaunchKernelGGL((k_jp_probe<FGPU_ANTI, FAST, RW, KW>), grid, b, lds, s, a); break;
     }
 }
-template <bool FAST>
+template <bool FAST, int KW>
 static void launch_probe_fast(hipStream_t s, const JoinPartArgs &a, dim3 grid, size_t lds) {
-    if (a.RW == 1) launch_probe_rw<FAST, 1>(s, a, grid, lds);
-    else if (a.RW == 2) launch_probe_rw<FAST, 2>(s, a, grid, lds);
-    else launch_probe_rw<FAST, 3>(s, a, grid, lds);
+    if (a.RW == 1) launch_probe_rw<FAST, 1, KW>(s, a, grid, lds);
+    else if (a.RW == 2) launch_probe_rw<FAST, 2, KW>(s, a, grid, lds);
+    else launch_probe_rw<FAST, 3, KW>(s, a, grid, lds);
 }
 
 hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int n_chunks) {
     const size_t lds = join_probe_lds(a);
-    if (lds > JOIN_LDS_MAX || (n_chunks > 0 && !a.chunks) || (n_chunks == 0 && a.chunks)) return hipErrorInvalidValue;
+    if (lds > JOIN_LDS_MAX || (n_chunks > 0 && !a.chunks) || (n_chunks == 0 && a.chunks) || !join_format_ok(a))
+        return hipErrorInvalidValue;
     if (hipError_t e = join_allow_lds_once()) return e;
     // cell mode: a.split workgroups per partition; chunk mode: one per chunk
     const dim3 grid(n_chunks > 0 ? (unsigned)n_chunks : (unsigned)(a.split << a.log_p));
@@ -1124,8 +1574,25 @@ hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int n_ch
     bool build_rows = false;
     for (int i = 0; i < a.pa.n_ops; ++i) build_rows |= a.pa.ops[i].side == 1 && a.pa.ops[i].col >= 0;
     const bool fast = a.pa.unique_keys && (!build_rows || a.build_pay_col >= 0);
-    if (fast) launch_probe_fast<true>(s, a, grid, lds);
-    else launch_probe_fast<false>(s, a, grid, lds);
+    if (a.kw == 4 && fast && n_chunks == 0 && jc_records_ok(a)) {  // the lean compact pass 2
+        const dim3 b(1024);
+        switch (a.pa.type * 4 + a.RW) {
+            case FGPU_INNER * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 1>), grid, b, lds, s, a); break;
+            case FGPU_LEFT * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 1>), grid, b, lds, s, a); break;
+            case FGPU_SEMI * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 1>), grid, b, lds, s, a); break;
+            case FGPU_ANTI * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 1>), grid, b, lds, s, a); break;
+            case FGPU_INNER * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 2>), grid, b, lds, s, a); break;
+            case FGPU_LEFT * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 2>), grid, b, lds, s, a); break;
+            case FGPU_SEMI * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 2>), grid, b, lds, s, a); break;
+            default: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 2>), grid, b, lds, s, a); break;
+        }
+    } else if (a.kw == 4) {
+        if (fast) launch_probe_fast<true, 4>(s, a, grid, lds);
+        else launch_probe_fast<false, 4>(s, a, grid, lds);
+    } else {
+        if (fast) launch_probe_fast<true, 8>(s, a, grid, lds);
+        else launch_probe_fast<false, 8>(s, a, grid, lds);
+    }
     return hipGetLastError();
 }
 

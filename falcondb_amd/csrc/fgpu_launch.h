@@ -242,7 +242,20 @@ struct JoinPartArgs {
     int need_vals;            // an aggregate reads a build column (pass 2 keeps slot values in LDS)
     const int64_t *xoff;      // exact partitioning: record offset of (w, p) at [p * G + w]
     const int64_t *chunks;    // exact partitioning: pass-2 work items {p, first record, records}
+    // Record format.  kw = 8: a record is RW int64 words {hash64(key), payloads}
+    // in `cells`.  kw = 4 (compact; build keys span kbits <= 31 + log_p bits):
+    // h = cmix(key - kmin) over kbits bits, the partition is h's top log_p
+    // bits and the record key its low rbits bits; record i is the uint32 key
+    // and the RW - 1 payloads (each as two uint32 words, low first) packed at
+    // byte 4 * (2 * RW - 1) * i of `cells`.  Probe keys outside [kmin, kmax] go
+    // to the NULL-key side partition.
+    int kw;
+    int kbits, rbits;
+    int64_t kmin, kmax;
+    int lds_budget;           // pass-1 LDS bytes per workgroup (the G workgroups share the CUs' LDS)
 };
+// The empty slot of a compact sub-table in LDS (record keys are < 2^31).
+constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
 // Partitioned join pass 1: JS_TPB threads per workgroup, join_scatter_rows()
 // rows each per tile.
 constexpr int JS_TPB = 1024;

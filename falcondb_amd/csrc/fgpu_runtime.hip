@@ -140,8 +140,9 @@ struct fgpu_table {
 // keys cannot be partitioned so (a sub-table would overflow).
 struct JoinSubTables {
     int log_p = 0, T = 0;
+    int kw = 8;                     // record format the slots hold (hash64 or compact keys)
     const void *val_col = nullptr;  // build column whose values the slots hold; nullptr: build rows
-    int64_t *sub = nullptr;         // P * T * 2 words: {hash, value or build row}
+    int64_t *sub = nullptr;         // P * T * 2 words: {record key, value or build row}
     uint32_t *min_key_rows = nullptr;
     int64_t n_min_key_rows = 0;
     int unique_keys = 0;
@@ -280,6 +281,7 @@ struct Tuning {
     int join_part_wgs_per_cu = 2;  // pass-1 workgroups per CU (two fit by registers)
     int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
     int join_slots_per_key = 4;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
+    int join_compact = 1;          // compact record keys: 0 never, 1 for key-only records, 2 always
 };
 
 struct fgpu_ctx {
@@ -1083,6 +1085,7 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_part_wgs_per_cu") t.join_part_wgs_per_cu = value;
     else if (n == "join_probe_wgs_per_cu") t.join_probe_wgs_per_cu = value;
     else if (n == "join_slots_per_key") t.join_slots_per_key = value;
+    else if (n == "join_compact") t.join_compact = value;
     else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
     if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
         t.join_block > 1024 || t.join_block % 64)
@@ -2191,6 +2194,16 @@ static int join_build_sub_tables(fgpu_ctx *c, JoinPartArgs &jp, const JoinBuildA
     return FGPU_OK;
 }
 
+// Pass-1 record storage for n records in jp's format: hash64 records are RW
+// int64 words each, compact records 2 * RW - 1 uint32 words.
+static int join_cells(fgpu_ctx *c, JoinPartArgs &jp, int64_t n) {
+    void *p;
+    const size_t bytes = jp.kw == 4 ? (size_t)n * 4 * (2 * jp.RW - 1) : (size_t)n * jp.RW * 8;
+    if (int rc = buf(c, B_CELLS, bytes + 64, &p)) return rc;
+    jp.cells = (int64_t *)p;
+    return FGPU_OK;
+}
+
 static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu_table *build,
                                 const fgpu_join_index *idx, const fgpu_join_spec *spec, JoinProbeArgs &a,
                                 const AggLayout &L) {
@@ -2234,20 +2247,46 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     }
     const DevCol &bkc = build->cols[spec->build_key];
     const int64_t nb = build->n_rows;
+    // Record format: compact (4-byte keys, cmix over the build key range) when
+    // the build side has a non-NULL key and its range spans at most
+    // 31 + log_p bits, else hash64 records.  By default only key-only records
+    // are compact: with a probe payload, 12-byte compact records over twice
+    // the partitions lose to 16-byte hash64 records over half of them.
+    int kbits = 0;
+    const bool has_keys = nb > 0 && bkc.null_count < nb && bkc.imin <= bkc.imax;
+    const bool compact_ok = tu.join_compact >= 2 || (tu.join_compact == 1 && pcols_used.empty());
+    if (compact_ok && has_keys) {
+        const uint64_t range = (uint64_t)bkc.imax - (uint64_t)bkc.imin;
+        kbits = range ? 64 - __builtin_clzll(range) : 1;
+    }
     // P partitions, each with a sub-table that pass 2 holds in LDS: at most
-    // tmax slots (8,192 of hash + build value, 16,384 of hash alone, so the
-    // hash-only form needs half the partitions and pass 1 writes longer runs),
-    // at load <= 0.5 on average (checked <= 0.75 after counting).
+    // tmax slots (a power of two: 8,192 of key + build value, 16,384 of key
+    // alone, so that form needs half the partitions and pass 1 writes longer
+    // runs), at load <= 0.5 on average (checked <= 0.75 after counting).
     const int64_t slot_bytes = bcols_used.empty() ? 8 : 16;
     const int64_t tmax = std::min<int64_t>(131072 / slot_bytes,
                                            next_pow2(((int64_t)tu.lds_limit - 1024) / slot_bytes + 1) / 2);
     const int64_t spk = std::max(2, tu.join_slots_per_key);  // slots per build key: load <= 1 / spk
     int64_t P = std::max<int64_t>(8, next_pow2((spk * nb + tmax - 1) / tmax));
     if (tu.join_partitions) P = std::max<int64_t>(P, next_pow2(tu.join_partitions));
-    if (P > 4096) return -1;
-    jp.T = (int)std::max<int64_t>(256, std::min<int64_t>(tmax, next_pow2((spk * nb + P - 1) / P)));
     jp.log_p = 0;
     while ((int64_t(1) << jp.log_p) < P) ++jp.log_p;
+    jp.kw = 8;
+    if (kbits > 0) {
+        // compact records keep kbits - log_p <= 31 bits: more partitions if needed
+        const int lp = std::max(jp.log_p, kbits - 31);
+        if (lp < kbits && lp <= 12) {
+            jp.kw = 4;
+            jp.log_p = lp;
+            P = int64_t(1) << lp;
+            jp.kbits = kbits;
+            jp.rbits = kbits - lp;
+            jp.kmin = bkc.imin;
+            jp.kmax = bkc.imax;
+        }
+    }
+    if (P > 4096) return -1;
+    jp.T = (int)std::max<int64_t>(256, std::min<int64_t>(tmax, next_pow2((spk * nb + P - 1) / P)));
     jp.RW = 1 + jp.n_pay;
     jp.need_vals = bcols_used.empty() ? 0 : 1;
     // pass 1: as many workgroups per CU as LDS allows (at most the tuned count);
@@ -2256,6 +2295,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     const int64_t fit1 = std::max<int64_t>(1, std::min<int64_t>(tu.join_part_wgs_per_cu,
                                                                  (int64_t)tu.lds_limit / (int64_t)join_scatter_lds(jp, 0)));
     jp.G = (int)(c->num_cus * fit1);
+    jp.lds_budget = (int)(tu.lds_limit / fit1);
     if (join_probe_lds(jp) > (size_t)tu.lds_limit) return -1;
     jp.rows_per_wg = (probe->n_rows + jp.G - 1) / jp.G;
     jp.Cw = ((int)std::ceil((double)jp.rows_per_wg / (double)P * 1.15) + 64 + 15) / 16 * 16;  // whole lines
@@ -2264,11 +2304,9 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
                                                                  (int64_t)tu.lds_limit / (int64_t)join_probe_lds(jp)));
     jp.split = (int)std::max<int64_t>(1, (int64_t)c->num_cus * fit2 / P);
     const int64_t P2 = P + 2;
-    const size_t cell_words = (size_t)jp.G * P2 * jp.Cw * jp.RW;
     int rc = FGPU_OK;
     void *p;
-    if ((rc = buf(c, B_CELLS, cell_words * 8, &p))) return rc;
-    jp.cells = (int64_t *)p;
+    if ((rc = join_cells(c, jp, (int64_t)jp.G * P2 * jp.Cw))) return rc;
     if ((rc = buf(c, B_CELLCNT, (size_t)jp.G * P2 * 4, &p))) return rc;
     jp.cell_cnt = (unsigned int *)p;
     unsigned long long *d_flags;
@@ -2279,7 +2317,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     // the partitioned path keys on hash64(key): the one key whose hash is the
     // empty-slot marker keeps its build rows in a side list
     const int64_t sentinel = (int64_t)unhash64((uint64_t)EMPTY_KEY);
-    const bool has_min = bkc.imin <= sentinel && sentinel <= bkc.imax;
+    const bool has_min = jp.kw == 8 && bkc.imin <= sentinel && sentinel <= bkc.imax;
     jp.min_key_cap = has_min ? std::max<int64_t>(nb, 1) : 0;
     JoinBuildArgs ba{};
     ba.keys_col = bkc.d_values;
@@ -2294,11 +2332,12 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
         std::lock_guard<std::mutex> lk(idx->sub_mu);
         const JoinSubTables *hit = nullptr;
         for (const auto &st : idx->subs)
-            if (st.log_p == jp.log_p && st.T == jp.T && st.val_col == val_col) hit = &st;
+            if (st.log_p == jp.log_p && st.T == jp.T && st.kw == jp.kw && st.val_col == val_col) hit = &st;
         if (!hit) {
             JoinSubTables st;
             st.log_p = jp.log_p;
             st.T = jp.T;
+            st.kw = jp.kw;
             st.val_col = val_col;
             const size_t sub_bytes = (size_t)P * jp.T * 16, min_bytes = (size_t)jp.min_key_cap * 4;
             HIPCHK(hipMalloc(&st.sub, sub_bytes));
@@ -2346,6 +2385,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
     jp.xoff = nullptr;
     jp.chunks = nullptr;
+    c->plan += jp.kw == 4 ? " records=compact" : " records=hash64";
     mark(c, 1);
     HIPCHK(launch_join_part_scatter(c->stream, jp, 0));
     HIPCHK(launch_join_part_probe(c->stream, jp, 0));
@@ -2373,8 +2413,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     HIPCHK(hipMemcpyAsync(offs.data(), d_off, (size_t)(nc + 1) * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     const int64_t total = offs[(size_t)nc];
-    if ((rc = buf(c, B_CELLS, (size_t)std::max<int64_t>(total, 1) * jp.RW * 8, &p))) return rc;
-    jp.cells = (int64_t *)p;
+    if ((rc = join_cells(c, jp, std::max<int64_t>(total, 1)))) return rc;
     jp.xoff = d_off;
     HIPCHK(launch_join_part_scatter(c->stream, jp, 2));
     const int64_t chunk = std::max<int64_t>(16384, (total + 2 * c->num_cus - 1) / (2 * c->num_cus));
@@ -2420,7 +2459,7 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     c->plan.clear();
     const int prc = join_agg_partitioned(c, probe, build, idx, spec, a, L);
     if (prc > 0) return prc;
-    c->plan += prc == 0 ? "join:partitioned" : "join:direct";
+    c->plan = prc == 0 ? "join:partitioned" + c->plan : "join:direct";
     if (prc < 0) {  // direct probe of the open-addressing index
         if (!idx) {
             if (int rc2 = join_build_impl(c, build, spec->build_key, &own)) return rc2;

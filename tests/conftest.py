@@ -32,6 +32,30 @@ def ctx_part():
 
 
 @pytest.fixture(scope="session")
+def ctx_part_compact():
+    """The radix-partitioned join for every probe size, with compact record
+    keys wherever the build key range allows them (payload records too)."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("join_part_min_rows", 1)
+    c.set_tuning("join_compact", 2)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="session")
+def ctx_part_hash64():
+    """The radix-partitioned join for every probe size, always with 8-byte
+    hash64 records (the compact 4-byte record keys switched off)."""
+    import falcondb_amd as fa
+    c = fa.Context(0)
+    c.set_tuning("join_part_min_rows", 1)
+    c.set_tuning("join_compact", 0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="session")
 def ctx_direct():
     """A context that always probes the open-addressing index directly."""
     import falcondb_amd as fa

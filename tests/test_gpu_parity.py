@@ -415,9 +415,10 @@ def _c3(nb, np_):
     return p, b
 
 
-@pytest.fixture(params=["partitioned", "direct"])
-def jctx(request, ctx_part, ctx_direct):
-    return ctx_part if request.param == "partitioned" else ctx_direct
+@pytest.fixture(params=["partitioned", "partitioned_compact", "partitioned_hash64", "direct"])
+def jctx(request, ctx_part, ctx_part_compact, ctx_part_hash64, ctx_direct):
+    return {"partitioned": ctx_part, "partitioned_compact": ctx_part_compact,
+            "partitioned_hash64": ctx_part_hash64, "direct": ctx_direct}[request.param]
 
 
 @pytest.mark.parametrize("jt", [INNER, LEFT, SEMI, ANTI])
@@ -541,10 +542,12 @@ def test_join_float_sums_with_inf_and_nan(jctx, jt):
             assert expect(s), (name, s)
 
 
-def test_join_skewed_probe_keys_exact_partitions(ctx_part):
+@pytest.mark.parametrize("fmt", ["compact", "hash64"])
+def test_join_skewed_probe_keys_exact_partitions(ctx_part_compact, ctx_part_hash64, fmt):
     # Zipf(1.1) probe keys: the hottest key alone fills its partition's cells,
     # so the partitioned join redoes the partitioning exactly (count, offsets,
     # chunked pass 2); results must still equal the oracle's
+    ctx_part = ctx_part_compact if fmt == "compact" else ctx_part_hash64
     K = dg.keys(20_000, 40, seed_=dg.seed(3, 9))
     probe = dg.c2_table(3_000_000, K)
     pcols = {"pk": probe["key"], "ppay": probe["v1"], "f": probe["f"]}
@@ -559,6 +562,7 @@ def test_join_skewed_probe_keys_exact_partitions(ctx_part):
                 aggs += [AggExpr(SUM, "bpay", side="build")]
             g = fa.execute_hash_join(ctx_part, pt, bt, "pk", "bk", jt, aggs, filter=p, index=idx)
             assert "exact partitions" in ctx_part.last_plan(), ctx_part.last_plan()
+            assert f"records={fmt}" in ctx_part.last_plan(), ctx_part.last_plan()
             o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=p)
             assert_batches_equal(g, o.exact, None)
     idx.free()

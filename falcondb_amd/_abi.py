@@ -25,6 +25,7 @@ P_CMP, P_AND, P_OR, P_IS_NULL, P_IS_NOT_NULL, P_IN, P_LIKE = range(7)
 COUNT, SUM, AVG, MIN, MAX = range(5)
 INNER, LEFT, SEMI, ANTI = range(4)
 NO_ROW = 0xFFFFFFFF
+MAX_GROUP_KEYS = 4  # FGPU_MAX_GROUP_KEYS
 NULL_CODE = 0xFFFFFFFF
 
 
@@ -132,6 +133,8 @@ DECLARED = {
                                      C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_void_p)]),
     "fgpu_groupby": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(GroupBySpec),
                                C.POINTER(C.c_void_p)]),
+    "fgpu_groupby_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(C.c_int32), C.c_int32,
+                                    C.POINTER(GroupBySpec), C.POINTER(C.c_void_p)]),
     "fgpu_groupby_static": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pred), C.POINTER(GroupBySpec),
                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p), C.c_int64, C.POINTER(C.c_int64)]),

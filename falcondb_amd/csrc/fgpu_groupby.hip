@@ -388,6 +388,48 @@ hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, in
     return hipGetLastError();
 }
 
+// GROUP BY over several key columns, or one Float64 key (the reference's slow
+// path, src/query/aggregate.rs:1232-1346, which keys a group on the key
+// values' display strings): one int64 key per row.  Float64 key: the value's
+// bits + 1, with every NaN made one NaN (they all print "NaN") and -0.0 kept
+// apart from 0.0 (they print "-0" and "0"); the + 1 keeps every value off
+// INT64_MIN, which stands for a NULL key.  Several columns: the mixed-radix
+// number of the columns' codes, code 0 for NULL, value - min + 1 for an
+// integer, dictionary code + 1 for text (or, for a NULL text key, the code of
+// the dictionary's "NULL" entry when it has one, as both print "NULL").
+__global__ void k_composite_key(CompositeKeyArgs a) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.n_rows; r += stride) {
+        if (a.float_key) {
+            uint64_t bits = ((const uint64_t *)a.v[0])[r];
+            const bool ok = !a.valid[0] || ((a.valid[0][r >> 6] >> (r & 63)) & 1);
+            if (((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull)) bits = CANON_NAN;
+            a.out[r] = ok ? (int64_t)(bits + 1) : EMPTY_KEY;
+            continue;
+        }
+        int64_t comp = 0;
+        for (int i = 0; i < a.n; ++i) {
+            const bool ok = !a.valid[i] || ((a.valid[i][r >> 6] >> (r & 63)) & 1);
+            int64_t code = 0;
+            if (a.dtype[i] == FGPU_UTF8) {
+                const uint32_t c = ((const uint32_t *)a.v[i])[r];
+                code = ok && c != NULL_CODE ? (int64_t)c + 1 : a.null_code[i];
+            } else {
+                const int64_t v = a.dtype[i] == FGPU_INT32 ? (int64_t)((const int32_t *)a.v[i])[r]
+                                                           : ((const int64_t *)a.v[i])[r];
+                code = ok ? v - a.kmin[i] + 1 : 0;
+            }
+            comp += code * a.mul[i];
+        }
+        a.out[r] = comp;
+    }
+}
+
+hipError_t launch_composite_key(hipStream_t s, const CompositeKeyArgs &a, int grid) {
+    hipLaunchKernelGGL(k_composite_key, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_emit(hipStream_t s, const EmitArgs &a) {
     const int64_t per_wg = 256 * EMIT_PER_THREAD;
     const int grid = (int)std::max<int64_t>(1, (a.n_slots + per_wg - 1) / per_wg);

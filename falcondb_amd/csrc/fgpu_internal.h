@@ -31,6 +31,7 @@ namespace fgpu {
 constexpr int64_t EMPTY_KEY = INT64_MIN;     // empty hash slot / NULL int key sentinel
 constexpr uint32_t NULL_CODE = 0xFFFFFFFFu;  // Utf8 NULL code
 constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
+constexpr uint64_t CANON_NAN = 0x7FF8000000000000ull;  // the quiet NaN results carry
 constexpr int MAX_COLS = 16;    // columns one kernel can read
 constexpr int MAX_OPS = 16;     // state operations per group
 constexpr int MAX_FX_WORDS = 8; // fixed-point words of one exact float sum

@@ -117,6 +117,23 @@ hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, in
                           size_t lds_bytes);
 hipError_t launch_emit(hipStream_t s, const EmitArgs &a);
 
+// One int64 GROUP BY key per row from several key columns or one Float64 key
+// (see k_composite_key).
+constexpr int MAX_GROUP_KEYS = 4;
+struct CompositeKeyArgs {
+    int n;                    // key columns
+    int float_key;            // 1: one Float64 column, out = canonical bits + 1 (NULL: INT64_MIN)
+    const void *v[MAX_GROUP_KEYS];
+    const uint64_t *valid[MAX_GROUP_KEYS];
+    int dtype[MAX_GROUP_KEYS];
+    int64_t kmin[MAX_GROUP_KEYS];       // integer columns: the value coded 1
+    int64_t mul[MAX_GROUP_KEYS];        // weight of the column's code
+    int64_t null_code[MAX_GROUP_KEYS];  // text columns: code of a NULL key
+    int64_t n_rows;
+    int64_t *out;
+};
+hipError_t launch_composite_key(hipStream_t s, const CompositeKeyArgs &a, int grid);
+
 // ─── WHERE ──────────────────────────────────────────────────────────────────
 hipError_t launch_filter_count(hipStream_t s, const DCols &c, const DPred &p, int64_t n,
                                unsigned long long *d_count, int grid);

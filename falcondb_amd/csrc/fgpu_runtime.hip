@@ -250,7 +250,7 @@ struct fgpu_result {
 // ─── context ────────────────────────────────────────────────────────────────
 enum BufSlot {
     B_KEYS, B_STATE, B_COUNTERS, B_OUT, B_MASK, B_OFFS, B_TMP, B_PASS, B_PAIRS, B_CNT,
-    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_LIVE, B_SORT, B_NSLOTS
+    B_CELLS, B_CELLCNT, B_SUB, B_SUBCNT, B_MINKEY, B_NULLBUF, B_TOPK, B_LIVE, B_SORT, B_GKEY, B_NSLOTS
 };
 
 struct Tuning {
@@ -1751,7 +1751,7 @@ static int gb_prepare(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, cons
     if (kc >= (int)t->cols.size()) return set_err(FGPU_E_COLUMN_NOT_FOUND, "group key column %d not in table", kc);
     key = kc >= 0 ? &t->cols[kc] : nullptr;
     if (key && key->dtype == FGPU_FLOAT64)
-        return set_err(FGPU_E_INVALID_QUERY, "GROUP BY on a Float64 column is not supported");
+        return set_err(FGPU_E_INVALID_QUERY, "a Float64 GROUP BY key goes through fgpu_groupby_keys");
     DPred dp;
     if (int rc = compile_pred(c, t, p, b, dp, pass_dst, pass_cap, pass_used)) return rc;
     a.pred = dp;
@@ -1827,12 +1827,150 @@ static int gb_pick_mode(const fgpu_ctx *c, const DevCol *key, const AggLayout &L
     return mode;
 }
 
+static int groupby_impl(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const fgpu_groupby_spec *spec,
+                        fgpu_result **out);
+
 int fgpu_groupby(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p,
                  const fgpu_groupby_spec *spec, fgpu_result **out) {
     if (!c || !t || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
         return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (int rc = set_device(c->device)) return rc;
+    return groupby_impl(c, t, p, spec, out);
+}
+
+// GROUP BY over several key columns or one Float64 key (the reference's slow
+// path): one int64 key per row (k_composite_key) appended to a view of the
+// table, the ordinary GROUP BY over it, then the group keys decoded back into
+// one typed column per key.
+int fgpu_groupby_keys(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const int32_t *key_cols,
+                      int32_t n_keys, const fgpu_groupby_spec *spec, fgpu_result **out) {
+    if (!c || !t || !spec || !out || !key_cols || (spec->n_aggs > 0 && !spec->aggs))
+        return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
+    if (n_keys < 1 || n_keys > FGPU_MAX_GROUP_KEYS)
+        return set_err(FGPU_E_INVALID_QUERY, "GROUP BY takes 1 to %d key columns, not %d", FGPU_MAX_GROUP_KEYS, n_keys);
+    for (int i = 0; i < n_keys; ++i)
+        if (key_cols[i] < 0 || key_cols[i] >= (int)t->cols.size())
+            return set_err(FGPU_E_COLUMN_NOT_FOUND, "group key column %d not in table", key_cols[i]);
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int rc = set_device(c->device)) return rc;
+    fgpu_groupby_spec s2 = *spec;
+    const DevCol &k0 = t->cols[key_cols[0]];
+    if (n_keys == 1 && k0.dtype != FGPU_FLOAT64) {  // the fast path
+        s2.key_col = key_cols[0];
+        return groupby_impl(c, t, p, &s2, out);
+    }
+    CompositeKeyArgs ka{};
+    ka.n = n_keys;
+    ka.n_rows = t->n_rows;
+    DevCol kc;
+    kc.name = "__group_key";
+    kc.dtype = FGPU_INT64;
+    int64_t card[FGPU_MAX_GROUP_KEYS] = {0};
+    if (n_keys == 1) {  // one Float64 key: its canonical bits + 1 (NULL: INT64_MIN)
+        ka.float_key = 1;
+        ka.v[0] = k0.d_values;
+        ka.valid[0] = k0.d_valid;
+        ka.dtype[0] = FGPU_FLOAT64;
+        kc.imin = -(int64_t(1) << 62);  // a range no dense table covers: the hash strategies
+        kc.imax = int64_t(1) << 62;
+        kc.null_count = k0.null_count;
+        kc.distinct = std::max<int64_t>(1, k0.distinct);
+    } else {
+        unsigned __int128 combos = 1;
+        double dist = 1.0;
+        for (int i = 0; i < n_keys; ++i) {
+            const DevCol &d = t->cols[key_cols[i]];
+            if (d.dtype == FGPU_FLOAT64)
+                return set_err(FGPU_E_UNSUPPORTED, "a Float64 GROUP BY key ('%s') must be the only key", d.name.c_str());
+            if (d.dtype == FGPU_UTF8) {
+                card[i] = (int64_t)d.dict.size() + 1;
+                ka.null_code[i] = 0;
+                for (size_t e = 0; e < d.dict.size(); ++e)
+                    if (d.dict[e] == "NULL") ka.null_code[i] = (int64_t)e + 1;  // NULL and "NULL" print alike
+            } else if (d.imin > d.imax) {  // no value: NULL only
+                card[i] = 1;
+                ka.kmin[i] = 0;
+            } else {
+                const unsigned __int128 span = (unsigned __int128)((__int128)d.imax - (__int128)d.imin) + 2;
+                if (span > ((unsigned __int128)1 << 62))
+                    return set_err(FGPU_E_UNSUPPORTED, "GROUP BY key '%s' spans more than 2^62 values", d.name.c_str());
+                card[i] = (int64_t)span;
+                ka.kmin[i] = d.imin;
+            }
+            ka.v[i] = d.d_values;
+            ka.valid[i] = d.d_valid;
+            ka.dtype[i] = d.dtype;
+            ka.mul[i] = (int64_t)combos;
+            combos *= (unsigned __int128)card[i];
+            if (combos > ((unsigned __int128)1 << 62))
+                return set_err(FGPU_E_UNSUPPORTED, "the GROUP BY keys have more than 2^62 combinations");
+            dist *= (double)std::max<int64_t>(1, d.distinct) + (d.null_count ? 1.0 : 0.0);
+        }
+        kc.imin = 0;
+        kc.imax = (int64_t)combos - 1;
+        kc.distinct = (int64_t)std::max(1.0, std::min({dist, (double)std::max<int64_t>(1, t->n_rows), (double)combos}));
+    }
+    void *kbuf;
+    if (int rc = buf(c, B_GKEY, (size_t)std::max<int64_t>(1, t->n_rows) * 8, &kbuf)) return rc;
+    ka.out = (int64_t *)kbuf;
+    kc.d_values = kbuf;
+    if (t->n_rows) HIPCHK(launch_composite_key(c->stream, ka, grid_for(c, t->n_rows, 256, 8)));
+    fgpu_table view;
+    view.device = t->device;
+    view.n_rows = t->n_rows;
+    view.batch_rows = t->batch_rows;
+    view.bytes = t->bytes;
+    view.cols = t->cols;
+    view.cols.push_back(kc);
+    s2.key_col = (int)view.cols.size() - 1;
+    fgpu_result *r = nullptr;
+    if (int rc = groupby_impl(c, &view, p, &s2, &r)) return rc;
+    // one column per key, decoded from the group's int64 key
+    std::vector<RCol> keys((size_t)n_keys);
+    const RCol &gk = r->cols[0];
+    const int64_t n = gk.xv ? r->rows : 0;  // 0: the columns stayed on the device
+    const int64_t *kv = (const int64_t *)gk.xv;
+    const uint8_t *kok = gk.xvalid;
+    for (int i = 0; i < n_keys; ++i) {
+        const DevCol &d = t->cols[key_cols[i]];
+        RCol &rc = keys[(size_t)i];
+        rc.name = d.name;
+        rc.dtype = d.dtype == FGPU_UTF8 ? FGPU_UTF8 : d.dtype == FGPU_FLOAT64 ? FGPU_FLOAT64 : FGPU_INT64;
+        rc.dict_src = d.dtype == FGPU_UTF8 ? &d : nullptr;
+        rc.valid.resize((size_t)n);
+        if (rc.dtype == FGPU_UTF8) rc.u32.resize((size_t)n);
+        else if (rc.dtype == FGPU_FLOAT64) rc.f64.resize((size_t)n);
+        else rc.i64.resize((size_t)n);
+    }
+    for (int64_t g = 0; g < n; ++g) {
+        if (ka.float_key) {
+            const uint64_t bits = (uint64_t)kv[g] - 1;
+            double x;
+            std::memcpy(&x, &bits, 8);
+            keys[0].f64[(size_t)g] = kok[g] ? x : 0.0;
+            keys[0].valid[(size_t)g] = kok[g] ? 1 : 0;
+            continue;
+        }
+        int64_t comp = kv[g];
+        for (int i = 0; i < n_keys; ++i) {
+            const int64_t code = comp % card[i];
+            comp /= card[i];
+            RCol &rc = keys[(size_t)i];
+            rc.valid[(size_t)g] = code != 0;
+            if (rc.dtype == FGPU_UTF8) rc.u32[(size_t)g] = code ? (uint32_t)(code - 1) : NULL_CODE;
+            else rc.i64[(size_t)g] = code ? ka.kmin[i] + code - 1 : 0;
+        }
+    }
+    std::vector<RCol> cols = std::move(keys);
+    for (size_t i = 1; i < r->cols.size(); ++i) cols.push_back(std::move(r->cols[i]));
+    r->cols = std::move(cols);
+    *out = r;
+    return FGPU_OK;
+}
+
+static int groupby_impl(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const fgpu_groupby_spec *spec,
+                        fgpu_result **out) {
     const DevCol *key = nullptr;
     Binder b(t);
     GbArgs a{};

@@ -442,13 +442,14 @@ FGPU_DEV void emit_row(const DOuts &outs, const uint64_t *st, int64_t o, int64_t
                 out = __double_as_longlong(s2);
                 break;
             }
+            // MIN / MAX fold like f64::min / f64::max: NaN values are skipped,
+            // a group whose values are all NaN yields NaN (the state still holds
+            // its identity), a group without values NULL
             case OUT_MIN_F:
-                out = (int64_t)f64_unorder(w);
-                valid = valid && w != ~0ull;
+                out = w == ~0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w);
                 break;
             case OUT_MAX_F:
-                out = (int64_t)f64_unorder(w);
-                valid = valid && w != 0ull;
+                out = w == 0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w);
                 break;
         }
         out_val[k][o] = out;

@@ -586,19 +586,24 @@ def execute_select(ctx: Context, table: Table, filter: Expr | None,
 
 
 class GroupByExec:
-    """GROUP BY over one key column with COUNT/SUM/AVG/MIN/MAX aggregates
-    (GroupByExec, src/query/aggregate.rs:535).  The WHERE filter is fused into
-    the same device pass instead of materialising filtered batches."""
+    """GROUP BY with COUNT/SUM/AVG/MIN/MAX aggregates (GroupByExec,
+    src/query/aggregate.rs:535).  One Int32 / Int64 / Utf8 key takes the
+    reference's fast path (fgpu_groupby); several keys, or one Float64 key,
+    its slow path (execute_slow, :1232-1346; fgpu_groupby_keys), which groups
+    on the keys' display strings.  The WHERE filter is fused into the same
+    device pass instead of materialising filtered batches."""
 
     def __init__(self, group_by: Sequence[str | ColumnRef], aggs: Sequence[AggExpr]):
-        if len(group_by) != 1:
-            raise FalconError(6, "GROUP BY takes exactly one key column on this backend")
-        g = group_by[0]
-        self.key = g.name if isinstance(g, ColumnRef) else g
+        if not 1 <= len(group_by) <= _abi.MAX_GROUP_KEYS:
+            raise FalconError(6, f"GROUP BY takes 1 to {_abi.MAX_GROUP_KEYS} key columns on this backend")
+        self.keys = [g.name if isinstance(g, ColumnRef) else g for g in group_by]
+        self.key = self.keys[0]
         self.aggs = list(aggs)
 
     def execute(self, ctx: Context, table: Table, filter: Expr | None = None) -> RecordBatch:
-        return _groupby(ctx, table, filter, table.column_index(self.key), self.aggs)
+        if len(self.keys) == 1 and table.dtypes[table.column_index(self.key)] != _abi.FLOAT64:
+            return _groupby(ctx, table, filter, table.column_index(self.key), self.aggs)
+        return _groupby_keys(ctx, table, filter, [table.column_index(k) for k in self.keys], self.aggs)
 
 
 class BatchGroupBy:
@@ -720,6 +725,16 @@ def _groupby(ctx, table, filter, key_col, aggs) -> RecordBatch:
     _check(_lib().fgpu_groupby(ctx.handle, table.handle, p, C.byref(spec), C.byref(h)))
     funcs = ([None] if key_col >= 0 else []) + [a.func for a in aggs]
     return _read_result(h, funcs)
+
+
+def _groupby_keys(ctx, table, filter, key_cols, aggs) -> RecordBatch:
+    p, keep = _pred(filter, table)
+    arr, keep2 = _aggs_struct(aggs, lambda side, name: table.column_index(name))
+    spec = _abi.GroupBySpec(-1, len(aggs), C.cast(arr, C.POINTER(_abi.Agg)))
+    kc = (C.c_int32 * len(key_cols))(*key_cols)
+    h = C.c_void_p()
+    _check(_lib().fgpu_groupby_keys(ctx.handle, table.handle, p, kc, len(key_cols), C.byref(spec), C.byref(h)))
+    return _read_result(h, [None] * len(key_cols) + [a.func for a in aggs])
 
 
 class JoinIndex:

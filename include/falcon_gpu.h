@@ -246,6 +246,21 @@ int fgpu_filter_select(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
 int fgpu_groupby(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p,
                  const fgpu_groupby_spec *spec, fgpu_result **out);
 
+/* GROUP BY over key_cols[0..n_keys) (1..FGPU_MAX_GROUP_KEYS columns), the
+ * general form of fgpu_groupby (spec->key_col is ignored).  One Int32 / Int64 /
+ * Utf8 key is GroupByExec's fast path and runs exactly as fgpu_groupby.  Several
+ * keys, or one Float64 key, are its slow path (GroupByExec::execute_slow /
+ * accumulate_slow, src/query/aggregate.rs:1232-1346), which keys a group on the
+ * display strings of its key values (GroupKey::from_row, :79-83): equal values
+ * group, NULL is a group of its own (a Utf8 NULL joins the text "NULL"), every
+ * NaN is one group and -0.0 is apart from 0.0.  Output: one column per key
+ * (Int32 / Int64 as Int64, Float64, Utf8; NULL keys invalid), then the
+ * aggregates.  The key columns' combinations must number at most 2^62 (Float64
+ * keys only alone); otherwise FGPU_E_UNSUPPORTED. */
+#define FGPU_MAX_GROUP_KEYS 4
+int fgpu_groupby_keys(fgpu_ctx *ctx, const fgpu_table *t, const fgpu_pred *p, const int32_t *key_cols,
+                      int32_t n_keys, const fgpu_groupby_spec *spec, fgpu_result **out);
+
 /* GROUP BY that allocates nothing.  No counterpart in the reference: this is
  * for callers that cannot allocate device or pinned memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).

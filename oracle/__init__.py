@@ -265,6 +265,86 @@ def groupby(columns: dict, expr: Expr | None, key: str | None, aggs: list[AggExp
     return _read(h, ([None] if key else []) + [a.func for a in aggs])
 
 
+def _unpack_mask(words: np.ndarray, n: int) -> np.ndarray:
+    """LSB-first u64 mask words (src/storage/column.rs:132-133) as n bools."""
+    return np.unpackbits(words.view(np.uint8), bitorder="little")[:n].astype(bool)
+
+
+def _display_keys(col, n: int) -> list[str]:
+    """Each row's key value as the reference prints it for GroupKey::from_row
+    (ScalarValue's Display, src/storage/types.rs:145-161): NULL prints "NULL",
+    text prints itself, numbers their value.  Only equality of the strings
+    matters, so floats print through repr (shortest round trip): every NaN
+    prints alike, -0.0 and 0.0 differ, as "NaN" / "-0" / "0" do."""
+    if isinstance(col, DictColumn):
+        d = list(col.dictionary)
+        return ["NULL" if c == _abi.NULL_CODE or c >= len(d) else d[c] for c in np.asarray(col.codes).tolist()]
+    if isinstance(col, NullableColumn):
+        vals, ok = np.asarray(col.values), np.asarray(col.valid, bool)
+    else:
+        vals, ok = np.asarray(col), np.ones(n, bool)
+    conv = repr if vals.dtype.kind == "f" else str
+    return [conv(v) if k else "NULL" for v, k in zip(vals.tolist(), ok.tolist())]
+
+
+def groupby_keys(columns: dict, expr: Expr | None, keys: list[str], aggs: list[AggExpr],
+                 batch_rows: int = 65536) -> RecordBatch:
+    """GroupByExec::execute_slow / accumulate_slow (src/query/aggregate.rs:1232-1346),
+    for small inputs: a group per distinct GroupKey::from_row (:79-83), the
+    display strings of a row's key values joined by NUL, over the rows that
+    pass the WHERE filter; a group's key values are those of its first row
+    (group_key_values, :1306-1310).  Each group's aggregates are the oracle's
+    exact ones (groupby over a dense group-id column).  Key columns come out as
+    Int64 for integers, Float64, Utf8 (scalar_vec_to_column, :3422-3462)."""
+    c = _Cols(columns)
+    n = c.n
+    passing = (_unpack_mask(filter_mask(columns, expr, batch_rows), n) if expr is not None
+               else np.ones(n, bool))
+    disp = [_display_keys(columns[k], n) for k in keys]
+    gid = np.zeros(n, np.int64)
+    first: dict[str, int] = {}
+    rep: list[int] = []
+    for r in range(n):
+        if not passing[r]:
+            continue
+        s = "\x00".join(d[r] for d in disp)
+        g = first.get(s)
+        if g is None:
+            g = first[s] = len(rep)
+            rep.append(r)
+        gid[r] = g
+    cols2 = dict(columns)
+    cols2["__gid"] = gid
+    ex = groupby(cols2, expr, "__gid", aggs, batch_rows=batch_rows).exact
+    order = np.argsort(ex.columns[0], kind="stable")
+    names, dts, out, valid = [], [], [], []
+    for k in keys:
+        col = columns[k]
+        rows = np.array(rep, np.int64)
+        names.append(k)
+        if isinstance(col, DictColumn):
+            d = list(col.dictionary)
+            codes = np.asarray(col.codes)[rows]
+            v = np.array([None if x == _abi.NULL_CODE or x >= len(d) else d[x] for x in codes.tolist()], dtype=object)
+            dts.append("Utf8")
+            valid.append(np.array([x is not None for x in v.tolist()], bool))
+        else:
+            vals = np.asarray(col.values if isinstance(col, NullableColumn) else col)
+            ok = np.asarray(col.valid, bool)[rows] if isinstance(col, NullableColumn) else np.ones(rows.size, bool)
+            isf = vals.dtype.kind == "f"
+            v = vals[rows].astype(np.float64 if isf else np.int64)
+            v[~ok] = 0
+            dts.append("Float64" if isf else "Int64")
+            valid.append(ok)
+        out.append(v)
+    for i in range(1, len(ex.names)):
+        names.append(ex.names[i])
+        dts.append(ex.dtypes[i])
+        out.append(ex.columns[i][order])
+        valid.append(ex.validity[i][order] if ex.validity[i] is not None else None)
+    return RecordBatch(names, dts, out, valid, [None] * len(keys) + [a.func for a in aggs])
+
+
 def join(probe: dict, build: dict, probe_key: str, build_key: str, join_type: int,
          aggs: list[AggExpr], expr: Expr | None = None, exact: bool = True,
          pairs: bool = False, batch_rows: int = 65536):

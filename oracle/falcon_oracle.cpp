@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -749,9 +750,9 @@ void exact_values(const std::vector<AggPlan> &plan, const std::vector<FBatch> &f
             oc.f64.resize(n_groups);
             for (size_t g = 0; g < n_groups; ++g) {
                 if (ap.func == FGPU_SUM) oc.f64[g] = cnt[g] ? fs[g].result() : 0.0;
-                else {
-                    oc.f64[g] = ap.func == FGPU_MIN ? fmn[g] : fmx[g];
-                    oc.valid[g] = fhas[g];
+                else {  // f64::min / f64::max folding: NaN skipped, all-NaN -> NaN
+                    oc.f64[g] = !fhas[g] ? std::numeric_limits<double>::quiet_NaN()
+                                : ap.func == FGPU_MIN ? fmn[g] : fmx[g];
                 }
             }
         } else if (ap.func == FGPU_SUM) {
@@ -1391,9 +1392,12 @@ int ora_join(const fgpu_column_desc *pcd, int pncols, int64_t np,
                 }
                 if (g.func == FGPU_AVG) v = nv ? v / (double)nv : 0.0;
             } else {
+                // f64::min / f64::max folding (src/query/aggregate.rs:283-302): NaN
+                // skipped, all-NaN -> NaN
                 bool has = false;
                 for (double x : vals)
                     if (!std::isnan(x) && (!has || (g.func == FGPU_MIN ? x < v : x > v))) { v = x; has = true; }
+                if (!has) v = std::numeric_limits<double>::quiet_NaN();
             }
             oc.f64.assign(1, v);
         } else {

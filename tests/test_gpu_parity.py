@@ -246,6 +246,44 @@ def test_groupby_partitioned_tile_paths(ctx_gbp, pred, tiles):
     assert_batches_equal(g, o.exact, "key", o.reference)
 
 
+@pytest.mark.parametrize("path", ["dense", "partitioned", "hash", "scalar", "join"])
+def test_float_minmax_nan_semantics(ctx, ctx_gbp, ctx_gbh, ctx_part, path):
+    # MIN / MAX fold like f64::min / f64::max: NaN skipped, a group of NaN
+    # values alone gives NaN, a group without values NULL; equal to the oracle
+    rng = np.random.default_rng(91)
+    n = 200_000
+    wide = path in ("partitioned", "hash")
+    key = rng.integers(0, 50_000 if wide else 40, n)
+    x = rng.normal(size=n)
+    x[rng.random(n) < 0.3] = np.nan
+    x[key < 5] = np.nan               # groups of NaN alone
+    x[key == 7] = np.where(np.arange(n)[key == 7] % 2 == 0, np.nan, 2.5)  # NaN first in row order
+    aggs = [AggExpr(MIN, "x"), AggExpr(MAX, "x"), AggExpr(COUNT)]
+    cols = {"key": key, "x": x}
+    if path == "join":
+        build = {"bk": np.arange(0, 60, dtype=np.int64), "bx": rng.random(60)}
+        pt, bt = _table(ctx_part, cols), _table(ctx_part, build)
+        for jt in (INNER, LEFT, SEMI, ANTI):
+            g = fa.execute_hash_join(ctx_part, pt, bt, "key", "bk", jt, aggs)
+            assert_batches_equal(g, ora.join(cols, build, "key", "bk", jt, aggs).exact, None)
+        return
+    c = {"dense": ctx, "partitioned": ctx_gbp, "hash": ctx_gbh, "scalar": ctx}[path]
+    t = _table(c, cols)
+    keyname = None if path == "scalar" else "key"
+    for p in (None, col("key") < 6):
+        g = (fa.GroupByExec([keyname], aggs).execute(c, t, p) if keyname
+             else fa.execute_agg_only(c, t, aggs, p))
+        o = ora.groupby(cols, p, keyname, aggs)
+        assert_batches_equal(g, o.exact, keyname)
+    if path == "scalar":  # NaN alone, and no rows at all
+        for p, expect_nan in ((col("key") < 5, True), (col("key") > 10**9, False)):
+            g = fa.execute_agg_only(c, t, aggs[:2], p)
+            if expect_nan:
+                assert np.isnan(g.columns[0][0]) and np.isnan(g.columns[1][0])
+            else:
+                assert not g.validity[0][0] and not g.validity[1][0]
+
+
 def _with_specials(x, rng, pinf=0, ninf=0, nan=0):
     """x with `pinf` +inf, `ninf` -inf and `nan` NaN values at distinct random rows."""
     x = x.copy()
@@ -720,7 +758,9 @@ def test_float_minmax_skip_nan_even_when_first(ctx, ctx_gbp, ctx_gbh):
     o = ora.groupby(cols, None, "k", aggs)
     exp = o.exact.sort_by("k")
     assert list(exp.columns[1][:2]) == [-2.0, 3.0] and list(exp.columns[2][:2]) == [1.0, 3.0]
-    assert not exp.validity[1][2] and not exp.validity[2][2]  # all-NaN group: NULL
+    # a group of NaN values alone: NaN, as f64::min / f64::max fold it
+    assert exp.validity[1][2] and exp.validity[2][2]
+    assert math.isnan(exp.columns[1][2]) and math.isnan(exp.columns[2][2])
     ref = o.reference.sort_by("k")
     assert math.isnan(ref.columns[1][0]) and math.isnan(ref.columns[2][0])
     for c in (ctx, ctx_gbp, ctx_gbh):

@@ -492,20 +492,57 @@ def _huge_column(n):
     return x
 
 
+def _fold_min_max(vals):
+    """f64::min / f64::max folded left to right (the NaN-skipping IEEE minNum /
+    maxNum): NaN unless some value is not NaN, then the min / max of those."""
+    clean = [v for v in vals if not math.isnan(v)]
+    return (min(clean), max(clean)) if clean else (math.nan, math.nan)
+
+
 def test_float_minmax_nan_semantics():
-    # Exact columns skip NaN in MIN / MAX, as the reference's execute_agg_only
-    # does (f64::min / f64::max, src/query/aggregate.rs:3296-3330); the
-    # reference-order columns restate the GROUP BY fast path, whose
-    # `!has_val || v < sum` keeps a NaN that comes first (:1093-1101).
-    k = np.array([0, 0, 0, 1, 1], np.int64)
-    x = np.array([np.nan, 1.0, -2.0, 3.0, np.nan])
+    # Exact columns fold MIN / MAX like f64::min / f64::max, as the reference's
+    # execute_agg_only (src/query/aggregate.rs:3296-3330) and its GROUP BY slow
+    # path (Accumulator::update, :283-302) do: NaN values are skipped and a
+    # group of NaN values alone yields NaN.  The reference-order columns
+    # restate the GROUP BY fast path, whose `!has_val || v < sum` keeps a NaN
+    # that comes first (:1093-1101).
+    k = np.array([0, 0, 0, 1, 1, 2, 2], np.int64)
+    x = np.array([np.nan, 1.0, -2.0, 3.0, np.nan, np.nan, np.nan])
     o = ora.groupby({"k": k, "x": x}, None, "k", [AggExpr(MIN, "x"), AggExpr(MAX, "x")])
     e, r = o.exact.sort_by("k"), o.reference.sort_by("k")
-    assert list(e.columns[1]) == [-2.0, 3.0] and list(e.columns[2]) == [1.0, 3.0]
+    for g in range(3):
+        lo, hi = _fold_min_max(list(x[k == g]))
+        assert (math.isnan(lo) and math.isnan(e.columns[1][g])) or e.columns[1][g] == lo
+        assert (math.isnan(hi) and math.isnan(e.columns[2][g])) or e.columns[2][g] == hi
+        assert e.validity[1] is None or e.validity[1][g]
+    assert list(e.columns[1][:2]) == [-2.0, 3.0] and list(e.columns[2][:2]) == [1.0, 3.0]
+    assert math.isnan(e.columns[1][2]) and math.isnan(e.columns[2][2])
     assert math.isnan(r.columns[1][0]) and math.isnan(r.columns[2][0])
-    assert list(r.columns[1][1:]) == [3.0] and list(r.columns[2][1:]) == [3.0]
+    assert list(r.columns[1][1:2]) == [3.0] and list(r.columns[2][1:2]) == [3.0]
     s = ora.groupby({"x": x}, None, None, [AggExpr(MIN, "x"), AggExpr(MAX, "x")]).exact
     assert [s.columns[0][0], s.columns[1][0]] == [-2.0, 3.0]
+    # no GROUP BY over NaN values alone: NaN (a value exists); over no rows: NULL
+    s = ora.groupby({"x": np.array([np.nan, np.nan])}, None, None, [AggExpr(MIN, "x"), AggExpr(MAX, "x")]).exact
+    assert math.isnan(s.columns[0][0]) and math.isnan(s.columns[1][0])
+    s = ora.groupby({"x": np.array([1.0])}, col("x") > 5, None, [AggExpr(MIN, "x")]).exact
+    assert s.validity[0] is not None and not s.validity[0][0]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_float_minmax_brute_force(seed):
+    # random groups with NaN sprinkled in, some groups NaN only
+    rng = np.random.default_rng(600 + seed)
+    n = 3000
+    k = rng.integers(0, 60, n)
+    x = rng.normal(size=n)
+    x[rng.random(n) < 0.2] = np.nan
+    x[k < 3] = np.nan
+    e = ora.groupby({"k": k, "x": x}, None, "k", [AggExpr(MIN, "x"), AggExpr(MAX, "x")]).exact.sort_by("k")
+    for i, g in enumerate(e.columns[0]):
+        lo, hi = _fold_min_max(list(x[k == g]))
+        got_lo, got_hi = e.columns[1][i], e.columns[2][i]
+        assert (math.isnan(lo) and math.isnan(got_lo)) or got_lo == lo
+        assert (math.isnan(hi) and math.isnan(got_hi)) or got_hi == hi
 
 
 @pytest.mark.parametrize("seed", range(3))

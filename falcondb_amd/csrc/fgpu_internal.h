@@ -34,7 +34,7 @@ constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
 constexpr uint64_t CANON_NAN = 0x7FF8000000000000ull;  // the quiet NaN results carry
 constexpr int MAX_COLS = 16;    // columns one kernel can read
 constexpr int MAX_OPS = 16;     // state operations per group
-constexpr int MAX_FX_WORDS = 8; // fixed-point words of one exact float sum
+constexpr int MAX_FX_WORDS = 15;  // fixed-point words of one exact float sum (a packed op's W field)
 constexpr int BLOCK = 256;      // threads per workgroup for the scan kernels
 
 // ─── hashing ────────────────────────────────────────────────────────────────
@@ -73,8 +73,17 @@ FGPU_HD uint64_t f64_unorder(uint64_t enc) {
 // ─── exact fixed-point digits of a double ───────────────────────────────────
 // Splits finite nonzero x into three signed digits (< 2^32 in magnitude except
 // the top one, < 2^22) added at words w0, w0+1, w0+2 of a fixed point number
-// with LSB 2^L.  Returns false for zero (nothing to add).  Requires x to be a
-// multiple of 2^L (true for every value whose mantissa-LSB exponent is >= L).
+// with LSB 2^L.  Returns false for zero (nothing to add).  A value that is not
+// a multiple of 2^L (a column whose exponents span more than the accumulator
+// holds, so L sits above its smallest mantissa LSB) is first rounded to the
+// nearest multiple of 2^L, ties to even; for a multiple of 2^L that rounding
+// changes nothing.
+// m >> s rounded to nearest, ties to even (0 < s; m < 2^53).
+FGPU_HD uint64_t rne_shift(uint64_t m, int s) {
+    if (s >= 64) return 0;
+    const uint64_t q = m >> s;
+    return (m + ((1ull << (s - 1)) - 1) + (q & 1)) >> s;
+}
 // Branch-free form of fx_digits for a finite value (zero yields three zero
 // digits).  The exact-sum layout leaves room for digits w0..w0+2 of every
 // value of the column, so callers add all three without tests.
@@ -82,8 +91,8 @@ FGPU_DEV void fx_digits3(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1
     const int be = (int)((bits >> 52) & 0x7ff);
     uint64_t m = (bits & 0x000fffffffffffffull) | (be ? 0x0010000000000000ull : 0ull);
     const int e = (be ? be : 1) - 1075;
-    const int s = L - e;  // > 0: the value's low s bits are zero (it is a multiple of 2^L)
-    m = s > 0 ? (s < 64 ? m >> s : 0ull) : m;
+    const int s = L - e;  // > 0: the value's low s bits are below the accumulator's LSB
+    m = s > 0 ? rne_shift(m, s) : m;
     const int shift = s > 0 ? 0 : -s;
     w0 = shift >> 5;
     const int off = shift & 31;
@@ -107,9 +116,9 @@ FGPU_HD bool fx_digits(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1, 
         m |= 0x0010000000000000ull;
         e = be - 1075;
     }
-    if (e < L) {  // trailing zeros cover the difference (x is a multiple of 2^L)
-        const int s = L - e;
-        m = s >= 64 ? 0 : (m >> s);
+    if (e < L) {  // bits below the accumulator's LSB: round to a multiple of 2^L
+        m = rne_shift(m, L - e);
+        if (m == 0) return false;
         e = L;
     }
     const int shift = e - L;

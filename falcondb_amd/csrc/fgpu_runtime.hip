@@ -663,7 +663,17 @@ struct AggLayout {
     std::vector<std::string> names;
 };
 
-int fx_words(const DevCol &c, int &L, int &W) {
+// Exact-sum layout of float column c: LSB exponent L and W words.  Exact for
+// columns whose mantissa-LSB exponents span at most 364 binades (15 words).
+// A wider column is refused unless `one_row` (the aggregate has one output
+// row: no GROUP BY key, or a join), where the accumulator's LSB moves up to
+// the largest L that MAX_FX_WORDS words still cover and every value is
+// rounded to a multiple of 2^L before it is added (fx_digits): the sum is
+// the exact sum of the rounded values, within n * 2^(L-1) of the exact sum,
+// far below 1e-12 of any sum that does not cancel against the column's
+// largest values.  Per group such a bound would be relative to the whole
+// column, not to the group, so GROUP BY refuses those columns.
+int fx_words(const DevCol &c, int &L, int &W, bool one_row) {
     if (c.emin > c.emax) {  // no finite nonzero value
         L = 0;
         W = 1;
@@ -673,10 +683,15 @@ int fx_words(const DevCol &c, int &L, int &W) {
     // a per-thread partial sum (< 2^31 addends) may be 31 bits larger still.
     L = c.emin;
     W = ((c.emax - c.emin) + 53 + 31 + 31) / 32 + 1;
-    if (W > MAX_FX_WORDS)
-        return set_err(FGPU_E_UNSUPPORTED,
-                       "column '%s': float exponent range 2^%d..2^%d is too wide for the exact SUM "
-                       "accumulator (%d words > %d)", c.name.c_str(), c.emin, c.emax + 52, W, MAX_FX_WORDS);
+    if (W > MAX_FX_WORDS) {
+        if (!one_row)
+            return set_err(FGPU_E_UNSUPPORTED,
+                           "column '%s': float exponent range 2^%d..2^%d is too wide for an exact SUM per group "
+                           "(%d words > %d)", c.name.c_str(), c.emin, c.emax + 52, W, MAX_FX_WORDS);
+        L = c.emax - ((MAX_FX_WORDS - 1) * 32 - (53 + 31 + 31) + 31);
+        W = ((c.emax - L) + 53 + 31 + 31) / 32 + 1;
+        if (W > MAX_FX_WORDS) return set_err(FGPU_E_DEVICE, "exact SUM layout of '%s'", c.name.c_str());
+    }
     return FGPU_OK;
 }
 
@@ -708,7 +723,7 @@ int fsum_scale(const DevCol &c, int64_t rows) {
 
 // Adds (or finds) an op; returns its index.  `col` is a table column index;
 // `rows` bounds the number of values any one group's aggregate takes in.
-int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot, int64_t rows) {
+int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot, int64_t rows, bool one_row) {
     for (size_t i = 0; i < L.keys.size(); ++i)
         if (L.keys[i].kind == kind && L.keys[i].side == side && L.keys[i].col == col) return (int)i;
     if (L.ops.n >= MAX_OPS) return set_err(FGPU_E_UNSUPPORTED, "too many aggregate states (max %d)", MAX_OPS) * 0 - 1;
@@ -720,7 +735,7 @@ int add_op(AggLayout &L, int kind, int side, int col, const DevCol &dc, int slot
     o.rscale = 1.0;
     if (kind == OP_FSUM) {
         int l, w;
-        if (fx_words(dc, l, w)) return -1;
+        if (fx_words(dc, l, w, one_row)) return -1;
         o.L = l;
         o.W = w;
         o.has_flags = dc.nonfinite ? 1 : 0;
@@ -768,8 +783,10 @@ std::string alias_of(const fgpu_agg &a, int i) {
 // `rows` bounds the number of values one group's aggregate takes in (-1: the
 // rows of t0).
 template <class SlotFn>
+// one_row: the aggregates have a single output row (no GROUP BY key, or a
+// join), which lets an exact SUM take a rounded window (fx_words).
 int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const fgpu_table *t1,
-                 SlotFn slot_of, AggLayout &L, int64_t rows = -1) {
+                 SlotFn slot_of, AggLayout &L, int64_t rows = -1, bool one_row = false) {
     if (rows < 0) rows = t0->n_rows;
     L = AggLayout{};
     L.ops.n = 0;
@@ -803,7 +820,7 @@ int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const f
         // A NULL build side (LEFT) makes every build column nullable.
         const bool need_cnt = nullable || side == 1 || a.func == FGPU_AVG || a.func == FGPU_COUNT;
         if (need_cnt && (nullable || side == 1)) {
-            const int ci = add_op(L, OP_CNT, side, a.col, dc, slot, rows);
+            const int ci = add_op(L, OP_CNT, side, a.col, dc, slot, rows, one_row);
             if (ci < 0) return FGPU_E_UNSUPPORTED;
             cnt_word = L.ops.op[ci].word;
         }
@@ -819,7 +836,7 @@ int build_layout(const fgpu_agg *aggs, int n_aggs, const fgpu_table *t0, const f
             case FGPU_MIN: kind = isf ? OP_FMIN : OP_IMIN; o.kind = isf ? OUT_MIN_F : OUT_MIN_I; break;
             default: kind = isf ? OP_FMAX : OP_IMAX; o.kind = isf ? OUT_MAX_F : OUT_MAX_I; break;
         }
-        const int oi = add_op(L, kind, side, a.col, dc, slot, rows);
+        const int oi = add_op(L, kind, side, a.col, dc, slot, rows, one_row);
         if (oi < 0) return g_err.empty() ? set_err(FGPU_E_UNSUPPORTED, "aggregate layout") : FGPU_E_UNSUPPORTED;
         const DOp &op = L.ops.op[oi];
         o.word = op.word;
@@ -1771,7 +1788,7 @@ static int gb_prepare(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, cons
         if (col < 256) in_of_col[col] = a.n_in;
         return a.n_in++;
     };
-    if (int rc = build_layout(spec->aggs, spec->n_aggs, t, nullptr, slot_of, L))
+    if (int rc = build_layout(spec->aggs, spec->n_aggs, t, nullptr, slot_of, L, -1, kc < 0))
         return too_many ? set_err(FGPU_E_UNSUPPORTED, "at most %d distinct aggregate input columns", MAX_IN) : rc;
     a.cols = b.dc;
     a.ops = L.ops;
@@ -2276,7 +2293,7 @@ static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *p
         const __int128 pairs = (__int128)probe->n_rows * std::max<int64_t>(build->n_rows, 1);
         out_rows = pairs > (__int128)INT64_MAX ? INT64_MAX : (int64_t)pairs;
     }
-    if (int rc = build_layout(spec->aggs, spec->n_aggs, probe, build, slot_of, L, out_rows)) return rc;
+    if (int rc = build_layout(spec->aggs, spec->n_aggs, probe, build, slot_of, L, out_rows, true)) return rc;
     if (L.ops.n > 4) return set_err(FGPU_E_UNSUPPORTED, "join aggregates need %d accumulators (max 4)", L.ops.n);
     a.pcols = pb.dc;
     a.bcols = bb.dc;
@@ -3046,7 +3063,7 @@ int fgpu_host_exact_sum(const double *x, int64_t n, double *out) {
     dc.emin = emin;
     dc.emax = emax;
     int L, W;
-    if (int rc = fx_words(dc, L, W)) return rc;
+    if (int rc = fx_words(dc, L, W, true)) return rc;
     int64_t w[MAX_FX_WORDS] = {0};
     for (int64_t i = 0; i < n; ++i) {
         const uint64_t b = __builtin_bit_cast(uint64_t, x[i]);

@@ -13,7 +13,7 @@ for its position in the SELECT list.
 
 Accepted:
     SELECT item, ... FROM t [alias] [[INNER | LEFT [OUTER]] JOIN b [alias] ON x.k = y.k]
-        [WHERE pred] [GROUP BY col] [ORDER BY col [ASC | DESC]] [LIMIT n]
+        [WHERE pred] [GROUP BY col, ...] [ORDER BY col [ASC | DESC]] [LIMIT n]
     item := [t.]col [AS a] | COUNT(*) | AGG([t.]col) [AS a]      AGG: COUNT SUM AVG MIN MAX
     pred := [t.]col op literal | literal op [t.]col | [t.]col IS [NOT] NULL
             | pred AND pred | pred OR pred | (pred)
@@ -100,7 +100,7 @@ class Query:
     alias: str | None
     join: tuple | None = None          # (type, table, alias, left ColName, right ColName)
     where: object = None               # predicate tree
-    group_by: ColName | None = None
+    group_by: list | None = None  # ColName per key
     order_by: list | None = None       # [(ColName, desc, nulls_last)]
     limit: int | None = None
     offset: int = 0
@@ -279,9 +279,9 @@ class _Parser:
             q.where = self.disjunction()
         if self.accept("kw", "GROUP"):
             self.expect("kw", "BY")
-            q.group_by = self.colname()
-            if self.peek() == ("punct", ","):
-                raise FalconError(UNSUPPORTED, "SQL: GROUP BY over one column only")
+            q.group_by = [self.colname()]
+            while self.accept("punct", ","):
+                q.group_by.append(self.colname())
         if self.accept("kw", "ORDER"):
             self.expect("kw", "BY")
             q.order_by = []
@@ -474,15 +474,16 @@ class Database:
                 n = count_star(self.ctx, t, where)  # the COUNT(*) fast path
                 return RecordBatch([names[0]], ["Int64"], [np.array([n], np.int64)], [None], [_abi.COUNT])
             return execute_agg_only(self.ctx, t, aggs, where)
-        key = q.group_by.name
+        keys = [c.name for c in q.group_by]
+        low = [k.lower() for k in keys]
         for it, _ in plain:
-            if it.arg.name.lower() != key.lower():
+            if it.arg.name.lower() not in low:
                 raise FalconError(INVALID_QUERY, f"column '{it.arg.name}' must appear in GROUP BY")
-        res = GroupByExec([key], aggs).execute(self.ctx, t, where)
-        # operator order: key, then the aggregates; SELECT order may differ
-        cols, dts, valid, funcs, ai = [], [], [], [], 1
+        res = GroupByExec(keys, aggs).execute(self.ctx, t, where)
+        # operator order: the keys, then the aggregates; SELECT order may differ
+        cols, dts, valid, funcs, ai = [], [], [], [], len(keys)
         for it, nm in zip(q.items, names):
-            j = 0 if it.func is None else ai
+            j = low.index(it.arg.name.lower()) if it.func is None else ai
             if it.func is not None:
                 ai += 1
             cols.append(res.columns[j])
@@ -516,7 +517,7 @@ class Database:
             bname = right.names[right.column_index(c.name)]
             return "right." + bname if _has_col(left, bname) else bname
         refs = [it.arg for it in q.items if it.arg is not None and it.arg.name != "*"]
-        refs += [c for c in [q.group_by] + [k[0] for k in q.order_by or []] if c is not None]
+        refs += [c for c in (q.group_by or []) + [k[0] for k in q.order_by or []] if c is not None]
         probe_cols, build_cols = [], []
         for c in refs:
             lst, nm = (probe_cols, left.names[left.column_index(c.name)]) if side(c) == "probe" else \
@@ -539,7 +540,7 @@ class Database:
                     items += [Item(None, ColName(None, n), None) for n in jt.names]
                 else:
                     items.append(Item(it.func, rename(it.arg), it.alias))
-            q2 = Query(items, "", None, group_by=rename(q.group_by),
+            q2 = Query(items, "", None, group_by=[rename(c) for c in q.group_by] if q.group_by else None,
                        order_by=[(rename(k[0]),) + tuple(k[1:]) for k in q.order_by] if q.order_by else None,
                        limit=q.limit, offset=q.offset)
             names2 = [_output_name(it, i) for i, it in enumerate(items)] if star else names

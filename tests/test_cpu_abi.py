@@ -65,6 +65,54 @@ def test_host_exact_sum_special_values():
     assert host_exact_sum(np.array([1.0, 2.0 ** -53, 2.0 ** -105])) == 1.0 + 2.0 ** -52
 
 
+def window_lsb(x) -> int | None:
+    """The exact-sum accumulator's LSB exponent L for a column: its smallest
+    mantissa-LSB exponent, or, when its exponents span more than 15 words
+    hold, 364 below its largest (values below 2^L are then rounded to it;
+    only aggregates with one output row take such a window)."""
+    e = [(int(b) >> 52 & 0x7FF) for b in np.asarray(x, np.float64).view(np.uint64)
+         if (int(b) >> 52 & 0x7FF) != 0x7FF and (int(b) & 0x7FFFFFFFFFFFFFFF)]
+    if not e:
+        return None
+    lsb = [max(v, 1) - 1075 for v in e]
+    emin, emax = min(lsb), max(lsb)
+    return emin if (emax - emin + 115) // 32 + 1 <= 15 else emax - 364
+
+
+def window_sum(x, L) -> float:
+    """The correctly rounded exact sum of the values, each first rounded to a
+    multiple of 2^L (ties to even): what the exact accumulator computes."""
+    from fractions import Fraction
+    q = Fraction(2) ** L
+    return float(sum((round(Fraction(float(v)) / q) * q for v in x), Fraction(0)))
+
+
+def test_host_exact_sum_wide_exponents():
+    # 1e-25 next to 1e25 spans ~166 binades: exact
+    x = np.array([1e25, 1e-25, -1e25])
+    assert host_exact_sum(x) == math.fsum(x) == 1e-25
+    # spans up to 364 binades stay exact
+    rng = np.random.default_rng(12)
+    for _ in range(50):
+        n = int(rng.integers(1, 60))
+        x = rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-50, 50, n)
+        assert host_exact_sum(x) == math.fsum(x)
+    # wider columns (one output row only): values below the window's LSB are
+    # rounded to it; the sum is the exact sum of the rounded values, within
+    # n * 2^(L-1) of the exact one
+    for _ in range(50):
+        n = int(rng.integers(1, 60))
+        x = rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-300, 300, n)
+        L = window_lsb(x)
+        got = host_exact_sum(x)
+        assert got == window_sum(x, L)
+        assert abs(got - math.fsum(x)) <= n * 2.0 ** (L - 1) + abs(math.fsum(x)) * 2.0 ** -52
+        y = np.abs(x)  # no cancellation: within 1e-12 relative of the exact sum
+        assert abs(host_exact_sum(y) - math.fsum(y)) <= 1e-12 * math.fsum(y)
+    # 1.0 next to a subnormal: the subnormal rounds away, ties to even
+    assert host_exact_sum(np.array([1.0, 5e-324])) == 1.0
+
+
 def test_generators_are_deterministic():
     K = dg.keys(1000, 40, seed_=5)
     assert len(set(K.tolist())) == 1000 and K.min() >= 0 and K.max() < 2 ** 40

@@ -284,6 +284,47 @@ def test_float_minmax_nan_semantics(ctx, ctx_gbp, ctx_gbh, ctx_part, path):
                 assert not g.validity[0][0] and not g.validity[1][0]
 
 
+@pytest.mark.parametrize("path", ["scalar", "dense", "partitioned", "hash", "join"])
+def test_float_sum_wide_exponents(ctx, ctx_gbp, ctx_gbh, ctx_part, path):
+    # Exponents spanning ~330 binades (1e-50 .. 1e50) fit the 15-word exact
+    # accumulator: every path equals the oracle's exact sums.  Spanning ~1660
+    # binades (1e-250 .. 1e250): aggregates with one output row (no GROUP BY,
+    # joins) round values to the window's LSB and stay within 1e-12 of the
+    # exact sum when it does not cancel; GROUP BY refuses the column, since a
+    # group far below the column's largest values would lose precision.
+    rng = np.random.default_rng(77)
+    n = 100_000
+    key = rng.integers(0, 30_000 if path in ("partitioned", "hash") else 50, n)
+    x = rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-50, 50, n)
+    w = np.abs(rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-250, 250, n))
+    cols = {"key": key, "x": x, "w": w}
+    aggs = [AggExpr(SUM, "x"), AggExpr(AVG, "x"), AggExpr(COUNT)]
+
+    def close(a, v):
+        e = math.fsum(v)
+        return abs(a - e) <= 1e-12 * abs(e)
+    if path == "join":
+        build = {"bk": np.arange(0, 50, dtype=np.int64), "bx": rng.random(50)}
+        pt, bt = _table(ctx_part, cols), _table(ctx_part, build)
+        for jt in (INNER, SEMI):
+            g = fa.execute_hash_join(ctx_part, pt, bt, "key", "bk", jt, aggs)
+            assert_batches_equal(g, ora.join(cols, build, "key", "bk", jt, aggs).exact, None)
+            g = fa.execute_hash_join(ctx_part, pt, bt, "key", "bk", jt, [AggExpr(SUM, "w")])
+            assert close(g.columns[0][0], w[key < 50])
+        return
+    c = {"scalar": ctx, "dense": ctx, "partitioned": ctx_gbp, "hash": ctx_gbh}[path]
+    t = _table(c, cols)
+    if path == "scalar":
+        assert_batches_equal(fa.execute_agg_only(c, t, aggs), ora.groupby(cols, None, None, aggs).exact, None)
+        assert close(fa.execute_agg_only(c, t, [AggExpr(SUM, "w")]).columns[0][0], w)
+        return
+    g = fa.GroupByExec(["key"], aggs).execute(c, t)
+    assert_batches_equal(g, ora.groupby(cols, None, "key", aggs).exact, "key")
+    with pytest.raises(fa.FalconError) as e:
+        fa.GroupByExec(["key"], [AggExpr(SUM, "w")]).execute(c, t)
+    assert e.value.code == 102 and "too wide" in str(e.value)
+
+
 def _with_specials(x, rng, pinf=0, ninf=0, nan=0):
     """x with `pinf` +inf, `ninf` -inf and `nan` NaN values at distinct random rows."""
     x = x.copy()

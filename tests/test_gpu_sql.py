@@ -43,6 +43,22 @@ def test_c1_groupby(db, ctx):
     assert_batches_equal(g, o.exact, "dept", o.reference)
 
 
+def test_groupby_several_keys(db, ctx):
+    # GROUP BY two keys (the reference's slow path): columns in SELECT order,
+    # the groups equal to the operator call's and to the oracle's restatement
+    from test_gpu_groupby_keys import _assert_same_groups
+    d, c1, _ = db
+    g = d.sql("SELECT COUNT(*) AS n, salary, dept, SUM(score) FROM t WHERE score > 90 GROUP BY dept, salary")
+    assert g.names == ["n", "salary", "dept", "col3"]
+    aggs = [AggExpr(COUNT, None, "n"), AggExpr(SUM, "score", "col3")]
+    o = ora.groupby_keys(c1, col("score") > 90, ["dept", "salary"], aggs)
+    # the SQL result in the oracle's column order: the keys, then the aggregates
+    idx = [2, 1, 0, 3]
+    g2 = fa.RecordBatch([g.names[i] for i in idx], [g.dtypes[i] for i in idx], [g.columns[i] for i in idx],
+                        [g.validity[i] for i in idx], [g.agg_funcs[i] for i in idx])
+    _assert_same_groups(g2, o, 2)
+
+
 def test_select_order_and_aliases(db, ctx):
     d, _, _ = db
     g = d.sql("SELECT COUNT(*) AS n, dept, MAX(salary) FROM t GROUP BY dept")

@@ -10,10 +10,15 @@ from falcondb_amd.sql import ColName, parse
 
 def test_groupby_statement():
     q = parse("SELECT dept, COUNT(*), SUM(score), AVG(salary) FROM t WHERE score > 50 GROUP BY dept")
-    assert q.table == "t" and q.group_by == ColName(None, "dept")
+    assert q.table == "t" and q.group_by == [ColName(None, "dept")]
     assert [(i.func, i.arg.name if i.arg else None) for i in q.items] == [
         (None, "dept"), (_abi.COUNT, None), (_abi.SUM, "score"), (_abi.AVG, "salary")]
     assert q.where == ("cmp", ColName(None, "score"), ">", 50, False)
+
+
+def test_groupby_several_keys():
+    q = parse("SELECT a, b, COUNT(*) FROM t GROUP BY a, t.b")
+    assert q.group_by == [ColName(None, "a"), ColName("t", "b")]
 
 
 def test_join_statement_with_aliases():
@@ -56,7 +61,6 @@ def test_order_by_several_keys_nulls_offset():
     ("SELECT a FROM t WHERE a ~ 3", 4),
     ("SELECT SUM(*) FROM t", 4),
     ("SELECT a FROM t LIMIT 2.5", 4),
-    ("SELECT a, b FROM t GROUP BY a, b", 102),
     ("SELECT a FROM t HAVING a > 1", 102),
     ("SELECT COUNT(*) FROM p JOIN b ON p.k < b.k", 102),
     ("SELECT COUNT(*) FROM p WHERE a = 1 OR EXISTS (SELECT 1 FROM b WHERE b.k = p.k)", 102),

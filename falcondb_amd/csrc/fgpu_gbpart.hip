@@ -153,6 +153,8 @@ struct GbpScan {
     ColRef in[NIN > 0 ? NIN : 1];
     OpList ops;
     int S, H, NB, log_p, Cw, RW, has_valid_word;
+    int pack, pk_kbits;  // packed row records (GbPartArgs)
+    int64_t pk_kmin, pk_vmin;
     unsigned long long *flags;
     int64_t *my;  // this workgroup's cells
 };
@@ -409,6 +411,13 @@ FGPU_DEV void gbp_row(const GbpScan<NPC, NIN> &g, const GbpLds &L, bool active, 
         return;
     }
     int64_t *rec = g.my + ((int64_t)p * g.Cw + pos) * g.RW;
+    if (NIN > 0 && g.pack) {  // key and input 0 in one word (no NULLs: no validity word)
+        rec[0] = (int64_t)(((uint64_t)key - (uint64_t)g.pk_kmin) |
+                           (((uint64_t)row.in[0] - (uint64_t)g.pk_vmin) << g.pk_kbits));
+#pragma unroll
+        for (int j = 1; j < NIN; ++j) rec[j] = row.in[j];
+        return;
+    }
     rec[0] = key;
 #pragma unroll
     for (int j = 0; j < NIN; ++j) rec[1 + j] = row.in[j];
@@ -435,6 +444,10 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     g.Cw = A.Cw;
     g.RW = A.RW;
     g.has_valid_word = A.has_valid_word;
+    g.pack = A.pack;
+    g.pk_kbits = A.pk_kbits;
+    g.pk_kmin = A.pk_kmin;
+    g.pk_vmin = A.pk_vmin;
     g.flags = A.flags;
     const int S = g.S, H = g.H, P = 1 << g.log_p;
     GbpLds L;
@@ -596,6 +609,18 @@ FGPU_DEV void gbp_agg_fetch(const GbPartArgs &A, const unsigned *pre, int G, con
         if (have[u]) {
             const int g = gbp_cell_of(pre, G, i);
             const int64_t *rec = part_cells + g * cell_stride + (int64_t)(i - pre[g]) * A.RW;
+            if (NIN > 0 && A.pack) {  // packed: key and input 0 in the first word
+                const uint64_t w = (uint64_t)rec[0];
+                key[u] = (int64_t)((uint64_t)A.pk_kmin + (w & ((1ull << A.pk_kbits) - 1)));
+                in[u].v0 = (int64_t)((uint64_t)A.pk_vmin + (w >> A.pk_kbits));
+                if (NIN > 1) in[u].v1 = rec[1];
+                if (NIN > 2) in[u].v2 = rec[2];
+                if (NIN > 3) in[u].v3 = rec[3];
+                if (NIN > 4) in[u].v4 = rec[4];
+                if (NIN > 5) in[u].v5 = rec[5];
+                in[u].valid = 0xffffffffu;
+                continue;
+            }
             key[u] = rec[0];
             if (NIN > 0) in[u].v0 = rec[1];
             if (NIN > 1) in[u].v1 = rec[2];

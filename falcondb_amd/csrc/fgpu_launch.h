@@ -98,6 +98,12 @@ struct GbPartArgs {
     unsigned long long *flags;  // [0] cell full, [1] pass-2 table full, [2] output full
     int null_group;           // the key column holds NULL or INT64_MIN rows: merge the NULL group
     int T2;                   // pass-2 LDS table slots (power of two)
+    // Packed row records: when pack is set, a row record's first word holds
+    // (key - pk_kmin) | ((input 0 - pk_vmin) << pk_kbits) (the key column
+    // has no NULL, input 0 is an integer column without NULLs, and the two
+    // ranges fit 64 bits), then inputs 1 .. n_in - 1; RW is one word shorter.
+    int pack, pk_kbits;
+    int64_t pk_kmin, pk_vmin;
     DOuts outs;
     int64_t out_cap;
     int64_t *out_key;

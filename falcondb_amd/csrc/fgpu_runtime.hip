@@ -282,6 +282,7 @@ struct Tuning {
     int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
     int join_slots_per_key = 4;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
     int join_compact = 1;          // compact record keys: 0 never, 1 for key-only records, 2 always
+    int gbp_pack = 1;              // partitioned GROUP BY: key and an integer input in one record word
 };
 
 struct fgpu_ctx {
@@ -1103,6 +1104,7 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_probe_wgs_per_cu") t.join_probe_wgs_per_cu = value;
     else if (n == "join_slots_per_key") t.join_slots_per_key = value;
     else if (n == "join_compact") t.join_compact = value;
+    else if (n == "gbp_pack") t.gbp_pack = value;
     else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
     if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
         t.join_block > 1024 || t.join_block % 64)
@@ -1641,7 +1643,29 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
         for (size_t col = 0; col < t->cols.size(); ++col)
             if (t->cols[col].d_values == b.dc.v[a.in_col[j]] && t->cols[col].null_count) nullable_inputs = true;
     A.has_valid_word = nullable_inputs ? 1 : 0;
-    A.RW = 1 + a.n_in + A.has_valid_word;
+    // rows with a NULL key, or the INT64_MIN key that shares the NULL group
+    A.null_group = key->null_count > 0 || key->imin == INT64_MIN ? 1 : 0;
+    // Packed row records: key and an integer input 0 in one word when their
+    // ranges (from the upload statistics) fit 64 bits together.
+    A.pack = 0;
+    if (tu.gbp_pack && a.n_in > 0 && !A.has_valid_word && !A.null_group && key->dtype != FGPU_UTF8 &&
+        key->imin <= key->imax) {
+        const DevCol *in0 = nullptr;
+        for (const DevCol &dc : t->cols)
+            if (dc.d_values == b.dc.v[a.in_col[0]]) in0 = &dc;
+        if (in0 && (in0->dtype == FGPU_INT64 || in0->dtype == FGPU_INT32) && in0->imin <= in0->imax) {
+            auto bits = [](uint64_t span) { return span ? 64 - __builtin_clzll(span) : 1; };
+            const int kb = bits((uint64_t)key->imax - (uint64_t)key->imin);
+            const int vb = bits((uint64_t)in0->imax - (uint64_t)in0->imin);
+            if (kb + vb <= 64) {
+                A.pack = 1;
+                A.pk_kbits = kb;
+                A.pk_kmin = key->imin;
+                A.pk_vmin = in0->imin;
+            }
+        }
+    }
+    A.RW = (A.pack ? 0 : 1) + a.n_in + A.has_valid_word;
     A.SR = (1 + S + A.RW - 1) / A.RW;
     A.G = c->num_cus * tu.gbp_scan_wgs_per_cu;
     // Pass-2 table: the most slots (a multiple of 64, at most the tuned cap)
@@ -1682,8 +1706,6 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     unsigned long long *d_flags;
     if ((rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_flags))) return rc;
     A.flags = d_flags;
-    // rows with a NULL key, or the INT64_MIN key that shares the NULL group
-    A.null_group = key->null_count > 0 || key->imin == INT64_MIN ? 1 : 0;
     OutBufs ob;
     if ((rc = out_bufs(c, out_cap, L.outs.n, ob))) return rc;
     A.out_key = ob.key;
@@ -1695,8 +1717,8 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     A.out_count = d_flags + 3;  // zeroed and read back with the flags
     char msg[256];
     const int fw = gbp_fast_width(A);
-    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d RW=%d tiles=%s", (long long)P, H, T2, A.G,
-             A.RW, fw == 4 ? "fast4" : fw == 8 ? "fast8" : "generic");
+    snprintf(msg, sizeof msg, "groupby:partitioned P=%lld H=%d T2=%d G=%d RW=%d%s tiles=%s", (long long)P, H, T2,
+             A.G, A.RW, A.pack ? " packed" : "", fw == 4 ? "fast4" : fw == 8 ? "fast8" : "generic");
     c->plan = msg;
     unsigned long long h[4] = {0, 0, 0, 0};  // the three flags, then the output row count
     mark(c, 0);

@@ -325,6 +325,45 @@ def test_float_sum_wide_exponents(ctx, ctx_gbp, ctx_gbh, ctx_part, path):
     assert e.value.code == 102 and "too wide" in str(e.value)
 
 
+@pytest.mark.parametrize("case", ["c2_like", "negative_ranges", "int32_input", "fits_64_bits", "no_fit"])
+def test_groupby_packed_records(ctx_gbp, case):
+    # The partitioned GROUP BY packs a row record's key and integer input 0
+    # into one word when their ranges fit 64 bits together: identical results
+    # with the packing on and off, and equal to the oracle.
+    rng = np.random.default_rng(sum(map(ord, case)))
+    n = 1_500_000
+    if case == "c2_like":
+        K = dg.keys(40_000, 40, seed_=dg.seed(2, 11))
+        cols = dg.c2_table(n, K)
+        key, v, expect = "key", "v1", True
+    else:
+        if case == "negative_ranges":
+            k = -(1 << 45) + rng.integers(0, 300_000, n) * 1009  # sparse: not a dense range
+            v1 = rng.integers(-(1 << 15), 1 << 15, n)
+        elif case == "int32_input":
+            k = rng.integers(0, 200_000, n) * 7919
+            v1 = rng.integers(-(1 << 31), (1 << 31) - 1, n).astype(np.int32)
+        elif case == "fits_64_bits":
+            k = rng.integers(0, 1 << 50, 200_000)[rng.integers(0, 200_000, n)]
+            v1 = rng.integers(-(1 << 13), 1 << 13, n)
+        else:
+            k = rng.integers(0, 1 << 55, 200_000)[rng.integers(0, 200_000, n)]
+            v1 = rng.integers(-(1 << 20), 1 << 20, n)
+        cols = {"key": k, "v1": v1, "v2": rng.normal(size=n), "f": rng.integers(0, 100, n).astype(np.int32)}
+        key, v, expect = "key", "v1", case != "no_fit"
+    aggs = [AggExpr(SUM, v), AggExpr(MIN, v), AggExpr(MAX, v), AggExpr(SUM, "v2"), AggExpr(COUNT)]
+    t = _table(ctx_gbp, cols)
+    res = []
+    for pack in (1, 0):
+        ctx_gbp.set_tuning("gbp_pack", pack)
+        res.append(fa.GroupByExec([key], aggs).execute(ctx_gbp, t, col("f") < 60))
+        plan = ctx_gbp.last_plan()
+        assert plan.startswith("groupby:partitioned") and ("packed" in plan) == (pack == 1 and expect), plan
+    ctx_gbp.set_tuning("gbp_pack", 1)
+    assert_batches_equal(res[0], res[1], key)
+    assert_batches_equal(res[0], ora.groupby(cols, col("f") < 60, key, aggs).exact, key)
+
+
 def _with_specials(x, rng, pinf=0, ninf=0, nan=0):
     """x with `pinf` +inf, `ninf` -inf and `nan` NaN values at distinct random rows."""
     x = x.copy()

@@ -421,6 +421,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--qps", default="100,300,1000,3000")
+    ap.add_argument("--gb-strategy", choices=["auto", "hash"], default="auto",
+                    help="C2 GROUP BY strategy: the planner's choice, or the single hash table "
+                         "(the radix-partitioned path switched off), to measure what partitioning buys")
     ap.add_argument("--ceiling", action="store_true",
                     help="only run the STREAM-like read probe (the scan's four columns, its tiling)")
     ap.add_argument("--calibrate", action="store_true",
@@ -452,6 +455,8 @@ def main():
     c2 = dg.c2_table(args.rows, K)
     log(f"[bench] generated C2 {args.rows} rows in {time.time() - t0:.1f}s")
     ctx = fa.Context(0)
+    if args.gb_strategy == "hash":
+        ctx.set_tuning("gb_partition", 0)
     t0 = time.time()
     table = fa.Table(ctx, c2)
     log(f"[bench] uploaded {table.device_bytes() / 1e9:.2f} GB in {time.time() - t0:.1f}s")

@@ -539,13 +539,17 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
                 continue;
             }
             if (KW == 4) {
-                uint32_t *rec = (uint32_t *)a.cells + ri * (2 * RW - 1);
+                uint32_t *rec = (uint32_t *)a.cells + ri * crec_words(RW, a.pw);
                 rec[0] = (uint32_t)stage[(size_t)i * RW];
+                if (RW == 2 && a.pw == 4) {
+                    rec[1] = (uint32_t)((uint64_t)stage[(size_t)i * RW + 1] - (uint64_t)a.pmin);
+                } else {
 #pragma unroll
-                for (int f = 1; f < RW; ++f) {
-                    const uint64_t x = (uint64_t)stage[(size_t)i * RW + f];
-                    rec[2 * f - 1] = (uint32_t)x;
-                    rec[2 * f] = (uint32_t)(x >> 32);
+                    for (int f = 1; f < RW; ++f) {
+                        const uint64_t x = (uint64_t)stage[(size_t)i * RW + f];
+                        rec[2 * f - 1] = (uint32_t)x;
+                        rec[2 * f] = (uint32_t)(x >> 32);
+                    }
                 }
             } else {
                 int64_t *rec = a.cells + ri * RW;
@@ -747,15 +751,17 @@ FGPU_DEV void fold_group(const EmitOps &e, JoinAcc &j, const bool (&live)[R], co
 }
 
 // The probe payloads of record ri (either record format).
-// A compact record's payload f (0-based) from its two uint32 words.
-FGPU_DEV int64_t crec_pay(const uint32_t *rec, int f) {
+// A compact record's payload f (0-based): two uint32 words, or (pw = 4) one
+// uint32 offset from pmin.
+FGPU_DEV int64_t crec_pay(const uint32_t *rec, int f, int pw, int64_t pmin) {
+    if (pw == 4) return (int64_t)((uint64_t)pmin + rec[1 + f]);
     return (int64_t)((uint64_t)rec[2 * f + 1] | (uint64_t)rec[2 * f + 2] << 32);
 }
 FGPU_DEV void rec_pay(const JoinPartArgs &a, int64_t ri, int64_t &f1, int64_t &f2) {
     if (a.kw == 4) {
-        const uint32_t *rec = (const uint32_t *)a.cells + ri * (2 * a.RW - 1);
-        f1 = a.RW > 1 ? crec_pay(rec, 0) : 0;
-        f2 = a.RW > 2 ? crec_pay(rec, 1) : 0;
+        const uint32_t *rec = (const uint32_t *)a.cells + ri * crec_words(a.RW, a.pw);
+        f1 = a.RW > 1 ? crec_pay(rec, 0, a.pw, a.pmin) : 0;
+        f2 = a.RW > 2 ? crec_pay(rec, 1, a.pw, a.pmin) : 0;
         return;
     }
     const int64_t *pp = a.cells + ri * a.RW + 1;
@@ -912,10 +918,10 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
             const unsigned i = ci0 + 64 * q + lane;
             const int64_t ri = i < cn ? cell + i : cell0;
             if (KW == 4) {
-                const uint32_t *rec = k32 + ri * (2 * RW - 1);
+                const uint32_t *rec = k32 + ri * crec_words(RW, a.pw);
                 kk[q] = (int64_t)__builtin_nontemporal_load(rec);
-                g1[q] = RW > 1 ? crec_pay(rec, 0) : 0;
-                g2[q] = RW > 2 ? crec_pay(rec, 1) : 0;
+                g1[q] = RW > 1 ? crec_pay(rec, 0, a.pw, a.pmin) : 0;
+                g2[q] = RW > 2 ? crec_pay(rec, 1, a.pw, a.pmin) : 0;
             } else if (RW == 1) {
                 kk[q] = __builtin_nontemporal_load(a.cells + ri);
                 g1[q] = g2[q] = 0;
@@ -1068,8 +1074,9 @@ __global__ __launch_bounds__(1024) void k_jp_probe(JoinPartArgs a) {
 // its bucket.
 typedef unsigned int JcU4 __attribute__((ext_vector_type(4)));
 
-template <int RW, int NPC, int U, bool K32>
+template <int RW, int NPC, int U, bool K32, int PW>
 __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
+    constexpr int RU = 1 + (RW - 1) * (PW == 4 ? 1 : 2);  // uint32 words per record
     extern __shared__ uint64_t jc_lds[];
     __shared__ int overflow;
     constexpr int TILE = JS_TPB * U;
@@ -1100,7 +1107,8 @@ __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
     const uint64_t kmask = (1ull << a.kbits) - 1, rmask = (1ull << a.rbits) - 1;
     const int64_t kmin = a.kmin, kmax = a.kmax;
     const int kbits = a.kbits, rbits = a.rbits;
-    char *recs = (char *)a.cells;  // packed records of (2 * RW - 1) uint32 words
+    char *recs = (char *)a.cells;  // packed records of RU uint32 words
+    const uint64_t pmin = (uint64_t)a.pmin;
     struct Raw {
         int64_t key[U];
         int64_t pay[U];
@@ -1165,9 +1173,11 @@ __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
                 continue;
             }
             // byte offsets stay below 2^32 (checked by the host)
-            uint32_t *rec = (uint32_t *)(recs + d * (uint32_t)(4 * (2 * RW - 1)));
+            uint32_t *rec = (uint32_t *)(recs + d * (uint32_t)(4 * RU));
             rec[0] = skey[i];
-            if (RW > 1) {
+            if (RW > 1 && PW == 4) {
+                rec[1] = (uint32_t)((uint64_t)spay[i] - pmin);
+            } else if (RW > 1) {
                 const uint64_t x = (uint64_t)spay[i];
                 rec[1] = (uint32_t)x;
                 rec[2] = (uint32_t)(x >> 32);
@@ -1186,8 +1196,9 @@ __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
     if (threadIdx.x == 0 && overflow) atomicExch(a.flags, 1ull);
 }
 
-template <int TYPE, int RW>
+template <int TYPE, int RW, int PW>
 __global__ __launch_bounds__(1024) void k_jc_probe(JoinPartArgs a) {
+    constexpr int RU = 1 + (RW - 1) * (PW == 4 ? 1 : 2);  // uint32 words per record
     extern __shared__ uint64_t jp_lds[];
     const int P = 1 << a.log_p, P2 = P + 2;
     const int T = a.T, G = a.G;
@@ -1218,7 +1229,8 @@ __global__ __launch_bounds__(1024) void k_jc_probe(JoinPartArgs a) {
     const EmitOps e = emit_ops(a);
     const uint32_t tmask = (uint32_t)T - 1;
     const unsigned pbase = (unsigned)((int64_t)p * G) * Cw;  // first record of the partition's cells
-    const char *recs = (const char *)a.cells;  // packed records of (2 * RW - 1) uint32 words
+    const char *recs = (const char *)a.cells;  // packed records of RU uint32 words
+    const uint64_t pmin = (uint64_t)a.pmin;
     // A wave's groups: cell w (w0 + wave, then every nw-th), records i0 .. i0 + 256
     // of it; lane l takes records i0 + 4l .. i0 + 4l + 3.
     struct Cur {
@@ -1247,10 +1259,18 @@ __global__ __launch_bounds__(1024) void k_jc_probe(JoinPartArgs a) {
         // four records: 16 bytes (keys only) or 48 bytes (keys and one payload),
         // 16-byte aligned as idx is a multiple of 4; byte offsets stay below
         // 2^32 (checked by the host)
-        const JcU4 *v = (const JcU4 *)(recs + idx * (uint32_t)(4 * (2 * RW - 1)));
+        const JcU4 *v = (const JcU4 *)(recs + idx * (uint32_t)(4 * RU));
         if (RW == 1) {
             g.k = __builtin_nontemporal_load(v);
             g.f[0] = g.f[1] = g.f[2] = g.f[3] = 0;
+        } else if (PW == 4) {
+            // records (key, payload offset) x 4: x = k0 p0 k1 p1, y = k2 p2 k3 p3
+            const JcU4 x = __builtin_nontemporal_load(v), y = __builtin_nontemporal_load(v + 1);
+            g.k = JcU4{x.x, x.z, y.x, y.z};
+            g.f[0] = (int64_t)(pmin + x.y);
+            g.f[1] = (int64_t)(pmin + x.w);
+            g.f[2] = (int64_t)(pmin + y.y);
+            g.f[3] = (int64_t)(pmin + y.w);
         } else {
             const JcU4 x = __builtin_nontemporal_load(v), y = __builtin_nontemporal_load(v + 1),
                        z = __builtin_nontemporal_load(v + 2);
@@ -1395,23 +1415,28 @@ static void join_fns(std::vector<const void *> &fns) {
     probe_fns<false, 2, KW>(fns);
     probe_fns<false, 3, KW>(fns);
 }
+template <bool K32, int PW>
+static void jc_fns2(std::vector<const void *> &v) {
+    v.push_back((const void *)k_jc_scatter<2, 0, 8, K32, PW>);
+    v.push_back((const void *)k_jc_scatter<2, 0, 4, K32, PW>);
+    v.push_back((const void *)k_jc_scatter<2, 1, 6, K32, PW>);
+    v.push_back((const void *)k_jc_scatter<2, 1, 4, K32, PW>);
+}
 template <bool K32>
 static void jc_fns(std::vector<const void *> &v) {
-    v.push_back((const void *)k_jc_scatter<1, 0, 8, K32>);
-    v.push_back((const void *)k_jc_scatter<1, 0, 4, K32>);
-    v.push_back((const void *)k_jc_scatter<1, 1, 8, K32>);
-    v.push_back((const void *)k_jc_scatter<1, 1, 4, K32>);
-    v.push_back((const void *)k_jc_scatter<2, 0, 8, K32>);
-    v.push_back((const void *)k_jc_scatter<2, 0, 4, K32>);
-    v.push_back((const void *)k_jc_scatter<2, 1, 6, K32>);
-    v.push_back((const void *)k_jc_scatter<2, 1, 4, K32>);
+    v.push_back((const void *)k_jc_scatter<1, 0, 8, K32, 8>);
+    v.push_back((const void *)k_jc_scatter<1, 0, 4, K32, 8>);
+    v.push_back((const void *)k_jc_scatter<1, 1, 8, K32, 8>);
+    v.push_back((const void *)k_jc_scatter<1, 1, 4, K32, 8>);
+    jc_fns2<K32, 4>(v);
+    jc_fns2<K32, 8>(v);
 }
-template <int RW>
+template <int RW, int PW>
 static void jc_probe_fns(std::vector<const void *> &v) {
-    v.push_back((const void *)k_jc_probe<FGPU_INNER, RW>);
-    v.push_back((const void *)k_jc_probe<FGPU_LEFT, RW>);
-    v.push_back((const void *)k_jc_probe<FGPU_SEMI, RW>);
-    v.push_back((const void *)k_jc_probe<FGPU_ANTI, RW>);
+    v.push_back((const void *)k_jc_probe<FGPU_INNER, RW, PW>);
+    v.push_back((const void *)k_jc_probe<FGPU_LEFT, RW, PW>);
+    v.push_back((const void *)k_jc_probe<FGPU_SEMI, RW, PW>);
+    v.push_back((const void *)k_jc_probe<FGPU_ANTI, RW, PW>);
 }
 static hipError_t join_allow_lds_once() {
     static std::once_flag once;
@@ -1422,8 +1447,9 @@ static hipError_t join_allow_lds_once() {
         join_fns<4>(fns);
         jc_fns<false>(fns);
         jc_fns<true>(fns);
-        jc_probe_fns<1>(fns);
-        jc_probe_fns<2>(fns);
+        jc_probe_fns<1, 8>(fns);
+        jc_probe_fns<2, 4>(fns);
+        jc_probe_fns<2, 8>(fns);
         for (const void *f : fns) {
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)JOIN_LDS_MAX);
             if (e != hipSuccess && err == hipSuccess) err = e;
@@ -1483,7 +1509,7 @@ static bool join_format_ok(const JoinPartArgs &a) {
 // and record byte offsets below 2^32.
 static bool jc_records_ok(const JoinPartArgs &a) {
     const int64_t recs = (int64_t)a.G * ((1 << a.log_p) + 2) * a.Cw;
-    return a.kw == 4 && a.RW <= 2 && recs * 4 * (2 * a.RW - 1) < (int64_t(1) << 32);
+    return a.kw == 4 && a.RW <= 2 && recs * 4 * crec_words(a.RW, a.pw) < (int64_t(1) << 32);
 }
 static size_t jc_scatter_lds(const JoinPartArgs &a, int u) {
     const size_t tile = (size_t)JS_TPB * u, P2 = ((size_t)1 << a.log_p) + 2;
@@ -1507,23 +1533,26 @@ static int jc_scatter_rows(const JoinPartArgs &a) {
         if (jc_scatter_lds(a, u) <= budget) return u;
     return 0;
 }
-template <int RW, int NPC, bool K32>
+template <int RW, int NPC, bool K32, int PW>
 static void launch_jc_scatter_u(hipStream_t s, const JoinPartArgs &a, int u) {
     const dim3 g((unsigned)a.G), b(JS_TPB);
     const size_t lds = jc_scatter_lds(a, u);
     constexpr int BIG = RW > 1 && NPC > 0 ? 6 : 8;
-    if (u == BIG) hipLaunchKernelGGL((k_jc_scatter<RW, NPC, BIG, K32>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_jc_scatter<RW, NPC, 4, K32>), g, b, lds, s, a);
+    if (u == BIG) hipLaunchKernelGGL((k_jc_scatter<RW, NPC, BIG, K32, PW>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_jc_scatter<RW, NPC, 4, K32, PW>), g, b, lds, s, a);
 }
 template <bool K32>
 static void launch_jc_scatter(hipStream_t s, const JoinPartArgs &a, int u) {
     const int npc = scatter_npc(a);
     if (a.RW == 1) {
-        if (npc == 0) launch_jc_scatter_u<1, 0, K32>(s, a, u);
-        else launch_jc_scatter_u<1, 1, K32>(s, a, u);
+        if (npc == 0) launch_jc_scatter_u<1, 0, K32, 8>(s, a, u);
+        else launch_jc_scatter_u<1, 1, K32, 8>(s, a, u);
+    } else if (a.pw == 4) {
+        if (npc == 0) launch_jc_scatter_u<2, 0, K32, 4>(s, a, u);
+        else launch_jc_scatter_u<2, 1, K32, 4>(s, a, u);
     } else {
-        if (npc == 0) launch_jc_scatter_u<2, 0, K32>(s, a, u);
-        else launch_jc_scatter_u<2, 1, K32>(s, a, u);
+        if (npc == 0) launch_jc_scatter_u<2, 0, K32, 8>(s, a, u);
+        else launch_jc_scatter_u<2, 1, K32, 8>(s, a, u);
     }
 }
 
@@ -1576,15 +1605,20 @@ hipError_t launch_join_part_probe(hipStream_t s, const JoinPartArgs &a, int n_ch
     const bool fast = a.pa.unique_keys && (!build_rows || a.build_pay_col >= 0);
     if (a.kw == 4 && fast && n_chunks == 0 && jc_records_ok(a)) {  // the lean compact pass 2
         const dim3 b(1024);
-        switch (a.pa.type * 4 + a.RW) {
-            case FGPU_INNER * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 1>), grid, b, lds, s, a); break;
-            case FGPU_LEFT * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 1>), grid, b, lds, s, a); break;
-            case FGPU_SEMI * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 1>), grid, b, lds, s, a); break;
-            case FGPU_ANTI * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 1>), grid, b, lds, s, a); break;
-            case FGPU_INNER * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 2>), grid, b, lds, s, a); break;
-            case FGPU_LEFT * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 2>), grid, b, lds, s, a); break;
-            case FGPU_SEMI * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 2>), grid, b, lds, s, a); break;
-            default: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 2>), grid, b, lds, s, a); break;
+        const int shape = a.RW == 1 ? 0 : a.pw == 4 ? 1 : 2;  // key only, 4-byte payload, 8-byte payload
+        switch (a.pa.type * 4 + shape) {
+            case FGPU_INNER * 4 + 0: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 1, 8>), grid, b, lds, s, a); break;
+            case FGPU_LEFT * 4 + 0: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 1, 8>), grid, b, lds, s, a); break;
+            case FGPU_SEMI * 4 + 0: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 1, 8>), grid, b, lds, s, a); break;
+            case FGPU_ANTI * 4 + 0: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 1, 8>), grid, b, lds, s, a); break;
+            case FGPU_INNER * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 2, 4>), grid, b, lds, s, a); break;
+            case FGPU_LEFT * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 2, 4>), grid, b, lds, s, a); break;
+            case FGPU_SEMI * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 2, 4>), grid, b, lds, s, a); break;
+            case FGPU_ANTI * 4 + 1: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 2, 4>), grid, b, lds, s, a); break;
+            case FGPU_INNER * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_INNER, 2, 8>), grid, b, lds, s, a); break;
+            case FGPU_LEFT * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_LEFT, 2, 8>), grid, b, lds, s, a); break;
+            case FGPU_SEMI * 4 + 2: hipLaunchKernelGGL((k_jc_probe<FGPU_SEMI, 2, 8>), grid, b, lds, s, a); break;
+            default: hipLaunchKernelGGL((k_jc_probe<FGPU_ANTI, 2, 8>), grid, b, lds, s, a); break;
         }
     } else if (a.kw == 4) {
         if (fast) launch_probe_fast<true, 4>(s, a, grid, lds);

@@ -269,16 +269,22 @@ struct JoinPartArgs {
     // in `cells`.  kw = 4 (compact; build keys span kbits <= 31 + log_p bits):
     // h = cmix(key - kmin) over kbits bits, the partition is h's top log_p
     // bits and the record key its low rbits bits; record i is the uint32 key
-    // and the RW - 1 payloads (each as two uint32 words, low first) packed at
-    // byte 4 * (2 * RW - 1) * i of `cells`.  Probe keys outside [kmin, kmax] go
-    // to the NULL-key side partition.
+    // and the RW - 1 payloads packed at byte 4 * crec_words(RW, pw) * i of
+    // `cells`: each payload as two uint32 words (low first), or, with pw = 4
+    // (one integer payload whose range fits 32 bits), as one uint32 offset
+    // from pmin.  Probe keys outside [kmin, kmax] go to the NULL-key side
+    // partition.
     int kw;
     int kbits, rbits;
     int64_t kmin, kmax;
+    int pw;
+    int64_t pmin;
     int lds_budget;           // pass-1 LDS bytes per workgroup (the G workgroups share the CUs' LDS)
 };
 // The empty slot of a compact sub-table in LDS (record keys are < 2^31).
 constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
+// uint32 words of a compact record with RW - 1 payloads of pw bytes.
+FGPU_HD int crec_words(int RW, int pw) { return 1 + (RW - 1) * (pw == 4 ? 1 : 2); }
 // Partitioned join pass 1: JS_TPB threads per workgroup, join_scatter_rows()
 // rows each per tile.
 constexpr int JS_TPB = 1024;

@@ -2372,10 +2372,10 @@ static int join_build_sub_tables(fgpu_ctx *c, JoinPartArgs &jp, const JoinBuildA
 }
 
 // Pass-1 record storage for n records in jp's format: hash64 records are RW
-// int64 words each, compact records 2 * RW - 1 uint32 words.
+// int64 words each, compact records crec_words(RW, pw) uint32 words.
 static int join_cells(fgpu_ctx *c, JoinPartArgs &jp, int64_t n) {
     void *p;
-    const size_t bytes = jp.kw == 4 ? (size_t)n * 4 * (2 * jp.RW - 1) : (size_t)n * jp.RW * 8;
+    const size_t bytes = jp.kw == 4 ? (size_t)n * 4 * crec_words(jp.RW, jp.pw) : (size_t)n * jp.RW * 8;
     if (int rc = buf(c, B_CELLS, bytes + 64, &p)) return rc;
     jp.cells = (int64_t *)p;
     return FGPU_OK;
@@ -2426,12 +2426,22 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     const int64_t nb = build->n_rows;
     // Record format: compact (4-byte keys, cmix over the build key range) when
     // the build side has a non-NULL key and its range spans at most
-    // 31 + log_p bits, else hash64 records.  By default only key-only records
-    // are compact: with a probe payload, 12-byte compact records over twice
-    // the partitions lose to 16-byte hash64 records over half of them.
+    // 31 + log_p bits, else hash64 records.  One integer probe payload whose
+    // range fits 32 bits rides as a 4-byte offset (8-byte records).  By
+    // default records are compact when they hold the key alone or such a
+    // payload: 12-byte compact records over twice the partitions lose to
+    // 16-byte hash64 records over half of them.
     int kbits = 0;
     const bool has_keys = nb > 0 && bkc.null_count < nb && bkc.imin <= bkc.imax;
-    const bool compact_ok = tu.join_compact >= 2 || (tu.join_compact == 1 && pcols_used.empty());
+    bool narrow_pay = false;
+    int64_t pay_min = 0;
+    if (pcols_used.size() == 1) {
+        const DevCol &pc = probe->cols[pcols_used[0]];
+        narrow_pay = (pc.dtype == FGPU_INT64 || pc.dtype == FGPU_INT32) && pc.imin <= pc.imax &&
+                     (uint64_t)pc.imax - (uint64_t)pc.imin <= 0xFFFFFFFFull;
+        pay_min = pc.imin;
+    }
+    const bool compact_ok = tu.join_compact >= 2 || (tu.join_compact == 1 && (pcols_used.empty() || narrow_pay));
     if (compact_ok && has_keys) {
         const uint64_t range = (uint64_t)bkc.imax - (uint64_t)bkc.imin;
         kbits = range ? 64 - __builtin_clzll(range) : 1;
@@ -2449,6 +2459,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     jp.log_p = 0;
     while ((int64_t(1) << jp.log_p) < P) ++jp.log_p;
     jp.kw = 8;
+    jp.pw = 8;
     if (kbits > 0) {
         // compact records keep kbits - log_p <= 31 bits: more partitions if needed
         const int lp = std::max(jp.log_p, kbits - 31);
@@ -2460,6 +2471,8 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
             jp.rbits = kbits - lp;
             jp.kmin = bkc.imin;
             jp.kmax = bkc.imax;
+            jp.pw = narrow_pay ? 4 : 8;
+            jp.pmin = pay_min;
         }
     }
     if (P > 4096) return -1;
@@ -2562,7 +2575,7 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     HIPCHK(launch_state_init(c->stream, nullptr, 0, a.state, 1, L.ops));
     jp.xoff = nullptr;
     jp.chunks = nullptr;
-    c->plan += jp.kw == 4 ? " records=compact" : " records=hash64";
+    c->plan += jp.kw == 4 ? (jp.RW == 2 && jp.pw == 4 ? " records=compact pay4" : " records=compact") : " records=hash64";
     mark(c, 1);
     HIPCHK(launch_join_part_scatter(c->stream, jp, 0));
     HIPCHK(launch_join_part_probe(c->stream, jp, 0));
@@ -2625,8 +2638,32 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     Binder pb(probe), bb(build);
     JoinProbeArgs a;
     AggLayout L;
+    struct OwnIndex {  // an index this call builds, freed on every return
+        fgpu_join_index *p = nullptr;
+        ~OwnIndex() { fgpu_join_index_free(p); }
+    } own_index;
+    fgpu_join_index *&own = own_index.p;
+    // An exact integer SUM is sized by the join's output rows.  Without an
+    // index the build keys may repeat, bounding INNER / LEFT output by
+    // probe x build rows; past 2^32 rows the sum could not be laid out.  An
+    // index built here tells whether the keys are unique (then the output
+    // has at most one row per probe row), and the call then uses it.
+    if (!idx && (spec->type == FGPU_INNER || spec->type == FGPU_LEFT) &&
+        (__int128)probe->n_rows * std::max<int64_t>(build->n_rows, 1) >= ((__int128)1 << 32)) {
+        bool int_sum = false;
+        for (int i = 0; i < spec->n_aggs; ++i) {
+            const fgpu_agg &g = spec->aggs[i];
+            const fgpu_table *tt = g.side == 1 ? build : probe;
+            if ((g.func == FGPU_SUM || g.func == FGPU_AVG) && g.col >= 0 && g.col < (int)tt->cols.size() &&
+                tt->cols[g.col].dtype != FGPU_FLOAT64)
+                int_sum = true;
+        }
+        if (int_sum) {
+            if (int rc = join_build_impl(c, build, spec->build_key, &own)) return rc;
+            idx = own;
+        }
+    }
     if (int rc = join_prepare(c, probe, pred, build, idx, spec, pb, bb, a, L)) return rc;
-    fgpu_join_index *own = nullptr;
     mark(c, 0);
     uint64_t *d_state;
     int rc = buf(c, B_STATE, (size_t)a.S * 8, (void **)&d_state);
@@ -2653,16 +2690,10 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
         if (e == hipSuccess && probe->n_rows) e = launch_join_probe_agg(c->stream, a, grid, c->tune.join_block);
         mark(c, 2);
     }
-    if (e != hipSuccess) {
-        fgpu_join_index_free(own);
-        return set_err(FGPU_E_DEVICE, "join probe: %s", hipGetErrorString(e));
-    }
+    if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "join probe: %s", hipGetErrorString(e));
     OutBufs ob;
     rc = out_bufs(c, 1, L.outs.n, ob);
-    if (rc) {
-        fgpu_join_index_free(own);
-        return rc;
-    }
+    if (rc) return rc;
     EmitArgs em{};
     em.outs = L.outs;
     em.state = d_state;
@@ -2678,16 +2709,12 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     e = launch_fill_u64(c->stream, (uint64_t *)ob.count, 1, 0);
     if (e == hipSuccess) e = launch_emit(c->stream, em);
     mark(c, 3);
-    if (e != hipSuccess) {
-        fgpu_join_index_free(own);
-        return set_err(FGPU_E_DEVICE, "join emit: %s", hipGetErrorString(e));
-    }
+    if (e != hipSuccess) return set_err(FGPU_E_DEVICE, "join emit: %s", hipGetErrorString(e));
     rc = collect(c, L, 1, nullptr, 0, "", nullptr, nullptr, ob.val, ob.valid, ob.count, out);
     int64_t bytes = 0;
     for (int s = 0; s < pb.dc.n; ++s) bytes += probe->n_rows * (pb.dc.dtype[s] == FGPU_INT64 || pb.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
     for (int s = 0; s < bb.dc.n; ++s) bytes += build->n_rows * (bb.dc.dtype[s] == FGPU_INT64 || bb.dc.dtype[s] == FGPU_FLOAT64 ? 8 : 4);
     if (!rc) finish_timing(c, bytes, own != nullptr);
-    fgpu_join_index_free(own);
     return rc;
 }
 

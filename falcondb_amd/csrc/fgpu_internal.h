@@ -37,6 +37,22 @@ constexpr int MAX_OPS = 16;     // state operations per group
 constexpr int MAX_FX_WORDS = 15;  // fixed-point words of one exact float sum (a packed op's W field)
 constexpr int BLOCK = 256;      // threads per workgroup for the scan kernels
 
+// Loads and stores through the global address space.  A pointer read from a
+// descriptor in memory is generic to the compiler, and a generic (flat)
+// access counts against both the vector-memory and the LDS wait counters, so
+// every LDS wait after it also waits for HBM.  The pointee must be device
+// memory.  Vector types are clang vectors (a class type would be copied
+// through a generic reference).
+typedef long long ll2 __attribute__((ext_vector_type(2)));
+template <class T>
+FGPU_DEV T gld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+FGPU_DEV void gst(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // ─── hashing ────────────────────────────────────────────────────────────────
 // MurmurHash3 64-bit finaliser (a bijection on 64-bit values).
 FGPU_HD uint64_t hash64(uint64_t h) {
@@ -223,15 +239,15 @@ struct DCols {
 // Utf8 code zero-extended, double bits).
 FGPU_DEV int64_t load_raw(const DCols &c, int k, int64_t r) {
     switch (c.dtype[k]) {
-        case FGPU_INT32: return (int64_t)((const int32_t *)c.v[k])[r];
-        case FGPU_UTF8: return (int64_t)((const uint32_t *)c.v[k])[r];
-        default: return ((const int64_t *)c.v[k])[r];
+        case FGPU_INT32: return (int64_t)gld((const int32_t *)c.v[k] + r);
+        case FGPU_UTF8: return (int64_t)gld((const uint32_t *)c.v[k] + r);
+        default: return gld((const int64_t *)c.v[k] + r);
     }
 }
 FGPU_DEV bool load_valid(const DCols &c, int k, int64_t r) {
-    if (c.dtype[k] == FGPU_UTF8) return ((const uint32_t *)c.v[k])[r] != NULL_CODE;
+    if (c.dtype[k] == FGPU_UTF8) return gld((const uint32_t *)c.v[k] + r) != NULL_CODE;
     const uint64_t *vb = c.valid[k];
-    return vb ? ((vb[r >> 6] >> (r & 63)) & 1) : true;
+    return vb ? ((gld(vb + (r >> 6)) >> (r & 63)) & 1) : true;
 }
 
 // Comparison classes resolved on the host from (column type, literal type),
@@ -379,12 +395,12 @@ FGPU_DEV bool eval_pred(const DPred &p, const DCols &c, int64_t r) {
 // Loads the value (64-bit pattern) and validity of column k at row r.
 FGPU_DEV void load_cell(const DCols &c, int k, int64_t r, int64_t &v, bool &ok) {
     switch (c.dtype[k]) {
-        case FGPU_INT32: v = (int64_t)((const int32_t *)c.v[k])[r]; break;
-        case FGPU_UTF8: v = (int64_t)((const uint32_t *)c.v[k])[r]; break;
-        default: v = ((const int64_t *)c.v[k])[r]; break;
+        case FGPU_INT32: v = (int64_t)gld((const int32_t *)c.v[k] + r); break;
+        case FGPU_UTF8: v = (int64_t)gld((const uint32_t *)c.v[k] + r); break;
+        default: v = gld((const int64_t *)c.v[k] + r); break;
     }
     const uint64_t *vb = c.valid[k];
-    ok = vb ? ((vb[r >> 6] >> (r & 63)) & 1) : (c.dtype[k] != FGPU_UTF8 || (uint32_t)v != NULL_CODE);
+    ok = vb ? ((gld(vb + (r >> 6)) >> (r & 63)) & 1) : (c.dtype[k] != FGPU_UTF8 || (uint32_t)v != NULL_CODE);
 }
 
 // ─── two-phase column reads ─────────────────────────────────────────────────
@@ -404,16 +420,16 @@ struct RawCell {
     uint32_t vword;
 };
 FGPU_DEV void cell_issue(const ColRef &c, int64_t r, RawCell &o) {
-    if (c.dtype == FGPU_INT32 || c.dtype == FGPU_UTF8) o.bits = (uint64_t)*(const uint32_t *)(c.base + r * 4);
-    else o.bits = *(const uint64_t *)(c.base + r * 8);
-    o.vword = c.valid ? c.valid[r >> 5] : 0xffffffffu;
+    if (c.dtype == FGPU_INT32 || c.dtype == FGPU_UTF8) o.bits = (uint64_t)gld((const uint32_t *)(c.base + r * 4));
+    else o.bits = gld((const uint64_t *)(c.base + r * 8));
+    o.vword = c.valid ? gld(c.valid + (r >> 5)) : 0xffffffffu;
 }
 // Issue for row tile0 + idx where tile0 is a multiple of 32: the column and
 // validity bases are uniform (SGPR) and the lane offset is 32-bit.
 FGPU_DEV void cell_issue_tile(const ColRef &c, int64_t tile0, uint32_t idx, RawCell &o) {
-    if (c.dtype == FGPU_INT32 || c.dtype == FGPU_UTF8) o.bits = (uint64_t)((const uint32_t *)c.base + tile0)[idx];
-    else o.bits = ((const uint64_t *)c.base + tile0)[idx];
-    o.vword = c.valid ? (c.valid + (tile0 >> 5))[idx >> 5] : 0xffffffffu;
+    if (c.dtype == FGPU_INT32 || c.dtype == FGPU_UTF8) o.bits = (uint64_t)gld((const uint32_t *)c.base + tile0 + idx);
+    else o.bits = gld((const uint64_t *)c.base + tile0 + idx);
+    o.vword = c.valid ? gld(c.valid + (tile0 >> 5) + (idx >> 5)) : 0xffffffffu;
 }
 FGPU_DEV void cell_finish(const ColRef &c, int64_t r, const RawCell &o, int64_t &v, bool &ok) {
     v = c.dtype == FGPU_INT32 ? (int64_t)(int32_t)(uint32_t)o.bits : (int64_t)o.bits;
@@ -431,25 +447,25 @@ struct RawCell3 {
 };
 FGPU_DEV void cell_issue3(const ColRef &c, int64_t r, RawCell3 &o) {
     if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) {
-        const uint64_t x = ((const uint64_t *)c.base)[r];
+        const uint64_t x = gld((const uint64_t *)c.base + r);
         o.lo = (uint32_t)x;
         o.hi = (uint32_t)(x >> 32);
     } else {
-        o.lo = o.hi = ((const uint32_t *)c.base)[r];
+        o.lo = o.hi = gld((const uint32_t *)c.base + r);
     }
-    o.vw = c.valid ? c.valid[r >> 5] : 0xffffffffu;
+    o.vw = c.valid ? gld(c.valid + (r >> 5)) : 0xffffffffu;
 }
 // Row tile0 + idx, tile0 a multiple of 32: the bases stay uniform (SGPR) and
 // the lane offsets 32-bit.
 FGPU_DEV void cell_issue3_tile(const ColRef &c, int64_t tile0, uint32_t idx, RawCell3 &o) {
     if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) {
-        const uint64_t x = ((const uint64_t *)c.base + tile0)[idx];
+        const uint64_t x = gld((const uint64_t *)c.base + tile0 + idx);
         o.lo = (uint32_t)x;
         o.hi = (uint32_t)(x >> 32);
     } else {
-        o.lo = o.hi = ((const uint32_t *)c.base + tile0)[idx];
+        o.lo = o.hi = gld((const uint32_t *)c.base + tile0 + idx);
     }
-    o.vw = c.valid ? (c.valid + (tile0 >> 5))[idx >> 5] : 0xffffffffu;
+    o.vw = c.valid ? gld(c.valid + (tile0 >> 5) + (idx >> 5)) : 0xffffffffu;
 }
 FGPU_DEV void cell_finish3(const ColRef &c, int64_t r, const RawCell3 &o, int64_t &v, bool &ok) {
     const bool wide = c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64;

@@ -324,6 +324,8 @@ struct BatchQuery {
     int shift;                // BQ_PART: partition = slot >> shift (2^shift slots each)
     int P;                    // BQ_PART partitions
     int S, RW, Cw, has_valid_word;
+    int pack, pk_kbits;       // BQ_PART records: slot | (input 0 - pk_vmin) << pk_kbits in word 0
+    int64_t pk_vmin;
     int64_t unit0, n_units;   // this query's pass-1 units
     int64_t *cells;           // BQ_PART: n_units * P * Cw * RW words
     unsigned int *cell_cnt;   // BQ_PART: n_units * P record counts
@@ -364,6 +366,7 @@ hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArg
 hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int grid, int block,
                               size_t lds);
 hipError_t batch_set_lds(size_t lds1, size_t lds2);
+hipError_t batch_occupancy(int npc, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2);
 
 // ─── ORDER BY with any LIMIT and any number of keys (fgpu_sort.hip) ────────
 hipError_t launch_mask_rows(hipStream_t s, const uint64_t *words, const int64_t *offs, int64_t n_words,

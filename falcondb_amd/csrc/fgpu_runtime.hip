@@ -282,7 +282,7 @@ struct Tuning {
     int join_probe_wgs_per_cu = 4; // pass-2 workgroups per CU
     int join_slots_per_key = 4;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
     int join_compact = 1;          // compact record keys: 0 never, 1 for key-only records, 2 always
-    int gbp_pack = 1;              // partitioned GROUP BY: key and an integer input in one record word
+    int gbp_pack = 1;              // partitioned and batched GROUP BY: key and an integer input in one record word
 };
 
 struct fgpu_ctx {
@@ -3137,7 +3137,7 @@ struct BatchCfg {
     int block = 1024;
     int64_t unit_rows = int64_t(1) << 18;
     int slots = 2048;
-    int wgs_per_cu = 1;
+    int wgs_per_cu = 8;  // cap on the resident workgroups per CU of each pass
 };
 
 struct BatchQ {
@@ -3154,6 +3154,8 @@ struct BatchQ {
     const DevCol *key = nullptr;
     GbArgs g{};
     int has_valid_word = 0;
+    int pack = 0, pk_kbits = 0;  // BQ_PART packed records (BatchQuery::pack)
+    int64_t pk_vmin = 0;
     int64_t Cw_need = 0;  // cell size seen by an overflowing run (0 = from the row count)
     BatchQuery q{};
 };
@@ -3170,6 +3172,7 @@ struct fgpu_batch {
     unsigned long long *d_work = nullptr, *d_flags = nullptr, *d_counts = nullptr;
     int64_t n_units = 0, n_units2 = 0;
     size_t lds1 = 0, lds2 = 0;
+    int per_cu1 = 1, per_cu2 = 1;  // persistent workgroups per CU of pass 1 / pass 2
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -3224,7 +3227,10 @@ int batch_plan(fgpu_batch *b) {
         const int S = Q.L.ops.S;
         q.S = S;
         q.has_valid_word = Q.has_valid_word;
-        q.RW = 1 + Q.g.n_in + Q.has_valid_word;
+        q.pack = Q.pack;
+        q.pk_kbits = Q.pk_kbits;
+        q.pk_vmin = Q.pk_vmin;
+        q.RW = (Q.pack ? 0 : 1) + Q.g.n_in + Q.has_valid_word;
         const DevCol &key = *Q.key;
         q.key_min = key.imin <= key.imax ? key.imin : 0;
         q.range = key.imin <= key.imax ? key.imax - key.imin + 1 : 1;
@@ -3233,13 +3239,14 @@ int batch_plan(fgpu_batch *b) {
         const int64_t per_unit = (n_rows + q.n_units - 1) / q.n_units;
         // pass-2 slots: a multiple-of-two table that fits next to the G + 1 prefix
         int64_t slots = cfg.slots;
-        while (slots > 64 && (size_t)slots * S * 8 + (size_t)(q.n_units + 1) * 4 > lim) slots >>= 1;
+        // (each slot also has a 4-byte entry in the emit list)
+        while (slots > 64 && (size_t)slots * (S * 8 + 4) + (size_t)(q.n_units + 1) * 4 > lim) slots >>= 1;
         if ((size_t)(S + q.range * S) * 8 <= std::min<size_t>(lim, 64 * 1024)) {
             q.mode = BQ_DENSE;
             b->lds1 = std::max(b->lds1, (size_t)(S + q.range * S) * 8);
             for (int64_t k0 = 0; k0 < q.range; k0 += slots)
                 units2.push_back(MergeUnit{(int32_t)qi, -1, k0, std::min(k0 + slots, q.range), k0 == 0, 0});
-            b->lds2 = std::max(b->lds2, (size_t)std::min(slots, q.range) * S * 8);
+            b->lds2 = std::max(b->lds2, (size_t)std::min(slots, q.range) * (S * 8 + 4));
         } else {
             q.mode = BQ_PART;
             int shift = 0;
@@ -3257,7 +3264,7 @@ int batch_plan(fgpu_batch *b) {
                 const int64_t k0 = (int64_t)p << shift;
                 units2.push_back(MergeUnit{(int32_t)qi, p, k0, std::min(k0 + (int64_t(1) << shift), q.range), p == 0, 0});
             }
-            b->lds2 = std::max(b->lds2, ((size_t)1 << shift) * S * 8 + (size_t)(q.n_units + 1) * 4);
+            b->lds2 = std::max(b->lds2, ((size_t)1 << shift) * (S * 8 + 4) + (size_t)(q.n_units + 1) * 4);
         }
         q.unit0 = (int64_t)units.size();
         for (int64_t u = 0; u < q.n_units; ++u)
@@ -3340,18 +3347,27 @@ int batch_plan(fgpu_batch *b) {
     A.work = b->d_work;
     A.flags = b->d_flags;
     A.lds_slots = cfg.slots;
-    const int grid = c->num_cus * cfg.wgs_per_cu;
+    // persistent grids: as many workgroups per CU as stay resident together
+    int per_cu1 = 1, per_cu2 = 1;
+    HIPCHK(batch_occupancy(b->npc, cfg.block, b->lds1, b->lds2, per_cu1, per_cu2));
+    b->per_cu1 = std::min(per_cu1, cfg.wgs_per_cu);
+    b->per_cu2 = std::min(per_cu2, cfg.wgs_per_cu);
+    const int grid1 = c->num_cus * b->per_cu1, grid2 = c->num_cus * b->per_cu2;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_fill_u64(c->stream, (uint64_t *)b->d_work, (int64_t)(4 + b->qs.size()), 0);
-    if (e == hipSuccess && b->n_units) e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, grid, cfg.block, b->lds1);
-    if (e == hipSuccess && b->n_units2) e = launch_batch_merge(c->stream, b->d_qs, A, grid, cfg.block, b->lds2);
+    if (e == hipSuccess && b->n_units)
+        e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, grid1, cfg.block, b->lds1);
+    if (e == hipSuccess && b->n_units2) e = launch_batch_merge(c->stream, b->d_qs, A, grid2, cfg.block, b->lds2);
     const hipError_t e2 = hipStreamEndCapture(c->stream, &b->graph);
     if (e != hipSuccess || e2 != hipSuccess)
         return set_err(FGPU_E_DEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    int n_packed = 0;
+    for (const BatchQ &Q : b->qs) n_packed += !Q.fallback && Q.q.mode == BQ_PART && Q.q.pack;
     char msg[200];
-    snprintf(msg, sizeof msg, "block=%d unit_rows=%lld slots=%d units=%lld/%lld plan_bytes=%zu", cfg.block,
-             (long long)cfg.unit_rows, cfg.slots, (long long)b->n_units, (long long)b->n_units2, total);
+    snprintf(msg, sizeof msg, "block=%d unit_rows=%lld slots=%d units=%lld/%lld wgs_per_cu=%d/%d plan_bytes=%zu packed=%d",
+             cfg.block, (long long)cfg.unit_rows, cfg.slots, (long long)b->n_units, (long long)b->n_units2, b->per_cu1,
+             b->per_cu2, total, n_packed);
     b->info = msg;
     return FGPU_OK;
 }
@@ -3551,6 +3567,23 @@ int fgpu_batch_prepare(fgpu_ctx *c, const fgpu_batch_query *queries, int n, fgpu
         for (int j = 0; j < a.n_in; ++j)
             for (const DevCol &dc : t->cols)
                 if (dc.d_values == a.cols.v[a.in_col[j]] && dc.null_count) Q.has_valid_word = 1;
+        // Packed records: the key slot and an integer input 0 in one word when
+        // their ranges (from the upload statistics) fit 64 bits together.
+        if (c->tune.gbp_pack && a.n_in > 0 && !Q.has_valid_word) {
+            const DevCol *in0 = nullptr;
+            for (const DevCol &dc : t->cols)
+                if (dc.d_values == a.cols.v[a.in_col[0]]) in0 = &dc;
+            if (in0 && (in0->dtype == FGPU_INT64 || in0->dtype == FGPU_INT32) && in0->imin <= in0->imax) {
+                auto bits = [](uint64_t span) { return span ? 64 - __builtin_clzll(span) : 1; };
+                const int kb = bits((uint64_t)range - 1);
+                const int vb = bits((uint64_t)in0->imax - (uint64_t)in0->imin);
+                if (kb + vb <= 64) {
+                    Q.pack = 1;
+                    Q.pk_kbits = kb;
+                    Q.pk_vmin = in0->imin;
+                }
+            }
+        }
     }
     if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -3575,10 +3608,10 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
     if (int rc = set_device(c->device)) return rc;
     std::vector<BatchCfg> cands;
     // slots: 4096 is capped at the largest table the state width lets fit
-    // (one pass-2 workgroup per CU); 512 and 256 let several share a CU
+    // (one pass-2 workgroup per CU); smaller tables let several share a CU
     for (int block : {1024, 512})
         for (int64_t ur : {int64_t(1) << 18, int64_t(1) << 16, int64_t(1) << 20})
-            for (int slots : {4096, 512, 256}) {
+            for (int slots : {4096, 1024, 512, 256}) {
                 BatchCfg k;
                 k.block = block;
                 k.unit_rows = ur;

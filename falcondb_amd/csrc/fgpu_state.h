@@ -151,9 +151,9 @@ struct GbRow {
     uint32_t vw[MAX_IN];
 };
 FGPU_DEV void gb_cell_issue(const ColRef &c, int64_t r, int64_t &v, uint32_t &vw) {
-    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) v = ((const int64_t *)c.base)[r];
-    else v = (int64_t)((const uint32_t *)c.base)[r];
-    vw = c.valid ? c.valid[r >> 5] : 0xffffffffu;
+    if (c.dtype == FGPU_INT64 || c.dtype == FGPU_FLOAT64) v = gld((const int64_t *)c.base + r);
+    else v = (int64_t)gld((const uint32_t *)c.base + r);
+    vw = c.valid ? gld(c.valid + (r >> 5)) : 0xffffffffu;
 }
 // load_raw / load_valid of an issued cell
 FGPU_DEV bool gb_cell_finish(const ColRef &c, int64_t r, int64_t &v, uint32_t vw) {
@@ -211,15 +211,18 @@ FGPU_DEV bool gb_finish(const GbArgs &a, const GbCols &g, int64_t r, GbRow &x, i
 // Fills n_rows group states of S words with the ops' identity values.
 // Consecutive threads write consecutive words; each thread steps its word
 // index instead of dividing per word (a 64-bit modulo is a long sequence).
-FGPU_DEV void fill_ident(uint64_t *dst, int64_t n_rows, int S, const DOps &ops) {
+FGPU_DEV void fill_ident(uint64_t *dst, int64_t n_rows, int S, const uint64_t *ident) {
     const int64_t total = n_rows * S;
     const int step = (int)(blockDim.x % (unsigned)S);
     int w = (int)(threadIdx.x % (unsigned)S);
     for (int64_t i = threadIdx.x; i < total; i += blockDim.x) {
-        dst[i] = ops.ident[w];
+        dst[i] = ident[w];
         w += step;
         if (w >= S) w -= S;
     }
+}
+FGPU_DEV void fill_ident(uint64_t *dst, int64_t n_rows, int S, const DOps &ops) {
+    fill_ident(dst, n_rows, S, ops.ident);
 }
 
 // Exclusive prefix of v over the workgroup and the workgroup total.  Every
@@ -452,8 +455,8 @@ FGPU_DEV void emit_row(const DOuts &outs, const uint64_t *st, int64_t o, int64_t
                 out = w == 0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w);
                 break;
         }
-        out_val[k][o] = out;
-        out_valid[k][o] = valid;
+        gst(out_val[k] + o, out);
+        gst(out_valid[k] + o, valid);
     }
 }
 

@@ -104,3 +104,55 @@ def test_batch_of_c4_shapes_small(ctx):
     for (t, key, aggs, filt, cols), got in zip(qs, res):
         assert_batches_equal(got, ora.groupby(cols, filt, key, aggs).exact, key)
     batch.free()
+
+
+def test_batch_packed_records_match_unpacked(ctx):
+    """BQ_PART records with the key slot and an integer input 0 in one word
+    (ranges from the upload statistics fit 64 bits), against unpacked records
+    and the oracle: Int64 and Int32 inputs, a negative input minimum, and a
+    query whose ranges are too wide to pack."""
+    rng = np.random.default_rng(41)
+    n = 400_000
+    shapes = [
+        dg.c4_batch(n, 300_000, dg.seed(4, 300)),
+        {"key": rng.integers(-50_000, 150_000, n).astype(np.int32),
+         "v32": rng.integers(-(1 << 31), (1 << 31) - 1, n).astype(np.int32), "score": rng.random(n) * 100},
+        # 18 key bits + 61 value bits: left unpacked
+        {"key": rng.integers(0, 250_000, n), "val": rng.integers(-(1 << 60), 1 << 60, n),
+         "score": rng.random(n)},
+    ]
+    aggs = [C4_AGGS, [AggExpr(COUNT), AggExpr(SUM, "v32"), AggExpr(MIN, "v32"), AggExpr(MAX, "v32"),
+                      AggExpr(AVG, "score")], [AggExpr(COUNT), AggExpr(MIN, "val"), AggExpr(MAX, "val")]]
+    filters = [col("score") < 40.0, col("score") > 10.0, None]
+    tabs = [_table(ctx, c) for c in shapes]
+    results = {}
+    try:
+        for pack in (1, 0):
+            ctx.set_tuning("gbp_pack", pack)
+            batch = fa.BatchGroupBy(ctx, [(t, "key", a, f) for t, a, f in zip(tabs, aggs, filters)])
+            assert f"packed={2 if pack else 0}" in batch.info(), batch.info()
+            results[pack] = batch.run()
+            batch.free()
+    finally:
+        ctx.set_tuning("gbp_pack", 1)
+    for cols, a, f, p1, p0 in zip(shapes, aggs, filters, results[1], results[0]):
+        assert_batches_equal(p1, p0, "key")
+        assert_batches_equal(p1, ora.groupby(cols, f, "key", a).exact, "key")
+
+
+def test_batch_six_nullable_inputs(ctx):
+    """The widest BQ_PART record: key slot, six inputs (some with NULLs) and
+    the validity word."""
+    rng = np.random.default_rng(43)
+    n = 300_000
+    cols = {"key": rng.integers(0, 120_000, n)}
+    aggs = [AggExpr(COUNT)]
+    for j in range(6):
+        vals = rng.integers(-10**9, 10**9, n) if j % 2 == 0 else rng.normal(size=n) * 1e3
+        cols[f"c{j}"] = NullableColumn(vals, rng.random(n) > 0.15 * (j % 3)) if j % 3 else vals
+        aggs.append(AggExpr([SUM, MIN, MAX, COUNT, AVG, SUM][j], f"c{j}"))
+    t = _table(ctx, cols)
+    batch = fa.BatchGroupBy(ctx, [(t, "key", aggs, col("c0") > -5 * 10**8)])
+    got = batch.run()[0]
+    assert_batches_equal(got, ora.groupby(cols, col("c0") > -5 * 10**8, "key", aggs).exact, "key")
+    batch.free()

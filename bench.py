@@ -214,8 +214,8 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
     mix of filtered COUNT(*), GROUP BY g / key, INNER / SEMI joins on
     predicate-selected probe ranges and ORDER BY v2 DESC LIMIT 10, Poisson
     arrivals at each offered QPS.  The same arrival streams are also served by
-    the plain per-query form: one slot, no shared scans.  Both servers are
-    warmed up first (scratch arenas allocated, request shapes learned)."""
+    the plain per-query form: one slot, no shared scans.  The two forms run in
+    separate phases, each on its own warmed-up server."""
     import falcondb_amd as fa
     from falcondb_amd import datagen as dg
     from falcondb_amd.operators import DictColumn
@@ -227,13 +227,21 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
     del cols
     dim = fa.Table(ctx, dg.c3_build(K))
     idx = fa.JoinIndex(ctx, dim, "bk")
-    served_srv = Server(0, slots=slots, hbm_budget=budget, share_counts=True)
-    plain_srv = Server(0, slots=1, hbm_budget=budget, share_counts=False)
-    # warm-up, and parity: served results equal the requests run alone
+    # the same request streams per offered QPS for both forms
+    streams = []
+    for i, q in enumerate(qps_levels):
+        n = int(min(max(q * 1.5, 60), 600))
+        mix = dg.c5_mix(n, seed_=dg.seed(5, 32 + i))
+        streams.append(([mix_request(sp, fact, dim, idx) for sp in mix], [sp["kind"] for sp in mix]))
     mix0 = dg.c5_mix(48, seed_=dg.seed(5, 1))
     reqs0 = [mix_request(sp, fact, dim, idx) for sp in mix0]
     same = True
-    for srv in (served_srv, plain_srv):
+    results = {}
+    # one form at a time: a server is created, warmed up (scratch arenas
+    # allocated, request shapes learned), checked against the requests run
+    # alone, swept over every offered QPS and closed before the other starts
+    for name, srv_slots, share in (("served", slots, True), ("plain", 1, False)):
+        srv = Server(0, slots=srv_slots, hbm_budget=budget, share_counts=share)
         ts = [srv.submit(r) for r in reqs0]
         for sp, r, t in zip(mix0, reqs0, ts):
             got, _ = srv.wait(t)
@@ -242,20 +250,15 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
                 got, exp = got.sort_by(sp["key"]), exp.sort_by(sp["key"])
             same &= got.num_rows == exp.num_rows and all(
                 np.array_equal(a, b) for a, b in zip(got.columns, exp.columns))
-    served, plain = [], []
-    for i, q in enumerate(qps_levels):
-        n = int(min(max(q * 1.5, 60), 600))
-        mix = dg.c5_mix(n, seed_=dg.seed(5, 32 + i))
-        reqs = [mix_request(sp, fact, dim, idx) for sp in mix]
-        kinds = [sp["kind"] for sp in mix]
-        served.append(_serve_stream(served_srv, reqs, kinds, q, 100 + i))
-        plain.append(_serve_stream(plain_srv, reqs, kinds, q, 100 + i))
-        log(f"[bench] C5 qps {q}: served p50 {served[-1]['p50_ms']:.2f} p99 {served[-1]['p99_ms']:.2f} ms "
-            f"{served[-1]['qps_served']:.0f} q/s {served[-1]['rows_per_s'] / 1e9:.1f} G rows/s | plain p50 "
-            f"{plain[-1]['p50_ms']:.2f} p99 {plain[-1]['p99_ms']:.2f} ms {plain[-1]['qps_served']:.0f} q/s "
-            f"{plain[-1]['rows_per_s'] / 1e9:.1f} G rows/s")
-    served_srv.close()
-    plain_srv.close()
+        results[name] = [_serve_stream(srv, reqs, kinds, q, 100 + i)
+                         for i, (q, (reqs, kinds)) in enumerate(zip(qps_levels, streams))]
+        srv.close()
+    served, plain = results["served"], results["plain"]
+    for q, a, b in zip(qps_levels, served, plain):
+        log(f"[bench] C5 qps {q}: served p50 {a['p50_ms']:.2f} p99 {a['p99_ms']:.2f} ms "
+            f"{a['qps_served']:.0f} q/s {a['rows_per_s'] / 1e9:.1f} G rows/s | plain p50 "
+            f"{b['p50_ms']:.2f} p99 {b['p99_ms']:.2f} ms {b['qps_served']:.0f} q/s "
+            f"{b['rows_per_s'] / 1e9:.1f} G rows/s")
     idx.free()
     dim.free()
     fact.free()

@@ -20,7 +20,7 @@
 
 namespace fgpu {
 
-constexpr int BATCH_UNROLL = 4;  // 64-row steps per wave tile
+constexpr int BATCH_UNROLL = 4;  // 64-row steps per wave tile (queries of at most two inputs)
 
 FGPU_DEV int64_t rfl64(unsigned long long v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
@@ -37,51 +37,103 @@ FGPU_DEV int64_t batch_next(unsigned long long *counter, unsigned long long *slo
     return u;
 }
 
-FGPU_DEV int64_t batch_input(const GbArgs &a, int j, int64_t r, uint32_t &valid) {
-    int64_t v = 0;
-    if (j < a.n_in) {
-        bool ok;
-        load_cell(a.cols, a.in_col[j], r, v, ok);
-        valid |= ok ? 1u << j : 0u;
-    }
-    return v;
-}
-
-FGPU_DEV Inputs batch_inputs(const GbArgs &a, int64_t r) {
-    Inputs in;
-    in.valid = 0;
-    in.v0 = batch_input(a, 0, r, in.valid);
-    in.v1 = batch_input(a, 1, r, in.valid);
-    in.v2 = batch_input(a, 2, r, in.valid);
-    in.v3 = batch_input(a, 3, r, in.valid);
-    in.v4 = batch_input(a, 4, r, in.valid);
-    in.v5 = batch_input(a, 5, r, in.valid);
-    return in;
-}
-
 // A BQ_PART record: word 0 is the key slot, or (packed) the slot with input 0
 // above it; then the remaining inputs and the validity word.  A two-word
 // record goes out as one 16-byte store (records are 16-byte aligned then).
-FGPU_DEV void batch_put(int64_t *rec, const BatchQuery &Q, int64_t slot, const Inputs &iu) {
+template <int NINC>
+FGPU_DEV void batch_put(int64_t *rec, const BatchQuery &Q, int64_t slot, const int64_t (&v)[NINC],
+                        uint32_t valid) {
     const int ni = Q.g.n_in;
-    int64_t w[MAX_IN + 2];  // at most the slot, MAX_IN inputs and the validity word
-    w[0] = Q.pack ? (int64_t)((uint64_t)slot | ((uint64_t)(iu.v0 - Q.pk_vmin) << Q.pk_kbits)) : slot;
+    int64_t w[NINC + 2];  // at most the slot, NINC inputs and the validity word
+    w[0] = Q.pack ? (int64_t)((uint64_t)slot | ((uint64_t)(v[0] - Q.pk_vmin) << Q.pk_kbits)) : slot;
     const int j0 = Q.pack ? 1 : 0;  // first input after word 0
 #pragma unroll
-    for (int k = 1; k <= MAX_IN + 1; ++k) {
-        const int j = j0 + k - 1;
-        w[k] = j < ni ? pick(iu, j) : (int64_t)iu.valid;
+    for (int k = 1; k <= NINC + 1; ++k) {
+        const int j = j0 + k - 1;  // input j, or the validity word after the last input
+        int64_t x = (int64_t)valid;
+#pragma unroll
+        for (int q = 0; q < NINC; ++q) x = (j == q && q < ni) ? v[q] : x;
+        w[k] = x;
     }
     if (Q.RW == 2) {
         gst((ll2 *)rec, ll2{w[0], w[1]});
         return;
     }
 #pragma unroll
-    for (int k = 0; k <= MAX_IN + 1; ++k)
+    for (int k = 0; k <= NINC + 1; ++k)
         if (k < Q.RW) gst(rec + k, w[k]);
 }
 
-template <int NPC>
+constexpr int FAST_UNROLL = 4;  // rows per lane per tile of the specialised pass-1 tile
+
+// A cell of a NULL-free column of 4 (INT32, sign-extended) or 8 bytes.
+FGPU_DEV int64_t fast_cell(const void *base, int w, int64_t r) {
+    return w == 4 ? (int64_t)gld((const int32_t *)base + r) : gld((const int64_t *)base + r);
+}
+
+// Pass-1 rows of a unit of a query with BatchQuery::fast set: each row is a
+// key, one or two inputs and at most one predicate cell, all NULL-free, so a
+// tile is FAST_UNROLL loads per column issued together, one comparison and
+// one record of one or two words per passing row.
+FGPU_DEV void batch_fast_rows(const BatchQuery &Q, const BatchUnit &U, int64_t *my_cells, unsigned *cursor,
+                              unsigned long long *flags, int wave, int nw) {
+    const GbArgs &a = Q.g;
+    const int lane = threadIdx.x & 63;
+    const void *kb = a.cols.v[a.key_col];
+    const void *ib0 = a.cols.v[a.in_col[0]];
+    const void *ib1 = a.cols.v[a.in_col[a.n_in > 1 ? 1 : 0]];
+    const void *pb = a.pred.n ? a.cols.v[a.pred.pcol[0]] : kb;
+    const DPredNode nd = a.pred.nodes[0];
+    const int kw = Q.fast_kw, pw = Q.fast_pw, iw0 = Q.fast_iw0, iw1 = Q.fast_iw1;
+    const bool two = a.n_in > 1;
+    const int64_t key_min = Q.key_min, vmin = Q.pk_vmin;
+    const int shift = Q.shift, kbits = Q.pk_kbits;
+    const bool pack = Q.pack != 0;
+    const unsigned Cw = (unsigned)Q.Cw;
+    const int RW = Q.RW;  // 1 (packed, one input) or 2
+    for (int64_t base = U.row0 + (int64_t)wave * 64 * FAST_UNROLL; base < U.row1;
+         base += (int64_t)nw * 64 * FAST_UNROLL) {
+        int64_t k[FAST_UNROLL], x0[FAST_UNROLL], x1[FAST_UNROLL], pv[FAST_UNROLL];
+        // every cell of the tile issued before any is examined (rows past the
+        // unit re-read its last row)
+#pragma unroll
+        for (int u = 0; u < FAST_UNROLL; ++u) {
+            const int64_t r0 = base + u * 64 + lane;
+            const int64_t r = r0 < U.row1 ? r0 : U.row1 - 1;
+            k[u] = fast_cell(kb, kw, r);
+            x0[u] = fast_cell(ib0, iw0, r);
+            x1[u] = two ? fast_cell(ib1, iw1, r) : 0;
+            pv[u] = pw ? fast_cell(pb, pw, r) : 0;
+        }
+        unsigned pos[FAST_UNROLL];
+        int part[FAST_UNROLL];
+        bool pass[FAST_UNROLL];
+#pragma unroll
+        for (int u = 0; u < FAST_UNROLL; ++u) {
+            pass[u] = base + u * 64 + lane < U.row1 && (pw == 0 || leaf_reg(nd, pv[u], true));
+            part[u] = (int)((k[u] - key_min) >> shift);
+            pos[u] = pass[u] ? atomicAdd(cursor + part[u], 1u) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < FAST_UNROLL; ++u) {
+            if (!pass[u]) continue;
+            if (pos[u] >= Cw) {
+                atomicExch(flags, 1ull);
+                continue;
+            }
+            const int64_t slot = k[u] - key_min;
+            // packed: slot | input 0 << kbits, then input 1 if any; else slot, input 0
+            const int64_t w0 = pack ? (int64_t)((uint64_t)slot | ((uint64_t)(x0[u] - vmin) << kbits)) : slot;
+            int64_t *rec = my_cells + ((int64_t)part[u] * Cw + pos[u]) * RW;
+            if (RW == 2) gst((ll2 *)rec, ll2{w0, pack ? x1[u] : x0[u]});
+            else gst(rec, w0);
+        }
+    }
+}
+
+// Pass 1.  NINC bounds the aggregate inputs of the batch's queries (2 or
+// MAX_IN: register arrays sized for it), NPC the predicate columns.
+template <int NPC, int NINC>
 __global__ __launch_bounds__(1024) void k_batch_scan(const BatchQuery *__restrict__ qs,
                                                      const BatchUnit *__restrict__ units, BatchArgs A) {
     extern __shared__ uint64_t lds[];
@@ -89,6 +141,7 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchQuery *__restric
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int nw = __builtin_amdgcn_readfirstlane((int)(blockDim.x >> 6));
+    constexpr int UR = NINC <= 2 ? BATCH_UNROLL : 2;  // rows per lane per tile (fits 128 VGPRs)
     for (;;) {
         const int64_t ui = batch_next(A.work, &unit_slot);
         if (ui >= A.n_units) return;
@@ -96,6 +149,7 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchQuery *__restric
         const BatchQuery &Q = qs[U.q];
         const GbArgs &a = Q.g;
         const int S = Q.S;
+        const int ni = a.n_in;
         const bool dense = Q.mode == BQ_DENSE;
         const OpList ops = op_list(a.ops, Q.has_valid_word != 0);
         uint64_t *null_st = lds;
@@ -109,58 +163,105 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchQuery *__restric
         }
         __syncthreads();
         int64_t *my_cells = dense ? nullptr : Q.cells + (int64_t)U.local * Q.P * Q.Cw * Q.RW;
-        for (int64_t base = U.row0 + (int64_t)wave * 64 * BATCH_UNROLL; base < U.row1;
-             base += (int64_t)nw * 64 * BATCH_UNROLL) {
-            int64_t rows[BATCH_UNROLL];
-            bool pass[BATCH_UNROLL];
+        if (Q.fast) {
+            batch_fast_rows(Q, U, my_cells, cursor, A.flags, wave, nw);
+        } else {
+        const ColRef kc = col_ref(a.cols, a.key_col);
+        ColRef ic[NINC];
 #pragma unroll
-            for (int u = 0; u < BATCH_UNROLL; ++u) rows[u] = base + u * 64 + lane;
-            pred_tile<BATCH_UNROLL, NPC>(a.pred, a.cols, rows, U.row1, pass);
-            int64_t key[BATCH_UNROLL];
-            bool has_key[BATCH_UNROLL];
-            Inputs in[BATCH_UNROLL];
+        for (int j = 0; j < NINC; ++j) ic[j] = col_ref(a.cols, j < ni ? a.in_col[j] : 0);
+        for (int64_t base = U.row0 + (int64_t)wave * 64 * UR; base < U.row1;
+             base += (int64_t)nw * 64 * UR) {
+            int64_t rows[UR];
 #pragma unroll
-            for (int u = 0; u < BATCH_UNROLL; ++u) {
-                key[u] = 0;
-                has_key[u] = false;
-                in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
-                if (pass[u]) {
-                    bool ok;
-                    load_cell(a.cols, a.key_col, rows[u], key[u], ok);
-                    has_key[u] = ok && key[u] != EMPTY_KEY;  // NULL and the sentinel form the NULL group
-                    in[u] = batch_inputs(a, rows[u]);
+            for (int u = 0; u < UR; ++u) rows[u] = base + u * 64 + lane;
+            // key and inputs of every row of the tile issued first (rows past
+            // the unit re-read its last row), then the predicate columns: one
+            // round trip for all of them.  At the selectivities these scans
+            // see, a row that fails still shares its cache lines with rows
+            // that pass.
+            RawCell3 kr[UR], vr[UR][NINC];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                const int64_t r = rows[u] < U.row1 ? rows[u] : U.row1 - 1;
+                cell_issue3(kc, r, kr[u]);
+#pragma unroll
+                for (int j = 0; j < NINC; ++j)
+                    if (j < ni) cell_issue3(ic[j], r, vr[u][j]);
+            }
+            bool pass[UR];
+            pred_tile<UR, NPC>(a.pred, a.cols, rows, U.row1, pass);
+            int64_t key[UR];
+            bool has_key[UR];
+            int64_t v[UR][NINC];
+            uint32_t valid[UR];
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                bool ok;
+                cell_finish3(kc, rows[u], kr[u], key[u], ok);
+                has_key[u] = ok && key[u] != EMPTY_KEY;  // NULL and the sentinel form the NULL group
+                valid[u] = 0;
+#pragma unroll
+                for (int j = 0; j < NINC; ++j) {
+                    v[u][j] = 0;
+                    if (j < ni) {
+                        bool okj;
+                        cell_finish3(ic[j], rows[u], vr[u][j], v[u][j], okj);
+                        valid[u] |= okj ? 1u << j : 0u;
+                    }
                 }
             }
-            // one row per iteration; the u-th values are picked with selects so
-            // that update_row has a single call site (the loop stays rolled)
+            if (!dense) {
+                // BQ_PART: every keyed row's cell position first (the cursor
+                // atomics in flight together), then its record
+                unsigned pos[UR];
+                int part[UR];
+#pragma unroll
+                for (int u = 0; u < UR; ++u) {
+                    // slot in [0, range): the range comes from the column's min / max
+                    part[u] = (int)((key[u] - Q.key_min) >> Q.shift);
+                    pos[u] = pass[u] && has_key[u] ? atomicAdd(cursor + part[u], 1u) : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < UR; ++u) {
+                    if (!pass[u] || !has_key[u]) continue;
+                    if (pos[u] >= (unsigned)Q.Cw) {
+                        atomicExch(A.flags, 1ull);
+                        continue;
+                    }
+                    batch_put<NINC>(my_cells + ((int64_t)part[u] * Q.Cw + pos[u]) * Q.RW, Q, key[u] - Q.key_min,
+                                    v[u], valid[u]);
+                }
+            }
+            // rows whose state is in LDS: the NULL group, and every row of a
+            // BQ_DENSE query.  One row per iteration, picked with selects, so
+            // that update_row has a single call site (the loop stays rolled).
 #pragma unroll 1
-            for (int u = 0; u < BATCH_UNROLL; ++u) {
+            for (int u = 0; u < UR; ++u) {
                 bool ps = pass[0], hk = has_key[0];
                 int64_t k = key[0];
-                Inputs iu = in[0];
+                int64_t vu[NINC];
+                uint32_t vl = valid[0];
 #pragma unroll
-                for (int q = 1; q < BATCH_UNROLL; ++q)
+                for (int j = 0; j < NINC; ++j) vu[j] = v[0][j];
+#pragma unroll
+                for (int q = 1; q < UR; ++q)
                     if (u == q) {
                         ps = pass[q];
                         hk = has_key[q];
                         k = key[q];
-                        iu = in[q];
+                        vl = valid[q];
+#pragma unroll
+                        for (int j = 0; j < NINC; ++j) vu[j] = v[q][j];
                     }
-                if (!ps) continue;
-                const int64_t slot = k - Q.key_min;  // in [0, range): the range comes from the column's min / max
-                uint64_t *st = !hk ? null_st : dense ? table + slot * S : nullptr;
-                if (st) {
-                    update_row<LDS_SCOPE>(st, ops, iu);
-                    continue;
-                }
-                const int p = (int)(slot >> Q.shift);
-                const unsigned pos = atomicAdd(cursor + p, 1u);
-                if (pos >= (unsigned)Q.Cw) {
-                    atomicExch(A.flags, 1ull);
-                    continue;
-                }
-                batch_put(my_cells + ((int64_t)p * Q.Cw + pos) * Q.RW, Q, slot, iu);
+                if (!ps || (hk && !dense)) continue;
+                int64_t w6[MAX_IN];
+#pragma unroll
+                for (int j = 0; j < MAX_IN; ++j) w6[j] = j < NINC ? vu[j < NINC ? j : 0] : 0;
+                const Inputs iu{w6[0], w6[1], w6[2], w6[3], w6[4], w6[5], vl};
+                update_row<LDS_SCOPE>(!hk ? null_st : table + (k - Q.key_min) * S, ops, iu);
             }
+        }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < S; i += blockDim.x) Q.null_blocks[(int64_t)U.local * S + i] = null_st[i];
@@ -482,12 +583,23 @@ __global__ __launch_bounds__(1024) void k_batch_merge(const BatchQuery *__restri
     }
 }
 
-hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int npc, int grid,
+// The pass-1 instantiation for a batch's predicate columns and inputs.
+const void *batch_scan_fn(int npc, int nin) {
+    if (nin <= 2) {
+        if (npc <= 1) return (const void *)k_batch_scan<1, 2>;
+        if (npc <= 2) return (const void *)k_batch_scan<2, 2>;
+        return (const void *)k_batch_scan<MAX_PCOLS, 2>;
+    }
+    if (npc <= 1) return (const void *)k_batch_scan<1, MAX_IN>;
+    if (npc <= 2) return (const void *)k_batch_scan<2, MAX_IN>;
+    return (const void *)k_batch_scan<MAX_PCOLS, MAX_IN>;
+}
+
+hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int npc, int nin, int grid,
                              int block, size_t lds) {
-    if (npc <= 1) hipLaunchKernelGGL(k_batch_scan<1>, dim3(grid), dim3(block), lds, s, qs, a.units, a);
-    else if (npc <= 2) hipLaunchKernelGGL(k_batch_scan<2>, dim3(grid), dim3(block), lds, s, qs, a.units, a);
-    else hipLaunchKernelGGL(k_batch_scan<MAX_PCOLS>, dim3(grid), dim3(block), lds, s, qs, a.units, a);
-    return hipGetLastError();
+    const BatchUnit *units = a.units;
+    void *args[] = {(void *)&qs, (void *)&units, (void *)&a};
+    return hipLaunchKernel(batch_scan_fn(npc, nin), dim3(grid), dim3(block), args, lds, s);
 }
 
 hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int grid, int block,
@@ -499,11 +611,8 @@ hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchAr
 // Resident workgroups per CU of each pass at this block size and LDS use:
 // the persistent grids launch that many per CU, so a small pass-2 table lets
 // several workgroups share a CU and hide each other's unit-start latency.
-hipError_t batch_occupancy(int npc, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2) {
-    const void *scan = npc <= 1 ? (const void *)k_batch_scan<1>
-                       : npc <= 2 ? (const void *)k_batch_scan<2>
-                                  : (const void *)k_batch_scan<MAX_PCOLS>;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu1, scan, block, lds1);
+hipError_t batch_occupancy(int npc, int nin, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2) {
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu1, batch_scan_fn(npc, nin), block, lds1);
     if (e != hipSuccess) return e;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, (const void *)k_batch_merge, block, lds2);
     per_cu1 = std::max(1, per_cu1);
@@ -513,12 +622,12 @@ hipError_t batch_occupancy(int npc, int block, size_t lds1, size_t lds2, int &pe
 
 // Raises the dynamic LDS limit of the kernels (outside any graph capture).
 hipError_t batch_set_lds(size_t lds1, size_t lds2) {
-    const void *scans[3] = {(const void *)k_batch_scan<1>, (const void *)k_batch_scan<2>,
-                            (const void *)k_batch_scan<MAX_PCOLS>};
-    for (const void *f : scans) {
-        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-        if (e != hipSuccess) return e;
-    }
+    for (int npc : {1, 2, MAX_PCOLS})
+        for (int nin : {2, MAX_IN}) {
+            const hipError_t e =
+                hipFuncSetAttribute(batch_scan_fn(npc, nin), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+            if (e != hipSuccess) return e;
+        }
     return hipFuncSetAttribute((const void *)k_batch_merge, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
 }
 

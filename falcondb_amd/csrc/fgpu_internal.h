@@ -155,52 +155,63 @@ FGPU_HD bool fx_digits(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1, 
 
 // Converts W carry-save words (LSB weight 2^L) to the nearest double
 // (round half to even).  `w` is read-only.
+// Two passes over the words, each with a loop index that is the same for
+// every lane (no per-lane indexing into a digit array): the first finds the
+// sign of the total, the second produces the base-2^32 digits of its
+// magnitude bottom up and keeps the three digits from the highest nonzero
+// one down, plus whether any digit below those is nonzero.
 FGPU_HD double fx_to_double(const int64_t *w, int W, int L) {
-    // Normalise into base-2^32 digits of a two's complement integer.
-    uint32_t dig[MAX_FX_WORDS + 3];
+    const int ND = W + 3;  // the total's digits: the words' span plus carries and sign
+    // pass 1: the top digit of the normalised two's complement integer
     int64_t carry = 0;
-    const int ND = W + 3;
+    uint32_t top = 0;
     for (int k = 0; k < ND; ++k) {
         // t = w[k] + carry may exceed int64 only if |w[k]| ~ 2^63; words stay
         // below 2^63 by construction (see header comment).
         const int64_t t = (k < W ? w[k] : 0) + carry;
-        dig[k] = (uint32_t)(t & 0xffffffffll);
+        top = (uint32_t)(t & 0xffffffffll);
         carry = t >> 32;  // arithmetic shift
     }
-    const bool neg = (dig[ND - 1] >> 31) & 1;
-    if (neg) {  // two's complement negate
-        uint64_t c = 1;
-        for (int k = 0; k < ND; ++k) {
-            const uint64_t v = (uint64_t)(uint32_t)~dig[k] + c;
-            dig[k] = (uint32_t)v;
-            c = v >> 32;
+    const bool neg = (top >> 31) & 1;
+    // pass 2: the magnitude's digits (two's complement negated on the fly)
+    carry = 0;
+    uint64_t nc = 1;                  // carry of the negation's +1
+    uint32_t l0 = 0, l1 = 0, l2 = 0;  // the last three digits, l2 the newest
+    bool below = false;               // a nonzero digit below l0
+    uint32_t x0 = 0, x1 = 0, x2 = 0;  // l0..l2 when the newest nonzero digit came
+    bool x_below = false;
+    int x_pos = -1;  // index of the highest nonzero digit
+    for (int k = 0; k < ND; ++k) {
+        const int64_t t = (k < W ? w[k] : 0) + carry;
+        carry = t >> 32;
+        uint32_t d = (uint32_t)(t & 0xffffffffll);
+        if (neg) {
+            const uint64_t v = (uint64_t)(uint32_t)~d + nc;
+            d = (uint32_t)v;
+            nc = v >> 32;
+        }
+        below = below || l0 != 0;
+        l0 = l1;
+        l1 = l2;
+        l2 = d;
+        if (d) {
+            x0 = l0;
+            x1 = l1;
+            x2 = l2;
+            x_below = below;
+            x_pos = k;
         }
     }
-    int top = ND - 1;
-    while (top >= 0 && dig[top] == 0) --top;
-    if (top < 0) return 0.0;
-    const int lz = __builtin_clz(dig[top]);
-    const int msb = top * 32 + (31 - lz);  // bit index of the leading one
-    // bits [lo, lo + 63] of the magnitude (bits below 0 read as 0)
-    auto digit = [&](int q) -> uint64_t { return (q >= 0 && q < ND) ? (uint64_t)dig[q] : 0ull; };
-    auto get64 = [&](int lo) -> uint64_t {
-        const int q = lo >= 0 ? lo / 32 : -((-lo + 31) / 32);
-        const int s = lo - q * 32;  // 0..31
-        const uint64_t a = digit(q) | (digit(q + 1) << 32);
-        const uint64_t b = digit(q + 2);
-        return s == 0 ? a : ((a >> s) | (b << (64 - s)));
-    };
-    const uint64_t win = get64(msb - 63);  // leading one at bit 63
+    if (x_pos < 0) return 0.0;
+    const int lz = __builtin_clz(x2);
+    const int msb = x_pos * 32 + (31 - lz);  // bit index of the leading one
+    // the 64 bits from the leading one down, and whether anything below them is set
+    const uint64_t hi = ((uint64_t)x2 << 32) | x1;
+    const uint64_t win = lz ? (hi << lz) | ((uint64_t)x0 >> (32 - lz)) : hi;
+    const bool sticky = x_below || (uint32_t)(x0 << lz) != 0;
     uint64_t mant = win >> 11;             // 53 bits
     const uint64_t rest = win & 0x7ffull;  // 11 bits below the mantissa
     const uint64_t half = 0x400ull;
-    bool sticky = false;
-    const int lim = msb - 64;  // highest bit index below the window
-    for (int q = 0; q < ND && 32 * q <= lim; ++q) {
-        const int span = lim - 32 * q + 1;  // bits of this digit below the window
-        const uint32_t m = span >= 32 ? 0xffffffffu : ((1u << span) - 1u);
-        if (dig[q] & m) sticky = true;
-    }
     int exp2 = msb - 52 + L;  // value = mant * 2^exp2
     if (rest > half || (rest == half && (sticky || (mant & 1)))) {
         ++mant;

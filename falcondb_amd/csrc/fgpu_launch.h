@@ -326,6 +326,11 @@ struct BatchQuery {
     int S, RW, Cw, has_valid_word;
     int pack, pk_kbits;       // BQ_PART records: slot | (input 0 - pk_vmin) << pk_kbits in word 0
     int64_t pk_vmin;
+    // Specialised pass-1 tile (fast != 0): a BQ_PART query of one- or two-word records
+    // whose key, inputs and predicate column have no NULLs and whose
+    // predicate is one comparison (or none).  Widths in bytes: key 4 / 8,
+    // predicate column 0 (none) / 4 / 8, inputs 4 / 8 (INT32 sign-extended).
+    int fast, fast_kw, fast_pw, fast_iw0, fast_iw1;
     int64_t unit0, n_units;   // this query's pass-1 units
     int64_t *cells;           // BQ_PART: n_units * P * Cw * RW words
     unsigned int *cell_cnt;   // BQ_PART: n_units * P record counts
@@ -361,12 +366,12 @@ struct BatchArgs {
     unsigned long long *flags;  // [0] cell full, [1] output full
     int lds_slots;              // LDS table slots (pass 1 BQ_DENSE range cap, pass 2 partition size)
 };
-hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int npc, int grid,
+hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int npc, int nin, int grid,
                              int block, size_t lds);
 hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int grid, int block,
                               size_t lds);
 hipError_t batch_set_lds(size_t lds1, size_t lds2);
-hipError_t batch_occupancy(int npc, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2);
+hipError_t batch_occupancy(int npc, int nin, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2);
 
 // ─── ORDER BY with any LIMIT and any number of keys (fgpu_sort.hip) ────────
 hipError_t launch_mask_rows(hipStream_t s, const uint64_t *words, const int64_t *offs, int64_t n_words,

@@ -3165,6 +3165,7 @@ struct fgpu_batch {
     std::vector<BatchQ> qs;
     BatchCfg cfg;
     int npc = 1;
+    int nin = 0;  // the most aggregate inputs of a batched query
     void *mem = nullptr;
     BatchQuery *d_qs = nullptr;
     BatchUnit *d_units = nullptr;
@@ -3189,6 +3190,43 @@ void batch_release(fgpu_batch *b) {
     b->graph = nullptr;
     if (b->mem) (void)hipFree(b->mem);
     b->mem = nullptr;
+}
+
+// Byte width of a column's cells for the specialised pass-1 tile (no NULLs;
+// INT32 4, INT64 / FLOAT64 8), else 0.
+int fast_width(const DCols &c, int k) {
+    if (c.valid[k]) return 0;
+    if (c.dtype[k] == FGPU_INT32) return 4;
+    if (c.dtype[k] == FGPU_INT64 || c.dtype[k] == FGPU_FLOAT64) return 8;
+    return 0;
+}
+
+// Sets q.fast and the widths when a BQ_PART query takes the specialised tile.
+void batch_fast_tile(BatchQuery &q) {
+    const GbArgs &g = q.g;
+    q.fast = 0;
+    if (q.mode != BQ_PART || q.has_valid_word || q.RW < 1 || q.RW > 2 || g.n_in < 1 || g.n_in > 2) return;
+    const int kw = fast_width(g.cols, g.key_col);
+    if (kw == 0 || g.cols.dtype[g.key_col] == FGPU_FLOAT64) return;
+    int pw = 0;
+    if (g.pred.n != 0) {
+        const DPredNode &nd = g.pred.nodes[0];
+        if (g.pred.n != 1 || g.pred.n_pcols != 1 || nd.kind != FGPU_P_CMP || nd.pc != 0) return;
+        pw = fast_width(g.cols, g.pred.pcol[0]);
+        const int dt = g.cols.dtype[g.pred.pcol[0]];
+        const bool cls_ok = (nd.cls == CMP_F64 && dt == FGPU_FLOAT64) || (nd.cls == CMP_I64 && dt == FGPU_INT64) ||
+                            (nd.cls == CMP_I64_AS_F64 && (dt == FGPU_INT64 || dt == FGPU_INT32)) ||
+                            (nd.cls == CMP_I32 && dt == FGPU_INT32);
+        if (pw == 0 || !cls_ok) return;
+    }
+    const int iw0 = fast_width(g.cols, g.in_col[0]);
+    const int iw1 = g.n_in > 1 ? fast_width(g.cols, g.in_col[1]) : 8;
+    if (iw0 == 0 || iw1 == 0) return;
+    q.fast = 1;
+    q.fast_kw = kw;
+    q.fast_pw = pw;
+    q.fast_iw0 = iw0;
+    q.fast_iw1 = iw1;
 }
 
 // Lays out every batched query for b->cfg, allocates the plan's device memory
@@ -3217,6 +3255,7 @@ int batch_plan(fgpu_batch *b) {
     std::vector<QPieces> pieces(b->qs.size());
     b->lds1 = b->lds2 = 8;
     b->npc = 1;
+    b->nin = 0;
     for (size_t qi = 0; qi < b->qs.size(); ++qi) {
         BatchQ &Q = b->qs[qi];
         if (Q.fallback) continue;
@@ -3231,6 +3270,7 @@ int batch_plan(fgpu_batch *b) {
         q.pk_kbits = Q.pk_kbits;
         q.pk_vmin = Q.pk_vmin;
         q.RW = (Q.pack ? 0 : 1) + Q.g.n_in + Q.has_valid_word;
+        q.fast = 0;
         const DevCol &key = *Q.key;
         q.key_min = key.imin <= key.imax ? key.imin : 0;
         q.range = key.imin <= key.imax ? key.imax - key.imin + 1 : 1;
@@ -3265,6 +3305,7 @@ int batch_plan(fgpu_batch *b) {
                 units2.push_back(MergeUnit{(int32_t)qi, p, k0, std::min(k0 + (int64_t(1) << shift), q.range), p == 0, 0});
             }
             b->lds2 = std::max(b->lds2, ((size_t)1 << shift) * (S * 8 + 4) + (size_t)(q.n_units + 1) * 4);
+            batch_fast_tile(q);
         }
         q.unit0 = (int64_t)units.size();
         for (int64_t u = 0; u < q.n_units; ++u)
@@ -3286,6 +3327,7 @@ int batch_plan(fgpu_batch *b) {
             pc.valid[k] = take((size_t)q.out_cap);
         }
         b->npc = std::max(b->npc, Q.g.pred.n_pcols);
+        b->nin = std::max(b->nin, Q.g.n_in);
     }
     // larger units first: the persistent workgroups finish together
     std::stable_sort(units.begin(), units.end(), [](const BatchUnit &x, const BatchUnit &y) {
@@ -3349,25 +3391,29 @@ int batch_plan(fgpu_batch *b) {
     A.lds_slots = cfg.slots;
     // persistent grids: as many workgroups per CU as stay resident together
     int per_cu1 = 1, per_cu2 = 1;
-    HIPCHK(batch_occupancy(b->npc, cfg.block, b->lds1, b->lds2, per_cu1, per_cu2));
+    HIPCHK(batch_occupancy(b->npc, b->nin, cfg.block, b->lds1, b->lds2, per_cu1, per_cu2));
     b->per_cu1 = std::min(per_cu1, cfg.wgs_per_cu);
     b->per_cu2 = std::min(per_cu2, cfg.wgs_per_cu);
     const int grid1 = c->num_cus * b->per_cu1, grid2 = c->num_cus * b->per_cu2;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_fill_u64(c->stream, (uint64_t *)b->d_work, (int64_t)(4 + b->qs.size()), 0);
     if (e == hipSuccess && b->n_units)
-        e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, grid1, cfg.block, b->lds1);
+        e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, b->nin, grid1, cfg.block, b->lds1);
     if (e == hipSuccess && b->n_units2) e = launch_batch_merge(c->stream, b->d_qs, A, grid2, cfg.block, b->lds2);
     const hipError_t e2 = hipStreamEndCapture(c->stream, &b->graph);
     if (e != hipSuccess || e2 != hipSuccess)
         return set_err(FGPU_E_DEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    int n_packed = 0;
-    for (const BatchQ &Q : b->qs) n_packed += !Q.fallback && Q.q.mode == BQ_PART && Q.q.pack;
+    int n_packed = 0, n_fast = 0;
+    for (const BatchQ &Q : b->qs) {
+        n_packed += !Q.fallback && Q.q.mode == BQ_PART && Q.q.pack;
+        n_fast += !Q.fallback && Q.q.fast;
+    }
     char msg[200];
-    snprintf(msg, sizeof msg, "block=%d unit_rows=%lld slots=%d units=%lld/%lld wgs_per_cu=%d/%d plan_bytes=%zu packed=%d",
+    snprintf(msg, sizeof msg,
+             "block=%d unit_rows=%lld slots=%d units=%lld/%lld wgs_per_cu=%d/%d plan_bytes=%zu packed=%d fast=%d",
              cfg.block, (long long)cfg.unit_rows, cfg.slots, (long long)b->n_units, (long long)b->n_units2, b->per_cu1,
-             b->per_cu2, total, n_packed);
+             b->per_cu2, total, n_packed, n_fast);
     b->info = msg;
     return FGPU_OK;
 }

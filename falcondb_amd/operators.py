@@ -104,6 +104,10 @@ class Context:
 
     def close(self) -> None:
         if self._h:
+            # a batch prepared on this context holds its stream and memory
+            for b in list(_LIVE_BATCHES):
+                if b.ctx is self:
+                    b.free()
             _lib().fgpu_close(self._h)
             self._h = None
 

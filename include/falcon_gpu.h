@@ -166,6 +166,8 @@ typedef struct {
 /* Opens device `device` and creates a stream and a scratch arena for this
  * context.  (Database::in_memory, src/db.rs:93; falcondb_open, src/api/c_ffi.rs:26) */
 int fgpu_open(int device, fgpu_ctx **out);
+/* Batches prepared on the context (fgpu_batch_prepare) use its stream: free
+ * them before closing it. */
 void fgpu_close(fgpu_ctx *ctx);
 /* Thread-local message of the last failure on this thread ("" if none). */
 const char *fgpu_last_error(void);
@@ -393,7 +395,7 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials);
 int fgpu_batch_run(fgpu_batch *b, fgpu_result **outs);
 const char *fgpu_batch_info(const fgpu_batch *b);  /* chosen configuration, as text */
 int fgpu_batch_fallbacks(const fgpu_batch *b);
-void fgpu_batch_free(fgpu_batch *b);
+void fgpu_batch_free(fgpu_batch *b);  /* before fgpu_close of its context */
 
 /* ── serving: continuous batching of mixed requests ───────────────────────
  * A request server over device-resident tables: the concurrent form of

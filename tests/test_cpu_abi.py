@@ -113,6 +113,28 @@ def test_host_exact_sum_wide_exponents():
     assert host_exact_sum(np.array([1.0, 5e-324])) == 1.0
 
 
+def test_host_exact_sum_cancellation_and_signs():
+    # the conversion of the accumulator to a double: negative totals, totals
+    # far below the terms (carries through many digits), ties at every
+    # position of the mantissa, and subnormal totals
+    rng = np.random.default_rng(77)
+    for _ in range(300):
+        n = int(rng.integers(2, 40))
+        big = rng.choice([-1.0, 1.0], n) * 2.0 ** rng.integers(-60, 90, n) * (1 + rng.random(n))
+        x = np.concatenate([big, -big[: n // 2], rng.normal(size=3) * 2.0 ** int(rng.integers(-80, 80))])
+        rng.shuffle(x)
+        assert host_exact_sum(x) == math.fsum(x), x.tolist()
+    for k in range(1, 60):  # 1 + 2^-53 * (odd/even) ties and near-ties, both signs
+        for s in (1.0, -1.0):
+            for extra in (0.0, 2.0 ** (-53 - k), -(2.0 ** (-53 - k))):
+                x = np.array([s, s * 2.0 ** -53, s * extra, s * 2.0 ** -52])
+                assert host_exact_sum(x) == math.fsum(x), (k, s, extra)
+    x = np.array([3e-308, -2.9e-308, 1e-320])
+    assert host_exact_sum(x) == math.fsum(x)
+    assert host_exact_sum(np.array([-5e-324])) == -5e-324
+    assert host_exact_sum(np.array([2.0 ** 60, -2.0 ** 60])) == 0.0
+
+
 def test_generators_are_deterministic():
     K = dg.keys(1000, 40, seed_=5)
     assert len(set(K.tolist())) == 1000 and K.min() >= 0 and K.max() < 2 ** 40

@@ -156,3 +156,47 @@ def test_batch_six_nullable_inputs(ctx):
     got = batch.run()[0]
     assert_batches_equal(got, ora.groupby(cols, col("c0") > -5 * 10**8, "key", aggs).exact, "key")
     batch.free()
+
+
+def test_batch_specialised_tile_shapes(ctx):
+    """The specialised pass-1 tile (NULL-free key, inputs and predicate column,
+    one comparison or none, records of one or two words) against the oracle:
+    Int32 and Int64 keys, Int32 / Int64 / Float64 inputs, predicates of each
+    comparison class and none; a nullable input keeps a query on the generic
+    tile."""
+    rng = np.random.default_rng(47)
+    n = 250_000
+    k32 = rng.integers(-70_000, 90_000, n).astype(np.int32)
+    k64 = rng.integers(10**12, 10**12 + 150_000, n)
+    i32 = rng.integers(-(1 << 31), (1 << 31) - 1, n).astype(np.int32)
+    i64 = rng.integers(-10**6, 10**6, n)
+    f64 = rng.normal(size=n) * 50
+    shapes = [
+        ({"key": k32, "v": i64, "x": f64}, [AggExpr(COUNT), AggExpr(SUM, "v"), AggExpr(MAX, "x")], col("x") < 10.0),
+        ({"key": k64, "v": i32, "w": i32}, [AggExpr(SUM, "v"), AggExpr(MIN, "v"), AggExpr(MAX, "v")], col("w") >= 0),
+        ({"key": k32, "x": f64}, [AggExpr(COUNT), AggExpr(SUM, "x"), AggExpr(MIN, "x")], col("key") != 5),
+        ({"key": k64, "v": i64, "w": i64}, [AggExpr(COUNT), AggExpr(SUM, "v"), AggExpr(AVG, "w")], None),
+        ({"key": k32, "v": i64, "w": i32}, [AggExpr(MAX, "v"), AggExpr(SUM, "w")], col("v") < 5.5e5),
+        ({"key": k32, "v": NullableColumn(i64, rng.random(n) > 0.1)}, [AggExpr(SUM, "v")], col("key") > 0),
+    ]
+    tabs = [_table(ctx, c) for c, _, _ in shapes]
+    batch = fa.BatchGroupBy(ctx, [(t, "key", a, f) for t, (_, a, f) in zip(tabs, shapes)])
+    assert "fast=5" in batch.info(), batch.info()
+    for rep in range(2):
+        for (cols, aggs, f), got in zip(shapes, batch.run()):
+            assert_batches_equal(got, ora.groupby(cols, f, "key", aggs).exact, "key")
+    batch.free()
+
+
+def test_batch_freed_when_its_context_closes():
+    # closing a context frees the batches prepared on it (their stream and
+    # memory belong to the context); a later free is a no-op
+    c = fa.Context(0)
+    cols = dg.c4_batch(20_000, 3_000, dg.seed(4, 400))
+    t = _table(c, cols)
+    b = fa.BatchGroupBy(c, [(t, "key", C4_AGGS, col("score") < 50.0)])
+    assert_batches_equal(b.run()[0], ora.groupby(cols, col("score") < 50.0, "key", C4_AGGS).exact, "key")
+    t.free()
+    c.close()
+    assert not b.handle
+    b.free()

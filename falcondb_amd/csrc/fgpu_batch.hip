@@ -66,9 +66,13 @@ FGPU_DEV void batch_put(int64_t *rec, const BatchQuery &Q, int64_t slot, const i
 
 constexpr int FAST_UNROLL = 4;  // rows per lane per tile of the specialised pass-1 tile
 
-// A cell of a NULL-free column of 4 (INT32, sign-extended) or 8 bytes.
+// A cell of a NULL-free column of 4 (INT32, sign-extended) or 8 bytes.  The
+// load is non-temporal: every column word is read once, and keeping the
+// stream out of L2 leaves L2 to the partly written record lines.
 FGPU_DEV int64_t fast_cell(const void *base, int w, int64_t r) {
-    return w == 4 ? (int64_t)gld((const int32_t *)base + r) : gld((const int64_t *)base + r);
+    typedef const __attribute__((address_space(1))) int32_t *g32;
+    typedef const __attribute__((address_space(1))) int64_t *g64;
+    return w == 4 ? (int64_t)__builtin_nontemporal_load((g32)base + r) : __builtin_nontemporal_load((g64)base + r);
 }
 
 // Pass-1 rows of a unit of a query with BatchQuery::fast set: each row is a

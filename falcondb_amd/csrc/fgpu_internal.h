@@ -155,20 +155,26 @@ FGPU_HD bool fx_digits(uint64_t bits, int L, int &w0, int64_t &d0, int64_t &d1, 
 
 // Converts W carry-save words (LSB weight 2^L) to the nearest double
 // (round half to even).  `w` is read-only.
-// Two passes over the words, each with a loop index that is the same for
-// every lane (no per-lane indexing into a digit array): the first finds the
+// The words are first read into registers (NW >= W of them, the rest zero),
+// then two fully unrolled passes run over the digits: the first finds the
 // sign of the total, the second produces the base-2^32 digits of its
 // magnitude bottom up and keeps the three digits from the highest nonzero
-// one down, plus whether any digit below those is nonzero.
-FGPU_HD double fx_to_double(const int64_t *w, int W, int L) {
-    const int ND = W + 3;  // the total's digits: the words' span plus carries and sign
+// one down, plus whether any digit below those is nonzero.  No loop trip
+// depends on a loaded value, so nothing waits on a chain of loads.
+template <int NW>
+FGPU_HD double fx_to_double_n(const int64_t *w, int W, int L) {
+    constexpr int ND = NW + 3;  // the total's digits: the words' span plus carries and sign
+    int64_t x[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) x[k] = k < W ? w[k] : 0;
     // pass 1: the top digit of the normalised two's complement integer
     int64_t carry = 0;
     uint32_t top = 0;
+#pragma unroll
     for (int k = 0; k < ND; ++k) {
-        // t = w[k] + carry may exceed int64 only if |w[k]| ~ 2^63; words stay
+        // t = x[k] + carry may exceed int64 only if |x[k]| ~ 2^63; words stay
         // below 2^63 by construction (see header comment).
-        const int64_t t = (k < W ? w[k] : 0) + carry;
+        const int64_t t = (k < NW ? x[k < NW ? k : 0] : 0) + carry;
         top = (uint32_t)(t & 0xffffffffll);
         carry = t >> 32;  // arithmetic shift
     }
@@ -181,8 +187,9 @@ FGPU_HD double fx_to_double(const int64_t *w, int W, int L) {
     uint32_t x0 = 0, x1 = 0, x2 = 0;  // l0..l2 when the newest nonzero digit came
     bool x_below = false;
     int x_pos = -1;  // index of the highest nonzero digit
+#pragma unroll
     for (int k = 0; k < ND; ++k) {
-        const int64_t t = (k < W ? w[k] : 0) + carry;
+        const int64_t t = (k < NW ? x[k < NW ? k : 0] : 0) + carry;
         carry = t >> 32;
         uint32_t d = (uint32_t)(t & 0xffffffffll);
         if (neg) {
@@ -229,6 +236,11 @@ FGPU_HD double fx_to_double(const int64_t *w, int W, int L) {
     r = __builtin_ldexp(r, exp2);
 #endif
     return neg ? -r : r;
+}
+FGPU_HD double fx_to_double(const int64_t *w, int W, int L) {
+    if (W <= 4) return fx_to_double_n<4>(w, W, L);
+    if (W <= 8) return fx_to_double_n<8>(w, W, L);
+    return fx_to_double_n<MAX_FX_WORDS>(w, W, L);
 }
 
 // Error-free addition (Knuth TwoSum): a + b = s + err exactly.

@@ -404,57 +404,58 @@ FGPU_DEV void fold_reg(uint64_t *row, const DOp &o, int64_t acc, double hi, doub
     }
 }
 
+// One output of a group state: its value bits and validity byte.
+FGPU_DEV void emit_one(const DOut &d, const uint64_t *st, int64_t &out, uint8_t &valid) {
+    const int64_t cnt = (int64_t)st[d.cnt_word];
+    const uint64_t w = st[d.word];
+    out = 0;
+    valid = cnt > 0;
+    switch (d.kind) {
+        case OUT_COUNT_ROWS: out = (int64_t)st[0]; valid = 1; break;
+        case OUT_COUNT: out = cnt; valid = 1; break;
+        case OUT_SUM_I:
+            out = (int64_t)w;
+            if (d.W == 2) {  // the exact total, or (valid = 2) its rounded double
+                uint64_t t_lo;
+                int64_t t_hi;
+                isum_total(w, (int64_t)st[d.word + 1], t_lo, t_hi);
+                out = (int64_t)t_lo;
+                if (!isum_fits(t_lo, t_hi)) {
+                    out = dbits(isum_to_double(w, (int64_t)st[d.word + 1]));
+                    valid = valid ? 2 : 0;
+                }
+            }
+            break;
+        case OUT_MIN_I:
+        case OUT_MAX_I: out = (int64_t)w; break;
+        case OUT_AVG_I: {
+            const double s2 = d.W == 2 ? isum_to_double(w, (int64_t)st[d.word + 1]) : (double)(int64_t)w;
+            out = dbits(cnt ? s2 / (double)cnt : 0.0);
+            break;
+        }
+        case OUT_SUM_F:
+        case OUT_AVG_F: {
+            double s2 = fx_finish((const int64_t *)(st + d.word), d.W, d.L, d.has_flags ? st[d.word + d.W] : 0ull);
+            if (d.kind == OUT_AVG_F) s2 = cnt ? s2 / (double)cnt : 0.0;
+            out = __double_as_longlong(s2);
+            break;
+        }
+        // MIN / MAX fold like f64::min / f64::max: NaN values are skipped, a
+        // group whose values are all NaN yields NaN (the state still holds its
+        // identity), a group without values NULL
+        case OUT_MIN_F: out = w == ~0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w); break;
+        case OUT_MAX_F: out = w == 0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w); break;
+    }
+}
+
 // Converts one group state into its output row o (one value + validity byte
 // per aggregate).
 FGPU_DEV void emit_row(const DOuts &outs, const uint64_t *st, int64_t o, int64_t *const *out_val,
                        uint8_t *const *out_valid) {
     for (int k = 0; k < outs.n; ++k) {
-        const DOut d = outs.o[k];
-        const int64_t cnt = (int64_t)st[d.cnt_word];
-        const uint64_t w = st[d.word];
-        int64_t out = 0;
-        uint8_t valid = cnt > 0;
-        switch (d.kind) {
-            case OUT_COUNT_ROWS: out = (int64_t)st[0]; valid = 1; break;
-            case OUT_COUNT: out = cnt; valid = 1; break;
-            case OUT_SUM_I:
-                out = (int64_t)w;
-                if (d.W == 2) {  // the exact total, or (valid = 2) its rounded double
-                    uint64_t t_lo;
-                    int64_t t_hi;
-                    isum_total(w, (int64_t)st[d.word + 1], t_lo, t_hi);
-                    out = (int64_t)t_lo;
-                    if (!isum_fits(t_lo, t_hi)) {
-                        out = dbits(isum_to_double(w, (int64_t)st[d.word + 1]));
-                        valid = valid ? 2 : 0;
-                    }
-                }
-                break;
-            case OUT_MIN_I:
-            case OUT_MAX_I: out = (int64_t)w; break;
-            case OUT_AVG_I: {
-                const double s2 = d.W == 2 ? isum_to_double(w, (int64_t)st[d.word + 1]) : (double)(int64_t)w;
-                out = dbits(cnt ? s2 / (double)cnt : 0.0);
-                break;
-            }
-            case OUT_SUM_F:
-            case OUT_AVG_F: {
-                double s2 = fx_finish((const int64_t *)(st + d.word), d.W, d.L,
-                                      d.has_flags ? st[d.word + d.W] : 0ull);
-                if (d.kind == OUT_AVG_F) s2 = cnt ? s2 / (double)cnt : 0.0;
-                out = __double_as_longlong(s2);
-                break;
-            }
-            // MIN / MAX fold like f64::min / f64::max: NaN values are skipped,
-            // a group whose values are all NaN yields NaN (the state still holds
-            // its identity), a group without values NULL
-            case OUT_MIN_F:
-                out = w == ~0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w);
-                break;
-            case OUT_MAX_F:
-                out = w == 0ull ? (int64_t)CANON_NAN : (int64_t)f64_unorder(w);
-                break;
-        }
+        int64_t out;
+        uint8_t valid;
+        emit_one(outs.o[k], st, out, valid);
         gst(out_val[k] + o, out);
         gst(out_valid[k] + o, valid);
     }

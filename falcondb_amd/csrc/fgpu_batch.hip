@@ -407,9 +407,11 @@ FGPU_DEV void batch_update(uint64_t *table, int S, const uint32_t *pk_s, int n_o
 // One round of a merge unit's records: R per lane, NW words each.  The
 // cells are found first, then every load is issued, and only then does the
 // first update wait for its data.
+// pre_lane: this lane's entry of the cell prefix (pre[lane] for lane <= G,
+// else all ones), used when G < 64.
 template <int R, int NW>
 FGPU_DEV void merge_round(uint64_t *table, int S, const uint32_t *pk_s, int n_ops, const RecFmt &f,
-                          const unsigned *pre, int G, unsigned n_rec, unsigned i0, unsigned step,
+                          const unsigned *pre, unsigned pre_lane, int G, unsigned n_rec, unsigned i0, unsigned step,
                           const int64_t *part_cells, int64_t cell_stride, int RW, int64_t slot0) {
     const int lane = threadIdx.x & 63;
     const int64_t *addr[R];
@@ -425,6 +427,12 @@ FGPU_DEV void merge_round(uint64_t *table, int S, const uint32_t *pk_s, int n_op
         addr[r] = part_cells;  // a lane without a record reads the first one
         if (ib >= n_rec) continue;
         int lo = 0, hi = G;
+        if (G < 64) {
+            // the entries <= ib, counted across the wave's lanes (pre[0] = 0
+            // counts, pre[G] = n_rec > ib does not)
+            lo = __popcll(__ballot(pre_lane <= ib)) - 1;
+            hi = lo + 1;
+        }
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (pre[mid] <= ib) lo = mid;
@@ -538,14 +546,15 @@ __global__ __launch_bounds__(1024) void k_batch_merge(const BatchQuery *__restri
             f.vmin = Q.pk_vmin;
             f.valid_word = Q.has_valid_word ? a.n_in + (Q.pack ? 0 : 1) : -1;
             const unsigned first = threadIdx.x - lane;
+            const unsigned pre_lane = lane <= G ? pre[lane] : ~0u;
             if (RW == 2) {  // the common shape: MERGE_INFLIGHT 16-byte records per lane in flight
                 for (unsigned i0 = first; i0 < n_rec; i0 += MERGE_INFLIGHT * step)
-                    merge_round<MERGE_INFLIGHT, 2>(table, S, pk_s, n_ops, f, pre, G, n_rec, i0, step, part_cells,
-                                                   cell_stride, RW, slot0);
+                    merge_round<MERGE_INFLIGHT, 2>(table, S, pk_s, n_ops, f, pre, pre_lane, G, n_rec, i0, step,
+                                                   part_cells, cell_stride, RW, slot0);
             } else {
                 for (unsigned i0 = first; i0 < n_rec; i0 += step)
-                    merge_round<1, MAX_IN + 2>(table, S, pk_s, n_ops, f, pre, G, n_rec, i0, step, part_cells,
-                                               cell_stride, RW, slot0);
+                    merge_round<1, MAX_IN + 2>(table, S, pk_s, n_ops, f, pre, pre_lane, G, n_rec, i0, step,
+                                               part_cells, cell_stride, RW, slot0);
             }
         }
         __syncthreads();

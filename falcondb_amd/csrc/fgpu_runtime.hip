@@ -283,6 +283,7 @@ struct Tuning {
     int join_slots_per_key = 4;    // pass-2 LDS sub-table slots per build key (load factor 1 / this)
     int join_compact = 1;          // compact record keys: 0 never, 1 for key-only records, 2 always
     int gbp_pack = 1;              // partitioned and batched GROUP BY: key and an integer input in one record word
+    int batch_unit_rows = 0;       // batched GROUP BY: rows per pass-1 unit before tuning (0: the default)
 };
 
 struct fgpu_ctx {
@@ -1105,6 +1106,7 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_slots_per_key") t.join_slots_per_key = value;
     else if (n == "join_compact") t.join_compact = value;
     else if (n == "gbp_pack") t.gbp_pack = value;
+    else if (n == "batch_unit_rows") t.batch_unit_rows = value;
     else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
     if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
         t.join_block > 1024 || t.join_block % 64)
@@ -3528,6 +3530,7 @@ int fgpu_batch_prepare(fgpu_ctx *c, const fgpu_batch_query *queries, int n, fgpu
     if (int rc = set_device(c->device)) return rc;
     auto *b = new fgpu_batch();
     b->c = c;
+    if (c->tune.batch_unit_rows >= 64) b->cfg.unit_rows = c->tune.batch_unit_rows;
     b->qs.resize((size_t)n);
     for (int i = 0; i < n; ++i) {
         const fgpu_batch_query &bq = queries[i];

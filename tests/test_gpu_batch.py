@@ -200,3 +200,24 @@ def test_batch_freed_when_its_context_closes():
     c.close()
     assert not b.handle
     b.free()
+
+
+@pytest.mark.parametrize("unit_rows", [4096, 1 << 16])
+def test_batch_many_units_per_query(ctx, unit_rows):
+    """Queries cut into more than 64 pass-1 units (the merge then finds a
+    record's cell by binary search over the units' counts) and into fewer."""
+    rng = np.random.default_rng(53)
+    qs = []
+    for n, card in ((600_000, 200_000), (300_000, 5_000), (90_000, 40_000)):
+        cols = {"key": rng.integers(0, card, n).astype(np.int32), "val": rng.integers(-10**6, 10**6, n),
+                "score": rng.random(n) * 100}
+        qs.append((_table(ctx, cols), cols))
+    try:
+        ctx.set_tuning("batch_unit_rows", unit_rows)
+        batch = fa.BatchGroupBy(ctx, [(t, "key", C4_AGGS, col("score") < 70.0) for t, _ in qs])
+    finally:
+        ctx.set_tuning("batch_unit_rows", 0)
+    assert f"unit_rows={unit_rows}" in batch.info(), batch.info()
+    for (t, cols), got in zip(qs, batch.run()):
+        assert_batches_equal(got, ora.groupby(cols, col("score") < 70.0, "key", C4_AGGS).exact, "key")
+    batch.free()

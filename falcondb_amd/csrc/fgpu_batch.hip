@@ -273,7 +273,7 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchQuery *__restric
             uint64_t *blk = Q.blocks + (int64_t)U.local * Q.range * S;
             for (int64_t i = threadIdx.x; i < Q.range * S; i += blockDim.x) blk[i] = table[i];
         } else {
-            for (int i = threadIdx.x; i < Q.P; i += blockDim.x) Q.cell_cnt[(int64_t)U.local * Q.P + i] = cursor[i];
+            for (int i = threadIdx.x; i < Q.P; i += blockDim.x) Q.cell_cnt[(int64_t)i * Q.n_units + U.local] = cursor[i];
         }
         __syncthreads();  // the next unit re-initialises the LDS
     }
@@ -469,6 +469,7 @@ __global__ __launch_bounds__(1024) void k_batch_merge(const BatchQuery *__restri
     __shared__ uint64_t null_row[64];
     __shared__ uint64_t ident_row[64];
     __shared__ uint32_t pk_s[MAX_OPS];
+    __shared__ unsigned long long out_base;
     const int lane = threadIdx.x & 63;
     if (threadIdx.x == 0) unit_slot = atomicAdd(A.work + 1, 1ull);
     __syncthreads();
@@ -493,12 +494,25 @@ __global__ __launch_bounds__(1024) void k_batch_merge(const BatchQuery *__restri
         if (threadIdx.x < n_ops) pk_s[threadIdx.x] = a.ops.pk[threadIdx.x];
         if (M.part >= 0)
             for (int g = threadIdx.x; g < G; g += blockDim.x) {
-                const unsigned c = gld(Q.cell_cnt + (int64_t)g * Q.P + M.part);
+                const unsigned c = gld(Q.cell_cnt + (int64_t)M.part * G + g);  // partition-major: contiguous
                 pre[g] = c < (unsigned)Q.Cw ? c : (unsigned)Q.Cw;
             }
         if (M.null_group && threadIdx.x < S) null_row[threadIdx.x] = a.ops.ident[threadIdx.x];
         __syncthreads();
-        fill_ident(table, n, S, ident_row);
+        if (S <= 16) {
+            // whole rows per thread, the identity row held in registers
+            uint64_t iv[16];
+#pragma unroll
+            for (int w = 0; w < 16; ++w) iv[w] = w < S ? ident_row[w] : 0;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                uint64_t *row = table + (int64_t)i * S;
+#pragma unroll
+                for (int w = 0; w < 16; ++w)
+                    if (w < S) row[w] = iv[w];
+            }
+        } else {
+            fill_ident(table, n, S, ident_row);
+        }
         if (M.part >= 0 && threadIdx.x < 64) {  // exclusive scan of the partition's cell counts
             const int per = (G + 63) / 64;
             unsigned s1 = 0;
@@ -565,10 +579,17 @@ __global__ __launch_bounds__(1024) void k_batch_merge(const BatchQuery *__restri
         for (int i = threadIdx.x; i < n; i += blockDim.x) n_have += table[(int64_t)i * S] > 0 ? 1u : 0u;
         unsigned total;
         unsigned pos = block_excl_scan(n_have, total);
+        const bool emit_null = M.null_group && null_row[0] > 0;  // uniform
+        // the output reservation is issued before the list is built and read
+        // after it, so the list hides the atomic's round trip
+        unsigned long long res = 0;
+        if (threadIdx.x == 0 && total + (emit_null ? 1u : 0u))
+            res = atomicAdd(Q.out_count, (unsigned long long)(total + (emit_null ? 1u : 0u)));
         for (int i = threadIdx.x; i < n; i += blockDim.x)
             if (table[(int64_t)i * S] > 0) list[pos++] = (unsigned)i;
-        const bool emit_null = M.null_group && null_row[0] > 0;  // uniform
-        const int64_t base = (int64_t)block_reserve(Q.out_count, total + (emit_null ? 1u : 0u));  // has barriers
+        if (threadIdx.x == 0) out_base = res;
+        __syncthreads();
+        const int64_t base = (int64_t)out_base;
         const int64_t key0 = Q.key_min + (M.part >= 0 ? ((int64_t)M.part << Q.shift) : M.k0);
         for (unsigned j = threadIdx.x; j < total; j += blockDim.x) {
             const int64_t o = base + j;

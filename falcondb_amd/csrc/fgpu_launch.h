@@ -333,7 +333,7 @@ struct BatchQuery {
     int fast, fast_kw, fast_pw, fast_iw0, fast_iw1;
     int64_t unit0, n_units;   // this query's pass-1 units
     int64_t *cells;           // BQ_PART: n_units * P * Cw * RW words
-    unsigned int *cell_cnt;   // BQ_PART: n_units * P record counts
+    unsigned int *cell_cnt;   // BQ_PART: P * n_units record counts (partition-major)
     uint64_t *blocks;         // BQ_DENSE: n_units * range * S words
     uint64_t *null_blocks;    // n_units * S words: each unit's NULL-key group state
     int64_t *out_key;

@@ -3658,7 +3658,7 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
     std::vector<BatchCfg> cands;
     // slots: 4096 is capped at the largest table the state width lets fit
     // (one pass-2 workgroup per CU); smaller tables let several share a CU
-    for (int block : {1024, 512})
+    for (int block : {1024, 512, 256})
         for (int64_t ur : {int64_t(1) << 18, int64_t(1) << 16, int64_t(1) << 20})
             for (int slots : {4096, 1024, 512, 256}) {
                 BatchCfg k;
@@ -3672,6 +3672,7 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
     float best_ms = 1e30f;
     std::vector<unsigned long long> counts;
     int tried = 0;
+    std::string trials;
     for (const BatchCfg &k : cands) {
         b->cfg = k;
         if (batch_plan(b) != FGPU_OK) {
@@ -3688,6 +3689,9 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
             ms += b->last_ms;
         }
         ++tried;
+        char t[64];
+        snprintf(t, sizeof t, " %d/%lld/%d:%.3f", k.block, (long long)k.unit_rows, k.slots, ms / 2);
+        trials += t;
         if (ms < best_ms) {
             best_ms = ms;
             best = b->cfg;
@@ -3696,8 +3700,9 @@ int fgpu_batch_tune(fgpu_batch *b, int max_trials) {
     b->cfg = best;
     if (int rc = batch_plan(b)) return rc;
     char msg[96];
-    snprintf(msg, sizeof msg, " tuned=%d trials best=%.3f ms", tried, best_ms / 2);
+    snprintf(msg, sizeof msg, " tuned=%d trials best=%.3f ms;", tried, best_ms / 2);
     b->info += msg;
+    b->info += " block/unit_rows/slots:ms" + trials;
     return FGPU_OK;
 }
 

@@ -64,10 +64,11 @@ def join_call(lib, ctx, probe, build, idx, spec):
     lib.fgpu_result_free(h)
 
 
-def bench_batched(ctx, lib, steps):
+def bench_batched(ctx, lib, steps, trials=18):
     """256 filter + GROUP BY queries over batches of log-uniform size (the C4
     recipe): the same queries as 256 plain fgpu_groupby calls, then as one
-    prepared batch (one graph replay per run, autotuned)."""
+    prepared batch (one graph replay per run, autotuned over `trials`
+    configurations; 1 = the first candidate only)."""
     import falcondb_amd as fa
     from falcondb_amd import _abi
     from falcondb_amd import datagen as dg
@@ -107,7 +108,7 @@ def bench_batched(ctx, lib, steps):
     for _ in range(steps):
         plain_all()
     plain_ms = (time.perf_counter() - t0) * 1e3 / steps
-    tune_info = batch.tune(18)
+    tune_info = batch.tune(trials)
 
     def batched():
         for h in batch.run_raw():
@@ -423,6 +424,8 @@ def main():
     ap.add_argument("--no-join", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-trials", type=int, default=18,
+                    help="configurations the C4 batch's autotuner tries (1: the first candidate only)")
     ap.add_argument("--qps", default="100,300,1000,3000")
     ap.add_argument("--gb-strategy", choices=["auto", "hash"], default="auto",
                     help="C2 GROUP BY strategy: the planner's choice, or the single hash table "
@@ -595,7 +598,7 @@ def main():
         secondary["zone_maps"] = bench_zone_maps(ctx, args.rows)
 
     if not args.no_batch:
-        secondary["c4_batched"] = bench_batched(ctx, lib, max(3, args.steps // 4))
+        secondary["c4_batched"] = bench_batched(ctx, lib, max(3, args.steps // 4), args.batch_trials)
 
     if not args.no_serve:
         table.free()

@@ -157,11 +157,14 @@ struct GbpScan {
     int64_t pk_kmin, pk_vmin;
     unsigned long long *flags;
     int64_t *my;  // this workgroup's cells
+    const uint8_t *live;  // zone maps: batches that may hold a passing row (null: all)
+    int64_t batch_rows;
 };
 
 template <int U, int NPC>
 struct GbpPredRaw {
     RawCell3 c[U][NPC];
+    bool dead;  // the zone maps rule the tile out: nothing was loaded
 };
 
 template <int U, int NIN>
@@ -195,6 +198,8 @@ FGPU_DEV uint32_t gbp_pred_idx(int64_t base, int u, int64_t r1) {
 
 template <int U, int NPC, int NIN>
 FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1, GbpPredRaw<U, NPC> &t) {
+    t.dead = zone_dead(g.live, g.batch_rows, base, base + 64 * U < r1 ? base + 64 * U : r1);
+    if (t.dead) return;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t i = gbp_pred_idx(base, u, r1);
@@ -208,6 +213,7 @@ FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r
 template <int U, int NPC, int NIN>
 FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &a, const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1,
                                   const GbpPredRaw<U, NPC> &t) {
+    if (t.dead) return 0;
     int64_t v[U][NPC];
     uint32_t ok[U];
     uint32_t exists = 0;
@@ -271,6 +277,7 @@ FGPU_DEV void gbp_finish_data(const GbpScan<NPC, NIN> &g, int64_t base, const Gb
 template <int U, int FP>
 struct GbpPredFast {
     uint64_t x[U];  // INT32 values sign-extended, 64-bit values as bits
+    bool dead;      // the zone maps rule the tile out: nothing was loaded
 };
 
 template <int U, int NIN>
@@ -290,6 +297,8 @@ FGPU_DEV T gbp_ld(const uint8_t *base, uint32_t off) {
 
 template <int U, int FP, int NPC, int NIN>
 FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1, GbpPredFast<U, FP> &t) {
+    t.dead = zone_dead(g.live, g.batch_rows, base, base + 64 * U < r1 ? base + 64 * U : r1);
+    if (t.dead) return;
     const uint8_t *b = g.pc[0].base + base * FP;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -301,6 +310,7 @@ FGPU_DEV void gbp_issue_pred(const GbpScan<NPC, NIN> &g, int64_t base, int64_t r
 template <int U, int FP, int NPC, int NIN>
 FGPU_DEV uint32_t gbp_finish_pred(const GbArgs &, const GbpScan<NPC, NIN> &g, int64_t base, int64_t r1,
                                   const GbpPredFast<U, FP> &t) {
+    if (t.dead) return 0;
     int64_t v[U][1];
     uint32_t ok[U];
     uint32_t exists = 0;
@@ -433,6 +443,8 @@ __global__ __launch_bounds__(1024) void k_gbp_scan(GbPartArgs A) {
     for (int j = 0; j < NPC; ++j) g.pc[j] = col_ref(a.cols, j < a.pred.n_pcols ? a.pred.pcol[j] : 0);
     g.n_pcols = a.pred.n_pcols;
     g.leaf0 = a.pred.nodes[0];
+    g.live = a.pred.live;
+    g.batch_rows = a.pred.batch_rows;
     g.key = col_ref(a.cols, a.key_col);
 #pragma unroll
     for (int j = 0; j < NIN; ++j) g.in[j] = col_ref(a.cols, a.in_col[j]);

@@ -106,6 +106,7 @@ __global__ void k_gb_dense_lds(GbArgs a) {
     const GbCols gc = gb_cols(a);
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        if (tile_dead(a.pred, base, std::min(base + tile, a.n_rows))) continue;  // zone maps
         int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
@@ -140,6 +141,7 @@ __global__ void k_gb_dense_global(GbArgs a) {
     const int S = a.ops.S;
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        if (tile_dead(a.pred, base, std::min(base + tile, a.n_rows))) continue;  // zone maps
         int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
@@ -181,6 +183,7 @@ __global__ void k_gb_hash(GbArgs a) {
     __syncthreads();
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        if (tile_dead(a.pred, base, std::min(base + tile, a.n_rows))) continue;  // zone maps
         int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll
@@ -257,6 +260,7 @@ __global__ void k_gb_scalar(GbArgs a) {
     }
     const int64_t tile = (int64_t)blockDim.x * GB_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_rows; base += (int64_t)gridDim.x * tile) {
+        if (tile_dead(a.pred, base, std::min(base + tile, a.n_rows))) continue;  // zone maps
         int64_t rows[GB_UNROLL];
         bool pass[GB_UNROLL];
 #pragma unroll

@@ -314,6 +314,19 @@ struct DPred {
     int64_t batch_rows;
 };
 
+// Zone-map pruning in the scans: true when p.live rules out every batch that
+// rows [r0, r1) touch.  Uniform arguments give a uniform answer, so a loop can
+// skip the whole tile.
+FGPU_DEV bool zone_dead(const uint8_t *live, int64_t batch_rows, int64_t r0, int64_t r1) {
+    if (!live || r1 <= r0) return false;
+    for (int64_t bi = r0 / batch_rows; bi <= (r1 - 1) / batch_rows; ++bi)
+        if (live[bi]) return false;
+    return true;
+}
+FGPU_DEV bool tile_dead(const DPred &p, int64_t r0, int64_t r1) {
+    return zone_dead(p.live, p.batch_rows, r0, r1);
+}
+
 // One batch's bounds for one column (upload-time zone map): integers as int64;
 // floats as f64_order keys over the non-NaN values; valid and NaN counts.
 struct BatchZone {

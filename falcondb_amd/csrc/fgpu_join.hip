@@ -223,6 +223,7 @@ __global__ void k_join_probe_agg(JoinProbeArgs a) {
     constexpr bool kPairs = TYPE == FGPU_INNER || TYPE == FGPU_LEFT;
     const int64_t tile = (int64_t)blockDim.x * JOIN_UNROLL;
     for (int64_t base = (int64_t)blockIdx.x * tile; base < a.n_probe; base += (int64_t)gridDim.x * tile) {
+        if (tile_dead(a.pred, base, std::min(base + tile, a.n_probe))) continue;  // zone maps
         int64_t rows[JOIN_UNROLL], key[JOIN_UNROLL];
         bool sel[JOIN_UNROLL], has[JOIN_UNROLL];
         uint64_t slot[JOIN_UNROLL];
@@ -452,7 +453,12 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
     // a row whose key matches nothing (NULL, or outside the compact range) is
     // an output row only for LEFT and ANTI; INNER and SEMI drop it here
     const int nomatch = a.pa.type == FGPU_LEFT || a.pa.type == FGPU_ANTI ? P2 - 2 : -1;
-    auto issue = [&](int64_t base, ScatterRaw<NPC, U> &t) {
+    // issue returns true for a tile the zone maps rule out (nothing loaded)
+    auto issue = [&](int64_t base, ScatterRaw<NPC, U> &t) -> bool {
+        if (tile_dead(pa.pred, base, base + TILE < r1 ? base + TILE : r1)) {
+            t = ScatterRaw<NPC, U>{};
+            return true;
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int64_t r = base + u * JS_TPB + threadIdx.x;
@@ -463,19 +469,21 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             if (STAGE && RW > 1) cell_issue3(pay0, r, t.f1[u]);
             if (STAGE && RW > 2) cell_issue3(pay1, r, t.f2[u]);
         }
+        return false;
     };
     ScatterRaw<NPC, U> raw;
-    if (r0 < r1) issue(r0, raw);
+    bool raw_dead = false;
+    if (r0 < r1) raw_dead = issue(r0, raw);
     for (int64_t base = r0; base < r1; base += TILE) {
         ScatterRaw<NPC, U> nxt;
-        issue(base + TILE, nxt);
+        const bool nxt_dead = issue(base + TILE, nxt);
         int part[U];
         int64_t h[U], f1[U], f2[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t r = base + u * JS_TPB + threadIdx.x;
             f1[u] = f2[u] = 0;
-            bool sel = r < r1;
+            bool sel = r < r1 && !raw_dead;
             if (NPC > 0 && pa.pred.n > 0) {
                 int64_t v[NPC > 0 ? NPC : 1];
                 uint32_t ok = 0;
@@ -507,6 +515,7 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
             for (int u = 0; u < U; ++u)
                 if (part[u] >= 0) atomicAdd(cur + part[u], 1u);
             raw = nxt;
+            raw_dead = nxt_dead;
             continue;
         }
         unsigned rank[U];
@@ -566,6 +575,7 @@ __global__ __launch_bounds__(JS_TPB) void k_jp_scatter(JoinPartArgs a) {
         }
         __syncthreads();
         raw = nxt;
+        raw_dead = nxt_dead;
     }
     __syncthreads();
     if (MODE == JS_COUNT) {
@@ -1116,7 +1126,12 @@ __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
     };
     // rows base + u * JS_TPB + tid through uniform bases and 32-bit offsets;
     // a row past the range re-reads the range's last row (every load issued)
-    auto issue = [&](int64_t base, Raw &t) {
+    // issue returns true for a tile the zone maps rule out (nothing loaded)
+    auto issue = [&](int64_t base, Raw &t) -> bool {
+        if (tile_dead(pa.pred, base, base + TILE < r1 ? base + TILE : r1)) {
+            t = Raw{};
+            return true;
+        }
         const int64_t last = r1 - 1 - base;  // may be negative past the end: clamped below
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1127,18 +1142,20 @@ __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
             if (RW > 1) t.pay[u] = pcol[r];
             if (NPC > 0) cell_issue3(pc, r, t.pc[u]);
         }
+        return false;
     };
     Raw raw;
-    if (r0 < r1) issue(r0, raw);
+    bool raw_dead = false;
+    if (r0 < r1) raw_dead = issue(r0, raw);
     for (int64_t base = r0; base < r1; base += TILE) {
         Raw nxt;
-        issue(base + TILE, nxt);
+        const bool nxt_dead = issue(base + TILE, nxt);
         int part[U];
         uint32_t rec[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t r = base + u * JS_TPB + threadIdx.x;
-            bool sel = r < r1;
+            bool sel = r < r1 && !raw_dead;
             if (NPC > 0 && pa.pred.n > 0) {
                 int64_t v[1];
                 bool okj;
@@ -1190,6 +1207,7 @@ __global__ __launch_bounds__(JS_TPB) void k_jc_scatter(JoinPartArgs a) {
         }
         __syncthreads();
         raw = nxt;
+        raw_dead = nxt_dead;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < P2; i += blockDim.x) a.cell_cnt[(int64_t)blockIdx.x * P2 + i] = cur[i];

@@ -1459,7 +1459,9 @@ static bool zone_may(const DevCol &col, const BatchZone &z, int64_t rows, const 
 
 // Sets dp.live to the batches that may hold a passing row, when the zone maps
 // rule some out (uploaded into the context's scratch); else leaves it null.
-static int prune_batches(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, DPred &dp) {
+// The plan note goes to *note when given, else onto c->plan.
+static int prune_batches(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, DPred &dp,
+                         std::string *note = nullptr) {
     dp.live = nullptr;
     dp.batch_rows = t->batch_rows;
     if (!c->tune.zone_maps || !p || dp.n == 0 || t->n_rows == 0) return FGPU_OK;
@@ -1486,7 +1488,7 @@ static int prune_batches(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, D
     }
     char msg[96];
     snprintf(msg, sizeof msg, " zone maps: %lld of %lld batches live", (long long)n_live, (long long)nb);
-    c->plan += msg;
+    (note ? *note : c->plan) += msg;
     if (n_live == nb) return FGPU_OK;
     void *d;
     if (int rc = buf(c, B_LIVE, (size_t)nb, &d)) return rc;
@@ -2010,6 +2012,9 @@ int fgpu_groupby_keys(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, cons
     return FGPU_OK;
 }
 
+static int groupby_run(fgpu_ctx *c, const fgpu_table *t, const DevCol *key, Binder &b, GbArgs &a, AggLayout &L,
+                       fgpu_result **out);
+
 static int groupby_impl(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, const fgpu_groupby_spec *spec,
                         fgpu_result **out) {
     const DevCol *key = nullptr;
@@ -2017,6 +2022,17 @@ static int groupby_impl(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, co
     GbArgs a{};
     AggLayout L;
     if (int rc = gb_prepare(c, t, p, spec, b, a, L, key)) return rc;
+    // zone maps: every strategy's scan skips the batches that cannot pass
+    std::string zone_note;
+    if (int rc = prune_batches(c, t, p, a.pred, &zone_note)) return rc;
+    const int rc = groupby_run(c, t, key, b, a, L, out);
+    c->plan += zone_note;
+    return rc;
+}
+
+// Runs a prepared GROUP BY with the strategy the key and state size call for.
+static int groupby_run(fgpu_ctx *c, const fgpu_table *t, const DevCol *key, Binder &b, GbArgs &a, AggLayout &L,
+                       fgpu_result **out) {
     const int S = L.ops.S;
     int64_t n_state_groups;
     const int mode = gb_pick_mode(c, key, L, a, n_state_groups);
@@ -2666,6 +2682,8 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
         }
     }
     if (int rc = join_prepare(c, probe, pred, build, idx, spec, pb, bb, a, L)) return rc;
+    std::string zone_note;  // the probe scans skip the batches that cannot pass
+    if (int rc = prune_batches(c, probe, pred, a.pred, &zone_note)) return rc;
     mark(c, 0);
     uint64_t *d_state;
     int rc = buf(c, B_STATE, (size_t)a.S * 8, (void **)&d_state);
@@ -2675,7 +2693,7 @@ int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, c
     c->plan.clear();
     const int prc = join_agg_partitioned(c, probe, build, idx, spec, a, L);
     if (prc > 0) return prc;
-    c->plan = prc == 0 ? "join:partitioned" + c->plan : "join:direct";
+    c->plan = (prc == 0 ? "join:partitioned" + c->plan : "join:direct") + zone_note;
     if (prc < 0) {  // direct probe of the open-addressing index
         if (!idx) {
             if (int rc2 = join_build_impl(c, build, spec->build_key, &own)) return rc2;
@@ -3476,6 +3494,8 @@ int fgpu_topk(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int32_t col,
     Binder b(t);
     DPred dp;
     if (int rc = compile_pred(c, t, p, b, dp)) return rc;
+    std::string zone_note;  // the scan skips the batches that cannot pass
+    if (int rc = prune_batches(c, t, p, dp, &zone_note)) return rc;
     const int slot = b.bind(col);
     if (slot < 0) return set_err(FGPU_E_UNSUPPORTED, "query reads too many columns");
     const int K = k <= 8 ? 8 : k <= 16 ? 16 : 32;
@@ -3520,7 +3540,7 @@ int fgpu_topk(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, int32_t col,
     }
     *n_out = (int64_t)rows.size();
     finish_timing(c, t->n_rows * (int64_t)dc.elem_bytes(), false);
-    c->plan = "topk";
+    c->plan = "topk" + zone_note;
     return FGPU_OK;
 }
 

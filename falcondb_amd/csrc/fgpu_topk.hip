@@ -111,6 +111,8 @@ __global__ __launch_bounds__(256) void k_topk_scan(DCols cols, DPred pred, int c
     const ColRef kc = col_ref(cols, col);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x * TOPK_UNROLL;
     for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x * TOPK_UNROLL + (threadIdx.x & ~63); w0 < n; w0 += stride) {
+        // zone maps: the wave's rows lie in [w0, w0 + (TOPK_UNROLL - 1) * blockDim.x + 64)
+        if (tile_dead(pred, w0, std::min<int64_t>(w0 + (int64_t)(TOPK_UNROLL - 1) * blockDim.x + 64, n))) continue;
         int64_t rows[TOPK_UNROLL];
 #pragma unroll
         for (int u = 0; u < TOPK_UNROLL; ++u) rows[u] = w0 + (int64_t)u * blockDim.x + lane;

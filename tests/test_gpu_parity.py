@@ -212,7 +212,7 @@ def test_groupby_high_cardinality_zipf(gctx, n, nk):
     if ctx.mode == "partitioned":
         assert plan.startswith("groupby:partitioned") and "fallback" not in plan, plan
     else:
-        assert plan == "groupby:hash", plan
+        assert plan.split(" zone maps:")[0] == "groupby:hash", plan
     o = ora.groupby(cols, p, "key", C2_AGGS)
     assert_batches_equal(g, o.exact, "key", o.reference)
     # reference output typing: SUM/MIN/MAX of Int64 come back as Float64
@@ -773,7 +773,7 @@ def test_groupby_partitioned_at_scale_matches_hash(ctx_gbp, ctx_gbh):
         plans.append(c.last_plan())
         t.free()
     assert "partitioned" in plans[0] and "fallback" not in plans[0], plans[0]
-    assert plans[1] == "groupby:hash", plans[1]
+    assert plans[1].split(" zone maps:")[0] == "groupby:hash", plans[1]
     assert res[0].num_rows > 500_000
     assert_batches_equal(res[0], res[1], "key")
 

@@ -2301,6 +2301,10 @@ struct JoinPlan {
     Binder *pb = nullptr, *bb = nullptr;
 };
 
+// Set when join_prepare refused a spec for needing more register accumulators
+// than the probe kernels hold (JOIN_MAX_OPS).
+static thread_local bool t_join_ops_over = false;
+
 static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred,
                         const fgpu_table *build, const fgpu_join_index *idx,
                         const fgpu_join_spec *spec, Binder &pb, Binder &bb, JoinProbeArgs &a,
@@ -2334,7 +2338,10 @@ static int join_prepare(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *p
         out_rows = pairs > (__int128)INT64_MAX ? INT64_MAX : (int64_t)pairs;
     }
     if (int rc = build_layout(spec->aggs, spec->n_aggs, probe, build, slot_of, L, out_rows, true)) return rc;
-    if (L.ops.n > 4) return set_err(FGPU_E_UNSUPPORTED, "join aggregates need %d accumulators (max 4)", L.ops.n);
+    if (L.ops.n > 4) {
+        t_join_ops_over = true;  // fgpu_join_agg then runs the aggregates in parts
+        return set_err(FGPU_E_UNSUPPORTED, "join aggregates need %d accumulators (max 4)", L.ops.n);
+    }
     a.pcols = pb.dc;
     a.bcols = bb.dc;
     a.n_ops = L.ops.n;
@@ -2647,12 +2654,67 @@ static int join_agg_partitioned(fgpu_ctx *c, const fgpu_table *probe, const fgpu
     return FGPU_OK;
 }
 
+static int join_agg_impl(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                         const fgpu_join_index *idx, const fgpu_join_spec *spec, fgpu_result **out);
+
+// A join's aggregates in as few calls as the probe kernels' register
+// accumulators allow: a spec that needs more is cut in two, each part run as
+// its own join, and the parts' columns (one output row each) joined in order.
+static int join_agg_parts(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                          const fgpu_join_index *idx, const fgpu_join_spec *spec, fgpu_result **out, int &parts) {
+    int rc;
+    if (spec->n_aggs <= FGPU_MAX_AGGS) {  // more aggregates than one output layout holds: split at once
+        t_join_ops_over = false;
+        rc = join_agg_impl(c, probe, pred, build, idx, spec, out);
+        if (rc != FGPU_E_UNSUPPORTED || !t_join_ops_over || spec->n_aggs < 2) {
+            ++parts;
+            return rc;
+        }
+        g_err.clear();
+    }
+    const int h = spec->n_aggs / 2;
+    fgpu_join_spec s1 = *spec, s2 = *spec;
+    s1.n_aggs = h;
+    s2.aggs = spec->aggs + h;
+    s2.n_aggs = spec->n_aggs - h;
+    fgpu_result *r1 = nullptr, *r2 = nullptr;
+    if ((rc = join_agg_parts(c, probe, pred, build, idx, &s1, &r1, parts))) return rc;
+    if ((rc = join_agg_parts(c, probe, pred, build, idx, &s2, &r2, parts))) {
+        delete r1;
+        return rc;
+    }
+    for (size_t j = 0; j < r2->cols.size(); ++j) {
+        RCol &col = r2->cols[j];
+        col.name = alias_of(spec->aggs[h + j], h + (int)j);  // the output position in the whole spec
+        if (col.xv) {  // values in r2's pinned block: copy the one row out
+            if (col.dtype == FGPU_FLOAT64) col.f64.assign(1, *(const double *)col.xv);
+            else if (col.dtype == FGPU_UTF8) col.u32.assign(1, *(const uint32_t *)col.xv);
+            else col.i64.assign(1, *(const int64_t *)col.xv);
+            if (col.xvalid) col.valid.assign(1, *col.xvalid);
+            col.xv = nullptr;
+            col.xvalid = nullptr;
+        }
+        r1->cols.push_back(std::move(col));
+    }
+    delete r2;
+    *out = r1;
+    return FGPU_OK;
+}
+
 int fgpu_join_agg(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
                   const fgpu_join_index *idx, const fgpu_join_spec *spec, fgpu_result **out) {
     if (!c || !probe || !build || !spec || !out || (spec->n_aggs > 0 && !spec->aggs))
         return set_err(FGPU_E_INVALID_QUERY, "NULL argument");
     std::lock_guard<std::mutex> lk(c->mu);
     if (int rc = set_device(c->device)) return rc;
+    int parts = 0;
+    const int rc = join_agg_parts(c, probe, pred, build, idx, spec, out, parts);
+    if (rc == FGPU_OK && parts > 1) c->plan += "; aggregates in " + std::to_string(parts) + " parts";
+    return rc;
+}
+
+static int join_agg_impl(fgpu_ctx *c, const fgpu_table *probe, const fgpu_pred *pred, const fgpu_table *build,
+                         const fgpu_join_index *idx, const fgpu_join_spec *spec, fgpu_result **out) {
     Binder pb(probe), bb(build);
     JoinProbeArgs a;
     AggLayout L;

@@ -305,7 +305,10 @@ int64_t fgpu_join_index_device_bytes(const fgpu_join_index *idx);
  * (execute_hash_join_projected, src/query/join.rs:184-282 + execute_agg_only,
  *  src/query/aggregate.rs:1394; execute_hash_semi_anti_join, src/query/join.rs:49-132)
  * `idx` may be NULL (the index is then built and dropped inside the call); it
- * must have been built on spec->build_key of `build`. */
+ * must have been built on spec->build_key of `build`.  Any number of
+ * aggregates: past what one probe pass holds (four accumulators, or
+ * FGPU_MAX_AGGS outputs) the call runs them in parts, one join each, and
+ * returns their columns in order. */
 int fgpu_join_agg(fgpu_ctx *ctx, const fgpu_table *probe, const fgpu_pred *probe_pred,
                   const fgpu_table *build, const fgpu_join_index *idx,
                   const fgpu_join_spec *spec, fgpu_result **out);

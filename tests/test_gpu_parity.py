@@ -848,3 +848,25 @@ def test_float_minmax_skip_nan_even_when_first(ctx, ctx_gbp, ctx_gbh):
         assert_batches_equal(fa.GroupByExec(["k"], aggs).execute(c, t), o.exact, "k")
         s = fa.execute_agg_only(c, t, aggs)
         assert s.columns[0][0] == -2.0 and s.columns[1][0] == 3.0
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT, SEMI, ANTI])
+def test_join_many_aggregates_run_in_parts(jctx, jt):
+    # more aggregates than the probe kernels' four register accumulators: the
+    # call runs them in parts (one join per part) and joins the columns in order
+    ctx = jctx
+    pcols, bcols = _c3(20_000, 300_000)
+    rng = np.random.default_rng(61)
+    pcols["w"] = rng.integers(-1000, 1000, 300_000)
+    pcols["z"] = rng.normal(size=300_000)
+    pt, bt = _table(ctx, pcols), _table(ctx, bcols)
+    aggs = [AggExpr(COUNT), AggExpr(SUM, "ppay"), AggExpr(MIN, "ppay"), AggExpr(MAX, "ppay"), AggExpr(SUM, "w"),
+            AggExpr(MIN, "z"), AggExpr(MAX, "z"), AggExpr(AVG, "w"), AggExpr(COUNT, "z")]
+    if jt in (INNER, LEFT):
+        aggs += [AggExpr(SUM, "bpay", side="build"), AggExpr(MAX, "bpay", side="build")]
+    g = fa.execute_hash_join(ctx, pt, bt, "pk", "bk", jt, aggs, filter=col("w") > -800)
+    assert "aggregates in" in ctx.last_plan(), ctx.last_plan()
+    o = ora.join(pcols, bcols, "pk", "bk", jt, aggs, expr=col("w") > -800)
+    assert_batches_equal(g, o.exact, None)
+    pt.free()
+    bt.free()

@@ -645,10 +645,11 @@ hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchAr
 // Resident workgroups per CU of each pass at this block size and LDS use:
 // the persistent grids launch that many per CU, so a small pass-2 table lets
 // several workgroups share a CU and hide each other's unit-start latency.
-hipError_t batch_occupancy(int npc, int nin, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2) {
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu1, batch_scan_fn(npc, nin), block, lds1);
+hipError_t batch_occupancy(int npc, int nin, int block1, int block2, size_t lds1, size_t lds2, int &per_cu1,
+                           int &per_cu2) {
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu1, batch_scan_fn(npc, nin), block1, lds1);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, (const void *)k_batch_merge, block, lds2);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, (const void *)k_batch_merge, block2, lds2);
     per_cu1 = std::max(1, per_cu1);
     per_cu2 = std::max(1, per_cu2);
     return e;

@@ -371,7 +371,8 @@ hipError_t launch_batch_scan(hipStream_t s, const BatchQuery *qs, const BatchArg
 hipError_t launch_batch_merge(hipStream_t s, const BatchQuery *qs, const BatchArgs &a, int grid, int block,
                               size_t lds);
 hipError_t batch_set_lds(size_t lds1, size_t lds2);
-hipError_t batch_occupancy(int npc, int nin, int block, size_t lds1, size_t lds2, int &per_cu1, int &per_cu2);
+hipError_t batch_occupancy(int npc, int nin, int block1, int block2, size_t lds1, size_t lds2, int &per_cu1,
+                           int &per_cu2);
 
 // ─── ORDER BY with any LIMIT and any number of keys (fgpu_sort.hip) ────────
 hipError_t launch_mask_rows(hipStream_t s, const uint64_t *words, const int64_t *offs, int64_t n_words,

@@ -284,6 +284,7 @@ struct Tuning {
     int join_compact = 1;          // compact record keys: 0 never, 1 for key-only records, 2 always
     int gbp_pack = 1;              // partitioned and batched GROUP BY: key and an integer input in one record word
     int batch_unit_rows = 0;       // batched GROUP BY: rows per pass-1 unit before tuning (0: the default)
+    int batch_scan_block = 0;      // batched GROUP BY: pass-1 workgroup size (0: the batch's tuned block)
 };
 
 struct fgpu_ctx {
@@ -1107,6 +1108,11 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     else if (n == "join_compact") t.join_compact = value;
     else if (n == "gbp_pack") t.gbp_pack = value;
     else if (n == "batch_unit_rows") t.batch_unit_rows = value;
+    else if (n == "batch_scan_block") {
+        if (value != 0 && (value < 64 || value > 1024 || value % 64))
+            return set_err(FGPU_E_INVALID_QUERY, "batch_scan_block must be 0 or a multiple of 64 in [64, 1024]");
+        t.batch_scan_block = value;
+    }
     else return set_err(FGPU_E_INVALID_QUERY, "unknown tuning knob '%s'", name);
     if (t.gb_block < 64 || t.gb_block > 1024 || t.gb_block % 64 || t.join_block < 64 ||
         t.join_block > 1024 || t.join_block % 64)
@@ -3473,14 +3479,16 @@ int batch_plan(fgpu_batch *b) {
     A.lds_slots = cfg.slots;
     // persistent grids: as many workgroups per CU as stay resident together
     int per_cu1 = 1, per_cu2 = 1;
-    HIPCHK(batch_occupancy(b->npc, b->nin, cfg.block, b->lds1, b->lds2, per_cu1, per_cu2));
+    // pass 1 may run at its own block size (batch_scan_block), pass 2 at cfg.block
+    const int block1 = c->tune.batch_scan_block ? c->tune.batch_scan_block : cfg.block;
+    HIPCHK(batch_occupancy(b->npc, b->nin, block1, cfg.block, b->lds1, b->lds2, per_cu1, per_cu2));
     b->per_cu1 = std::min(per_cu1, cfg.wgs_per_cu);
     b->per_cu2 = std::min(per_cu2, cfg.wgs_per_cu);
     const int grid1 = c->num_cus * b->per_cu1, grid2 = c->num_cus * b->per_cu2;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_fill_u64(c->stream, (uint64_t *)b->d_work, (int64_t)(4 + b->qs.size()), 0);
     if (e == hipSuccess && b->n_units)
-        e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, b->nin, grid1, cfg.block, b->lds1);
+        e = launch_batch_scan(c->stream, b->d_qs, A, b->npc, b->nin, grid1, block1, b->lds1);
     if (e == hipSuccess && b->n_units2) e = launch_batch_merge(c->stream, b->d_qs, A, grid2, cfg.block, b->lds2);
     const hipError_t e2 = hipStreamEndCapture(c->stream, &b->graph);
     if (e != hipSuccess || e2 != hipSuccess)

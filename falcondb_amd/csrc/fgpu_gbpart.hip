@@ -654,6 +654,7 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     uint64_t *tstate = lds + T2;
     unsigned *pre = (unsigned *)(tstate + (int64_t)T2 * S);  // G + 1: row records
     unsigned *spre = pre + G + 1;                             // G + 1: state records
+    unsigned *list = spre + G + 1;                            // T2: the occupied slots, in output order
     const int p = blockIdx.x;
     const int lane = threadIdx.x & 63;
     // pass 1 overflowed a cell: the host resizes and runs both passes again
@@ -759,26 +760,27 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
         merge_state<LDS_SCOPE>(tstate + (int64_t)li * S, (const uint64_t *)(rec + 1), a.ops);
     }
     __syncthreads();
-    // emit: one output reservation for the whole partition
-    unsigned have = 0, n_have = 0;
-    for (int i = threadIdx.x, k = 0; i < T2; i += blockDim.x, ++k) {
-        const bool h = tkeys[i] != EMPTY_KEY && tstate[(int64_t)i * S] > 0;
-        have |= h ? 1u << k : 0u;
-        n_have += h ? 1u : 0u;
-    }
+    // emit: the occupied slots are listed in output order first, so that the
+    // emitting lanes are all busy and their stores contiguous; one output
+    // reservation for the whole partition
+    unsigned n_have = 0;
+    for (int i = threadIdx.x; i < T2; i += blockDim.x)
+        n_have += tkeys[i] != EMPTY_KEY && tstate[(int64_t)i * S] > 0 ? 1u : 0u;
     unsigned total;
-    const unsigned mine = block_excl_scan(n_have, total);
-    int64_t o = (int64_t)block_reserve(A.out_count, total) + mine;
-    for (int i = threadIdx.x, k = 0; i < T2; i += blockDim.x, ++k) {
-        if (!((have >> k) & 1u)) continue;
+    unsigned pos = block_excl_scan(n_have, total);
+    for (int i = threadIdx.x; i < T2; i += blockDim.x)
+        if (tkeys[i] != EMPTY_KEY && tstate[(int64_t)i * S] > 0) list[pos++] = (unsigned)i;
+    const int64_t base = (int64_t)block_reserve(A.out_count, total);  // its barriers publish the list
+    for (unsigned j = threadIdx.x; j < total; j += blockDim.x) {
+        const int64_t o = base + j;
         if (o >= A.out_cap) {
             atomicExch(A.flags + 2, 1ull);
             break;
         }
-        A.out_key[o] = tkeys[i];
-        A.out_key_valid[o] = 1;
+        const unsigned i = list[j];
+        gst(A.out_key + o, tkeys[i]);
+        gst(A.out_key_valid + o, (uint8_t)1);
         emit_row(A.outs, tstate + (int64_t)i * S, o, A.out_val, A.out_valid);
-        ++o;
     }
 }
 

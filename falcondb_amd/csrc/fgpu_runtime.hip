@@ -1683,7 +1683,9 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     // scan.  The fan-out keeps a partition's expected keys at <= 0.6 of that,
     // then the table shrinks to ~2x the expected keys (more workgroups per CU).
     const int64_t lds_fixed2 = 8 * (int64_t)(A.G + 1) + 256;
-    const int64_t t2max = std::min<int64_t>(tu.gbp_table_slots, ((int64_t)tu.lds_limit - lds_fixed2) / (8 * (S + 1))) / 64 * 64;
+    // (a slot is its key, its state and a 4-byte entry of the emit list)
+    const int64_t t2max =
+        std::min<int64_t>(tu.gbp_table_slots, ((int64_t)tu.lds_limit - lds_fixed2) / (8 * (S + 1) + 4)) / 64 * 64;
     if (t2max < 64) return -1;
     const int64_t distinct = std::max<int64_t>(key->distinct, 1);
     int64_t P = tu.gbp_partitions ? tu.gbp_partitions
@@ -1691,7 +1693,7 @@ static int groupby_partitioned(fgpu_ctx *c, const fgpu_table *t, const DevCol *k
     if (P > 16384) return -1;
     const int T2 = (int)std::max<int64_t>(64, std::min<int64_t>(t2max, ((int64_t)std::ceil(2.0 * distinct / P) + 63) / 64 * 64));
     A.T2 = T2;
-    auto lds2 = [&](int t2) { return (size_t)t2 * (8 + 8 * (size_t)S) + (size_t)lds_fixed2; };
+    auto lds2 = [&](int t2) { return (size_t)t2 * (12 + 8 * (size_t)S) + (size_t)lds_fixed2; };
     A.log_p = 0;
     while ((int64_t(1) << A.log_p) < P) ++A.log_p;
     // Pass-1 cache: the most slots (a multiple of 8, the ways) beside the cell

@@ -120,15 +120,17 @@ __global__ void k_gb_dense_lds(GbArgs a) {
             x[u] = GbRow{};
             if (pass[u]) gb_issue(a, gc, rows[u], x[u]);
         }
+        // the tile's rows folded op-major: each op's descriptor read once
+        uint64_t *st[GB_UNROLL];
+        Inputs in[GB_UNROLL];
 #pragma unroll
         for (int u = 0; u < GB_UNROLL; ++u) {
-            if (!pass[u]) continue;
-            int64_t key;
-            Inputs in;
-            const bool has_key = gb_finish(a, gc, rows[u], x[u], key, in);
-            const int64_t g = has_key ? (key - a.key_min) : a.range;
-            update_row<LDS_SCOPE>(lds + g * S, a.ops, in);
+            int64_t key = 0;
+            in[u] = Inputs{0, 0, 0, 0, 0, 0, 0};
+            const bool has_key = pass[u] && gb_finish(a, gc, rows[u], x[u], key, in[u]);
+            st[u] = lds + (has_key ? (key - a.key_min) : a.range) * S;
         }
+        update_rows<LDS_SCOPE, GB_UNROLL>(st, pass, a.ops, in);
     }
     __syncthreads();
     for (int64_t g = threadIdx.x; g < groups; g += blockDim.x)

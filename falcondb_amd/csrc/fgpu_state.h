@@ -128,6 +128,99 @@ FGPU_DEV void update_row(uint64_t *st, const OpList &ops, const Inputs &in) {
     }
 }
 
+// update_row for R rows at once, op-major: each op's packed descriptor is
+// read once (made uniform) for the R rows, and their atomics issue back to
+// back.  Same effect as update_row on each row with ok[r] set (every update is
+// an integer add, min, max or or: order-free).
+template <int SCOPE, int R>
+FGPU_DEV void update_rows(uint64_t *const (&st)[R], const bool (&ok)[R], const DOps &ops, const Inputs (&in)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (ok[r]) a_add<SCOPE>(st[r], 1ull);
+    for (int i = 0; i < ops.n; ++i) {
+        const uint32_t d = __builtin_amdgcn_readfirstlane(ops.pk[i]);
+        const int col = opk_col(d), word = opk_word(d);
+        bool on[R];
+        int64_t v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            on[r] = ok[r] && ((in[r].valid >> col) & 1);
+            v[r] = pick(in[r], col);
+        }
+        switch (opk_kind(d)) {
+            case OP_CNT:
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (on[r]) a_add<SCOPE>(st[r] + word, 1ull);
+                break;
+            case OP_ISUM:
+                if (opk_W(d) == 2) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (on[r]) {
+                            uint64_t lo;
+                            int64_t hi;
+                            isum_digits(v[r], lo, hi);
+                            a_add<SCOPE>(st[r] + word, lo);
+                            a_add<SCOPE>(st[r] + word + 1, (uint64_t)hi);
+                        }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (on[r]) a_add<SCOPE>(st[r] + word, (uint64_t)v[r]);
+                }
+                break;
+            case OP_IMIN:
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (on[r]) a_smin<SCOPE>(st[r] + word, v[r]);
+                break;
+            case OP_IMAX:
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (on[r]) a_smax<SCOPE>(st[r] + word, v[r]);
+                break;
+            case OP_FSUM: {
+                const bool has_flags = opk_flags(d);
+                const int W = opk_W(d), L = opk_L(d);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (!on[r]) continue;
+                    const uint64_t bits = (uint64_t)v[r];
+                    // the flags word exists only for columns holding NaN / inf
+                    if (has_flags && ((bits >> 52) & 0x7ff) == 0x7ff) {
+                        const uint64_t fl = (bits & 0x000fffffffffffffull) ? FL_NAN
+                                            : (bits >> 63)                ? FL_NINF
+                                                                          : FL_PINF;
+                        a_or<SCOPE>(st[r] + word + W, fl);
+                        continue;
+                    }
+                    int w0;
+                    int64_t d0, d1, d2;
+                    fx_digits3(bits, L, w0, d0, d1, d2);
+                    a_add<SCOPE>(st[r] + word + w0, (uint64_t)d0);
+                    a_add<SCOPE>(st[r] + word + w0 + 1, (uint64_t)d1);
+                    a_add<SCOPE>(st[r] + word + w0 + 2, (uint64_t)d2);
+                }
+                break;
+            }
+            case OP_FMIN:
+            case OP_FMAX: {
+                const bool is_min = opk_kind(d) == OP_FMIN;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t bits = (uint64_t)v[r];
+                    if (!on[r] || (((bits >> 52) & 0x7ff) == 0x7ff && (bits & 0x000fffffffffffffull))) continue;
+                    const uint64_t e = f64_order(bits);
+                    if (is_min) a_umin<SCOPE>(st[r] + word, e);
+                    else a_umax<SCOPE>(st[r] + word, e);
+                }
+                break;
+            }
+        }
+    }
+}
+
 // Two-phase reads of a row's key and inputs: gb_issue issues the loads of
 // row r from uniform column bases (one value load per cell, a validity word
 // only for a column with NULLs) into a GbRow; gb_finish turns them into what

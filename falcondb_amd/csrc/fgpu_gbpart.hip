@@ -727,16 +727,16 @@ __global__ __launch_bounds__(1024) void k_gbp_agg(GbPartArgs A) {
     gbp_agg_fetch<NIN, U>(A, pre, G, part_cells, cell_stride, n_rows, threadIdx.x, key, in, have);
     for (unsigned i0 = threadIdx.x; i0 < n_rows; i0 += step) {
         gbp_agg_fetch<NIN, U>(A, pre, G, part_cells, cell_stride, n_rows, i0 + step, nkey, nin, nhave);
+        uint64_t *st[U];
+        bool ok[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (!have[u]) continue;
-            const int li = gbp_slot(tkeys, T2, T2, key[u], hash64((uint64_t)key[u]));
-            if (li < 0) {
-                atomicExch(A.flags + 1, 1ull);
-                continue;
-            }
-            update_row<LDS_SCOPE>(tstate + (int64_t)li * S, a.ops, in[u]);
+            const int li = have[u] ? gbp_slot(tkeys, T2, T2, key[u], hash64((uint64_t)key[u])) : 0;
+            if (li < 0) atomicExch(A.flags + 1, 1ull);
+            ok[u] = have[u] && li >= 0;
+            st[u] = tstate + (int64_t)(ok[u] ? li : 0) * S;
         }
+        update_rows<LDS_SCOPE, U>(st, ok, a.ops, in);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             key[u] = nkey[u];

@@ -134,7 +134,7 @@ __global__ void k_gb_dense_lds(GbArgs a) {
     }
     __syncthreads();
     for (int64_t g = threadIdx.x; g < groups; g += blockDim.x)
-        if (lds[g * S]) merge_state(a.state + g * S, lds + g * S, a.ops);
+        if (lds[g * S]) merge_state(a.state + (int64_t)(blockIdx.x % a.replicas) * a.rep_stride + g * S, lds + g * S, a.ops);
 }
 
 // ─── dense keys, HBM states ─────────────────────────────────────────────────
@@ -163,12 +163,10 @@ __global__ void k_gb_dense_global(GbArgs a) {
                 in[u] = load_inputs(a, rows[u]);
             }
         }
+        uint64_t *st[GB_UNROLL];
 #pragma unroll
-        for (int u = 0; u < GB_UNROLL; ++u) {
-            if (!pass[u]) continue;
-            const int64_t g = has_key[u] ? (key[u] - a.key_min) : a.range;
-            update_row<DEV_SCOPE>(a.state + g * S, a.ops, in[u]);
-        }
+        for (int u = 0; u < GB_UNROLL; ++u) st[u] = a.state + (has_key[u] ? (key[u] - a.key_min) : a.range) * S;
+        update_rows<DEV_SCOPE, GB_UNROLL>(st, pass, a.ops, in);
     }
 }
 
@@ -301,7 +299,19 @@ __global__ void k_gb_scalar(GbArgs a) {
     for (int i = 0; i < SCALAR_OPS; ++i)
         if (i < a.ops.n) fold_reg(lrow, a.ops.op[i], acc[i], hi[i], lo[i]);
     __syncthreads();
-    if (threadIdx.x == 0) merge_state(a.state, lrow, a.ops);
+    if (threadIdx.x == 0) merge_state(a.state + (int64_t)(blockIdx.x % a.replicas) * a.rep_stride, lrow, a.ops);
+}
+
+// Folds replicas 1.. of the dense-LDS / scalar state rows into replica 0, one
+// thread per group; a replica row with a zero row count holds identities only.
+__global__ void k_gb_fold(GbArgs a, int64_t n_groups) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const int S = a.ops.S;
+    for (int r = 1; r < a.replicas; ++r) {
+        const uint64_t *src = a.state + (int64_t)r * a.rep_stride + g * S;
+        if (src[0]) merge_state(a.state + g * S, src, a.ops);
+    }
 }
 
 // ─── emit: occupied groups -> output columns ────────────────────────────────
@@ -376,6 +386,12 @@ hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint6
     return hipGetLastError();
 }
 
+hipError_t launch_gb_fold(hipStream_t s, const GbArgs &a, int64_t n_groups) {
+    if (a.replicas <= 1 || n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gb_fold, dim3((unsigned)((n_groups + 255) / 256)), dim3(256), 0, s, a, n_groups);
+    return hipGetLastError();
+}
+
 template <int NPC>
 static void launch_gb_npc(hipStream_t s, int mode, const GbArgs &a, int grid, int block, size_t lds) {
     switch (mode) {
@@ -388,9 +404,11 @@ static void launch_gb_npc(hipStream_t s, int mode, const GbArgs &a, int grid, in
 
 hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,
                           size_t lds_bytes) {
-    if (a.pred.n_pcols <= 1) launch_gb_npc<1>(s, mode, a, grid, block, lds_bytes);
-    else if (a.pred.n_pcols == 2) launch_gb_npc<2>(s, mode, a, grid, block, lds_bytes);
-    else launch_gb_npc<MAX_PCOLS>(s, mode, a, grid, block, lds_bytes);
+    GbArgs b = a;
+    if (b.replicas < 1) b.replicas = 1;  // one state table
+    if (b.pred.n_pcols <= 1) launch_gb_npc<1>(s, mode, b, grid, block, lds_bytes);
+    else if (b.pred.n_pcols == 2) launch_gb_npc<2>(s, mode, b, grid, block, lds_bytes);
+    else launch_gb_npc<MAX_PCOLS>(s, mode, b, grid, block, lds_bytes);
     return hipGetLastError();
 }
 

@@ -55,6 +55,11 @@ struct GbArgs {
     unsigned long long *counters;  // [0] groups inserted, [1] overflow flag
     int lds_slots;            // hash mode: LDS cache slots (power of two, 0 = off)
     int probe_limit;          // hash mode: LDS probes before falling back to HBM
+    // dense-LDS and scalar modes: workgroup b merges its states into replica
+    // b % replicas (rows rep_stride words apart), so that fewer workgroups
+    // contend for each HBM word; launch_gb_fold folds them into replica 0
+    int replicas;
+    int64_t rep_stride;
 };
 
 struct EmitArgs {
@@ -119,6 +124,7 @@ hipError_t launch_gbp_agg(hipStream_t s, const GbPartArgs &a, int block, size_t 
 
 hipError_t launch_state_init(hipStream_t s, int64_t *keys, int64_t n_keys, uint64_t *state,
                              int64_t n_groups, const DOps &ops);
+hipError_t launch_gb_fold(hipStream_t s, const GbArgs &a, int64_t n_groups);
 hipError_t launch_groupby(hipStream_t s, int mode, const GbArgs &a, int grid, int block,
                           size_t lds_bytes);
 hipError_t launch_emit(hipStream_t s, const EmitArgs &a);

@@ -257,6 +257,7 @@ struct Tuning {
     int zone_maps = 1;             // WHERE scans skip batches whose bounds rule them out
     int gb_block = 256;
     int gb_waves_per_cu = 16;      // target resident waves per CU for streaming kernels
+    int gb_replicas = 4;           // dense-LDS / scalar GROUP BY: HBM state replicas
     int hash_lds_slots = 1024;
     int probe_limit = 8;
     int dense_lds_bytes = 48 * 1024;
@@ -1025,6 +1026,7 @@ int fgpu_open(int device, fgpu_ctx **out) {
     Tuning &t = c->tune;
     t.gb_block = env_int("FGPU_GB_BLOCK", t.gb_block);
     t.gb_waves_per_cu = env_int("FGPU_GB_WAVES", t.gb_waves_per_cu);
+    t.gb_replicas = env_int("FGPU_GB_REPLICAS", t.gb_replicas);
     t.hash_lds_slots = env_int("FGPU_HASH_LDS_SLOTS", t.hash_lds_slots);
     t.probe_limit = env_int("FGPU_PROBE_LIMIT", t.probe_limit);
     t.join_block = env_int("FGPU_JOIN_BLOCK", t.join_block);
@@ -1081,6 +1083,7 @@ int fgpu_set_tuning(fgpu_ctx *c, const char *name, int value) {
     const std::string n = name;
     if (n == "gb_block") t.gb_block = value;
     else if (n == "gb_waves_per_cu") t.gb_waves_per_cu = value;
+    else if (n == "gb_replicas") t.gb_replicas = std::max(1, std::min(value, 256));
     else if (n == "hash_lds_slots") t.hash_lds_slots = value;
     else if (n == "probe_limit") t.probe_limit = value;
     else if (n == "dense_lds_bytes") t.dense_lds_bytes = value;
@@ -2038,6 +2041,18 @@ static int groupby_impl(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, co
     return rc;
 }
 
+// State replicas of the dense-LDS and scalar GROUP BY: every workgroup merges
+// its states into HBM with device atomics at its end, and about a thousand
+// workgroups merging into the same few words queue behind one another; with
+// R replicas each word takes grid / R merges and k_gb_fold adds R - 1 rows per
+// group.  Never more replicas than workgroups.
+static int gb_replicas(const fgpu_ctx *c, int mode, int64_t n_rows) {
+    if (mode != GB_DENSE_LDS && mode != GB_SCALAR) return 1;
+    const int block = mode == GB_SCALAR ? 256 : c->tune.gb_block;
+    const int grid = grid_for(c, (n_rows + 3) / 4, block, c->tune.gb_waves_per_cu);
+    return std::max(1, std::min(c->tune.gb_replicas, grid));
+}
+
 // Runs a prepared GROUP BY with the strategy the key and state size call for.
 static int groupby_run(fgpu_ctx *c, const fgpu_table *t, const DevCol *key, Binder &b, GbArgs &a, AggLayout &L,
                        fgpu_result **out) {
@@ -2059,21 +2074,24 @@ static int groupby_run(fgpu_ctx *c, const fgpu_table *t, const DevCol *key, Bind
         cap = next_pow2(std::max<int64_t>(1024, (int64_t)(2.2 * (double)std::max<int64_t>(key->distinct, 1))));
     unsigned long long *d_cnt;
     if (int rc = buf(c, B_COUNTERS, 64 * 8, (void **)&d_cnt)) return rc;
+    const int reps = gb_replicas(c, mode, t->n_rows);
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (mode == GB_HASH) n_state_groups = cap + 1;
         int64_t *d_keys = nullptr;
         uint64_t *d_state;
         if (mode == GB_HASH)
             if (int rc = buf(c, B_KEYS, (size_t)cap * 8, (void **)&d_keys)) return rc;
-        if (int rc = buf(c, B_STATE, (size_t)n_state_groups * S * 8, (void **)&d_state)) return rc;
+        if (int rc = buf(c, B_STATE, (size_t)n_state_groups * S * 8 * reps, (void **)&d_state)) return rc;
         a.keys = d_keys;
         a.state = d_state;
         a.cap_mask = cap ? (uint64_t)(cap - 1) : 0;
         a.counters = d_cnt;
+        a.replicas = reps;
+        a.rep_stride = n_state_groups * S;
         mark(c, 0);
         HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_cnt, 8, 0));
         mark(c, 4);
-        HIPCHK(launch_state_init(c->stream, d_keys, cap, d_state, n_state_groups, L.ops));
+        HIPCHK(launch_state_init(c->stream, d_keys, cap, d_state, n_state_groups * reps, L.ops));
         mark(c, 5);
         // launch shape
         int block = c->tune.gb_block;
@@ -2090,6 +2108,7 @@ static int groupby_run(fgpu_ctx *c, const fgpu_table *t, const DevCol *key, Bind
         const int grid = grid_for(c, (t->n_rows + 3) / 4, block, c->tune.gb_waves_per_cu);
         mark(c, 1);
         if (t->n_rows) HIPCHK(launch_groupby(c->stream, mode, a, grid, block, lds));
+        if (t->n_rows) HIPCHK(launch_gb_fold(c->stream, a, n_state_groups));
         mark(c, 2);
         unsigned long long h_cnt[2] = {0, 0};
         if (mode == GB_HASH) {
@@ -2170,6 +2189,9 @@ int fgpu_groupby_static(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, co
                        key->name.c_str(), (long long)a.range);
     const size_t state_off = (pass_off + pass_used + 255) & ~size_t(255);
     const size_t state_bytes = (size_t)n_state_groups * S * 8;
+    // replicas only while they fit beside the one table the query needs
+    const int reps = (int)std::max<size_t>(1, std::min<size_t>(gb_replicas(c, mode, t->n_rows),
+        state_off + state_bytes > STATIC_SCRATCH_BYTES ? 1 : (STATIC_SCRATCH_BYTES - state_off) / std::max<size_t>(state_bytes, 1)));
     if (state_off + state_bytes > STATIC_SCRATCH_BYTES)
         return set_err(FGPU_E_UNSUPPORTED, "group state of %lld slots x %d words needs %zu bytes; the static scratch "
                        "holds %zu", (long long)n_state_groups, S, state_off + state_bytes, STATIC_SCRATCH_BYTES);
@@ -2187,13 +2209,16 @@ int fgpu_groupby_static(fgpu_ctx *c, const fgpu_table *t, const fgpu_pred *p, co
     a.state = d_state;
     a.cap_mask = 0;
     a.counters = d_cnt;
+    a.replicas = reps;
+    a.rep_stride = n_state_groups * S;
     // the array keeps the last call's contents: clear counters and the row count
     HIPCHK(launch_fill_u64(c->stream, (uint64_t *)d_cnt, 65, 0));
-    HIPCHK(launch_state_init(c->stream, nullptr, 0, d_state, n_state_groups, L.ops));
+    HIPCHK(launch_state_init(c->stream, nullptr, 0, d_state, n_state_groups * reps, L.ops));
     const int block = mode == GB_SCALAR ? 256 : c->tune.gb_block;
     const size_t lds = mode == GB_DENSE_LDS ? (size_t)n_state_groups * S * 8 : 0;
     const int grid = grid_for(c, (t->n_rows + 3) / 4, block, c->tune.gb_waves_per_cu);
     if (t->n_rows) HIPCHK(launch_groupby(c->stream, mode, a, grid, block, lds));
+    if (t->n_rows) HIPCHK(launch_gb_fold(c->stream, a, n_state_groups));
     EmitArgs e{};
     e.outs = L.outs;
     e.keys = nullptr;

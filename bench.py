@@ -209,7 +209,14 @@ def _serve_stream(srv, reqs, kinds, qps, seed_):
             "failed": int(st1["failed"] - st0["failed"])}
 
 
-def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
+SERVE_FORMS = {  # name -> (slots, shared COUNT(*) scans, overlap at any load)
+    "served": (4, True, False),
+    "served_overlap_always": (4, True, True),
+    "plain": (1, False, False),
+}
+
+
+def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30, forms=("served", "plain")):
     """The serving form: a device-resident fact table (the 100M-row group-by
     table plus a 12-value column g) and the 1M-row dimension table, a request
     mix of filtered COUNT(*), GROUP BY g / key, INNER / SEMI joins on
@@ -241,8 +248,10 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
     # one form at a time: a server is created, warmed up (scratch arenas
     # allocated, request shapes learned), checked against the requests run
     # alone, swept over every offered QPS and closed before the other starts
-    for name, srv_slots, share in (("served", slots, True), ("plain", 1, False)):
-        srv = Server(0, slots=srv_slots, hbm_budget=budget, share_counts=share)
+    for name in forms:
+        srv_slots, share, always = SERVE_FORMS[name]
+        srv_slots = slots if srv_slots > 1 else 1
+        srv = Server(0, slots=srv_slots, hbm_budget=budget, share_counts=share, overlap_always=always)
         ts = [srv.submit(r) for r in reqs0]
         for sp, r, t in zip(mix0, reqs0, ts):
             got, _ = srv.wait(t)
@@ -254,19 +263,19 @@ def bench_serving(ctx, K, rows, qps_levels, slots=4, budget=32 << 30):
         results[name] = [_serve_stream(srv, reqs, kinds, q, 100 + i)
                          for i, (q, (reqs, kinds)) in enumerate(zip(qps_levels, streams))]
         srv.close()
-    served, plain = results["served"], results["plain"]
-    for q, a, b in zip(qps_levels, served, plain):
-        log(f"[bench] C5 qps {q}: served p50 {a['p50_ms']:.2f} p99 {a['p99_ms']:.2f} ms "
-            f"{a['qps_served']:.0f} q/s {a['rows_per_s'] / 1e9:.1f} G rows/s | plain p50 "
-            f"{b['p50_ms']:.2f} p99 {b['p99_ms']:.2f} ms {b['qps_served']:.0f} q/s "
-            f"{b['rows_per_s'] / 1e9:.1f} G rows/s")
+    for i, q in enumerate(qps_levels):
+        log(f"[bench] C5 qps {q}: " + " | ".join(
+            f"{name} p50 {r[i]['p50_ms']:.2f} p99 {r[i]['p99_ms']:.2f} ms {r[i]['qps_served']:.0f} q/s "
+            f"{r[i]['rows_per_s'] / 1e9:.1f} G rows/s" for name, r in results.items()))
     idx.free()
     dim.free()
     fact.free()
     return {"rows": rows, "dim_rows": int(K.size), "slots": slots, "budget_GB": budget / 1e9,
             "mix": "40% COUNT(*) WHERE f<t; 30% GROUP BY g|key WHERE f<t; 20% INNER|SEMI join WHERE f<t; "
                    "10% ORDER BY v2 DESC LIMIT 10",
-            "parity_vs_alone": bool(same), "served": served, "plain_one_slot": plain}
+            "served_gate": "a request starts beside a running one only while >= 2 wait",
+            "parity_vs_alone": bool(same),
+            **{("plain_one_slot" if n == "plain" else n): r for n, r in results.items()}}
 
 
 def cpu_info():
@@ -427,6 +436,10 @@ def main():
     ap.add_argument("--batch-trials", type=int, default=18,
                     help="configurations the C4 batch's autotuner tries (1: the first candidate only)")
     ap.add_argument("--qps", default="100,300,1000,3000")
+    ap.add_argument("--serve-forms", default="served,plain",
+                    help="serving forms swept, from: " + ", ".join(SERVE_FORMS))
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving leg (no JSON bench line)")
     ap.add_argument("--gb-strategy", choices=["auto", "hash"], default="auto",
                     help="C2 GROUP BY strategy: the planner's choice, or the single hash table "
                          "(the radix-partitioned path switched off), to measure what partitioning buys")
@@ -456,6 +469,14 @@ def main():
     from falcondb_amd.operators import AggExpr, _aggs_struct, col
 
     lib = _abi.load()
+    forms = tuple(f for f in args.serve_forms.split(",") if f)
+    if any(f not in SERVE_FORMS for f in forms):
+        sys.exit(f"--serve-forms: unknown form in {args.serve_forms!r}")
+    if args.serve_only:
+        res = bench_serving(fa.Context(0), dg.keys(C2_KEYS, 40), args.rows,
+                            [int(q) for q in args.qps.split(",")], forms=forms)
+        print(json.dumps({"c5_serving": res}), flush=True)
+        return
     t0 = time.time()
     K = dg.keys(C2_KEYS, 40)
     c2 = dg.c2_table(args.rows, K)
@@ -602,7 +623,8 @@ def main():
 
     if not args.no_serve:
         table.free()
-        secondary["c5_serving"] = bench_serving(ctx, K, args.rows, [int(q) for q in args.qps.split(",")])
+        secondary["c5_serving"] = bench_serving(ctx, K, args.rows, [int(q) for q in args.qps.split(",")],
+                                                forms=forms)
 
     cpu = None
     if not args.no_cpu:

@@ -44,6 +44,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 constexpr int MAX_SHARED = 32;    // requests per shared COUNT(*) scan
 constexpr int LOOKAHEAD = 64;     // queued requests a free slot looks at
 constexpr int MAX_RETRIES = 4;    // budget re-queues of one request
+constexpr int GATE_DEPTH = 2;     // waiting requests that open the backlog gate
 
 struct Req {
     uint64_t ticket = 0;
@@ -81,6 +82,7 @@ struct fgpu_server {
     int n_slots = 0;
     int64_t budget = 0;
     bool share = true;
+    bool gated = true;  // backlog gate on concurrent starts (pick)
     std::vector<fgpu_ctx *> ctx;
     std::vector<int64_t> held;  // bytes each slot holds: arena when idle, reservation when running
     std::vector<char> busy;
@@ -204,6 +206,13 @@ bool fgpu_server::pick(int s, std::vector<Req *> &batch, std::vector<int> &trim)
     trim.clear();
     // busy[s] while idle: another worker is releasing this slot's arena
     if (busy[s] || queue.empty() || (paused && !stop)) return false;
+    // Backlog gate: beside a running request a new one starts only once
+    // GATE_DEPTH requests wait.  The operators are HBM-bound, so two that
+    // share the device both finish late; run one after the other, the first
+    // finishes early and the second no later.  Overlap pays by hiding each
+    // request's host work (launches, read-backs, result copies) behind the
+    // other slots' kernels, which matters once a queue has formed.
+    if (gated && running > 0 && (int)queue.size() < GATE_DEPTH) return false;
     int64_t others = held_total() - held[s];
     const int n = (int)std::min<size_t>(queue.size(), LOOKAHEAD);
     for (int i = 0; i < n; ++i) {
@@ -341,7 +350,10 @@ void fgpu_server::worker(int s) {
         Req *r = batch[0];
         if (over && batch.size() == 1 && r->retries < MAX_RETRIES) {
             // outgrew its reservation: queue it again, first in line, with more
-            const int64_t want = std::max(2 * r->need, used + used / 4);
+            // doubling past the budget: one more try with the whole budget,
+            // unless what the call already asked for is over it
+            int64_t want = std::max(2 * r->need, used + used / 4);
+            if (want > budget && used <= budget && r->need < budget) want = budget;
             learned[r->shape] = std::max(learned[r->shape], want);
             r->need = want;
             ++r->retries;
@@ -382,7 +394,7 @@ void fgpu_server::worker(int s) {
 
 extern "C" {
 
-int fgpu_server_open(int device, int n_slots, int64_t hbm_budget_bytes, int share_counts, fgpu_server **out) {
+int fgpu_server_open(int device, int n_slots, int64_t hbm_budget_bytes, int flags, fgpu_server **out) {
     if (!out) return fgpu_host::set_error(FGPU_E_INVALID_QUERY, "out is NULL");
     if (n_slots < 1 || n_slots > 64) return fgpu_host::set_error(FGPU_E_INVALID_QUERY, "n_slots must be in [1, 64]");
     if (hbm_budget_bytes < (1 << 20))
@@ -391,7 +403,12 @@ int fgpu_server_open(int device, int n_slots, int64_t hbm_budget_bytes, int shar
     s->device = device;
     s->n_slots = n_slots;
     s->budget = hbm_budget_bytes;
-    s->share = share_counts != 0;
+    if (flags & ~(FGPU_SERVE_SHARE_COUNTS | FGPU_SERVE_OVERLAP_ALWAYS)) {
+        delete s;
+        return fgpu_host::set_error(FGPU_E_INVALID_QUERY, "unknown server flags");
+    }
+    s->share = (flags & FGPU_SERVE_SHARE_COUNTS) != 0;
+    s->gated = (flags & FGPU_SERVE_OVERLAP_ALWAYS) == 0;
     for (int i = 0; i < n_slots; ++i) {
         fgpu_ctx *c = nullptr;
         if (int rc = fgpu_open(device, &c)) {

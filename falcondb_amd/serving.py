@@ -170,9 +170,12 @@ class Server:
     as a context manager) before freeing the tables its requests read."""
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 8 << 30,
-                 share_counts: bool = True):
+                 share_counts: bool = True, overlap_always: bool = False):
+        # overlap_always=False: beside a running request a new one starts only
+        # while at least two wait (one after the other under light load)
         h = C.c_void_p()
-        _check(_lib().fgpu_server_open(device, slots, hbm_budget, int(share_counts), C.byref(h)))
+        flags = (1 if share_counts else 0) | (2 if overlap_always else 0)
+        _check(_lib().fgpu_server_open(device, slots, hbm_budget, flags, C.byref(h)))
         self._h = h
         self._funcs: dict[int, list] = {}
         _LIVE_SERVERS.add(self)

@@ -460,8 +460,15 @@ typedef struct {
     int64_t budget_bytes;
 } fgpu_server_stats;
 
-/* share_counts = 0 turns the shared COUNT(*) scan off (every request runs alone). */
-int fgpu_server_open(int device, int n_slots, int64_t hbm_budget_bytes, int share_counts,
+/* flags for fgpu_server_open.  FGPU_SERVE_SHARE_COUNTS: COUNT(*) requests that
+ * test one numeric column of the same table with one comparison ride one
+ * shared scan.  Without FGPU_SERVE_OVERLAP_ALWAYS a request starts beside a
+ * running one only while at least two requests wait (under light load the
+ * slots run requests one after the other, which gives the lower tail
+ * latency); with it, any free slot starts the oldest request that fits. */
+#define FGPU_SERVE_SHARE_COUNTS 1
+#define FGPU_SERVE_OVERLAP_ALWAYS 2
+int fgpu_server_open(int device, int n_slots, int64_t hbm_budget_bytes, int flags,
                      fgpu_server **out);
 /* Copies the request (predicate, specs and strings) and queues it; returns at once. */
 int fgpu_server_submit(fgpu_server *s, const fgpu_request *req, uint64_t *ticket);

@@ -166,3 +166,11 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_server_open_refuses_unknown_flags_without_gpu():
+    lib = _abi.load()
+    h = C.c_void_p()
+    assert lib.fgpu_server_open(0, 4, 1 << 30, 4, C.byref(h)) != 0
+    assert b"flags" in lib.fgpu_last_error()
+    assert not h.value

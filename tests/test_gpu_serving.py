@@ -150,3 +150,25 @@ def test_topk_served(c5):
     sel = np.nonzero(cols["f"] < 3)[0]
     o2 = sel[np.lexsort((sel, -cols["v2"][sel]))][:7]
     assert np.array_equal(b.columns[0], o2) and np.array_equal(b.columns[1], cols["v2"][o2])
+
+
+def test_backlog_gate_runs_light_load_one_at_a_time(ctx, c5):
+    """Without overlap_always a request starts beside a running one only while
+    at least two wait: two queued requests run one after the other.  With
+    overlap_always the same requests give the same results."""
+    cols, fact, dim, idx = c5
+    reqs = [groupby_request(fact, "key", [AggExpr(COUNT), AggExpr(SUM, "v1")], filter=col("f") < 50),
+            groupby_request(fact, "g", [AggExpr(MIN, "v2"), AggExpr(MAX, "v2")], filter=col("f") < 20)]
+    alone = [run_alone(ctx, r) for r in reqs]
+    for always in (False, True):
+        with Server(0, slots=4, hbm_budget=1 << 30, overlap_always=always) as srv:
+            srv.pause(True)
+            tickets = [srv.submit(r) for r in reqs]
+            srv.pause(False)
+            served = [srv.wait(t)[0] for t in tickets]
+            st = srv.stats()
+        for got, exp, key in zip(served, alone, ("key", "g")):
+            _same(got, exp, key)
+        assert st["completed"] == 2 and st["failed"] == 0
+        if not always:
+            assert st["peak_running"] == 1
